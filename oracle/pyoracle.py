@@ -1,0 +1,325 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the CPU-baseline leg
+of bench.py.  Fields are numpy complex128 arrays of shape (N1, N2, N3//2+1), C-order, the
+reference's layout (trunk/lib/fields/Cvf.cc:112-114).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class Grid(C.Structure):
+    _fields_ = [("N1", C.c_int), ("N2", C.c_int), ("N3", C.c_int),
+                ("f1", C.c_double), ("f2", C.c_double), ("f3", C.c_double),
+                ("basis", C.c_int), ("dealias", C.c_int)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        d, i, p, vp = C.c_double, C.c_int, C.POINTER(Grid), C.c_void_p
+        pp = C.POINTER(C.c_void_p)
+        sig = {
+            "orc_fft1d": (None, [i, i, vp]),
+            "orc_set_threads": (None, [i]),
+            "orc_get_threads": (i, []),
+            "orc_kx": (i, [p, i]), "orc_ky": (i, [p, i]),
+            "orc_kmag": (d, [p, i, i, i]),
+            "orc_multiplicity": (d, [p, i, i, i]),
+            "orc_min_radius_outside": (i, [p]),
+            "orc_max_radius_inside": (i, [p]),
+            "orc_approx_modes_in_shell": (d, [p, i]),
+            "orc_shell_ek_size": (i, [p]),
+            "orc_kept": (i, [p, i, i, i]),
+            "orc_forward": (None, [p, vp, i]),
+            "orc_inverse": (None, [p, vp, i]),
+            "orc_deriv": (None, [p, i, vp, vp, i]),
+            "orc_dealias": (None, [p, vp]),
+            "orc_real_mult": (None, [p, vp, vp, vp]),
+            "orc_divide_ksqr": (None, [p, vp]),
+            "orc_exp_ksqr": (None, [p, vp, d]),
+            "orc_satisfy_reality": (None, [p, vp]),
+            "orc_nlin_ns": (None, [p, pp, pp]),
+            "orc_pressure": (None, [p, pp, vp]),
+            "orc_rhs": (None, [p, pp, vp]),
+            "orc_rk4_step_ns": (None, [p, pp, d, d]),
+            "orc_euler_step_ns": (None, [p, pp, d, d]),
+            "orc_rk2_step_ns": (None, [p, pp, d, d]),
+            "orc_nlin_mhd": (None, [p, pp, pp, pp, pp]),
+            "orc_rk4_step_mhd": (None, [p, pp, pp, d, d, d]),
+            "orc_energy": (d, [p, vp, vp]),
+            "orc_Sn": (d, [p, vp, d]),
+            "orc_shell_spectrum": (None, [p, vp, vp, d, vp, vp]),
+            "orc_dt": (d, [p, pp, d]),
+            "orc_fill_array_shell": (None, [p, vp, vp, d, d]),
+            "orc_shell_mult_single": (d, [p, vp, vp, d, d]),
+            "orc_shell_mult_all": (None, [p, pp, pp, vp, i, vp]),
+            "orc_et_nlin": (None, [p, pp, pp, pp]),
+            "orc_flux_ns": (None, [p, pp, vp, i, vp]),
+            "orc_shell_tr_ns": (None, [p, pp, vp, i, vp]),
+            "orc_flux_mhd": (None, [p, pp, pp, vp, i] + [vp] * 9),
+            "orc_et_radii": (None, [p, i, i, vp]),
+            "orc_nlin_rb": (None, [p, pp, vp, pp, vp]),
+            "orc_rk4_step_rb": (None, [p, pp, vp, d, d, d]),
+            "orc_nusselt": (d, [p, pp, vp, d, d]),
+            "orc_ic_taylor_green": (None, [p, pp, i, d]),
+            "orc_ic_random": (None, [p, pp, C.c_ulonglong, d, d, i]),
+            "orc_ic_random_scalar": (None, [p, vp, C.c_ulonglong, d]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(_lib, name)
+            f.restype = res
+            f.argtypes = args
+    return _lib
+
+
+def _ptr(a):
+    assert a.dtype == np.complex128 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _pp(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+class Oracle:
+    """FP64 CPU restatement bound to one grid."""
+
+    def __init__(self, N, kfactor=(1.0, 1.0, 1.0), basis="FOUR", dealias=True, threads=0):
+        self.N = tuple(int(n) for n in N)
+        self.g = Grid(self.N[0], self.N[1], self.N[2], float(kfactor[0]), float(kfactor[1]),
+                      float(kfactor[2]), 0 if basis == "FOUR" else 1, 1 if dealias else 0)
+        self.gp = C.byref(self.g)
+        self.L = lib()
+        if threads:
+            self.L.orc_set_threads(int(threads))
+
+    @property
+    def shape(self):
+        return (self.N[0], self.N[1], self.N[2] // 2 + 1)
+
+    def zeros(self, n=None):
+        if n is None:
+            return np.zeros(self.shape, np.complex128)
+        return [np.zeros(self.shape, np.complex128) for _ in range(n)]
+
+    # transforms and spectral ops
+    def forward(self, A, parity=0):
+        self.L.orc_forward(self.gp, _ptr(A), parity)
+        return A
+
+    def inverse(self, A, parity=0):
+        self.L.orc_inverse(self.gp, _ptr(A), parity)
+        return A
+
+    def deriv(self, axis, A, parity=0):
+        B = np.empty_like(A)
+        self.L.orc_deriv(self.gp, axis, _ptr(A), _ptr(B), parity)
+        return B
+
+    def dealias(self, A):
+        self.L.orc_dealias(self.gp, _ptr(A))
+        return A
+
+    def real_mult(self, A, B):
+        Cc = np.empty_like(A)
+        self.L.orc_real_mult(self.gp, _ptr(A), _ptr(B), _ptr(Cc))
+        return Cc
+
+    def divide_ksqr(self, A):
+        self.L.orc_divide_ksqr(self.gp, _ptr(A))
+        return A
+
+    def exp_ksqr(self, A, factor):
+        self.L.orc_exp_ksqr(self.gp, _ptr(A), factor)
+        return A
+
+    def satisfy_reality(self, A):
+        self.L.orc_satisfy_reality(self.gp, _ptr(A))
+        return A
+
+    # wavenumbers / binning
+    def kx(self, l1):
+        return self.L.orc_kx(self.gp, l1)
+
+    def ky(self, l2):
+        return self.L.orc_ky(self.gp, l2)
+
+    def kmag(self, l1, l2, l3):
+        return self.L.orc_kmag(self.gp, l1, l2, l3)
+
+    def multiplicity(self, l1, l2, l3):
+        return self.L.orc_multiplicity(self.gp, l1, l2, l3)
+
+    def min_radius_outside(self):
+        return self.L.orc_min_radius_outside(self.gp)
+
+    def max_radius_inside(self):
+        return self.L.orc_max_radius_inside(self.gp)
+
+    def shell_ek_size(self):
+        return self.L.orc_shell_ek_size(self.gp)
+
+    def kept(self, l1, l2, l3):
+        return self.L.orc_kept(self.gp, l1, l2, l3)
+
+    def approx_modes(self, r):
+        return self.L.orc_approx_modes_in_shell(self.gp, r)
+
+    # physics
+    def nlin_ns(self, V):
+        N = self.zeros(3)
+        self.L.orc_nlin_ns(self.gp, _pp(V), _pp(N))
+        return N
+
+    def pressure(self, nlin):
+        p = self.zeros()
+        self.L.orc_pressure(self.gp, _pp(nlin), _ptr(p))
+        return p
+
+    def rhs(self, nlin, p):
+        self.L.orc_rhs(self.gp, _pp(nlin), _ptr(p))
+        return nlin
+
+    def rk4_ns(self, V, nu, dt, steps=1):
+        for _ in range(steps):
+            self.L.orc_rk4_step_ns(self.gp, _pp(V), nu, dt)
+        return V
+
+    def euler_ns(self, V, nu, dt):
+        self.L.orc_euler_step_ns(self.gp, _pp(V), nu, dt)
+        return V
+
+    def rk2_ns(self, V, nu, dt):
+        self.L.orc_rk2_step_ns(self.gp, _pp(V), nu, dt)
+        return V
+
+    def nlin_mhd(self, U, B):
+        nU, nB = self.zeros(3), self.zeros(3)
+        self.L.orc_nlin_mhd(self.gp, _pp(U), _pp(B), _pp(nU), _pp(nB))
+        return nU, nB
+
+    def rk4_mhd(self, U, B, nu, eta, dt, steps=1):
+        for _ in range(steps):
+            self.L.orc_rk4_step_mhd(self.gp, _pp(U), _pp(B), nu, eta, dt)
+        return U, B
+
+    def nlin_rb(self, V, T):
+        nV, nT = self.zeros(3), self.zeros()
+        self.L.orc_nlin_rb(self.gp, _pp(V), _ptr(T), _pp(nV), _ptr(nT))
+        return nV, nT
+
+    def rk4_rb(self, V, T, Ra, Pr, dt, steps=1):
+        for _ in range(steps):
+            self.L.orc_rk4_step_rb(self.gp, _pp(V), _ptr(T), Ra, Pr, dt)
+        return V, T
+
+    def nusselt(self, V, T, Ra, Pr):
+        return self.L.orc_nusselt(self.gp, _pp(V), _ptr(T), Ra, Pr)
+
+    def energy(self, A, B=None):
+        return self.L.orc_energy(self.gp, _ptr(A), _ptr(B) if B is not None else None)
+
+    def Sn(self, A, n):
+        return self.L.orc_Sn(self.gp, _ptr(A), float(n))
+
+    def total_energy(self, V):
+        return sum(self.energy(a) for a in V)
+
+    def total_dissipation(self, V):
+        return 2 * sum(self.Sn(a, 2) for a in V)
+
+    def shell_spectrum(self, A, B=None, n=0):
+        size = self.shell_ek_size()
+        Sk = np.zeros(size)
+        cnt = np.zeros(size)
+        self.L.orc_shell_spectrum(self.gp, _ptr(A), _ptr(B) if B is not None else None, float(n),
+                                  Sk.ctypes.data_as(C.c_void_p), cnt.ctypes.data_as(C.c_void_p))
+        return Sk, cnt
+
+    def vector_spectrum(self, V, n=0):
+        return sum(self.shell_spectrum(a, None, n)[0] for a in V)
+
+    def dt(self, V, dt_fixed):
+        return self.L.orc_dt(self.gp, _pp(V), dt_fixed)
+
+    # energy transfer
+    def et_radii(self, scheme, no_shells):
+        r = np.zeros(no_shells + 1)
+        self.L.orc_et_radii(self.gp, scheme, no_shells, r.ctypes.data_as(C.c_void_p))
+        return r
+
+    def fill_shell(self, A, inner, outer):
+        B = np.empty_like(A)
+        self.L.orc_fill_array_shell(self.gp, _ptr(A), _ptr(B), inner, outer)
+        return B
+
+    def shell_mult_single(self, A, B, inner, outer):
+        return self.L.orc_shell_mult_single(self.gp, _ptr(A), _ptr(B), inner, outer)
+
+    def shell_mult_all(self, A, B, radii):
+        res = np.zeros(len(radii))
+        self.L.orc_shell_mult_all(self.gp, _pp(A), _pp(B), radii.ctypes.data_as(C.c_void_p),
+                                  len(radii), res.ctypes.data_as(C.c_void_p))
+        return res
+
+    def et_nlin(self, H, G):
+        N = self.zeros(3)
+        self.L.orc_et_nlin(self.gp, _pp(H), _pp(G), _pp(N))
+        return N
+
+    def flux_ns(self, V, radii):
+        nsph = len(radii) - 1
+        flux = np.zeros(nsph + 1)
+        self.L.orc_flux_ns(self.gp, _pp(V), radii.ctypes.data_as(C.c_void_p), nsph,
+                           flux.ctypes.data_as(C.c_void_p))
+        return flux
+
+    def shell_tr_ns(self, V, radii):
+        nsh = len(radii) - 1
+        T = np.zeros((nsh + 1, nsh + 1))
+        self.L.orc_shell_tr_ns(self.gp, _pp(V), radii.ctypes.data_as(C.c_void_p), nsh,
+                               T.ctypes.data_as(C.c_void_p))
+        return T
+
+    def flux_mhd(self, U, B, radii, elsasser=False):
+        nsph = len(radii) - 1
+        outs = [np.zeros(nsph + 1) for _ in range(9)]
+        ptrs = [o.ctypes.data_as(C.c_void_p) for o in outs]
+        if not elsasser:
+            ptrs[7] = None
+            ptrs[8] = None
+        self.L.orc_flux_mhd(self.gp, _pp(U), _pp(B), radii.ctypes.data_as(C.c_void_p), nsph, *ptrs)
+        names = ["u2u", "b2b", "u2b_in_out", "u2b_in_in", "b2u_in_out", "b2u_in_in",
+                 "u_out_b_out", "zp", "zm"]
+        return dict(zip(names, outs))
+
+    # initial conditions
+    def ic_taylor_green(self, k0=1, amp=1.0):
+        V = self.zeros(3)
+        self.L.orc_ic_taylor_green(self.gp, _pp(V), k0, amp)
+        return V
+
+    def ic_random(self, seed, E0=0.5, kp=4.0, project=True):
+        V = self.zeros(3)
+        self.L.orc_ic_random(self.gp, _pp(V), seed, E0, kp, 1 if project else 0)
+        return V
+
+    def ic_random_scalar(self, seed, amp=1e-3):
+        T = self.zeros()
+        self.L.orc_ic_random_scalar(self.gp, _ptr(T), seed, amp)
+        return T
