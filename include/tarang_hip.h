@@ -1,0 +1,159 @@
+/*
+ * tarang_hip.h -- C-ABI of the MI355X-native pseudo-spectral time step (libtarang_hip.so).
+ *
+ * Drop-in boundary for the reference's basis-transform, nonlinear-term and energy-transfer
+ * interfaces (hirdeshpharasi/tarang, paths under trunk/lib/).  Every entry point names the
+ * reference interface it replaces.  All functions are extern "C", take plain pointers and
+ * sizes, return an int status (0 = ok, < 0 = error; th_last_error() gives the message) and
+ * never throw.
+ *
+ * Layout: a spectral component is the reference's C-order array A[l1][l2][l3] of shape
+ * (N1, N2, N3/2+1) (trunk/lib/fields/Cvf.cc:112-114) in device memory, complex double for
+ * TH_FP64 plans and complex float for TH_FP32_MIXED plans.  Real space is the same buffer with
+ * z-packed reals: element (x, y, j) = (u(x,y,2j), u(x,y,2j+1)); plane j = N3/2 is padding.
+ * Calls are asynchronous on the plan's HIP stream unless they return host values (reductions),
+ * which synchronise.  A plan must be used by one host thread at a time.
+ */
+#ifndef TARANG_HIP_H
+#define TARANG_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TH_OK 0
+#define TH_E_ARG (-1)
+#define TH_E_HIP (-2)
+#define TH_E_UNSUPPORTED (-3)
+#define TH_E_STATE (-4)
+#define TH_E_NOMEM (-5)
+
+#define TH_FOUR 0          /* basis_type "FOUR" */
+#define TH_SCFT 1          /* basis_type "SCFT" (sin/cos along x, Fourier along y, z) */
+#define TH_FP64 0
+#define TH_FP32_MIXED 1    /* FP32 storage/transforms; FP64 twiddles, integrating factor, sums */
+
+#define TH_NS 0            /* incompressible Navier-Stokes (IncFluid + IncVF) */
+#define TH_MHD 1           /* incompressible MHD, velocity U and magnetic field B */
+#define TH_RB 2            /* Rayleigh-Benard, free-slip, PRLARGE / ULARGE scaling */
+
+typedef struct th_ctx th_ctx;       /* device + stream + autotune cache */
+typedef struct th_plan th_plan;     /* one box: basis, N, kfactor, precision, radix plans, scratch */
+typedef struct th_state th_state;   /* device-resident fields of one box in the fused layout */
+
+typedef struct {
+    double nu;      /* dissipation_coefficient (IncVF), velocity */
+    double eta;     /* dissipation_coefficient of W (MHD magnetic field) */
+    double kappa;   /* dissipation_coefficient of the scalar (RB); ignored when Ra > 0 and Pr > 0 */
+    double Ra, Pr;  /* RB: Ra, Pr; nu = sqrt(Pr/Ra), kappa = 1/sqrt(Pr Ra) (RB_slip_main.cc:238-239) */
+} th_phys;
+
+/* ---- context and plans ----
+ * replaces: Init_fftw_plan_array universal_tr.cc:48, Init_fftw_plan_FOUR four_tr.cc:44-98,
+ *           Init_fftw_plan_SCFT scft_tr.cc:49-74 and the process globals extern_def_vars.h:95-142 */
+int th_ctx_create(th_ctx **ctx, int device, const char *tune_cache_path);
+int th_ctx_destroy(th_ctx *ctx);
+const char *th_last_error(void);
+const char *th_version(void);
+int th_plan_create(th_ctx *ctx, th_plan **plan, int basis, int N1, int N2, int N3, const double kfactor[3],
+                   int precision, int dealias);
+int th_plan_destroy(th_plan *plan);
+size_t th_plan_field_elems(const th_plan *plan);      /* N1*N2*(N3/2+1) */
+size_t th_plan_elem_bytes(const th_plan *plan);       /* 16 (FP64) or 8 (FP32_MIXED) */
+int th_shell_ek_size(const th_plan *plan);            /* CV_shell_ek_size = Min_radius_outside + 1 */
+int th_max_radius_inside(const th_plan *plan);        /* Max_radius_inside four_inline.h:233 */
+int th_sync(th_plan *plan);
+
+/* ---- basis transforms (in place, reference layout) ----
+ * th_forward == Forward_transform_array universal_tr.cc:66-86 (ArrayFFT_FOUR four_tr.cc:199-209,
+ *               ArraySFT/CFT_SCFT scft_tr.cc:230-431): normalised, padding zeroed first.
+ * th_inverse == Inverse_transform_array universal_tr.cc:126-146 (ArrayIFFT_FOUR four_tr.cc:229-295,
+ *               ArrayISFT/ICFT_SCFT scft_tr.cc:440-639): unnormalised, padding plane zeroed. */
+int th_forward(th_plan *plan, void *A, int parity);
+int th_inverse(th_plan *plan, void *A, int parity);
+/* th_deriv == Xderiv/Yderiv/Zderiv universal_tr.cc:186-248 (axis 1, 2, 3); B may equal A */
+int th_deriv(th_plan *plan, int axis, const void *A, void *B, int parity);
+/* == Dealias_array universal_misc.cc:118-159 */
+int th_dealias(th_plan *plan, void *A);
+/* == Array_real_mult array_basic.cc:62-71 (C = (Re A Re B, Im A Im B)) */
+int th_real_mult(th_plan *plan, const void *A, const void *B, void *C);
+/* == Array_divide_ksqr_FOUR four_basic.cc:220-245 / _SCFT scft_basic.cc:247 */
+int th_divide_ksqr(th_plan *plan, void *A);
+/* == Array_exp_ksqr_FOUR four_basic.cc:312-337 (A *= exp(factor K^2), factor in FP64) */
+int th_exp_ksqr(th_plan *plan, void *A, double factor);
+
+/* ---- nonlinear terms, pressure, rhs: the reference's operation sequence on the GPU ----
+ * th_nlin_ns  == IncVF::Compute_nlin() compute_nlin.cc:58-108 (9 transforms, dealiased)
+ * th_pressure == IncVF::Compute_pressure() compute_pressure.cc:54-62
+ * th_rhs      == IncFluid::Compute_rhs() compute_rhs.cc:52-60 (nlin <- -(nlin + grad p))
+ * th_nlin_mhd == IncVF::Compute_nlin(IncVF&) compute_nlin_VF_MHD.cc:72-164 (U, B are modified
+ *                in place by the Elsasser round trip exactly as in the reference)
+ * th_nlin_rb  == IncVF::Compute_nlin_scalar(IncSF&) compute_nlin_SF.cc:80-177
+ * th_et_nlin  == IncVF::EnergyTr_Compute_nlin() ET_compute_nlin.cc:85-120 with helper H, giver G */
+int th_nlin_ns(th_plan *plan, void *const V[3], void *nlin[3]);
+int th_pressure(th_plan *plan, void *const nlin[3], void *p);
+int th_rhs(th_plan *plan, void *nlin[3], const void *p);
+int th_nlin_mhd(th_plan *plan, void *U[3], void *B[3], void *nU[3], void *nB[3]);
+int th_nlin_rb(th_plan *plan, void *const V[3], const void *T, void *nV[3], void *nT);
+int th_et_nlin(th_plan *plan, void *const H[3], void *const G[3], void *nlin[3]);
+
+/* ---- plain steppers on reference-layout device fields (one time step, force = 0 or RB buoyancy) ----
+ * == IncFluid::Time_advance() time_advance.cc:60-163 ("EULER", "RK2", "RK4"),
+ *    Time_advance(IncVF&) time_advance_VF_MHD.cc:59-180, Time_advance(IncSF&) time_advance_SF.cc:61-233.
+ * fields: NS V[0..2]; MHD U[0..2], B[0..2]; RB V[0..2], T.  scheme: 0 EULER, 1 RK2, 2 RK4. */
+int th_step_plain(th_plan *plan, int physics, int scheme, void *const fields[], double dt, const th_phys *p);
+
+/* ---- energies, spectra, transfers (return host doubles; synchronise) ----
+ * th_energy   == Get_total_energy four_energy.cc:88-168 (B == NULL: |A|^2; else Re A conj B)
+ * th_Sn       == Get_total_Sn four_energy.cc:179-228
+ * th_shell_spectrum == Compute_shell_spectrum universal_energy.cc:112-153 (with the partial-shell
+ *                rescale four_energy.cc:324-336); count[] = Sk_count (may be NULL)
+ * th_fill_array_shell == Fill_array_shell universal_ET.cc:45-65
+ * th_shell_mult_single == sum of 3 Shell_mult_single (prod_nlinV.cc:62-136)
+ * th_shell_mult_all    == Shell_mult_all (dot product) universal_ET.cc:160-207
+ * th_flux_ns  == IncVF::Compute_flux() compute_flux.cc:52-69 (sphere radii[1..nsph])
+ * th_shell_tr_ns == IncVF::Compute_shell_tr() compute_shell_tr.cc:59-79, T is (nsh+1)^2 row-major
+ * th_nusselt  == IncVF::Get_Nusselt_no nusselt.cc:53-89 (PRLARGE / ULARGE) */
+int th_energy(th_plan *plan, const void *A, const void *B, double *out);
+int th_Sn(th_plan *plan, const void *A, double n, double *out);
+int th_shell_spectrum(th_plan *plan, const void *A, const void *B, int n, double *Sk, double *count);
+int th_fill_array_shell(th_plan *plan, const void *A, void *B, double inner, double outer);
+int th_shell_mult_single(th_plan *plan, void *const A[3], void *const B[3], double inner, double outer, double *out);
+int th_shell_mult_all(th_plan *plan, void *const A[3], void *const B[3], const double *radii, int nr, double *out);
+int th_flux_ns(th_plan *plan, void *const V[3], const double *radii, int nsph, double *flux);
+int th_shell_tr_ns(th_plan *plan, void *const V[3], const double *radii, int nsh, double *T);
+int th_nusselt(th_plan *plan, void *const V[3], const void *T, double Ra, double Pr, double *Nu);
+
+/* ---- initial conditions on reference-layout device fields ----
+ * th_ic_taylor_green == IncVF::Setup_Taylor_Green_field ic_model.cc:58-107
+ * th_ic_random: seeded counter-based random-phase field, projected, Hermitian on kz = 0
+ *   (Satisfy_reality_array universal_misc.cc:51-109), dealiased, unit shells rescaled to
+ *   E(k) = C k^4 exp(-2 (k/kp)^2) with total energy E0 (replaces the time(0)-seeded
+ *   Init_cond_energy_spectrum init_cond_energy.cc:65-125)
+ * th_ic_random_scalar: seeded temperature perturbation of amplitude amp, T(0,:,:) = 0
+ *   (Zero_modes_RB_slip zero_mode_RB_slip.cc:50-58) */
+int th_ic_taylor_green(th_plan *plan, void *V[3], int k0, double amp);
+int th_ic_random(th_plan *plan, void *V[3], unsigned long long seed, double E0, double kp);
+int th_ic_random_scalar(th_plan *plan, void *T, unsigned long long seed, double amp);
+
+/* ---- fused, dealias-pruned step on a device-resident state ----
+ * The state holds the kept (2/3-rule) modes of every component in a compact layout; upload and
+ * download convert from / to the reference layout (src/dst on host or device).
+ * th_step == nsteps x IncFluid::Time_advance() with the RK4 integrating-factor scheme
+ *   (time_advance.cc:100-155, _VF_MHD.cc:105-180, _SF.cc:105-222), fused passes, hipGraph replay.
+ * comp indexes: NS V1..V3 = 0..2; MHD U 0..2, B 3..5; RB V 0..2, T 3. */
+int th_state_create(th_plan *plan, th_state **state, int physics);
+int th_state_destroy(th_state *state);
+int th_state_upload(th_state *state, int comp, const void *src, int src_on_host);
+int th_state_download(th_state *state, int comp, void *dst, int dst_on_host);
+int th_step(th_plan *plan, th_state *state, double dt, const th_phys *p, int nsteps);
+/* Pi(R_i) for spheres radii[1..nsh] and T(n,m) for giver shells n = 1..nsh-1 over the state's
+ * velocity (Compute_flux + Compute_shell_tr), from one giver-masked fused pipeline per shell. */
+int th_flux_shell_tr(th_plan *plan, th_state *state, const double *radii, int nsh, double *Pi, double *T);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,7 @@
+"""tarang_amd: MI355X-native pseudo-spectral time step (Tarang FOUR / SCFT bases).
+
+The product is the C-ABI library libtarang_hip.so (include/tarang_hip.h) built for gfx950.
+This package only locates / builds it and gives Python a thin ctypes view of it (tests, bench).
+There is no CPU fallback: if the library is missing, loading fails loudly.
+"""
+from .abi import Context, Plan, State, lib, LIB_PATH  # noqa: F401

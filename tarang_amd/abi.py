@@ -1,0 +1,452 @@
+"""ctypes binding of libtarang_hip.so (the C-ABI in include/tarang_hip.h).
+
+Device arrays are torch tensors on cuda (torch is used only for device memory): complex128 for
+FP64 plans, complex64 for FP32-mixed plans, shape (N1, N2, N3//2+1) = the reference layout.
+The library runs on its own HIP stream; every wrapper synchronises torch's stream before the call
+and the library stream after it, so callers can mix the two freely.
+"""
+import atexit
+import ctypes as C
+import os
+import weakref
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtarang_hip.so")
+
+TH_FOUR, TH_SCFT = 0, 1
+TH_FP64, TH_FP32_MIXED = 0, 1
+TH_NS, TH_MHD, TH_RB = 0, 1, 2
+
+_lib = None
+_live_plans = weakref.WeakSet()
+_live_ctx = weakref.WeakSet()
+_shutting_down = False
+
+
+@atexit.register
+def _release_all():
+    """Destroy plans, then contexts, before the HIP runtime is torn down at process exit."""
+    global _shutting_down
+    for obj in list(_live_plans) + list(_live_states if "_live_states" in globals() else []):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    for c in list(_live_ctx):
+        try:
+            c.close()
+        except Exception:
+            pass
+    _shutting_down = True
+
+
+class Phys(C.Structure):
+    _fields_ = [("nu", C.c_double), ("eta", C.c_double), ("kappa", C.c_double),
+                ("Ra", C.c_double), ("Pr", C.c_double)]
+
+
+def lib():
+    """Load the gfx950 library.  Raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m tarang_amd.build` "
+                           "(the HIP path has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d, sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+    pp = C.POINTER(C.c_void_p)
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "th_ctx_create": (i, [C.POINTER(vp), i, C.c_char_p]),
+        "th_ctx_destroy": (i, [vp]),
+        "th_last_error": (C.c_char_p, []),
+        "th_version": (C.c_char_p, []),
+        "th_plan_create": (i, [vp, C.POINTER(vp), i, i, i, i, dp, i, i]),
+        "th_plan_destroy": (i, [vp]),
+        "th_plan_field_elems": (sz, [vp]),
+        "th_plan_elem_bytes": (sz, [vp]),
+        "th_shell_ek_size": (i, [vp]),
+        "th_max_radius_inside": (i, [vp]),
+        "th_sync": (i, [vp]),
+        "th_forward": (i, [vp, vp, i]),
+        "th_inverse": (i, [vp, vp, i]),
+        "th_deriv": (i, [vp, i, vp, vp, i]),
+        "th_dealias": (i, [vp, vp]),
+        "th_real_mult": (i, [vp, vp, vp, vp]),
+        "th_divide_ksqr": (i, [vp, vp]),
+        "th_exp_ksqr": (i, [vp, vp, d]),
+        "th_nlin_ns": (i, [vp, pp, pp]),
+        "th_pressure": (i, [vp, pp, vp]),
+        "th_rhs": (i, [vp, pp, vp]),
+        "th_nlin_mhd": (i, [vp, pp, pp, pp, pp]),
+        "th_nlin_rb": (i, [vp, pp, vp, pp, vp]),
+        "th_et_nlin": (i, [vp, pp, pp, pp]),
+        "th_step_plain": (i, [vp, i, i, pp, d, C.POINTER(Phys)]),
+        "th_energy": (i, [vp, vp, vp, dp]),
+        "th_Sn": (i, [vp, vp, d, dp]),
+        "th_shell_spectrum": (i, [vp, vp, vp, i, dp, dp]),
+        "th_fill_array_shell": (i, [vp, vp, vp, d, d]),
+        "th_shell_mult_single": (i, [vp, pp, pp, d, d, dp]),
+        "th_shell_mult_all": (i, [vp, pp, pp, dp, i, dp]),
+        "th_flux_ns": (i, [vp, pp, dp, i, dp]),
+        "th_shell_tr_ns": (i, [vp, pp, dp, i, dp]),
+        "th_nusselt": (i, [vp, pp, vp, d, d, dp]),
+        "th_ic_taylor_green": (i, [vp, pp, i, d]),
+        "th_ic_random": (i, [vp, pp, C.c_ulonglong, d, d]),
+        "th_ic_random_scalar": (i, [vp, vp, C.c_ulonglong, d]),
+        "th_state_create": (i, [vp, C.POINTER(vp), i]),
+        "th_state_destroy": (i, [vp]),
+        "th_state_upload": (i, [vp, i, vp, i]),
+        "th_state_download": (i, [vp, i, vp, i]),
+        "th_step": (i, [vp, vp, d, C.POINTER(Phys), i]),
+        "th_flux_shell_tr": (i, [vp, vp, dp, i, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().th_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _pp(ts):
+    return (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+
+def _dbl(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    def __init__(self, device=0, tune_cache=None):
+        self.h = C.c_void_p()
+        _check(lib().th_ctx_create(C.byref(self.h), device,
+                                   tune_cache.encode() if tune_cache else None), "th_ctx_create")
+        self.device = device
+        _live_ctx.add(self)
+
+    def close(self):
+        if self.h and not _shutting_down:
+            if lib().th_ctx_destroy(self.h) == 0:
+                self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """One box: basis, grid, kfactor, precision, dealias switch."""
+
+    def __init__(self, ctx, N, kfactor=(1.0, 1.0, 1.0), basis="FOUR", precision="fp64", dealias=True):
+        self.ctx = ctx
+        self.N = tuple(int(n) for n in N)
+        self.basis = TH_FOUR if basis == "FOUR" else TH_SCFT
+        self.prec = TH_FP64 if precision == "fp64" else TH_FP32_MIXED
+        kf = (C.c_double * 3)(*[float(v) for v in kfactor])
+        self.h = C.c_void_p()
+        _check(lib().th_plan_create(ctx.h, C.byref(self.h), self.basis, *self.N, kf, self.prec,
+                                    1 if dealias else 0), "th_plan_create")
+        _live_plans.add(self)
+
+    def close(self):
+        if self.h and not _shutting_down:
+            for st in list(_live_states):
+                if st.plan is self:
+                    st.close()
+            lib().th_plan_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.N[0], self.N[1], self.N[2] // 2 + 1)
+
+    @property
+    def dtype(self):
+        torch = _torch()
+        return torch.complex128 if self.prec == TH_FP64 else torch.complex64
+
+    def empty(self, n=None):
+        torch = _torch()
+        dev = f"cuda:{self.ctx.device}"
+        if n is None:
+            return torch.zeros(self.shape, dtype=self.dtype, device=dev)
+        return [torch.zeros(self.shape, dtype=self.dtype, device=dev) for _ in range(n)]
+
+    def to_dev(self, arr):
+        torch = _torch()
+        t = torch.from_numpy(arr).to(device=f"cuda:{self.ctx.device}", dtype=self.dtype).contiguous()
+        return t
+
+    def _pre(self):
+        _torch().cuda.synchronize(self.ctx.device)
+
+    def _post(self, rc, what):
+        _check(rc, what)
+        _check(lib().th_sync(self.h), "th_sync")
+
+    # transforms and ops
+    def forward(self, A, parity=0):
+        self._pre()
+        self._post(lib().th_forward(self.h, _ptr(A), parity), "th_forward")
+        return A
+
+    def inverse(self, A, parity=0):
+        self._pre()
+        self._post(lib().th_inverse(self.h, _ptr(A), parity), "th_inverse")
+        return A
+
+    def deriv(self, axis, A, parity=0, out=None):
+        B = out if out is not None else _torch().empty_like(A)
+        self._pre()
+        self._post(lib().th_deriv(self.h, axis, _ptr(A), _ptr(B), parity), "th_deriv")
+        return B
+
+    def dealias(self, A):
+        self._pre()
+        self._post(lib().th_dealias(self.h, _ptr(A)), "th_dealias")
+        return A
+
+    def real_mult(self, A, B):
+        Cc = _torch().empty_like(A)
+        self._pre()
+        self._post(lib().th_real_mult(self.h, _ptr(A), _ptr(B), _ptr(Cc)), "th_real_mult")
+        return Cc
+
+    def divide_ksqr(self, A):
+        self._pre()
+        self._post(lib().th_divide_ksqr(self.h, _ptr(A)), "th_divide_ksqr")
+        return A
+
+    def exp_ksqr(self, A, factor):
+        self._pre()
+        self._post(lib().th_exp_ksqr(self.h, _ptr(A), factor), "th_exp_ksqr")
+        return A
+
+    def nlin_ns(self, V):
+        N = self.empty(3)
+        self._pre()
+        self._post(lib().th_nlin_ns(self.h, _pp(V), _pp(N)), "th_nlin_ns")
+        return N
+
+    def pressure(self, nl):
+        p = self.empty()
+        self._pre()
+        self._post(lib().th_pressure(self.h, _pp(nl), _ptr(p)), "th_pressure")
+        return p
+
+    def rhs(self, nl, p):
+        self._pre()
+        self._post(lib().th_rhs(self.h, _pp(nl), _ptr(p)), "th_rhs")
+        return nl
+
+    def nlin_mhd(self, U, B):
+        nU, nB = self.empty(3), self.empty(3)
+        self._pre()
+        self._post(lib().th_nlin_mhd(self.h, _pp(U), _pp(B), _pp(nU), _pp(nB)), "th_nlin_mhd")
+        return nU, nB
+
+    def nlin_rb(self, V, T):
+        nV, nT = self.empty(3), self.empty()
+        self._pre()
+        self._post(lib().th_nlin_rb(self.h, _pp(V), _ptr(T), _pp(nV), _ptr(nT)), "th_nlin_rb")
+        return nV, nT
+
+    def et_nlin(self, H, G):
+        N = self.empty(3)
+        self._pre()
+        self._post(lib().th_et_nlin(self.h, _pp(H), _pp(G), _pp(N)), "th_et_nlin")
+        return N
+
+    def step_plain(self, physics, fields, dt, nu=0.0, eta=0.0, kappa=0.0, Ra=0.0, Pr=0.0, scheme=2,
+                   steps=1):
+        ph = Phys(nu, eta, kappa, Ra, Pr)
+        self._pre()
+        for _ in range(steps):
+            _check(lib().th_step_plain(self.h, physics, scheme, _pp(fields), dt, C.byref(ph)),
+                   "th_step_plain")
+        self._post(0, "")
+        return fields
+
+    # diagnostics
+    def energy(self, A, B=None):
+        out = C.c_double()
+        self._pre()
+        _check(lib().th_energy(self.h, _ptr(A), _ptr(B) if B is not None else None, C.byref(out)),
+               "th_energy")
+        return out.value
+
+    def Sn(self, A, n):
+        out = C.c_double()
+        self._pre()
+        _check(lib().th_Sn(self.h, _ptr(A), float(n), C.byref(out)), "th_Sn")
+        return out.value
+
+    def total_energy(self, V):
+        return sum(self.energy(a) for a in V)
+
+    def total_dissipation(self, V):
+        return 2 * sum(self.Sn(a, 2) for a in V)
+
+    def shell_spectrum(self, A, B=None, n=0):
+        import numpy as np
+        size = lib().th_shell_ek_size(self.h)
+        Sk = np.zeros(size)
+        cnt = np.zeros(size)
+        self._pre()
+        _check(lib().th_shell_spectrum(self.h, _ptr(A), _ptr(B) if B is not None else None, int(n),
+                                       Sk.ctypes.data_as(C.POINTER(C.c_double)),
+                                       cnt.ctypes.data_as(C.POINTER(C.c_double))), "th_shell_spectrum")
+        return Sk, cnt
+
+    def vector_spectrum(self, V, n=0):
+        return sum(self.shell_spectrum(a, None, n)[0] for a in V)
+
+    def fill_shell(self, A, inner, outer):
+        B = _torch().empty_like(A)
+        self._pre()
+        self._post(lib().th_fill_array_shell(self.h, _ptr(A), _ptr(B), inner, outer), "th_fill_array_shell")
+        return B
+
+    def shell_mult_single(self, A, B, inner, outer):
+        out = C.c_double()
+        self._pre()
+        _check(lib().th_shell_mult_single(self.h, _pp(A), _pp(B), inner, outer, C.byref(out)),
+               "th_shell_mult_single")
+        return out.value
+
+    def shell_mult_all(self, A, B, radii):
+        import numpy as np
+        r, rp = _dbl(radii)
+        out = np.zeros(len(r))
+        self._pre()
+        _check(lib().th_shell_mult_all(self.h, _pp(A), _pp(B), rp, len(r),
+                                       out.ctypes.data_as(C.POINTER(C.c_double))), "th_shell_mult_all")
+        return out
+
+    def flux_ns(self, V, radii):
+        import numpy as np
+        r, rp = _dbl(radii)
+        flux = np.zeros(len(r))
+        self._pre()
+        _check(lib().th_flux_ns(self.h, _pp(V), rp, len(r) - 1, flux.ctypes.data_as(C.POINTER(C.c_double))),
+               "th_flux_ns")
+        return flux
+
+    def shell_tr_ns(self, V, radii):
+        import numpy as np
+        r, rp = _dbl(radii)
+        nsh = len(r) - 1
+        T = np.zeros((nsh + 1, nsh + 1))
+        self._pre()
+        _check(lib().th_shell_tr_ns(self.h, _pp(V), rp, nsh, T.ctypes.data_as(C.POINTER(C.c_double))),
+               "th_shell_tr_ns")
+        return T
+
+    def nusselt(self, V, T, Ra, Pr):
+        out = C.c_double()
+        self._pre()
+        _check(lib().th_nusselt(self.h, _pp(V), _ptr(T), Ra, Pr, C.byref(out)), "th_nusselt")
+        return out.value
+
+    # initial conditions
+    def ic_taylor_green(self, k0=1, amp=1.0):
+        V = self.empty(3)
+        self._pre()
+        self._post(lib().th_ic_taylor_green(self.h, _pp(V), k0, amp), "th_ic_taylor_green")
+        return V
+
+    def ic_random(self, seed, E0=0.5, kp=4.0):
+        V = self.empty(3)
+        self._pre()
+        self._post(lib().th_ic_random(self.h, _pp(V), seed, E0, kp), "th_ic_random")
+        return V
+
+    def ic_random_scalar(self, seed, amp=1e-3):
+        T = self.empty()
+        self._pre()
+        self._post(lib().th_ic_random_scalar(self.h, _ptr(T), seed, amp), "th_ic_random_scalar")
+        return T
+
+
+_live_states = weakref.WeakSet()
+
+
+class State:
+    """Device-resident fused-layout fields of one box (th_state)."""
+
+    def __init__(self, plan, physics=TH_NS):
+        self.plan = plan
+        self.physics = physics
+        self.h = C.c_void_p()
+        _check(lib().th_state_create(plan.h, C.byref(self.h), physics), "th_state_create")
+        self.ncomp = {TH_NS: 3, TH_MHD: 6, TH_RB: 4}[physics]
+        _live_states.add(self)
+
+    def close(self):
+        if self.h and not _shutting_down:
+            lib().th_state_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, fields):
+        self.plan._pre()
+        for c, f in enumerate(fields):
+            _check(lib().th_state_upload(self.h, c, _ptr(f), 0), "th_state_upload")
+        self.plan._post(0, "")
+
+    def download(self):
+        out = self.plan.empty(self.ncomp)
+        self.plan._pre()
+        for c, f in enumerate(out):
+            _check(lib().th_state_download(self.h, c, _ptr(f), 0), "th_state_download")
+        self.plan._post(0, "")
+        return out
+
+    def step(self, dt, nu=0.0, eta=0.0, kappa=0.0, Ra=0.0, Pr=0.0, steps=1):
+        ph = Phys(nu, eta, kappa, Ra, Pr)
+        self.plan._pre()
+        self.plan._post(lib().th_step(self.plan.h, self.h, dt, C.byref(ph), steps), "th_step")
+
+    def flux_shell_tr(self, radii):
+        import numpy as np
+        r, rp = _dbl(radii)
+        nsh = len(r) - 1
+        Pi = np.zeros(nsh + 1)
+        T = np.zeros((nsh + 1, nsh + 1))
+        self.plan._pre()
+        _check(lib().th_flux_shell_tr(self.plan.h, self.h, rp, nsh,
+                                      Pi.ctypes.data_as(C.POINTER(C.c_double)),
+                                      T.ctypes.data_as(C.POINTER(C.c_double))), "th_flux_shell_tr")
+        return Pi, T

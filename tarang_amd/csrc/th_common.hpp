@@ -1,0 +1,28 @@
+// th_common.hpp -- shared device/host definitions for the gfx950 pseudo-spectral kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace th {
+
+template <typename R> struct alignas(2 * sizeof(R)) cplx { R x, y; };
+using c32 = cplx<float>;
+using c64 = cplx<double>;
+
+template <typename R> __host__ __device__ __forceinline__ cplx<R> mk(R a, R b) { cplx<R> c; c.x = a; c.y = b; return c; }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) { return mk<R>(a.x + b.x, a.y + b.y); }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) { return mk<R>(a.x - b.x, a.y - b.y); }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) { return mk<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> cmulc(cplx<R> a, cplx<R> b) { return mk<R>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); } // a*conj(b)
+template <typename R> __host__ __device__ __forceinline__ cplx<R> cscale(cplx<R> a, R s) { return mk<R>(a.x * s, a.y * s); }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> conjg(cplx<R> a) { return mk<R>(a.x, -a.y); }
+template <typename R> __host__ __device__ __forceinline__ cplx<R> mul_i(cplx<R> a) { return mk<R>(-a.y, a.x); }      // i*a
+template <typename R> __host__ __device__ __forceinline__ cplx<R> mul_mi(cplx<R> a) { return mk<R>(a.y, -a.x); }     // -i*a
+template <typename R> __host__ __device__ __forceinline__ cplx<R> czero() { return mk<R>(R(0), R(0)); }
+
+// Wavenumber helpers (trunk/lib/four/four_inline.h:88-138, trunk/lib/scft/scft_inline.h:86)
+__host__ __device__ __forceinline__ int kx_four(int l1, int N1) { return (l1 <= N1 / 2) ? l1 : l1 - N1; }
+__host__ __device__ __forceinline__ int ky_of(int l2, int N2) { return (l2 <= N2 / 2) ? l2 : l2 - N2; }
+
+}  // namespace th

@@ -1,0 +1,165 @@
+// th_fft.hpp -- cooperative line FFT for gfx950: P threads of a workgroup own one line of length L
+// (E = L/P points each, held in VGPRs).  Stockham autosort stages (radix 8/4/2/3) run in registers;
+// between stages the line is exchanged through LDS.  The first stage reads its inputs from the
+// registers the caller filled in "input order" and the last stage leaves outputs in natural order
+// in registers, so HBM is touched only by the caller's coalesced loads and stores.
+//
+// Convention (FFTW's, as used by trunk/lib/four/four_tr.cc): SIGN = -1 forward exp(-2 pi i jk/L),
+// SIGN = +1 backward, no normalisation.  Twiddles W_L^m = exp(-2 pi i m/L) come from a table that
+// the host fills from FP64 libm (rounded to the storage precision).
+#pragma once
+#include "th_common.hpp"
+
+namespace th {
+
+struct Sched {
+    int n;
+    int r[16];
+};
+
+// Greedy radix schedule: each radix must divide the remaining length and the points per thread.
+constexpr Sched make_sched(int L, int E)
+{
+    Sched s{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+    int m = L;
+    while (m > 1) {
+        int R = 0;
+        if (m % 8 == 0 && E % 8 == 0) R = 8;
+        else if (m % 4 == 0 && E % 4 == 0) R = 4;
+        else if (m % 2 == 0 && E % 2 == 0) R = 2;
+        else if (m % 3 == 0 && E % 3 == 0) R = 3;
+        if (R == 0) { s.n = -1; return s; }
+        s.r[s.n++] = R;
+        m /= R;
+    }
+    return s;
+}
+
+template <typename R> struct K {
+    static constexpr R s12 = R(0.70710678118654752440084436210485);   // sqrt(1/2)
+    static constexpr R s32 = R(0.86602540378443864676372317075294);   // sqrt(3)/2
+};
+
+template <int SIGN, typename R> __device__ __forceinline__ cplx<R> mul_si(cplx<R> a)   // (SIGN*i)*a
+{
+    return SIGN < 0 ? mul_mi(a) : mul_i(a);
+}
+
+template <int RADIX, int SIGN, typename R> struct DFT;
+
+template <int SIGN, typename R> struct DFT<2, SIGN, R> {
+    static __device__ __forceinline__ void run(cplx<R>* v)
+    {
+        cplx<R> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    }
+};
+
+template <int SIGN, typename R> struct DFT<4, SIGN, R> {
+    static __device__ __forceinline__ void run(cplx<R>* v)
+    {
+        cplx<R> t0 = v[0] + v[2], t1 = v[0] - v[2];
+        cplx<R> t2 = v[1] + v[3], t3 = mul_si<SIGN>(v[1] - v[3]);
+        v[0] = t0 + t2;
+        v[2] = t0 - t2;
+        v[1] = t1 + t3;
+        v[3] = t1 - t3;
+    }
+};
+
+template <int SIGN, typename R> struct DFT<8, SIGN, R> {
+    static __device__ __forceinline__ void run(cplx<R>* v)
+    {
+        const R c = K<R>::s12;
+        cplx<R> a[4], b[4];
+#pragma unroll
+        for (int n = 0; n < 4; n++) { a[n] = v[n] + v[n + 4]; b[n] = v[n] - v[n + 4]; }
+        // b[n] *= W8^n, W8 = exp(SIGN i pi/4)
+        b[1] = mk<R>(c * (b[1].x - SIGN * b[1].y), c * (b[1].y + SIGN * b[1].x));
+        b[2] = mul_si<SIGN>(b[2]);
+        b[3] = mk<R>(-c * (b[3].x + SIGN * b[3].y), c * (SIGN * b[3].x - b[3].y));
+        DFT<4, SIGN, R>::run(a);
+        DFT<4, SIGN, R>::run(b);
+#pragma unroll
+        for (int k = 0; k < 4; k++) { v[2 * k] = a[k]; v[2 * k + 1] = b[k]; }
+    }
+};
+
+template <int SIGN, typename R> struct DFT<3, SIGN, R> {
+    static __device__ __forceinline__ void run(cplx<R>* v)
+    {
+        cplx<R> t1 = v[1] + v[2];
+        cplx<R> t2 = v[0] - cscale(t1, R(0.5));
+        cplx<R> t3 = cscale(v[1] - v[2], R(SIGN) * K<R>::s32);
+        v[0] = v[0] + t1;
+        v[1] = t2 + mul_i(t3);
+        v[2] = t2 - mul_i(t3);
+    }
+};
+
+// One line of length L, P cooperating threads, E = L/P points per thread.
+//   before run(): v[i] = x[in_pos(t, i)]       after run(): v[i] = X[out_pos(t, i)]
+// lds points at element 0 of this line; element m lives at lds[m * LS].
+template <typename R, int L, int P, int SIGN> struct LineFFT {
+    static constexpr int E = L / P;
+    static constexpr Sched S = make_sched(L, E);
+    static_assert(L % P == 0, "P must divide L");
+    static_assert(S.n > 0, "no radix schedule for (L, E)");
+    static constexpr int NST = S.n;
+    static constexpr int R0 = S.r[0];
+    static constexpr int RL = S.r[S.n - 1];
+
+    static __device__ __forceinline__ int in_pos(int t, int i) { return t + P * (i / R0) + (i % R0) * (L / R0); }
+    static __device__ __forceinline__ int out_pos(int t, int i) { return t + P * (i / RL) + (i % RL) * (L / RL); }
+
+    template <int LS>
+    static __device__ __forceinline__ void run(cplx<R> (&v)[E], cplx<R>* lds, int t, const cplx<R>* __restrict__ tw)
+    {
+        stage<0, 1, LS>(v, lds, t, tw);
+    }
+
+    template <int s, int Ns, int LS>
+    static __device__ __forceinline__ void stage(cplx<R> (&v)[E], cplx<R>* lds, int t, const cplx<R>* __restrict__ tw)
+    {
+        constexpr int Rr = S.r[s];
+        constexpr int B = E / Rr;
+#pragma unroll
+        for (int b = 0; b < B; b++) {
+            if constexpr (Ns > 1) {
+                const int j = t + P * b;
+                const int k = j % Ns;
+#pragma unroll
+                for (int r = 1; r < Rr; r++) {
+                    cplx<R> w = tw[r * k * (L / (Ns * Rr))];
+                    if (SIGN > 0) w.y = -w.y;
+                    v[b * Rr + r] = cmul(v[b * Rr + r], w);
+                }
+            }
+            DFT<Rr, SIGN, R>::run(&v[b * Rr]);
+        }
+        if constexpr (s + 1 < NST) {
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                const int j = t + P * b;
+                const int base = (j / Ns) * Ns * Rr + (j % Ns);
+#pragma unroll
+                for (int r = 0; r < Rr; r++) lds[(base + r * Ns) * LS] = v[b * Rr + r];
+            }
+            __syncthreads();
+            constexpr int Rn = S.r[s + 1];
+#pragma unroll
+            for (int i = 0; i < E; i++) v[i] = lds[(t + P * (i / Rn) + (i % Rn) * (L / Rn)) * LS];
+            stage<s + 1, Ns * Rr, LS>(v, lds, t, tw);
+        }
+    }
+};
+
+// Points per thread used by the generic passes for a line of length L.
+constexpr int pts_per_thread(int L)
+{
+    return (L % 3 == 0) ? (L >= 12 ? 12 : L) : (L >= 512 ? 16 : (L >= 8 ? 8 : L));
+}
+
+}  // namespace th

@@ -1,0 +1,106 @@
+// th_internal.hpp -- host-side internals shared by the translation units of libtarang_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <map>
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "th_common.hpp"
+
+struct th_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string tune_path;
+    std::atomic<int> nplans{0};
+};
+
+namespace th {
+
+// status codes (mirrored in include/tarang_hip.h)
+enum { OK = 0, E_ARG = -1, E_HIP = -2, E_UNSUPPORTED = -3, E_STATE = -4, E_NOMEM = -5 };
+
+void set_error(const char* fmt, ...);
+
+#define TH_CHECK_HIP(expr)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            th::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); \
+            return th::E_HIP;                                                              \
+        }                                                                                  \
+    } while (0)
+
+// Geometry of one box (reference layout (N1, N2, N3/2+1) complex, C-order).
+struct Geo {
+    int basis = 0;          // 0 FOUR, 1 SCFT
+    int N1 = 0, N2 = 0, N3 = 0, H = 0;
+    double kf[3] = {1, 1, 1};
+    int dealias = 1;
+    size_t ncplx() const { return (size_t)N1 * N2 * H; }
+};
+
+// Twiddle tables W_L^m = exp(-2 pi i m / L), m < L, built from FP64 libm on the host.
+struct TwiddleCache {
+    std::map<int, void*> f32, f64;
+    const void* get(int L, int prec /*0 fp64, 1 fp32*/, hipStream_t s);
+    void release();
+};
+
+// ---------------- generic (plain) kernels: th_generic.hip ----------------
+template <typename R> int fft_strided(cplx<R>* data, long sO, long sL, int O, int L, int M, int sign, R scale,
+                                      const cplx<R>* tw, hipStream_t st);
+template <typename R> int fft_z_r2c(cplx<R>* data, long nlines, int N3, R scale, const cplx<R>* twL,
+                                    const cplx<R>* twN, hipStream_t st);
+template <typename R> int fft_z_c2r(cplx<R>* data, long nlines, int N3, const cplx<R>* twL, const cplx<R>* twN,
+                                    hipStream_t st);
+template <typename R> int r2r_x(cplx<R>* data, const Geo& g, int kind, cplx<R>* work, const cplx<R>* tw2N,
+                                const cplx<R>* tw4N, hipStream_t st);
+template <typename R> int shift_x(cplx<R>* data, const Geo& g, int dir, hipStream_t st);
+
+// elementwise spectral ops on reference-layout arrays
+template <typename R> int k_deriv(const Geo& g, int axis, int parity, const cplx<R>* A, cplx<R>* B, hipStream_t st);
+template <typename R> int k_add_deriv(const Geo& g, int axis, int parity, const cplx<R>* A, cplx<R>* B,
+                                      hipStream_t st);       // B += D(A)
+template <typename R> int k_dealias(const Geo& g, cplx<R>* A, hipStream_t st);
+template <typename R> int k_real_mult(size_t n, const cplx<R>* A, const cplx<R>* B, cplx<R>* C, hipStream_t st);
+template <typename R> int k_divide_ksqr(const Geo& g, cplx<R>* A, hipStream_t st);
+template <typename R> int k_exp_ksqr(const Geo& g, cplx<R>* A, double factor, hipStream_t st);
+template <typename R> int k_axpy(size_t n, cplx<R>* Y, double a, const cplx<R>* X, hipStream_t st);   // Y = Y + a X
+template <typename R> int k_axpby(size_t n, cplx<R>* Y, double a, const cplx<R>* X, double b, hipStream_t st); // Y = a X + b Y
+template <typename R> int k_scale(size_t n, cplx<R>* Y, double a, hipStream_t st);
+template <typename R> int k_fill_shell(const Geo& g, const cplx<R>* A, cplx<R>* B, double inner, double outer,
+                                       hipStream_t st);
+template <typename R> int k_zero_plane_x0(const Geo& g, cplx<R>* A, hipStream_t st);
+template <typename R> int k_zero_padding(const Geo& g, cplx<R>* A, hipStream_t st);
+
+// ---------------- reductions: th_reduce.hip ----------------
+struct RedScratch {
+    double* d = nullptr;
+    size_t cap = 0;
+    double* h = nullptr;
+    size_t hcap = 0;
+    int ensure(size_t n_dev, size_t n_host);
+    void release();
+};
+// binned sum over modes: out[index] += w(mode) * f(mode); modes with index outside [0, nbins) skipped
+// mode selects the weight/binning rule (spectrum, shell_mult_all, shell_mult_single ...)
+template <typename R> int reduce_shell(const Geo& g, const cplx<R>* A, const cplx<R>* B, double n, int nbins,
+                                       double* Sk, double* cnt, RedScratch& rs, hipStream_t st, bool rescale = true);
+template <typename R> int reduce_energy(const Geo& g, const cplx<R>* A, const cplx<R>* B, double* out,
+                                        RedScratch& rs, hipStream_t st);
+template <typename R> int reduce_Sn(const Geo& g, const cplx<R>* A, double n, double* out, RedScratch& rs,
+                                    hipStream_t st);
+template <typename R> int reduce_shell_mult_single(const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3],
+                                                   double inner, double outer, double* out, RedScratch& rs,
+                                                   hipStream_t st);
+template <typename R> int reduce_shell_mult_all(const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3],
+                                                const double* radii_dev, int nr, double* out, RedScratch& rs,
+                                                hipStream_t st);
+
+// host-side binning helpers (four_inline.h, scft_inline.h)
+int min_radius_outside(const Geo& g);
+int max_radius_inside(const Geo& g);
+double approx_modes_in_shell(const Geo& g, int radius);
+
+}  // namespace th

@@ -1,0 +1,368 @@
+// th_reduce.hip -- FP64 shell binning and sums over the reference layout.
+//   Compute_local_shell_spectrum_FOUR  trunk/lib/four/four_energy.cc:244-288 (SCFT scft_energy.cc:238-290)
+//   Get_local_energy_FOUR              four_energy.cc:48-168 (SCFT scft_energy.cc:51-101)
+//   Get_local_Sn_FOUR                  four_energy.cc:179-218
+//   Local_shell_mult_single_FOUR       trunk/lib/four/four_ET.cc:49-80
+//   Local_shell_mult_all_FOUR (dot)    four_ET.cc:181-229, Get_shell_index basis_basicfn.cc:288-299
+// K = sqrt((kx f1)^2 + (ky f2)^2 + (kz f3)^2) is evaluated in FP64 with contraction disabled, so the
+// shell index ceil(K) is bit-identical to the CPU's.  Per-block partial sums are FP64 (LDS atomics
+// for binned sums, tree reduction for scalars); blocks are merged with Neumaier-compensated sums.
+#include <math.h>
+#include "th_internal.hpp"
+
+namespace th {
+
+struct GeoR {
+    int basis, N1, N2, N3, H;
+    double f1, f2, f3;
+};
+static GeoR gr(const Geo& g) { return GeoR{g.basis, g.N1, g.N2, g.N3, g.H, g.kf[0], g.kf[1], g.kf[2]}; }
+
+__device__ __forceinline__ double kmag_r(const GeoR& g, int l1, int l2, int l3)
+{
+#pragma clang fp contract(off)
+    int kx = g.basis == 1 ? l1 : kx_four(l1, g.N1);
+    double a = kx * g.f1, b = ky_of(l2, g.N2) * g.f2, c = l3 * g.f3;
+    return sqrt(a * a + b * b + c * c);
+}
+
+__device__ __forceinline__ double mult_r(const GeoR& g, int l1, int l2, int l3)
+{
+    int kx = g.basis == 1 ? l1 : kx_four(l1, g.N1);
+    int ky = ky_of(l2, g.N2);
+    double factor;
+    if (g.basis == 0) {
+        factor = (l3 > 0) ? 1.0 : 0.5;
+        if ((kx == g.N1 / 2) || ((ky == g.N2 / 2) && (g.N2 > 1))) return 2 * factor;
+        return factor;
+    }
+    if (l3 > 0) factor = (kx > 0) ? 2.0 : 1.0;
+    else factor = (kx > 0) ? 1.0 : 0.5;
+    if ((ky == g.N2 / 2) && (g.N2 > 1)) return 2 * factor;
+    return factor;
+}
+
+__device__ __forceinline__ double my_pow_r(double x, int n)
+{
+    if (n == 0) return 1.0;
+    if (n == 2) return x * x;
+    return x * x * x * x;
+}
+
+template <typename R> __device__ __forceinline__ double sq(const cplx<R>& a) { return (double)a.x * a.x + (double)a.y * a.y; }
+template <typename R> __device__ __forceinline__ double rdot(const cplx<R>& a, const cplx<R>& b)
+{
+    return (double)a.x * b.x + (double)a.y * b.y;
+}
+
+__device__ __forceinline__ void decode_r(const GeoR& g, long i, int& l1, int& l2, int& l3)
+{
+    l3 = (int)(i % g.H);
+    long r = i / g.H;
+    l2 = (int)(r % g.N2);
+    l1 = (int)(r / g.N2);
+}
+
+constexpr int RBLK = 256;
+constexpr int RSEG = 16;      // consecutive modes handled by one thread (run-length accumulation)
+
+// MODE 0: spectrum (bins = ceil(K) <= kkmax; Sk and count), MODE 1: shell_mult_all (bins by radii,
+// K <= kin, value 2*mult*sum_c Re(A_c conj(B_c)))
+template <typename R, int MODE>
+__global__ __launch_bounds__(RBLK) void k_binned(GeoR g, const cplx<R>* __restrict__ A0, const cplx<R>* __restrict__ A1,
+                                                 const cplx<R>* __restrict__ A2, const cplx<R>* __restrict__ B0,
+                                                 const cplx<R>* __restrict__ B1, const cplx<R>* __restrict__ B2,
+                                                 int npow, int kkmax, const double* __restrict__ radii, int nr,
+                                                 int nbins, double* __restrict__ partial)
+{
+    extern __shared__ double sh[];
+    double* sS = sh;
+    double* sC = sh + nbins;
+    for (int b = threadIdx.x; b < 2 * nbins; b += RBLK) sh[b] = 0.0;
+    __syncthreads();
+    const long n = (long)g.N1 * g.N2 * g.H;
+    const long nseg = (n + RSEG - 1) / RSEG;
+    for (long s = (long)blockIdx.x * RBLK + threadIdx.x; s < nseg; s += (long)gridDim.x * RBLK) {
+        int cur = -1;
+        double accS = 0.0, accC = 0.0;
+        const long i0 = s * RSEG, i1 = (i0 + RSEG < n) ? i0 + RSEG : n;
+        for (long i = i0; i < i1; i++) {
+            int l1, l2, l3;
+            decode_r(g, i, l1, l2, l3);
+            double km = kmag_r(g, l1, l2, l3);
+            int idx;
+            double val, cnt = 0.0;
+            if (MODE == 0) {
+                idx = (int)ceil(km);
+                if (idx > kkmax) continue;
+                double f = mult_r(g, l1, l2, l3);
+                double v = B0 ? rdot(A0[i], B0[i]) : sq(A0[i]);
+                val = f * my_pow_r(km, npow) * v;
+                cnt = (g.basis == 0) ? f : f / 2;
+            } else {
+                if (km > (double)kkmax) continue;
+                idx = 0;
+                while (idx < nr && !(km <= radii[idx])) idx++;
+                if (idx >= nbins) continue;
+                double f = mult_r(g, l1, l2, l3);
+                val = 2 * f * (rdot(A0[i], B0[i]) + rdot(A1[i], B1[i]) + rdot(A2[i], B2[i]));
+            }
+            if (idx != cur) {
+                if (cur >= 0) {
+                    atomicAdd(&sS[cur], accS);
+                    if (MODE == 0) atomicAdd(&sC[cur], accC);
+                }
+                cur = idx;
+                accS = 0.0;
+                accC = 0.0;
+            }
+            accS += val;
+            accC += cnt;
+        }
+        if (cur >= 0) {
+            atomicAdd(&sS[cur], accS);
+            if (MODE == 0) atomicAdd(&sC[cur], accC);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 2 * nbins; b += RBLK) partial[(long)blockIdx.x * 2 * nbins + b] = sh[b];
+}
+
+// MODE 0: energy weights (Get_local_energy), MODE 1: Sn, MODE 2: shell_mult_single over 3 comps
+template <typename R, int MODE>
+__global__ __launch_bounds__(RBLK) void k_scalar(GeoR g, const cplx<R>* __restrict__ A0, const cplx<R>* __restrict__ A1,
+                                                 const cplx<R>* __restrict__ A2, const cplx<R>* __restrict__ B0,
+                                                 const cplx<R>* __restrict__ B1, const cplx<R>* __restrict__ B2,
+                                                 int npow, int kkmax, double inner, double outer,
+                                                 double* __restrict__ partial)
+{
+    __shared__ double red[RBLK];
+    double s = 0.0, c = 0.0;                 // Neumaier
+    const long n = (long)g.N1 * g.N2 * g.H;
+    for (long i = (long)blockIdx.x * RBLK + threadIdx.x; i < n; i += (long)gridDim.x * RBLK) {
+        int l1, l2, l3;
+        decode_r(g, i, l1, l2, l3);
+        double v;
+        if (MODE == 0) {
+            double w;
+            if (g.basis == 0) {
+                double ey = (l2 == g.N2 / 2 && g.N2 > 1) ? 1.0 : 0.0;
+                double ex = (l1 == g.N1 / 2) ? 1.0 : 0.0;
+                w = 1.0 + ey + ex;
+                if (l3 == 0) w -= 0.5 * (1.0 + ey + ex);
+            } else {
+                w = 2.0 - (l3 == 0 ? 1.0 : 0.0) - (l1 == 0 ? 1.0 : 0.0) + ((l1 == 0 && l3 == 0) ? 0.5 : 0.0);
+                if (l2 == g.N2 / 2 && g.N2 > 2) w *= 2.0;
+            }
+            if (i == 0) w -= 0.5;
+            v = w * (B0 ? rdot(A0[i], B0[i]) : sq(A0[i]));
+        } else if (MODE == 1) {
+            double km = kmag_r(g, l1, l2, l3);
+            if (!(km <= (double)kkmax)) continue;
+            v = mult_r(g, l1, l2, l3) * my_pow_r(km, npow) * sq(A0[i]);
+            if (i == 0 && npow == 0) v -= mult_r(g, 0, 0, 0) * sq(A0[0]);
+        } else {
+            double km = kmag_r(g, l1, l2, l3);
+            if (!((km > inner) && (km <= outer))) continue;
+            v = 2 * mult_r(g, l1, l2, l3) * (rdot(A0[i], B0[i]) + rdot(A1[i], B1[i]) + rdot(A2[i], B2[i]));
+        }
+        double t = s + v;
+        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+        s = t;
+    }
+    red[threadIdx.x] = s + c;
+    __syncthreads();
+    for (int w = RBLK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// out[b] = compensated sum over blocks of partial[blk * stride + b]
+__global__ void k_finalize(const double* __restrict__ partial, int nblk, int stride, int nval, double* __restrict__ out)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nval) return;
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < nblk; k++) {
+        double v = partial[(long)k * stride + b];
+        double t = s + v;
+        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+        s = t;
+    }
+    out[b] = s + c;
+}
+
+int RedScratch::ensure(size_t n_dev, size_t n_host)
+{
+    if (n_dev > cap) {
+        if (d) (void)hipFree(d);
+        d = nullptr;
+        TH_CHECK_HIP(hipMalloc(&d, n_dev * sizeof(double)));
+        cap = n_dev;
+    }
+    if (n_host > hcap) {
+        if (h) (void)hipHostFree(h);
+        h = nullptr;
+        TH_CHECK_HIP(hipHostMalloc(&h, n_host * sizeof(double)));
+        hcap = n_host;
+    }
+    return OK;
+}
+void RedScratch::release()
+{
+    if (d) (void)hipFree(d);
+    if (h) (void)hipHostFree(h);
+    d = nullptr;
+    h = nullptr;
+    cap = hcap = 0;
+}
+
+static int nblocks_for(size_t n)
+{
+    size_t b = (n + RBLK * RSEG - 1) / (RBLK * RSEG);
+    if (b > 1024) b = 1024;
+    return (int)(b ? b : 1);
+}
+
+int min_radius_outside(const Geo& g)
+{
+    auto p2 = [](double x) { return x * x; };
+    int N1 = g.N1, N2 = g.N2, N3 = g.N3;
+    if (g.basis == 0) {
+        if (g.dealias) return (int)ceil(sqrt(p2((N1 / 3) * g.kf[0]) + p2(N2 / 3 * g.kf[1]) + p2(N3 / 3 * g.kf[2])));
+        return (int)ceil(sqrt(p2((N1 / 2) * g.kf[0]) + p2(N2 / 2 * g.kf[1]) + p2(N3 / 2 * g.kf[2])));
+    }
+    if (g.dealias) return (int)ceil(sqrt(p2(2 * N1 / 3 * g.kf[0]) + p2(N2 / 3 * g.kf[1]) + p2(N3 / 3 * g.kf[2])));
+    return (int)ceil(sqrt(p2((N1 - 1) * g.kf[0]) + p2(N2 / 2 * g.kf[1]) + p2(N3 / 2 * g.kf[2])));
+}
+
+int max_radius_inside(const Geo& g)
+{
+    auto mn = [](double a, double b) { return a < b ? a : b; };
+    int N1 = g.N1, N2 = g.N2, N3 = g.N3;
+    double k;
+    if (g.basis == 0) {
+        if (g.dealias) { k = mn((N1 / 3) * g.kf[0], (N2 / 3) * g.kf[1]); if (N3 > 2) k = mn(k, (N3 / 3) * g.kf[2]); }
+        else { k = mn((N1 / 2) * g.kf[0], (N2 / 2) * g.kf[1]); if (N3 > 2) k = mn(k, (N3 / 2) * g.kf[2]); }
+    } else {
+        if (g.dealias) { k = mn(2 * N1 / 3 * g.kf[0], (N2 / 3) * g.kf[1]); if (N3 > 2) k = mn(k, (N3 / 3) * g.kf[2]); }
+        else { k = mn((N1 - 1) * g.kf[0], (N2 / 2) * g.kf[1]); if (N3 > 2) k = mn(k, (N3 / 2) * g.kf[2]); }
+    }
+    return (int)k;
+}
+
+double approx_modes_in_shell(const Geo& g, int r)
+{
+    double v = M_PI * r * r / (g.kf[0] * g.kf[1] * g.kf[2]);
+    return g.basis == 0 ? 2 * v : v;
+}
+
+template <typename R>
+int reduce_shell(const Geo& g, const cplx<R>* A, const cplx<R>* B, double n, int nbins, double* Sk, double* cnt,
+                 RedScratch& rs, hipStream_t st, bool rescale)
+{
+    int nb = nblocks_for(g.ncplx());
+    int kkmax = min_radius_outside(g);
+    if (nbins != kkmax + 1) { set_error("reduce_shell: nbins %d != shell_ek_size %d", nbins, kkmax + 1); return E_ARG; }
+    int r = rs.ensure((size_t)nb * 2 * nbins + 2 * nbins, 2 * nbins);
+    if (r) return r;
+    double* out = rs.d + (size_t)nb * 2 * nbins;
+    hipLaunchKernelGGL((k_binned<R, 0>), dim3(nb), dim3(RBLK), 2 * nbins * sizeof(double), st, gr(g), A,
+                       (const cplx<R>*)nullptr, (const cplx<R>*)nullptr, B, (const cplx<R>*)nullptr,
+                       (const cplx<R>*)nullptr, (int)n, kkmax, (const double*)nullptr, 0, nbins, rs.d);
+    TH_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3((2 * nbins + 255) / 256), dim3(256), 0, st, rs.d, nb, 2 * nbins, 2 * nbins, out);
+    TH_CHECK_HIP(hipGetLastError());
+    TH_CHECK_HIP(hipMemcpyAsync(rs.h, out, 2 * nbins * sizeof(double), hipMemcpyDeviceToHost, st));
+    TH_CHECK_HIP(hipStreamSynchronize(st));
+    int kin = max_radius_inside(g);
+    for (int k = 0; k < nbins; k++) {
+        double s = rs.h[k], c = rs.h[nbins + k];
+        if (rescale && k > kin && k <= kkmax && c >= 1) s = s * approx_modes_in_shell(g, k) / c;   // four_energy.cc:324-336
+        Sk[k] = s;
+        if (cnt) cnt[k] = c;
+    }
+    return OK;
+}
+
+template <typename R>
+static int run_scalar(int mode, const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3], int npow,
+                      double inner, double outer, double* out, RedScratch& rs, hipStream_t st)
+{
+    int nb = nblocks_for(g.ncplx() / 4 + 1);
+    int r = rs.ensure((size_t)nb + 1, 1);
+    if (r) return r;
+    int kkmax = min_radius_outside(g);
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(RBLK), 0, st, gr(g), A[0], A[1], A[2], B[0], B[1], B[2], npow, kkmax,
+                           inner, outer, rs.d);
+    };
+    if (mode == 0) launch(k_scalar<R, 0>);
+    else if (mode == 1) launch(k_scalar<R, 1>);
+    else launch(k_scalar<R, 2>);
+    TH_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, rs.d, nb, 1, 1, rs.d + nb);
+    TH_CHECK_HIP(hipGetLastError());
+    TH_CHECK_HIP(hipMemcpyAsync(rs.h, rs.d + nb, sizeof(double), hipMemcpyDeviceToHost, st));
+    TH_CHECK_HIP(hipStreamSynchronize(st));
+    *out = rs.h[0];
+    return OK;
+}
+
+template <typename R>
+int reduce_energy(const Geo& g, const cplx<R>* A, const cplx<R>* B, double* out, RedScratch& rs, hipStream_t st)
+{
+    const cplx<R>* As[3] = {A, nullptr, nullptr};
+    const cplx<R>* Bs[3] = {B, nullptr, nullptr};
+    return run_scalar<R>(0, g, As, Bs, 0, 0, 0, out, rs, st);
+}
+template <typename R>
+int reduce_Sn(const Geo& g, const cplx<R>* A, double n, double* out, RedScratch& rs, hipStream_t st)
+{
+    const cplx<R>* As[3] = {A, nullptr, nullptr};
+    const cplx<R>* Bs[3] = {nullptr, nullptr, nullptr};
+    return run_scalar<R>(1, g, As, Bs, (int)n, 0, 0, out, rs, st);
+}
+template <typename R>
+int reduce_shell_mult_single(const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3], double inner,
+                             double outer, double* out, RedScratch& rs, hipStream_t st)
+{
+    return run_scalar<R>(2, g, A, B, 0, inner, outer, out, rs, st);
+}
+
+template <typename R>
+int reduce_shell_mult_all(const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3], const double* radii_dev,
+                          int nr, double* out, RedScratch& rs, hipStream_t st)
+{
+    int nb = nblocks_for(g.ncplx());
+    int nbins = nr;
+    int r = rs.ensure((size_t)nb * 2 * nbins + 2 * nbins, 2 * nbins);
+    if (r) return r;
+    double* dout = rs.d + (size_t)nb * 2 * nbins;
+    int kin = max_radius_inside(g);
+    hipLaunchKernelGGL((k_binned<R, 1>), dim3(nb), dim3(RBLK), 2 * nbins * sizeof(double), st, gr(g), A[0], A[1], A[2],
+                       B[0], B[1], B[2], 0, kin, radii_dev, nr, nbins, rs.d);
+    TH_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3((nbins + 255) / 256), dim3(256), 0, st, rs.d, nb, 2 * nbins, nbins, dout);
+    TH_CHECK_HIP(hipGetLastError());
+    TH_CHECK_HIP(hipMemcpyAsync(rs.h, dout, nbins * sizeof(double), hipMemcpyDeviceToHost, st));
+    TH_CHECK_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < nbins; k++) out[k] = rs.h[k];
+    return OK;
+}
+
+#define TH_RINST(R)                                                                                                  \
+    template int reduce_shell<R>(const Geo&, const cplx<R>*, const cplx<R>*, double, int, double*, double*,         \
+                                 RedScratch&, hipStream_t, bool);                                                    \
+    template int reduce_energy<R>(const Geo&, const cplx<R>*, const cplx<R>*, double*, RedScratch&, hipStream_t);   \
+    template int reduce_Sn<R>(const Geo&, const cplx<R>*, double, double*, RedScratch&, hipStream_t);               \
+    template int reduce_shell_mult_single<R>(const Geo&, const cplx<R>* const*, const cplx<R>* const*, double,     \
+                                             double, double*, RedScratch&, hipStream_t);                             \
+    template int reduce_shell_mult_all<R>(const Geo&, const cplx<R>* const*, const cplx<R>* const*, const double*, \
+                                          int, double*, RedScratch&, hipStream_t);
+TH_RINST(float)
+TH_RINST(double)
+
+}  // namespace th

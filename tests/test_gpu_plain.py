@@ -1,0 +1,299 @@
+"""GPU parity of the plain HIP path (reference operation sequence) against the FP64 oracle.
+
+Tolerances: FP64 plans match the oracle to a few ulps of the largest value (transforms 1e-13,
+nonlinear terms 1e-12 relative to max |.|); FP32-mixed plans to 2e-6 relative.  Binning and
+mode counts are compared bit for bit."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from pyoracle import Oracle  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    from tarang_amd import Context
+    return Context(0)
+
+
+def H(t):
+    return t.detach().cpu().numpy().astype(np.complex128)
+
+
+def rel(a, b):
+    m = np.abs(b).max()
+    return np.abs(a - b).max() / (m if m > 0 else 1.0)
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+GRIDS = [(8, 8, 8), (16, 12, 24), (32, 32, 32), (64, 48, 16)]
+
+
+@pytest.mark.parametrize("N", GRIDS)
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 2e-6)])
+def test_four_transforms(ctx, N, prec, tol):
+    from tarang_amd import Plan
+    log(f"four transforms N={N} {prec}")
+    o = Oracle(N, dealias=False)
+    p = Plan(ctx, N, precision=prec, dealias=False)
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+    ref = A.copy()
+    o.inverse(ref)
+    d = p.to_dev(A)
+    p.inverse(d)
+    assert rel(H(d), ref) < tol
+    ref2 = ref.copy()
+    o.forward(ref2)
+    p.forward(d)
+    assert rel(H(d), ref2) < tol
+
+
+@pytest.mark.parametrize("N", [(8, 8, 8), (16, 12, 24), (32, 32, 16)])
+@pytest.mark.parametrize("parity", [0, 1])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 3e-6)])
+def test_scft_transforms(ctx, N, parity, prec, tol):
+    from tarang_amd import Plan
+    log(f"scft transforms N={N} parity={parity} {prec}")
+    kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
+    o = Oracle(N, kfactor=kf, basis="SCFT", dealias=False)
+    p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision=prec, dealias=False)
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+    if parity:
+        A[0] = 0
+    ref = A.copy()
+    o.inverse(ref, parity)
+    d = p.to_dev(A)
+    p.inverse(d, parity)
+    assert rel(H(d), ref) < tol
+    ref2 = ref.copy()
+    o.forward(ref2, parity)
+    p.forward(d, parity)
+    assert rel(H(d), ref2) < tol
+
+
+def test_golden_fixtures_on_gpu(ctx, golden_dir):
+    """numpy rfftn golden vectors directly against the GPU transforms (FP64)."""
+    from tarang_amd import Plan
+    G = np.load(f"{golden_dir}/four_fft.npz")
+    for tag in ("a", "c"):
+        N = tuple(int(v) for v in G[f"{tag}_N"])
+        p = Plan(ctx, N, dealias=False)
+        A = np.zeros(p.shape, np.complex128)
+        A.view(np.float64).reshape(N[0], N[1], -1)[:, :, :N[2]] = G[f"{tag}_u"]
+        d = p.to_dev(A)
+        p.forward(d)
+        assert rel(H(d), G[f"{tag}_fwd"]) < 1e-13
+        d = p.to_dev(G[f"{tag}_spec"])
+        p.inverse(d)
+        got = H(d).view(np.float64).reshape(N[0], N[1], -1)[:, :, :N[2]]
+        assert np.abs(got - G[f"{tag}_inv"]).max() < 1e-12 * np.abs(G[f"{tag}_inv"]).max()
+
+
+@pytest.mark.parametrize("basis", ["FOUR", "SCFT"])
+def test_spectral_ops_bitwise(ctx, basis):
+    """Derivatives, dealias, real products, 1/K^2 and exp(f K^2) (FP64) match the oracle to <= 1 ulp."""
+    from tarang_amd import Plan
+    N = (16, 24, 12)
+    kf = (1.0, 0.5, 2.0)
+    o = Oracle(N, kfactor=kf, basis=basis)
+    p = Plan(ctx, N, kfactor=kf, basis=basis)
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+    B = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+    dA, dB = p.to_dev(A), p.to_dev(B)
+    for axis in (1, 2, 3):
+        for par in (0, 1):
+            assert rel(H(p.deriv(axis, dA, par)), o.deriv(axis, A, par)) <= 1e-16
+    assert np.array_equal(H(p.dealias(dA.clone())), o.dealias(A.copy()))
+    assert np.array_equal(H(p.real_mult(dA, dB)), o.real_mult(A, B))
+    assert rel(H(p.divide_ksqr(dA.clone())), o.divide_ksqr(A.copy())) < 1e-15
+    assert rel(H(p.exp_ksqr(dA.clone(), -0.01)), o.exp_ksqr(A.copy(), -0.01)) < 1e-15
+
+
+@pytest.mark.parametrize("N", [(16, 16, 16), (24, 32, 16)])
+def test_nlin_pressure_rhs_ns(ctx, N):
+    from tarang_amd import Plan
+    log(f"nlin ns N={N}")
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    V = o.ic_random(11)
+    nl = o.nlin_ns([v.copy() for v in V])
+    dV = [p.to_dev(v) for v in V]
+    dn = p.nlin_ns(dV)
+    for i in range(3):
+        assert rel(H(dn[i]), nl[i]) < 1e-12
+    pr = o.pressure(nl)
+    dp = p.pressure(dn)
+    assert rel(H(dp), pr) < 1e-12
+    o.rhs(nl, pr)
+    p.rhs(dn, dp)
+    for i in range(3):
+        assert rel(H(dn[i]), nl[i]) < 1e-12
+
+
+def test_nlin_mhd_rb_et(ctx):
+    from tarang_amd import Plan
+    N = (16, 16, 16)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    U, B = o.ic_random(2), o.ic_random(3)
+    dU, dB = [p.to_dev(a) for a in U], [p.to_dev(a) for a in B]
+    nU, nB = o.nlin_mhd([a.copy() for a in U], [a.copy() for a in B])
+    gU, gB = p.nlin_mhd(dU, dB)
+    for i in range(3):
+        assert rel(H(gU[i]), nU[i]) < 1e-12 and rel(H(gB[i]), nB[i]) < 1e-12
+    G = [o.fill_shell(a, 2.0, 4.0) for a in U]
+    gG = [p.fill_shell(a, 2.0, 4.0) for a in [p.to_dev(u) for u in U]]
+    for i in range(3):
+        assert np.array_equal(H(gG[i]), G[i])
+    ne = o.et_nlin(U, G)
+    ge = p.et_nlin([p.to_dev(a) for a in U], gG)
+    for i in range(3):
+        assert rel(H(ge[i]), ne[i]) < 1e-12
+    kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
+    orb = Oracle(N, kfactor=kf, basis="SCFT")
+    prb = Plan(ctx, N, kfactor=kf, basis="SCFT")
+    rng = np.random.default_rng(4)
+    V = [orb.dealias(rng.standard_normal(orb.shape) + 1j * rng.standard_normal(orb.shape)) for _ in range(3)]
+    T = orb.ic_random_scalar(4, 1e-2)
+    nV, nT = orb.nlin_rb(V, T)
+    gV, gT = prb.nlin_rb([prb.to_dev(a) for a in V], prb.to_dev(T))
+    for i in range(3):
+        assert rel(H(gV[i]), nV[i]) < 1e-12
+    assert rel(H(gT), nT) < 1e-12
+
+
+def test_binning_counts_bitexact_and_spectra(ctx):
+    """Shell index ceil(K) on the GPU equals the CPU's for every mode: the per-shell mode counts
+    (Sk_count) agree exactly, including the RB kfactor (pi, pi/sqrt2, pi/sqrt2)."""
+    from tarang_amd import Plan
+    for basis, kf, N in [("FOUR", (1, 1, 1), (64, 64, 64)), ("FOUR", (1, 1, 1), (96, 48, 32)),
+                         ("SCFT", (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2)), (32, 64, 64))]:
+        o = Oracle(N, kfactor=kf, basis=basis)
+        p = Plan(ctx, N, kfactor=kf, basis=basis)
+        rng = np.random.default_rng(5)
+        A = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+        Sk, cnt = o.shell_spectrum(A)
+        gSk, gcnt = p.shell_spectrum(p.to_dev(A))
+        assert np.array_equal(gcnt, cnt), basis
+        assert np.abs(gSk - Sk).max() <= 1e-13 * np.abs(Sk).max()
+        assert abs(p.energy(p.to_dev(A)) - o.energy(A)) < 1e-13 * abs(o.energy(A))
+        assert abs(p.Sn(p.to_dev(A), 2) - o.Sn(A, 2)) < 1e-13 * abs(o.Sn(A, 2))
+
+
+def test_ic_parity(ctx):
+    from tarang_amd import Plan
+    N = (32, 32, 32)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    V = o.ic_random(1, 0.5, 4.0)
+    gV = p.ic_random(1, 0.5, 4.0)
+    for i in range(3):
+        assert rel(H(gV[i]), V[i]) < 1e-13
+    T = p.ic_taylor_green(1, 1.0)
+    To = o.ic_taylor_green(1, 1.0)
+    for i in range(3):
+        assert np.array_equal(H(T[i]), To[i])
+    assert abs(p.total_energy(T) - 0.125) < 1e-16
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+def test_plain_steps_ns(ctx, scheme):
+    from tarang_amd import Plan
+    N = (16, 16, 16)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    V = o.ic_random(3)
+    gV = [p.to_dev(a) for a in V]
+    nu, dt = 0.01, 2e-3
+    for _ in range(3):
+        if scheme == 0:
+            o.euler_ns(V, nu, dt)
+        elif scheme == 1:
+            o.rk2_ns(V, nu, dt)
+        else:
+            o.rk4_ns(V, nu, dt)
+    p.step_plain(0, gV, dt, nu=nu, scheme=scheme, steps=3)
+    for i in range(3):
+        assert rel(H(gV[i]), V[i]) < 1e-13
+
+
+def test_plain_steps_mhd_rb(ctx):
+    from tarang_amd import Plan
+    N = (16, 16, 16)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    U, B = o.ic_random(2), o.ic_random(3)
+    g = [p.to_dev(a) for a in U + B]
+    o.rk4_mhd(U, B, 0.01, 0.02, 2e-3, steps=2)
+    p.step_plain(1, g, 2e-3, nu=0.01, eta=0.02, steps=2)
+    for i, ref in enumerate(U + B):
+        assert rel(H(g[i]), ref) < 1e-13
+    kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
+    orb = Oracle(N, kfactor=kf, basis="SCFT")
+    prb = Plan(ctx, N, kfactor=kf, basis="SCFT")
+    V = orb.zeros(3)
+    T = orb.ic_random_scalar(4, 1e-3)
+    g = [prb.to_dev(a) for a in V + [T]]
+    orb.rk4_rb(V, T, 1e5, 1.0, 1e-2, steps=3)
+    prb.step_plain(2, g, 1e-2, Ra=1e5, Pr=1.0, steps=3)
+    for i, ref in enumerate(V + [T]):
+        assert rel(H(g[i]), ref) < 1e-12
+    assert abs(prb.nusselt(g[:3], g[3], 1e5, 1.0) - orb.nusselt(V, T, 1e5, 1.0)) < 1e-13
+
+
+def test_flux_and_shell_transfer_plain(ctx):
+    from tarang_amd import Plan
+    N = (16, 16, 16)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    V = o.ic_random(5)
+    radii = np.concatenate([np.arange(8), [10000.0]])
+    gV = [p.to_dev(a) for a in V]
+    f_ref = o.flux_ns(V, radii)
+    f = p.flux_ns(gV, radii)
+    assert np.abs(f - f_ref).max() < 1e-12 * np.abs(f_ref).max()
+    T_ref = o.shell_tr_ns(V, radii)
+    T = p.shell_tr_ns(gV, radii)
+    assert np.abs(T - T_ref).max() < 1e-12 * np.abs(T_ref).max()
+
+
+def test_config1_taylor_green_64_100_steps(ctx):
+    """Config 1: TG 64^3 FP64, RK4, dt = 1e-3, 100 steps, nu = 0.01, 2/3 dealias.  The plain GPU path
+    matches the oracle to <= 1e-10 relative in E(k) (shells with E >= 1e-10 max E) and in Pi(R)."""
+    from tarang_amd import Plan
+    N = (64, 64, 64)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    V = o.ic_taylor_green(1, 1.0)
+    g = p.ic_taylor_green(1, 1.0)
+    log("config1: oracle 100 steps")
+    o.rk4_ns(V, 0.01, 1e-3, steps=100)
+    log("config1: gpu 100 steps")
+    p.step_plain(0, g, 1e-3, nu=0.01, steps=100)
+    Ek = o.vector_spectrum(V)
+    gEk = p.vector_spectrum(g)
+    sel = Ek >= 1e-10 * Ek.max()
+    err = np.abs(gEk[sel] - Ek[sel]) / Ek[sel]
+    log(f"config1 max rel E(k) err {err.max():.3e} over {sel.sum()} shells")
+    assert err.max() <= 1e-10
+    radii = np.concatenate([np.arange(1, 22, dtype=float), [10000.0]])
+    radii = np.concatenate([[0.0], radii])
+    fo = o.flux_ns(V, radii)
+    fg = p.flux_ns(g, radii)
+    sel = np.abs(fo) >= 1e-10 * np.abs(fo).max()
+    ferr = (np.abs(fg - fo)[sel] / np.abs(fo)[sel]).max()
+    log(f"config1 max rel flux err {ferr:.3e}")
+    assert ferr <= 1e-10
