@@ -32,6 +32,17 @@ constexpr Sched make_sched(int L, int E)
         s.r[s.n++] = R;
         m /= R;
     }
+    // Prefer a schedule whose first and last radix agree: then the input order of a forward FFT
+    // equals the output order of an inverse one and fused passes skip an LDS shuffle.
+    if (s.n >= 3 && s.r[0] != s.r[s.n - 1]) {
+        for (int i = 1; i < s.n - 1; i++)
+            if (s.r[i] == s.r[0]) {
+                int t = s.r[i];
+                for (int j = i; j < s.n - 1; j++) s.r[j] = s.r[j + 1];
+                s.r[s.n - 1] = t;
+                break;
+            }
+    }
     return s;
 }
 

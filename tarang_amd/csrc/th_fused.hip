@@ -1,24 +1,988 @@
-// th_fused.hip -- dealias-pruned fused pipeline (placeholder until the fused passes land).
+// th_fused.hip -- dealias-pruned, fused pseudo-spectral RK stage (FOUR basis).
+//
+// State layout: only the modes kept by Dealias_array (universal_misc.cc:118-159) are stored, in the
+// reference's (kx, ky, kz) order, compacted per axis:  S[cx][cy][cz] with cz padded to KZP (128-byte
+// rows).  A stage of the IF-RK4 scheme (time_advance.cc:100-155) is five passes, each a batch of
+// LDS-staged line FFTs over 128-byte column tiles; nothing else touches HBM:
+//   1  x-inverse   S[cx][m]    -> A[x][m]          (m = (cy,cz); zero rows of the gap are never read)
+//   2  y-inverse   A[x][cy][z] -> B[x][y][z]
+//   3  z-fused     B[x][y][:]  -> c2r, real-space products (Array_real_mult), r2c -> C[x][y][kz<N3/3]
+//   4  y-forward   C[x][y][z]  -> D[x][cy][z]      (only kept ky written)
+//   5  x-forward   D[x][m]     -> kept kx only; epilogue: N_i = sum_j i K_j F(u_i u_j) (RSprod.cc),
+//                  projection -P(K) N (compute_pressure.cc, compute_rhs.cc), FP64 integrating factor
+//                  exp(-nu K^2 a dt) and the RK4 stage update (single_time_step.cc, Compute_RK4_parts)
+// MHD uses the Elsasser form (compute_nlin_VF_MHD.cc) with 6 inverse and 9 product lines; the energy
+// transfer (ET_compute_nlin.cc) uses helper U and a shell-masked giver G and bins 2 mult Re(N U*)
+// (four_ET.cc) in the pass-5 epilogue instead of writing N.
+#include <math.h>
+#include <string.h>
+#include <vector>
 #include "../../include/tarang_hip.h"
+#include "th_fft.hpp"
 #include "th_internal.hpp"
 #include "th_plan.hpp"
 
-using namespace th;
-
 namespace th {
+
+// Line of logical length N with nk stored entries: logical [0,h) and [N-(nk-h), N).
+struct LMap {
+    int N, nk, h;
+};
+__host__ __device__ __forceinline__ bool lkept(const LMap& m, int l) { return l < m.h || l >= m.N - (m.nk - m.h); }
+__host__ __device__ __forceinline__ int lq(const LMap& m, int l) { return l < m.h ? l : l - (m.N - m.nk); }
+__host__ __device__ __forceinline__ int ll(const LMap& m, int q) { return q < m.h ? q : q + (m.N - m.nk); }
+
+struct FG {
+    int N1, N2, N3, LZ;
+    LMap mx, my;
+    int nkz, KZP;
+    long sx;     // x-row stride of the state and of A/D: nky * KZP
+    long sB;     // x-plane stride of B/C: N2 * KZP
+    long cS;     // component stride of the state: nkx * sx
+    long cA;     // component stride of A/D: N1 * sx
+    long cB;     // component stride of B/C: N1 * sB
+    double f1, f2, f3;
+    double norm; // 1 / (N1 N2 N3)
+};
+
+struct StageCoef {
+    int stage;          // 1..4
+    double dt;
+    double fh[2];       // (-diss dt) * 0.5 for field groups 0 (U) and 1 (B / scalar)
+    double f1[2];       // (-diss dt) * 1.0
+};
+
+struct ShellSel {       // giver mask inner < |K| <= outer (Fill_array_shell universal_ET.cc:45-65)
+    double inner, outer;
+};
+
+__device__ __forceinline__ double kmag_f(double a, double b, double c)
+{
+#pragma clang fp contract(off)
+    return sqrt(a * a + b * b + c * c);
+}
+__device__ __forceinline__ double ksq_f(double a, double b, double c)
+{
+#pragma clang fp contract(off)
+    return a * a + b * b + c * c;
+}
+
+template <typename R> __device__ __forceinline__ cplx<R> ld(const cplx<R>* p) { return *p; }
+
+// ------------------------------------------------------------------------------------------
+// pass 1: x-inverse.  MODE 0: copy comps; 1: Elsasser Z+ = U + B (comps 0-2), Z- = U - B (3-5);
+// 2: giver G = U masked to the shell sel.
+// ------------------------------------------------------------------------------------------
+template <typename R, int L, int P, int T, int MODE>
+__global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restrict__ S, cplx<R>* __restrict__ A,
+                                                  const cplx<R>* __restrict__ tw, ShellSel sel)
+{
+    using F = LineFFT<R, L, P, +1>;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int ntz = (g.nkz + T - 1) / T;
+    const int cy = blockIdx.x / ntz;
+    const int cz = (blockIdx.x % ntz) * T + c;
+    const bool ok = cz < g.nkz;
+    const long m = (long)cy * g.KZP + cz;
+    const int oc = blockIdx.y;
+    double Ky = 0, Kz = 0;
+    if (MODE == 2) {
+        Ky = ky_of(ll(g.my, cy), g.N2) * g.f2;
+        Kz = cz * g.f3;
+    }
+    cplx<R> v[F::E];
+#pragma unroll
+    for (int i = 0; i < F::E; i++) {
+        const int l = F::in_pos(t, i);
+        cplx<R> a = czero<R>();
+        if (ok && lkept(g.mx, l)) {
+            const long off = (long)lq(g.mx, l) * g.sx + m;
+            if (MODE == 0) {
+                a = S[oc * g.cS + off];
+            } else if (MODE == 1) {
+                const int cc = oc % 3;
+                cplx<R> u = S[cc * g.cS + off], b = S[(3 + cc) * g.cS + off];
+                a = (oc < 3) ? (u + b) : (u - b);
+            } else {
+                double km = kmag_f(kx_four(l, g.N1) * g.f1, Ky, Kz);
+                if ((km > sel.inner) && (km <= sel.outer)) a = S[oc * g.cS + off];
+            }
+        }
+        v[i] = a;
+    }
+    F::template run<T>(v, lds + c, t, tw);
+    if (ok) {
+        cplx<R>* out = A + oc * g.cA + m;
+#pragma unroll
+        for (int i = 0; i < F::E; i++) out[(long)F::out_pos(t, i) * g.sx] = v[i];
+    }
+}
+
+// pass 2: y-inverse, A[x][cy][cz] -> B[x][y][cz]
+template <typename R, int L, int P, int T>
+__global__ __launch_bounds__(T * P) void k_fy_inv(FG g, const cplx<R>* __restrict__ A, cplx<R>* __restrict__ B,
+                                                  const cplx<R>* __restrict__ tw)
+{
+    using F = LineFFT<R, L, P, +1>;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int cz = blockIdx.x * T + c;
+    const bool ok = cz < g.nkz;
+    const int x = blockIdx.y, comp = blockIdx.z;
+    const cplx<R>* a = A + comp * g.cA + (long)x * g.sx + cz;
+    cplx<R> v[F::E];
+#pragma unroll
+    for (int i = 0; i < F::E; i++) {
+        const int l = F::in_pos(t, i);
+        v[i] = (ok && lkept(g.my, l)) ? a[(long)lq(g.my, l) * g.KZP] : czero<R>();
+    }
+    F::template run<T>(v, lds + c, t, tw);
+    if (ok) {
+        cplx<R>* b = B + comp * g.cB + (long)x * g.sB + cz;
+#pragma unroll
+        for (int i = 0; i < F::E; i++) b[(long)F::out_pos(t, i) * g.KZP] = v[i];
+    }
+}
+
+// pass 3: fused z.  Input lines hold kz < nkz; c2r to the z-packed real line, products, r2c.
+// MODE 0 (NS):  in u0..u2,            out u0u0 u1u1 u2u2 u0u1 u1u2 u2u0          (RSprod.cc:52-77)
+// MODE 1 (MHD): in Z+0..2, Z-0..2,    out P(i,j) = Z+_i Z-_j at 3i+j             (RSprod.cc:117-151)
+// MODE 2 (ET):  in H0..2,  G0..2,     out P(i,j) = G_i H_j at 3i+j               (ET_RSprod.cc:49-85)
+template <int MODE> struct ZProd;
+template <> struct ZProd<0> {
+    static constexpr int NIN = 3, NOUT = 6;
+    static __device__ __forceinline__ int a(int p) { return p < 3 ? p : p - 3; }
+    static __device__ __forceinline__ int b(int p) { return p < 3 ? p : (p == 5 ? 0 : p - 2); }
+};
+template <> struct ZProd<1> {
+    static constexpr int NIN = 6, NOUT = 9;
+    static __device__ __forceinline__ int a(int p) { return p / 3; }
+    static __device__ __forceinline__ int b(int p) { return 3 + p % 3; }
+};
+template <> struct ZProd<2> {
+    static constexpr int NIN = 6, NOUT = 9;
+    static __device__ __forceinline__ int a(int p) { return 3 + p / 3; }
+    static __device__ __forceinline__ int b(int p) { return p % 3; }
+};
+
+template <typename R, int LZ, int P, int TZ, int MODE>
+__global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ Cout,
+                                               const cplx<R>* __restrict__ twL, const cplx<R>* __restrict__ twN)
+{
+    using FI = LineFFT<R, LZ, P, +1>;
+    using FF = LineFFT<R, LZ, P, -1>;
+    using ZP = ZProd<MODE>;
+    constexpr int E = FI::E;
+    __shared__ cplx<R> lds[TZ * LZ];
+    const int t = threadIdx.x % P, c = threadIdx.x / P;
+    const long q = (long)blockIdx.x * TZ + c;
+    const bool ok = q < (long)g.N1 * g.N2;
+    const long off = q * g.KZP;
+    cplx<R>* Z = lds + c * LZ;
+    cplx<R> u[ZP::NIN][E];
+#pragma unroll
+    for (int a = 0; a < ZP::NIN; a++) {
+        const cplx<R>* X = B + a * g.cB + off;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int k = FI::in_pos(t, i), kk = LZ - k;
+            cplx<R> xa = (ok && k < g.nkz) ? X[k] : czero<R>();
+            cplx<R> xb = (ok && kk < g.nkz) ? X[kk] : czero<R>();
+            if (k == 0) { xa.y = R(0); xb.y = R(0); }
+            xb.y = -xb.y;
+            cplx<R> s = xa + xb, d = xa - xb;
+            cplx<R> w = twN[k];
+            w.y = -w.y;
+            u[a][i] = s + mul_i(cmul(w, d));
+        }
+        FI::template run<1>(u[a], Z, t, twL);
+    }
+#pragma unroll
+    for (int p = 0; p < ZP::NOUT; p++) {
+        const int ia = ZP::a(p), ib = ZP::b(p);
+        cplx<R> w[E];
+#pragma unroll
+        for (int i = 0; i < E; i++) w[i] = mk<R>(u[ia][i].x * u[ib][i].x, u[ia][i].y * u[ib][i].y);
+        if constexpr (FI::R0 != FI::RL) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; i++) Z[FI::out_pos(t, i)] = w[i];
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < E; i++) w[i] = Z[FF::in_pos(t, i)];
+        }
+        FF::template run<1>(w, Z, t, twL);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E; i++) Z[FF::out_pos(t, i)] = w[i];
+        __syncthreads();
+        if (ok) {
+            cplx<R>* X = Cout + p * g.cB + off;
+            for (int k = t; k < g.nkz; k += P) {
+                cplx<R> zk = Z[k % LZ], zc = conjg(Z[(LZ - k) % LZ]);
+                cplx<R> e = cscale(zk + zc, R(0.5));
+                cplx<R> tt = cmul(twN[k], zk - zc);
+                X[k] = mk<R>(e.x + R(0.5) * tt.y, e.y - R(0.5) * tt.x);
+            }
+        }
+    }
+}
+
+// pass 4: y-forward, C[x][y][cz] -> D[x][cy][cz] (kept ky only)
+template <typename R, int L, int P, int T>
+__global__ __launch_bounds__(T * P) void k_fy_fwd(FG g, const cplx<R>* __restrict__ Cin, cplx<R>* __restrict__ D,
+                                                  const cplx<R>* __restrict__ tw)
+{
+    using F = LineFFT<R, L, P, -1>;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int cz = blockIdx.x * T + c;
+    const bool ok = cz < g.nkz;
+    const int x = blockIdx.y, comp = blockIdx.z;
+    const cplx<R>* a = Cin + comp * g.cB + (long)x * g.sB + cz;
+    cplx<R> v[F::E];
+#pragma unroll
+    for (int i = 0; i < F::E; i++) v[i] = ok ? a[(long)F::in_pos(t, i) * g.KZP] : czero<R>();
+    F::template run<T>(v, lds + c, t, tw);
+    if (ok) {
+        cplx<R>* d = D + comp * g.cA + (long)x * g.sx + cz;
+#pragma unroll
+        for (int i = 0; i < F::E; i++) {
+            const int l = F::out_pos(t, i);
+            if (lkept(g.my, l)) d[(long)lq(g.my, l) * g.KZP] = v[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// pass 5: x-forward of the product lines with the nonlinear-term contraction accumulated in
+// registers, then the projection and the IF-RK4 stage update.
+//   MODE 0 (NS):  N_i = sum_j i K_j P_ij                         (Derivative_RSprod_VV RSprod.cc:94-105)
+//   MODE 1 (MHD): N+_i = sum_j i K_j P(i,j), N-_i = sum_j i K_j P(j,i); N_u = (N+ + N-)/2,
+//                 N_b = (N+ - N-)/2 (Elsasser_to_UB_nlin elsasser.cc:117-130); U projected, B not.
+// ------------------------------------------------------------------------------------------
+template <typename R> __device__ __forceinline__ void acc_iK(cplx<R>& n, R K, const cplx<R>& v)
+{
+    n.x -= K * v.y;
+    n.y += K * v.x;
+}
+
+struct Dc {
+    double x, y;
+};
+__device__ __forceinline__ Dc dmk(double a, double b) { Dc r; r.x = a; r.y = b; return r; }
+template <typename R> __device__ __forceinline__ Dc dl(const cplx<R>& a) { return dmk((double)a.x, (double)a.y); }
+template <typename R> __device__ __forceinline__ cplx<R> ds(const Dc& a) { return mk<R>((R)a.x, (R)a.y); }
+__device__ __forceinline__ Dc dsc(const Dc& a, double s) { return dmk(a.x * s, a.y * s); }
+__device__ __forceinline__ Dc dadd(const Dc& a, const Dc& b) { return dmk(a.x + b.x, a.y + b.y); }
+
+// One RK4 stage update of a field value V0 (state), stage field Vs and accumulator tot
+// (single_time_step.cc:54-86 with a,b,c of time_advance.cc:120-153).
+template <typename R>
+__device__ __forceinline__ void rk_update(int stage, double dt, double eh, double e1, const Dc& rhs, cplx<R>* S,
+                                          cplx<R>* Vs, cplx<R>* acc)
+{
+    Dc V0 = dl(*S);
+    if (stage == 1) {
+        Dc v = dadd(V0, dsc(rhs, 0.5 * dt));
+        *Vs = ds<R>(dsc(v, eh));
+        *acc = ds<R>(dsc(dsc(rhs, e1), dt / 6));
+    } else if (stage == 2) {
+        *Vs = ds<R>(dadd(dsc(V0, eh), dsc(rhs, 0.5 * dt)));
+        Dc a = dl(*acc);
+        *acc = ds<R>(dadd(a, dsc(dsc(rhs, eh), dt / 3)));
+    } else if (stage == 3) {
+        Dc v = dadd(dsc(V0, eh), dsc(rhs, 1.0 * dt));
+        *Vs = ds<R>(dsc(v, eh));
+        Dc a = dl(*acc);
+        *acc = ds<R>(dadd(a, dsc(dsc(rhs, eh), dt / 3)));
+    } else {
+        Dc a = dadd(dl(*acc), dsc(rhs, dt / 6));
+        *S = ds<R>(dadd(dsc(V0, e1), a));
+    }
+}
+
+template <typename R, int L, int P, int T, int MODE>
+__global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restrict__ D, cplx<R>* S, cplx<R>* Vs,
+                                                  cplx<R>* acc, const cplx<R>* __restrict__ tw, StageCoef sc)
+{
+    using F = LineFFT<R, L, P, -1>;
+    constexpr int E = F::E;
+    constexpr int NP = (MODE == 0) ? 6 : 9;
+    constexpr int NA = (MODE == 0) ? 3 : 6;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int ntz = (g.nkz + T - 1) / T;
+    const int cy = blockIdx.x / ntz;
+    const int cz = (blockIdx.x % ntz) * T + c;
+    const bool ok = cz < g.nkz;
+    const long m = (long)cy * g.KZP + cz;
+    const R Ky = (R)(ky_of(ll(g.my, cy), g.N2) * g.f2);
+    const R Kz = (R)(cz * g.f3);
+    cplx<R> Nacc[NA][E];
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+        for (int i = 0; i < E; i++) Nacc[a][i] = czero<R>();
+#pragma unroll
+    for (int p = 0; p < NP; p++) {
+        cplx<R> v[E];
+        const cplx<R>* src = D + p * g.cA + m;
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = ok ? src[(long)F::in_pos(t, i) * g.sx] : czero<R>();
+        F::template run<T>(v, lds + c, t, tw);
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int l = F::out_pos(t, i);
+            const R Kx = (R)(kx_four(l, g.N1) * g.f1);
+            if (MODE == 0) {
+                if (p == 0) acc_iK(Nacc[0][i], Kx, v[i]);
+                else if (p == 1) acc_iK(Nacc[1][i], Ky, v[i]);
+                else if (p == 2) acc_iK(Nacc[2][i], Kz, v[i]);
+                else if (p == 3) { acc_iK(Nacc[0][i], Ky, v[i]); acc_iK(Nacc[1][i], Kx, v[i]); }
+                else if (p == 4) { acc_iK(Nacc[1][i], Kz, v[i]); acc_iK(Nacc[2][i], Ky, v[i]); }
+                else { acc_iK(Nacc[2][i], Kx, v[i]); acc_iK(Nacc[0][i], Kz, v[i]); }
+            } else {
+                const int pi = p / 3, pj = p % 3;            // P(pi,pj) = Z+_pi Z-_pj
+                const R Kj = pj == 0 ? Kx : (pj == 1 ? Ky : Kz);
+                const R Ki = pi == 0 ? Kx : (pi == 1 ? Ky : Kz);
+                acc_iK(Nacc[pi][i], Kj, v[i]);               // N+_pi += i K_pj P(pi,pj)
+                acc_iK(Nacc[3 + pj][i], Ki, v[i]);           // N-_pj += i K_pi P(pi,pj)
+            }
+        }
+    }
+    if (!ok) return;
+    const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int l = F::out_pos(t, i);
+        if (!lkept(g.mx, l)) continue;
+        const double dKx = kx_four(l, g.N1) * g.f1;
+        const double K2 = ksq_f(dKx, dKy, dKz);
+        const long idx = (long)lq(g.mx, l) * g.sx + m;
+        Dc Nu[3];
+        if (MODE == 0) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) Nu[a] = dsc(dl(Nacc[a][i]), g.norm);
+        } else {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
+                Nu[a] = dsc(dadd(np, nm), 0.5 * g.norm);
+            }
+        }
+        Dc rhs[3];
+        if (K2 > 0) {
+            Dc kn = dmk(dKx * Nu[0].x + dKy * Nu[1].x + dKz * Nu[2].x, dKx * Nu[0].y + dKy * Nu[1].y + dKz * Nu[2].y);
+            const double Kc[3] = {dKx, dKy, dKz};
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                rhs[a] = dmk(-(Nu[a].x - Kc[a] * kn.x / K2), -(Nu[a].y - Kc[a] * kn.y / K2));
+        } else {
+#pragma unroll
+            for (int a = 0; a < 3; a++) rhs[a] = dmk(-Nu[a].x, -Nu[a].y);
+        }
+        const double eh = exp(sc.fh[0] * K2), e1 = exp(sc.f1[0] * K2);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            rk_update<R>(sc.stage, sc.dt, eh, e1, rhs[a], S + a * g.cS + idx, Vs + a * g.cS + idx, acc + a * g.cS + idx);
+        if (MODE == 1) {
+            const double ehb = exp(sc.fh[1] * K2), e1b = exp(sc.f1[1] * K2);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
+                Dc nb = dsc(dmk(np.x - nm.x, np.y - nm.y), 0.5 * g.norm);
+                Dc rb = dmk(-nb.x, -nb.y);
+                rk_update<R>(sc.stage, sc.dt, ehb, e1b, rb, S + (3 + a) * g.cS + idx, Vs + (3 + a) * g.cS + idx,
+                             acc + (3 + a) * g.cS + idx);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// energy-transfer epilogue: x-forward of the 9 ET product lines, N_i = sum_j i K_j F(H_j G_i), then
+// 2 mult Re(N . U*) binned by shell (Get_shell_index over radii) separately for |K| <= kmax_inside
+// (Shell_mult_all) and |K| > kmax_inside (which only Shell_mult_single counts).
+// ------------------------------------------------------------------------------------------
+template <typename R, int L, int P, int T>
+__global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __restrict__ D, const cplx<R>* __restrict__ U,
+                                                     const cplx<R>* __restrict__ tw, const double* __restrict__ radii,
+                                                     int nr, int kin, double* __restrict__ partial)
+{
+    using F = LineFFT<R, L, P, -1>;
+    constexpr int E = F::E;
+    __shared__ cplx<R> lds[L * T];
+    extern __shared__ double bins[];      // 2 nr doubles: |K| <= kin binned by shell, then |K| > kin by shell
+    for (int b = threadIdx.x; b < 2 * nr; b += T * P) bins[b] = 0.0;
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int ntz = (g.nkz + T - 1) / T;
+    const int cy = blockIdx.x / ntz;
+    const int cz = (blockIdx.x % ntz) * T + c;
+    const bool ok = cz < g.nkz;
+    const long m = (long)cy * g.KZP + cz;
+    const R Ky = (R)(ky_of(ll(g.my, cy), g.N2) * g.f2);
+    const R Kz = (R)(cz * g.f3);
+    cplx<R> Nacc[3][E];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int i = 0; i < E; i++) Nacc[a][i] = czero<R>();
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+        cplx<R> v[E];
+        const cplx<R>* src = D + p * g.cA + m;
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = ok ? src[(long)F::in_pos(t, i) * g.sx] : czero<R>();
+        F::template run<T>(v, lds + c, t, tw);
+        const int pi = p / 3, pj = p % 3;        // P(pi,pj) = G_pi H_pj -> N_pi += i K_pj P
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int l = F::out_pos(t, i);
+            const R Kx = (R)(kx_four(l, g.N1) * g.f1);
+            const R Kj = pj == 0 ? Kx : (pj == 1 ? Ky : Kz);
+            acc_iK(Nacc[pi][i], Kj, v[i]);
+        }
+    }
+    __syncthreads();
+    if (ok) {
+        const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
+        const int l2 = ll(g.my, cy);
+        for (int i = 0; i < E; i++) {
+            const int l = F::out_pos(t, i);
+            if (!lkept(g.mx, l)) continue;
+            const int kx = kx_four(l, g.N1);
+            const double km = kmag_f(kx * g.f1, dKy, dKz);
+            // Multiplicity_factor_FOUR four_inline.h:347-365
+            double f = (cz > 0) ? 1.0 : 0.5;
+            if (kx == g.N1 / 2 || ky_of(l2, g.N2) == g.N2 / 2) f *= 2;
+            const long idx = (long)lq(g.mx, l) * g.sx + m;
+            double s = 0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                cplx<R> u = U[a * g.cS + idx];
+                s += (double)Nacc[a][i].x * (double)u.x + (double)Nacc[a][i].y * (double)u.y;
+            }
+            s *= 2 * f * g.norm;
+            int b = 0;
+            while (b < nr && !(km <= radii[b])) b++;
+            if (b >= nr) continue;
+            atomicAdd(&bins[(km <= (double)kin) ? b : nr + b], s);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 2 * nr; b += T * P) partial[(long)blockIdx.x * (2 * nr) + b] = bins[b];
+}
+
+// compact <-> reference layout
+template <typename R>
+__global__ void k_gather(FG g, const cplx<R>* __restrict__ ref, cplx<R>* __restrict__ S)
+{
+    long n = (long)g.mx.nk * g.sx;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int cz = (int)(i % g.KZP);
+        long r = i / g.KZP;
+        int cy = (int)(r % g.my.nk), cx = (int)(r / g.my.nk);
+        if (cz >= g.nkz) { S[i] = czero<R>(); continue; }
+        int l1 = ll(g.mx, cx), l2 = ll(g.my, cy);
+        S[i] = ref[((long)l1 * g.N2 + l2) * (g.N3 / 2 + 1) + cz];
+    }
+}
+template <typename R>
+__global__ void k_scatter(FG g, const cplx<R>* __restrict__ S, cplx<R>* __restrict__ ref)
+{
+    const int H = g.N3 / 2 + 1;
+    long n = (long)g.N1 * g.N2 * H;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int l3 = (int)(i % H);
+        long r = i / H;
+        int l2 = (int)(r % g.N2), l1 = (int)(r / g.N2);
+        cplx<R> v = czero<R>();
+        if (l3 < g.nkz && lkept(g.mx, l1) && lkept(g.my, l2)) v = S[(long)lq(g.mx, l1) * g.sx + (long)lq(g.my, l2) * g.KZP + l3];
+        ref[i] = v;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------------------------
 struct Fused {};
+
 void release_fused(th_plan* p)
 {
     delete p->fused;
     p->fused = nullptr;
 }
+
+template <typename R> constexpr int tileT() { return 128 / (int)sizeof(cplx<R>); }
+
+// kernel dispatch on line length
+#define TH_FLENS(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+#define TH_ZLENS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+
+template <typename R, int L, int MODE>
+static int launch_xinv(const FG& g, const cplx<R>* S, cplx<R>* A, int ncomp, const cplx<R>* tw, ShellSel sel, hipStream_t st)
+{
+    constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
+    const int ntz = (g.nkz + T - 1) / T;
+    hipLaunchKernelGGL((k_fx_inv<R, L, P, T, MODE>), dim3(g.my.nk * ntz, ncomp), dim3(T * P), 0, st, g, S, A, tw, sel);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int L>
+static int launch_yinv(const FG& g, const cplx<R>* A, cplx<R>* B, int ncomp, const cplx<R>* tw, hipStream_t st)
+{
+    constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
+    hipLaunchKernelGGL((k_fy_inv<R, L, P, T>), dim3((g.nkz + T - 1) / T, g.N1, ncomp), dim3(T * P), 0, st, g, A, B, tw);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int LZ, int MODE>
+static int launch_z(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* twL, const cplx<R>* twN, hipStream_t st)
+{
+    constexpr int E = pts_per_thread(LZ), P = LZ / E;
+    constexpr int TZ = (256 / P) < 1 ? 1 : 256 / P;
+    const long lines = (long)g.N1 * g.N2;
+    hipLaunchKernelGGL((k_fz<R, LZ, P, TZ, MODE>), dim3((unsigned)((lines + TZ - 1) / TZ)), dim3(TZ * P), 0, st, g, B, Cc, twL, twN);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int L>
+static int launch_yfwd(const FG& g, const cplx<R>* Cc, cplx<R>* D, int ncomp, const cplx<R>* tw, hipStream_t st)
+{
+    constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
+    hipLaunchKernelGGL((k_fy_fwd<R, L, P, T>), dim3((g.nkz + T - 1) / T, g.N1, ncomp), dim3(T * P), 0, st, g, Cc, D, tw);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int L, int MODE>
+static int launch_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
+                       const StageCoef& sc, hipStream_t st)
+{
+    constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
+    const int ntz = (g.nkz + T - 1) / T;
+    hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE>), dim3(g.my.nk * ntz), dim3(T * P), 0, st, g, D, S, Vs, acc, tw, sc);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int L>
+static int launch_xfwd_et(const FG& g, const cplx<R>* D, const cplx<R>* U, const cplx<R>* tw, const double* radii, int nr,
+                          int kin, double* partial, int* nblk, hipStream_t st)
+{
+    constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
+    const int ntz = (g.nkz + T - 1) / T;
+    *nblk = g.my.nk * ntz;
+    if (!partial) return OK;
+    hipLaunchKernelGGL((k_fx_fwd_et<R, L, P, T>), dim3(*nblk), dim3(T * P), 2 * nr * sizeof(double), st, g, D, U, tw,
+                       radii, nr, kin, partial);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename R, int MODE>
+static int d_xinv(const FG& g, const cplx<R>* S, cplx<R>* A, int nc, const cplx<R>* tw, ShellSel sel, hipStream_t st)
+{
+#define C_(n) case n: return launch_xinv<R, n, MODE>(g, S, A, nc, tw, sel, st);
+    switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R>
+static int d_yinv(const FG& g, const cplx<R>* A, cplx<R>* B, int nc, const cplx<R>* tw, hipStream_t st)
+{
+#define C_(n) case n: return launch_yinv<R, n>(g, A, B, nc, tw, st);
+    switch (g.N2) { TH_FLENS(C_) default: set_error("fused: N2 %d unsupported", g.N2); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R, int MODE>
+static int d_z(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* twL, const cplx<R>* twN, hipStream_t st)
+{
+#define C_(n) case n: return launch_z<R, n, MODE>(g, B, Cc, twL, twN, st);
+    switch (g.LZ) { TH_ZLENS(C_) default: set_error("fused: N3 %d unsupported", g.N3); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R>
+static int d_yfwd(const FG& g, const cplx<R>* Cc, cplx<R>* D, int nc, const cplx<R>* tw, hipStream_t st)
+{
+#define C_(n) case n: return launch_yfwd<R, n>(g, Cc, D, nc, tw, st);
+    switch (g.N2) { TH_FLENS(C_) default: set_error("fused: N2 %d unsupported", g.N2); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R, int MODE>
+static int d_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
+                  const StageCoef& sc, hipStream_t st)
+{
+#define C_(n) case n: return launch_xfwd<R, n, MODE>(g, D, S, Vs, acc, tw, sc, st);
+    switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R>
+static int d_xfwd_et(const FG& g, const cplx<R>* D, const cplx<R>* U, const cplx<R>* tw, const double* radii, int nr,
+                     int kin, double* partial, int* nblk, hipStream_t st)
+{
+#define C_(n) case n: return launch_xfwd_et<R, n>(g, D, U, tw, radii, nr, kin, partial, nblk, st);
+    switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
+#undef C_
+}
+
+static bool fused_len_ok(int n)
+{
+    const int ok[] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512};
+    for (int v : ok)
+        if (v == n) return true;
+    return false;
+}
+
+FG make_fg(const Geo& geo, int prec)
+{
+    FG g;
+    memset(&g, 0, sizeof(g));
+    g.N1 = geo.N1;
+    g.N2 = geo.N2;
+    g.N3 = geo.N3;
+    g.LZ = geo.N3 / 2;
+    if (geo.dealias) {
+        int hx = geo.N1 / 3, nkx = 2 * (geo.N1 / 3) - 1;
+        g.mx = LMap{geo.N1, nkx, hx};
+        int ylo = geo.N2 / 3, yhi = geo.N2 - (2 * geo.N2 / 3 + 2);
+        g.my = LMap{geo.N2, ylo + yhi, ylo};
+        g.nkz = geo.N3 / 3;
+    } else {
+        g.mx = LMap{geo.N1, geo.N1, geo.N1};
+        g.my = LMap{geo.N2, geo.N2, geo.N2};
+        g.nkz = geo.N3 / 2 + 1;
+    }
+    int T = prec ? 16 : 8;
+    g.KZP = (g.nkz + T - 1) / T * T;
+    g.sx = (long)g.my.nk * g.KZP;
+    g.sB = (long)g.N2 * g.KZP;
+    g.cS = (long)g.mx.nk * g.sx;
+    g.cA = (long)g.N1 * g.sx;
+    g.cB = (long)g.N1 * g.sB;
+    g.f1 = geo.kf[0];
+    g.f2 = geo.kf[1];
+    g.f3 = geo.kf[2];
+    g.norm = 1.0 / ((double)geo.N1 * (double)geo.N2 * (double)geo.N3);
+    return g;
+}
+
 }  // namespace th
 
-struct th_state {};
+using namespace th;
 
-extern "C" int th_state_create(th_plan*, th_state**, int) { set_error("fused path not built"); return E_UNSUPPORTED; }
-extern "C" int th_state_destroy(th_state* s) { delete s; return OK; }
-extern "C" int th_state_upload(th_state*, int, const void*, int) { return E_UNSUPPORTED; }
-extern "C" int th_state_download(th_state*, int, void*, int) { return E_UNSUPPORTED; }
-extern "C" int th_step(th_plan*, th_state*, double, const th_phys*, int) { return E_UNSUPPORTED; }
-extern "C" int th_flux_shell_tr(th_plan*, th_state*, const double*, int, double*, double*) { return E_UNSUPPORTED; }
+struct th_state {
+    th_plan* plan = nullptr;
+    int physics = TH_NS;
+    int ncomp = 3;
+    int nin = 3, nout = 6;       // pipeline component counts
+    FG fg;
+    size_t esz = 16;
+    void* S = nullptr;           // state (compact), ncomp comps
+    void* Vs = nullptr;          // stage field
+    void* acc = nullptr;         // RK4 accumulator
+    void* X1 = nullptr;          // A / D layout, max(nin, nout) comps
+    void* X2 = nullptr;          // B layout, nin comps (ET: 6)
+    void* X3 = nullptr;          // C layout, nout comps (ET: 9)
+    void* ET_U = nullptr;        // ET helper copy (B layout, 3 comps) -- unused when X2 is shared
+    double* partial = nullptr;
+    size_t partial_cap = 0;
+    double* radii = nullptr;
+    int radii_cap = 0;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    double g_dt = -1, g_nu = -1, g_eta = -1;
+    ~th_state()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void* q : {S, Vs, acc, X1, X2, X3, ET_U}) if (q) (void)hipFree(q);
+        if (partial) (void)hipFree(partial);
+        if (radii) (void)hipFree(radii);
+    }
+};
+
+#define TRY(x)             \
+    do {                   \
+        int r_ = (x);      \
+        if (r_) return r_; \
+    } while (0)
+
+extern "C" int th_state_create(th_plan* p, th_state** out, int physics)
+{
+    if (!p || !out) { set_error("th_state_create: null"); return E_ARG; }
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    if (p->g.basis != TH_FOUR) { set_error("th_state_create: the fused path covers the FOUR basis; use th_step_plain for SCFT"); return E_UNSUPPORTED; }
+    if (physics != TH_NS && physics != TH_MHD) { set_error("th_state_create: physics %d not fused (use th_step_plain)", physics); return E_UNSUPPORTED; }
+    if (!fused_len_ok(p->g.N1) || !fused_len_ok(p->g.N2) || !fused_len_ok(p->g.N3)) {
+        set_error("th_state_create: fused path needs N in {16,24,32,48,64,96,128,192,256,384,512}");
+        return E_UNSUPPORTED;
+    }
+    th_state* s = new (std::nothrow) th_state();
+    if (!s) return E_NOMEM;
+    s->plan = p;
+    s->physics = physics;
+    s->ncomp = physics == TH_NS ? 3 : 6;
+    s->nin = physics == TH_NS ? 3 : 6;
+    s->nout = physics == TH_NS ? 6 : 9;
+    s->fg = make_fg(p->g, p->prec);
+    s->esz = p->prec ? 8 : 16;
+    const FG& g = s->fg;
+    auto alloc = [&](void** q, size_t elems) -> int {
+        if (hipMalloc(q, elems * s->esz) != hipSuccess) { set_error("th_state_create: out of device memory"); return E_NOMEM; }
+        if (hipMemset(*q, 0, elems * s->esz) != hipSuccess) return E_HIP;
+        return OK;
+    };
+    int nA = s->nout > 6 ? s->nout : 9;   // ET needs 9 product comps
+    int nBin = 6, nC = 9;                 // room for MHD / ET
+    int r = OK;
+    if ((r = alloc(&s->S, s->ncomp * g.cS)) || (r = alloc(&s->Vs, s->ncomp * g.cS)) || (r = alloc(&s->acc, s->ncomp * g.cS)) ||
+        (r = alloc(&s->X1, nA * g.cA)) || (r = alloc(&s->X2, nBin * g.cB)) || (r = alloc(&s->X3, nC * g.cB))) {
+        delete s;
+        return r;
+    }
+    *out = s;
+    return OK;
+}
+
+extern "C" int th_state_destroy(th_state* s)
+{
+    if (!s) return OK;
+    (void)hipSetDevice(s->plan->ctx->device);
+    (void)hipStreamSynchronize(s->plan->ctx->stream);
+    delete s;
+    return OK;
+}
+
+static dim3 g1d(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    return dim3((unsigned)(b ? b : 1));
+}
+
+extern "C" int th_state_upload(th_state* s, int comp, const void* src, int src_on_host)
+{
+    if (!s || !src || comp < 0 || comp >= s->ncomp) { set_error("th_state_upload: bad args"); return E_ARG; }
+    th_plan* p = s->plan;
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t st = p->ctx->stream;
+    const void* dsrc = src;
+    void* tmp = nullptr;
+    size_t bytes = p->g.ncplx() * s->esz;
+    if (src_on_host) {
+        TH_CHECK_HIP(hipMalloc(&tmp, bytes));
+        TH_CHECK_HIP(hipMemcpyAsync(tmp, src, bytes, hipMemcpyHostToDevice, st));
+        dsrc = tmp;
+    }
+    const FG& g = s->fg;
+    size_t n = (size_t)g.mx.nk * g.sx;
+    if (p->prec)
+        hipLaunchKernelGGL((k_gather<float>), g1d(n), dim3(256), 0, st, g, (const cplx<float>*)dsrc,
+                           (cplx<float>*)s->S + comp * g.cS);
+    else
+        hipLaunchKernelGGL((k_gather<double>), g1d(n), dim3(256), 0, st, g, (const cplx<double>*)dsrc,
+                           (cplx<double>*)s->S + comp * g.cS);
+    hipError_t e = hipGetLastError();
+    if (tmp) {
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(tmp);
+    }
+    if (e != hipSuccess) { set_error("k_gather: %s", hipGetErrorString(e)); return E_HIP; }
+    return OK;
+}
+
+extern "C" int th_state_download(th_state* s, int comp, void* dst, int dst_on_host)
+{
+    if (!s || !dst || comp < 0 || comp >= s->ncomp) { set_error("th_state_download: bad args"); return E_ARG; }
+    th_plan* p = s->plan;
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t st = p->ctx->stream;
+    size_t bytes = p->g.ncplx() * s->esz;
+    void* ddst = dst;
+    void* tmp = nullptr;
+    if (dst_on_host) {
+        TH_CHECK_HIP(hipMalloc(&tmp, bytes));
+        ddst = tmp;
+    }
+    const FG& g = s->fg;
+    size_t n = p->g.ncplx();
+    if (p->prec)
+        hipLaunchKernelGGL((k_scatter<float>), g1d(n), dim3(256), 0, st, g, (const cplx<float>*)s->S + comp * g.cS, (cplx<float>*)ddst);
+    else
+        hipLaunchKernelGGL((k_scatter<double>), g1d(n), dim3(256), 0, st, g, (const cplx<double>*)s->S + comp * g.cS, (cplx<double>*)ddst);
+    TH_CHECK_HIP(hipGetLastError());
+    if (tmp) {
+        TH_CHECK_HIP(hipMemcpyAsync(dst, tmp, bytes, hipMemcpyDeviceToHost, st));
+        TH_CHECK_HIP(hipStreamSynchronize(st));
+        (void)hipFree(tmp);
+    }
+    return OK;
+}
+
+template <typename R> static int enqueue_step(th_state* s, double dt, double nu, double eta)
+{
+    th_plan* p = s->plan;
+    hipStream_t st = p->ctx->stream;
+    const FG& g = s->fg;
+    const cplx<R>* twx = p->twd<R>(g.N1);
+    const cplx<R>* twy = p->twd<R>(g.N2);
+    const cplx<R>* twz = p->twd<R>(g.LZ);
+    const cplx<R>* twn = p->twd<R>(g.N3);
+    cplx<R>* S = (cplx<R>*)s->S;
+    cplx<R>* Vs = (cplx<R>*)s->Vs;
+    cplx<R>* acc = (cplx<R>*)s->acc;
+    cplx<R>* X1 = (cplx<R>*)s->X1;
+    cplx<R>* X2 = (cplx<R>*)s->X2;
+    cplx<R>* X3 = (cplx<R>*)s->X3;
+    ShellSel sel{0, 0};
+    for (int stage = 1; stage <= 4; stage++) {
+        StageCoef sc;
+        sc.stage = stage;
+        sc.dt = dt;
+        sc.fh[0] = (-nu * dt) * 0.5;
+        sc.f1[0] = (-nu * dt) * 1.0;
+        sc.fh[1] = (-eta * dt) * 0.5;
+        sc.f1[1] = (-eta * dt) * 1.0;
+        const cplx<R>* src = stage == 1 ? S : Vs;
+        if (s->physics == TH_NS) {
+            TRY((d_xinv<R, 0>(g, src, X1, 3, twx, sel, st)));
+            TRY(d_yinv<R>(g, X1, X2, 3, twy, st));
+            TRY((d_z<R, 0>(g, X2, X3, twz, twn, st)));
+            TRY(d_yfwd<R>(g, X3, X1, 6, twy, st));
+            TRY((d_xfwd<R, 0>(g, X1, S, Vs, acc, twx, sc, st)));
+        } else {
+            TRY((d_xinv<R, 1>(g, src, X1, 6, twx, sel, st)));
+            TRY(d_yinv<R>(g, X1, X2, 6, twy, st));
+            TRY((d_z<R, 1>(g, X2, X3, twz, twn, st)));
+            TRY(d_yfwd<R>(g, X3, X1, 9, twy, st));
+            TRY((d_xfwd<R, 1>(g, X1, S, Vs, acc, twx, sc, st)));
+        }
+    }
+    return OK;
+}
+
+extern "C" int th_step(th_plan* p, th_state* s, double dt, const th_phys* ph, int nsteps)
+{
+    if (!p || !s || !ph || s->plan != p || nsteps < 0) { set_error("th_step: bad args"); return E_ARG; }
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    hipStream_t st = p->ctx->stream;
+    double nu = ph->nu, eta = ph->eta;
+    // twiddle tables must exist before graph capture (their creation allocates)
+    const FG& g = s->fg;
+    for (int L : {g.N1, g.N2, g.LZ, g.N3})
+        if (!p->tw.get(L, p->prec, st)) { set_error("twiddle table allocation failed"); return E_NOMEM; }
+    if (!s->gexec || s->g_dt != dt || s->g_nu != nu || s->g_eta != eta) {
+        if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+        if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+        TH_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int r = p->prec ? enqueue_step<float>(s, dt, nu, eta) : enqueue_step<double>(s, dt, nu, eta);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (r) { if (graph) (void)hipGraphDestroy(graph); return r; }
+        if (e != hipSuccess) { set_error("hipStreamEndCapture: %s", hipGetErrorString(e)); return E_HIP; }
+        s->graph = graph;
+        TH_CHECK_HIP(hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0));
+        s->g_dt = dt;
+        s->g_nu = nu;
+        s->g_eta = eta;
+    }
+    for (int i = 0; i < nsteps; i++) TH_CHECK_HIP(hipGraphLaunch(s->gexec, st));
+    return OK;
+}
+
+static __global__ void th_finalize_bins(const double* __restrict__ partial, int nblk, int stride, int nval,
+                                        double* __restrict__ out)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nval) return;
+    double s = 0.0, c = 0.0;
+    for (int k = 0; k < nblk; k++) {
+        double v = partial[(long)k * stride + b];
+        double t = s + v;
+        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+        s = t;
+    }
+    out[b] = s + c;
+}
+
+// Energy transfer from the state's velocity: one giver-masked fused pipeline per shell n gives
+// T(n, m) (Shell_mult_all, |K| <= kmax_inside) and the same sums over |K| > kmax_inside ("corner",
+// which Shell_mult_single counts); then Pi(R_i) = sum_{n<=i} sum_{m>i} [T(n,m) - corner(n,m)]
+// (sphere i = union of shells 1..i and N[G] is linear in G).
+template <typename R> static int flux_impl(th_state* s, const double* radii, int nsh, double* Pi, double* T)
+{
+    th_plan* p = s->plan;
+    hipStream_t st = p->ctx->stream;
+    const FG& g = s->fg;
+    const cplx<R>* twx = p->twd<R>(g.N1);
+    const cplx<R>* twy = p->twd<R>(g.N2);
+    const cplx<R>* twz = p->twd<R>(g.LZ);
+    const cplx<R>* twn = p->twd<R>(g.N3);
+    cplx<R>* S = (cplx<R>*)s->S;
+    cplx<R>* X1 = (cplx<R>*)s->X1;
+    cplx<R>* X2 = (cplx<R>*)s->X2;
+    cplx<R>* X3 = (cplx<R>*)s->X3;
+    const int nr = nsh + 1;
+    if (s->radii_cap < nr) {
+        if (s->radii) (void)hipFree(s->radii);
+        s->radii = nullptr;
+        TH_CHECK_HIP(hipMalloc(&s->radii, nr * sizeof(double)));
+        s->radii_cap = nr;
+    }
+    TH_CHECK_HIP(hipMemcpyAsync(s->radii, radii, nr * sizeof(double), hipMemcpyHostToDevice, st));
+    int nblk = 0;
+    TRY(d_xfwd_et<R>(g, X1, S, twx, s->radii, nr, 0, nullptr, &nblk, st));
+    size_t need = (size_t)nblk * (2 * nr) + (2 * nr);
+    if (need > s->partial_cap) {
+        if (s->partial) (void)hipFree(s->partial);
+        s->partial = nullptr;
+        TH_CHECK_HIP(hipMalloc(&s->partial, need * sizeof(double)));
+        s->partial_cap = need;
+    }
+    const int kin = max_radius_inside(p->g);
+    ShellSel all{-1.0, 1e300};
+    // helper H = U: inverse x, y once, kept in X2 comps 0..2 (B layout)
+    TRY((d_xinv<R, 0>(g, S, X1, 3, twx, all, st)));
+    TRY(d_yinv<R>(g, X1, X2, 3, twy, st));
+    std::vector<double> hostbuf(2 * nr);
+    std::vector<double> Tm((size_t)nr * nr, 0.0), corner((size_t)nr * nr, 0.0);
+    double* outd = s->partial + (size_t)nblk * (2 * nr);
+    for (int n = 1; n < nsh; n++) {
+        ShellSel sel{radii[n - 1], radii[n]};
+        TRY((d_xinv<R, 2>(g, S, X1, 3, twx, sel, st)));
+        TRY(d_yinv<R>(g, X1, X2 + 3 * g.cB, 3, twy, st));
+        TRY((d_z<R, 2>(g, X2, X3, twz, twn, st)));
+        TRY(d_yfwd<R>(g, X3, X1, 9, twy, st));
+        TRY(d_xfwd_et<R>(g, X1, S, twx, s->radii, nr, kin, s->partial, &nblk, st));
+        hipLaunchKernelGGL(th_finalize_bins, dim3((2 * nr + 255) / 256), dim3(256), 0, st, (const double*)s->partial,
+                           nblk, 2 * nr, 2 * nr, outd);
+        TH_CHECK_HIP(hipGetLastError());
+        TH_CHECK_HIP(hipMemcpyAsync(hostbuf.data(), outd, 2 * nr * sizeof(double), hipMemcpyDeviceToHost, st));
+        TH_CHECK_HIP(hipStreamSynchronize(st));
+        for (int m = 0; m < nr; m++) {
+            Tm[(size_t)n * nr + m] = -hostbuf[m];
+            corner[(size_t)n * nr + m] = hostbuf[nr + m];
+        }
+    }
+    if (T)
+        for (size_t i = 0; i < (size_t)nr * nr; i++) T[i] = Tm[i];
+    if (Pi) {
+        Pi[0] = 0;
+        for (int i = 1; i <= nsh; i++) {
+            double acc_ = 0;
+            for (int n = 1; n <= i && n < nsh; n++)
+                for (int m = i + 1; m < nr; m++) acc_ += Tm[(size_t)n * nr + m] - corner[(size_t)n * nr + m];
+            Pi[i] = (i >= nsh) ? 0.0 : acc_;
+        }
+    }
+    return OK;
+}
+
+extern "C" int th_flux_shell_tr(th_plan* p, th_state* s, const double* radii, int nsh, double* Pi, double* T)
+{
+    if (!p || !s || s->plan != p || !radii || nsh < 2) { set_error("th_flux_shell_tr: bad args"); return E_ARG; }
+    if (s->physics != TH_NS) { set_error("th_flux_shell_tr: NS state required"); return E_UNSUPPORTED; }
+    if (radii[nsh] < 1e3) { set_error("th_flux_shell_tr: radii[nsh] must be the infinite radius"); return E_ARG; }
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    return p->prec ? flux_impl<float>(s, radii, nsh, Pi, T) : flux_impl<double>(s, radii, nsh, Pi, T);
+}

@@ -1,0 +1,151 @@
+"""GPU parity of the fused, dealias-pruned path (th_state / th_step / th_flux_shell_tr).
+
+The fused step assumes a 2/3-dealiased state, which the reference keeps after every step
+(time_advance.cc:162) and after its ICs.  FP64 runs match the oracle to <= 1e-12 relative per
+few steps and <= 1e-10 in E(k) and Pi(k) after 100 steps (config 1); FP32-mixed runs match the
+FP64 oracle to <= 1e-5 relative in E(k)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from pyoracle import Oracle  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    from tarang_amd import Context
+    return Context(0)
+
+
+def H(t):
+    return t.detach().cpu().numpy().astype(np.complex128)
+
+
+def rel(a, b):
+    m = np.abs(b).max()
+    return np.abs(a - b).max() / (m if m > 0 else 1.0)
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("N", [(16, 16, 16), (48, 32, 24)])
+def test_upload_download_roundtrip(ctx, N):
+    from tarang_amd import Plan, State
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    s = State(p)
+    rng = np.random.default_rng(0)
+    V = [rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape) for _ in range(3)]
+    s.upload([p.to_dev(v) for v in V])
+    out = s.download()
+    for i in range(3):
+        assert np.array_equal(H(out[i]), o.dealias(V[i].copy()))
+
+
+@pytest.mark.parametrize("N", [(16, 16, 16), (32, 32, 32), (48, 32, 24), (64, 64, 64)])
+def test_fused_rk4_ns_fp64(ctx, N):
+    from tarang_amd import Plan, State
+    log(f"fused ns fp64 N={N}")
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    s = State(p)
+    V = o.ic_random(21)
+    s.upload([p.to_dev(v) for v in V])
+    nu, dt = 0.01, 2e-3
+    o.rk4_ns(V, nu, dt, steps=3)
+    s.step(dt, nu=nu, steps=3)
+    out = s.download()
+    for i in range(3):
+        assert rel(H(out[i]), V[i]) < 1e-12
+
+
+def test_fused_rk4_ns_fp32_mixed(ctx):
+    from tarang_amd import Plan, State
+    N = (32, 32, 32)
+    o = Oracle(N)
+    p = Plan(ctx, N, precision="fp32")
+    s = State(p)
+    V = o.ic_random(22)
+    s.upload([p.to_dev(v) for v in V])
+    o.rk4_ns(V, 0.01, 2e-3, steps=5)
+    s.step(2e-3, nu=0.01, steps=5)
+    out = s.download()
+    Ek = o.vector_spectrum(V)
+    p64 = Plan(ctx, N)
+    gEk = p64.vector_spectrum([p64.to_dev(H(a)) for a in out])
+    sel = Ek >= 1e-10 * Ek.max()
+    err = (np.abs(gEk - Ek)[sel] / Ek[sel]).max()
+    log(f"fp32-mixed E(k) max rel err {err:.2e}")
+    assert err < 1e-5
+
+
+def test_fused_mhd_fp64(ctx):
+    from tarang_amd import Plan, State, abi
+    N = (32, 32, 32)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    s = State(p, abi.TH_MHD)
+    U, B = o.ic_random(2), o.ic_random(3)
+    s.upload([p.to_dev(a) for a in U + B])
+    o.rk4_mhd(U, B, 0.01, 0.02, 2e-3, steps=3)
+    s.step(2e-3, nu=0.01, eta=0.02, steps=3)
+    out = s.download()
+    for i, ref in enumerate(U + B):
+        assert rel(H(out[i]), ref) < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 2e-5)])
+def test_fused_flux_and_shell_transfer(ctx, prec, tol):
+    """Pi(R) from shell givers + corner bins equals the reference's sphere loop (Compute_flux) and
+    T(n,m) equals Compute_shell_tr."""
+    from tarang_amd import Plan, State
+    N = (32, 32, 32)
+    o = Oracle(N)
+    p = Plan(ctx, N, precision=prec)
+    s = State(p)
+    V = o.ic_random(5, kp=3.0)
+    s.upload([p.to_dev(v) for v in V])
+    radii = np.concatenate([np.arange(12, dtype=float), [10000.0]])
+    Pi, T = s.flux_shell_tr(radii)
+    f_ref = o.flux_ns(V, radii)
+    T_ref = o.shell_tr_ns(V, radii)
+    scale_f = np.abs(f_ref).max()
+    assert np.abs(Pi - f_ref).max() < tol * scale_f
+    assert np.abs(T - T_ref).max() < tol * np.abs(T_ref).max()
+
+
+def test_config1_fused_taylor_green_64(ctx):
+    """Config 1 through the fused path: TG 64^3 FP64, RK4, dt = 1e-3, 100 steps, nu = 0.01."""
+    from tarang_amd import Plan, State
+    N = (64, 64, 64)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    s = State(p)
+    V = o.ic_taylor_green(1, 1.0)
+    s.upload([p.to_dev(v) for v in V])
+    o.rk4_ns(V, 0.01, 1e-3, steps=100)
+    s.step(1e-3, nu=0.01, steps=100)
+    out = s.download()
+    Ek = o.vector_spectrum(V)
+    gEk = p.vector_spectrum(out)
+    sel = Ek >= 1e-10 * Ek.max()
+    err = (np.abs(gEk - Ek)[sel] / Ek[sel]).max()
+    log(f"config1 fused max rel E(k) err {err:.3e}")
+    assert err <= 1e-10
+    radii = np.concatenate([np.arange(0, 22, dtype=float), [10000.0]])
+    s2 = State(p)
+    s2.upload(out)
+    Pi, _ = s2.flux_shell_tr(radii)
+    fo = o.flux_ns(V, radii)
+    sel = np.abs(fo) >= 1e-10 * np.abs(fo).max()
+    ferr = (np.abs(Pi - fo)[sel] / np.abs(fo)[sel]).max()
+    log(f"config1 fused max rel flux err {ferr:.3e}")
+    assert ferr <= 1e-10
