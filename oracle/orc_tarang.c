@@ -1453,3 +1453,17 @@ void orc_ic_random_scalar(const orc_grid *g, double *T, unsigned long long seed,
     orc_satisfy_reality(g, T);
     if (g->dealias) orc_dealias(g, T);
 }
+
+/* First stage of the NS RK4 step (time_advance.cc:117-125): rhs at V, V <- E_1/2 (V + dt/2 N),
+ * tot = dt/6 E_1 N.  Used as the bounded CPU-baseline sample (one RK stage of the workload). */
+void orc_rk4_stage1_ns(const orc_grid *g, double *V[3], double nu, double dt)
+{
+    stepper_t s; memset(&s, 0, sizeof(s));
+    s.nvec = 1; s.nc = 3;
+    double *tot[3];
+    for (int i = 0; i < 3; i++) { s.F[i] = V[i]; s.N[i] = alloc_field(g); s.diss[i] = nu; tot[i] = alloc_field(g); }
+    rhs_ns(g, &s, NULL);
+    single_time_step(g, &s, dt, 0.5, 0.5, 0.5);
+    rk4_parts(g, &s, tot, dt, 1.0, dt / 6);
+    for (int i = 0; i < 3; i++) { free(s.N[i]); free(tot[i]); }
+}

@@ -59,6 +59,7 @@ def lib():
             "orc_rk4_step_ns": (None, [p, pp, d, d]),
             "orc_euler_step_ns": (None, [p, pp, d, d]),
             "orc_rk2_step_ns": (None, [p, pp, d, d]),
+            "orc_rk4_stage1_ns": (None, [p, pp, d, d]),
             "orc_nlin_mhd": (None, [p, pp, pp, pp, pp]),
             "orc_rk4_step_mhd": (None, [p, pp, pp, d, d, d]),
             "orc_energy": (d, [p, vp, vp]),
@@ -198,6 +199,10 @@ class Oracle:
     def rk4_ns(self, V, nu, dt, steps=1):
         for _ in range(steps):
             self.L.orc_rk4_step_ns(self.gp, _pp(V), nu, dt)
+        return V
+
+    def rk4_stage1_ns(self, V, nu, dt):
+        self.L.orc_rk4_stage1_ns(self.gp, _pp(V), nu, dt)
         return V
 
     def euler_ns(self, V, nu, dt):

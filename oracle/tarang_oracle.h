@@ -66,6 +66,7 @@ void orc_rhs(const orc_grid *g, double *nlin[3], const double *p);
 void orc_rk4_step_ns(const orc_grid *g, double *V[3], double nu, double dt);
 void orc_euler_step_ns(const orc_grid *g, double *V[3], double nu, double dt);
 void orc_rk2_step_ns(const orc_grid *g, double *V[3], double nu, double dt);
+void orc_rk4_stage1_ns(const orc_grid *g, double *V[3], double nu, double dt);
 
 /* ---- MHD (compute_nlin_VF_MHD.cc, time_advance_VF_MHD.cc) ---- */
 void orc_nlin_mhd(const orc_grid *g, double *U[3], double *B[3], double *nU[3], double *nB[3]);

@@ -101,6 +101,7 @@ def lib():
         "th_state_download": (i, [vp, i, vp, i]),
         "th_step": (i, [vp, vp, d, C.POINTER(Phys), i]),
         "th_flux_shell_tr": (i, [vp, vp, dp, i, dp, dp]),
+        "th_profile_passes": (i, [vp, vp, d, C.POINTER(Phys), i, i, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -438,6 +439,17 @@ class State:
         ph = Phys(nu, eta, kappa, Ra, Pr)
         self.plan._pre()
         self.plan._post(lib().th_step(self.plan.h, self.h, dt, C.byref(ph), steps), "th_step")
+
+    def profile_passes(self, dt, nu=0.0, eta=0.0, stage=2, nrep=5):
+        import numpy as np
+        ph = Phys(nu, eta, 0.0, 0.0, 0.0)
+        ms = np.zeros(5)
+        b = np.zeros(5)
+        self.plan._pre()
+        _check(lib().th_profile_passes(self.plan.h, self.h, dt, C.byref(ph), stage, nrep,
+                                       ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                       b.ctypes.data_as(C.POINTER(C.c_double))), "th_profile_passes")
+        return ms, b
 
     def flux_shell_tr(self, radii):
         import numpy as np

@@ -986,3 +986,85 @@ extern "C" int th_flux_shell_tr(th_plan* p, th_state* s, const double* radii, in
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
     return p->prec ? flux_impl<float>(s, radii, nsh, Pi, T) : flux_impl<double>(s, radii, nsh, Pi, T);
 }
+
+// ------------------------------------------------------------------------------------------
+// Per-pass timing of one fused RK stage with HIP events on the plan's stream, and each pass's
+// algorithmic HBM bytes (compulsory reads + writes of kept data; padding excluded).
+// ------------------------------------------------------------------------------------------
+template <typename R> static int profile_impl(th_state* s, double dt, double nu, double eta, int stage, int nrep,
+                                              double* ms, double* bytes)
+{
+    th_plan* p = s->plan;
+    hipStream_t st = p->ctx->stream;
+    const FG& g = s->fg;
+    const cplx<R>* twx = p->twd<R>(g.N1);
+    const cplx<R>* twy = p->twd<R>(g.N2);
+    const cplx<R>* twz = p->twd<R>(g.LZ);
+    const cplx<R>* twn = p->twd<R>(g.N3);
+    cplx<R>* S = (cplx<R>*)s->S;
+    cplx<R>* Vs = (cplx<R>*)s->Vs;
+    cplx<R>* acc = (cplx<R>*)s->acc;
+    cplx<R>* X1 = (cplx<R>*)s->X1;
+    cplx<R>* X2 = (cplx<R>*)s->X2;
+    cplx<R>* X3 = (cplx<R>*)s->X3;
+    StageCoef sc;
+    sc.stage = stage;
+    sc.dt = dt;
+    sc.fh[0] = (-nu * dt) * 0.5;
+    sc.f1[0] = (-nu * dt) * 1.0;
+    sc.fh[1] = (-eta * dt) * 0.5;
+    sc.f1[1] = (-eta * dt) * 1.0;
+    ShellSel sel{0, 0};
+    const bool mhd = s->physics == TH_MHD;
+    const int nin = mhd ? 6 : 3, nout = mhd ? 9 : 6, nst = mhd ? 6 : 3;
+    const cplx<R>* src = stage == 1 ? S : Vs;
+    hipEvent_t ev[6];
+    for (int i = 0; i < 6; i++) TH_CHECK_HIP(hipEventCreate(&ev[i]));
+    double tot[5] = {0, 0, 0, 0, 0};
+    for (int rep = 0; rep < nrep; rep++) {
+        TH_CHECK_HIP(hipEventRecord(ev[0], st));
+        if (mhd) TRY((d_xinv<R, 1>(g, src, X1, 6, twx, sel, st)));
+        else TRY((d_xinv<R, 0>(g, src, X1, 3, twx, sel, st)));
+        TH_CHECK_HIP(hipEventRecord(ev[1], st));
+        TRY(d_yinv<R>(g, X1, X2, nin, twy, st));
+        TH_CHECK_HIP(hipEventRecord(ev[2], st));
+        if (mhd) TRY((d_z<R, 1>(g, X2, X3, twz, twn, st)));
+        else TRY((d_z<R, 0>(g, X2, X3, twz, twn, st)));
+        TH_CHECK_HIP(hipEventRecord(ev[3], st));
+        TRY(d_yfwd<R>(g, X3, X1, nout, twy, st));
+        TH_CHECK_HIP(hipEventRecord(ev[4], st));
+        if (mhd) TRY((d_xfwd<R, 1>(g, X1, S, Vs, acc, twx, sc, st)));
+        else TRY((d_xfwd<R, 0>(g, X1, S, Vs, acc, twx, sc, st)));
+        TH_CHECK_HIP(hipEventRecord(ev[5], st));
+        TH_CHECK_HIP(hipEventSynchronize(ev[5]));
+        for (int k = 0; k < 5; k++) {
+            float t = 0;
+            TH_CHECK_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+            tot[k] += t;
+        }
+    }
+    for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
+    const double e = (double)sizeof(cplx<R>);
+    const double cS = (double)g.mx.nk * g.my.nk * g.nkz, cA = (double)g.N1 * g.my.nk * g.nkz,
+                 cB = (double)g.N1 * g.N2 * g.nkz;
+    const int state_rw = (stage == 1 || stage == 4) ? 3 : 4;   // V0 (+acc) read, Vs / acc / S written
+    const double b[5] = {e * (nst * cS + nin * cA), e * (nin * cA + nin * cB), e * (nin * cB + nout * cB),
+                         e * (nout * cB + nout * cA), e * (nout * cA + state_rw * nst * cS)};
+    for (int k = 0; k < 5; k++) {
+        ms[k] = tot[k] / nrep;
+        bytes[k] = b[k];
+    }
+    return OK;
+}
+
+extern "C" int th_profile_passes(th_plan* p, th_state* s, double dt, const th_phys* ph, int stage, int nrep, double* ms,
+                                 double* bytes)
+{
+    if (!p || !s || s->plan != p || !ph || stage < 1 || stage > 4 || nrep < 1 || !ms || !bytes) {
+        set_error("th_profile_passes: bad args");
+        return E_ARG;
+    }
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    return p->prec ? profile_impl<float>(s, dt, ph->nu, ph->eta, stage, nrep, ms, bytes)
+                   : profile_impl<double>(s, dt, ph->nu, ph->eta, stage, nrep, ms, bytes);
+}
