@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a known-size kernel "
+                         "that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs one box per GPU on a single MI355X; multi-GPU runs "
@@ -107,6 +110,13 @@ def main():
 
     N = (args.n, args.n, args.n)
     nu = 1e-3
+    if args.calib_copy_mib > 0:
+        src = torch.ones(args.calib_copy_mib * (1 << 18), dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {args.calib_copy_mib} MiB done")
+        del src, dst
     ctx = Context(0)
     p64 = Plan(ctx, N, precision="fp64")
     p32 = Plan(ctx, N, precision="fp32")

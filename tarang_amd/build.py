@@ -66,6 +66,12 @@ def build(jobs=8, force=False, verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    # C++ driver over the host classes (links the library in place, rpath $ORIGIN)
+    drv = os.path.join(HERE, "tarang_run")
+    _run([HIPCC, "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+          "-o", drv + ".tmp", os.path.join(CSRC, "tarang_run.cpp"), "-L" + HERE, "-ltarang_hip",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(drv + ".tmp", drv)
     return LIB
 
 

@@ -1,0 +1,141 @@
+// tarang_host.hpp -- C++ host classes mirroring the reference's hot-path interfaces, implemented
+// over the C-ABI of libtarang_hip.so.  Method names and argument meanings follow the reference:
+//   CVF      trunk/lib/fields/Cvf.h:82      (V1..V3 spectral, CV_Forward/Inverse_transform,
+//                                            CV_Compute_totalenergy_diss, CV_Compute_shell_spectrum, CV_Dealias)
+//   RVF      trunk/lib/fields/Rvf.h:92      (V1r..V3r, RV_Inverse_transform, RV_Forward_transform_RSprod)
+//   NLIN     trunk/lib/IncFlow/Nlin.h:51    (nlin1..3, F (pressure), NLIN_diag_Forward_transform_derivative)
+//   IncVF    trunk/lib/IncFlow/IncVF.h:75   (Compute_nlin, Compute_nlin(IncVF&), Compute_pressure,
+//                                            Compute_flux, Compute_shell_tr)
+//   IncFluid trunk/lib/IncFluid/IncFluid.h:82 (Compute_rhs, Time_advance, Get_dt, Tnow, Tdt)
+//   EnergyTr trunk/lib/IncFlow/EnergyTr.h:97 (sphere / shell radii, flux_self, shelltoshell_self)
+// Fields live in device memory in the reference layout (N1, N2, N3/2+1); errors become
+// std::runtime_error carrying th_last_error().  Time_advance uses the fused dealias-pruned step;
+// Compute_nlin & co. use the plain path that keeps the reference's operation sequence.
+#pragma once
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "tarang_hip.h"
+
+namespace tarang {
+
+void check(int rc, const char* what);
+
+struct Grid {
+    int N[4];                 // 1-based like the reference's N[]
+    double kfactor[4];        // 1-based
+    std::string basis_type;   // "FOUR" | "SCFT"
+    std::string alias_switch; // "DEALIAS" | "ALIAS"
+    std::string precision;    // "FP64" | "FP32_MIXED"
+};
+
+class Context {
+public:
+    explicit Context(int device = 0);
+    ~Context();
+    th_ctx* h = nullptr;
+};
+
+// A device field: one spectral component in the reference layout.
+class DField {
+public:
+    DField() = default;
+    DField(size_t elems, size_t esize);
+    ~DField();
+    DField(const DField&) = delete;
+    DField& operator=(const DField&) = delete;
+    DField(DField&& o) noexcept;
+    DField& operator=(DField&& o) noexcept;
+    void* data() const { return p_; }
+    void upload(const void* host);
+    void download(void* host) const;
+    void copy_from(const DField& o);
+    void zero();
+    size_t bytes() const { return n_ * es_; }
+
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0, es_ = 0;
+};
+
+class CVF {
+public:
+    CVF(Context& ctx, const Grid& g);
+    virtual ~CVF();
+    void CV_Forward_transform();                     // Cvf.cc:294 (in place, parity V1 SFT, V2/V3 CFT)
+    void CV_Inverse_transform();
+    void CV_Compute_totalenergy_diss();              // Cvf.cc:385-397
+    void CV_Compute_shell_spectrum(std::vector<double>& result); // Cvf.cc:419-449 (n = 0, summed comps)
+    void CV_Dealias();                               // Cvf.cc:699-704
+    int shell_ek_size() const;
+    th_plan* plan() const { return plan_; }
+    DField V1, V2, V3;
+    double CV_total_energy = 0, CV_total_dissipation = 0;
+    Grid grid;
+
+protected:
+    Context& ctx_;
+    th_plan* plan_ = nullptr;
+    size_t elems_ = 0, esize_ = 0;
+    void* v(int i) const { return i == 0 ? V1.data() : i == 1 ? V2.data() : V3.data(); }
+};
+
+class RVF {
+public:
+    RVF(CVF& owner);
+    void RV_Inverse_transform();                     // Rvf.cc:135 (V1r SFT, V2r/V3r CFT)
+    void RV_Forward_transform_RSprod();              // Rvf.cc:179 (V1V2, V3V1 SFT; V2V3 CFT)
+    DField V1r, V2r, V3r;
+
+private:
+    CVF& o_;
+};
+
+class NLIN {
+public:
+    NLIN(CVF& owner);
+    void NLIN_diag_Forward_transform_derivative();   // Nlin.cc:85-94
+    DField nlin1, nlin2, nlin3, F;
+
+private:
+    CVF& o_;
+};
+
+class EnergyTr {
+public:
+    EnergyTr(const CVF& f, int no_shells, int shell_input_scheme);   // EnergyTr.cc:44-170
+    std::vector<double> shell_radius, sphere_radius;
+    std::vector<double> flux_self;                   // (no_spheres+1)
+    std::vector<double> shelltoshell_self;           // (no_shells+1)^2 row-major
+    int no_shells, no_spheres;
+};
+
+class IncVF : public CVF, public NLIN, public EnergyTr {
+public:
+    IncVF(Context& ctx, const Grid& g, double dissipation_coefficient, int no_shells = 32, int shell_scheme = 1);
+    void Compute_nlin();                             // compute_nlin.cc:58-108 (plain path)
+    void Compute_pressure();                         // compute_pressure.cc:54-62
+    void Compute_flux();                             // compute_flux.cc:52-69
+    void Compute_shell_tr();                         // compute_shell_tr.cc:59-79
+    void Setup_Taylor_Green_field(int k0, double amp);   // ic_model.cc:58-107
+    double dissipation_coefficient;
+};
+
+class IncFluid : public IncVF {
+public:
+    IncFluid(Context& ctx, const Grid& g, double nu, double Tinit, double Tdt_fixed, const std::string& scheme = "RK4",
+             bool fused = true, int no_shells = 32);
+    ~IncFluid() override;
+    void Compute_rhs();                              // compute_rhs.cc:52-60
+    void Time_advance();                             // time_advance.cc:60-163
+    double Get_dt();                                 // compute_dt.cc:47-59
+    double Tnow, Tdt, Tdt_fixed;
+    std::string integ_scheme;
+
+private:
+    bool fused_;
+    th_state* state_ = nullptr;
+    bool state_fresh_ = false;
+};
+
+}  // namespace tarang

@@ -325,31 +325,39 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restric
     for (int a = 0; a < NA; a++)
 #pragma unroll
         for (int i = 0; i < E; i++) Nacc[a][i] = czero<R>();
-#pragma unroll
+    // The product loop stays rolled (one product line live at a time).  Each product p adds
+    // i (alpha_a Kx + beta_a) P_p to accumulator a, so accumulators are indexed statically.
+#pragma unroll 1
     for (int p = 0; p < NP; p++) {
         cplx<R> v[E];
         const cplx<R>* src = D + p * g.cA + m;
 #pragma unroll
         for (int i = 0; i < E; i++) v[i] = ok ? src[(long)F::in_pos(t, i) * g.sx] : czero<R>();
         F::template run<T>(v, lds + c, t, tw);
+        // contributions (a1 <- K_k1) and (a2 <- K_k2) of product p; k = 0 means Kx (per row)
+        int a1, k1, a2, k2;
+        if (MODE == 0) {           // P_ij, (i,j) = (0,0),(1,1),(2,2),(0,1),(1,2),(2,0): N_i += iK_j P, N_j += iK_i P
+            const int pi = p < 3 ? p : p - 3, pj = p < 3 ? p : (p == 5 ? 0 : p - 2);
+            a1 = pi; k1 = pj;
+            a2 = (pi != pj) ? pj : -1; k2 = pi;
+        } else {                   // P(pi,pj) = Z+_pi Z-_pj: N+_pi += iK_pj P, N-_pj += iK_pi P
+            const int pi = p / 3, pj = p % 3;
+            a1 = pi; k1 = pj;
+            a2 = 3 + pj; k2 = pi;
+        }
+        R al[NA], be[NA];
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+            al[a] = R(0);
+            be[a] = R(0);
+            if (a == a1) { if (k1 == 0) al[a] += R(1); else be[a] += (k1 == 1 ? Ky : Kz); }
+            if (a == a2) { if (k2 == 0) al[a] += R(1); else be[a] += (k2 == 1 ? Ky : Kz); }
+        }
 #pragma unroll
         for (int i = 0; i < E; i++) {
-            const int l = F::out_pos(t, i);
-            const R Kx = (R)(kx_four(l, g.N1) * g.f1);
-            if (MODE == 0) {
-                if (p == 0) acc_iK(Nacc[0][i], Kx, v[i]);
-                else if (p == 1) acc_iK(Nacc[1][i], Ky, v[i]);
-                else if (p == 2) acc_iK(Nacc[2][i], Kz, v[i]);
-                else if (p == 3) { acc_iK(Nacc[0][i], Ky, v[i]); acc_iK(Nacc[1][i], Kx, v[i]); }
-                else if (p == 4) { acc_iK(Nacc[1][i], Kz, v[i]); acc_iK(Nacc[2][i], Ky, v[i]); }
-                else { acc_iK(Nacc[2][i], Kx, v[i]); acc_iK(Nacc[0][i], Kz, v[i]); }
-            } else {
-                const int pi = p / 3, pj = p % 3;            // P(pi,pj) = Z+_pi Z-_pj
-                const R Kj = pj == 0 ? Kx : (pj == 1 ? Ky : Kz);
-                const R Ki = pi == 0 ? Kx : (pi == 1 ? Ky : Kz);
-                acc_iK(Nacc[pi][i], Kj, v[i]);               // N+_pi += i K_pj P(pi,pj)
-                acc_iK(Nacc[3 + pj][i], Ki, v[i]);           // N-_pj += i K_pi P(pi,pj)
-            }
+            const R Kx = (R)(kx_four(F::out_pos(t, i), g.N1) * g.f1);
+#pragma unroll
+            for (int a = 0; a < NA; a++) acc_iK(Nacc[a][i], al[a] * Kx + be[a], v[i]);
         }
     }
     if (!ok) return;
@@ -429,7 +437,7 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __rest
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int i = 0; i < E; i++) Nacc[a][i] = czero<R>();
-#pragma unroll
+#pragma unroll 1
     for (int p = 0; p < 9; p++) {
         cplx<R> v[E];
         const cplx<R>* src = D + p * g.cA + m;
@@ -437,12 +445,17 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __rest
         for (int i = 0; i < E; i++) v[i] = ok ? src[(long)F::in_pos(t, i) * g.sx] : czero<R>();
         F::template run<T>(v, lds + c, t, tw);
         const int pi = p / 3, pj = p % 3;        // P(pi,pj) = G_pi H_pj -> N_pi += i K_pj P
+        R al[3], be[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            al[a] = (a == pi && pj == 0) ? R(1) : R(0);
+            be[a] = (a == pi && pj != 0) ? (pj == 1 ? Ky : Kz) : R(0);
+        }
 #pragma unroll
         for (int i = 0; i < E; i++) {
-            const int l = F::out_pos(t, i);
-            const R Kx = (R)(kx_four(l, g.N1) * g.f1);
-            const R Kj = pj == 0 ? Kx : (pj == 1 ? Ky : Kz);
-            acc_iK(Nacc[pi][i], Kj, v[i]);
+            const R Kx = (R)(kx_four(F::out_pos(t, i), g.N1) * g.f1);
+#pragma unroll
+            for (int a = 0; a < 3; a++) acc_iK(Nacc[a][i], al[a] * Kx + be[a], v[i]);
         }
     }
     __syncthreads();
@@ -556,11 +569,21 @@ static int launch_yfwd(const FG& g, const cplx<R>* Cc, cplx<R>* D, int ncomp, co
     TH_CHECK_HIP(hipGetLastError());
     return OK;
 }
+// The x-forward pass keeps 3 (NS) or 6 (MHD) accumulator lines per thread next to the FFT line, so
+// it uses fewer points per thread than the other passes to stay inside the VGPR file (no spills).
+constexpr int xfwd_E(int L) { return L < 8 ? L : ((L % 3 == 0) ? (L >= 12 ? 12 : L) : 8); }
+template <typename R, int L, int MODE> constexpr int xfwd_T()
+{
+    constexpr int P = L / xfwd_E(L);
+    constexpr int t = (MODE == 0 && sizeof(R) == 4) ? 16 : 8;
+    return (t * P > 512) ? (512 / P < 1 ? 1 : 512 / P) : t;
+}
+
 template <typename R, int L, int MODE>
 static int launch_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
                        const StageCoef& sc, hipStream_t st)
 {
-    constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
+    constexpr int E = xfwd_E(L), P = L / E, T = xfwd_T<R, L, MODE>();
     const int ntz = (g.nkz + T - 1) / T;
     hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE>), dim3(g.my.nk * ntz), dim3(T * P), 0, st, g, D, S, Vs, acc, tw, sc);
     TH_CHECK_HIP(hipGetLastError());
@@ -570,7 +593,7 @@ template <typename R, int L>
 static int launch_xfwd_et(const FG& g, const cplx<R>* D, const cplx<R>* U, const cplx<R>* tw, const double* radii, int nr,
                           int kin, double* partial, int* nblk, hipStream_t st)
 {
-    constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
+    constexpr int E = xfwd_E(L), P = L / E, T = xfwd_T<R, L, 0>();
     const int ntz = (g.nkz + T - 1) / T;
     *nblk = g.my.nk * ntz;
     if (!partial) return OK;

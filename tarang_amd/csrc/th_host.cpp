@@ -1,0 +1,244 @@
+// th_host.cpp -- the C++ host classes of tarang_host.hpp over the C-ABI.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <string>
+#include "tarang_host.hpp"
+
+namespace tarang {
+
+void check(int rc, const char* what)
+{
+    if (rc != TH_OK) throw std::runtime_error(std::string(what) + ": " + th_last_error());
+}
+
+static void hipc(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Context::Context(int device) { check(th_ctx_create(&h, device, nullptr), "th_ctx_create"); }
+Context::~Context() { th_ctx_destroy(h); }
+
+DField::DField(size_t elems, size_t esize) : n_(elems), es_(esize)
+{
+    hipc(hipMalloc(&p_, n_ * es_), "hipMalloc");
+    hipc(hipMemset(p_, 0, n_ * es_), "hipMemset");
+}
+DField::~DField()
+{
+    if (p_) (void)hipFree(p_);
+}
+DField::DField(DField&& o) noexcept : p_(o.p_), n_(o.n_), es_(o.es_) { o.p_ = nullptr; }
+DField& DField::operator=(DField&& o) noexcept
+{
+    if (this != &o) {
+        if (p_) (void)hipFree(p_);
+        p_ = o.p_;
+        n_ = o.n_;
+        es_ = o.es_;
+        o.p_ = nullptr;
+    }
+    return *this;
+}
+void DField::upload(const void* host) { hipc(hipMemcpy(p_, host, bytes(), hipMemcpyHostToDevice), "upload"); }
+void DField::download(void* host) const { hipc(hipMemcpy(host, p_, bytes(), hipMemcpyDeviceToHost), "download"); }
+void DField::copy_from(const DField& o) { hipc(hipMemcpy(p_, o.p_, bytes(), hipMemcpyDeviceToDevice), "copy"); }
+void DField::zero() { hipc(hipMemset(p_, 0, bytes()), "zero"); }
+
+// ---------------------------------------------------------------- CVF
+CVF::CVF(Context& ctx, const Grid& g) : grid(g), ctx_(ctx)
+{
+    const int basis = g.basis_type == "SCFT" ? TH_SCFT : TH_FOUR;
+    const int prec = g.precision == "FP32_MIXED" ? TH_FP32_MIXED : TH_FP64;
+    const double kf[3] = {g.kfactor[1], g.kfactor[2], g.kfactor[3]};
+    check(th_plan_create(ctx.h, &plan_, basis, g.N[1], g.N[2], g.N[3], kf, prec, g.alias_switch != "ALIAS"),
+          "th_plan_create");
+    elems_ = th_plan_field_elems(plan_);
+    esize_ = th_plan_elem_bytes(plan_);
+    V1 = DField(elems_, esize_);
+    V2 = DField(elems_, esize_);
+    V3 = DField(elems_, esize_);
+}
+CVF::~CVF() { th_plan_destroy(plan_); }
+
+void CVF::CV_Forward_transform()
+{
+    check(th_forward(plan_, V1.data(), 1), "CV_Forward_transform");
+    check(th_forward(plan_, V2.data(), 0), "CV_Forward_transform");
+    check(th_forward(plan_, V3.data(), 0), "CV_Forward_transform");
+}
+void CVF::CV_Inverse_transform()
+{
+    check(th_inverse(plan_, V1.data(), 1), "CV_Inverse_transform");
+    check(th_inverse(plan_, V2.data(), 0), "CV_Inverse_transform");
+    check(th_inverse(plan_, V3.data(), 0), "CV_Inverse_transform");
+}
+void CVF::CV_Compute_totalenergy_diss()
+{
+    double e[3], s[3];
+    for (int i = 0; i < 3; i++) {
+        check(th_energy(plan_, v(i), nullptr, &e[i]), "th_energy");
+        check(th_Sn(plan_, v(i), 2, &s[i]), "th_Sn");
+    }
+    CV_total_energy = e[0] + e[1] + e[2];
+    CV_total_dissipation = 2 * (s[0] + s[1] + s[2]);
+}
+int CVF::shell_ek_size() const { return th_shell_ek_size(plan_); }
+void CVF::CV_Compute_shell_spectrum(std::vector<double>& result)
+{
+    int n = shell_ek_size();
+    result.assign(n, 0.0);
+    std::vector<double> tmp(n);
+    for (int i = 0; i < 3; i++) {
+        check(th_shell_spectrum(plan_, v(i), nullptr, 0, tmp.data(), nullptr), "th_shell_spectrum");
+        for (int k = 0; k < n; k++) result[k] += tmp[k];
+    }
+}
+void CVF::CV_Dealias()
+{
+    for (int i = 0; i < 3; i++) check(th_dealias(plan_, v(i)), "th_dealias");
+}
+
+// ---------------------------------------------------------------- RVF / NLIN
+RVF::RVF(CVF& o) : o_(o)
+{
+    size_t n = th_plan_field_elems(o.plan()), e = th_plan_elem_bytes(o.plan());
+    V1r = DField(n, e);
+    V2r = DField(n, e);
+    V3r = DField(n, e);
+}
+void RVF::RV_Inverse_transform()
+{
+    check(th_inverse(o_.plan(), V1r.data(), 1), "RV_Inverse_transform");
+    check(th_inverse(o_.plan(), V2r.data(), 0), "RV_Inverse_transform");
+    check(th_inverse(o_.plan(), V3r.data(), 0), "RV_Inverse_transform");
+}
+void RVF::RV_Forward_transform_RSprod()
+{
+    check(th_forward(o_.plan(), V1r.data(), 1), "RV_Forward_transform_RSprod");
+    check(th_forward(o_.plan(), V2r.data(), 0), "RV_Forward_transform_RSprod");
+    check(th_forward(o_.plan(), V3r.data(), 1), "RV_Forward_transform_RSprod");
+}
+
+NLIN::NLIN(CVF& o) : o_(o)
+{
+    size_t n = th_plan_field_elems(o.plan()), e = th_plan_elem_bytes(o.plan());
+    nlin1 = DField(n, e);
+    nlin2 = DField(n, e);
+    nlin3 = DField(n, e);
+    F = DField(n, e);
+}
+void NLIN::NLIN_diag_Forward_transform_derivative()
+{
+    th_plan* p = o_.plan();
+    check(th_forward(p, nlin1.data(), 0), "NLIN forward");
+    check(th_forward(p, nlin2.data(), 0), "NLIN forward");
+    check(th_forward(p, nlin3.data(), 0), "NLIN forward");
+    check(th_deriv(p, 1, nlin1.data(), nlin1.data(), 0), "Xderiv");
+    check(th_deriv(p, 2, nlin2.data(), nlin2.data(), 0), "Yderiv");
+    check(th_deriv(p, 3, nlin3.data(), nlin3.data(), 0), "Zderiv");
+}
+
+// ---------------------------------------------------------------- EnergyTr
+EnergyTr::EnergyTr(const CVF& f, int nsh, int scheme) : no_shells(nsh), no_spheres(nsh)
+{
+    shell_radius.assign(nsh + 1, 0.0);
+    if (scheme == 0) {
+        double mri = 1.0 * th_max_radius_inside(f.plan());
+        shell_radius[0] = 0;
+        shell_radius[1] = 2;
+        shell_radius[2] = 4;
+        shell_radius[3] = 8;
+        shell_radius[nsh - 2] = mri / 2;
+        shell_radius[nsh - 1] = mri;
+        if (nsh > 6) {
+            double s = log2(mri / 16.0) / (nsh - 5);
+            for (int i = 4; i <= nsh - 3; i++) shell_radius[i] = 8 * pow(2, s * (i - 3));
+        }
+    } else {
+        for (int i = 0; i < nsh; i++) shell_radius[i] = i;
+    }
+    shell_radius[nsh] = 10000;      // INF_RADIUS extern_def_vars.h:88
+    sphere_radius = shell_radius;
+    flux_self.assign(nsh + 1, 0.0);
+    shelltoshell_self.assign((size_t)(nsh + 1) * (nsh + 1), 0.0);
+}
+
+// ---------------------------------------------------------------- IncVF
+IncVF::IncVF(Context& ctx, const Grid& g, double nu, int nsh, int scheme)
+    : CVF(ctx, g), NLIN(static_cast<CVF&>(*this)), EnergyTr(static_cast<CVF&>(*this), nsh, scheme),
+      dissipation_coefficient(nu)
+{
+}
+
+void IncVF::Compute_nlin()
+{
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    void* N[3] = {nlin1.data(), nlin2.data(), nlin3.data()};
+    check(th_nlin_ns(plan_, V, N), "Compute_nlin");
+}
+void IncVF::Compute_pressure()
+{
+    void* N[3] = {nlin1.data(), nlin2.data(), nlin3.data()};
+    check(th_pressure(plan_, N, F.data()), "Compute_pressure");
+}
+void IncVF::Compute_flux()
+{
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    check(th_flux_ns(plan_, V, sphere_radius.data(), no_spheres, flux_self.data()), "Compute_flux");
+}
+void IncVF::Compute_shell_tr()
+{
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    check(th_shell_tr_ns(plan_, V, shell_radius.data(), no_shells, shelltoshell_self.data()), "Compute_shell_tr");
+}
+void IncVF::Setup_Taylor_Green_field(int k0, double amp)
+{
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    check(th_ic_taylor_green(plan_, V, k0, amp), "Setup_Taylor_Green_field");
+}
+
+// ---------------------------------------------------------------- IncFluid
+IncFluid::IncFluid(Context& ctx, const Grid& g, double nu, double Tinit, double tdt_fixed, const std::string& scheme,
+                   bool fused, int nsh)
+    : IncVF(ctx, g, nu, nsh, 1), Tnow(Tinit), Tdt(tdt_fixed), Tdt_fixed(tdt_fixed), integ_scheme(scheme),
+      fused_(fused && g.basis_type == "FOUR" && scheme == "RK4")
+{
+    if (fused_) check(th_state_create(plan_, &state_, TH_NS), "th_state_create");
+}
+IncFluid::~IncFluid()
+{
+    if (state_) th_state_destroy(state_);
+}
+void IncFluid::Compute_rhs()
+{
+    void* N[3] = {nlin1.data(), nlin2.data(), nlin3.data()};
+    check(th_rhs(plan_, N, F.data()), "Compute_rhs");
+}
+double IncFluid::Get_dt()
+{
+    CV_Compute_totalenergy_diss();
+    double kmax = th_max_radius_inside(plan_);
+    double dx = (2 * M_PI) / kmax;
+    double dt = dx / (sqrt(CV_total_energy) * 20);
+    return dt < Tdt_fixed ? dt : Tdt_fixed;
+}
+void IncFluid::Time_advance()
+{
+    th_phys ph{dissipation_coefficient, 0, 0, 0, 0};
+    if (fused_) {
+        void* V[3] = {V1.data(), V2.data(), V3.data()};
+        for (int i = 0; i < 3; i++) check(th_state_upload(state_, i, V[i], 0), "th_state_upload");
+        check(th_step(plan_, state_, Tdt, &ph, 1), "th_step");
+        for (int i = 0; i < 3; i++) check(th_state_download(state_, i, V[i], 0), "th_state_download");
+        check(th_sync(plan_), "th_sync");
+        return;
+    }
+    int sch = integ_scheme == "EULER" ? 0 : integ_scheme == "RK2" ? 1 : 2;
+    void* const V[3] = {V1.data(), V2.data(), V3.data()};
+    check(th_step_plain(plan_, TH_NS, sch, V, Tdt, &ph), "th_step_plain");
+    check(th_sync(plan_), "th_sync");
+}
+
+}  // namespace tarang
