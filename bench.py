@@ -8,6 +8,7 @@ the FP64 GPU path is timed on the same IC and reported beside it, with the E(k) 
 FP32-mixed run against the FP64 run after the same number of steps.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-fp64]
+  python bench.py --workload ensemble [--steps K] [--warmup W] [--no-cpu]
 Prints one JSON line.  Runs on one GPU; --gpus > 1 exits with a message (single-GPU build).
 """
 import argparse
@@ -86,8 +87,127 @@ def load_traffic(workload):
     return None
 
 
+ENSEMBLE_SHAPES = [64] * 24 + [96] * 24 + [128] * 16
+
+
+def bench_ensemble(args):
+    """The uneven 64-box ensemble (24 x 64^3, 24 x 96^3, 16 x 128^3), FP32-mixed, box b seeded
+    100 + b, nu_b ~ U[5e-4, 5e-3] (numpy default_rng(5)), dt_b from CFL 0.5 at t = 0 (FP64).
+    value = sum_b N_b^3 x 4 stages per step / wall of the batched, graph-captured ensemble step;
+    the same boxes stepped one after another with the single-box fused step are timed beside it."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from tarang_amd import Context, Ensemble, Plan, State, abi
+
+    shapes = [(n, n, n) for n in ENSEMBLE_SHAPES]
+    nb = len(shapes)
+    nu = np.random.default_rng(5).uniform(5e-4, 5e-3, nb)
+    ctx = Context(0)
+    ens = Ensemble(ctx, shapes, precision="fp32")
+    p64 = {n: Plan(ctx, (n, n, n), precision="fp64") for n in sorted(set(ENSEMBLE_SHAPES))}
+    dt = np.zeros(nb)
+    E0 = np.zeros(nb)
+    singles = []
+    host_ic = []
+    for b, N in enumerate(shapes):
+        V = p64[N[0]].ic_random(100 + b, 0.5, 4.0)
+        dt[b], _ = cfl_dt(p64[N[0]], V)
+        E0[b] = p64[N[0]].total_energy(V)
+        V32 = [v.to(torch.complex64) for v in V]
+        ens.upload(b, V32)
+        ps = Plan(ctx, N, precision="fp32")
+        st = State(ps)
+        st.upload(V32)
+        singles.append(st)
+        if not args.no_cpu and b in (0, 24, 48):
+            host_ic.append((b, [v.cpu().numpy() for v in V]))
+        del V, V32
+    log(f"bench: ensemble of {nb} boxes ready, dt in [{dt.min():.3e}, {dt.max():.3e}]")
+    L = abi.lib()
+    phys = [abi.Phys(float(nu[b]), 0.0, 0.0, 0.0, 0.0) for b in range(nb)]
+
+    def seq(steps):
+        # each box advanced alone by its own captured single-box step, one box after another
+        for b, st in enumerate(singles):
+            abi._check(L.th_step(st.plan.h, st.h, float(dt[b]), C.byref(phys[b]), steps), "th_step")
+        for st in singles:
+            abi._check(L.th_sync(st.plan.h), "th_sync")
+
+    def batched(steps):
+        ens.step(dt, nu, steps=steps)
+
+    res = {}
+    for name, fn in (("ensemble", batched), ("sequential", seq)):
+        fn(args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn(args.steps)
+        torch.cuda.synchronize()
+        res[name] = time.perf_counter() - t0
+        log(f"bench: {name} {args.steps} steps {res[name]:.4f} s")
+    # the batched and the sequential runs advanced every box by the same steps from the same IC
+    worst = 0.0
+    for b in (0, 24, 48, nb - 1):
+        a, r = ens.download(b), singles[b].download()
+        worst = max(worst, max(float((x - y).abs().max()) for x, y in zip(a, r)))
+    pts = float(sum(N[0] * N[1] * N[2] for N in shapes))
+    value = pts * 4 * args.steps / res["ensemble"] / 1e6
+    seqv = pts * 4 * args.steps / res["sequential"] / 1e6
+    # algorithmic bytes per RK stage: 51 S_b per box, S_b = N1 N2 (N3/2+1) x 8 B (FP32 complex)
+    stage_bytes = sum(51.0 * N[0] * N[1] * (N[2] // 2 + 1) * 8 for N in shapes)
+    ach = stage_bytes * 4 * args.steps / res["ensemble"] / 1e9
+    cpu = None
+    if not args.no_cpu and host_ic:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from pyoracle import Oracle
+        threads = args.cpu_threads if args.cpu_threads > 0 else min(os.cpu_count() or 1, 16)
+        tot_pts, tot_s, used = 0.0, 0.0, 1
+        for b, Vh in host_ic:
+            N = shapes[b]
+            o = Oracle(N, threads=threads)
+            V = [v.copy() for v in Vh]
+            t0 = time.perf_counter()
+            o.rk4_ns(V, float(nu[b]), float(dt[b]), steps=1)
+            tot_s += time.perf_counter() - t0
+            tot_pts += 4.0 * N[0] * N[1] * N[2]
+            used = o.L.orc_get_threads()
+        cpu = {"value": round(tot_pts / tot_s / 1e6, 3), "unit": "Mpts*RKstage/s", "cores": int(used),
+               "kind": "port", "seconds": round(tot_s, 2),
+               "sample": "1 full RK4 step of one 64^3, one 96^3 and one 128^3 box of the ensemble "
+                         "(boxes 0, 24, 48) by the FP64 oracle, OpenMP"}
+    line = {
+        "metric": "Mpts*RKstage/s, uneven 64-box NS ensemble, FP32-mixed; speedup over sequential per-box",
+        "value": round(value, 1),
+        "unit": "Mpts*RKstage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * res["ensemble"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-mixed (fp32 storage/FFT; fp64 integrating factor, twiddles, sums)",
+        "data": "synthetic (seeded random-phase ICs generated on the GPU)",
+        "config": {"workload": "ensemble_24x64^3_24x96^3_16x128^3_ns_rk4", "boxes": nb,
+                   "points": int(pts), "nu": "U[5e-4,5e-3] rng 5", "dt": "per box, CFL 0.5 at t=0",
+                   "ic": "random-phase k^4 exp(-2(k/4)^2), E=0.5, seed 100+b", "precision": "fp32-mixed"},
+        "sequential": {"value": round(seqv, 1), "ms_per_step": round(1e3 * res["sequential"] / args.steps, 3),
+                       "speedup_ensemble_over_sequential": round(value / seqv, 3),
+                       "max_abs_diff_sampled_boxes": worst},
+        "roofline": {"bound": "hbm", "kernel": "whole_ensemble_step", "achieved": round(ach, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["ns512", "ensemble"], default="ns512",
+                    help="ns512: the N^3 decaying-turbulence step (default, N from --n); "
+                         "ensemble: the uneven 64-box ensemble")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -103,6 +223,9 @@ def main():
         print("bench.py: this build runs one box per GPU on a single MI355X; multi-GPU runs "
               "(--gpus > 1) are not part of it", flush=True)
         sys.exit(2)
+    if args.workload == "ensemble":
+        bench_ensemble(args)
+        return
 
     import numpy as np
     import torch

@@ -160,6 +160,23 @@ int th_flux_shell_tr(th_plan *plan, th_state *state, const double *radii, int ns
 int th_profile_passes(th_plan *plan, th_state *state, double dt, const th_phys *p, int stage, int nrep,
                       double ms[5], double bytes[5]);
 
+/* ---- ensemble of independent NS boxes of uneven sizes ----
+ * The reference runs one grid size per process (its transform and energy routines keep static
+ * scratch arrays sized for one grid, four_tr.cc:173-174, four_energy.cc:302-303), so an ensemble
+ * there is a sequence of runs, each an Ifluid_main.cc time loop.  Here the boxes (FOUR basis,
+ * kfactor 1, dealiased, box b of size N[3b..3b+2]) advance together: th_ensemble_step ==
+ * nsteps x IncFluid::Time_advance() (RK4, time_advance.cc:100-155) for every box, box b with its
+ * own dt[b] and viscosity nu[b].  Boxes of one size share each pass launch; the size classes run
+ * as concurrent branches of one captured step.  th_ensemble_plan returns the plan of a box's size
+ * (for initial conditions and spectra on reference-layout buffers). */
+typedef struct th_ensemble th_ensemble;
+int th_ensemble_create(th_ctx *ctx, th_ensemble **ens, int nbox, const int *N, int precision);
+int th_ensemble_destroy(th_ensemble *ens);
+int th_ensemble_plan(th_ensemble *ens, int box, th_plan **plan);
+int th_ensemble_upload(th_ensemble *ens, int box, int comp, const void *src, int src_on_host);
+int th_ensemble_download(th_ensemble *ens, int box, int comp, void *dst, int dst_on_host);
+int th_ensemble_step(th_ensemble *ens, const double *dt, const double *nu, int nsteps);
+
 #ifdef __cplusplus
 }
 #endif

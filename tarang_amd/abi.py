@@ -20,14 +20,15 @@ TH_NS, TH_MHD, TH_RB = 0, 1, 2
 _lib = None
 _live_plans = weakref.WeakSet()
 _live_ctx = weakref.WeakSet()
+_live_ens = weakref.WeakSet()
 _shutting_down = False
 
 
 @atexit.register
 def _release_all():
-    """Destroy plans, then contexts, before the HIP runtime is torn down at process exit."""
+    """Destroy ensembles, plans, then contexts, before the HIP runtime is torn down at process exit."""
     global _shutting_down
-    for obj in list(_live_plans) + list(_live_states if "_live_states" in globals() else []):
+    for obj in list(_live_ens) + list(_live_plans) + list(_live_states if "_live_states" in globals() else []):
         try:
             obj.close()
         except Exception:
@@ -102,6 +103,12 @@ def lib():
         "th_step": (i, [vp, vp, d, C.POINTER(Phys), i]),
         "th_flux_shell_tr": (i, [vp, vp, dp, i, dp, dp]),
         "th_profile_passes": (i, [vp, vp, d, C.POINTER(Phys), i, i, dp, dp]),
+        "th_ensemble_create": (i, [vp, C.POINTER(vp), i, C.POINTER(C.c_int), i]),
+        "th_ensemble_destroy": (i, [vp]),
+        "th_ensemble_plan": (i, [vp, i, C.POINTER(vp)]),
+        "th_ensemble_upload": (i, [vp, i, i, vp, i]),
+        "th_ensemble_download": (i, [vp, i, i, vp, i]),
+        "th_ensemble_step": (i, [vp, dp, dp, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -168,10 +175,18 @@ class Plan:
         self.h = C.c_void_p()
         _check(lib().th_plan_create(ctx.h, C.byref(self.h), self.basis, *self.N, kf, self.prec,
                                     1 if dealias else 0), "th_plan_create")
+        self._owned = True
         _live_plans.add(self)
 
+    @classmethod
+    def _borrowed(cls, ctx, handle, N, prec):
+        """A plan owned by the library (an ensemble's size class); close() leaves it alone."""
+        p = cls.__new__(cls)
+        p.ctx, p.N, p.basis, p.prec, p.h, p._owned = ctx, tuple(N), TH_FOUR, prec, handle, False
+        return p
+
     def close(self):
-        if self.h and not _shutting_down:
+        if self.h and self._owned and not _shutting_down:
             for st in list(_live_states):
                 if st.plan is self:
                     st.close()
@@ -462,3 +477,66 @@ class State:
                                       Pi.ctypes.data_as(C.POINTER(C.c_double)),
                                       T.ctypes.data_as(C.POINTER(C.c_double))), "th_flux_shell_tr")
         return Pi, T
+
+
+class Ensemble:
+    """Independent NS boxes of uneven sizes stepped together (th_ensemble)."""
+
+    def __init__(self, ctx, shapes, precision="fp32"):
+        self.ctx = ctx
+        self.shapes = [tuple(int(n) for n in N) for N in shapes]
+        self.prec = TH_FP64 if precision == "fp64" else TH_FP32_MIXED
+        flat = (C.c_int * (3 * len(self.shapes)))(*[n for N in self.shapes for n in N])
+        self.h = C.c_void_p()
+        _check(lib().th_ensemble_create(ctx.h, C.byref(self.h), len(self.shapes), flat, self.prec),
+               "th_ensemble_create")
+        self._plans = {}
+        _live_ens.add(self)
+
+    def close(self):
+        if self.h and not _shutting_down:
+            lib().th_ensemble_destroy(self.h)
+            self.h = C.c_void_p()
+            self._plans = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self, box):
+        """The library-owned plan of box `box`'s size (ICs, spectra on reference-layout arrays)."""
+        N = self.shapes[box]
+        if N not in self._plans:
+            h = C.c_void_p()
+            _check(lib().th_ensemble_plan(self.h, box, C.byref(h)), "th_ensemble_plan")
+            self._plans[N] = Plan._borrowed(self.ctx, h, N, self.prec)
+        return self._plans[N]
+
+    def upload(self, box, fields):
+        p = self.plan(box)
+        p._pre()
+        for c, f in enumerate(fields):
+            _check(lib().th_ensemble_upload(self.h, box, c, _ptr(f), 0), "th_ensemble_upload")
+        p._post(0, "")
+
+    def download(self, box):
+        p = self.plan(box)
+        out = p.empty(3)
+        p._pre()
+        for c, f in enumerate(out):
+            _check(lib().th_ensemble_download(self.h, box, c, _ptr(f), 0), "th_ensemble_download")
+        p._post(0, "")
+        return out
+
+    def step(self, dt, nu, steps=1, sync=True):
+        dt_, dtp = _dbl(dt)
+        nu_, nup = _dbl(nu)
+        if len(dt_) != len(self.shapes) or len(nu_) != len(self.shapes):
+            raise ValueError("dt and nu need one value per box")
+        p = self.plan(0)
+        p._pre()
+        _check(lib().th_ensemble_step(self.h, dtp, nup, steps), "th_ensemble_step")
+        if sync:
+            _check(lib().th_sync(p.h), "th_sync")

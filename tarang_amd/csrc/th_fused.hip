@@ -16,6 +16,7 @@
 // (four_ET.cc) in the pass-5 epilogue instead of writing N.
 #include <math.h>
 #include <string.h>
+#include <array>
 #include <vector>
 #include "../../include/tarang_hip.h"
 #include "th_fft.hpp"
@@ -41,6 +42,8 @@ struct FG {
     long cS;     // component stride of the state: nkx * sx
     long cA;     // component stride of A/D: N1 * sx
     long cB;     // component stride of B/C: N1 * sB
+    long bS, bA, bB, bC;   // box strides of the state, A/D, B and C buffers (batched boxes of one size)
+    int nbox;              // boxes of this size advanced by one launch per pass
     double f1, f2, f3;
     double norm; // 1 / (N1 N2 N3)
 };
@@ -86,6 +89,8 @@ __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restric
     const bool ok = cz < g.nkz;
     const long m = (long)cy * g.KZP + cz;
     const int oc = blockIdx.y;
+    S += blockIdx.z * g.bS;
+    A += blockIdx.z * g.bA;
     double Ky = 0, Kz = 0;
     if (MODE == 2) {
         Ky = ky_of(ll(g.my, cy), g.N2) * g.f2;
@@ -122,14 +127,16 @@ __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restric
 // pass 2: y-inverse, A[x][cy][cz] -> B[x][y][cz]
 template <typename R, int L, int P, int T>
 __global__ __launch_bounds__(T * P) void k_fy_inv(FG g, const cplx<R>* __restrict__ A, cplx<R>* __restrict__ B,
-                                                  const cplx<R>* __restrict__ tw)
+                                                  const cplx<R>* __restrict__ tw, int ncomp)
 {
     using F = LineFFT<R, L, P, +1>;
     __shared__ cplx<R> lds[L * T];
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int cz = blockIdx.x * T + c;
     const bool ok = cz < g.nkz;
-    const int x = blockIdx.y, comp = blockIdx.z;
+    const int x = blockIdx.y, comp = blockIdx.z % ncomp, box = blockIdx.z / ncomp;
+    A += box * g.bA;
+    B += box * g.bB;
     const cplx<R>* a = A + comp * g.cA + (long)x * g.sx + cz;
     cplx<R> v[F::E];
 #pragma unroll
@@ -179,6 +186,8 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
     const long q = (long)blockIdx.x * TZ + c;
     const bool ok = q < (long)g.N1 * g.N2;
     const long off = q * g.KZP;
+    B += blockIdx.y * g.bB;
+    Cout += blockIdx.y * g.bC;
     cplx<R>* Z = lds + c * LZ;
     cplx<R> u[ZP::NIN][E];
 #pragma unroll
@@ -232,14 +241,16 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
 // pass 4: y-forward, C[x][y][cz] -> D[x][cy][cz] (kept ky only)
 template <typename R, int L, int P, int T>
 __global__ __launch_bounds__(T * P) void k_fy_fwd(FG g, const cplx<R>* __restrict__ Cin, cplx<R>* __restrict__ D,
-                                                  const cplx<R>* __restrict__ tw)
+                                                  const cplx<R>* __restrict__ tw, int ncomp)
 {
     using F = LineFFT<R, L, P, -1>;
     __shared__ cplx<R> lds[L * T];
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int cz = blockIdx.x * T + c;
     const bool ok = cz < g.nkz;
-    const int x = blockIdx.y, comp = blockIdx.z;
+    const int x = blockIdx.y, comp = blockIdx.z % ncomp, box = blockIdx.z / ncomp;
+    Cin += box * g.bC;
+    D += box * g.bA;
     const cplx<R>* a = Cin + comp * g.cB + (long)x * g.sB + cz;
     cplx<R> v[F::E];
 #pragma unroll
@@ -305,7 +316,8 @@ __device__ __forceinline__ void rk_update(int stage, double dt, double eh, doubl
 
 template <typename R, int L, int P, int T, int MODE>
 __global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restrict__ D, cplx<R>* S, cplx<R>* Vs,
-                                                  cplx<R>* acc, const cplx<R>* __restrict__ tw, StageCoef sc)
+                                                  cplx<R>* acc, const cplx<R>* __restrict__ tw,
+                                                  const StageCoef* __restrict__ coefs)
 {
     using F = LineFFT<R, L, P, -1>;
     constexpr int E = F::E;
@@ -318,6 +330,12 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restric
     const int cz = (blockIdx.x % ntz) * T + c;
     const bool ok = cz < g.nkz;
     const long m = (long)cy * g.KZP + cz;
+    const int box = blockIdx.y;
+    const StageCoef sc = coefs[box];
+    D += box * g.bA;
+    S += box * g.bS;
+    Vs += box * g.bS;
+    acc += box * g.bS;
     const R Ky = (R)(ky_of(ll(g.my, cy), g.N2) * g.f2);
     const R Kz = (R)(cz * g.f3);
     cplx<R> Nacc[NA][E];
@@ -539,7 +557,7 @@ static int launch_xinv(const FG& g, const cplx<R>* S, cplx<R>* A, int ncomp, con
 {
     constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
     const int ntz = (g.nkz + T - 1) / T;
-    hipLaunchKernelGGL((k_fx_inv<R, L, P, T, MODE>), dim3(g.my.nk * ntz, ncomp), dim3(T * P), 0, st, g, S, A, tw, sel);
+    hipLaunchKernelGGL((k_fx_inv<R, L, P, T, MODE>), dim3(g.my.nk * ntz, ncomp, g.nbox), dim3(T * P), 0, st, g, S, A, tw, sel);
     TH_CHECK_HIP(hipGetLastError());
     return OK;
 }
@@ -547,7 +565,8 @@ template <typename R, int L>
 static int launch_yinv(const FG& g, const cplx<R>* A, cplx<R>* B, int ncomp, const cplx<R>* tw, hipStream_t st)
 {
     constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
-    hipLaunchKernelGGL((k_fy_inv<R, L, P, T>), dim3((g.nkz + T - 1) / T, g.N1, ncomp), dim3(T * P), 0, st, g, A, B, tw);
+    hipLaunchKernelGGL((k_fy_inv<R, L, P, T>), dim3((g.nkz + T - 1) / T, g.N1, ncomp * g.nbox), dim3(T * P), 0, st, g, A, B,
+                       tw, ncomp);
     TH_CHECK_HIP(hipGetLastError());
     return OK;
 }
@@ -557,7 +576,8 @@ static int launch_z(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* t
     constexpr int E = pts_per_thread(LZ), P = LZ / E;
     constexpr int TZ = (256 / P) < 1 ? 1 : 256 / P;
     const long lines = (long)g.N1 * g.N2;
-    hipLaunchKernelGGL((k_fz<R, LZ, P, TZ, MODE>), dim3((unsigned)((lines + TZ - 1) / TZ)), dim3(TZ * P), 0, st, g, B, Cc, twL, twN);
+    hipLaunchKernelGGL((k_fz<R, LZ, P, TZ, MODE>), dim3((unsigned)((lines + TZ - 1) / TZ), g.nbox), dim3(TZ * P), 0, st, g, B, Cc,
+                       twL, twN);
     TH_CHECK_HIP(hipGetLastError());
     return OK;
 }
@@ -565,7 +585,8 @@ template <typename R, int L>
 static int launch_yfwd(const FG& g, const cplx<R>* Cc, cplx<R>* D, int ncomp, const cplx<R>* tw, hipStream_t st)
 {
     constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
-    hipLaunchKernelGGL((k_fy_fwd<R, L, P, T>), dim3((g.nkz + T - 1) / T, g.N1, ncomp), dim3(T * P), 0, st, g, Cc, D, tw);
+    hipLaunchKernelGGL((k_fy_fwd<R, L, P, T>), dim3((g.nkz + T - 1) / T, g.N1, ncomp * g.nbox), dim3(T * P), 0, st, g, Cc, D,
+                       tw, ncomp);
     TH_CHECK_HIP(hipGetLastError());
     return OK;
 }
@@ -581,11 +602,12 @@ template <typename R, int L, int MODE> constexpr int xfwd_T()
 
 template <typename R, int L, int MODE>
 static int launch_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
-                       const StageCoef& sc, hipStream_t st)
+                       const StageCoef* sc, hipStream_t st)
 {
     constexpr int E = xfwd_E(L), P = L / E, T = xfwd_T<R, L, MODE>();
     const int ntz = (g.nkz + T - 1) / T;
-    hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE>), dim3(g.my.nk * ntz), dim3(T * P), 0, st, g, D, S, Vs, acc, tw, sc);
+    hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D, S, Vs, acc, tw,
+                       sc);
     TH_CHECK_HIP(hipGetLastError());
     return OK;
 }
@@ -633,7 +655,7 @@ static int d_yfwd(const FG& g, const cplx<R>* Cc, cplx<R>* D, int nc, const cplx
 }
 template <typename R, int MODE>
 static int d_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
-                  const StageCoef& sc, hipStream_t st)
+                  const StageCoef* sc, hipStream_t st)
 {
 #define C_(n) case n: return launch_xfwd<R, n, MODE>(g, D, S, Vs, acc, tw, sc, st);
     switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
@@ -686,6 +708,7 @@ FG make_fg(const Geo& geo, int prec)
     g.f2 = geo.kf[1];
     g.f3 = geo.kf[2];
     g.norm = 1.0 / ((double)geo.N1 * (double)geo.N2 * (double)geo.N3);
+    g.nbox = 1;
     return g;
 }
 
@@ -698,6 +721,7 @@ struct th_state {
     int physics = TH_NS;
     int ncomp = 3;
     int nin = 3, nout = 6;       // pipeline component counts
+    int nbox = 1;                // boxes of the plan's size held box-major in every buffer
     FG fg;
     size_t esz = 16;
     void* S = nullptr;           // state (compact), ncomp comps
@@ -706,21 +730,27 @@ struct th_state {
     void* X1 = nullptr;          // A / D layout, max(nin, nout) comps
     void* X2 = nullptr;          // B layout, nin comps (ET: 6)
     void* X3 = nullptr;          // C layout, nout comps (ET: 9)
-    void* ET_U = nullptr;        // ET helper copy (B layout, 3 comps) -- unused when X2 is shared
     double* partial = nullptr;
     size_t partial_cap = 0;
     double* radii = nullptr;
     int radii_cap = 0;
+    StageCoef* dcoef = nullptr;  // device [4 stages][nbox]; the captured step reads it
+    StageCoef* hcoef = nullptr;  // pinned staging for dcoef
+    std::vector<StageCoef> cur;  // what dcoef holds once the last upload has run
+    hipEvent_t cev = nullptr;    // recorded after the last upload from hcoef
+    bool cev_live = false;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
-    double g_dt = -1, g_nu = -1, g_eta = -1;
     ~th_state()
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (void* q : {S, Vs, acc, X1, X2, X3, ET_U}) if (q) (void)hipFree(q);
+        for (void* q : {S, Vs, acc, X1, X2, X3}) if (q) (void)hipFree(q);
         if (partial) (void)hipFree(partial);
         if (radii) (void)hipFree(radii);
+        if (dcoef) (void)hipFree(dcoef);
+        if (hcoef) (void)hipHostFree(hcoef);
+        if (cev) (void)hipEventDestroy(cev);
     }
 };
 
@@ -730,9 +760,12 @@ struct th_state {
         if (r_) return r_; \
     } while (0)
 
-extern "C" int th_state_create(th_plan* p, th_state** out, int physics)
+// Fused state for nbox boxes of the plan's size. Single states (nbox == 1) also keep room for the
+// energy-transfer pipeline (6 B-layout and 9 C-layout comps); batched states hold only what the
+// RK step needs.
+static int state_create(th_plan* p, th_state** out, int physics, int nbox)
 {
-    if (!p || !out) { set_error("th_state_create: null"); return E_ARG; }
+    if (!p || !out || nbox < 1) { set_error("th_state_create: bad args"); return E_ARG; }
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
     if (p->g.basis != TH_FOUR) { set_error("th_state_create: the fused path covers the FOUR basis; use th_step_plain for SCFT"); return E_UNSUPPORTED; }
     if (physics != TH_NS && physics != TH_MHD) { set_error("th_state_create: physics %d not fused (use th_step_plain)", physics); return E_UNSUPPORTED; }
@@ -744,28 +777,43 @@ extern "C" int th_state_create(th_plan* p, th_state** out, int physics)
     if (!s) return E_NOMEM;
     s->plan = p;
     s->physics = physics;
+    s->nbox = nbox;
     s->ncomp = physics == TH_NS ? 3 : 6;
     s->nin = physics == TH_NS ? 3 : 6;
     s->nout = physics == TH_NS ? 6 : 9;
     s->fg = make_fg(p->g, p->prec);
     s->esz = p->prec ? 8 : 16;
-    const FG& g = s->fg;
+    FG& g = s->fg;
+    const int nA = nbox == 1 ? 9 : s->nout, nBin = nbox == 1 ? 6 : s->nin, nC = nbox == 1 ? 9 : s->nout;
+    g.nbox = nbox;
+    g.bS = s->ncomp * g.cS;
+    g.bA = nA * g.cA;
+    g.bB = nBin * g.cB;
+    g.bC = nC * g.cB;
     auto alloc = [&](void** q, size_t elems) -> int {
         if (hipMalloc(q, elems * s->esz) != hipSuccess) { set_error("th_state_create: out of device memory"); return E_NOMEM; }
         if (hipMemset(*q, 0, elems * s->esz) != hipSuccess) return E_HIP;
         return OK;
     };
-    int nA = s->nout > 6 ? s->nout : 9;   // ET needs 9 product comps
-    int nBin = 6, nC = 9;                 // room for MHD / ET
     int r = OK;
-    if ((r = alloc(&s->S, s->ncomp * g.cS)) || (r = alloc(&s->Vs, s->ncomp * g.cS)) || (r = alloc(&s->acc, s->ncomp * g.cS)) ||
-        (r = alloc(&s->X1, nA * g.cA)) || (r = alloc(&s->X2, nBin * g.cB)) || (r = alloc(&s->X3, nC * g.cB))) {
+    const size_t nb = (size_t)nbox;
+    if ((r = alloc(&s->S, nb * g.bS)) || (r = alloc(&s->Vs, nb * g.bS)) || (r = alloc(&s->acc, nb * g.bS)) ||
+        (r = alloc(&s->X1, nb * g.bA)) || (r = alloc(&s->X2, nb * g.bB)) || (r = alloc(&s->X3, nb * g.bC))) {
         delete s;
         return r;
+    }
+    const size_t cb = 4 * nb * sizeof(StageCoef);
+    if (hipMalloc((void**)&s->dcoef, cb) != hipSuccess || hipHostMalloc((void**)&s->hcoef, cb, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&s->cev, hipEventDisableTiming) != hipSuccess) {
+        set_error("th_state_create: coefficient buffers");
+        delete s;
+        return E_NOMEM;
     }
     *out = s;
     return OK;
 }
+
+extern "C" int th_state_create(th_plan* p, th_state** out, int physics) { return state_create(p, out, physics, 1); }
 
 extern "C" int th_state_destroy(th_state* s)
 {
@@ -783,12 +831,11 @@ static dim3 g1d(size_t n)
     return dim3((unsigned)(b ? b : 1));
 }
 
-extern "C" int th_state_upload(th_state* s, int comp, const void* src, int src_on_host)
+static int state_upload(th_state* s, int box, int comp, const void* src, int src_on_host, hipStream_t st)
 {
-    if (!s || !src || comp < 0 || comp >= s->ncomp) { set_error("th_state_upload: bad args"); return E_ARG; }
+    if (!s || !src || comp < 0 || comp >= s->ncomp || box < 0 || box >= s->nbox) { set_error("th_state_upload: bad args"); return E_ARG; }
     th_plan* p = s->plan;
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
-    hipStream_t st = p->ctx->stream;
     const void* dsrc = src;
     void* tmp = nullptr;
     size_t bytes = p->g.ncplx() * s->esz;
@@ -798,13 +845,12 @@ extern "C" int th_state_upload(th_state* s, int comp, const void* src, int src_o
         dsrc = tmp;
     }
     const FG& g = s->fg;
-    size_t n = (size_t)g.mx.nk * g.sx;
+    const size_t n = (size_t)g.mx.nk * g.sx;
+    const long off = box * g.bS + comp * g.cS;
     if (p->prec)
-        hipLaunchKernelGGL((k_gather<float>), g1d(n), dim3(256), 0, st, g, (const cplx<float>*)dsrc,
-                           (cplx<float>*)s->S + comp * g.cS);
+        hipLaunchKernelGGL((k_gather<float>), g1d(n), dim3(256), 0, st, g, (const cplx<float>*)dsrc, (cplx<float>*)s->S + off);
     else
-        hipLaunchKernelGGL((k_gather<double>), g1d(n), dim3(256), 0, st, g, (const cplx<double>*)dsrc,
-                           (cplx<double>*)s->S + comp * g.cS);
+        hipLaunchKernelGGL((k_gather<double>), g1d(n), dim3(256), 0, st, g, (const cplx<double>*)dsrc, (cplx<double>*)s->S + off);
     hipError_t e = hipGetLastError();
     if (tmp) {
         (void)hipStreamSynchronize(st);
@@ -814,12 +860,11 @@ extern "C" int th_state_upload(th_state* s, int comp, const void* src, int src_o
     return OK;
 }
 
-extern "C" int th_state_download(th_state* s, int comp, void* dst, int dst_on_host)
+static int state_download(th_state* s, int box, int comp, void* dst, int dst_on_host, hipStream_t st)
 {
-    if (!s || !dst || comp < 0 || comp >= s->ncomp) { set_error("th_state_download: bad args"); return E_ARG; }
+    if (!s || !dst || comp < 0 || comp >= s->ncomp || box < 0 || box >= s->nbox) { set_error("th_state_download: bad args"); return E_ARG; }
     th_plan* p = s->plan;
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
-    hipStream_t st = p->ctx->stream;
     size_t bytes = p->g.ncplx() * s->esz;
     void* ddst = dst;
     void* tmp = nullptr;
@@ -828,24 +873,80 @@ extern "C" int th_state_download(th_state* s, int comp, void* dst, int dst_on_ho
         ddst = tmp;
     }
     const FG& g = s->fg;
-    size_t n = p->g.ncplx();
+    const size_t n = p->g.ncplx();
+    const long off = box * g.bS + comp * g.cS;
     if (p->prec)
-        hipLaunchKernelGGL((k_scatter<float>), g1d(n), dim3(256), 0, st, g, (const cplx<float>*)s->S + comp * g.cS, (cplx<float>*)ddst);
+        hipLaunchKernelGGL((k_scatter<float>), g1d(n), dim3(256), 0, st, g, (const cplx<float>*)s->S + off, (cplx<float>*)ddst);
     else
-        hipLaunchKernelGGL((k_scatter<double>), g1d(n), dim3(256), 0, st, g, (const cplx<double>*)s->S + comp * g.cS, (cplx<double>*)ddst);
-    TH_CHECK_HIP(hipGetLastError());
-    if (tmp) {
-        TH_CHECK_HIP(hipMemcpyAsync(dst, tmp, bytes, hipMemcpyDeviceToHost, st));
-        TH_CHECK_HIP(hipStreamSynchronize(st));
-        (void)hipFree(tmp);
+        hipLaunchKernelGGL((k_scatter<double>), g1d(n), dim3(256), 0, st, g, (const cplx<double>*)s->S + off, (cplx<double>*)ddst);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && tmp) {
+        e = hipMemcpyAsync(dst, tmp, bytes, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
     }
+    if (tmp) (void)hipFree(tmp);
+    if (e != hipSuccess) { set_error("th_state_download: %s", hipGetErrorString(e)); return E_HIP; }
     return OK;
 }
 
-template <typename R> static int enqueue_step(th_state* s, double dt, double nu, double eta)
+extern "C" int th_state_upload(th_state* s, int comp, const void* src, int src_on_host)
+{
+    if (!s) { set_error("th_state_upload: null"); return E_ARG; }
+    return state_upload(s, 0, comp, src, src_on_host, s->plan->ctx->stream);
+}
+
+extern "C" int th_state_download(th_state* s, int comp, void* dst, int dst_on_host)
+{
+    if (!s) { set_error("th_state_download: null"); return E_ARG; }
+    return state_download(s, 0, comp, dst, dst_on_host, s->plan->ctx->stream);
+}
+
+static StageCoef stage_coef(int stage, double dt, double nu, double eta)
+{
+    StageCoef c;
+    memset(&c, 0, sizeof(c));
+    c.stage = stage;
+    c.dt = dt;
+    c.fh[0] = (-nu * dt) * 0.5;
+    c.f1[0] = (-nu * dt) * 1.0;
+    c.fh[1] = (-eta * dt) * 0.5;
+    c.f1[1] = (-eta * dt) * 1.0;
+    return c;
+}
+
+static bool same_coef(const StageCoef& a, const StageCoef& b)
+{
+    return a.stage == b.stage && a.dt == b.dt && a.fh[0] == b.fh[0] && a.fh[1] == b.fh[1] && a.f1[0] == b.f1[0] &&
+           a.f1[1] == b.f1[1];
+}
+
+// Per-box stage coefficients (dt, -nu dt / 2, -nu dt, ...) live in device memory, laid out
+// [stage][box], so a captured step serves any dt and viscosity. A new set is staged through
+// pinned memory and copied on stream st ahead of the step that uses it.
+static int set_coefs(th_state* s, const double* dt, const double* nu, const double* eta, hipStream_t st)
+{
+    const int nb = s->nbox;
+    std::vector<StageCoef> c((size_t)4 * nb);
+    for (int k = 0; k < 4; k++)
+        for (int b = 0; b < nb; b++) c[(size_t)k * nb + b] = stage_coef(k + 1, dt[b], nu[b], eta ? eta[b] : 0.0);
+    if (s->cur.size() == c.size()) {
+        bool same = true;
+        for (size_t i = 0; i < c.size() && same; i++) same = same_coef(c[i], s->cur[i]);
+        if (same) return OK;
+    }
+    if (s->cev_live) TH_CHECK_HIP(hipEventSynchronize(s->cev));   // the previous copy has read hcoef
+    memcpy(s->hcoef, c.data(), c.size() * sizeof(StageCoef));
+    TH_CHECK_HIP(hipMemcpyAsync(s->dcoef, s->hcoef, c.size() * sizeof(StageCoef), hipMemcpyHostToDevice, st));
+    TH_CHECK_HIP(hipEventRecord(s->cev, st));
+    s->cev_live = true;
+    s->cur.swap(c);
+    return OK;
+}
+
+// One RK4 step of every box in s, as 4 stages x 5 pass launches on stream st.
+template <typename R> static int enqueue_step(th_state* s, hipStream_t st)
 {
     th_plan* p = s->plan;
-    hipStream_t st = p->ctx->stream;
     const FG& g = s->fg;
     const cplx<R>* twx = p->twd<R>(g.N1);
     const cplx<R>* twy = p->twd<R>(g.N2);
@@ -859,13 +960,7 @@ template <typename R> static int enqueue_step(th_state* s, double dt, double nu,
     cplx<R>* X3 = (cplx<R>*)s->X3;
     ShellSel sel{0, 0};
     for (int stage = 1; stage <= 4; stage++) {
-        StageCoef sc;
-        sc.stage = stage;
-        sc.dt = dt;
-        sc.fh[0] = (-nu * dt) * 0.5;
-        sc.f1[0] = (-nu * dt) * 1.0;
-        sc.fh[1] = (-eta * dt) * 0.5;
-        sc.f1[1] = (-eta * dt) * 1.0;
+        const StageCoef* sc = s->dcoef + (size_t)(stage - 1) * s->nbox;
         const cplx<R>* src = stage == 1 ? S : Vs;
         if (s->physics == TH_NS) {
             TRY((d_xinv<R, 0>(g, src, X1, 3, twx, sel, st)));
@@ -884,30 +979,30 @@ template <typename R> static int enqueue_step(th_state* s, double dt, double nu,
     return OK;
 }
 
+static int ensure_twiddles(th_plan* p, const FG& g, hipStream_t st)
+{
+    for (int L : {g.N1, g.N2, g.LZ, g.N3})
+        if (!p->tw.get(L, p->prec, st)) { set_error("twiddle table allocation failed"); return E_NOMEM; }
+    return OK;
+}
+
 extern "C" int th_step(th_plan* p, th_state* s, double dt, const th_phys* ph, int nsteps)
 {
     if (!p || !s || !ph || s->plan != p || nsteps < 0) { set_error("th_step: bad args"); return E_ARG; }
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
     hipStream_t st = p->ctx->stream;
-    double nu = ph->nu, eta = ph->eta;
     // twiddle tables must exist before graph capture (their creation allocates)
-    const FG& g = s->fg;
-    for (int L : {g.N1, g.N2, g.LZ, g.N3})
-        if (!p->tw.get(L, p->prec, st)) { set_error("twiddle table allocation failed"); return E_NOMEM; }
-    if (!s->gexec || s->g_dt != dt || s->g_nu != nu || s->g_eta != eta) {
-        if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
-        if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    TRY(ensure_twiddles(p, s->fg, st));
+    TRY(set_coefs(s, &dt, &ph->nu, &ph->eta, st));
+    if (!s->gexec) {
         TH_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int r = p->prec ? enqueue_step<float>(s, dt, nu, eta) : enqueue_step<double>(s, dt, nu, eta);
+        int r = p->prec ? enqueue_step<float>(s, st) : enqueue_step<double>(s, st);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (r) { if (graph) (void)hipGraphDestroy(graph); return r; }
         if (e != hipSuccess) { set_error("hipStreamEndCapture: %s", hipGetErrorString(e)); return E_HIP; }
         s->graph = graph;
         TH_CHECK_HIP(hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0));
-        s->g_dt = dt;
-        s->g_nu = nu;
-        s->g_eta = eta;
     }
     for (int i = 0; i < nsteps; i++) TH_CHECK_HIP(hipGraphLaunch(s->gexec, st));
     return OK;
@@ -1030,13 +1125,8 @@ template <typename R> static int profile_impl(th_state* s, double dt, double nu,
     cplx<R>* X1 = (cplx<R>*)s->X1;
     cplx<R>* X2 = (cplx<R>*)s->X2;
     cplx<R>* X3 = (cplx<R>*)s->X3;
-    StageCoef sc;
-    sc.stage = stage;
-    sc.dt = dt;
-    sc.fh[0] = (-nu * dt) * 0.5;
-    sc.f1[0] = (-nu * dt) * 1.0;
-    sc.fh[1] = (-eta * dt) * 0.5;
-    sc.f1[1] = (-eta * dt) * 1.0;
+    TRY(set_coefs(s, &dt, &nu, &eta, st));
+    const StageCoef* sc = s->dcoef + (stage - 1);
     ShellSel sel{0, 0};
     const bool mhd = s->physics == TH_MHD;
     const int nin = mhd ? 6 : 3, nout = mhd ? 9 : 6, nst = mhd ? 6 : 3;
@@ -1090,4 +1180,166 @@ extern "C" int th_profile_passes(th_plan* p, th_state* s, double dt, const th_ph
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
     return p->prec ? profile_impl<float>(s, dt, ph->nu, ph->eta, stage, nrep, ms, bytes)
                    : profile_impl<double>(s, dt, ph->nu, ph->eta, stage, nrep, ms, bytes);
+}
+
+// ------------------------------------------------------------------------------------------
+// Ensemble of independent NS boxes of uneven sizes. Boxes of one size form a class held in one
+// batched state (box-major buffers, one launch per pass covers every box of the class); the
+// classes run as concurrent branches of one captured RK4 step. Per-box dt and viscosity come from
+// the classes' device coefficient tables, so the captured step is reused for any dt / nu.
+// ------------------------------------------------------------------------------------------
+struct th_ensemble {
+    th_ctx* ctx = nullptr;
+    int nbox = 0, prec = 0;
+    struct Cls {
+        th_plan* plan = nullptr;
+        th_state* st = nullptr;
+        std::vector<int> boxes;       // ensemble box ids, in slot order
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    std::vector<Cls> cls;
+    std::vector<int> cls_of, slot_of;
+    hipEvent_t fork = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<double> dtc, nuc;      // per-class scratch for coefficient upload
+    ~th_ensemble()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (Cls& c : cls) {
+            if (c.st) (void)th_state_destroy(c.st);
+            if (c.plan) (void)th_plan_destroy(c.plan);
+            if (c.done) (void)hipEventDestroy(c.done);
+            if (c.stream) (void)hipStreamDestroy(c.stream);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+    }
+};
+
+extern "C" int th_ensemble_create(th_ctx* ctx, th_ensemble** out, int nbox, const int* N, int precision)
+{
+    if (!ctx || !out || nbox < 1 || !N || (precision != TH_FP64 && precision != TH_FP32_MIXED)) {
+        set_error("th_ensemble_create: bad args");
+        return E_ARG;
+    }
+    TH_CHECK_HIP(hipSetDevice(ctx->device));
+    th_ensemble* e = new (std::nothrow) th_ensemble();
+    if (!e) return E_NOMEM;
+    e->ctx = ctx;
+    e->nbox = nbox;
+    e->prec = precision;
+    e->cls_of.assign(nbox, -1);
+    e->slot_of.assign(nbox, -1);
+    std::vector<std::array<int, 3>> shapes;
+    for (int b = 0; b < nbox; b++) {
+        std::array<int, 3> n{N[3 * b], N[3 * b + 1], N[3 * b + 2]};
+        int c = -1;
+        for (size_t i = 0; i < shapes.size(); i++)
+            if (shapes[i] == n) c = (int)i;
+        if (c < 0) {
+            c = (int)shapes.size();
+            shapes.push_back(n);
+            e->cls.emplace_back();
+        }
+        e->cls_of[b] = c;
+        e->slot_of[b] = (int)e->cls[c].boxes.size();
+        e->cls[c].boxes.push_back(b);
+    }
+    const double kf[3] = {1.0, 1.0, 1.0};
+    for (size_t c = 0; c < e->cls.size(); c++) {
+        th_ensemble::Cls& k = e->cls[c];
+        int r = th_plan_create(ctx, &k.plan, TH_FOUR, shapes[c][0], shapes[c][1], shapes[c][2], kf, precision, 1);
+        if (!r) r = state_create(k.plan, &k.st, TH_NS, (int)k.boxes.size());
+        if (!r && (hipStreamCreateWithFlags(&k.stream, hipStreamNonBlocking) != hipSuccess ||
+                   hipEventCreateWithFlags(&k.done, hipEventDisableTiming) != hipSuccess)) {
+            set_error("th_ensemble_create: stream / event creation failed");
+            r = E_HIP;
+        }
+        if (r) {
+            delete e;
+            return r;
+        }
+    }
+    if (hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess) {
+        delete e;
+        set_error("th_ensemble_create: event creation failed");
+        return E_HIP;
+    }
+    *out = e;
+    return OK;
+}
+
+extern "C" int th_ensemble_destroy(th_ensemble* e)
+{
+    if (!e) return OK;
+    (void)hipSetDevice(e->ctx->device);
+    (void)hipStreamSynchronize(e->ctx->stream);
+    delete e;
+    return OK;
+}
+
+extern "C" int th_ensemble_plan(th_ensemble* e, int box, th_plan** plan)
+{
+    if (!e || !plan || box < 0 || box >= e->nbox) { set_error("th_ensemble_plan: bad args"); return E_ARG; }
+    *plan = e->cls[e->cls_of[box]].plan;
+    return OK;
+}
+
+extern "C" int th_ensemble_upload(th_ensemble* e, int box, int comp, const void* src, int src_on_host)
+{
+    if (!e || box < 0 || box >= e->nbox) { set_error("th_ensemble_upload: bad args"); return E_ARG; }
+    return state_upload(e->cls[e->cls_of[box]].st, e->slot_of[box], comp, src, src_on_host, e->ctx->stream);
+}
+
+extern "C" int th_ensemble_download(th_ensemble* e, int box, int comp, void* dst, int dst_on_host)
+{
+    if (!e || box < 0 || box >= e->nbox) { set_error("th_ensemble_download: bad args"); return E_ARG; }
+    return state_download(e->cls[e->cls_of[box]].st, e->slot_of[box], comp, dst, dst_on_host, e->ctx->stream);
+}
+
+extern "C" int th_ensemble_step(th_ensemble* e, const double* dt, const double* nu, int nsteps)
+{
+    if (!e || !dt || !nu || nsteps < 0) { set_error("th_ensemble_step: bad args"); return E_ARG; }
+    for (int b = 0; b < e->nbox; b++)
+        if (!(dt[b] > 0.0) || !(nu[b] >= 0.0)) { set_error("th_ensemble_step: box %d has dt %g nu %g", b, dt[b], nu[b]); return E_ARG; }
+    TH_CHECK_HIP(hipSetDevice(e->ctx->device));
+    hipStream_t st = e->ctx->stream;
+    for (th_ensemble::Cls& k : e->cls) {
+        TRY(ensure_twiddles(k.plan, k.st->fg, st));
+        e->dtc.clear();
+        e->nuc.clear();
+        for (int b : k.boxes) {
+            e->dtc.push_back(dt[b]);
+            e->nuc.push_back(nu[b]);
+        }
+        TRY(set_coefs(k.st, e->dtc.data(), e->nuc.data(), nullptr, st));
+    }
+    if (!e->gexec) {
+        TH_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int r = OK;
+        hipError_t he = hipEventRecord(e->fork, st);
+        for (size_t c = 0; c < e->cls.size() && !r && he == hipSuccess; c++) {
+            th_ensemble::Cls& k = e->cls[c];
+            he = hipStreamWaitEvent(k.stream, e->fork, 0);
+            if (he != hipSuccess) break;
+            r = e->prec ? enqueue_step<float>(k.st, k.stream) : enqueue_step<double>(k.st, k.stream);
+            if (r) break;
+            he = hipEventRecord(k.done, k.stream);
+            if (he == hipSuccess) he = hipStreamWaitEvent(st, k.done, 0);
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t ee = hipStreamEndCapture(st, &graph);
+        if (r || he != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            if (r) return r;
+            set_error("th_ensemble_step capture: %s", hipGetErrorString(he != hipSuccess ? he : ee));
+            return E_HIP;
+        }
+        e->graph = graph;
+        TH_CHECK_HIP(hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < nsteps; i++) TH_CHECK_HIP(hipGraphLaunch(e->gexec, st));
+    return OK;
 }
