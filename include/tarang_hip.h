@@ -124,6 +124,12 @@ int th_shell_mult_single(th_plan *plan, void *const A[3], void *const B[3], doub
 int th_shell_mult_all(th_plan *plan, void *const A[3], void *const B[3], const double *radii, int nr, double *out);
 int th_flux_ns(th_plan *plan, void *const V[3], const double *radii, int nsph, double *flux);
 int th_shell_tr_ns(th_plan *plan, void *const V[3], const double *radii, int nsh, double *T);
+/* th_flux_mhd == IncVF::Compute_flux(IncVF& W) compute_flux.cc:148-262 for spheres radii[1..nsph];
+ * out is 9 x (nsph+1), row-major, rows: 0 U<->U> (flux_self), 1 B<->B> (W.flux_self),
+ * 2 U<->B> (flux_VF_in_out), 3 U<->B< (flux_VF_in_in), 4 B<->U> (W.flux_VF_in_out),
+ * 5 B<->U< (W.flux_VF_in_in), 6 U>->B> (flux_VF_out_out), 7 Zp<->Zp> (flux_Elsasser),
+ * 8 Zm<->Zm> (W.flux_Elsasser); column 0 is 0. */
+int th_flux_mhd(th_plan *plan, void *const U[3], void *const B[3], const double *radii, int nsph, double *out);
 int th_nusselt(th_plan *plan, void *const V[3], const void *T, double Ra, double Pr, double *Nu);
 
 /* ---- initial conditions on reference-layout device fields ----

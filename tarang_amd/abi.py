@@ -92,6 +92,7 @@ def lib():
         "th_shell_mult_all": (i, [vp, pp, pp, dp, i, dp]),
         "th_flux_ns": (i, [vp, pp, dp, i, dp]),
         "th_shell_tr_ns": (i, [vp, pp, dp, i, dp]),
+        "th_flux_mhd": (i, [vp, pp, pp, dp, i, dp]),
         "th_nusselt": (i, [vp, pp, vp, d, d, dp]),
         "th_ic_taylor_green": (i, [vp, pp, i, d]),
         "th_ic_random": (i, [vp, pp, C.c_ulonglong, d, d]),
@@ -374,6 +375,16 @@ class Plan:
         _check(lib().th_flux_ns(self.h, _pp(V), rp, len(r) - 1, flux.ctypes.data_as(C.POINTER(C.c_double))),
                "th_flux_ns")
         return flux
+
+    def flux_mhd(self, U, B, radii):
+        """Rows: u2u, b2b, u2b_io, u2b_ii, b2u_io, b2u_ii, uo_bo, zp, zm (each over spheres 0..nsph)."""
+        import numpy as np
+        r, rp = _dbl(radii)
+        out = np.zeros((9, len(r)))
+        self._pre()
+        _check(lib().th_flux_mhd(self.h, _pp(U), _pp(B), rp, len(r) - 1,
+                                 out.ctypes.data_as(C.POINTER(C.c_double))), "th_flux_mhd")
+        return out
 
     def shell_tr_ns(self, V, radii):
         import numpy as np

@@ -857,6 +857,72 @@ extern "C" int th_flux_ns(th_plan* p, void* const V[3], const double* radii, int
     return DISPATCH(p, do_flux_ns, p, V, radii, nsph, flux);
 }
 
+// IncVF::Compute_flux(IncVF& W) compute_flux.cc:148-262: for each sphere s, G = U<, B<, U> (or the
+// Elsasser Zp<, Zm<), N = EnergyTr_Compute_nlin with the helper field named there, and the products
+// over |K| > R (Prod_out_sphere_nlinV) or 0 < |K| <= R (Prod_in_sphere_nlinV).
+template <typename R> static int do_flux_mhd(th_plan* p, void* const U[3], void* const B[3], const double* radii,
+                                             int nsph, double* out)
+{
+    Ops<R> o(p);
+    const double INF = 10000.0;
+    cplx<R>* u[3] = {CP(R, U[0]), CP(R, U[1]), CP(R, U[2])};
+    cplx<R>* b[3] = {CP(R, B[0]), CP(R, B[1]), CP(R, B[2])};
+    cplx<R>* G[3] = {o.S(SC_G0), o.S(SC_G0 + 1), o.S(SC_G0 + 2)};
+    cplx<R>* Nn[3] = {o.S(SC_NE0), o.S(SC_NE0 + 1), o.S(SC_NE0 + 2)};
+    cplx<R>* zp[3] = {o.S(SC_COPY0), o.S(SC_COPY0 + 1), o.S(SC_COPY0 + 2)};
+    cplx<R>* zm[3] = {o.S(SC_COPY0 + 3), o.S(SC_COPY0 + 4), o.S(SC_COPY0 + 5)};
+    const int W = nsph + 1;
+    for (int i = 0; i < 9 * W; i++) out[i] = 0.0;
+    auto fill = [&](cplx<R>* const* F, double in, double outr) -> int {
+        for (int i = 0; i < 3; i++) TRY(k_fill_shell<R>(p->g, F[i], G[i], in, outr, o.st));
+        return OK;
+    };
+    auto prod = [&](cplx<R>* const* F, double in, double outr, double* r) -> int {
+        const cplx<R>* a[3] = {Nn[0], Nn[1], Nn[2]};
+        const cplx<R>* f[3] = {F[0], F[1], F[2]};
+        return reduce_shell_mult_single<R>(p->g, a, f, in, outr, r, p->rs, o.st);
+    };
+    for (int s = 1; s <= nsph; s++) {
+        const double Rs = radii[s];
+        double r;
+        TRY(fill(u, 0.0, Rs));                                    // G = U<
+        TRY(o.et_nlin(u, G, Nn));                                 // U.grad U<
+        TRY(prod(u, Rs, INF, &r)); out[0 * W + s] = -r;           // flux_self
+        TRY(o.et_nlin(b, G, Nn));                                 // B.grad U<
+        TRY(prod(b, Rs, INF, &r)); out[2 * W + s] = r;            // flux_VF_in_out
+        TRY(prod(b, 0.0, Rs, &r)); out[3 * W + s] = r;            // flux_VF_in_in
+        TRY(fill(b, 0.0, Rs));                                    // G = B<
+        TRY(o.et_nlin(u, G, Nn));                                 // U.grad B<
+        TRY(prod(b, Rs, INF, &r)); out[1 * W + s] = -r;           // W.flux_self
+        TRY(prod(u, Rs, INF, &r)); out[4 * W + s] = r;            // W.flux_VF_in_out
+        TRY(prod(u, 0.0, Rs, &r)); out[5 * W + s] = r;            // W.flux_VF_in_in
+        TRY(fill(u, Rs, INF));                                    // G = U>
+        TRY(o.et_nlin(b, G, Nn));                                 // B.grad U>
+        TRY(prod(b, Rs, INF, &r)); out[6 * W + s] = r;            // flux_VF_out_out
+    }
+    for (int i = 0; i < 3; i++) {                                 // UB_to_Elsasser_field
+        TRY(o.copy(zp[i], u[i])); TRY(o.axpy(zp[i], 1.0, b[i]));
+        TRY(o.copy(zm[i], u[i])); TRY(o.axpy(zm[i], -1.0, b[i]));
+    }
+    for (int s = 1; s <= nsph; s++) {
+        const double Rs = radii[s];
+        double r;
+        TRY(fill(zp, 0.0, Rs));                                   // G = Zp<
+        TRY(o.et_nlin(zm, G, Nn));                                // Zm.grad Zp<
+        TRY(prod(zp, Rs, INF, &r)); out[7 * W + s] = -r;          // flux_Elsasser
+        TRY(fill(zm, 0.0, Rs));                                   // G = Zm<
+        TRY(o.et_nlin(zp, G, Nn));                                // Zp.grad Zm<
+        TRY(prod(zm, Rs, INF, &r)); out[8 * W + s] = -r;          // W.flux_Elsasser
+    }
+    return OK;
+}
+extern "C" int th_flux_mhd(th_plan* p, void* const U[3], void* const B[3], const double* radii, int nsph, double* out)
+{
+    TRY(check_plan(p));
+    if (!U || !B || !radii || nsph < 1 || !out) { set_error("th_flux_mhd: bad args"); return E_ARG; }
+    return DISPATCH(p, do_flux_mhd, p, U, B, radii, nsph, out);
+}
+
 template <typename R> static int do_shell_tr(th_plan* p, void* const V[3], const double* radii, int nsh, double* T)
 {
     Ops<R> o(p);

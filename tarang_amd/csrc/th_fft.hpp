@@ -56,6 +56,54 @@ template <int SIGN, typename R> __device__ __forceinline__ cplx<R> mul_si(cplx<R
     return SIGN < 0 ? mul_mi(a) : mul_i(a);
 }
 
+// Orders the LDS accesses of one wavefront (the compiler may not move LDS reads or writes across
+// it).  A wavefront's LDS operations execute in issue order, so lines owned by the lanes of one
+// wavefront exchange through LDS without a workgroup barrier.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// LDS placement of a line's elements during exchange s: element A lives at A + k (A >> sh).
+// The identity suits the interleaved layouts (LS > 1: consecutive lanes own different lines).
+// ZPad suits line-contiguous layouts (LS = 1: consecutive lanes own one line, lines `line`
+// elements apart), where the Stockham scatter A = (j / Ns) Ns Rr + j % Ns + r Ns of consecutive
+// j would otherwise pile onto a few banks.  (sh, k) per exchange and the line stride minimise the
+// conflict degree of the gfx950 LDS (64 banks of 4 bytes; a 64-bit access serves 32 lanes per
+// pass) over every access of the radix schedules below.
+struct NoPad {
+    static constexpr int map(int, int A) { return A; }
+};
+template <int L> struct ZPad {
+    static constexpr int code(int s)
+    {
+        // sh * 16 + k; 0 = identity
+        return L == 64 ? (s == 0 ? 3 * 16 + 1 : 0)
+             : L == 96 ? (s <= 2 ? 3 * 16 + 1 : 0)
+             : L == 128 ? (s <= 1 ? 4 * 16 + 1 : 0)
+             : L == 192 ? (s == 0 || s == 2 ? 4 * 16 + 1 : s == 1 ? 2 * 16 + 1 : 0)
+             : L == 256 ? (s == 0 ? 5 * 16 + 1 : s == 1 ? 3 * 16 + 2 : 0)
+             : 0;
+    }
+    static constexpr int map(int s, int A) { return code(s) ? A + (code(s) & 15) * (A >> (code(s) >> 4)) : A; }
+    static constexpr int need()
+    {
+        int m = L;
+        for (int s = 0; s < 8; s++) m = map(s, L - 1) + 1 > m ? map(s, L - 1) + 1 : m;
+        return m;
+    }
+    static constexpr int line = L == 64 ? 72 : L == 96 ? 120 : L == 128 ? 144 : L == 192 ? 240 : L == 256 ? 318 : L;
+    static_assert(line >= need(), "ZPad line stride too short");
+};
+
+template <bool WAVE> __device__ __forceinline__ void line_sync()
+{
+    if constexpr (WAVE) wave_sync();
+    else __syncthreads();
+}
+
 template <int RADIX, int SIGN, typename R> struct DFT;
 
 template <int SIGN, typename R> struct DFT<2, SIGN, R> {
@@ -124,13 +172,15 @@ template <typename R, int L, int P, int SIGN> struct LineFFT {
     static __device__ __forceinline__ int in_pos(int t, int i) { return t + P * (i / R0) + (i % R0) * (L / R0); }
     static __device__ __forceinline__ int out_pos(int t, int i) { return t + P * (i / RL) + (i % RL) * (L / RL); }
 
-    template <int LS>
+    // WAVE: the P threads of every line sit in one wavefront (P divides 64 and each line's
+    // threads are consecutive lanes), so LDS exchanges need no workgroup barrier.
+    template <int LS, bool WAVE = false, class Lay = NoPad>
     static __device__ __forceinline__ void run(cplx<R> (&v)[E], cplx<R>* lds, int t, const cplx<R>* __restrict__ tw)
     {
-        stage<0, 1, LS>(v, lds, t, tw);
+        stage<0, 1, LS, WAVE, Lay>(v, lds, t, tw);
     }
 
-    template <int s, int Ns, int LS>
+    template <int s, int Ns, int LS, bool WAVE, class Lay>
     static __device__ __forceinline__ void stage(cplx<R> (&v)[E], cplx<R>* lds, int t, const cplx<R>* __restrict__ tw)
     {
         constexpr int Rr = S.r[s];
@@ -150,19 +200,19 @@ template <typename R, int L, int P, int SIGN> struct LineFFT {
             DFT<Rr, SIGN, R>::run(&v[b * Rr]);
         }
         if constexpr (s + 1 < NST) {
-            __syncthreads();
+            line_sync<WAVE>();
 #pragma unroll
             for (int b = 0; b < B; b++) {
                 const int j = t + P * b;
                 const int base = (j / Ns) * Ns * Rr + (j % Ns);
 #pragma unroll
-                for (int r = 0; r < Rr; r++) lds[(base + r * Ns) * LS] = v[b * Rr + r];
+                for (int r = 0; r < Rr; r++) lds[Lay::map(s, base + r * Ns) * LS] = v[b * Rr + r];
             }
-            __syncthreads();
+            line_sync<WAVE>();
             constexpr int Rn = S.r[s + 1];
 #pragma unroll
-            for (int i = 0; i < E; i++) v[i] = lds[(t + P * (i / Rn) + (i % Rn) * (L / Rn)) * LS];
-            stage<s + 1, Ns * Rr, LS>(v, lds, t, tw);
+            for (int i = 0; i < E; i++) v[i] = lds[Lay::map(s, t + P * (i / Rn) + (i % Rn) * (L / Rn)) * LS];
+            stage<s + 1, Ns * Rr, LS, WAVE, Lay>(v, lds, t, tw);
         }
     }
 };

@@ -33,6 +33,35 @@ __host__ __device__ __forceinline__ bool lkept(const LMap& m, int l) { return l 
 __host__ __device__ __forceinline__ int lq(const LMap& m, int l) { return l < m.h ? l : l - (m.N - m.nk); }
 __host__ __device__ __forceinline__ int ll(const LMap& m, int q) { return q < m.h ? q : q + (m.N - m.nk); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (linear id mod 8), each XCD with its own
+// L2.  xcd_tile renumbers a linear workgroup id so that every XCD gets one contiguous range of
+// logical tiles: neighbouring tiles, which share 128-byte lines, then meet in the same L2.
+// A zero the compiler cannot see through.  Offsetting a table pointer by it inside a rolled loop
+// keeps the table loads in the loop (L1 hits) instead of hoisting every twiddle into registers
+// for the whole kernel, which would cost occupancy.
+__device__ __forceinline__ int opaque_zero()
+{
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    return z;
+}
+
+// Fused states are 2/3-dealiased, so along x only l < L/3 and l > L - L/3 are kept.  Lane t of an
+// x line holds positions t + P m (m = 0..E-1, register i = (m % (E/RL)) RL + m / (E/RL)); xdrop
+// tells, at compile time, whether register i holds no kept position for any lane.
+template <int L, int P, int E, int RL> __host__ __device__ constexpr bool xdrop(int i)
+{
+    const int m = i / RL + (i % RL) * (E / RL), h = L / 3;
+    return m * P >= h && m * P + P - 1 <= L - h;
+}
+
+__device__ __forceinline__ int xcd_tile(int bid, int nblk)
+{
+    const int xcd = bid & 7, loc = bid >> 3;
+    const int per = nblk >> 3, rem = nblk & 7;
+    return (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + loc;
+}
+
 struct FG {
     int N1, N2, N3, LZ;
     LMap mx, my;
@@ -159,19 +188,31 @@ __global__ __launch_bounds__(T * P) void k_fy_inv(FG g, const cplx<R>* __restric
 template <int MODE> struct ZProd;
 template <> struct ZProd<0> {
     static constexpr int NIN = 3, NOUT = 6;
-    static __device__ __forceinline__ int a(int p) { return p < 3 ? p : p - 3; }
-    static __device__ __forceinline__ int b(int p) { return p < 3 ? p : (p == 5 ? 0 : p - 2); }
+    static constexpr int a(int p) { return p < 3 ? p : p - 3; }
+    static constexpr int b(int p) { return p < 3 ? p : (p == 5 ? 0 : p - 2); }
 };
 template <> struct ZProd<1> {
     static constexpr int NIN = 6, NOUT = 9;
-    static __device__ __forceinline__ int a(int p) { return p / 3; }
-    static __device__ __forceinline__ int b(int p) { return 3 + p % 3; }
+    static constexpr int a(int p) { return p / 3; }
+    static constexpr int b(int p) { return 3 + p % 3; }
 };
 template <> struct ZProd<2> {
     static constexpr int NIN = 6, NOUT = 9;
-    static __device__ __forceinline__ int a(int p) { return 3 + p / 3; }
-    static __device__ __forceinline__ int b(int p) { return p % 3; }
+    static constexpr int a(int p) { return 3 + p / 3; }
+    static constexpr int b(int p) { return p % 3; }
 };
+
+template <typename R> __device__ __forceinline__ cplx<R> shfl_c(const cplx<R>& v, int src, int width)
+{
+    return mk<R>(__shfl(v.x, src, width), __shfl(v.y, src, width));
+}
+
+// Lane t of a z line holds the points at positions t + P m, m = 0..E-1, in a register order fixed
+// by the radix Rr of the FFT stage that reads (first stage) or writes (last stage) them.
+template <int E, int Rr> __device__ __forceinline__ constexpr int zreg(int m)
+{
+    return (m % (E / Rr)) * Rr + m / (E / Rr);
+}
 
 template <typename R, int LZ, int P, int TZ, int MODE>
 __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ Cout,
@@ -181,59 +222,102 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
     using FF = LineFFT<R, LZ, P, -1>;
     using ZP = ZProd<MODE>;
     constexpr int E = FI::E;
-    __shared__ cplx<R> lds[TZ * LZ];
+    static_assert(64 % P == 0, "a z line's threads must share one wavefront");
+    constexpr int LZS = ZPad<LZ>::line;   // padded line stride in LDS
+    __shared__ cplx<R> lds[TZ * LZS];
     const int t = threadIdx.x % P, c = threadIdx.x / P;
     const long q = (long)blockIdx.x * TZ + c;
     const bool ok = q < (long)g.N1 * g.N2;
     const long off = q * g.KZP;
     B += blockIdx.y * g.bB;
     Cout += blockIdx.y * g.bC;
-    cplx<R>* Z = lds + c * LZ;
+    cplx<R>* Z = lds + c * LZS;
+    // The mirror point LZ - (t + P m) sits in lane (P - t) mod P at m' = E - 1 - m (t > 0) or
+    // m' = (E - m) mod E (t = 0): the half-length packing reads it by a lane shuffle.
+    const int tp = (P - t) % P;
+    cplx<R> twk[E];   // W_N3^(t + P m)
+#pragma unroll
+    for (int m = 0; m < E; m++) twk[m] = twN[t + P * m];
     cplx<R> u[ZP::NIN][E];
 #pragma unroll
     for (int a = 0; a < ZP::NIN; a++) {
         const cplx<R>* X = B + a * g.cB + off;
+        cplx<R> xa[E];
 #pragma unroll
-        for (int i = 0; i < E; i++) {
-            const int k = FI::in_pos(t, i), kk = LZ - k;
-            cplx<R> xa = (ok && k < g.nkz) ? X[k] : czero<R>();
-            cplx<R> xb = (ok && kk < g.nkz) ? X[kk] : czero<R>();
-            if (k == 0) { xa.y = R(0); xb.y = R(0); }
-            xb.y = -xb.y;
-            cplx<R> s = xa + xb, d = xa - xb;
-            cplx<R> w = twN[k];
-            w.y = -w.y;
-            u[a][i] = s + mul_i(cmul(w, d));
+        for (int m = 0; m < E; m++) {
+            const int k = t + P * m;
+            xa[m] = (ok && k < g.nkz) ? X[k] : czero<R>();
         }
-        FI::template run<1>(u[a], Z, t, twL);
+        const cplx<R> xny = (ok && t == 0 && LZ < g.nkz) ? X[LZ] : czero<R>();   // kz = N3/2 (undealiased)
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            cplx<R> xr = shfl_c(xa[E - 1 - m], tp, P);
+            cplx<R> xb = (t == 0) ? (m == 0 ? xny : xa[(E - m) % E]) : xr;
+            cplx<R> xv = xa[m];
+            if (t == 0 && m == 0) { xv.y = R(0); xb.y = R(0); }
+            xb.y = -xb.y;
+            cplx<R> s = xv + xb, d = xv - xb;
+            cplx<R> w = conjg(twk[m]);
+            u[a][zreg<E, FI::R0>(m)] = s + mul_i(cmul(w, d));
+        }
+        FI::template run<1, true, ZPad<LZ>>(u[a], Z, t, twL);
     }
-#pragma unroll
-    for (int p = 0; p < ZP::NOUT; p++) {
-        const int ia = ZP::a(p), ib = ZP::b(p);
-        cplx<R> w[E];
-#pragma unroll
-        for (int i = 0; i < E; i++) w[i] = mk<R>(u[ia][i].x * u[ib][i].x, u[ia][i].y * u[ib][i].y);
+    // One product line: forward half-length FFT, r2c unpacking, store of the kept kz.
+    auto product_out = [&](cplx<R> (&w)[E], int p) {
         if constexpr (FI::R0 != FI::RL) {
-            __syncthreads();
+            wave_sync();
 #pragma unroll
             for (int i = 0; i < E; i++) Z[FI::out_pos(t, i)] = w[i];
-            __syncthreads();
+            wave_sync();
 #pragma unroll
             for (int i = 0; i < E; i++) w[i] = Z[FF::in_pos(t, i)];
         }
-        FF::template run<1>(w, Z, t, twL);
-        __syncthreads();
+        FF::template run<1, true, ZPad<LZ>>(w, Z, t, twL);
+        const cplx<R>* twp = twN + opaque_zero();   // W_N3^(t + P m), reloaded per product (L1)
+        cplx<R>* X = Cout + p * g.cB + off;
 #pragma unroll
-        for (int i = 0; i < E; i++) Z[FF::out_pos(t, i)] = w[i];
-        __syncthreads();
-        if (ok) {
-            cplx<R>* X = Cout + p * g.cB + off;
-            for (int k = t; k < g.nkz; k += P) {
-                cplx<R> zk = Z[k % LZ], zc = conjg(Z[(LZ - k) % LZ]);
-                cplx<R> e = cscale(zk + zc, R(0.5));
-                cplx<R> tt = cmul(twN[k], zk - zc);
-                X[k] = mk<R>(e.x + R(0.5) * tt.y, e.y - R(0.5) * tt.x);
+        for (int m = 0; m < E; m++) {
+            const int k = t + P * m;
+            cplx<R> yr = shfl_c(w[zreg<E, FF::RL>(E - 1 - m)], tp, P);
+            cplx<R> zk = w[zreg<E, FF::RL>(m)];
+            cplx<R> zc = conjg((t == 0) ? w[zreg<E, FF::RL>((E - m) % E)] : yr);
+            cplx<R> e = cscale(zk + zc, R(0.5));
+            cplx<R> tt = cmul(twp[t + P * m], zk - zc);
+            if (ok && k < g.nkz) X[k] = mk<R>(e.x + R(0.5) * tt.y, e.y - R(0.5) * tt.x);
+        }
+        if (ok && t == 0 && LZ < g.nkz) {   // kz = N3/2 (undealiased): Y[0] - ... with W_N3^(N3/2) = -1
+            cplx<R> z0 = w[zreg<E, FF::RL>(0)];
+            X[LZ] = mk<R>(z0.x - z0.y, R(0));
+        }
+    };
+    if constexpr (sizeof(R) == 4) {
+        // FP32: a rolled loop with operands picked by selects keeps the compiler from
+        // interleaving products and hoisting their twiddles (registers, occupancy).
+#pragma unroll 1
+        for (int p = 0; p < ZP::NOUT; p++) {
+            const int ia = ZP::a(p), ib = ZP::b(p);
+            cplx<R> w[E];
+#pragma unroll
+            for (int i = 0; i < E; i++) {
+                cplx<R> x = u[0][i], y = u[0][i];
+#pragma unroll
+                for (int a = 1; a < ZP::NIN; a++) {
+                    x = (ia == a) ? u[a][i] : x;
+                    y = (ib == a) ? u[a][i] : y;
+                }
+                w[i] = mk<R>(x.x * y.x, x.y * y.y);
             }
+            product_out(w, p);
+        }
+    } else {
+        // FP64: selects over the arrays would become dynamic indexing (scratch); unroll instead
+#pragma unroll
+        for (int p = 0; p < ZP::NOUT; p++) {
+            const int ia = ZP::a(p), ib = ZP::b(p);
+            cplx<R> w[E];
+#pragma unroll
+            for (int i = 0; i < E; i++) w[i] = mk<R>(u[ia][i].x * u[ib][i].x, u[ia][i].y * u[ib][i].y);
+            product_out(w, p);
         }
     }
 }
@@ -315,7 +399,7 @@ __device__ __forceinline__ void rk_update(int stage, double dt, double eh, doubl
 }
 
 template <typename R, int L, int P, int T, int MODE>
-__global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restrict__ D, cplx<R>* S, cplx<R>* Vs,
+__global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 0 && sizeof(R) == 4) ? 4 : 1))) void k_fx_fwd(FG g, const cplx<R>* __restrict__ D, cplx<R>* S, cplx<R>* Vs,
                                                   cplx<R>* acc, const cplx<R>* __restrict__ tw,
                                                   const StageCoef* __restrict__ coefs)
 {
@@ -326,11 +410,12 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restric
     __shared__ cplx<R> lds[L * T];
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int ntz = (g.nkz + T - 1) / T;
-    const int cy = blockIdx.x / ntz;
-    const int cz = (blockIdx.x % ntz) * T + c;
+    const int lin = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int tile = lin % gridDim.x, box = lin / gridDim.x;
+    const int cy = tile / ntz;
+    const int cz = (tile % ntz) * T + c;
     const bool ok = cz < g.nkz;
     const long m = (long)cy * g.KZP + cz;
-    const int box = blockIdx.y;
     const StageCoef sc = coefs[box];
     D += box * g.bA;
     S += box * g.bS;
@@ -345,13 +430,20 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restric
         for (int i = 0; i < E; i++) Nacc[a][i] = czero<R>();
     // The product loop stays rolled (one product line live at a time).  Each product p adds
     // i (alpha_a Kx + beta_a) P_p to accumulator a, so accumulators are indexed statically.
+    // Addresses split into a uniform row base (SGPRs) and one 32-bit lane offset: lane (c, t)
+    // reads column cz = tile * T + c of row in_pos(t, i) = t + in_pos(0, i).
+    const long mu = (long)cy * g.KZP + (long)(tile % ntz) * T;
+    const int lane_off = c + t * (int)g.sx;
 #pragma unroll 1
     for (int p = 0; p < NP; p++) {
         cplx<R> v[E];
-        const cplx<R>* src = D + p * g.cA + m;
+        const cplx<R>* src = D + p * g.cA + mu;
 #pragma unroll
-        for (int i = 0; i < E; i++) v[i] = ok ? src[(long)F::in_pos(t, i) * g.sx] : czero<R>();
-        F::template run<T>(v, lds + c, t, tw);
+        for (int i = 0; i < E; i++) {
+            const cplx<R>* row = src + (long)F::in_pos(0, i) * g.sx;
+            v[i] = ok ? row[lane_off] : czero<R>();
+        }
+        F::template run<T>(v, lds + c, t, tw + opaque_zero());
         // contributions (a1 <- K_k1) and (a2 <- K_k2) of product p; k = 0 means Kx (per row)
         int a1, k1, a2, k2;
         if (MODE == 0) {           // P_ij, (i,j) = (0,0),(1,1),(2,2),(0,1),(1,2),(2,0): N_i += iK_j P, N_j += iK_i P
@@ -373,6 +465,7 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restric
         }
 #pragma unroll
         for (int i = 0; i < E; i++) {
+            if (xdrop<L, P, E, F::RL>(i)) continue;
             const R Kx = (R)(kx_four(F::out_pos(t, i), g.N1) * g.f1);
 #pragma unroll
             for (int a = 0; a < NA; a++) acc_iK(Nacc[a][i], al[a] * Kx + be[a], v[i]);
@@ -383,10 +476,13 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restric
 #pragma unroll
     for (int i = 0; i < E; i++) {
         const int l = F::out_pos(t, i);
-        if (!lkept(g.mx, l)) continue;
+        if (xdrop<L, P, E, F::RL>(i) || !lkept(g.mx, l)) continue;
         const double dKx = kx_four(l, g.N1) * g.f1;
         const double K2 = ksq_f(dKx, dKy, dKz);
-        const long idx = (long)lq(g.mx, l) * g.sx + m;
+        // compact row of l = t + l0: the kept lanes of one register all sit on one side of the
+        // dealiased gap (P < L/3), so the row shift is uniform per register
+        const int l0 = F::out_pos(0, i);
+        const long idx = (long)(l0 - (l0 >= g.mx.h ? g.mx.N - g.mx.nk : 0)) * g.sx + mu + lane_off;
         Dc Nu[3];
         if (MODE == 0) {
 #pragma unroll
@@ -424,6 +520,7 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd(FG g, const cplx<R>* __restric
                              acc + (3 + a) * g.cS + idx);
             }
         }
+        __builtin_amdgcn_sched_barrier(0);   // one mode at a time: bounds the live state loads
     }
 }
 
@@ -461,7 +558,7 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __rest
         const cplx<R>* src = D + p * g.cA + m;
 #pragma unroll
         for (int i = 0; i < E; i++) v[i] = ok ? src[(long)F::in_pos(t, i) * g.sx] : czero<R>();
-        F::template run<T>(v, lds + c, t, tw);
+        F::template run<T>(v, lds + c, t, tw + opaque_zero());
         const int pi = p / 3, pj = p % 3;        // P(pi,pj) = G_pi H_pj -> N_pi += i K_pj P
         R al[3], be[3];
 #pragma unroll
@@ -471,6 +568,7 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __rest
         }
 #pragma unroll
         for (int i = 0; i < E; i++) {
+            if (xdrop<L, P, E, F::RL>(i)) continue;
             const R Kx = (R)(kx_four(F::out_pos(t, i), g.N1) * g.f1);
 #pragma unroll
             for (int a = 0; a < 3; a++) acc_iK(Nacc[a][i], al[a] * Kx + be[a], v[i]);
@@ -480,9 +578,10 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __rest
     if (ok) {
         const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
         const int l2 = ll(g.my, cy);
+#pragma unroll
         for (int i = 0; i < E; i++) {
             const int l = F::out_pos(t, i);
-            if (!lkept(g.mx, l)) continue;
+            if (xdrop<L, P, E, F::RL>(i) || !lkept(g.mx, l)) continue;
             const int kx = kx_four(l, g.N1);
             const double km = kmag_f(kx * g.f1, dKy, dKz);
             // Multiplicity_factor_FOUR four_inline.h:347-365
@@ -769,6 +868,7 @@ static int state_create(th_plan* p, th_state** out, int physics, int nbox)
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
     if (p->g.basis != TH_FOUR) { set_error("th_state_create: the fused path covers the FOUR basis; use th_step_plain for SCFT"); return E_UNSUPPORTED; }
     if (physics != TH_NS && physics != TH_MHD) { set_error("th_state_create: physics %d not fused (use th_step_plain)", physics); return E_UNSUPPORTED; }
+    if (!p->g.dealias) { set_error("th_state_create: the fused path stores 2/3-dealiased modes only; create the plan with dealias = 1"); return E_UNSUPPORTED; }
     if (!fused_len_ok(p->g.N1) || !fused_len_ok(p->g.N2) || !fused_len_ok(p->g.N3)) {
         set_error("th_state_create: fused path needs N in {16,24,32,48,64,96,128,192,256,384,512}");
         return E_UNSUPPORTED;

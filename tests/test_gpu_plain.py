@@ -270,6 +270,23 @@ def test_flux_and_shell_transfer_plain(ctx):
     assert np.abs(T - T_ref).max() < 1e-12 * np.abs(T_ref).max()
 
 
+@pytest.mark.parametrize("N", [(16, 16, 16), (24, 16, 32)])
+def test_flux_mhd_plain(ctx, N):
+    """th_flux_mhd (IncVF::Compute_flux(IncVF& W), compute_flux.cc:148-262): the nine MHD and
+    Elsasser fluxes match the oracle over unit spheres."""
+    from tarang_amd import Plan
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    U, B = o.ic_random(2), o.ic_random(3)
+    radii = np.concatenate([np.arange(7), [10000.0]])
+    ref = o.flux_mhd(U, B, radii, elsasser=True)
+    got = p.flux_mhd([p.to_dev(a) for a in U], [p.to_dev(a) for a in B], radii)
+    names = ["u2u", "b2b", "u2b_in_out", "u2b_in_in", "b2u_in_out", "b2u_in_in", "u_out_b_out", "zp", "zm"]
+    for r, k in enumerate(names):
+        sc = max(np.abs(ref[k]).max(), 1e-300)
+        assert np.abs(got[r] - ref[k]).max() < 1e-12 * sc, k
+
+
 def test_config1_taylor_green_64_100_steps(ctx):
     """Config 1: TG 64^3 FP64, RK4, dt = 1e-3, 100 steps, nu = 0.01, 2/3 dealias.  The plain GPU path
     matches the oracle to <= 1e-10 relative in E(k) (shells with E >= 1e-10 max E) and in Pi(R)."""

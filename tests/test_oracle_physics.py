@@ -220,6 +220,38 @@ def test_mhd_elsasser_reductions():
     assert max(np.abs(a).max() for a in nU + nB) < 1e-16
 
 
+def test_mhd_flux_reductions_and_role_symmetry():
+    """IncVF::Compute_flux(IncVF& W) (compute_flux.cc:148-262): with B = 0 the U<->U> flux and both
+    Elsasser fluxes equal the NS flux and every U-B term vanishes; with B = U the self fluxes equal
+    the NS flux, U<->B> = B<->U> = -flux and the Elsasser fluxes vanish (Z- = 0); swapping U and B
+    exchanges U<->B and B<->U (which fixes helper, giver and receiver of each term)."""
+    o = Oracle((16, 16, 16))
+    U = o.ic_random(2)
+    B = o.ic_random(3)
+    radii = np.array([0.0, 1, 2, 3, 4, 5, 7, 10000.0])
+    fns = o.flux_ns(U, radii)
+    sc = np.abs(fns).max()
+    f = o.flux_mhd(U, o.zeros(3), radii, elsasser=True)
+    for k in ("u2u", "zp", "zm"):
+        assert np.abs(f[k] - fns).max() < 1e-13 * sc, k
+    for k in ("b2b", "u2b_in_out", "u2b_in_in", "b2u_in_out", "b2u_in_in", "u_out_b_out"):
+        assert np.abs(f[k]).max() < 1e-16, k
+    f = o.flux_mhd(U, [u.copy() for u in U], radii, elsasser=True)
+    for k in ("u2u", "b2b"):
+        assert np.abs(f[k] - fns).max() < 1e-13 * sc, k
+    for k in ("u2b_in_out", "b2u_in_out"):
+        assert np.abs(f[k] + fns).max() < 1e-13 * sc, k
+    assert np.abs(f["u2b_in_in"] - f["b2u_in_in"]).max() < 1e-13 * sc
+    for k in ("zp", "zm"):
+        assert np.abs(f[k]).max() < 1e-16, k
+    a = o.flux_mhd(U, B, radii)
+    b = o.flux_mhd(B, U, radii)
+    s2 = max(np.abs(a[k]).max() for k in a)
+    assert np.abs(a["u2b_in_out"] - b["b2u_in_out"]).max() < 1e-13 * s2
+    assert np.abs(a["u2b_in_in"] - b["b2u_in_in"]).max() < 1e-13 * s2
+    assert np.abs(a["b2u_in_out"]).max() > 1e-6 * s2
+
+
 def test_mhd_ideal_energy_conservation():
     """Ideal MHD (nu = eta = 0): one IF-RK4 step conserves E_u + E_b to O(dt^5)."""
     o = Oracle((16, 16, 16))
