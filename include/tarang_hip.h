@@ -166,6 +166,19 @@ int th_flux_shell_tr(th_plan *plan, th_state *state, const double *radii, int ns
 int th_profile_passes(th_plan *plan, th_state *state, double dt, const th_phys *p, int stage, int nrep,
                       double ms[5], double bytes[5]);
 
+/* ---- launch-shape autotuner (no counterpart in the reference, whose transforms are planned by
+ * FFTW_MEASURE in Init_fftw_plan_FOUR four_tr.cc:44-98) ----
+ * th_tune times every launch-shape variant of the five fused passes on the state's own buffers
+ * (one RK stage, nrep launches each), keeps the fastest in the state and records it in the
+ * context's tune cache (th_ctx_create's tune_cache_path), from which later states of the same
+ * workload start.  var (may be NULL) receives the chosen variant per pass: x-inverse, y-inverse,
+ * fused z, y-forward, x-forward.  The state's fields are unchanged.
+ * th_state_set_variants sets them directly (tests, reproducing a tuned run); th_state_variants
+ * reads them (a new state starts from the cache entry of its workload, else all 0). */
+int th_tune(th_plan *plan, th_state *state, double dt, const th_phys *p, int nrep, int var[5]);
+int th_state_set_variants(th_state *state, const int var[5]);
+int th_state_variants(const th_state *state, int var[5]);
+
 /* ---- ensemble of independent NS boxes of uneven sizes ----
  * The reference runs one grid size per process (its transform and energy routines keep static
  * scratch arrays sized for one grid, four_tr.cc:173-174, four_energy.cc:302-303), so an ensemble
@@ -182,6 +195,8 @@ int th_ensemble_plan(th_ensemble *ens, int box, th_plan **plan);
 int th_ensemble_upload(th_ensemble *ens, int box, int comp, const void *src, int src_on_host);
 int th_ensemble_download(th_ensemble *ens, int box, int comp, void *dst, int dst_on_host);
 int th_ensemble_step(th_ensemble *ens, const double *dt, const double *nu, int nsteps);
+/* th_tune for every size class of the ensemble (per-box dt and nu as in th_ensemble_step) */
+int th_ensemble_tune(th_ensemble *ens, const double *dt, const double *nu, int nrep);
 
 #ifdef __cplusplus
 }

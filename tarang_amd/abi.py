@@ -104,6 +104,10 @@ def lib():
         "th_step": (i, [vp, vp, d, C.POINTER(Phys), i]),
         "th_flux_shell_tr": (i, [vp, vp, dp, i, dp, dp]),
         "th_profile_passes": (i, [vp, vp, d, C.POINTER(Phys), i, i, dp, dp]),
+        "th_tune": (i, [vp, vp, d, C.POINTER(Phys), i, C.POINTER(C.c_int)]),
+        "th_state_set_variants": (i, [vp, C.POINTER(C.c_int)]),
+        "th_state_variants": (i, [vp, C.POINTER(C.c_int)]),
+        "th_ensemble_tune": (i, [vp, dp, dp, i]),
         "th_ensemble_create": (i, [vp, C.POINTER(vp), i, C.POINTER(C.c_int), i]),
         "th_ensemble_destroy": (i, [vp]),
         "th_ensemble_plan": (i, [vp, i, C.POINTER(vp)]),
@@ -477,6 +481,26 @@ class State:
                                        b.ctypes.data_as(C.POINTER(C.c_double))), "th_profile_passes")
         return ms, b
 
+    def tune(self, dt, nu=0.0, eta=0.0, nrep=3):
+        """Autotune the launch shapes of the five fused passes; returns the chosen variants."""
+        ph = Phys(nu, eta, 0.0, 0.0, 0.0)
+        v = (C.c_int * 5)()
+        self.plan._pre()
+        self.plan._post(lib().th_tune(self.plan.h, self.h, dt, C.byref(ph), nrep, v), "th_tune")
+        return list(v)
+
+    @property
+    def variants(self):
+        v = (C.c_int * 5)()
+        _check(lib().th_state_variants(self.h, v), "th_state_variants")
+        return list(v)
+
+    @variants.setter
+    def variants(self, var):
+        v = (C.c_int * 5)(*[int(x) for x in var])
+        self.plan._pre()
+        _check(lib().th_state_set_variants(self.h, v), "th_state_set_variants")
+
     def flux_shell_tr(self, radii):
         import numpy as np
         r, rp = _dbl(radii)
@@ -540,6 +564,13 @@ class Ensemble:
             _check(lib().th_ensemble_download(self.h, box, c, _ptr(f), 0), "th_ensemble_download")
         p._post(0, "")
         return out
+
+    def tune(self, dt, nu, nrep=3):
+        dt_, dtp = _dbl(dt)
+        nu_, nup = _dbl(nu)
+        p = self.plan(0)
+        p._pre()
+        p._post(lib().th_ensemble_tune(self.h, dtp, nup, nrep), "th_ensemble_tune")
 
     def step(self, dt, nu, steps=1, sync=True):
         dt_, dtp = _dbl(dt)

@@ -70,6 +70,55 @@ void TwiddleCache::release()
 // ------------------------------------------------------------------------------------------
 // context / plan
 // ------------------------------------------------------------------------------------------
+namespace th {
+static void tune_load(th_ctx* c)
+{
+    if (c->tune_path.empty()) return;
+    FILE* f = fopen(c->tune_path.c_str(), "r");
+    if (!f) return;
+    char line[512];
+    while (fgets(line, sizeof line, f)) {
+        char key[256];
+        int n = 0, off = 0;
+        if (line[0] == '#' || sscanf(line, "%255s%n", key, &off) != 1) continue;
+        std::vector<int> v;
+        const char* q = line + off;
+        int x;
+        while (sscanf(q, "%d%n", &x, &n) == 1) {
+            v.push_back(x);
+            q += n;
+        }
+        if (!v.empty()) c->tune[key] = v;
+    }
+    fclose(f);
+}
+bool tune_lookup(th_ctx* c, const std::string& key, std::vector<int>& v)
+{
+    std::lock_guard<std::mutex> lk(c->tune_mu);
+    auto it = c->tune.find(key);
+    if (it == c->tune.end()) return false;
+    v = it->second;
+    return true;
+}
+void tune_store(th_ctx* c, const std::string& key, const std::vector<int>& v)
+{
+    std::lock_guard<std::mutex> lk(c->tune_mu);
+    c->tune[key] = v;
+    if (c->tune_path.empty()) return;
+    std::string tmp = c->tune_path + ".tmp";
+    FILE* f = fopen(tmp.c_str(), "w");
+    if (!f) return;
+    fprintf(f, "# tarang_hip launch-shape cache: key, then one variant per pass\n");
+    for (auto& kv : c->tune) {
+        fprintf(f, "%s", kv.first.c_str());
+        for (int x : kv.second) fprintf(f, " %d", x);
+        fprintf(f, "\n");
+    }
+    fclose(f);
+    (void)rename(tmp.c_str(), c->tune_path.c_str());
+}
+}  // namespace th
+
 extern "C" int th_ctx_create(th_ctx** ctx, int device, const char* tune_cache_path)
 {
     if (!ctx) { set_error("th_ctx_create: null"); return E_ARG; }
@@ -81,6 +130,7 @@ extern "C" int th_ctx_create(th_ctx** ctx, int device, const char* tune_cache_pa
     if (!c) return E_NOMEM;
     c->device = device;
     if (tune_cache_path) c->tune_path = tune_cache_path;
+    tune_load(c);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; set_error("hipStreamCreate: %s", hipGetErrorString(e)); return E_HIP; }
     *ctx = c;

@@ -1,0 +1,780 @@
+// th_fused_kern.hpp -- kernels of the dealias-pruned, fused pseudo-spectral RK stage (FOUR basis).
+//
+// State layout: only the modes kept by Dealias_array (universal_misc.cc:118-159) are stored, in the
+// reference's (kx, ky, kz) order, compacted per axis:  S[cx][cy][cz] with cz padded to KZP (128-byte
+// rows).  A stage of the IF-RK4 scheme (time_advance.cc:100-155) is five passes, each a batch of
+// LDS-staged line FFTs over 128-byte column tiles; nothing else touches HBM:
+//   1  x-inverse   S[cx][m]    -> A[x][m]          (m = (cy,cz); zero rows of the gap are never read)
+//   2  y-inverse   A[x][cy][z] -> B[x][y][z]
+//   3  z-fused     B[x][y][:]  -> c2r, real-space products (Array_real_mult), r2c -> C[x][y][kz<N3/3]
+//   4  y-forward   C[x][y][z]  -> D[x][cy][z]      (only kept ky written)
+//   5  x-forward   D[x][m]     -> kept kx only; epilogue: N_i = sum_j i K_j F(u_i u_j) (RSprod.cc),
+//                  projection -P(K) N (compute_pressure.cc, compute_rhs.cc), FP64 integrating factor
+//                  exp(-nu K^2 a dt) and the RK4 stage update (single_time_step.cc, Compute_RK4_parts)
+// MHD uses the Elsasser form (compute_nlin_VF_MHD.cc) with 6 inverse and 9 product lines; the energy
+// transfer (ET_compute_nlin.cc) uses helper U and a shell-masked giver G and bins 2 mult Re(N U*)
+// (four_ET.cc) in the pass-5 epilogue instead of writing N.
+
+#pragma once
+#include "th_fused.hpp"
+
+namespace th {
+
+template <typename R> __device__ __forceinline__ cplx<R> ld(const cplx<R>* p) { return *p; }
+
+// ------------------------------------------------------------------------------------------
+// pass 1: x-inverse.  MODE 0: copy comps; 1: Elsasser Z+ = U + B (comps 0-2), Z- = U - B (3-5);
+// 2: giver G = U masked to the shell sel.
+// ------------------------------------------------------------------------------------------
+template <typename R, int L, int P, int T, int MODE>
+__global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restrict__ S, cplx<R>* __restrict__ A,
+                                                  const cplx<R>* __restrict__ tw, ShellSel sel)
+{
+    using F = LineFFT<R, L, P, +1>;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int ntz = (g.nkz + T - 1) / T;
+    const int cy = blockIdx.x / ntz;
+    const int cz = (blockIdx.x % ntz) * T + c;
+    const bool ok = cz < g.nkz;
+    const long m = (long)cy * g.KZP + cz;
+    const int oc = blockIdx.y;
+    S += blockIdx.z * g.bS;
+    A += blockIdx.z * g.bA;
+    double Ky = 0, Kz = 0;
+    if (MODE == 2) {
+        Ky = ky_of(ll(g.my, cy), g.N2) * g.f2;
+        Kz = cz * g.f3;
+    }
+    cplx<R> v[F::E];
+#pragma unroll
+    for (int i = 0; i < F::E; i++) {
+        const int l = F::in_pos(t, i);
+        cplx<R> a = czero<R>();
+        if (ok && lkept(g.mx, l)) {
+            const long off = (long)lq(g.mx, l) * g.sx + m;
+            if (MODE == 0) {
+                a = S[oc * g.cS + off];
+            } else if (MODE == 1) {
+                const int cc = oc % 3;
+                cplx<R> u = S[cc * g.cS + off], b = S[(3 + cc) * g.cS + off];
+                a = (oc < 3) ? (u + b) : (u - b);
+            } else {
+                double km = kmag_f(kx_four(l, g.N1) * g.f1, Ky, Kz);
+                if ((km > sel.inner) && (km <= sel.outer)) a = S[oc * g.cS + off];
+            }
+        }
+        v[i] = a;
+    }
+    F::template run<T>(v, lds + c, t, tw);
+    if (ok) {
+        cplx<R>* out = A + oc * g.cA + m;
+#pragma unroll
+        for (int i = 0; i < F::E; i++) out[(long)F::out_pos(t, i) * g.sx] = v[i];
+    }
+}
+
+// pass 2: y-inverse, A[x][cy][cz] -> B[x][y][cz]
+template <typename R, int L, int P, int T>
+__global__ __launch_bounds__(T * P) void k_fy_inv(FG g, const cplx<R>* __restrict__ A, cplx<R>* __restrict__ B,
+                                                  const cplx<R>* __restrict__ tw, int ncomp)
+{
+    using F = LineFFT<R, L, P, +1>;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int cz = blockIdx.x * T + c;
+    const bool ok = cz < g.nkz;
+    const int x = blockIdx.y, comp = blockIdx.z % ncomp, box = blockIdx.z / ncomp;
+    A += box * g.bA;
+    B += box * g.bB;
+    const cplx<R>* a = A + comp * g.cA + (long)x * g.sx + cz;
+    cplx<R> v[F::E];
+#pragma unroll
+    for (int i = 0; i < F::E; i++) {
+        const int l = F::in_pos(t, i);
+        v[i] = (ok && lkept(g.my, l)) ? a[(long)lq(g.my, l) * g.KZP] : czero<R>();
+    }
+    F::template run<T>(v, lds + c, t, tw);
+    if (ok) {
+        cplx<R>* b = B + comp * g.cB + (long)x * g.sB + cz;
+#pragma unroll
+        for (int i = 0; i < F::E; i++) b[(long)F::out_pos(t, i) * g.KZP] = v[i];
+    }
+}
+
+// pass 3: fused z.  Input lines hold kz < nkz; c2r to the z-packed real line, products, r2c.
+// MODE 0 (NS):  in u0..u2,            out u0u0 u1u1 u2u2 u0u1 u1u2 u2u0          (RSprod.cc:52-77)
+// MODE 1 (MHD): in Z+0..2, Z-0..2,    out P(i,j) = Z+_i Z-_j at 3i+j             (RSprod.cc:117-151)
+// MODE 2 (ET):  in H0..2,  G0..2,     out P(i,j) = G_i H_j at 3i+j               (ET_RSprod.cc:49-85)
+template <int MODE> struct ZProd;
+template <> struct ZProd<0> {
+    static constexpr int NIN = 3, NOUT = 6;
+    static constexpr int a(int p) { return p < 3 ? p : p - 3; }
+    static constexpr int b(int p) { return p < 3 ? p : (p == 5 ? 0 : p - 2); }
+};
+template <> struct ZProd<1> {
+    static constexpr int NIN = 6, NOUT = 9;
+    static constexpr int a(int p) { return p / 3; }
+    static constexpr int b(int p) { return 3 + p % 3; }
+};
+template <> struct ZProd<2> {
+    static constexpr int NIN = 6, NOUT = 9;
+    static constexpr int a(int p) { return 3 + p / 3; }
+    static constexpr int b(int p) { return p % 3; }
+};
+
+template <typename R> __device__ __forceinline__ cplx<R> shfl_c(const cplx<R>& v, int src, int width)
+{
+    return mk<R>(__shfl(v.x, src, width), __shfl(v.y, src, width));
+}
+
+// Lane t of a z line holds the points at positions t + P m, m = 0..E-1, in a register order fixed
+// by the radix Rr of the FFT stage that reads (first stage) or writes (last stage) them.
+template <int E, int Rr> __device__ __forceinline__ constexpr int zreg(int m)
+{
+    return (m % (E / Rr)) * Rr + m / (E / Rr);
+}
+
+template <typename R, int LZ, int P, int TZ, int MODE>
+__global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ Cout,
+                                               const cplx<R>* __restrict__ twL, const cplx<R>* __restrict__ twN)
+{
+    using FI = LineFFT<R, LZ, P, +1>;
+    using FF = LineFFT<R, LZ, P, -1>;
+    using ZP = ZProd<MODE>;
+    constexpr int E = FI::E;
+    static_assert(64 % P == 0, "a z line's threads must share one wavefront");
+    constexpr int LZS = ZPad<LZ>::line;   // padded line stride in LDS
+    __shared__ cplx<R> lds[TZ * LZS];
+    const int t = threadIdx.x % P, c = threadIdx.x / P;
+    const long q = (long)blockIdx.x * TZ + c;
+    const bool ok = q < (long)g.N1 * g.N2;
+    const long off = q * g.KZP;
+    B += blockIdx.y * g.bB;
+    Cout += blockIdx.y * g.bC;
+    cplx<R>* Z = lds + c * LZS;
+    // The mirror point LZ - (t + P m) sits in lane (P - t) mod P at m' = E - 1 - m (t > 0) or
+    // m' = (E - m) mod E (t = 0): the half-length packing reads it by a lane shuffle.
+    const int tp = (P - t) % P;
+    cplx<R> twk[E];   // W_N3^(t + P m)
+#pragma unroll
+    for (int m = 0; m < E; m++) twk[m] = twN[t + P * m];
+    cplx<R> u[ZP::NIN][E];
+#pragma unroll
+    for (int a = 0; a < ZP::NIN; a++) {
+        const cplx<R>* X = B + a * g.cB + off;
+        cplx<R> xa[E];
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int k = t + P * m;
+            xa[m] = (ok && k < g.nkz) ? X[k] : czero<R>();
+        }
+        const cplx<R> xny = (ok && t == 0 && LZ < g.nkz) ? X[LZ] : czero<R>();   // kz = N3/2 (undealiased)
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            cplx<R> xr = shfl_c(xa[E - 1 - m], tp, P);
+            cplx<R> xb = (t == 0) ? (m == 0 ? xny : xa[(E - m) % E]) : xr;
+            cplx<R> xv = xa[m];
+            if (t == 0 && m == 0) { xv.y = R(0); xb.y = R(0); }
+            xb.y = -xb.y;
+            cplx<R> s = xv + xb, d = xv - xb;
+            cplx<R> w = conjg(twk[m]);
+            u[a][zreg<E, FI::R0>(m)] = s + mul_i(cmul(w, d));
+        }
+        FI::template run<1, true, ZPad<LZ>>(u[a], Z, t, twL);
+    }
+    // One product line: forward half-length FFT, r2c unpacking, store of the kept kz.
+    auto product_out = [&](cplx<R> (&w)[E], int p) {
+        if constexpr (FI::R0 != FI::RL) {
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < E; i++) Z[FI::out_pos(t, i)] = w[i];
+            wave_sync();
+#pragma unroll
+            for (int i = 0; i < E; i++) w[i] = Z[FF::in_pos(t, i)];
+        }
+        FF::template run<1, true, ZPad<LZ>>(w, Z, t, twL);
+        const cplx<R>* twp = twN + opaque_zero();   // W_N3^(t + P m), reloaded per product (L1)
+        cplx<R>* X = Cout + p * g.cB + off;
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+            const int k = t + P * m;
+            cplx<R> yr = shfl_c(w[zreg<E, FF::RL>(E - 1 - m)], tp, P);
+            cplx<R> zk = w[zreg<E, FF::RL>(m)];
+            cplx<R> zc = conjg((t == 0) ? w[zreg<E, FF::RL>((E - m) % E)] : yr);
+            cplx<R> e = cscale(zk + zc, R(0.5));
+            cplx<R> tt = cmul(twp[t + P * m], zk - zc);
+            if (ok && k < g.nkz) X[k] = mk<R>(e.x + R(0.5) * tt.y, e.y - R(0.5) * tt.x);
+        }
+        if (ok && t == 0 && LZ < g.nkz) {   // kz = N3/2 (undealiased): Y[0] - ... with W_N3^(N3/2) = -1
+            cplx<R> z0 = w[zreg<E, FF::RL>(0)];
+            X[LZ] = mk<R>(z0.x - z0.y, R(0));
+        }
+    };
+    if constexpr (sizeof(R) == 4) {
+        // FP32: a rolled loop with operands picked by selects keeps the compiler from
+        // interleaving products and hoisting their twiddles (registers, occupancy).
+#pragma unroll 1
+        for (int p = 0; p < ZP::NOUT; p++) {
+            const int ia = ZP::a(p), ib = ZP::b(p);
+            cplx<R> w[E];
+#pragma unroll
+            for (int i = 0; i < E; i++) {
+                cplx<R> x = u[0][i], y = u[0][i];
+#pragma unroll
+                for (int a = 1; a < ZP::NIN; a++) {
+                    x = (ia == a) ? u[a][i] : x;
+                    y = (ib == a) ? u[a][i] : y;
+                }
+                w[i] = mk<R>(x.x * y.x, x.y * y.y);
+            }
+            product_out(w, p);
+        }
+    } else {
+        // FP64: selects over the arrays would become dynamic indexing (scratch); unroll instead
+#pragma unroll
+        for (int p = 0; p < ZP::NOUT; p++) {
+            const int ia = ZP::a(p), ib = ZP::b(p);
+            cplx<R> w[E];
+#pragma unroll
+            for (int i = 0; i < E; i++) w[i] = mk<R>(u[ia][i].x * u[ib][i].x, u[ia][i].y * u[ib][i].y);
+            product_out(w, p);
+        }
+    }
+}
+
+// pass 4: y-forward, C[x][y][cz] -> D[x][cy][cz] (kept ky only)
+template <typename R, int L, int P, int T>
+__global__ __launch_bounds__(T * P) void k_fy_fwd(FG g, const cplx<R>* __restrict__ Cin, cplx<R>* __restrict__ D,
+                                                  const cplx<R>* __restrict__ tw, int ncomp)
+{
+    using F = LineFFT<R, L, P, -1>;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int cz = blockIdx.x * T + c;
+    const bool ok = cz < g.nkz;
+    const int x = blockIdx.y, comp = blockIdx.z % ncomp, box = blockIdx.z / ncomp;
+    Cin += box * g.bC;
+    D += box * g.bA;
+    const cplx<R>* a = Cin + comp * g.cB + (long)x * g.sB + cz;
+    cplx<R> v[F::E];
+#pragma unroll
+    for (int i = 0; i < F::E; i++) v[i] = ok ? a[(long)F::in_pos(t, i) * g.KZP] : czero<R>();
+    F::template run<T>(v, lds + c, t, tw);
+    if (ok) {
+        cplx<R>* d = D + comp * g.cA + (long)x * g.sx + cz;
+#pragma unroll
+        for (int i = 0; i < F::E; i++) {
+            const int l = F::out_pos(t, i);
+            if (lkept(g.my, l)) d[(long)lq(g.my, l) * g.KZP] = v[i];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// pass 5: x-forward of the product lines with the nonlinear-term contraction accumulated in
+// registers, then the projection and the IF-RK4 stage update.
+//   MODE 0 (NS):  N_i = sum_j i K_j P_ij                         (Derivative_RSprod_VV RSprod.cc:94-105)
+//   MODE 1 (MHD): N+_i = sum_j i K_j P(i,j), N-_i = sum_j i K_j P(j,i); N_u = (N+ + N-)/2,
+//                 N_b = (N+ - N-)/2 (Elsasser_to_UB_nlin elsasser.cc:117-130); U projected, B not.
+// ------------------------------------------------------------------------------------------
+template <typename R> __device__ __forceinline__ void acc_iK(cplx<R>& n, R K, const cplx<R>& v)
+{
+    n.x -= K * v.y;
+    n.y += K * v.x;
+}
+
+struct Dc {
+    double x, y;
+};
+__device__ __forceinline__ Dc dmk(double a, double b) { Dc r; r.x = a; r.y = b; return r; }
+template <typename R> __device__ __forceinline__ Dc dl(const cplx<R>& a) { return dmk((double)a.x, (double)a.y); }
+template <typename R> __device__ __forceinline__ cplx<R> ds(const Dc& a) { return mk<R>((R)a.x, (R)a.y); }
+__device__ __forceinline__ Dc dsc(const Dc& a, double s) { return dmk(a.x * s, a.y * s); }
+__device__ __forceinline__ Dc dadd(const Dc& a, const Dc& b) { return dmk(a.x + b.x, a.y + b.y); }
+
+// One RK4 stage update of a field value V0 (state), stage field Vs and accumulator tot
+// (single_time_step.cc:54-86 with a,b,c of time_advance.cc:120-153).
+template <typename R>
+__device__ __forceinline__ void rk_update(int stage, double dt, double eh, double e1, const Dc& rhs, cplx<R>* S,
+                                          cplx<R>* Vs, cplx<R>* acc)
+{
+    Dc V0 = dl(*S);
+    if (stage == 1) {
+        Dc v = dadd(V0, dsc(rhs, 0.5 * dt));
+        *Vs = ds<R>(dsc(v, eh));
+        *acc = ds<R>(dsc(dsc(rhs, e1), dt / 6));
+    } else if (stage == 2) {
+        *Vs = ds<R>(dadd(dsc(V0, eh), dsc(rhs, 0.5 * dt)));
+        Dc a = dl(*acc);
+        *acc = ds<R>(dadd(a, dsc(dsc(rhs, eh), dt / 3)));
+    } else if (stage == 3) {
+        Dc v = dadd(dsc(V0, eh), dsc(rhs, 1.0 * dt));
+        *Vs = ds<R>(dsc(v, eh));
+        Dc a = dl(*acc);
+        *acc = ds<R>(dadd(a, dsc(dsc(rhs, eh), dt / 3)));
+    } else {
+        Dc a = dadd(dl(*acc), dsc(rhs, dt / 6));
+        *S = ds<R>(dadd(dsc(V0, e1), a));
+    }
+}
+
+template <typename R, int L, int P, int T, int MODE>
+__global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 0 && sizeof(R) == 4 && L % 3 != 0) ? 4 : 1))) void k_fx_fwd(FG g, const cplx<R>* __restrict__ D, cplx<R>* S, cplx<R>* Vs,
+                                                  cplx<R>* acc, const cplx<R>* __restrict__ tw,
+                                                  const StageCoef* __restrict__ coefs)
+{
+    using F = LineFFT<R, L, P, -1>;
+    constexpr int E = F::E;
+    constexpr int NP = (MODE == 0) ? 6 : 9;
+    constexpr int NA = (MODE == 0) ? 3 : 6;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int ntz = (g.nkz + T - 1) / T;
+    const int lin = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int tile = lin % gridDim.x, box = lin / gridDim.x;
+    const int cy = tile / ntz;
+    const int cz = (tile % ntz) * T + c;
+    const bool ok = cz < g.nkz;
+    const long m = (long)cy * g.KZP + cz;
+    const StageCoef sc = coefs[box];
+    D += box * g.bA;
+    S += box * g.bS;
+    Vs += box * g.bS;
+    acc += box * g.bS;
+    const R Ky = (R)(ky_of(ll(g.my, cy), g.N2) * g.f2);
+    const R Kz = (R)(cz * g.f3);
+    cplx<R> Nacc[NA][E];
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+        for (int i = 0; i < E; i++) Nacc[a][i] = czero<R>();
+    // The product loop stays rolled (one product line live at a time).  Each product p adds
+    // i (alpha_a Kx + beta_a) P_p to accumulator a, so accumulators are indexed statically.
+    // Addresses split into a uniform row base (SGPRs) and one 32-bit lane offset: lane (c, t)
+    // reads column cz = tile * T + c of row in_pos(t, i) = t + in_pos(0, i).
+    const long mu = (long)cy * g.KZP + (long)(tile % ntz) * T;
+    const int lane_off = c + t * (int)g.sx;
+#pragma unroll 1
+    for (int p = 0; p < NP; p++) {
+        cplx<R> v[E];
+        const cplx<R>* src = D + p * g.cA + mu;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const cplx<R>* row = src + (long)F::in_pos(0, i) * g.sx;
+            v[i] = ok ? row[lane_off] : czero<R>();
+        }
+        F::template run<T>(v, lds + c, t, tw + opaque_zero());
+        // contributions (a1 <- K_k1) and (a2 <- K_k2) of product p; k = 0 means Kx (per row)
+        int a1, k1, a2, k2;
+        if (MODE == 0) {           // P_ij, (i,j) = (0,0),(1,1),(2,2),(0,1),(1,2),(2,0): N_i += iK_j P, N_j += iK_i P
+            const int pi = p < 3 ? p : p - 3, pj = p < 3 ? p : (p == 5 ? 0 : p - 2);
+            a1 = pi; k1 = pj;
+            a2 = (pi != pj) ? pj : -1; k2 = pi;
+        } else {                   // P(pi,pj) = Z+_pi Z-_pj: N+_pi += iK_pj P, N-_pj += iK_pi P
+            const int pi = p / 3, pj = p % 3;
+            a1 = pi; k1 = pj;
+            a2 = 3 + pj; k2 = pi;
+        }
+        R al[NA], be[NA];
+#pragma unroll
+        for (int a = 0; a < NA; a++) {
+            al[a] = R(0);
+            be[a] = R(0);
+            if (a == a1) { if (k1 == 0) al[a] += R(1); else be[a] += (k1 == 1 ? Ky : Kz); }
+            if (a == a2) { if (k2 == 0) al[a] += R(1); else be[a] += (k2 == 1 ? Ky : Kz); }
+        }
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            if (xdrop<L, P, E, F::RL>(i)) continue;
+            const R Kx = (R)(kx_four(F::out_pos(t, i), g.N1) * g.f1);
+#pragma unroll
+            for (int a = 0; a < NA; a++) acc_iK(Nacc[a][i], al[a] * Kx + be[a], v[i]);
+        }
+    }
+    if (!ok) return;
+    const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int l = F::out_pos(t, i);
+        if (xdrop<L, P, E, F::RL>(i) || !lkept(g.mx, l)) continue;
+        const double dKx = kx_four(l, g.N1) * g.f1;
+        const double K2 = ksq_f(dKx, dKy, dKz);
+        // compact row of l = t + l0: the kept lanes of one register all sit on one side of the
+        // dealiased gap (P < L/3), so the row shift is uniform per register
+        const int l0 = F::out_pos(0, i);
+        const long idx = (long)(l0 - (l0 >= g.mx.h ? g.mx.N - g.mx.nk : 0)) * g.sx + mu + lane_off;
+        Dc Nu[3];
+        if (MODE == 0) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) Nu[a] = dsc(dl(Nacc[a][i]), g.norm);
+        } else {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
+                Nu[a] = dsc(dadd(np, nm), 0.5 * g.norm);
+            }
+        }
+        Dc rhs[3];
+        if (K2 > 0) {
+            Dc kn = dmk(dKx * Nu[0].x + dKy * Nu[1].x + dKz * Nu[2].x, dKx * Nu[0].y + dKy * Nu[1].y + dKz * Nu[2].y);
+            const double Kc[3] = {dKx, dKy, dKz};
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+                rhs[a] = dmk(-(Nu[a].x - Kc[a] * kn.x / K2), -(Nu[a].y - Kc[a] * kn.y / K2));
+        } else {
+#pragma unroll
+            for (int a = 0; a < 3; a++) rhs[a] = dmk(-Nu[a].x, -Nu[a].y);
+        }
+        const double eh = exp(sc.fh[0] * K2), e1 = exp(sc.f1[0] * K2);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            rk_update<R>(sc.stage, sc.dt, eh, e1, rhs[a], S + a * g.cS + idx, Vs + a * g.cS + idx, acc + a * g.cS + idx);
+        if (MODE == 1) {
+            const double ehb = exp(sc.fh[1] * K2), e1b = exp(sc.f1[1] * K2);
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
+                Dc nb = dsc(dmk(np.x - nm.x, np.y - nm.y), 0.5 * g.norm);
+                Dc rb = dmk(-nb.x, -nb.y);
+                rk_update<R>(sc.stage, sc.dt, ehb, e1b, rb, S + (3 + a) * g.cS + idx, Vs + (3 + a) * g.cS + idx,
+                             acc + (3 + a) * g.cS + idx);
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);   // one mode at a time: bounds the live state loads
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// energy-transfer epilogue: x-forward of the 9 ET product lines, N_i = sum_j i K_j F(H_j G_i), then
+// 2 mult Re(N . U*) binned by shell (Get_shell_index over radii) separately for |K| <= kmax_inside
+// (Shell_mult_all) and |K| > kmax_inside (which only Shell_mult_single counts).
+// ------------------------------------------------------------------------------------------
+template <typename R, int L, int P, int T>
+__global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __restrict__ D, const cplx<R>* __restrict__ U,
+                                                     const cplx<R>* __restrict__ tw, const double* __restrict__ radii,
+                                                     int nr, int kin, double* __restrict__ partial)
+{
+    using F = LineFFT<R, L, P, -1>;
+    constexpr int E = F::E;
+    __shared__ cplx<R> lds[L * T];
+    extern __shared__ double bins[];      // 2 nr doubles: |K| <= kin binned by shell, then |K| > kin by shell
+    for (int b = threadIdx.x; b < 2 * nr; b += T * P) bins[b] = 0.0;
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int ntz = (g.nkz + T - 1) / T;
+    const int cy = blockIdx.x / ntz;
+    const int cz = (blockIdx.x % ntz) * T + c;
+    const bool ok = cz < g.nkz;
+    const long m = (long)cy * g.KZP + cz;
+    const R Ky = (R)(ky_of(ll(g.my, cy), g.N2) * g.f2);
+    const R Kz = (R)(cz * g.f3);
+    cplx<R> Nacc[3][E];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int i = 0; i < E; i++) Nacc[a][i] = czero<R>();
+#pragma unroll 1
+    for (int p = 0; p < 9; p++) {
+        cplx<R> v[E];
+        const cplx<R>* src = D + p * g.cA + m;
+#pragma unroll
+        for (int i = 0; i < E; i++) v[i] = ok ? src[(long)F::in_pos(t, i) * g.sx] : czero<R>();
+        F::template run<T>(v, lds + c, t, tw + opaque_zero());
+        const int pi = p / 3, pj = p % 3;        // P(pi,pj) = G_pi H_pj -> N_pi += i K_pj P
+        R al[3], be[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            al[a] = (a == pi && pj == 0) ? R(1) : R(0);
+            be[a] = (a == pi && pj != 0) ? (pj == 1 ? Ky : Kz) : R(0);
+        }
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            if (xdrop<L, P, E, F::RL>(i)) continue;
+            const R Kx = (R)(kx_four(F::out_pos(t, i), g.N1) * g.f1);
+#pragma unroll
+            for (int a = 0; a < 3; a++) acc_iK(Nacc[a][i], al[a] * Kx + be[a], v[i]);
+        }
+    }
+    __syncthreads();
+    if (ok) {
+        const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
+        const int l2 = ll(g.my, cy);
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int l = F::out_pos(t, i);
+            if (xdrop<L, P, E, F::RL>(i) || !lkept(g.mx, l)) continue;
+            const int kx = kx_four(l, g.N1);
+            const double km = kmag_f(kx * g.f1, dKy, dKz);
+            // Multiplicity_factor_FOUR four_inline.h:347-365
+            double f = (cz > 0) ? 1.0 : 0.5;
+            if (kx == g.N1 / 2 || ky_of(l2, g.N2) == g.N2 / 2) f *= 2;
+            const long idx = (long)lq(g.mx, l) * g.sx + m;
+            double s = 0;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                cplx<R> u = U[a * g.cS + idx];
+                s += (double)Nacc[a][i].x * (double)u.x + (double)Nacc[a][i].y * (double)u.y;
+            }
+            s *= 2 * f * g.norm;
+            int b = 0;
+            while (b < nr && !(km <= radii[b])) b++;
+            if (b >= nr) continue;
+            atomicAdd(&bins[(km <= (double)kin) ? b : nr + b], s);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 2 * nr; b += T * P) partial[(long)blockIdx.x * (2 * nr) + b] = bins[b];
+}
+
+// compact <-> reference layout
+template <typename R>
+__global__ void k_gather(FG g, const cplx<R>* __restrict__ ref, cplx<R>* __restrict__ S)
+{
+    long n = (long)g.mx.nk * g.sx;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int cz = (int)(i % g.KZP);
+        long r = i / g.KZP;
+        int cy = (int)(r % g.my.nk), cx = (int)(r / g.my.nk);
+        if (cz >= g.nkz) { S[i] = czero<R>(); continue; }
+        int l1 = ll(g.mx, cx), l2 = ll(g.my, cy);
+        S[i] = ref[((long)l1 * g.N2 + l2) * (g.N3 / 2 + 1) + cz];
+    }
+}
+template <typename R>
+__global__ void k_scatter(FG g, const cplx<R>* __restrict__ S, cplx<R>* __restrict__ ref)
+{
+    const int H = g.N3 / 2 + 1;
+    long n = (long)g.N1 * g.N2 * H;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int l3 = (int)(i % H);
+        long r = i / H;
+        int l2 = (int)(r % g.N2), l1 = (int)(r / g.N2);
+        cplx<R> v = czero<R>();
+        if (l3 < g.nkz && lkept(g.mx, l1) && lkept(g.my, l2)) v = S[(long)lq(g.mx, l1) * g.sx + (long)lq(g.my, l2) * g.KZP + l3];
+        ref[i] = v;
+    }
+}
+
+template <typename R> constexpr int tileT() { return 128 / (int)sizeof(cplx<R>); }
+
+// kernel dispatch on line length
+#define TH_FLENS(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+#define TH_ZLENS(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+
+// launch shape of variant V of an x/y line pass
+constexpr int alt_pts(int L) { return (L % 3 == 0) ? 24 : (pts_per_thread(L) == 16 ? 8 : 16); }
+template <int L, int V> constexpr int var_E()
+{
+    return ((V & 2) && L >= 128 && L % alt_pts(L) == 0) ? alt_pts(L) : pts_per_thread(L);
+}
+template <typename R, int L, int V> constexpr int var_T() { return ((V & 1) && L >= 128) ? tileT<R>() / 2 : tileT<R>(); }
+
+template <typename R, int L, int MODE, int V>
+static int launch_xinv(const FG& g, const cplx<R>* S, cplx<R>* A, int ncomp, const cplx<R>* tw, ShellSel sel, hipStream_t st)
+{
+    constexpr int E = var_E<L, V>(), P = L / E, T = var_T<R, L, V>();
+    const int ntz = (g.nkz + T - 1) / T;
+    hipLaunchKernelGGL((k_fx_inv<R, L, P, T, MODE>), dim3(g.my.nk * ntz, ncomp, g.nbox), dim3(T * P), 0, st, g, S, A, tw, sel);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int L, int V>
+static int launch_yinv(const FG& g, const cplx<R>* A, cplx<R>* B, int ncomp, const cplx<R>* tw, hipStream_t st)
+{
+    constexpr int E = var_E<L, V>(), P = L / E, T = var_T<R, L, V>();
+    hipLaunchKernelGGL((k_fy_inv<R, L, P, T>), dim3((g.nkz + T - 1) / T, g.N1, ncomp * g.nbox), dim3(T * P), 0, st, g, A, B,
+                       tw, ncomp);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int LZ, int MODE, int V>
+static int launch_z(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* twL, const cplx<R>* twN, hipStream_t st)
+{
+    constexpr int E = pts_per_thread(LZ), P = LZ / E;
+    constexpr int NT = (V == 1) ? 128 : (V == 2) ? 512 : 256;
+    constexpr int TZ = (NT / P) < 1 ? 1 : NT / P;
+    const long lines = (long)g.N1 * g.N2;
+    hipLaunchKernelGGL((k_fz<R, LZ, P, TZ, MODE>), dim3((unsigned)((lines + TZ - 1) / TZ), g.nbox), dim3(TZ * P), 0, st, g, B, Cc,
+                       twL, twN);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int L, int V>
+static int launch_yfwd(const FG& g, const cplx<R>* Cc, cplx<R>* D, int ncomp, const cplx<R>* tw, hipStream_t st)
+{
+    constexpr int E = var_E<L, V>(), P = L / E, T = var_T<R, L, V>();
+    hipLaunchKernelGGL((k_fy_fwd<R, L, P, T>), dim3((g.nkz + T - 1) / T, g.N1, ncomp * g.nbox), dim3(T * P), 0, st, g, Cc, D,
+                       tw, ncomp);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+// The x-forward pass keeps 3 (NS) or 6 (MHD) accumulator lines per thread next to the FFT line, so
+// it uses fewer points per thread than the other passes to stay inside the VGPR file (no spills).
+constexpr int xfwd_E(int L) { return L < 8 ? L : ((L % 3 == 0) ? (L >= 12 ? 12 : L) : 8); }
+template <typename R, int L, int MODE> constexpr int xfwd_T()
+{
+    constexpr int P = L / xfwd_E(L);
+    constexpr int t = (MODE == 0 && sizeof(R) == 4) ? 16 : 8;
+    return (t * P > 512) ? (512 / P < 1 ? 1 : 512 / P) : t;
+}
+
+template <typename R, int L, int MODE, int V>
+static int launch_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
+                       const StageCoef* sc, hipStream_t st)
+{
+    constexpr int E = xfwd_E(L), P = L / E, T0 = xfwd_T<R, L, MODE>();
+    constexpr int T = (V == 1 && L >= 128 && T0 > 1) ? T0 / 2 : T0;
+    const int ntz = (g.nkz + T - 1) / T;
+    hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D, S, Vs, acc, tw,
+                       sc);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int L>
+static int launch_xfwd_et(const FG& g, const cplx<R>* D, const cplx<R>* U, const cplx<R>* tw, const double* radii, int nr,
+                          int kin, double* partial, int* nblk, hipStream_t st)
+{
+    constexpr int E = xfwd_E(L), P = L / E, T = xfwd_T<R, L, 0>();
+    const int ntz = (g.nkz + T - 1) / T;
+    *nblk = g.my.nk * ntz;
+    if (!partial) return OK;
+    hipLaunchKernelGGL((k_fx_fwd_et<R, L, P, T>), dim3(*nblk), dim3(T * P), 2 * nr * sizeof(double), st, g, D, U, tw,
+                       radii, nr, kin, partial);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+
+// variant dispatch per line length (short lines have one shape)
+template <typename R, int MODE, int L>
+static int xinv_v(const FG& g, const cplx<R>* S, cplx<R>* A, int nc, const cplx<R>* tw, ShellSel sel, int var, hipStream_t st)
+{
+    if constexpr (L >= 128) {
+        if (var == 1) return launch_xinv<R, L, MODE, 1>(g, S, A, nc, tw, sel, st);
+        if (var == 2) return launch_xinv<R, L, MODE, 2>(g, S, A, nc, tw, sel, st);
+        if (var == 3) return launch_xinv<R, L, MODE, 3>(g, S, A, nc, tw, sel, st);
+    }
+    return launch_xinv<R, L, MODE, 0>(g, S, A, nc, tw, sel, st);
+}
+template <typename R, int L>
+static int yinv_v(const FG& g, const cplx<R>* A, cplx<R>* B, int nc, const cplx<R>* tw, int var, hipStream_t st)
+{
+    if constexpr (L >= 128) {
+        if (var == 1) return launch_yinv<R, L, 1>(g, A, B, nc, tw, st);
+        if (var == 2) return launch_yinv<R, L, 2>(g, A, B, nc, tw, st);
+        if (var == 3) return launch_yinv<R, L, 3>(g, A, B, nc, tw, st);
+    }
+    return launch_yinv<R, L, 0>(g, A, B, nc, tw, st);
+}
+template <typename R, int MODE, int LZ>
+static int z_v(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* twL, const cplx<R>* twN, int var, hipStream_t st)
+{
+    if constexpr (LZ >= 64) {
+        if (var == 1) return launch_z<R, LZ, MODE, 1>(g, B, Cc, twL, twN, st);
+        // 512-thread workgroups cap a wave at 128 VGPRs: only the FP32 NS radix-2^k lines fit
+        if constexpr (sizeof(R) == 4 && MODE == 0 && LZ % 3 != 0)
+            if (var == 2) return launch_z<R, LZ, MODE, 2>(g, B, Cc, twL, twN, st);
+    }
+    return launch_z<R, LZ, MODE, 0>(g, B, Cc, twL, twN, st);
+}
+template <typename R, int L>
+static int yfwd_v(const FG& g, const cplx<R>* Cc, cplx<R>* D, int nc, const cplx<R>* tw, int var, hipStream_t st)
+{
+    if constexpr (L >= 128) {
+        if (var == 1) return launch_yfwd<R, L, 1>(g, Cc, D, nc, tw, st);
+        if (var == 2) return launch_yfwd<R, L, 2>(g, Cc, D, nc, tw, st);
+        if (var == 3) return launch_yfwd<R, L, 3>(g, Cc, D, nc, tw, st);
+    }
+    return launch_yfwd<R, L, 0>(g, Cc, D, nc, tw, st);
+}
+template <typename R, int MODE, int L>
+static int xfwd_v(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw, const StageCoef* sc,
+                  int var, hipStream_t st)
+{
+    if constexpr (L >= 128)
+        if (var == 1) return launch_xfwd<R, L, MODE, 1>(g, D, S, Vs, acc, tw, sc, st);
+    return launch_xfwd<R, L, MODE, 0>(g, D, S, Vs, acc, tw, sc, st);
+}
+
+template <typename R, int MODE>
+int d_xinv(const FG& g, const cplx<R>* S, cplx<R>* A, int nc, const cplx<R>* tw, ShellSel sel, int var, hipStream_t st)
+{
+#define C_(n) case n: return xinv_v<R, MODE, n>(g, S, A, nc, tw, sel, var, st);
+    switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R>
+int d_yinv(const FG& g, const cplx<R>* A, cplx<R>* B, int nc, const cplx<R>* tw, int var, hipStream_t st)
+{
+#define C_(n) case n: return yinv_v<R, n>(g, A, B, nc, tw, var, st);
+    switch (g.N2) { TH_FLENS(C_) default: set_error("fused: N2 %d unsupported", g.N2); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R, int MODE>
+int d_z(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* twL, const cplx<R>* twN, int var, hipStream_t st)
+{
+#define C_(n) case n: return z_v<R, MODE, n>(g, B, Cc, twL, twN, var, st);
+    switch (g.LZ) { TH_ZLENS(C_) default: set_error("fused: N3 %d unsupported", g.N3); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R>
+int d_yfwd(const FG& g, const cplx<R>* Cc, cplx<R>* D, int nc, const cplx<R>* tw, int var, hipStream_t st)
+{
+#define C_(n) case n: return yfwd_v<R, n>(g, Cc, D, nc, tw, var, st);
+    switch (g.N2) { TH_FLENS(C_) default: set_error("fused: N2 %d unsupported", g.N2); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R, int MODE>
+int d_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw, const StageCoef* sc,
+           int var, hipStream_t st)
+{
+#define C_(n) case n: return xfwd_v<R, MODE, n>(g, D, S, Vs, acc, tw, sc, var, st);
+    switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R>
+int d_xfwd_et(const FG& g, const cplx<R>* D, const cplx<R>* U, const cplx<R>* tw, const double* radii, int nr, int kin,
+              double* partial, int* nblk, hipStream_t st)
+{
+#define C_(n) case n: return launch_xfwd_et<R, n>(g, D, U, tw, radii, nr, kin, partial, nblk, st);
+    switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
+#undef C_
+}
+
+static dim3 grid_1d(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    return dim3((unsigned)(b ? b : 1));
+}
+template <typename R> int d_gather(const FG& g, const cplx<R>* ref, cplx<R>* S, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_gather<R>), grid_1d((size_t)g.mx.nk * g.sx), dim3(256), 0, st, g, ref, S);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R> int d_scatter(const FG& g, const cplx<R>* S, cplx<R>* ref, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_scatter<R>), grid_1d((size_t)g.N1 * g.N2 * (g.N3 / 2 + 1)), dim3(256), 0, st, g, S, ref);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+
+#define TH_FUSED_INSTANTIATE(R)                                                                                       \
+    template int d_xinv<R, 0>(const FG&, const cplx<R>*, cplx<R>*, int, const cplx<R>*, ShellSel, int, hipStream_t);  \
+    template int d_xinv<R, 1>(const FG&, const cplx<R>*, cplx<R>*, int, const cplx<R>*, ShellSel, int, hipStream_t);  \
+    template int d_xinv<R, 2>(const FG&, const cplx<R>*, cplx<R>*, int, const cplx<R>*, ShellSel, int, hipStream_t);  \
+    template int d_yinv<R>(const FG&, const cplx<R>*, cplx<R>*, int, const cplx<R>*, int, hipStream_t);              \
+    template int d_z<R, 0>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, int, hipStream_t);   \
+    template int d_z<R, 1>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, int, hipStream_t);   \
+    template int d_z<R, 2>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, int, hipStream_t);   \
+    template int d_yfwd<R>(const FG&, const cplx<R>*, cplx<R>*, int, const cplx<R>*, int, hipStream_t);              \
+    template int d_xfwd<R, 0>(const FG&, const cplx<R>*, cplx<R>*, cplx<R>*, cplx<R>*, const cplx<R>*,              \
+                              const StageCoef*, int, hipStream_t);                                                     \
+    template int d_xfwd<R, 1>(const FG&, const cplx<R>*, cplx<R>*, cplx<R>*, cplx<R>*, const cplx<R>*,              \
+                              const StageCoef*, int, hipStream_t);                                                     \
+    template int d_xfwd_et<R>(const FG&, const cplx<R>*, const cplx<R>*, const cplx<R>*, const double*, int, int,    \
+                              double*, int*, hipStream_t);                                                             \
+    template int d_gather<R>(const FG&, const cplx<R>*, cplx<R>*, hipStream_t);                                        \
+    template int d_scatter<R>(const FG&, const cplx<R>*, cplx<R>*, hipStream_t);
+
+}  // namespace th

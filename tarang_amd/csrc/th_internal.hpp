@@ -11,9 +11,18 @@
 struct th_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    std::string tune_path;
+    std::string tune_path;                     // tune cache file ("" = in memory only)
+    std::mutex tune_mu;
+    std::map<std::string, std::vector<int>> tune;   // launch-shape choices by workload key
     std::atomic<int> nplans{0};
 };
+
+namespace th {
+// Tune cache: one line per workload, "<key> <v0> <v1> ...", read when the context is created and
+// rewritten whenever the autotuner adds an entry.
+bool tune_lookup(th_ctx* c, const std::string& key, std::vector<int>& v);
+void tune_store(th_ctx* c, const std::string& key, const std::vector<int>& v);
+}  // namespace th
 
 namespace th {
 

@@ -1,0 +1,145 @@
+"""Launch-shape variants and the autotuner (th_tune, th_state_set_variants, th_ensemble_tune):
+every variant of every fused pass computes the same RK4 step as the oracle, tuning leaves the
+state unchanged, and the tune cache carries the choice to a new context."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from pyoracle import Oracle  # noqa: E402
+
+NVAR = [4, 4, 3, 4, 2]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    from tarang_amd import Context
+    return Context(0)
+
+
+def H(t):
+    return t.detach().cpu().numpy().astype(np.complex128)
+
+
+def rel(a, b):
+    m = np.abs(b).max()
+    return np.abs(a - b).max() / (m if m > 0 else 1.0)
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ref128():
+    N = (128, 128, 128)
+    o = Oracle(N)
+    V0 = o.ic_random(31)
+    V = [a.copy() for a in V0]
+    o.rk4_ns(V, 2e-3, 1e-3, steps=1)
+    return N, V0, V
+
+
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
+def test_variants_match_oracle_fp64(ctx, ref128, v):
+    from tarang_amd import Plan, State
+    N, V0, V1 = ref128
+    var = [min(v, n - 1) for n in NVAR]
+    log(f"variants {var} fp64 N={N}")
+    p = Plan(ctx, N)
+    s = State(p)
+    s.variants = var
+    assert s.variants == var
+    s.upload([p.to_dev(a) for a in V0])
+    s.step(1e-3, nu=2e-3, steps=1)
+    out = s.download()
+    for i in range(3):
+        assert rel(H(out[i]), V1[i]) < 1e-12, (var, i)
+
+
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
+def test_variants_fp32_mixed(ctx, ref128, v):
+    from tarang_amd import Plan, State
+    N, V0, V1 = ref128
+    var = [min(v, n - 1) for n in NVAR]
+    log(f"variants {var} fp32 N={N}")
+    p = Plan(ctx, N, precision="fp32")
+    s = State(p)
+    s.variants = var
+    s.upload([p.to_dev(a) for a in V0])
+    s.step(1e-3, nu=2e-3, steps=1)
+    out = s.download()
+    for i in range(3):
+        assert rel(H(out[i]), V1[i]) < 2e-5, (var, i)
+
+
+def test_tune_keeps_state_and_step_matches_oracle(ctx, ref128):
+    from tarang_amd import Plan, State
+    N, V0, V1 = ref128
+    p = Plan(ctx, N)
+    s = State(p)
+    s.upload([p.to_dev(a) for a in V0])
+    before = [H(a) for a in s.download()]
+    var = s.tune(1e-3, nu=2e-3, nrep=2)
+    log(f"tuned variants {var}")
+    assert all(0 <= x < n for x, n in zip(var, NVAR))
+    after = [H(a) for a in s.download()]
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    s.step(1e-3, nu=2e-3, steps=1)
+    out = s.download()
+    for i in range(3):
+        assert rel(H(out[i]), V1[i]) < 1e-12
+
+
+def test_tune_cache_round_trip(tmp_path):
+    import torch
+    assert torch.cuda.is_available()
+    from tarang_amd import Context, Plan, State
+    path = str(tmp_path / "tune.txt")
+    c1 = Context(0, tune_cache=path)
+    p = Plan(c1, (128, 128, 128), precision="fp32")
+    s = State(p)
+    p.ic_random(3)
+    s.upload(p.ic_random(3))
+    var = s.tune(1e-3, nu=1e-3, nrep=2)
+    assert os.path.exists(path)
+    text = open(path).read()
+    assert "fused:gfx950:fp32:ns:N128x128x128:boxes1" in text
+    s.close()
+    p.close()
+    c2 = Context(0, tune_cache=path)
+    p2 = Plan(c2, (128, 128, 128), precision="fp32")
+    s2 = State(p2)
+    assert s2.variants == var
+    s2.close()
+    p2.close()
+
+
+def test_ensemble_tune_then_step_matches_single_boxes(ctx):
+    from tarang_amd import Ensemble, Plan, State
+    shapes = [(128, 128, 128), (64, 64, 64), (128, 128, 128)]
+    dt = np.array([1e-3, 2e-3, 1.5e-3])
+    nu = np.array([1e-3, 3e-3, 2e-3])
+    ens = Ensemble(ctx, shapes, precision="fp64")
+    singles = []
+    for b, N in enumerate(shapes):
+        V = ens.plan(b).ic_random(100 + b)
+        ens.upload(b, V)
+        p = Plan(ctx, N)
+        s = State(p)
+        s.upload(V)
+        singles.append(s)
+    ens.tune(dt, nu, nrep=2)
+    ens.step(dt, nu, steps=2)
+    for b, s in enumerate(singles):
+        s.step(float(dt[b]), nu=float(nu[b]), steps=2)
+        got, ref = ens.download(b), s.download()
+        for i in range(3):
+            assert rel(H(got[i]), H(ref[i])) < 1e-12, (b, i)
