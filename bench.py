@@ -9,6 +9,7 @@ FP32-mixed run against the FP64 run after the same number of steps.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-fp64]
   python bench.py --workload ensemble [--steps K] [--warmup W] [--no-cpu]
+  python bench.py --workload mhd [--n 512]      python bench.py --workload rb [--rb-n 256]
 Prints one JSON line.  Runs on one GPU; --gpus > 1 exits with a message (single-GPU build).
 """
 import argparse
@@ -213,11 +214,115 @@ def bench_ensemble(args):
     print(json.dumps(line), flush=True)
 
 
+def _line(metric, value, args, ms_step, config, extra):
+    line = {"metric": metric, "value": round(value, 1), "unit": "Mpts*RKstage/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None}
+    line.update(extra)
+    line["config"] = config
+    return line
+
+
+def bench_mhd(args):
+    """512^3 incompressible MHD (Elsasser products), FP32-mixed fused step, nu = eta = 1e-3, U seed 2 and
+    B seed 3 with the random-phase k^4 exp(-2(k/4)^2) spectrum (E = 0.5 each), dt from CFL 0.5 on
+    max(|u|, |b|) at t = 0.  Algorithmic bytes per stage: 90 S (S = N^2 (N/2+1) x 8 B)."""
+    import numpy as np
+    import torch
+    from tarang_amd import Context, Plan, State, abi
+    n = args.n
+    N = (n, n, n)
+    nu = eta = 1e-3
+    ctx = Context(0)
+    p64 = Plan(ctx, N, precision="fp64")
+    p32 = Plan(ctx, N, precision="fp32")
+    U = p64.ic_random(2, 0.5, 4.0)
+    B = p64.ic_random(3, 0.5, 4.0)
+    dtu, umax = cfl_dt(p64, U)
+    dtb, bmax = cfl_dt(p64, B)
+    dt = min(dtu, dtb)
+    Eu, Eb = p64.total_energy(U), p64.total_energy(B)
+    Hc = sum(p64.energy(a, b) for a, b in zip(U, B))
+    s = State(p32, abi.TH_MHD)
+    s.upload([a.to(torch.complex64) for a in U + B])
+    del U, B
+    var = s.tune(dt, nu=nu, eta=eta, nrep=3) if not args.no_tune else s.variants
+    s.step(dt, nu=nu, eta=eta, steps=args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(dt, nu=nu, eta=eta, steps=args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms, nbytes = s.profile_passes(dt, nu=nu, eta=eta, stage=2, nrep=3)
+    pts = float(n) ** 3
+    value = pts * 4 * args.steps / el / 1e6
+    k = int(np.argmax(ms))
+    ach = nbytes[k] / (ms[k] * 1e-3) / 1e9
+    passes = {PASS_NAMES[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
+                              "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
+    out = s.download()
+    E1 = p32.total_energy(out[:3]) + p32.total_energy(out[3:])
+    print(json.dumps(_line(
+        f"Mpts*RKstage/s at {n}^3 MHD (Elsasser), FP32-mixed fused step", value, args, 1e3 * el / args.steps,
+        {"workload": f"mhd_{n}^3_rk4_elsasser", "N": n, "nu": nu, "eta": eta, "dt": dt, "cfl": 0.5,
+         "ic": "U seed 2, B seed 3, k^4 exp(-2(k/4)^2), E=0.5 each", "precision": "fp32-mixed"},
+        {"dtype": "fp32-mixed (fp32 storage/FFT; fp64 integrating factor, twiddles, sums)",
+         "data": "synthetic (seeded random-phase IC generated on the GPU)",
+         "ic_check": {"E_u": Eu, "E_b": Eb, "Hc_over_E": abs(Hc) / (Eu + Eb), "umax": umax, "bmax": bmax},
+         "energy_after": E1, "launch_variants": var,
+         "roofline": {"bound": "hbm", "kernel": PASS_NAMES[k], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
+         "stage": {"algorithmic_GB_per_stage": round(float(sum(nbytes)) / 1e9, 3), "passes": passes},
+         "cpu_baseline": None})), flush=True)
+
+
+def bench_rb(args):
+    """Rayleigh-Benard, SCFT basis with the sin/cos axis first, 256^3, kfactor (pi, pi/sqrt2, pi/sqrt2),
+    Pr = 1, Ra = 1e7 (PRLARGE/ULARGE scaling), V = 0, temperature perturbation amplitude 1e-3 (seed 4),
+    dt = 1e-3 fixed; RK4 on the plain GPU path (the reference's operation sequence), FP64 and FP32-mixed.
+    Reports Nu after the timed steps."""
+    import torch
+    from tarang_amd import Context, Plan, abi
+    n = args.rb_n
+    N = (n, n, n)
+    kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
+    Ra, Pr, dt = 1e7, 1.0, 1e-3
+    ctx = Context(0)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision=prec)
+        f = p.empty(3) + [p.ic_random_scalar(4, 1e-3)]
+        p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[prec] = (el, p.nusselt(f[:3], f[3], Ra, Pr))
+        del f, p
+    pts = float(n) ** 3
+    el, Nu = res["fp32"]
+    value = pts * 4 * args.steps / el / 1e6
+    v64 = pts * 4 * args.steps / res["fp64"][0] / 1e6
+    print(json.dumps(_line(
+        f"Mpts*RKstage/s at {n}^3 Rayleigh-Benard (SCFT), FP32-mixed plain GPU path", value, args,
+        1e3 * el / args.steps,
+        {"workload": f"rb_scft_{n}^3_rk4", "N": n, "kfactor": "pi, pi/sqrt2, pi/sqrt2", "Ra": Ra, "Pr": Pr, "dt": dt,
+         "ic": "V = 0, theta amplitude 1e-3 seed 4", "precision": "fp32-mixed"},
+        {"dtype": "fp32-mixed (fp32 storage/FFT; fp64 integrating factor, twiddles, sums)",
+         "data": "synthetic (seeded temperature perturbation generated on the GPU)",
+         "path": "plain (separate transforms and array passes, the reference's sequence)",
+         "Nu_fp32": Nu, "Nu_fp64": res["fp64"][1],
+         "fp64_gpu": {"value": round(v64, 1), "ms_per_step": round(1e3 * res["fp64"][0] / args.steps, 3)},
+         "roofline": None, "cpu_baseline": None})), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["ns512", "ensemble"], default="ns512",
+    ap.add_argument("--workload", choices=["ns512", "ensemble", "mhd", "rb"], default="ns512",
                     help="ns512: the N^3 decaying-turbulence step (default, N from --n); "
-                         "ensemble: the uneven 64-box ensemble")
+                         "ensemble: the uneven 64-box ensemble; mhd: N^3 MHD; rb: Rayleigh-Benard (SCFT)")
+    ap.add_argument("--rb-n", type=int, default=256)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -236,6 +341,12 @@ def main():
         sys.exit(2)
     if args.workload == "ensemble":
         bench_ensemble(args)
+        return
+    if args.workload == "mhd":
+        bench_mhd(args)
+        return
+    if args.workload == "rb":
+        bench_rb(args)
         return
 
     import numpy as np

@@ -80,7 +80,11 @@ template <int L> struct ZPad {
     static constexpr int code(int s)
     {
         // sh * 16 + k; 0 = identity
-        return L == 64 ? (s == 0 ? 3 * 16 + 1 : 0)
+        return L == 16 ? (s == 0 ? 3 * 16 + 1 : 0)
+             : L == 24 ? (s <= 1 ? 2 * 16 + 1 : 0)
+             : L == 32 ? (s == 0 ? 3 * 16 + 1 : 0)
+             : L == 48 ? (s <= 1 ? 2 * 16 + 1 : 0)
+             : L == 64 ? (s == 0 ? 3 * 16 + 1 : 0)
              : L == 96 ? (s <= 2 ? 3 * 16 + 1 : 0)
              : L == 128 ? (s <= 1 ? 4 * 16 + 1 : 0)
              : L == 192 ? (s == 0 || s == 2 ? 4 * 16 + 1 : s == 1 ? 2 * 16 + 1 : 0)
@@ -94,7 +98,7 @@ template <int L> struct ZPad {
         for (int s = 0; s < 8; s++) m = map(s, L - 1) + 1 > m ? map(s, L - 1) + 1 : m;
         return m;
     }
-    static constexpr int line = L == 64 ? 72 : L == 96 ? 120 : L == 128 ? 144 : L == 192 ? 240 : L == 256 ? 318 : L;
+    static constexpr int line = L == 16 ? 18 : L == 24 ? 30 : L == 32 ? 36 : L == 48 ? 60 : L == 64 ? 72 : L == 96 ? 120 : L == 128 ? 144 : L == 192 ? 240 : L == 256 ? 318 : L;
     static_assert(line >= need(), "ZPad line stride too short");
 };
 
