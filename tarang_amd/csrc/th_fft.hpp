@@ -18,13 +18,15 @@ struct Sched {
 };
 
 // Greedy radix schedule: each radix must divide the remaining length and the points per thread.
-constexpr Sched make_sched(int L, int E)
+// Radix 16 only on request (it needs all 16 points of a butterfly live at once).
+constexpr Sched make_sched(int L, int E, bool r16 = false)
 {
     Sched s{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
     int m = L;
     while (m > 1) {
         int R = 0;
-        if (m % 8 == 0 && E % 8 == 0) R = 8;
+        if (r16 && m % 16 == 0 && E % 16 == 0) R = 16;
+        else if (m % 8 == 0 && E % 8 == 0) R = 8;
         else if (m % 4 == 0 && E % 4 == 0) R = 4;
         else if (m % 2 == 0 && E % 2 == 0) R = 2;
         else if (m % 3 == 0 && E % 3 == 0) R = 3;
@@ -73,6 +75,12 @@ __device__ __forceinline__ void wave_sync()
 // j would otherwise pile onto a few banks.  (sh, k) per exchange and the line stride minimise the
 // conflict degree of the gfx950 LDS (64 banks of 4 bytes; a 64-bit access serves 32 lanes per
 // pass) over every access of the radix schedules below.
+// Radix-16 z schedules ([16, 4], [16, 8], [16, 16]): one padding element every 16 and a line stride
+// of 16 mod 32 elements keep every access conflict-free under the same bank model.
+template <int L> struct ZPad16 {
+    static constexpr int map(int, int A) { return A + (A >> 4); }
+    static constexpr int line = L + L / 16;
+};
 struct NoPad {
     static constexpr int map(int, int A) { return A; }
 };
@@ -149,6 +157,45 @@ template <int SIGN, typename R> struct DFT<8, SIGN, R> {
     }
 };
 
+// 16 points as 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddles W16^(n2 k1), DFT4 over n2;
+// X[k1 + 4 k2] lands in v[k1 + 4 k2].
+template <int SIGN, typename R> struct DFT<16, SIGN, R> {
+    static __device__ __forceinline__ cplx<R> wmul(const cplx<R>& a, R c, R s)   // a * (c + SIGN i s)
+    {
+        return mk<R>(a.x * c - R(SIGN) * a.y * s, a.y * c + R(SIGN) * a.x * s);
+    }
+    static __device__ __forceinline__ void run(cplx<R>* v)
+    {
+        const R c1 = R(0.92387953251128675612818318939678828682);   // cos(pi/8)
+        const R s1 = R(0.38268343236508977172845998403039886676);   // sin(pi/8)
+        const R h = K<R>::s12;
+        cplx<R> a[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) {
+            cplx<R> q[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+            DFT<4, SIGN, R>::run(q);
+#pragma unroll
+            for (int k1 = 0; k1 < 4; k1++) a[n2][k1] = q[k1];
+        }
+        a[1][1] = wmul(a[1][1], c1, s1);    // W^1
+        a[1][2] = wmul(a[1][2], h, h);      // W^2
+        a[1][3] = wmul(a[1][3], s1, c1);    // W^3
+        a[2][1] = wmul(a[2][1], h, h);      // W^2
+        a[2][2] = mul_si<SIGN>(a[2][2]);    // W^4
+        a[2][3] = wmul(a[2][3], -h, h);     // W^6
+        a[3][1] = wmul(a[3][1], s1, c1);    // W^3
+        a[3][2] = wmul(a[3][2], -h, h);     // W^6
+        a[3][3] = wmul(a[3][3], -c1, -s1);  // W^9
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            cplx<R> q[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+            DFT<4, SIGN, R>::run(q);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = q[k2];
+        }
+    }
+};
+
 template <int SIGN, typename R> struct DFT<3, SIGN, R> {
     static __device__ __forceinline__ void run(cplx<R>* v)
     {
@@ -164,9 +211,9 @@ template <int SIGN, typename R> struct DFT<3, SIGN, R> {
 // One line of length L, P cooperating threads, E = L/P points per thread.
 //   before run(): v[i] = x[in_pos(t, i)]       after run(): v[i] = X[out_pos(t, i)]
 // lds points at element 0 of this line; element m lives at lds[m * LS].
-template <typename R, int L, int P, int SIGN> struct LineFFT {
+template <typename R, int L, int P, int SIGN, bool R16 = false> struct LineFFT {
     static constexpr int E = L / P;
-    static constexpr Sched S = make_sched(L, E);
+    static constexpr Sched S = make_sched(L, E, R16);
     static_assert(L % P == 0, "P must divide L");
     static_assert(S.n > 0, "no radix schedule for (L, E)");
     static constexpr int NST = S.n;

@@ -93,11 +93,12 @@ __device__ __forceinline__ double ksq_f(double a, double b, double c)
 // choice is kept per (pass, grid, precision, physics, boxes) in the context's tune cache.
 //   x/y passes (PASS_XINV, PASS_YINV, PASS_YFWD): bit 0 halves the lines per workgroup, bit 1
 //     switches to the alternative points per thread; lines shorter than 128 have variant 0 only.
-//   z pass: 0 = 256 threads per workgroup, 1 = 128, 2 = 512 (z lines of 64 or more; 512 only for
-//     FP32 NS power-of-two lines, the others would spill).
+//   z pass: 0 = 256 threads per workgroup, 1 = 128, 2 = 512, 3 = 16 points per thread with
+//     radix-16 stages (z lines of 64 or more; 2 and 3 only for FP32 NS power-of-two lines, the
+//     others would spill).
 //   x-forward epilogue: 0 = default lines per workgroup, 1 = half (lines of 128 or more).
 enum { PASS_XINV = 0, PASS_YINV = 1, PASS_Z = 2, PASS_YFWD = 3, PASS_XFWD = 4, NPASS = 5 };
-constexpr int pass_nvar(int pass) { return pass == PASS_Z ? 3 : pass == PASS_XFWD ? 2 : 4; }
+constexpr int pass_nvar(int pass) { return pass == PASS_Z ? 4 : pass == PASS_XFWD ? 2 : 4; }
 
 template <typename R, int MODE>
 int d_xinv(const FG& g, const cplx<R>* S, cplx<R>* A, int nc, const cplx<R>* tw, ShellSel sel, int var, hipStream_t st);

@@ -16,6 +16,7 @@
 // (four_ET.cc) in the pass-5 epilogue instead of writing N.
 
 #pragma once
+#include <type_traits>
 #include "th_fused.hpp"
 
 namespace th {
@@ -139,12 +140,14 @@ template <typename R, int LZ, int P, int TZ, int MODE>
 __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__ B, cplx<R>* __restrict__ Cout,
                                                const cplx<R>* __restrict__ twL, const cplx<R>* __restrict__ twN)
 {
-    using FI = LineFFT<R, LZ, P, +1>;
-    using FF = LineFFT<R, LZ, P, -1>;
+    constexpr bool R16 = (LZ / P) % 16 == 0;   // 16 points per thread: radix-16 stages
+    using FI = LineFFT<R, LZ, P, +1, R16>;
+    using FF = LineFFT<R, LZ, P, -1, R16>;
+    using Lay = std::conditional_t<R16, ZPad16<LZ>, ZPad<LZ>>;
     using ZP = ZProd<MODE>;
     constexpr int E = FI::E;
     static_assert(64 % P == 0, "a z line's threads must share one wavefront");
-    constexpr int LZS = ZPad<LZ>::line;   // padded line stride in LDS
+    constexpr int LZS = Lay::line;   // padded line stride in LDS
     __shared__ cplx<R> lds[TZ * LZS];
     const int t = threadIdx.x % P, c = threadIdx.x / P;
     const long q = (long)blockIdx.x * TZ + c;
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
             cplx<R> w = conjg(twk[m]);
             u[a][zreg<E, FI::R0>(m)] = s + mul_i(cmul(w, d));
         }
-        FI::template run<1, true, ZPad<LZ>>(u[a], Z, t, twL);
+        FI::template run<1, true, Lay>(u[a], Z, t, twL);
     }
     // One product line: forward half-length FFT, r2c unpacking, store of the kept kz.
     auto product_out = [&](cplx<R> (&w)[E], int p) {
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
 #pragma unroll
             for (int i = 0; i < E; i++) w[i] = Z[FF::in_pos(t, i)];
         }
-        FF::template run<1, true, ZPad<LZ>>(w, Z, t, twL);
+        FF::template run<1, true, Lay>(w, Z, t, twL);
         const cplx<R>* twp = twN + opaque_zero();   // W_N3^(t + P m), reloaded per product (L1)
         cplx<R>* X = Cout + p * g.cB + off;
 #pragma unroll
@@ -590,7 +593,7 @@ static int launch_yinv(const FG& g, const cplx<R>* A, cplx<R>* B, int ncomp, con
 template <typename R, int LZ, int MODE, int V>
 static int launch_z(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* twL, const cplx<R>* twN, hipStream_t st)
 {
-    constexpr int E = pts_per_thread(LZ), P = LZ / E;
+    constexpr int E = (V == 3) ? 16 : pts_per_thread(LZ), P = LZ / E;
     constexpr int NT = (V == 1) ? 128 : (V == 2) ? 512 : 256;
     constexpr int TZ = (NT / P) < 1 ? 1 : NT / P;
     const long lines = (long)g.N1 * g.N2;
@@ -670,9 +673,12 @@ static int z_v(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* twL, c
 {
     if constexpr (LZ >= 64) {
         if (var == 1) return launch_z<R, LZ, MODE, 1>(g, B, Cc, twL, twN, st);
-        // 512-thread workgroups cap a wave at 128 VGPRs: only the FP32 NS radix-2^k lines fit
-        if constexpr (sizeof(R) == 4 && MODE == 0 && LZ % 3 != 0)
+        // 512-thread workgroups cap a wave at 128 VGPRs, and 16 points per thread need the room of
+        // one line in 32 VGPRs: both only for FP32 NS power-of-two lines
+        if constexpr (sizeof(R) == 4 && MODE == 0 && LZ % 3 != 0) {
             if (var == 2) return launch_z<R, LZ, MODE, 2>(g, B, Cc, twL, twN, st);
+            if (var == 3) return launch_z<R, LZ, MODE, 3>(g, B, Cc, twL, twN, st);
+        }
     }
     return launch_z<R, LZ, MODE, 0>(g, B, Cc, twL, twN, st);
 }
