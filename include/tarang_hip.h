@@ -149,11 +149,16 @@ int th_ic_random_scalar(th_plan *plan, void *T, unsigned long long seed, double 
  * download convert from / to the reference layout (src/dst on host or device).
  * th_step == nsteps x IncFluid::Time_advance() with the RK4 integrating-factor scheme
  *   (time_advance.cc:100-155, _VF_MHD.cc:105-180, _SF.cc:105-222), fused passes, hipGraph replay.
+ * RB (TH_RB, SCFT basis) == Time_advance(IncSF&) time_advance_SF.cc:105-222 with the DCT/DST x
+ *   passes; dissipation sqrt(Pr/Ra), 1/sqrt(Pr Ra) when Ra, Pr > 0, else nu and kappa.
  * comp indexes: NS V1..V3 = 0..2; MHD U 0..2, B 3..5; RB V 0..2, T 3. */
 int th_state_create(th_plan *plan, th_state **state, int physics);
 int th_state_destroy(th_state *state);
 int th_state_upload(th_state *state, int comp, const void *src, int src_on_host);
 int th_state_download(th_state *state, int comp, void *dst, int dst_on_host);
+/* Diagnostics (no counterpart in the reference): which = 0 state, 1 RK stage field,
+ * 2 RK4 accumulator, each in the reference layout. */
+int th_state_download_buffer(th_state *state, int which, int comp, void *dst, int dst_on_host);
 int th_step(th_plan *plan, th_state *state, double dt, const th_phys *p, int nsteps);
 /* Pi(R_i) for spheres radii[1..nsh] and T(n,m) for giver shells n = 1..nsh-1 over the state's
  * velocity (Compute_flux + Compute_shell_tr), from one giver-masked fused pipeline per shell. */

@@ -102,6 +102,7 @@ def lib():
         "th_state_upload": (i, [vp, i, vp, i]),
         "th_state_download": (i, [vp, i, vp, i]),
         "th_step": (i, [vp, vp, d, C.POINTER(Phys), i]),
+        "th_state_download_buffer": (i, [vp, i, i, vp, i]),
         "th_flux_shell_tr": (i, [vp, vp, dp, i, dp, dp]),
         "th_profile_passes": (i, [vp, vp, d, C.POINTER(Phys), i, i, dp, dp]),
         "th_tune": (i, [vp, vp, d, C.POINTER(Phys), i, C.POINTER(C.c_int)]),
@@ -465,14 +466,23 @@ class State:
         self.plan._post(0, "")
         return out
 
+    def download_buffer(self, which):
+        """Diagnostics: 1 = RK stage field, 2 = RK4 accumulator (reference layout)."""
+        out = self.plan.empty(self.ncomp)
+        self.plan._pre()
+        for c, f in enumerate(out):
+            _check(lib().th_state_download_buffer(self.h, which, c, _ptr(f), 0), "th_state_download_buffer")
+        self.plan._post(0, "")
+        return out
+
     def step(self, dt, nu=0.0, eta=0.0, kappa=0.0, Ra=0.0, Pr=0.0, steps=1):
         ph = Phys(nu, eta, kappa, Ra, Pr)
         self.plan._pre()
         self.plan._post(lib().th_step(self.plan.h, self.h, dt, C.byref(ph), steps), "th_step")
 
-    def profile_passes(self, dt, nu=0.0, eta=0.0, stage=2, nrep=5):
+    def profile_passes(self, dt, nu=0.0, eta=0.0, stage=2, nrep=5, kappa=0.0, Ra=0.0, Pr=0.0):
         import numpy as np
-        ph = Phys(nu, eta, 0.0, 0.0, 0.0)
+        ph = Phys(nu, eta, kappa, Ra, Pr)
         ms = np.zeros(5)
         b = np.zeros(5)
         self.plan._pre()
@@ -481,9 +491,9 @@ class State:
                                        b.ctypes.data_as(C.POINTER(C.c_double))), "th_profile_passes")
         return ms, b
 
-    def tune(self, dt, nu=0.0, eta=0.0, nrep=3):
+    def tune(self, dt, nu=0.0, eta=0.0, nrep=3, kappa=0.0, Ra=0.0, Pr=0.0):
         """Autotune the launch shapes of the five fused passes; returns the chosen variants."""
-        ph = Phys(nu, eta, 0.0, 0.0, 0.0)
+        ph = Phys(nu, eta, kappa, Ra, Pr)
         v = (C.c_int * 5)()
         self.plan._pre()
         self.plan._post(lib().th_tune(self.plan.h, self.h, dt, C.byref(ph), nrep, v), "th_tune")

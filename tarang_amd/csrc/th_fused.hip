@@ -35,9 +35,11 @@ FG make_fg(const Geo& geo, int prec)
     g.N2 = geo.N2;
     g.N3 = geo.N3;
     g.LZ = geo.N3 / 2;
+    g.scft = geo.basis == TH_SCFT;
     if (geo.dealias) {
         int hx = geo.N1 / 3, nkx = 2 * (geo.N1 / 3) - 1;
         g.mx = LMap{geo.N1, nkx, hx};
+        if (g.scft) g.mx = LMap{geo.N1, 2 * geo.N1 / 3, 2 * geo.N1 / 3};   // kx >= 0, kept kx < 2 N1 / 3
         int ylo = geo.N2 / 3, yhi = geo.N2 - (2 * geo.N2 / 3 + 2);
         g.my = LMap{geo.N2, ylo + yhi, ylo};
         g.nkz = geo.N3 / 3;
@@ -57,6 +59,7 @@ FG make_fg(const Geo& geo, int prec)
     g.f2 = geo.kf[1];
     g.f3 = geo.kf[2];
     g.norm = 1.0 / ((double)geo.N1 * (double)geo.N2 * (double)geo.N3);
+    if (g.scft) g.norm *= 0.5;                                          // Norm_SCFT: 1 / (2 N1 N2 N3)
     g.nbox = 1;
     return g;
 }
@@ -114,7 +117,7 @@ static std::string tune_key(const th_state* s)
 {
     char k[160];
     snprintf(k, sizeof k, "fused:gfx950:%s:%s:N%dx%dx%d:boxes%d", s->esz == 8 ? "fp32" : "fp64",
-             s->physics == TH_MHD ? "mhd" : "ns", s->fg.N1, s->fg.N2, s->fg.N3, s->nbox);
+             s->physics == TH_MHD ? "mhd" : s->physics == TH_RB ? "rb" : "ns", s->fg.N1, s->fg.N2, s->fg.N3, s->nbox);
     return k;
 }
 
@@ -125,8 +128,11 @@ static int state_create(th_plan* p, th_state** out, int physics, int nbox)
 {
     if (!p || !out || nbox < 1) { set_error("th_state_create: bad args"); return E_ARG; }
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
-    if (p->g.basis != TH_FOUR) { set_error("th_state_create: the fused path covers the FOUR basis; use th_step_plain for SCFT"); return E_UNSUPPORTED; }
-    if (physics != TH_NS && physics != TH_MHD) { set_error("th_state_create: physics %d not fused (use th_step_plain)", physics); return E_UNSUPPORTED; }
+    if (physics != TH_NS && physics != TH_MHD && physics != TH_RB) { set_error("th_state_create: physics %d unknown", physics); return E_ARG; }
+    if ((physics == TH_RB) != (p->g.basis == TH_SCFT)) {
+        set_error("th_state_create: the fused path runs NS/MHD on the FOUR basis and RB on the SCFT basis");
+        return E_UNSUPPORTED;
+    }
     if (!p->g.dealias) { set_error("th_state_create: the fused path stores 2/3-dealiased modes only; create the plan with dealias = 1"); return E_UNSUPPORTED; }
     if (!fused_len_ok(p->g.N1) || !fused_len_ok(p->g.N2) || !fused_len_ok(p->g.N3)) {
         set_error("th_state_create: fused path needs N in {16,24,32,48,64,96,128,192,256,384,512}");
@@ -137,8 +143,8 @@ static int state_create(th_plan* p, th_state** out, int physics, int nbox)
     s->plan = p;
     s->physics = physics;
     s->nbox = nbox;
-    s->ncomp = physics == TH_NS ? 3 : 6;
-    s->nin = physics == TH_NS ? 3 : 6;
+    s->ncomp = physics == TH_NS ? 3 : physics == TH_MHD ? 6 : 4;
+    s->nin = s->ncomp;
     s->nout = physics == TH_NS ? 6 : 9;
     s->fg = make_fg(p->g, p->prec);
     s->esz = p->prec ? 8 : 16;
@@ -237,6 +243,18 @@ static int state_download(th_state* s, int box, int comp, void* dst, int dst_on_
     return OK;
 }
 
+// Diagnostics: reference-layout copy of the stage field (which = 1) or the RK4 accumulator
+// (which = 2) instead of the state (which = 0).
+extern "C" int th_state_download_buffer(th_state* s, int which, int comp, void* dst, int dst_on_host)
+{
+    if (!s || which < 0 || which > 2) { set_error("th_state_download_buffer: bad args"); return E_ARG; }
+    void* keep = s->S;
+    s->S = which == 0 ? s->S : which == 1 ? s->Vs : s->acc;
+    int r = state_download(s, 0, comp, dst, dst_on_host, s->plan->ctx->stream);
+    s->S = keep;
+    return r;
+}
+
 extern "C" int th_state_upload(th_state* s, int comp, const void* src, int src_on_host)
 {
     if (!s) { set_error("th_state_upload: null"); return E_ARG; }
@@ -291,45 +309,72 @@ static int set_coefs(th_state* s, const double* dt, const double* nu, const doub
     return OK;
 }
 
-// One RK4 step of every box in s, as 4 stages x 5 pass launches on stream st.
-template <typename R> static int enqueue_step(th_state* s, hipStream_t st)
+// Pass `pass` of RK stage `stage` (1..4) for every box of s, launch-shape variant var.  Stage 1
+// reads the state, later stages the stage field; pass 5 writes the stage field / accumulator
+// (and the state in stage 4).
+template <typename R> static int run_pass(th_state* s, int pass, int var, int stage, hipStream_t st)
 {
     th_plan* p = s->plan;
     const FG& g = s->fg;
     const cplx<R>* twx = p->twd<R>(g.N1);
     const cplx<R>* twy = p->twd<R>(g.N2);
-    const cplx<R>* twz = p->twd<R>(g.LZ);
-    const cplx<R>* twn = p->twd<R>(g.N3);
     cplx<R>* S = (cplx<R>*)s->S;
     cplx<R>* Vs = (cplx<R>*)s->Vs;
     cplx<R>* acc = (cplx<R>*)s->acc;
     cplx<R>* X1 = (cplx<R>*)s->X1;
     cplx<R>* X2 = (cplx<R>*)s->X2;
     cplx<R>* X3 = (cplx<R>*)s->X3;
+    const StageCoef* sc = s->dcoef + (size_t)(stage - 1) * s->nbox;
+    const cplx<R>* src = stage == 1 ? S : Vs;
+    const int ph = s->physics;
+    const int nin = s->nin, nout = s->nout;
     ShellSel sel{0, 0};
-    for (int stage = 1; stage <= 4; stage++) {
-        const StageCoef* sc = s->dcoef + (size_t)(stage - 1) * s->nbox;
-        const cplx<R>* src = stage == 1 ? S : Vs;
-        if (s->physics == TH_NS) {
-            TRY((d_xinv<R, 0>(g, src, X1, 3, twx, sel, s->var[PASS_XINV], st)));
-            TRY(d_yinv<R>(g, X1, X2, 3, twy, s->var[PASS_YINV], st));
-            TRY((d_z<R, 0>(g, X2, X3, twz, twn, s->var[PASS_Z], st)));
-            TRY(d_yfwd<R>(g, X3, X1, 6, twy, s->var[PASS_YFWD], st));
-            TRY((d_xfwd<R, 0>(g, X1, S, Vs, acc, twx, sc, s->var[PASS_XFWD], st)));
-        } else {
-            TRY((d_xinv<R, 1>(g, src, X1, 6, twx, sel, s->var[PASS_XINV], st)));
-            TRY(d_yinv<R>(g, X1, X2, 6, twy, s->var[PASS_YINV], st));
-            TRY((d_z<R, 1>(g, X2, X3, twz, twn, s->var[PASS_Z], st)));
-            TRY(d_yfwd<R>(g, X3, X1, 9, twy, s->var[PASS_YFWD], st));
-            TRY((d_xfwd<R, 1>(g, X1, S, Vs, acc, twx, sc, s->var[PASS_XFWD], st)));
+    switch (pass) {
+    case PASS_XINV:
+        if (ph == TH_RB) return d_sx_inv<R>(g, src, X1, twx, p->twd<R>(4 * g.N1), st);
+        return ph == TH_MHD ? d_xinv<R, 1>(g, src, X1, nin, twx, sel, var, st)
+                            : d_xinv<R, 0>(g, src, X1, nin, twx, sel, var, st);
+    case PASS_YINV: return d_yinv<R>(g, X1, X2, nin, twy, var, st);
+    case PASS_Z: {
+        const cplx<R>* twz = p->twd<R>(g.LZ);
+        const cplx<R>* twn = p->twd<R>(g.N3);
+        if (ph == TH_RB) return d_z<R, 3>(g, X2, X3, twz, twn, var, st);
+        return ph == TH_MHD ? d_z<R, 1>(g, X2, X3, twz, twn, var, st) : d_z<R, 0>(g, X2, X3, twz, twn, var, st);
+    }
+    case PASS_YFWD: return d_yfwd<R>(g, X3, X1, nout, twy, var, st);
+    default:
+        if (ph == TH_RB) return d_sx_fwd_rb<R>(g, X1, S, Vs, acc, twx, p->twd<R>(4 * g.N1), sc, st);
+        return ph == TH_MHD ? d_xfwd<R, 1>(g, X1, S, Vs, acc, twx, sc, var, st)
+                            : d_xfwd<R, 0>(g, X1, S, Vs, acc, twx, sc, var, st);
+    }
+}
+
+// Dissipation of the two field groups: (nu, eta) for NS / MHD; for RB (nu, kappa), or with Ra > 0
+// and Pr > 0 the PRLARGE / ULARGE values sqrt(Pr/Ra), 1/sqrt(Pr Ra) (RB_slip_main.cc:238-239).
+static void diss_of(const th_state* s, const th_phys* ph, double* d0, double* d1)
+{
+    *d0 = ph->nu;
+    *d1 = ph->eta;
+    if (s->physics == TH_RB) {
+        *d1 = ph->kappa;
+        if (ph->Ra > 0 && ph->Pr > 0) {
+            *d0 = sqrt(ph->Pr / ph->Ra);
+            *d1 = 1.0 / sqrt(ph->Pr * ph->Ra);
         }
     }
+}
+
+// One RK4 step of every box in s, as 4 stages x 5 pass launches on stream st.
+template <typename R> static int enqueue_step(th_state* s, hipStream_t st)
+{
+    for (int stage = 1; stage <= 4; stage++)
+        for (int k = 0; k < NPASS; k++) TRY(run_pass<R>(s, k, s->var[k], stage, st));
     return OK;
 }
 
 static int ensure_twiddles(th_plan* p, const FG& g, hipStream_t st)
 {
-    for (int L : {g.N1, g.N2, g.LZ, g.N3})
+    for (int L : {g.N1, g.N2, g.LZ, g.N3, g.scft ? 4 * g.N1 : g.N1})
         if (!p->tw.get(L, p->prec, st)) { set_error("twiddle table allocation failed"); return E_NOMEM; }
     return OK;
 }
@@ -341,7 +386,9 @@ extern "C" int th_step(th_plan* p, th_state* s, double dt, const th_phys* ph, in
     hipStream_t st = p->ctx->stream;
     // twiddle tables must exist before graph capture (their creation allocates)
     TRY(ensure_twiddles(p, s->fg, st));
-    TRY(set_coefs(s, &dt, &ph->nu, &ph->eta, st));
+    double d0, d1;
+    diss_of(s, ph, &d0, &d1);
+    TRY(set_coefs(s, &dt, &d0, &d1, st));
     if (!s->gexec) {
         TH_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         int r = p->prec ? enqueue_step<float>(s, st) : enqueue_step<double>(s, st);
@@ -463,40 +510,16 @@ template <typename R> static int profile_impl(th_state* s, double dt, double nu,
     th_plan* p = s->plan;
     hipStream_t st = p->ctx->stream;
     const FG& g = s->fg;
-    const cplx<R>* twx = p->twd<R>(g.N1);
-    const cplx<R>* twy = p->twd<R>(g.N2);
-    const cplx<R>* twz = p->twd<R>(g.LZ);
-    const cplx<R>* twn = p->twd<R>(g.N3);
-    cplx<R>* S = (cplx<R>*)s->S;
-    cplx<R>* Vs = (cplx<R>*)s->Vs;
-    cplx<R>* acc = (cplx<R>*)s->acc;
-    cplx<R>* X1 = (cplx<R>*)s->X1;
-    cplx<R>* X2 = (cplx<R>*)s->X2;
-    cplx<R>* X3 = (cplx<R>*)s->X3;
     TRY(set_coefs(s, &dt, &nu, &eta, st));
-    const StageCoef* sc = s->dcoef + (stage - 1);
-    ShellSel sel{0, 0};
-    const bool mhd = s->physics == TH_MHD;
-    const int nin = mhd ? 6 : 3, nout = mhd ? 9 : 6, nst = mhd ? 6 : 3;
-    const cplx<R>* src = stage == 1 ? S : Vs;
     hipEvent_t ev[6];
     for (int i = 0; i < 6; i++) TH_CHECK_HIP(hipEventCreate(&ev[i]));
     double tot[5] = {0, 0, 0, 0, 0};
     for (int rep = 0; rep < nrep; rep++) {
         TH_CHECK_HIP(hipEventRecord(ev[0], st));
-        if (mhd) TRY((d_xinv<R, 1>(g, src, X1, 6, twx, sel, s->var[PASS_XINV], st)));
-        else TRY((d_xinv<R, 0>(g, src, X1, 3, twx, sel, s->var[PASS_XINV], st)));
-        TH_CHECK_HIP(hipEventRecord(ev[1], st));
-        TRY(d_yinv<R>(g, X1, X2, nin, twy, s->var[PASS_YINV], st));
-        TH_CHECK_HIP(hipEventRecord(ev[2], st));
-        if (mhd) TRY((d_z<R, 1>(g, X2, X3, twz, twn, s->var[PASS_Z], st)));
-        else TRY((d_z<R, 0>(g, X2, X3, twz, twn, s->var[PASS_Z], st)));
-        TH_CHECK_HIP(hipEventRecord(ev[3], st));
-        TRY(d_yfwd<R>(g, X3, X1, nout, twy, s->var[PASS_YFWD], st));
-        TH_CHECK_HIP(hipEventRecord(ev[4], st));
-        if (mhd) TRY((d_xfwd<R, 1>(g, X1, S, Vs, acc, twx, sc, s->var[PASS_XFWD], st)));
-        else TRY((d_xfwd<R, 0>(g, X1, S, Vs, acc, twx, sc, s->var[PASS_XFWD], st)));
-        TH_CHECK_HIP(hipEventRecord(ev[5], st));
+        for (int k = 0; k < NPASS; k++) {
+            TRY(run_pass<R>(s, k, s->var[k], stage, st));
+            TH_CHECK_HIP(hipEventRecord(ev[k + 1], st));
+        }
         TH_CHECK_HIP(hipEventSynchronize(ev[5]));
         for (int k = 0; k < 5; k++) {
             float t = 0;
@@ -505,12 +528,15 @@ template <typename R> static int profile_impl(th_state* s, double dt, double nu,
         }
     }
     for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
-    const double e = (double)sizeof(cplx<R>);
+    const double e = (double)sizeof(cplx<R>) * s->nbox;
+    const int nin = s->nin, nout = s->nout, nst = s->ncomp;
     const double cS = (double)g.mx.nk * g.my.nk * g.nkz, cA = (double)g.N1 * g.my.nk * g.nkz,
                  cB = (double)g.N1 * g.N2 * g.nkz;
-    const int state_rw = (stage == 1 || stage == 4) ? 3 : 4;   // V0 (+acc) read, Vs / acc / S written
+    int state_rw = (stage == 1 || stage == 4) ? 3 : 4;   // V0 (+acc) read, Vs / acc / S written
+    double extra = 0;
+    if (s->physics == TH_RB && stage > 1) extra = 2 * cS;  // V1 and T of the stage field (force)
     const double b[5] = {e * (nst * cS + nin * cA), e * (nin * cA + nin * cB), e * (nin * cB + nout * cB),
-                         e * (nout * cB + nout * cA), e * (nout * cA + state_rw * nst * cS)};
+                         e * (nout * cB + nout * cA), e * (nout * cA + state_rw * nst * cS + extra)};
     for (int k = 0; k < 5; k++) {
         ms[k] = tot[k] / nrep;
         bytes[k] = b[k];
@@ -526,8 +552,10 @@ extern "C" int th_profile_passes(th_plan* p, th_state* s, double dt, const th_ph
         return E_ARG;
     }
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
-    return p->prec ? profile_impl<float>(s, dt, ph->nu, ph->eta, stage, nrep, ms, bytes)
-                   : profile_impl<double>(s, dt, ph->nu, ph->eta, stage, nrep, ms, bytes);
+    double d0, d1;
+    diss_of(s, ph, &d0, &d1);
+    return p->prec ? profile_impl<float>(s, dt, d0, d1, stage, nrep, ms, bytes)
+                   : profile_impl<double>(s, dt, d0, d1, stage, nrep, ms, bytes);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -536,49 +564,20 @@ extern "C" int th_profile_passes(th_plan* p, th_state* s, double dt, const th_ph
 // in the context's tune cache.  Stage 2 writes only the stage field and the accumulator, which
 // the next step's first stage overwrites, so the state itself is left unchanged.
 // ------------------------------------------------------------------------------------------
-template <typename R> static int run_pass(th_state* s, int pass, int var, hipStream_t st)
-{
-    th_plan* p = s->plan;
-    const FG& g = s->fg;
-    const bool mhd = s->physics == TH_MHD;
-    const int nin = mhd ? 6 : 3, nout = mhd ? 9 : 6;
-    cplx<R>* S = (cplx<R>*)s->S;
-    cplx<R>* Vs = (cplx<R>*)s->Vs;
-    cplx<R>* acc = (cplx<R>*)s->acc;
-    cplx<R>* X1 = (cplx<R>*)s->X1;
-    cplx<R>* X2 = (cplx<R>*)s->X2;
-    cplx<R>* X3 = (cplx<R>*)s->X3;
-    const StageCoef* sc = s->dcoef + s->nbox;   // stage 2
-    ShellSel sel{0, 0};
-    switch (pass) {
-    case PASS_XINV:
-        return mhd ? d_xinv<R, 1>(g, Vs, X1, nin, p->twd<R>(g.N1), sel, var, st)
-                   : d_xinv<R, 0>(g, Vs, X1, nin, p->twd<R>(g.N1), sel, var, st);
-    case PASS_YINV: return d_yinv<R>(g, X1, X2, nin, p->twd<R>(g.N2), var, st);
-    case PASS_Z:
-        return mhd ? d_z<R, 1>(g, X2, X3, p->twd<R>(g.LZ), p->twd<R>(g.N3), var, st)
-                   : d_z<R, 0>(g, X2, X3, p->twd<R>(g.LZ), p->twd<R>(g.N3), var, st);
-    case PASS_YFWD: return d_yfwd<R>(g, X3, X1, nout, p->twd<R>(g.N2), var, st);
-    default:
-        return mhd ? d_xfwd<R, 1>(g, X1, S, Vs, acc, p->twd<R>(g.N1), sc, var, st)
-                   : d_xfwd<R, 0>(g, X1, S, Vs, acc, p->twd<R>(g.N1), sc, var, st);
-    }
-}
-
 template <typename R> static int tune_impl(th_state* s, int nrep, hipStream_t st, double* ms)
 {
     hipEvent_t e0, e1;
     TH_CHECK_HIP(hipEventCreate(&e0));
     TH_CHECK_HIP(hipEventCreate(&e1));
     int r = OK;
-    for (int k = 0; k < NPASS && !r; k++) r = run_pass<R>(s, k, s->var[k], st);   // live inputs for every pass
+    for (int k = 0; k < NPASS && !r; k++) r = run_pass<R>(s, k, s->var[k], 2, st);   // live inputs for every pass
     for (int k = 0; k < NPASS && !r; k++) {
         int best = s->var[k];
         double tbest = 1e300;
         for (int v = 0; v < pass_nvar(k) && !r; v++) {
-            if ((r = run_pass<R>(s, k, v, st))) break;
+            if ((r = run_pass<R>(s, k, v, 2, st))) break;
             if (hipEventRecord(e0, st) != hipSuccess) { r = E_HIP; break; }
-            for (int i = 0; i < nrep && !r; i++) r = run_pass<R>(s, k, v, st);
+            for (int i = 0; i < nrep && !r; i++) r = run_pass<R>(s, k, v, 2, st);
             if (r || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { r = r ? r : E_HIP; break; }
             float t = 0;
             (void)hipEventElapsedTime(&t, e0, e1);
@@ -609,7 +608,9 @@ extern "C" int th_tune(th_plan* p, th_state* s, double dt, const th_phys* ph, in
 {
     if (!p || !s || s->plan != p || !ph || nrep < 1) { set_error("th_tune: bad args"); return E_ARG; }
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
-    TRY(state_tune(s, &dt, &ph->nu, &ph->eta, nrep, p->ctx->stream));
+    double d0, d1;
+    diss_of(s, ph, &d0, &d1);
+    TRY(state_tune(s, &dt, &d0, &d1, nrep, p->ctx->stream));
     if (var)
         for (int k = 0; k < NPASS; k++) var[k] = s->var[k];
     return OK;

@@ -60,6 +60,7 @@ struct FG {
     long cB;     // component stride of B/C: N1 * sB
     long bS, bA, bB, bC;   // box strides of the state, A/D, B and C buffers (batched boxes of one size)
     int nbox;              // boxes of this size advanced by one launch per pass
+    int scft;              // sin/cos along x (SCFT basis), else Fourier
     double f1, f2, f3;
     double norm; // 1 / (N1 N2 N3)
 };
@@ -114,6 +115,11 @@ int d_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc,
 template <typename R>
 int d_xfwd_et(const FG& g, const cplx<R>* D, const cplx<R>* U, const cplx<R>* tw, const double* radii, int nr, int kin,
               double* partial, int* nblk, hipStream_t st);
+template <typename R>
+int d_sx_inv(const FG& g, const cplx<R>* S, cplx<R>* A, const cplx<R>* tw, const cplx<R>* tw4, hipStream_t st);
+template <typename R>
+int d_sx_fwd_rb(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
+                const cplx<R>* tw4, const StageCoef* sc, hipStream_t st);
 template <typename R> int d_gather(const FG& g, const cplx<R>* ref, cplx<R>* S, hipStream_t st);
 template <typename R> int d_scatter(const FG& g, const cplx<R>* S, cplx<R>* ref, hipStream_t st);
 

@@ -123,6 +123,12 @@ template <> struct ZProd<2> {
     static constexpr int a(int p) { return 3 + p / 3; }
     static constexpr int b(int p) { return p % 3; }
 };
+// MODE 3 (RB): in V1 V2 V3 T, out V1V1 V2V2 V3V3 V1V2 V2V3 V3V1 V1T V2T V3T (compute_nlin_SF.cc:80-177)
+template <> struct ZProd<3> {
+    static constexpr int NIN = 4, NOUT = 9;
+    static constexpr int a(int p) { return p % 3; }
+    static constexpr int b(int p) { return p < 3 ? p : (p < 6 ? (p - 2) % 3 : 3); }
+};
 
 template <typename R> __device__ __forceinline__ cplx<R> shfl_c(const cplx<R>& v, int src, int width)
 {
@@ -449,6 +455,190 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
 }
 
 // ------------------------------------------------------------------------------------------
+// SCFT (Rayleigh-Benard) x passes.  Along x the reference uses FFTW's RODFT10/REDFT10 forward and
+// RODFT01/REDFT01 inverse (scft_tr.cc:49-639) with sine coefficients shifted right by one
+// (ArrayShiftRight/Left_SCFT :153-219) and the 1/(2 N1 N2 N3) norm.  Here each r2r line of
+// length N costs one N-point complex FFT (the complex data are two real sequences, and both maps
+// are complex-linear):
+//   inverse DCT-III:  V_k = conj(w_k) (X_k - i X_{N-k}) (X_N = 0), v = sum_k V_k e^{+2 pi i jk/N}
+//   forward DCT-II:   V = FFT(v), X_k = w_k V_k + conj(w_k) V_{N-k},      w_k = e^{-i pi k / 2N}
+// where v_j is the real-space sample x_{2j} (j < N/2) or x_{2N-2j-1} (j >= N/2).  Real space is
+// only used point-wise (products in the z pass), so rows stay in that order and the permutation
+// cancels between the passes.  Sine lines: y_j = (-1)^j DCT-III(c_{N-l})_j inverse and
+// A_l = DCT-II((-1)^j x_j)_{N-l} = i (w_l V_l - conj(w_l) V_{N-l}) forward; (-1)^j of the sample in
+// row j is +1 for j < N/2 and -1 above.
+// ------------------------------------------------------------------------------------------
+template <typename R, int L, int P, int T>
+__global__ __launch_bounds__(T * P) void k_sx_inv(FG g, const cplx<R>* __restrict__ S, cplx<R>* __restrict__ A,
+                                                  const cplx<R>* __restrict__ tw, const cplx<R>* __restrict__ tw4)
+{
+    using F = LineFFT<R, L, P, +1>;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int ntz = (g.nkz + T - 1) / T;
+    const int cy = blockIdx.x / ntz;
+    const int cz = (blockIdx.x % ntz) * T + c;
+    const bool ok = cz < g.nkz;
+    const long m = (long)cy * g.KZP + cz;
+    const int oc = blockIdx.y;                    // V1 (sin), V2, V3 (cos), T (sin)
+    const bool sn = (oc == 0 || oc == 3);
+    S += blockIdx.z * g.bS;
+    A += blockIdx.z * g.bA;
+    const cplx<R>* col = S + oc * g.cS + m;
+    const int nk = g.mx.nk;
+    cplx<R> v[F::E];
+#pragma unroll
+    for (int i = 0; i < F::E; i++) {
+        const int k = F::in_pos(t, i);
+        cplx<R> ck = (ok && k < nk && !(sn && k == 0)) ? col[(long)k * g.sx] : czero<R>();
+        cplx<R> cm = (ok && k > 0 && L - k < nk) ? col[(long)(L - k) * g.sx] : czero<R>();
+        const cplx<R> a = sn ? cm : ck, b = sn ? ck : cm;          // X_k - i X_{N-k}
+        const cplx<R> x = mk<R>(a.x + b.y, a.y - b.x);
+        v[i] = cmul(conjg(tw4[k]), x);
+    }
+    F::template run<T>(v, lds + c, t, tw);
+    if (ok) {
+        cplx<R>* out = A + oc * g.cA + m;
+#pragma unroll
+        for (int i = 0; i < F::E; i++) {
+            const int j = F::out_pos(t, i);
+            const R sg = (sn && j >= L / 2) ? R(-1) : R(1);
+            out[(long)j * g.sx] = cscale(v[i], sg);
+        }
+    }
+}
+
+// registers holding only positions l >= N1 - N1/3 (dropped by the SCFT 2/3 rule) for every lane
+template <int L, int P, int E, int RL> __host__ __device__ constexpr bool sdrop(int i)
+{
+    const int m = i / RL + (i % RL) * (E / RL);
+    return m * P >= 2 * L / 3;
+}
+
+// RB x-forward + epilogue: forward DCT-II / DST-II of the 9 product lines, N_i = sum_j d_j(V_i V_j)
+// and N_T = sum_j d_j(V_j T) with Xderiv_Sin (+k f1) / Xderiv_Cos (-k f1) along x (scft_tr.cc:
+// 649-672), the RB force (Force1 = T, T.Force = V1; compute_force_RB.cc:51-105, add_force.cc:57-63),
+// pressure projection (compute_pressure.cc:54-62 with the SCFT divergence), Compute_rhs and the
+// FP64 IF-RK4 update with nu for V and kappa for T (time_advance_SF.cc:105-222).
+template <typename R, int L, int P, int T>
+__global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __restrict__ D, cplx<R>* S, cplx<R>* Vs,
+                                                     cplx<R>* acc, const cplx<R>* __restrict__ tw,
+                                                     const cplx<R>* __restrict__ tw4,
+                                                     const StageCoef* __restrict__ coefs)
+{
+    using F = LineFFT<R, L, P, -1>;
+    constexpr int E = F::E;
+    __shared__ cplx<R> lds[L * T];
+    const int c = threadIdx.x % T, t = threadIdx.x / T;
+    const int ntz = (g.nkz + T - 1) / T;
+    const int lin = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int tile = lin % gridDim.x, box = lin / gridDim.x;
+    const int cy = tile / ntz;
+    const int cz = (tile % ntz) * T + c;
+    const bool ok = cz < g.nkz;
+    const long m = (long)cy * g.KZP + cz;
+    const StageCoef sc = coefs[box];
+    D += box * g.bA;
+    S += box * g.bS;
+    Vs += box * g.bS;
+    acc += box * g.bS;
+    const R Ky = (R)(ky_of(ll(g.my, cy), g.N2) * g.f2);
+    const R Kz = (R)(cz * g.f3);
+    const int nk = g.mx.nk;
+    cplx<R> Nacc[4][E];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int i = 0; i < E; i++) Nacc[a][i] = czero<R>();
+    const long mu = (long)cy * g.KZP + (long)(tile % ntz) * T;
+    const int lane_off = c + t * (int)g.sx;
+#pragma unroll 1
+    for (int p = 0; p < 9; p++) {
+        // forward kind of product p: sine for V1V2 (3), V3V1 (5), V2T (7), V3T (8)
+        const bool psin = ((0x1A8u >> p) & 1u) != 0;
+        cplx<R> v[E];
+        const cplx<R>* src = D + p * g.cA + mu;
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            const int j0 = F::in_pos(0, i);
+            const cplx<R>* row = src + (long)j0 * g.sx;
+            const cplx<R> x = ok ? row[lane_off] : czero<R>();
+            const R sg = (psin && j0 + t >= L / 2) ? R(-1) : R(1);
+            v[i] = cscale(x, sg);
+        }
+        F::template run<T>(v, lds + c, t, tw + opaque_zero());
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < E; i++) lds[F::out_pos(t, i) * T + c] = v[i];
+        __syncthreads();
+        // product p adds d_j F(p) to N_a: Xderiv_Cos (-Kx) / Xderiv_Sin (+Kx) along x, i Ky, i Kz
+        const int a1 = (p == 0 || p == 3 || p == 5) ? 0 : (p == 1 || p == 4) ? 1 : (p == 2) ? 2 : 3;
+        const R al1 = (p == 0 || p == 6) ? R(-1) : R(0);
+        const R be1 = (p == 1 || p == 3 || p == 7) ? Ky : (p == 2 || p == 4 || p == 5 || p == 8) ? Kz : R(0);
+        const int a2 = (p == 3) ? 1 : (p == 4 || p == 5) ? 2 : -1;      // second target
+        const R al2 = (p == 3 || p == 5) ? R(1) : R(0);
+        const R be2 = (p == 4) ? Ky : R(0);
+        const cplx<R>* w4 = tw4 + opaque_zero();
+#pragma unroll
+        for (int i = 0; i < E; i++) {
+            if (sdrop<L, P, E, F::RL>(i)) continue;
+            const int l = F::out_pos(t, i);
+            const cplx<R> w = w4[l];
+            const cplx<R> Pv = cmul(w, v[i]);
+            const cplx<R> Qv = cmul(conjg(w), lds[((L - l) % L) * T + c]);
+            const cplx<R> dv = Pv - Qv, sv = Pv + Qv;
+            cplx<R> Av = psin ? mk<R>(-dv.y, dv.x) : sv;                        // i (P - Q) or P + Q
+            if (psin && l == 0) Av = czero<R>();
+            const R Kx = (R)(l * g.f1);
+            const cplx<R> c1 = mk<R>(al1 * Kx * Av.x - be1 * Av.y, al1 * Kx * Av.y + be1 * Av.x);
+            const cplx<R> c2 = mk<R>(al2 * Kx * Av.x - be2 * Av.y, al2 * Kx * Av.y + be2 * Av.x);
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                if (a == a1) Nacc[a][i] = Nacc[a][i] + c1;
+                if (a == a2) Nacc[a][i] = Nacc[a][i] + c2;
+            }
+        }
+    }
+    if (!ok) return;
+    const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
+    const cplx<R>* F0 = (sc.stage == 1) ? S : Vs;     // field the stage's rhs is evaluated on
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        const int l = F::out_pos(t, i);
+        if (sdrop<L, P, E, F::RL>(i) || l >= nk) continue;
+        const double dKx = l * g.f1;
+        const double K2 = ksq_f(dKx, dKy, dKz);
+        const long idx = (long)l * g.sx + m;
+        Dc Nn[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) Nn[a] = dsc(dl(Nacc[a][i]), g.norm);
+        const Dc V1 = dl(F0[idx]), Tf = dl(F0[3 * g.cS + idx]);
+        Nn[0] = dmk(Nn[0].x - Tf.x, Nn[0].y - Tf.y);            // nlin - Force1, Force1 = T
+        Nn[3] = dmk(Nn[3].x - V1.x, Nn[3].y - V1.y);            // T.nlin - T.Force, T.Force = V1
+        Dc pr = dmk(0.0, 0.0);
+        if (K2 > 0) {
+            // div N = Xderiv_Sin(N1) + i Ky N2 + i Kz N3, p = div / K^2
+            const double dr = dKx * Nn[0].x - dKy * Nn[1].y - dKz * Nn[2].y;
+            const double di = dKx * Nn[0].y + dKy * Nn[1].x + dKz * Nn[2].x;
+            pr = dmk(dr / K2, di / K2);
+        }
+        // rhs = -(N + grad p): Xderiv_Cos(p) = -Kx p, d_y p = i Ky p, d_z p = i Kz p
+        Dc rhs[4];
+        rhs[0] = dmk(-(Nn[0].x - dKx * pr.x), -(Nn[0].y - dKx * pr.y));
+        rhs[1] = dmk(-(Nn[1].x - dKy * pr.y), -(Nn[1].y + dKy * pr.x));
+        rhs[2] = dmk(-(Nn[2].x - dKz * pr.y), -(Nn[2].y + dKz * pr.x));
+        rhs[3] = dmk(-Nn[3].x, -Nn[3].y);
+        const double eh = exp(sc.fh[0] * K2), e1 = exp(sc.f1[0] * K2);
+        const double ehT = exp(sc.fh[1] * K2), e1T = exp(sc.f1[1] * K2);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+            rk_update<R>(sc.stage, sc.dt, a < 3 ? eh : ehT, a < 3 ? e1 : e1T, rhs[a], S + a * g.cS + idx,
+                         Vs + a * g.cS + idx, acc + a * g.cS + idx);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // energy-transfer epilogue: x-forward of the 9 ET product lines, N_i = sum_j i K_j F(H_j G_i), then
 // 2 mult Re(N . U*) binned by shell (Get_shell_index over radii) separately for |K| <= kmax_inside
 // (Shell_mult_all) and |K| > kmax_inside (which only Shell_mult_single counts).
@@ -647,6 +837,27 @@ static int launch_xfwd_et(const FG& g, const cplx<R>* D, const cplx<R>* U, const
     return OK;
 }
 
+template <typename R, int L>
+static int launch_sxinv(const FG& g, const cplx<R>* S, cplx<R>* A, const cplx<R>* tw, const cplx<R>* tw4, hipStream_t st)
+{
+    constexpr int E = pts_per_thread(L), P = L / E, T = tileT<R>();
+    const int ntz = (g.nkz + T - 1) / T;
+    hipLaunchKernelGGL((k_sx_inv<R, L, P, T>), dim3(g.my.nk * ntz, 4, g.nbox), dim3(T * P), 0, st, g, S, A, tw, tw4);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+template <typename R, int L>
+static int launch_sxfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
+                        const cplx<R>* tw4, const StageCoef* sc, hipStream_t st)
+{
+    constexpr int E = xfwd_E(L), P = L / E, T = xfwd_T<R, L, 0>();
+    const int ntz = (g.nkz + T - 1) / T;
+    hipLaunchKernelGGL((k_sx_fwd_rb<R, L, P, T>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D, S, Vs, acc, tw,
+                       tw4, sc);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+
 // variant dispatch per line length (short lines have one shape)
 template <typename R, int MODE, int L>
 static int xinv_v(const FG& g, const cplx<R>* S, cplx<R>* A, int nc, const cplx<R>* tw, ShellSel sel, int var, hipStream_t st)
@@ -673,8 +884,8 @@ static int z_v(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* twL, c
 {
     if constexpr (LZ >= 64) {
         if (var == 1) return launch_z<R, LZ, MODE, 1>(g, B, Cc, twL, twN, st);
-        // 512-thread workgroups cap a wave at 128 VGPRs, and 16 points per thread need the room of
-        // one line in 32 VGPRs: both only for FP32 NS power-of-two lines
+        // 512-thread workgroups cap a wave at 256 VGPRs and 16 points per thread hold a line in 32
+        // VGPRs: without spills only for FP32 NS power-of-two lines
         if constexpr (sizeof(R) == 4 && MODE == 0 && LZ % 3 != 0) {
             if (var == 2) return launch_z<R, LZ, MODE, 2>(g, B, Cc, twL, twN, st);
             if (var == 3) return launch_z<R, LZ, MODE, 3>(g, B, Cc, twL, twN, st);
@@ -746,6 +957,22 @@ int d_xfwd_et(const FG& g, const cplx<R>* D, const cplx<R>* U, const cplx<R>* tw
 #undef C_
 }
 
+template <typename R>
+int d_sx_inv(const FG& g, const cplx<R>* S, cplx<R>* A, const cplx<R>* tw, const cplx<R>* tw4, hipStream_t st)
+{
+#define C_(n) case n: return launch_sxinv<R, n>(g, S, A, tw, tw4, st);
+    switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R>
+int d_sx_fwd_rb(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
+                const cplx<R>* tw4, const StageCoef* sc, hipStream_t st)
+{
+#define C_(n) case n: return launch_sxfwd<R, n>(g, D, S, Vs, acc, tw, tw4, sc, st);
+    switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
+#undef C_
+}
+
 static dim3 grid_1d(size_t n)
 {
     size_t b = (n + 255) / 256;
@@ -773,6 +1000,10 @@ template <typename R> int d_scatter(const FG& g, const cplx<R>* S, cplx<R>* ref,
     template int d_z<R, 0>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, int, hipStream_t);   \
     template int d_z<R, 1>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, int, hipStream_t);   \
     template int d_z<R, 2>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, int, hipStream_t);   \
+    template int d_z<R, 3>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, int, hipStream_t);   \
+    template int d_sx_inv<R>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, hipStream_t);      \
+    template int d_sx_fwd_rb<R>(const FG&, const cplx<R>*, cplx<R>*, cplx<R>*, cplx<R>*, const cplx<R>*,            \
+                                const cplx<R>*, const StageCoef*, hipStream_t);                                       \
     template int d_yfwd<R>(const FG&, const cplx<R>*, cplx<R>*, int, const cplx<R>*, int, hipStream_t);              \
     template int d_xfwd<R, 0>(const FG&, const cplx<R>*, cplx<R>*, cplx<R>*, cplx<R>*, const cplx<R>*,              \
                               const StageCoef*, int, hipStream_t);                                                     \

@@ -149,3 +149,52 @@ def test_config1_fused_taylor_green_64(ctx):
     ferr = (np.abs(Pi - fo)[sel] / np.abs(fo)[sel]).max()
     log(f"config1 fused max rel flux err {ferr:.3e}")
     assert ferr <= 1e-10
+
+
+def _rb_fields(o, amp=0.1, seed=4, warm=3):
+    """Non-trivial dealiased RB fields: the oracle's RK4 run from V = 0 and a random temperature."""
+    V = o.zeros(3)
+    T = o.dealias(o.ic_random_scalar(seed, amp))
+    o.rk4_rb(V, T, 1e5, 1.0, 1e-2, steps=warm)
+    return V, T
+
+
+@pytest.mark.parametrize("N", [(32, 32, 32), (48, 32, 24), (64, 48, 32)])
+def test_fused_rb_scft_fp64(ctx, N):
+    """Fused SCFT/RB step (DCT/DST x passes, RB products, force, projection, IF-RK4 for V and T)
+    equals the oracle's Time_advance(IncSF&) step."""
+    from tarang_amd import Plan, State, abi
+    import math
+    kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
+    log(f"fused rb fp64 N={N}")
+    o = Oracle(N, kfactor=kf, basis="SCFT")
+    V, T = _rb_fields(o)
+    p = Plan(ctx, N, kfactor=kf, basis="SCFT")
+    s = State(p, abi.TH_RB)
+    s.upload([p.to_dev(a) for a in V + [T]])
+    back = s.download()
+    for i, ref in enumerate(V + [T]):
+        assert np.array_equal(H(back[i]), ref), i
+    o.rk4_rb(V, T, 1e5, 1.0, 1e-2, steps=3)
+    s.step(1e-2, Ra=1e5, Pr=1.0, steps=3)
+    out = s.download()
+    for i, ref in enumerate(V + [T]):
+        assert rel(H(out[i]), ref) < 1e-12, i
+    assert abs(p.nusselt(out[:3], out[3], 1e5, 1.0) - o.nusselt(V, T, 1e5, 1.0)) < 1e-12
+
+
+def test_fused_rb_scft_fp32_mixed(ctx):
+    from tarang_amd import Plan, State, abi
+    import math
+    N = (32, 32, 32)
+    kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
+    o = Oracle(N, kfactor=kf, basis="SCFT")
+    V, T = _rb_fields(o)
+    p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision="fp32")
+    s = State(p, abi.TH_RB)
+    s.upload([p.to_dev(a) for a in V + [T]])
+    o.rk4_rb(V, T, 1e5, 1.0, 1e-2, steps=3)
+    s.step(1e-2, Ra=1e5, Pr=1.0, steps=3)
+    out = s.download()
+    for i, ref in enumerate(V + [T]):
+        assert rel(H(out[i]), ref) < 2e-5, i
