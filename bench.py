@@ -30,7 +30,7 @@ PASS_NAMES = ["x_inverse", "y_inverse", "z_fused_c2r_products_r2c", "y_forward",
 # writes whole spectral-size component arrays once (NS: 3 in / 6 products; MHD: 6 in / 9 products).
 # The fused kernels move less (dealias-compacted rows, kz < N3/3); their own byte counts come from
 # th_profile_passes and the measured HBM traffic from rocprofv3.
-CANON_S = {"ns": [6, 6, 9, 12, 18], "mhd": [12, 12, 15, 18, 33]}
+CANON_S = {"ns": [6, 6, 9, 12, 18], "mhd": [12, 12, 15, 18, 33], "rb": [8, 8, 13, 18, 25]}
 
 
 def roofline(ms, nbytes, N, esize, physics, traffic=None):
@@ -298,44 +298,70 @@ def bench_mhd(args):
 
 
 def bench_rb(args):
-    """Rayleigh-Benard, SCFT basis with the sin/cos axis first, 256^3, kfactor (pi, pi/sqrt2, pi/sqrt2),
-    Pr = 1, Ra = 1e7 (PRLARGE/ULARGE scaling), V = 0, temperature perturbation amplitude 1e-3 (seed 4),
-    dt = 1e-3 fixed; RK4 on the plain GPU path (the reference's operation sequence), FP64 and FP32-mixed.
-    Reports Nu after the timed steps."""
+    """Rayleigh-Benard, SCFT basis with the sin/cos axis first, N^3 (default 256^3), kfactor
+    (pi, pi/sqrt2, pi/sqrt2), Pr = 1, Ra = 1e7 (PRLARGE/ULARGE scaling), V = 0, temperature
+    perturbation amplitude 1e-3 (seed 4), dt = 1e-3 fixed.  The fused RB step (DCT/DST x passes, 9
+    products in the z pass, force + projection + IF-RK4 epilogue) is timed in FP32-mixed and FP64; the
+    plain path (separate transforms and array passes, the reference's operation sequence) is timed beside
+    it in FP32-mixed.  Reports Nu after the timed steps from each path.  Algorithmic bytes per stage:
+    72 S (S = N1 N2 (N3/2+1) x 8 B)."""
     import torch
-    from tarang_amd import Context, Plan, abi
+    from tarang_amd import Context, Plan, State, abi
     n = args.rb_n
     N = (n, n, n)
     kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
     Ra, Pr, dt = 1e7, 1.0, 1e-3
     ctx = Context(0)
     res = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp32", "fp64"):
         p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision=prec)
-        f = p.empty(3) + [p.ic_random_scalar(4, 1e-3)]
-        p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.warmup)
+        s = State(p, abi.TH_RB)
+        s.upload(p.empty(3) + [p.ic_random_scalar(4, 1e-3)])
+        var = s.tune(dt, Ra=Ra, Pr=Pr, nrep=3) if not args.no_tune else s.variants
+        s.step(dt, Ra=Ra, Pr=Pr, steps=args.warmup)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.steps)
+        s.step(dt, Ra=Ra, Pr=Pr, steps=args.steps)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        res[prec] = (el, p.nusselt(f[:3], f[3], Ra, Pr))
-        del f, p
+        ms, nbytes = s.profile_passes(dt, Ra=Ra, Pr=Pr, stage=2, nrep=3)
+        out = s.download()
+        res[prec] = (el, p.nusselt(out[:3], out[3], Ra, Pr), ms, nbytes, var)
+        del s, out, p
+    p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision="fp32")
+    f = p.empty(3) + [p.ic_random_scalar(4, 1e-3)]
+    p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.steps)
+    torch.cuda.synchronize()
+    el_plain = time.perf_counter() - t0
+    Nu_plain = p.nusselt(f[:3], f[3], Ra, Pr)
+    del f, p
     pts = float(n) ** 3
-    el, Nu = res["fp32"]
+    el, Nu, ms, nbytes, var = res["fp32"]
     value = pts * 4 * args.steps / el / 1e6
     v64 = pts * 4 * args.steps / res["fp64"][0] / 1e6
+    roof, canon_stage = roofline(ms, nbytes, N, 8, "rb")
+    passes = {PASS_NAMES[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
+                              "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
     print(json.dumps(_line(
-        f"Mpts*RKstage/s at {n}^3 Rayleigh-Benard (SCFT), FP32-mixed plain GPU path", value, args,
+        f"Mpts*RKstage/s at {n}^3 Rayleigh-Benard (SCFT), FP32-mixed fused step", value, args,
         1e3 * el / args.steps,
         {"workload": f"rb_scft_{n}^3_rk4", "N": n, "kfactor": "pi, pi/sqrt2, pi/sqrt2", "Ra": Ra, "Pr": Pr, "dt": dt,
          "ic": "V = 0, theta amplitude 1e-3 seed 4", "precision": "fp32-mixed"},
         {"dtype": "fp32-mixed (fp32 storage/FFT; fp64 integrating factor, twiddles, sums)",
          "data": "synthetic (seeded temperature perturbation generated on the GPU)",
-         "path": "plain (separate transforms and array passes, the reference's sequence)",
-         "Nu_fp32": Nu, "Nu_fp64": res["fp64"][1],
+         "path": "fused (5 passes per RK stage, hipGraph replay)", "launch_variants": var,
+         "Nu_fp32": Nu, "Nu_fp64": res["fp64"][1], "Nu_plain_fp32": Nu_plain,
          "fp64_gpu": {"value": round(v64, 1), "ms_per_step": round(1e3 * res["fp64"][0] / args.steps, 3)},
-         "roofline": None, "cpu_baseline": None})), flush=True)
+         "plain_fp32_gpu": {"value": round(pts * 4 * args.steps / el_plain / 1e6, 1),
+                            "ms_per_step": round(1e3 * el_plain / args.steps, 3)},
+         "roofline": roof,
+         "stage": {"canonical_GB_per_stage": round(canon_stage / 1e9, 3),
+                   "pruned_GB_per_stage": round(float(sum(nbytes)) / 1e9, 3),
+                   "GB_s_canonical": round(canon_stage / (float(sum(ms)) * 1e-3) / 1e9, 1), "passes": passes},
+         "cpu_baseline": None})), flush=True)
 
 
 def main():

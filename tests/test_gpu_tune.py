@@ -143,3 +143,36 @@ def test_ensemble_tune_then_step_matches_single_boxes(ctx):
         got, ref = ens.download(b), s.download()
         for i in range(3):
             assert rel(H(got[i]), H(ref[i])) < 1e-12, (b, i)
+
+
+@pytest.fixture(scope="module")
+def ref_rb128():
+    import math
+    N = (128, 128, 128)
+    kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
+    o = Oracle(N, kfactor=kf, basis="SCFT")
+    V = o.zeros(3)
+    T = o.dealias(o.ic_random_scalar(7, 0.1))
+    o.rk4_rb(V, T, 1e5, 1.0, 1e-2, steps=1)        # non-zero velocity from the buoyancy force
+    V0 = [a.copy() for a in V + [T]]
+    o.rk4_rb(V, T, 1e5, 1.0, 1e-2, steps=1)
+    return N, kf, V0, V + [T]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
+def test_variants_rb_scft(ctx, ref_rb128, v, prec):
+    """Every launch variant of the fused RB (SCFT) step gives the oracle's Time_advance(IncSF&)."""
+    from tarang_amd import Plan, State, abi
+    N, kf, V0, V1 = ref_rb128
+    var = [min(v, n - 1) for n in NVAR]
+    log(f"rb variants {var} {prec} N={N}")
+    p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision=prec)
+    s = State(p, abi.TH_RB)
+    s.variants = var
+    s.upload([p.to_dev(a) for a in V0])
+    s.step(1e-2, Ra=1e5, Pr=1.0, steps=1)
+    out = s.download()
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    for i in range(4):
+        assert rel(H(out[i]), V1[i]) < tol, (var, i)
