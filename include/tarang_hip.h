@@ -59,6 +59,7 @@ const char *th_last_error(void);
 const char *th_version(void);
 int th_plan_create(th_ctx *ctx, th_plan **plan, int basis, int N1, int N2, int N3, const double kfactor[3],
                    int precision, int dealias);
+/* A plan with live fused states is released when its last state is destroyed. */
 int th_plan_destroy(th_plan *plan);
 size_t th_plan_field_elems(const th_plan *plan);      /* N1*N2*(N3/2+1) */
 size_t th_plan_elem_bytes(const th_plan *plan);       /* 16 (FP64) or 8 (FP32_MIXED) */

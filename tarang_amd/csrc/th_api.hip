@@ -193,14 +193,25 @@ th_plan::~th_plan()
     release_fused(this);
 }
 
-extern "C" int th_plan_destroy(th_plan* p)
+void th::plan_free(th_plan* p)
 {
-    if (!p) return OK;
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
     th_ctx* c = p->ctx;
     delete p;
     c->nplans--;
+}
+
+// A plan that still has fused states is released by the destruction of its last state, so states
+// and plans may be destroyed in either order.
+extern "C" int th_plan_destroy(th_plan* p)
+{
+    if (!p || p->dead) return OK;
+    if (p->nstates > 0) {
+        p->dead = true;
+        return OK;
+    }
+    th::plan_free(p);
     return OK;
 }
 

@@ -177,6 +177,7 @@ static int state_create(th_plan* p, th_state** out, int physics, int nbox)
         delete s;
         return E_NOMEM;
     }
+    p->nstates++;
     *out = s;
     return OK;
 }
@@ -186,9 +187,11 @@ extern "C" int th_state_create(th_plan* p, th_state** out, int physics) { return
 extern "C" int th_state_destroy(th_state* s)
 {
     if (!s) return OK;
-    (void)hipSetDevice(s->plan->ctx->device);
-    (void)hipStreamSynchronize(s->plan->ctx->stream);
+    th_plan* p = s->plan;
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
     delete s;
+    if (--p->nstates == 0 && p->dead) plan_free(p);
     return OK;
 }
 

@@ -108,26 +108,40 @@ __global__ __launch_bounds__(T * P) void k_fy_inv(FG g, const cplx<R>* __restric
 // MODE 1 (MHD): in Z+0..2, Z-0..2,    out P(i,j) = Z+_i Z-_j at 3i+j             (RSprod.cc:117-151)
 // MODE 2 (ET):  in H0..2,  G0..2,     out P(i,j) = G_i H_j at 3i+j               (ET_RSprod.cc:49-85)
 template <int MODE> struct ZProd;
+// Products are also listed as 3 rotation steps r of NPR products each: product j of step r multiplies
+// x = u[G0 + r] by operand yop(j) (-1 - k: u[G0 + (r + k) % 3]; k >= 0: input k) into output pout(r, j).
 template <> struct ZProd<0> {
     static constexpr int NIN = 3, NOUT = 6;
     static constexpr int a(int p) { return p < 3 ? p : p - 3; }
     static constexpr int b(int p) { return p < 3 ? p : (p == 5 ? 0 : p - 2); }
+    static constexpr int G0 = 0, NPR = 2;
+    static constexpr int yop(int j) { return j == 0 ? -1 : -2; }
+    static constexpr int pout(int r, int j) { return j == 0 ? r : 3 + r; }
 };
 template <> struct ZProd<1> {
     static constexpr int NIN = 6, NOUT = 9;
     static constexpr int a(int p) { return p / 3; }
     static constexpr int b(int p) { return 3 + p % 3; }
+    static constexpr int G0 = 0, NPR = 3;
+    static constexpr int yop(int j) { return 3 + j; }
+    static constexpr int pout(int r, int j) { return 3 * r + j; }
 };
 template <> struct ZProd<2> {
     static constexpr int NIN = 6, NOUT = 9;
     static constexpr int a(int p) { return 3 + p / 3; }
     static constexpr int b(int p) { return p % 3; }
+    static constexpr int G0 = 3, NPR = 3;
+    static constexpr int yop(int j) { return j; }
+    static constexpr int pout(int r, int j) { return 3 * r + j; }
 };
 // MODE 3 (RB): in V1 V2 V3 T, out V1V1 V2V2 V3V3 V1V2 V2V3 V3V1 V1T V2T V3T (compute_nlin_SF.cc:80-177)
 template <> struct ZProd<3> {
     static constexpr int NIN = 4, NOUT = 9;
     static constexpr int a(int p) { return p % 3; }
     static constexpr int b(int p) { return p < 3 ? p : (p < 6 ? (p - 2) % 3 : 3); }
+    static constexpr int G0 = 0, NPR = 3;
+    static constexpr int yop(int j) { return j == 0 ? -1 : j == 1 ? -2 : 3; }
+    static constexpr int pout(int r, int j) { return 3 * j + r; }
 };
 
 template <typename R> __device__ __forceinline__ cplx<R> shfl_c(const cplx<R>& v, int src, int width)
@@ -155,6 +169,12 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
     static_assert(64 % P == 0, "a z line's threads must share one wavefront");
     constexpr int LZS = Lay::line;   // padded line stride in LDS
     __shared__ cplx<R> lds[TZ * LZS];
+    // Twiddles live in LDS: W_LZ^j for the line FFTs, then W_N3^k (k < LZ) for the half-length
+    // packing.  Global twiddle loads inside the product loop would queue behind the product stores
+    // (one in-order vector-memory counter) and stall every FFT stage on store completion.
+    __shared__ cplx<R> tws[2 * LZ];
+    for (int i = threadIdx.x; i < 2 * LZ; i += TZ * P) tws[i] = i < LZ ? twL[i] : twN[i - LZ];
+    __syncthreads();
     const int t = threadIdx.x % P, c = threadIdx.x / P;
     const long q = (long)blockIdx.x * TZ + c;
     const bool ok = q < (long)g.N1 * g.N2;
@@ -167,7 +187,7 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
     const int tp = (P - t) % P;
     cplx<R> twk[E];   // W_N3^(t + P m)
 #pragma unroll
-    for (int m = 0; m < E; m++) twk[m] = twN[t + P * m];
+    for (int m = 0; m < E; m++) twk[m] = tws[LZ + t + P * m];
     cplx<R> u[ZP::NIN][E];
 #pragma unroll
     for (int a = 0; a < ZP::NIN; a++) {
@@ -190,7 +210,7 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
             cplx<R> w = conjg(twk[m]);
             u[a][zreg<E, FI::R0>(m)] = s + mul_i(cmul(w, d));
         }
-        FI::template run<1, true, Lay>(u[a], Z, t, twL);
+        FI::template run<1, true, Lay>(u[a], Z, t, tws);
     }
     // One product line: forward half-length FFT, r2c unpacking, store of the kept kz.
     auto product_out = [&](cplx<R> (&w)[E], int p) {
@@ -202,18 +222,38 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
 #pragma unroll
             for (int i = 0; i < E; i++) w[i] = Z[FF::in_pos(t, i)];
         }
-        FF::template run<1, true, Lay>(w, Z, t, twL);
-        const cplx<R>* twp = twN + opaque_zero();   // W_N3^(t + P m), reloaded per product (L1)
+        // twiddles re-read per product (opaque offset): hoisting them would hold 2 E more VGPRs
+        const cplx<R>* tw = tws + opaque_zero();
+        FF::template run<1, true, Lay>(w, Z, t, tw);
+        const cplx<R>* twp = tw + LZ;   // W_N3^(t + P m)
         cplx<R>* X = Cout + p * g.cB + off;
-#pragma unroll
-        for (int m = 0; m < E; m++) {
-            const int k = t + P * m;
+        auto unpack = [&](int m) {
             cplx<R> yr = shfl_c(w[zreg<E, FF::RL>(E - 1 - m)], tp, P);
             cplx<R> zk = w[zreg<E, FF::RL>(m)];
             cplx<R> zc = conjg((t == 0) ? w[zreg<E, FF::RL>((E - m) % E)] : yr);
             cplx<R> e = cscale(zk + zc, R(0.5));
             cplx<R> tt = cmul(twp[t + P * m], zk - zc);
-            if (ok && k < g.nkz) X[k] = mk<R>(e.x + R(0.5) * tt.y, e.y - R(0.5) * tt.x);
+            return mk<R>(e.x + R(0.5) * tt.y, e.y - R(0.5) * tt.x);
+        };
+        if constexpr (sizeof(R) == 4) {
+            // unpack every point first, then store: no LDS read waits between the stores
+            cplx<R> o[E];
+#pragma unroll
+            for (int m = 0; m < E; m++) o[m] = unpack(m);
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                const int k = t + P * m;
+                if (ok && k < g.nkz) X[k] = o[m];
+            }
+        } else {
+            // FP64: holding every unpacked point would cost 4 E VGPRs.  The unpack (its lane
+            // shuffle needs every lane of the line active) stays outside the store condition.
+#pragma unroll
+            for (int m = 0; m < E; m++) {
+                const int k = t + P * m;
+                const cplx<R> o = unpack(m);
+                if (ok && k < g.nkz) X[k] = o;
+            }
         }
         if (ok && t == 0 && LZ < g.nkz) {   // kz = N3/2 (undealiased): Y[0] - ... with W_N3^(N3/2) = -1
             cplx<R> z0 = w[zreg<E, FF::RL>(0)];
@@ -221,23 +261,32 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
         }
     };
     if constexpr (sizeof(R) == 4) {
-        // FP32: a rolled loop with operands picked by selects keeps the compiler from
-        // interleaving products and hoisting their twiddles (registers, occupancy).
+        // FP32: a rolled loop over the 3 rotation steps keeps the compiler from interleaving
+        // products and hoisting their twiddles (registers, occupancy); rotating the group's
+        // registers by one between steps puts every operand in a fixed register, so no per-product
+        // operand selects are needed.
+        constexpr int G0 = ZP::G0;
 #pragma unroll 1
-        for (int p = 0; p < ZP::NOUT; p++) {
-            const int ia = ZP::a(p), ib = ZP::b(p);
-            cplx<R> w[E];
+        for (int r = 0; r < 3; r++) {
+#pragma unroll
+            for (int j = 0; j < ZP::NPR; j++) {
+                const int yo = ZP::yop(j);
+                cplx<R> w[E];
+#pragma unroll
+                for (int i = 0; i < E; i++) {
+                    const cplx<R> x = u[G0][i];
+                    const cplx<R> y = yo < 0 ? u[G0 - 1 - yo][i] : u[yo][i];
+                    w[i] = mk<R>(x.x * y.x, x.y * y.y);
+                }
+                product_out(w, ZP::pout(r, j));
+            }
 #pragma unroll
             for (int i = 0; i < E; i++) {
-                cplx<R> x = u[0][i], y = u[0][i];
-#pragma unroll
-                for (int a = 1; a < ZP::NIN; a++) {
-                    x = (ia == a) ? u[a][i] : x;
-                    y = (ib == a) ? u[a][i] : y;
-                }
-                w[i] = mk<R>(x.x * y.x, x.y * y.y);
+                const cplx<R> t0 = u[G0][i];
+                u[G0][i] = u[G0 + 1][i];
+                u[G0 + 1][i] = u[G0 + 2][i];
+                u[G0 + 2][i] = t0;
             }
-            product_out(w, p);
         }
     } else {
         // FP64: selects over the arrays would become dynamic indexing (scratch); unroll instead
@@ -304,28 +353,43 @@ __device__ __forceinline__ Dc dadd(const Dc& a, const Dc& b) { return dmk(a.x + 
 
 // One RK4 stage update of a field value V0 (state), stage field Vs and accumulator tot
 // (single_time_step.cc:54-86 with a,b,c of time_advance.cc:120-153).
+// s0 and a0 are the values of S and acc at this point (a0 unused in stage 1), loaded by the caller.
 template <typename R>
-__device__ __forceinline__ void rk_update(int stage, double dt, double eh, double e1, const Dc& rhs, cplx<R>* S,
-                                          cplx<R>* Vs, cplx<R>* acc)
+__device__ __forceinline__ void rk_store(int stage, double dt, double eh, double e1, const Dc& rhs, const cplx<R>& s0,
+                                         const cplx<R>& a0, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc)
 {
-    Dc V0 = dl(*S);
+    const Dc V0 = dl(s0);
     if (stage == 1) {
         Dc v = dadd(V0, dsc(rhs, 0.5 * dt));
         *Vs = ds<R>(dsc(v, eh));
         *acc = ds<R>(dsc(dsc(rhs, e1), dt / 6));
     } else if (stage == 2) {
         *Vs = ds<R>(dadd(dsc(V0, eh), dsc(rhs, 0.5 * dt)));
-        Dc a = dl(*acc);
-        *acc = ds<R>(dadd(a, dsc(dsc(rhs, eh), dt / 3)));
+        *acc = ds<R>(dadd(dl(a0), dsc(dsc(rhs, eh), dt / 3)));
     } else if (stage == 3) {
         Dc v = dadd(dsc(V0, eh), dsc(rhs, 1.0 * dt));
         *Vs = ds<R>(dsc(v, eh));
-        Dc a = dl(*acc);
-        *acc = ds<R>(dadd(a, dsc(dsc(rhs, eh), dt / 3)));
+        *acc = ds<R>(dadd(dl(a0), dsc(dsc(rhs, eh), dt / 3)));
     } else {
-        Dc a = dadd(dl(*acc), dsc(rhs, dt / 6));
+        Dc a = dadd(dl(a0), dsc(rhs, dt / 6));
         *S = ds<R>(dadd(dsc(V0, e1), a));
     }
+}
+template <typename R>
+__device__ __forceinline__ void rk_update(int stage, double dt, double eh, double e1, const Dc& rhs, cplx<R>* S,
+                                          cplx<R>* Vs, cplx<R>* acc)
+{
+    const cplx<R> s0 = *S;
+    const cplx<R> a0 = stage > 1 ? *acc : czero<R>();
+    rk_store<R>(stage, dt, eh, e1, rhs, s0, a0, S, Vs, acc);
+}
+
+// first register after (i) that xdrop keeps, or -1
+template <int L, int P, int E, int RL> constexpr int xnext(int i)
+{
+    for (int j = i + 1; j < E; j++)
+        if (!xdrop<L, P, E, RL>(j)) return j;
+    return -1;
 }
 
 template <typename R, int L, int P, int T, int MODE>
@@ -403,54 +467,86 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     }
     if (!ok) return;
     const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
-#pragma unroll
-    for (int i = 0; i < E; i++) {
-        const int l = F::out_pos(t, i);
-        if (xdrop<L, P, E, F::RL>(i) || !lkept(g.mx, l)) continue;
-        const double dKx = kx_four(l, g.N1) * g.f1;
-        const double K2 = ksq_f(dKx, dKy, dKz);
+    // Epilogue, software-pipelined over the kept registers: the state and accumulator values of
+    // the next register are loaded before this register's results are stored.  Vector-memory
+    // operations retire through one in-order counter, so a load issued after a store would wait
+    // for the store to complete.
+    constexpr int NC = (MODE == 1) ? 6 : 3;
+    const int stage = sc.stage;
+    auto row_idx = [&](int i) -> long {
         // compact row of l = t + l0: the kept lanes of one register all sit on one side of the
         // dealiased gap (P < L/3), so the row shift is uniform per register
         const int l0 = F::out_pos(0, i);
-        const long idx = (long)(l0 - (l0 >= g.mx.h ? g.mx.N - g.mx.nk : 0)) * g.sx + mu + lane_off;
-        Dc Nu[3];
-        if (MODE == 0) {
+        return (long)(l0 - (l0 >= g.mx.h ? g.mx.N - g.mx.nk : 0)) * g.sx + mu + lane_off;
+    };
+    // Unconditional loads (a branch around them would pull the first use, and its wait, up to the
+    // load): lanes in the dealiased gap read row 0 of their column, and stage 1 reads acc; neither
+    // value is used.
+    auto fetch = [&](int i, cplx<R> (&s0)[NC], cplx<R> (&a0)[NC]) {
+        const long idx = lkept(g.mx, F::out_pos(t, i)) ? row_idx(i) : mu + lane_off;
 #pragma unroll
-            for (int a = 0; a < 3; a++) Nu[a] = dsc(dl(Nacc[a][i]), g.norm);
-        } else {
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
-                Nu[a] = dsc(dadd(np, nm), 0.5 * g.norm);
-            }
+        for (int a = 0; a < NC; a++) {
+            s0[a] = S[a * g.cS + idx];
+            a0[a] = acc[a * g.cS + idx];
         }
-        Dc rhs[3];
-        if (K2 > 0) {
-            Dc kn = dmk(dKx * Nu[0].x + dKy * Nu[1].x + dKz * Nu[2].x, dKx * Nu[0].y + dKy * Nu[1].y + dKz * Nu[2].y);
-            const double Kc[3] = {dKx, dKy, dKz};
+    };
+    cplx<R> nS[NC], nA[NC];
+    fetch(xnext<L, P, E, F::RL>(-1), nS, nA);
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+        if (xdrop<L, P, E, F::RL>(i)) continue;
+        cplx<R> s0[NC], a0[NC];
+#pragma unroll
+        for (int a = 0; a < NC; a++) {
+            s0[a] = nS[a];
+            a0[a] = nA[a];
+        }
+        if (xnext<L, P, E, F::RL>(i) >= 0) fetch(xnext<L, P, E, F::RL>(i), nS, nA);
+        const int l = F::out_pos(t, i);
+        if (lkept(g.mx, l)) {
+            const double dKx = kx_four(l, g.N1) * g.f1;
+            const double K2 = ksq_f(dKx, dKy, dKz);
+            const long idx = row_idx(i);
+            Dc Nu[3];
+            if (MODE == 0) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) Nu[a] = dsc(dl(Nacc[a][i]), g.norm);
+            } else {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
+                    Nu[a] = dsc(dadd(np, nm), 0.5 * g.norm);
+                }
+            }
+            Dc rhs[3];
+            if (K2 > 0) {
+                Dc kn = dmk(dKx * Nu[0].x + dKy * Nu[1].x + dKz * Nu[2].x, dKx * Nu[0].y + dKy * Nu[1].y + dKz * Nu[2].y);
+                const double Kc[3] = {dKx, dKy, dKz};
+#pragma unroll
+                for (int a = 0; a < 3; a++)
+                    rhs[a] = dmk(-(Nu[a].x - Kc[a] * kn.x / K2), -(Nu[a].y - Kc[a] * kn.y / K2));
+            } else {
+#pragma unroll
+                for (int a = 0; a < 3; a++) rhs[a] = dmk(-Nu[a].x, -Nu[a].y);
+            }
+            const double eh = exp(sc.fh[0] * K2), e1 = exp(sc.f1[0] * K2);
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                rhs[a] = dmk(-(Nu[a].x - Kc[a] * kn.x / K2), -(Nu[a].y - Kc[a] * kn.y / K2));
-        } else {
+                rk_store<R>(stage, sc.dt, eh, e1, rhs[a], s0[a], a0[a], S + a * g.cS + idx, Vs + a * g.cS + idx,
+                            acc + a * g.cS + idx);
+            if (MODE == 1) {
+                const double ehb = exp(sc.fh[1] * K2), e1b = exp(sc.f1[1] * K2);
 #pragma unroll
-            for (int a = 0; a < 3; a++) rhs[a] = dmk(-Nu[a].x, -Nu[a].y);
-        }
-        const double eh = exp(sc.fh[0] * K2), e1 = exp(sc.f1[0] * K2);
-#pragma unroll
-        for (int a = 0; a < 3; a++)
-            rk_update<R>(sc.stage, sc.dt, eh, e1, rhs[a], S + a * g.cS + idx, Vs + a * g.cS + idx, acc + a * g.cS + idx);
-        if (MODE == 1) {
-            const double ehb = exp(sc.fh[1] * K2), e1b = exp(sc.f1[1] * K2);
-#pragma unroll
-            for (int a = 0; a < 3; a++) {
-                Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
-                Dc nb = dsc(dmk(np.x - nm.x, np.y - nm.y), 0.5 * g.norm);
-                Dc rb = dmk(-nb.x, -nb.y);
-                rk_update<R>(sc.stage, sc.dt, ehb, e1b, rb, S + (3 + a) * g.cS + idx, Vs + (3 + a) * g.cS + idx,
-                             acc + (3 + a) * g.cS + idx);
+                for (int a = 0; a < 3; a++) {
+                    Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
+                    Dc nb = dsc(dmk(np.x - nm.x, np.y - nm.y), 0.5 * g.norm);
+                    Dc rb = dmk(-nb.x, -nb.y);
+                    rk_store<R>(stage, sc.dt, ehb, e1b, rb, s0[3 + a], a0[3 + a], S + (3 + a) * g.cS + idx,
+                                Vs + (3 + a) * g.cS + idx, acc + (3 + a) * g.cS + idx);
+                }
             }
         }
-        __builtin_amdgcn_sched_barrier(0);   // one mode at a time: bounds the live state loads
+        __builtin_amdgcn_sched_barrier(0);   // one register at a time: bounds the live state values
     }
 }
 
@@ -513,6 +609,13 @@ template <int L, int P, int E, int RL> __host__ __device__ constexpr bool sdrop(
 {
     const int m = i / RL + (i % RL) * (E / RL);
     return m * P >= 2 * L / 3;
+}
+// first register after (i) that sdrop keeps, or -1
+template <int L, int P, int E, int RL> constexpr int snext(int i)
+{
+    for (int j = i + 1; j < E; j++)
+        if (!sdrop<L, P, E, RL>(j)) return j;
+    return -1;
 }
 
 // RB x-forward + epilogue: forward DCT-II / DST-II of the 9 product lines, N_i = sum_j d_j(V_i V_j)
@@ -601,39 +704,67 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
     }
     if (!ok) return;
     const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
-    const cplx<R>* F0 = (sc.stage == 1) ? S : Vs;     // field the stage's rhs is evaluated on
+    const int stage = sc.stage;
+    const cplx<R>* F0 = (stage == 1) ? S : Vs;     // field the stage's rhs is evaluated on
+    // Epilogue software-pipelined like the FOUR x-forward: the next kept register's S, acc and
+    // stage-field values are loaded (unconditionally; lanes past the kept rows read row 0, unused)
+    // before this register's results are stored.
+    auto fetch = [&](int i, cplx<R> (&s0)[4], cplx<R> (&a0)[4], cplx<R> (&f0)[2]) {
+        const int l = F::out_pos(t, i);
+        const long idx = (l < nk ? (long)l * g.sx : 0) + m;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            s0[a] = S[a * g.cS + idx];
+            a0[a] = acc[a * g.cS + idx];
+        }
+        f0[0] = F0[idx];
+        f0[1] = F0[3 * g.cS + idx];
+    };
+    cplx<R> nS[4], nA[4], nF[2];
+    fetch(snext<L, P, E, F::RL>(-1), nS, nA, nF);
 #pragma unroll
     for (int i = 0; i < E; i++) {
-        const int l = F::out_pos(t, i);
-        if (sdrop<L, P, E, F::RL>(i) || l >= nk) continue;
-        const double dKx = l * g.f1;
-        const double K2 = ksq_f(dKx, dKy, dKz);
-        const long idx = (long)l * g.sx + m;
-        Dc Nn[4];
+        if (sdrop<L, P, E, F::RL>(i)) continue;
+        cplx<R> s0[4], a0[4], f0[2];
 #pragma unroll
-        for (int a = 0; a < 4; a++) Nn[a] = dsc(dl(Nacc[a][i]), g.norm);
-        const Dc V1 = dl(F0[idx]), Tf = dl(F0[3 * g.cS + idx]);
-        Nn[0] = dmk(Nn[0].x - Tf.x, Nn[0].y - Tf.y);            // nlin - Force1, Force1 = T
-        Nn[3] = dmk(Nn[3].x - V1.x, Nn[3].y - V1.y);            // T.nlin - T.Force, T.Force = V1
-        Dc pr = dmk(0.0, 0.0);
-        if (K2 > 0) {
-            // div N = Xderiv_Sin(N1) + i Ky N2 + i Kz N3, p = div / K^2
-            const double dr = dKx * Nn[0].x - dKy * Nn[1].y - dKz * Nn[2].y;
-            const double di = dKx * Nn[0].y + dKy * Nn[1].x + dKz * Nn[2].x;
-            pr = dmk(dr / K2, di / K2);
+        for (int a = 0; a < 4; a++) {
+            s0[a] = nS[a];
+            a0[a] = nA[a];
         }
-        // rhs = -(N + grad p): Xderiv_Cos(p) = -Kx p, d_y p = i Ky p, d_z p = i Kz p
-        Dc rhs[4];
-        rhs[0] = dmk(-(Nn[0].x - dKx * pr.x), -(Nn[0].y - dKx * pr.y));
-        rhs[1] = dmk(-(Nn[1].x - dKy * pr.y), -(Nn[1].y + dKy * pr.x));
-        rhs[2] = dmk(-(Nn[2].x - dKz * pr.y), -(Nn[2].y + dKz * pr.x));
-        rhs[3] = dmk(-Nn[3].x, -Nn[3].y);
-        const double eh = exp(sc.fh[0] * K2), e1 = exp(sc.f1[0] * K2);
-        const double ehT = exp(sc.fh[1] * K2), e1T = exp(sc.f1[1] * K2);
+        f0[0] = nF[0];
+        f0[1] = nF[1];
+        if (snext<L, P, E, F::RL>(i) >= 0) fetch(snext<L, P, E, F::RL>(i), nS, nA, nF);
+        const int l = F::out_pos(t, i);
+        if (l < nk) {
+            const double dKx = l * g.f1;
+            const double K2 = ksq_f(dKx, dKy, dKz);
+            const long idx = (long)l * g.sx + m;
+            Dc Nn[4];
 #pragma unroll
-        for (int a = 0; a < 4; a++)
-            rk_update<R>(sc.stage, sc.dt, a < 3 ? eh : ehT, a < 3 ? e1 : e1T, rhs[a], S + a * g.cS + idx,
-                         Vs + a * g.cS + idx, acc + a * g.cS + idx);
+            for (int a = 0; a < 4; a++) Nn[a] = dsc(dl(Nacc[a][i]), g.norm);
+            const Dc V1 = dl(f0[0]), Tf = dl(f0[1]);
+            Nn[0] = dmk(Nn[0].x - Tf.x, Nn[0].y - Tf.y);            // nlin - Force1, Force1 = T
+            Nn[3] = dmk(Nn[3].x - V1.x, Nn[3].y - V1.y);            // T.nlin - T.Force, T.Force = V1
+            Dc pr = dmk(0.0, 0.0);
+            if (K2 > 0) {
+                // div N = Xderiv_Sin(N1) + i Ky N2 + i Kz N3, p = div / K^2
+                const double dr = dKx * Nn[0].x - dKy * Nn[1].y - dKz * Nn[2].y;
+                const double di = dKx * Nn[0].y + dKy * Nn[1].x + dKz * Nn[2].x;
+                pr = dmk(dr / K2, di / K2);
+            }
+            // rhs = -(N + grad p): Xderiv_Cos(p) = -Kx p, d_y p = i Ky p, d_z p = i Kz p
+            Dc rhs[4];
+            rhs[0] = dmk(-(Nn[0].x - dKx * pr.x), -(Nn[0].y - dKx * pr.y));
+            rhs[1] = dmk(-(Nn[1].x - dKy * pr.y), -(Nn[1].y + dKy * pr.x));
+            rhs[2] = dmk(-(Nn[2].x - dKz * pr.y), -(Nn[2].y + dKz * pr.x));
+            rhs[3] = dmk(-Nn[3].x, -Nn[3].y);
+            const double eh = exp(sc.fh[0] * K2), e1 = exp(sc.f1[0] * K2);
+            const double ehT = exp(sc.fh[1] * K2), e1T = exp(sc.f1[1] * K2);
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+                rk_store<R>(stage, sc.dt, a < 3 ? eh : ehT, a < 3 ? e1 : e1T, rhs[a], s0[a], a0[a], S + a * g.cS + idx,
+                            Vs + a * g.cS + idx, acc + a * g.cS + idx);
+        }
         __builtin_amdgcn_sched_barrier(0);
     }
 }

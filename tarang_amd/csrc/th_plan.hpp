@@ -35,6 +35,8 @@ struct th_plan {
     double* radii_dev = nullptr;
     int radii_cap = 0;
     th::Fused* fused = nullptr;
+    int nstates = 0;                 // fused states created on this plan and not yet destroyed
+    bool dead = false;               // th_plan_destroy called while states were alive
     void* scr(int i);
     template <typename R> const th::cplx<R>* twd(int L);
     ~th_plan();
@@ -42,6 +44,7 @@ struct th_plan {
 
 namespace th {
 void release_fused(th_plan* p);
+void plan_free(th_plan* p);           // waits for the plan's stream, frees it, uncounts it in its context
 template <typename R> int plain_forward(th_plan* p, cplx<R>* A, int parity);
 template <typename R> int plain_inverse(th_plan* p, cplx<R>* A, int parity);
 }  // namespace th

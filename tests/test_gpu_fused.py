@@ -50,7 +50,7 @@ def test_upload_download_roundtrip(ctx, N):
         assert np.array_equal(H(out[i]), o.dealias(V[i].copy()))
 
 
-@pytest.mark.parametrize("N", [(16, 16, 16), (32, 32, 32), (48, 32, 24), (64, 64, 64)])
+@pytest.mark.parametrize("N", [(16, 16, 16), (32, 32, 32), (48, 32, 24), (64, 64, 64), (64, 64, 128)])
 def test_fused_rk4_ns_fp64(ctx, N):
     from tarang_amd import Plan, State
     log(f"fused ns fp64 N={N}")
@@ -198,3 +198,26 @@ def test_fused_rb_scft_fp32_mixed(ctx):
     out = s.download()
     for i, ref in enumerate(V + [T]):
         assert rel(H(out[i]), ref) < 2e-5, i
+
+
+def test_plan_destroyed_before_its_state(ctx):
+    """th_plan_destroy on a plan with a live state defers the release to th_state_destroy (the order
+    a garbage collector may pick for a reference cycle), and the state keeps working meanwhile."""
+    from tarang_amd import Plan, State, abi
+    N = (16, 16, 16)
+    o = Oracle(N)
+    V = o.ic_random(5)
+    p = Plan(ctx, N)
+    s = State(p)
+    s.upload([p.to_dev(a) for a in V])
+    lib = abi.lib()
+    assert lib.th_plan_destroy(p.h) == 0          # deferred: the state is alive
+    assert lib.th_plan_destroy(p.h) == 0          # a second call is a no-op
+    s.step(1e-3, nu=1e-2, steps=1)
+    o.rk4_ns(V, 1e-2, 1e-3, steps=1)
+    out = s.download()
+    for i in range(3):
+        assert rel(H(out[i]), V[i]) < 1e-12, i
+    assert lib.th_state_destroy(s.h) == 0         # releases the plan too
+    s.h = None
+    p.h = None
