@@ -27,12 +27,22 @@ template <typename R> __device__ __forceinline__ cplx<R> ld(const cplx<R>* p) { 
 // pass 1: x-inverse.  MODE 0: copy comps; 1: Elsasser Z+ = U + B (comps 0-2), Z- = U - B (3-5);
 // 2: giver G = U masked to the shell sel.
 // ------------------------------------------------------------------------------------------
+// Twiddle tables are staged in LDS once per workgroup: read from global memory in every FFT stage
+// they would cost as many vector-memory instructions as the line data itself.
+template <typename R, int N> __device__ __forceinline__ void stage_tw(cplx<R>* dst, const cplx<R>* __restrict__ src)
+{
+    for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = src[i];
+}
+
 template <typename R, int L, int P, int T, int MODE>
 __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restrict__ S, cplx<R>* __restrict__ A,
                                                   const cplx<R>* __restrict__ tw, ShellSel sel)
 {
     using F = LineFFT<R, L, P, +1>;
     __shared__ cplx<R> lds[L * T];
+    __shared__ cplx<R> twl[L];
+    stage_tw<R, L>(twl, tw);
+    __syncthreads();
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int ntz = (g.nkz + T - 1) / T;
     const int cy = blockIdx.x / ntz;
@@ -67,7 +77,7 @@ __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restric
         }
         v[i] = a;
     }
-    F::template run<T>(v, lds + c, t, tw);
+    F::template run<T>(v, lds + c, t, twl);
     if (ok) {
         cplx<R>* out = A + oc * g.cA + m;
 #pragma unroll
@@ -82,6 +92,9 @@ __global__ __launch_bounds__(T * P) void k_fy_inv(FG g, const cplx<R>* __restric
 {
     using F = LineFFT<R, L, P, +1>;
     __shared__ cplx<R> lds[L * T];
+    __shared__ cplx<R> twl[L];
+    stage_tw<R, L>(twl, tw);
+    __syncthreads();
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int cz = blockIdx.x * T + c;
     const bool ok = cz < g.nkz;
@@ -95,7 +108,7 @@ __global__ __launch_bounds__(T * P) void k_fy_inv(FG g, const cplx<R>* __restric
         const int l = F::in_pos(t, i);
         v[i] = (ok && lkept(g.my, l)) ? a[(long)lq(g.my, l) * g.KZP] : czero<R>();
     }
-    F::template run<T>(v, lds + c, t, tw);
+    F::template run<T>(v, lds + c, t, twl);
     if (ok) {
         cplx<R>* b = B + comp * g.cB + (long)x * g.sB + cz;
 #pragma unroll
@@ -308,6 +321,9 @@ __global__ __launch_bounds__(T * P) void k_fy_fwd(FG g, const cplx<R>* __restric
 {
     using F = LineFFT<R, L, P, -1>;
     __shared__ cplx<R> lds[L * T];
+    __shared__ cplx<R> twl[L];
+    stage_tw<R, L>(twl, tw);
+    __syncthreads();
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int cz = blockIdx.x * T + c;
     const bool ok = cz < g.nkz;
@@ -318,7 +334,7 @@ __global__ __launch_bounds__(T * P) void k_fy_fwd(FG g, const cplx<R>* __restric
     cplx<R> v[F::E];
 #pragma unroll
     for (int i = 0; i < F::E; i++) v[i] = ok ? a[(long)F::in_pos(t, i) * g.KZP] : czero<R>();
-    F::template run<T>(v, lds + c, t, tw);
+    F::template run<T>(v, lds + c, t, twl);
     if (ok) {
         cplx<R>* d = D + comp * g.cA + (long)x * g.sx + cz;
 #pragma unroll
@@ -402,6 +418,9 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     constexpr int NP = (MODE == 0) ? 6 : 9;
     constexpr int NA = (MODE == 0) ? 3 : 6;
     __shared__ cplx<R> lds[L * T];
+    __shared__ cplx<R> twl[L];
+    stage_tw<R, L>(twl, tw);
+    __syncthreads();
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int ntz = (g.nkz + T - 1) / T;
     const int lin = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -437,7 +456,7 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
             const cplx<R>* row = src + (long)F::in_pos(0, i) * g.sx;
             v[i] = ok ? row[lane_off] : czero<R>();
         }
-        F::template run<T>(v, lds + c, t, tw + opaque_zero());
+        F::template run<T>(v, lds + c, t, twl + opaque_zero());
         // contributions (a1 <- K_k1) and (a2 <- K_k2) of product p; k = 0 means Kx (per row)
         int a1, k1, a2, k2;
         if (MODE == 0) {           // P_ij, (i,j) = (0,0),(1,1),(2,2),(0,1),(1,2),(2,0): N_i += iK_j P, N_j += iK_i P
@@ -570,6 +589,11 @@ __global__ __launch_bounds__(T * P) void k_sx_inv(FG g, const cplx<R>* __restric
 {
     using F = LineFFT<R, L, P, +1>;
     __shared__ cplx<R> lds[L * T];
+    __shared__ cplx<R> twl[L];
+    stage_tw<R, L>(twl, tw);
+    __shared__ cplx<R> tw4l[L];
+    stage_tw<R, L>(tw4l, tw4);
+    __syncthreads();
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int ntz = (g.nkz + T - 1) / T;
     const int cy = blockIdx.x / ntz;
@@ -590,9 +614,9 @@ __global__ __launch_bounds__(T * P) void k_sx_inv(FG g, const cplx<R>* __restric
         cplx<R> cm = (ok && k > 0 && L - k < nk) ? col[(long)(L - k) * g.sx] : czero<R>();
         const cplx<R> a = sn ? cm : ck, b = sn ? ck : cm;          // X_k - i X_{N-k}
         const cplx<R> x = mk<R>(a.x + b.y, a.y - b.x);
-        v[i] = cmul(conjg(tw4[k]), x);
+        v[i] = cmul(conjg(tw4l[k]), x);
     }
-    F::template run<T>(v, lds + c, t, tw);
+    F::template run<T>(v, lds + c, t, twl);
     if (ok) {
         cplx<R>* out = A + oc * g.cA + m;
 #pragma unroll
@@ -632,6 +656,9 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
     using F = LineFFT<R, L, P, -1>;
     constexpr int E = F::E;
     __shared__ cplx<R> lds[L * T];
+    __shared__ cplx<R> twl[L];
+    stage_tw<R, L>(twl, tw);
+    __syncthreads();
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int ntz = (g.nkz + T - 1) / T;
     const int lin = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -669,7 +696,7 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
             const R sg = (psin && j0 + t >= L / 2) ? R(-1) : R(1);
             v[i] = cscale(x, sg);
         }
-        F::template run<T>(v, lds + c, t, tw + opaque_zero());
+        F::template run<T>(v, lds + c, t, twl + opaque_zero());
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < E; i++) lds[F::out_pos(t, i) * T + c] = v[i];
@@ -681,7 +708,7 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
         const int a2 = (p == 3) ? 1 : (p == 4 || p == 5) ? 2 : -1;      // second target
         const R al2 = (p == 3 || p == 5) ? R(1) : R(0);
         const R be2 = (p == 4) ? Ky : R(0);
-        const cplx<R>* w4 = tw4 + opaque_zero();
+        const cplx<R>* w4 = tw4 + opaque_zero();   // read per product (no stores in the loop)
 #pragma unroll
         for (int i = 0; i < E; i++) {
             if (sdrop<L, P, E, F::RL>(i)) continue;
@@ -782,6 +809,9 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __rest
     using F = LineFFT<R, L, P, -1>;
     constexpr int E = F::E;
     __shared__ cplx<R> lds[L * T];
+    __shared__ cplx<R> twl[L];
+    stage_tw<R, L>(twl, tw);
+    __syncthreads();
     extern __shared__ double bins[];      // 2 nr doubles: |K| <= kin binned by shell, then |K| > kin by shell
     for (int b = threadIdx.x; b < 2 * nr; b += T * P) bins[b] = 0.0;
     const int c = threadIdx.x % T, t = threadIdx.x / T;
@@ -803,7 +833,7 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __rest
         const cplx<R>* src = D + p * g.cA + m;
 #pragma unroll
         for (int i = 0; i < E; i++) v[i] = ok ? src[(long)F::in_pos(t, i) * g.sx] : czero<R>();
-        F::template run<T>(v, lds + c, t, tw + opaque_zero());
+        F::template run<T>(v, lds + c, t, twl + opaque_zero());
         const int pi = p / 3, pj = p % 3;        // P(pi,pj) = G_pi H_pj -> N_pi += i K_pj P
         R al[3], be[3];
 #pragma unroll
