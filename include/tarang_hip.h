@@ -131,6 +131,14 @@ int th_shell_tr_ns(th_plan *plan, void *const V[3], const double *radii, int nsh
  * 5 B<->U< (W.flux_VF_in_in), 6 U>->B> (flux_VF_out_out), 7 Zp<->Zp> (flux_Elsasser),
  * 8 Zm<->Zm> (W.flux_Elsasser); column 0 is 0. */
 int th_flux_mhd(th_plan *plan, void *const U[3], void *const B[3], const double *radii, int nsph, double *out);
+/* th_flux_scalar == IncVF::Compute_flux_scalar(IncSF&) compute_flux.cc:89-109, the branch
+ * Compute_flux_RB :113-141 takes for the PRLARGE / ULARGE scalings (passive scalar on FOUR, RB on SCFT):
+ * out is 2 x (nsph+1), row 0 = flux_self (as th_flux_ns), row 1 = flux_SF = -(U.grad T<).T> per sphere.
+ * th_shell_tr_scalar == IncVF::Compute_shell_tr_scalar compute_shell_tr.cc:98-117 (Compute_shell_tr_RB
+ * :120-150): Tself as th_shell_tr_ns, Tsf[n][m] = -(U.grad T_n).T over shell m; both (nsh+1)^2 row-major. */
+int th_flux_scalar(th_plan *plan, void *const V[3], void *T, const double *radii, int nsph, double *out);
+int th_shell_tr_scalar(th_plan *plan, void *const V[3], void *T, const double *radii, int nsh, double *Tself,
+                       double *Tsf);
 int th_nusselt(th_plan *plan, void *const V[3], const void *T, double Ra, double Pr, double *Nu);
 
 /* ---- initial conditions on reference-layout device fields ----

@@ -1268,6 +1268,82 @@ void orc_shell_tr_ns(const orc_grid *g, double *const V[3], const double *radii,
     free(row);
 }
 
+/* EnergyTr_Compute_nlin(IncSF& T) ET_compute_nlin.cc:190-234: with H real, G = T< (sine parity),
+ * nlin = D_x F(H1 G) (cos parity) + D_y F(H2 G) + D_z F(H3 G) (sin parity); not dealiased */
+void orc_et_nlin_scalar(const orc_grid *g, double *const Hh[3], const double *G, double *nlin)
+{
+    double *Vr[3], *Gr = alloc_field(g), *tmp = alloc_field(g);
+    for (int i = 0; i < 3; i++) { Vr[i] = alloc_field(g); fcopy(g, Vr[i], Hh[i]); }
+    fcopy(g, Gr, G);
+    orc_inverse(g, Vr[0], 1); orc_inverse(g, Vr[1], 0); orc_inverse(g, Vr[2], 0);
+    orc_inverse(g, Gr, 1);
+    orc_real_mult(g, Vr[0], Gr, nlin);
+    orc_forward(g, nlin, 0);
+    orc_deriv(g, 1, nlin, nlin, 0);
+    orc_real_mult(g, Vr[1], Gr, tmp);
+    orc_forward(g, tmp, 1);
+    orc_deriv(g, 2, tmp, tmp, 0);
+    fadd(g, nlin, tmp);
+    orc_real_mult(g, Vr[2], Gr, tmp);
+    orc_forward(g, tmp, 1);
+    orc_deriv(g, 3, tmp, tmp, 0);
+    fadd(g, nlin, tmp);
+    for (int i = 0; i < 3; i++) free(Vr[i]);
+    free(Gr); free(tmp);
+}
+
+/* Shell_mult_all for one component (universal_ET.cc:160-207 with a scalar pair) */
+static void shell_mult_all_1(const orc_grid *g, const double *A, const double *B, const double *radii, int nr,
+                             double *result)
+{
+    int N1 = g->N1, N2 = g->N2, H = H3(g);
+    int kin = orc_max_radius_inside(g);
+    for (int i = 0; i < nr; i++) result[i] = 0;
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 < H; l3++) {
+                double km = orc_kmag(g, l1, l2, l3);
+                if (km <= kin) {
+                    int si = get_shell_index(km, radii, nr);
+                    result[si] += 2 * orc_multiplicity(g, l1, l2, l3) * re_prod(A, B, IDX(g, l1, l2, l3));
+                }
+            }
+}
+
+/* Compute_flux_scalar compute_flux.cc:89-109, which Compute_flux_RB :113-141 runs for the PRLARGE and
+ * ULARGE scalings: flux_self = Compute_flux(), flux_sf[s] = -(U.grad T<_s) . T over |K| > R_s */
+void orc_flux_scalar(const orc_grid *g, double *const V[3], const double *T, const double *radii, int nsph,
+                     double *flux_self, double *flux_sf)
+{
+    double *G = alloc_field(g), *N = alloc_field(g);
+    orc_flux_ns(g, V, radii, nsph, flux_self);
+    flux_sf[0] = 0.0;
+    for (int s = 1; s <= nsph; s++) {
+        orc_fill_array_shell(g, T, G, 0, radii[s]);
+        orc_et_nlin_scalar(g, V, G, N);
+        flux_sf[s] = -orc_shell_mult_single(g, N, T, radii[s], 10000);
+    }
+    free(G); free(N);
+}
+
+/* Compute_shell_tr_scalar compute_shell_tr.cc:98-117 (Compute_shell_tr_RB :120-150 for PRLARGE / ULARGE):
+ * Tself = Compute_shell_tr(); Tsf[n][m] = -(U.grad T_n) . T over shell m, n = 1..nsh-1 */
+void orc_shell_tr_scalar(const orc_grid *g, double *const V[3], const double *T, const double *radii, int nsh,
+                         double *Tself, double *Tsf)
+{
+    double *G = alloc_field(g), *N = alloc_field(g);
+    double *row = (double *)calloc(nsh + 1, sizeof(double));
+    orc_shell_tr_ns(g, V, radii, nsh, Tself);
+    for (int i = 0; i < (nsh + 1) * (nsh + 1); i++) Tsf[i] = 0;
+    for (int n = 1; n < nsh; n++) {
+        orc_fill_array_shell(g, T, G, radii[n - 1], radii[n]);
+        orc_et_nlin_scalar(g, V, G, N);
+        shell_mult_all_1(g, N, T, radii, nsh + 1, row);
+        for (int m = 0; m <= nsh; m++) Tsf[n * (nsh + 1) + m] = -row[m];
+    }
+    free(G); free(N); free(row);
+}
+
 /* Compute_flux(IncVF& W) compute_flux.cc:148-278 (U = velocity, B = magnetic field) */
 void orc_flux_mhd(const orc_grid *g, double *U[3], double *B[3], const double *radii, int nsph,
                   double *u2u, double *b2b, double *u2b_io, double *u2b_ii,

@@ -74,6 +74,9 @@ def lib():
             "orc_shell_tr_ns": (None, [p, pp, vp, i, vp]),
             "orc_flux_mhd": (None, [p, pp, pp, vp, i] + [vp] * 9),
             "orc_et_radii": (None, [p, i, i, vp]),
+            "orc_et_nlin_scalar": (None, [p, pp, vp, vp]),
+            "orc_flux_scalar": (None, [p, pp, vp, vp, i, vp, vp]),
+            "orc_shell_tr_scalar": (None, [p, pp, vp, vp, i, vp, vp]),
             "orc_nlin_rb": (None, [p, pp, vp, pp, vp]),
             "orc_rk4_step_rb": (None, [p, pp, vp, d, d, d]),
             "orc_nusselt": (d, [p, pp, vp, d, d]),
@@ -286,6 +289,27 @@ class Oracle:
         N = self.zeros(3)
         self.L.orc_et_nlin(self.gp, _pp(H), _pp(G), _pp(N))
         return N
+
+    def et_nlin_scalar(self, H, G):
+        N = self.zeros()
+        self.L.orc_et_nlin_scalar(self.gp, _pp(H), _ptr(G), _ptr(N))
+        return N
+
+    def flux_scalar(self, V, T, radii):
+        """(flux_self, flux_SF) of Compute_flux_scalar / Compute_flux_RB for spheres radii[1..]."""
+        nsph = len(radii) - 1
+        fs, ft = np.zeros(nsph + 1), np.zeros(nsph + 1)
+        self.L.orc_flux_scalar(self.gp, _pp(V), _ptr(T), radii.ctypes.data_as(C.c_void_p), nsph,
+                               fs.ctypes.data_as(C.c_void_p), ft.ctypes.data_as(C.c_void_p))
+        return fs, ft
+
+    def shell_tr_scalar(self, V, T, radii):
+        """(T_self, T_SF), each (nsh+1)^2, of Compute_shell_tr_scalar / Compute_shell_tr_RB."""
+        nsh = len(radii) - 1
+        Ts, Tt = np.zeros((nsh + 1, nsh + 1)), np.zeros((nsh + 1, nsh + 1))
+        self.L.orc_shell_tr_scalar(self.gp, _pp(V), _ptr(T), radii.ctypes.data_as(C.c_void_p), nsh,
+                                   Ts.ctypes.data_as(C.c_void_p), Tt.ctypes.data_as(C.c_void_p))
+        return Ts, Tt
 
     def flux_ns(self, V, radii):
         nsph = len(radii) - 1

@@ -87,6 +87,11 @@ void   orc_shell_mult_all(const orc_grid *g, double *const A[3], double *const B
 void   orc_et_nlin(const orc_grid *g, double *const H[3], double *const G[3], double *nlin[3]);
 void   orc_flux_ns(const orc_grid *g, double *const V[3], const double *radii, int nsph, double *flux);
 void   orc_shell_tr_ns(const orc_grid *g, double *const V[3], const double *radii, int nsh, double *T);
+void   orc_et_nlin_scalar(const orc_grid *g, double *const H[3], const double *G, double *nlin);
+void   orc_flux_scalar(const orc_grid *g, double *const V[3], const double *T, const double *radii, int nsph,
+                       double *flux_self, double *flux_sf);
+void   orc_shell_tr_scalar(const orc_grid *g, double *const V[3], const double *T, const double *radii, int nsh,
+                           double *Tself, double *Tsf);
 void   orc_flux_mhd(const orc_grid *g, double *U[3], double *B[3], const double *radii, int nsph,
                     double *u2u, double *b2b, double *u2b_in_out, double *u2b_in_in,
                     double *b2u_in_out, double *b2u_in_in, double *u_out_b_out,

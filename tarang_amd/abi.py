@@ -93,6 +93,8 @@ def lib():
         "th_flux_ns": (i, [vp, pp, dp, i, dp]),
         "th_shell_tr_ns": (i, [vp, pp, dp, i, dp]),
         "th_flux_mhd": (i, [vp, pp, pp, dp, i, dp]),
+        "th_flux_scalar": (i, [vp, pp, vp, dp, i, dp]),
+        "th_shell_tr_scalar": (i, [vp, pp, vp, dp, i, dp, dp]),
         "th_nusselt": (i, [vp, pp, vp, d, d, dp]),
         "th_ic_taylor_green": (i, [vp, pp, i, d]),
         "th_ic_random": (i, [vp, pp, C.c_ulonglong, d, d]),
@@ -390,6 +392,27 @@ class Plan:
         _check(lib().th_flux_mhd(self.h, _pp(U), _pp(B), rp, len(r) - 1,
                                  out.ctypes.data_as(C.POINTER(C.c_double))), "th_flux_mhd")
         return out
+
+    def flux_scalar(self, V, T, radii):
+        """(flux_self, flux_SF) over spheres 0..nsph (Compute_flux_scalar / Compute_flux_RB)."""
+        import numpy as np
+        r, rp = _dbl(radii)
+        out = np.zeros((2, len(r)))
+        self._pre()
+        _check(lib().th_flux_scalar(self.h, _pp(V), _ptr(T), rp, len(r) - 1,
+                                    out.ctypes.data_as(C.POINTER(C.c_double))), "th_flux_scalar")
+        return out[0], out[1]
+
+    def shell_tr_scalar(self, V, T, radii):
+        """(T_self, T_SF), each (nsh+1)^2 (Compute_shell_tr_scalar / Compute_shell_tr_RB)."""
+        import numpy as np
+        r, rp = _dbl(radii)
+        nsh = len(r) - 1
+        Ts, Tt = np.zeros((nsh + 1, nsh + 1)), np.zeros((nsh + 1, nsh + 1))
+        self._pre()
+        _check(lib().th_shell_tr_scalar(self.h, _pp(V), _ptr(T), rp, nsh, Ts.ctypes.data_as(C.POINTER(C.c_double)),
+                                        Tt.ctypes.data_as(C.POINTER(C.c_double))), "th_shell_tr_scalar")
+        return Ts, Tt
 
     def shell_tr_ns(self, V, radii):
         import numpy as np

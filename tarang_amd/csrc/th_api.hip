@@ -583,6 +583,31 @@ template <typename R> struct Ops {
         return OK;
     }
 
+    // IncVF::EnergyTr_Compute_nlin(IncSF& T) ET_compute_nlin.cc:190-234 (helper H, scalar giver G of
+    // sine parity): nl = D_x F(H1 G) (cos parity) + D_y F(H2 G) + D_z F(H3 G) (sin parity); not dealiased
+    int et_nlin_scalar(cplx<R>* const Hh[3], const cplx<R>* G, cplx<R>* nl)
+    {
+        cplx<R>* Vr[3] = {S(SC_VR0), S(SC_VR0 + 1), S(SC_VR0 + 2)};
+        cplx<R>* Gr = S(SC_TR);
+        cplx<R>* tmp = S(SC_SFT);
+        for (int i = 0; i < 3; i++) TRY(copy(Vr[i], Hh[i]));
+        TRY(copy(Gr, G));
+        TRY(inv(Vr[0], 1)); TRY(inv(Vr[1], 0)); TRY(inv(Vr[2], 0));
+        TRY(inv(Gr, 1));
+        TRY(rmul(Vr[0], Gr, nl));
+        TRY(fwd(nl, 0));
+        TRY(der(1, nl, nl, 0));
+        TRY(rmul(Vr[1], Gr, tmp));
+        TRY(fwd(tmp, 1));
+        TRY(der(2, tmp, tmp, 0));
+        TRY(axpy(nl, 1.0, tmp));
+        TRY(rmul(Vr[2], Gr, tmp));
+        TRY(fwd(tmp, 1));
+        TRY(der(3, tmp, tmp, 0));
+        TRY(axpy(nl, 1.0, tmp));
+        return OK;
+    }
+
     // ---------------- steppers (time_advance.cc, single_time_step.cc) ----------------
     int nc = 0;
     cplx<R>* F[7];
@@ -984,6 +1009,38 @@ extern "C" int th_flux_mhd(th_plan* p, void* const U[3], void* const B[3], const
     return DISPATCH(p, do_flux_mhd, p, U, B, radii, nsph, out);
 }
 
+// Compute_flux_scalar compute_flux.cc:89-109 (what Compute_flux_RB :113-141 runs for the PRLARGE and
+// ULARGE scalings): out row 0 = flux_self (Compute_flux), row 1 = flux_SF with, for each sphere s,
+// G = T< (Fill_in_sphere(s, T)), N = U.grad G and flux_SF[s] = -Prod_out_sphere_nlinV(s, T).
+template <typename R> static int do_flux_scalar(th_plan* p, void* const V[3], void* T, const double* radii, int nsph,
+                                                double* out)
+{
+    TRY(do_flux_ns<R>(p, V, radii, nsph, out));
+    Ops<R> o(p);
+    cplx<R>* v[3] = {CP(R, V[0]), CP(R, V[1]), CP(R, V[2])};
+    cplx<R>* th = CP(R, T);
+    cplx<R>* G = o.S(SC_G0);
+    cplx<R>* Nn = o.S(SC_NE0);
+    double* sf = out + nsph + 1;
+    sf[0] = 0;
+    for (int s = 1; s <= nsph; s++) {
+        TRY(k_fill_shell<R>(p->g, th, G, 0.0, radii[s], o.st));
+        TRY(o.et_nlin_scalar(v, G, Nn));
+        const cplx<R>* a[3] = {Nn, nullptr, nullptr};
+        const cplx<R>* b[3] = {th, nullptr, nullptr};
+        double r;
+        TRY(reduce_shell_mult_single<R>(p->g, a, b, radii[s], 10000.0, &r, p->rs, o.st));
+        sf[s] = -r;
+    }
+    return OK;
+}
+extern "C" int th_flux_scalar(th_plan* p, void* const V[3], void* T, const double* radii, int nsph, double* out)
+{
+    TRY(check_plan(p));
+    if (!V || !T || !radii || nsph < 1 || !out) { set_error("th_flux_scalar: bad args"); return E_ARG; }
+    return DISPATCH(p, do_flux_scalar, p, V, T, radii, nsph, out);
+}
+
 template <typename R> static int do_shell_tr(th_plan* p, void* const V[3], const double* radii, int nsh, double* T)
 {
     Ops<R> o(p);
@@ -1008,6 +1065,37 @@ extern "C" int th_shell_tr_ns(th_plan* p, void* const V[3], const double* radii,
     TRY(check_plan(p));
     if (!radii || nsh < 2 || !T) { set_error("th_shell_tr_ns: bad args"); return E_ARG; }
     return DISPATCH(p, do_shell_tr, p, V, radii, nsh, T);
+}
+
+// Compute_shell_tr_scalar compute_shell_tr.cc:98-117 (Compute_shell_tr_RB :120-150 for PRLARGE and
+// ULARGE): Tself = Compute_shell_tr(); Tsf[n][m] = -Shell_mult_all(U.grad T_n, T)[m], n = 1..nsh-1.
+template <typename R> static int do_shell_tr_scalar(th_plan* p, void* const V[3], void* T, const double* radii,
+                                                    int nsh, double* Tself, double* Tsf)
+{
+    TRY(do_shell_tr<R>(p, V, radii, nsh, Tself));
+    Ops<R> o(p);
+    cplx<R>* v[3] = {CP(R, V[0]), CP(R, V[1]), CP(R, V[2])};
+    cplx<R>* th = CP(R, T);
+    cplx<R>* G = o.S(SC_G0);
+    cplx<R>* Nn = o.S(SC_NE0);
+    std::vector<double> row(nsh + 1);
+    for (int i = 0; i < (nsh + 1) * (nsh + 1); i++) Tsf[i] = 0;
+    for (int nn = 1; nn < nsh; nn++) {
+        TRY(k_fill_shell<R>(p->g, th, G, radii[nn - 1], radii[nn], o.st));
+        TRY(o.et_nlin_scalar(v, G, Nn));
+        const cplx<R>* a[3] = {Nn, nullptr, nullptr};
+        const cplx<R>* b[3] = {th, nullptr, nullptr};
+        TRY(reduce_shell_mult_all<R>(p->g, a, b, p->radii_dev, nsh + 1, row.data(), p->rs, o.st));
+        for (int m = 0; m <= nsh; m++) Tsf[nn * (nsh + 1) + m] = -row[m];
+    }
+    return OK;
+}
+extern "C" int th_shell_tr_scalar(th_plan* p, void* const V[3], void* T, const double* radii, int nsh, double* Tself,
+                                  double* Tsf)
+{
+    TRY(check_plan(p));
+    if (!V || !T || !radii || nsh < 2 || !Tself || !Tsf) { set_error("th_shell_tr_scalar: bad args"); return E_ARG; }
+    return DISPATCH(p, do_shell_tr_scalar, p, V, T, radii, nsh, Tself, Tsf);
 }
 
 extern "C" int th_nusselt(th_plan* p, void* const V[3], const void* T, double Ra, double Pr, double* Nu)

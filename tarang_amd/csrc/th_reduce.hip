@@ -105,7 +105,10 @@ __global__ __launch_bounds__(RBLK) void k_binned(GeoR g, const cplx<R>* __restri
                 while (idx < nr && !(km <= radii[idx])) idx++;
                 if (idx >= nbins) continue;
                 double f = mult_r(g, l1, l2, l3);
-                val = 2 * f * (rdot(A0[i], B0[i]) + rdot(A1[i], B1[i]) + rdot(A2[i], B2[i]));
+                double d = rdot(A0[i], B0[i]);
+                if (A1) d += rdot(A1[i], B1[i]);          // null: one-component (scalar) pair
+                if (A2) d += rdot(A2[i], B2[i]);
+                val = 2 * f * d;
             }
             if (idx != cur) {
                 if (cur >= 0) {
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(RBLK) void k_binned(GeoR g, const cplx<R>* __restri
     for (int b = threadIdx.x; b < 2 * nbins; b += RBLK) partial[(long)blockIdx.x * 2 * nbins + b] = sh[b];
 }
 
-// MODE 0: energy weights (Get_local_energy), MODE 1: Sn, MODE 2: shell_mult_single over 3 comps
+// MODE 0: energy weights (Get_local_energy), MODE 1: Sn, MODE 2: shell_mult_single over 3 comps (or 1)
 template <typename R, int MODE>
 __global__ __launch_bounds__(RBLK) void k_scalar(GeoR g, const cplx<R>* __restrict__ A0, const cplx<R>* __restrict__ A1,
                                                  const cplx<R>* __restrict__ A2, const cplx<R>* __restrict__ B0,
@@ -164,7 +167,10 @@ __global__ __launch_bounds__(RBLK) void k_scalar(GeoR g, const cplx<R>* __restri
         } else {
             double km = kmag_r(g, l1, l2, l3);
             if (!((km > inner) && (km <= outer))) continue;
-            v = 2 * mult_r(g, l1, l2, l3) * (rdot(A0[i], B0[i]) + rdot(A1[i], B1[i]) + rdot(A2[i], B2[i]));
+            double d = rdot(A0[i], B0[i]);
+            if (A1) d += rdot(A1[i], B1[i]);              // null: one-component (scalar) pair
+            if (A2) d += rdot(A2[i], B2[i]);
+            v = 2 * mult_r(g, l1, l2, l3) * d;
         }
         double t = s + v;
         c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);

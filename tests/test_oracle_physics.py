@@ -301,3 +301,56 @@ def test_rb_linear_growth_rate():
     E2 = o.energy(T)
     rate = math.log(E2 / E1) / (2 * 100 * dt)
     assert abs(rate - sigma) < 1e-6 * abs(sigma) + 1e-9
+
+
+def _kmag_grid(o):
+    N1, N2, H = o.shape
+    K = np.empty((N1, N2, H))
+    for l1 in range(N1):
+        for l2 in range(N2):
+            for l3 in range(H):
+                K[l1, l2, l3] = o.kmag(l1, l2, l3)
+    return K
+
+
+@pytest.mark.parametrize("basis", ["SCFT", "FOUR"])
+def test_scalar_transfer_antisymmetry_conservation_and_flux_identity(basis):
+    """Compute_shell_tr_scalar / Compute_flux_scalar (compute_shell_tr.cc:98-117, compute_flux.cc:89-141,
+    the RB drivers for PRLARGE/ULARGE): with U divergence-free, T_SF(n,m) = -T_SF(m,n), sum T_SF = 0, and
+    with every mode inside kmax_inside Pi_SF(R_i) = sum_{n<=i} sum_{m>i} T_SF(n,m); the velocity rows are
+    Compute_flux / Compute_shell_tr.  On SCFT the scalar transfer of the whole field is the scalar part
+    of the RB nonlinear term (compute_nlin_SF.cc:80-177)."""
+    N = (16, 16, 16)
+    if basis == "SCFT":
+        f = (np.pi, np.pi / np.sqrt(2), np.pi / np.sqrt(2))
+        o = Oracle(N, kfactor=f, basis="SCFT")
+        V = o.zeros(3)
+        T = o.dealias(o.ic_random_scalar(4, 0.5))
+        o.rk4_rb(V, T, 1e5, 1.0, 1e-2, steps=3)
+    else:
+        o = Oracle(N)
+        V = o.ic_random(5, E0=0.5, kp=2.0)
+        T = o.dealias(o.ic_random_scalar(6, 0.5))
+    kin = o.max_radius_inside()
+    K = _kmag_grid(o)
+    for a in V + [T]:
+        a[K > kin] = 0
+    assert np.abs(V[0]).max() > 0 and np.abs(T).max() > 0
+    nsh = kin + 2
+    radii = np.concatenate([np.arange(nsh), [10000.0]]).astype(float)
+    Ts, Tt = o.shell_tr_scalar(V, T, radii)
+    assert np.array_equal(Ts, o.shell_tr_ns(V, radii))
+    Tin = Tt[1:nsh, 1:nsh]
+    scale = np.abs(Tin).max()
+    assert scale > 0
+    assert np.abs(Tin + Tin.T).max() < 1e-12 * scale
+    assert abs(Tt.sum()) < 1e-12 * scale
+    fs, ft = o.flux_scalar(V, T, radii)
+    assert np.array_equal(fs, o.flux_ns(V, radii))
+    assert ft[nsh] == 0.0
+    for i in range(1, nsh - 1):
+        assert abs(ft[i] - Tt[1:i + 1, i + 1:].sum()) < 1e-12 * np.abs(ft).max()
+    if basis == "SCFT":
+        ou = Oracle(N, kfactor=f, basis="SCFT", dealias=False)
+        _, nT = ou.nlin_rb(V, T)
+        assert np.abs(ou.et_nlin_scalar(V, T) - nT).max() <= 1e-14 * np.abs(nT).max()
