@@ -417,10 +417,13 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     constexpr int E = F::E;
     constexpr int NP = (MODE == 0) ? 6 : 9;
     constexpr int NA = (MODE == 0) ? 3 : 6;
+    constexpr int NX = (MODE == 1) ? 4 : 2;
     __shared__ cplx<R> lds[L * T];
     __shared__ cplx<R> twl[L];
+    // exp(c Kx^2) per row for the integrating factors c = fh[0], f1[0] (MHD: and fh[1], f1[1]):
+    // exp(c K^2) = exp(c Kx^2) exp(c (Ky^2 + Kz^2)), the second factor once per thread
+    __shared__ double exl[NX][L];
     stage_tw<R, L>(twl, tw);
-    __syncthreads();
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int ntz = (g.nkz + T - 1) / T;
     const int lin = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -430,6 +433,16 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     const bool ok = cz < g.nkz;
     const long m = (long)cy * g.KZP + cz;
     const StageCoef sc = coefs[box];
+    for (int l = threadIdx.x; l < L; l += T * P) {
+        const double kx = kx_four(l, g.N1) * g.f1, kx2 = kx * kx;
+        exl[0][l] = exp(sc.fh[0] * kx2);
+        exl[1][l] = exp(sc.f1[0] * kx2);
+        if constexpr (MODE == 1) {
+            exl[2][l] = exp(sc.fh[1] * kx2);
+            exl[3][l] = exp(sc.f1[1] * kx2);
+        }
+    }
+    __syncthreads();
     D += box * g.bA;
     S += box * g.bS;
     Vs += box * g.bS;
@@ -499,18 +512,22 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
         return (long)(l0 - (l0 >= g.mx.h ? g.mx.N - g.mx.nk : 0)) * g.sx + mu + lane_off;
     };
     // Unconditional loads (a branch around them would pull the first use, and its wait, up to the
-    // load): lanes in the dealiased gap read row 0 of their column, and stage 1 reads acc; neither
-    // value is used.
+    // load): lanes in the dealiased gap read row 0 of their column, and stage 1, which has no
+    // accumulator yet, reads S a second time (a cache hit); neither value is used.
+    const cplx<R>* accr = stage > 1 ? acc : S;
     auto fetch = [&](int i, cplx<R> (&s0)[NC], cplx<R> (&a0)[NC]) {
         const long idx = lkept(g.mx, F::out_pos(t, i)) ? row_idx(i) : mu + lane_off;
 #pragma unroll
         for (int a = 0; a < NC; a++) {
             s0[a] = S[a * g.cS + idx];
-            a0[a] = acc[a * g.cS + idx];
+            a0[a] = accr[a * g.cS + idx];
         }
     };
     cplx<R> nS[NC], nA[NC];
     fetch(xnext<L, P, E, F::RL>(-1), nS, nA);
+    const double kyz2 = dKy * dKy + dKz * dKz;
+    const double eyh = exp(sc.fh[0] * kyz2), ey1 = exp(sc.f1[0] * kyz2);
+    const double eyhb = (MODE == 1) ? exp(sc.fh[1] * kyz2) : 0.0, ey1b = (MODE == 1) ? exp(sc.f1[1] * kyz2) : 0.0;
 #pragma unroll
     for (int i = 0; i < E; i++) {
         if (xdrop<L, P, E, F::RL>(i)) continue;
@@ -539,22 +556,23 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
             }
             Dc rhs[3];
             if (K2 > 0) {
-                Dc kn = dmk(dKx * Nu[0].x + dKy * Nu[1].x + dKz * Nu[2].x, dKx * Nu[0].y + dKy * Nu[1].y + dKz * Nu[2].y);
+                const double iK2 = 1.0 / K2;
+                Dc kn = dmk((dKx * Nu[0].x + dKy * Nu[1].x + dKz * Nu[2].x) * iK2,
+                            (dKx * Nu[0].y + dKy * Nu[1].y + dKz * Nu[2].y) * iK2);
                 const double Kc[3] = {dKx, dKy, dKz};
 #pragma unroll
-                for (int a = 0; a < 3; a++)
-                    rhs[a] = dmk(-(Nu[a].x - Kc[a] * kn.x / K2), -(Nu[a].y - Kc[a] * kn.y / K2));
+                for (int a = 0; a < 3; a++) rhs[a] = dmk(-(Nu[a].x - Kc[a] * kn.x), -(Nu[a].y - Kc[a] * kn.y));
             } else {
 #pragma unroll
                 for (int a = 0; a < 3; a++) rhs[a] = dmk(-Nu[a].x, -Nu[a].y);
             }
-            const double eh = exp(sc.fh[0] * K2), e1 = exp(sc.f1[0] * K2);
+            const double eh = exl[0][l] * eyh, e1 = exl[1][l] * ey1;
 #pragma unroll
             for (int a = 0; a < 3; a++)
                 rk_store<R>(stage, sc.dt, eh, e1, rhs[a], s0[a], a0[a], S + a * g.cS + idx, Vs + a * g.cS + idx,
                             acc + a * g.cS + idx);
-            if (MODE == 1) {
-                const double ehb = exp(sc.fh[1] * K2), e1b = exp(sc.f1[1] * K2);
+            if constexpr (MODE == 1) {
+                const double ehb = exl[2][l] * eyhb, e1b = exl[3][l] * ey1b;
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
@@ -657,8 +675,10 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
     constexpr int E = F::E;
     __shared__ cplx<R> lds[L * T];
     __shared__ cplx<R> twl[L];
+    // exp(c Kx^2) per row for c = fh[0], f1[0] (velocity) and fh[1], f1[1] (temperature); the
+    // (Ky, Kz) factor of exp(c K^2) is computed once per thread
+    __shared__ double exl[4][L];
     stage_tw<R, L>(twl, tw);
-    __syncthreads();
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int ntz = (g.nkz + T - 1) / T;
     const int lin = xcd_tile(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
@@ -668,6 +688,15 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
     const bool ok = cz < g.nkz;
     const long m = (long)cy * g.KZP + cz;
     const StageCoef sc = coefs[box];
+    for (int l = threadIdx.x; l < L; l += T * P) {
+        const double kx = l * g.f1, kx2 = kx * kx;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            exl[2 * q][l] = exp(sc.fh[q] * kx2);
+            exl[2 * q + 1][l] = exp(sc.f1[q] * kx2);
+        }
+    }
+    __syncthreads();
     D += box * g.bA;
     S += box * g.bS;
     Vs += box * g.bS;
@@ -736,19 +765,23 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
     // Epilogue software-pipelined like the FOUR x-forward: the next kept register's S, acc and
     // stage-field values are loaded (unconditionally; lanes past the kept rows read row 0, unused)
     // before this register's results are stored.
+    const cplx<R>* accr = stage > 1 ? acc : S;         // stage 1: no accumulator yet (value unused)
     auto fetch = [&](int i, cplx<R> (&s0)[4], cplx<R> (&a0)[4], cplx<R> (&f0)[2]) {
         const int l = F::out_pos(t, i);
         const long idx = (l < nk ? (long)l * g.sx : 0) + m;
 #pragma unroll
         for (int a = 0; a < 4; a++) {
             s0[a] = S[a * g.cS + idx];
-            a0[a] = acc[a * g.cS + idx];
+            a0[a] = accr[a * g.cS + idx];
         }
         f0[0] = F0[idx];
         f0[1] = F0[3 * g.cS + idx];
     };
     cplx<R> nS[4], nA[4], nF[2];
     fetch(snext<L, P, E, F::RL>(-1), nS, nA, nF);
+    const double kyz2 = dKy * dKy + dKz * dKz;
+    const double eyh = exp(sc.fh[0] * kyz2), ey1 = exp(sc.f1[0] * kyz2);
+    const double eyhT = exp(sc.fh[1] * kyz2), ey1T = exp(sc.f1[1] * kyz2);
 #pragma unroll
     for (int i = 0; i < E; i++) {
         if (sdrop<L, P, E, F::RL>(i)) continue;
@@ -775,9 +808,10 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
             Dc pr = dmk(0.0, 0.0);
             if (K2 > 0) {
                 // div N = Xderiv_Sin(N1) + i Ky N2 + i Kz N3, p = div / K^2
+                const double iK2 = 1.0 / K2;
                 const double dr = dKx * Nn[0].x - dKy * Nn[1].y - dKz * Nn[2].y;
                 const double di = dKx * Nn[0].y + dKy * Nn[1].x + dKz * Nn[2].x;
-                pr = dmk(dr / K2, di / K2);
+                pr = dmk(dr * iK2, di * iK2);
             }
             // rhs = -(N + grad p): Xderiv_Cos(p) = -Kx p, d_y p = i Ky p, d_z p = i Kz p
             Dc rhs[4];
@@ -785,8 +819,8 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
             rhs[1] = dmk(-(Nn[1].x - dKy * pr.y), -(Nn[1].y + dKy * pr.x));
             rhs[2] = dmk(-(Nn[2].x - dKz * pr.y), -(Nn[2].y + dKz * pr.x));
             rhs[3] = dmk(-Nn[3].x, -Nn[3].y);
-            const double eh = exp(sc.fh[0] * K2), e1 = exp(sc.f1[0] * K2);
-            const double ehT = exp(sc.fh[1] * K2), e1T = exp(sc.f1[1] * K2);
+            const double eh = exl[0][l] * eyh, e1 = exl[1][l] * ey1;
+            const double ehT = exl[2][l] * eyhT, e1T = exl[3][l] * ey1T;
 #pragma unroll
             for (int a = 0; a < 4; a++)
                 rk_store<R>(stage, sc.dt, a < 3 ? eh : ehT, a < 3 ? e1 : e1T, rhs[a], s0[a], a0[a], S + a * g.cS + idx,
