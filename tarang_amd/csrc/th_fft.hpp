@@ -106,7 +106,7 @@ template <int L> struct ZPad {
         for (int s = 0; s < 8; s++) m = map(s, L - 1) + 1 > m ? map(s, L - 1) + 1 : m;
         return m;
     }
-    static constexpr int line = L == 16 ? 18 : L == 24 ? 30 : L == 32 ? 36 : L == 48 ? 60 : L == 64 ? 72 : L == 96 ? 120 : L == 128 ? 144 : L == 192 ? 240 : L == 256 ? 318 : L;
+    static constexpr int line = L == 16 ? 18 : L == 24 ? 30 : L == 32 ? 36 : L == 48 ? 60 : L == 64 ? 72 : L == 96 ? 120 : L == 128 ? 144 : L == 192 ? 240 : L == 256 ? 318 : L + 1;
     static_assert(line >= need(), "ZPad line stride too short");
 };
 

@@ -157,11 +157,6 @@ template <> struct ZProd<3> {
     static constexpr int pout(int r, int j) { return 3 * j + r; }
 };
 
-template <typename R> __device__ __forceinline__ cplx<R> shfl_c(const cplx<R>& v, int src, int width)
-{
-    return mk<R>(__shfl(v.x, src, width), __shfl(v.y, src, width));
-}
-
 // Lane t of a z line holds the points at positions t + P m, m = 0..E-1, in a register order fixed
 // by the radix Rr of the FFT stage that reads (first stage) or writes (last stage) them.
 template <int E, int Rr> __device__ __forceinline__ constexpr int zreg(int m)
@@ -181,6 +176,7 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
     constexpr int E = FI::E;
     static_assert(64 % P == 0, "a z line's threads must share one wavefront");
     constexpr int LZS = Lay::line;   // padded line stride in LDS
+    static_assert(LZS >= LZ + 1, "the mirror exchange uses positions 0..LZ of the line's LDS region");
     __shared__ cplx<R> lds[TZ * LZS];
     // Twiddles live in LDS: W_LZ^j for the line FFTs, then W_N3^k (k < LZ) for the half-length
     // packing.  Global twiddle loads inside the product loop would queue behind the product stores
@@ -195,9 +191,9 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
     B += blockIdx.y * g.bB;
     Cout += blockIdx.y * g.bC;
     cplx<R>* Z = lds + c * LZS;
-    // The mirror point LZ - (t + P m) sits in lane (P - t) mod P at m' = E - 1 - m (t > 0) or
-    // m' = (E - m) mod E (t = 0): the half-length packing reads it by a lane shuffle.
-    const int tp = (P - t) % P;
+    // The half-length packing pairs point k = t + P m with LZ - k.  The line's points go to
+    // Z[0..LZ) (plain order) with Z[LZ] = the k = N3/2 coefficient (inverse) or point 0 (forward,
+    // period LZ), so every lane reads its mirror at Z[LZ - t - P m] with no lane-0 special case.
     cplx<R> twk[E];   // W_N3^(t + P m)
 #pragma unroll
     for (int m = 0; m < E; m++) twk[m] = tws[LZ + t + P * m];
@@ -212,10 +208,14 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
             xa[m] = (ok && k < g.nkz) ? X[k] : czero<R>();
         }
         const cplx<R> xny = (ok && t == 0 && LZ < g.nkz) ? X[LZ] : czero<R>();   // kz = N3/2 (undealiased)
+        wave_sync();
+#pragma unroll
+        for (int m = 0; m < E; m++) Z[t + P * m] = xa[m];
+        if (t == 0) Z[LZ] = xny;
+        wave_sync();
 #pragma unroll
         for (int m = 0; m < E; m++) {
-            cplx<R> xr = shfl_c(xa[E - 1 - m], tp, P);
-            cplx<R> xb = (t == 0) ? (m == 0 ? xny : xa[(E - m) % E]) : xr;
+            cplx<R> xb = Z[LZ - t - P * m];
             cplx<R> xv = xa[m];
             if (t == 0 && m == 0) { xv.y = R(0); xb.y = R(0); }
             xb.y = -xb.y;
@@ -240,10 +240,14 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
         FF::template run<1, true, Lay>(w, Z, t, tw);
         const cplx<R>* twp = tw + LZ;   // W_N3^(t + P m)
         cplx<R>* X = Cout + p * g.cB + off;
+        wave_sync();
+#pragma unroll
+        for (int m = 0; m < E; m++) Z[t + P * m] = w[zreg<E, FF::RL>(m)];
+        if (t == 0) Z[LZ] = w[zreg<E, FF::RL>(0)];
+        wave_sync();
         auto unpack = [&](int m) {
-            cplx<R> yr = shfl_c(w[zreg<E, FF::RL>(E - 1 - m)], tp, P);
             cplx<R> zk = w[zreg<E, FF::RL>(m)];
-            cplx<R> zc = conjg((t == 0) ? w[zreg<E, FF::RL>((E - m) % E)] : yr);
+            cplx<R> zc = conjg(Z[LZ - t - P * m]);
             cplx<R> e = cscale(zk + zc, R(0.5));
             cplx<R> tt = cmul(twp[t + P * m], zk - zc);
             return mk<R>(e.x + R(0.5) * tt.y, e.y - R(0.5) * tt.x);
@@ -259,8 +263,7 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
                 if (ok && k < g.nkz) X[k] = o[m];
             }
         } else {
-            // FP64: holding every unpacked point would cost 4 E VGPRs.  The unpack (its lane
-            // shuffle needs every lane of the line active) stays outside the store condition.
+            // FP64: holding every unpacked point would cost 4 E VGPRs
 #pragma unroll
             for (int m = 0; m < E; m++) {
                 const int k = t + P * m;
