@@ -287,14 +287,14 @@ def bench_mhd(args):
     s.step(dt, nu=nu, eta=eta, steps=args.steps)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    out = s.download()
+    E1 = p32.total_energy(out[:3]) + p32.total_energy(out[3:])
     ms, nbytes = pass_times(s, dt, nu=nu, eta=eta)
     pts = float(n) ** 3
     value = pts * 4 * args.steps / el / 1e6
     roof, canon_stage = roofline(ms, nbytes, N, 8, "mhd")
     passes = {PASS_NAMES[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
                               "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
-    out = s.download()
-    E1 = p32.total_energy(out[:3]) + p32.total_energy(out[3:])
     print(json.dumps(_line(
         f"Mpts*RKstage/s at {n}^3 MHD (Elsasser), FP32-mixed fused step", value, args, 1e3 * el / args.steps,
         {"workload": f"mhd_{n}^3_rk4_elsasser", "N": n, "nu": nu, "eta": eta, "dt": dt, "cfl": 0.5,
@@ -337,8 +337,8 @@ def bench_rb(args):
         s.step(dt, Ra=Ra, Pr=Pr, steps=args.steps)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        ms, nbytes = pass_times(s, dt, Ra=Ra, Pr=Pr)
         out = s.download()
+        ms, nbytes = pass_times(s, dt, Ra=Ra, Pr=Pr)
         res[prec] = (el, p.nusselt(out[:3], out[3], Ra, Pr), ms, nbytes, var)
         del s, out, p
     p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision="fp32")
@@ -442,8 +442,8 @@ def main():
     t32 = time_steps(s32, args.steps, dt, nu)
     log(f"bench: fp32-mixed {args.steps} steps {t32:.4f} s")
     # per-pass timing with HIP events on the library stream (mean over RK stages 1-4)
-    ms, nbytes = pass_times(s32, dt, nu=nu)
     out32 = s32.download()
+    ms, nbytes = pass_times(s32, dt, nu=nu)
 
     pts = float(N[0]) * N[1] * N[2]
     value = pts * 4 * args.steps / t32 / 1e6

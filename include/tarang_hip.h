@@ -176,7 +176,8 @@ int th_flux_shell_tr(th_plan *plan, th_state *state, const double *radii, int ns
 /* Measurement only (no counterpart in the reference): runs RK stage `stage` (1..4) of the fused
  * pipeline nrep times pass by pass, timing each of the 5 passes with HIP events on the plan's
  * stream.  ms[k] = mean duration of pass k (x-inverse, y-inverse, fused z, y-forward,
- * x-forward + epilogue); bytes[k] = its algorithmic HBM bytes.  Overwrites the stage buffers only. */
+ * x-forward + epilogue); bytes[k] = its algorithmic HBM bytes.  Overwrites the stage field and the
+ * accumulator; the state is restored after a stage-4 run. */
 int th_profile_passes(th_plan *plan, th_state *state, double dt, const th_phys *p, int stage, int nrep,
                       double ms[5], double bytes[5]);
 

@@ -514,6 +514,13 @@ template <typename R> static int profile_impl(th_state* s, double dt, double nu,
     hipStream_t st = p->ctx->stream;
     const FG& g = s->fg;
     TRY(set_coefs(s, &dt, &nu, &eta, st));
+    // stage 4 writes the state itself: keep a copy and put it back afterwards
+    const size_t sbytes = (size_t)s->nbox * g.bS * s->esz;
+    void* keep = nullptr;
+    if (stage == 4) {
+        if (hipMalloc(&keep, sbytes) != hipSuccess) { set_error("th_profile_passes: out of device memory"); return E_NOMEM; }
+        TH_CHECK_HIP(hipMemcpyAsync(keep, s->S, sbytes, hipMemcpyDeviceToDevice, st));
+    }
     hipEvent_t ev[6];
     for (int i = 0; i < 6; i++) TH_CHECK_HIP(hipEventCreate(&ev[i]));
     double tot[5] = {0, 0, 0, 0, 0};
@@ -531,6 +538,11 @@ template <typename R> static int profile_impl(th_state* s, double dt, double nu,
         }
     }
     for (int i = 0; i < 6; i++) (void)hipEventDestroy(ev[i]);
+    if (keep) {
+        TH_CHECK_HIP(hipMemcpyAsync(s->S, keep, sbytes, hipMemcpyDeviceToDevice, st));
+        TH_CHECK_HIP(hipStreamSynchronize(st));
+        (void)hipFree(keep);
+    }
     const double e = (double)sizeof(cplx<R>) * s->nbox;
     const int nin = s->nin, nout = s->nout, nst = s->ncomp;
     const double cS = (double)g.mx.nk * g.my.nk * g.nkz, cA = (double)g.N1 * g.my.nk * g.nkz,

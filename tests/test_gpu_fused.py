@@ -221,3 +221,20 @@ def test_plan_destroyed_before_its_state(ctx):
     assert lib.th_state_destroy(s.h) == 0         # releases the plan too
     s.h = None
     p.h = None
+
+
+def test_profile_passes_leaves_the_state(ctx):
+    """th_profile_passes replays single RK stages for timing; the state itself (which stage 4 writes)
+    is restored, so what follows a measurement sees the state the timed steps left."""
+    from tarang_amd import Plan, State
+    N = (32, 32, 32)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    s = State(p)
+    s.upload([p.to_dev(a) for a in o.ic_random(9)])
+    before = [H(a) for a in s.download()]
+    for stage in (1, 2, 3, 4):
+        s.profile_passes(1e-3, nu=1e-2, stage=stage, nrep=2)
+    after = [H(a) for a in s.download()]
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
