@@ -29,6 +29,7 @@ extern "C" {
 #define TH_E_UNSUPPORTED (-3)
 #define TH_E_STATE (-4)
 #define TH_E_NOMEM (-5)
+#define TH_E_IO (-6)        /* a file could not be opened, written or parsed */
 
 #define TH_FOUR 0          /* basis_type "FOUR" */
 #define TH_SCFT 1          /* basis_type "SCFT" (sin/cos along x, Fourier along y, z) */
@@ -152,6 +153,9 @@ int th_nusselt(th_plan *plan, void *const V[3], const void *T, double Ra, double
 int th_ic_taylor_green(th_plan *plan, void *V[3], int k0, double amp);
 int th_ic_random(th_plan *plan, void *V[3], unsigned long long seed, double E0, double kp);
 int th_ic_random_scalar(th_plan *plan, void *T, unsigned long long seed, double amp);
+/* th_satisfy_reality == Satisfy_reality_array universal_misc.cc:51-109 on the kz = 0 and kz = N3/2
+ * planes of A (what Init_cond applies to a field read from file when apply_realitycond_IC_switch = 1) */
+int th_satisfy_reality(th_plan *plan, void *A);
 
 /* ---- fused, dealias-pruned step on a device-resident state ----
  * The state holds the kept (2/3-rule) modes of every component in a compact layout; upload and
@@ -212,6 +216,22 @@ int th_ensemble_download(th_ensemble *ens, int box, int comp, void *dst, int dst
 int th_ensemble_step(th_ensemble *ens, const double *dt, const double *nu, int nsteps);
 /* th_tune for every size class of the ensemble (per-box dt and nu as in th_ensemble_step) */
 int th_ensemble_tune(th_ensemble *ens, const double *dt, const double *nu, int nrep);
+
+/* ---- field files (host arrays only; no GPU is touched) ----
+ * th_field_write == IncFluid::Output_field output_field.cc:43-58 (CV_output Cvf.cc:715-720,
+ *   Write_data_MPI array_basic.cc:814-858): writes (append = 0) or appends one snapshot, a
+ *   "%% Time = t" line and ncomp components of N1 x N2 x (N3/2+1) values "(re,im) " at 12 significant
+ *   digits (MY_PRECISION, extern_def_vars.h:78), each component followed by a blank line.
+ * th_field_read == the reading half of IncFluid::Init_cond init_cond_field.cc:47-86 with
+ *   field_input_proc = 1 (CV_input Cvf.cc:739-744, Read_data_MPI array_basic.cc:294-335): snapshot
+ *   number `snapshot` (0 = first) of such a file into fields; *time = its "%% Time" value, NaN when the
+ *   file has no such line.  The caller applies Satisfy_reality / Dealias as Init_cond does.
+ * fields[c]: host arrays of complex double (TH_FP64) or complex float (TH_FP32_MIXED) in the
+ * reference layout; N = {N1, N2, N3}. */
+int th_field_write(const char *path, const int N[3], int precision, int ncomp, const void *const fields[],
+                   double time, int append);
+int th_field_read(const char *path, const int N[3], int precision, int ncomp, void *const fields[], int snapshot,
+                  double *time);
 
 #ifdef __cplusplus
 }

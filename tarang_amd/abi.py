@@ -61,6 +61,8 @@ def lib():
     sig = {
         "th_ctx_create": (i, [C.POINTER(vp), i, C.c_char_p]),
         "th_ctx_destroy": (i, [vp]),
+        "th_field_write": (i, [C.c_char_p, C.POINTER(C.c_int), i, i, pp, d, i]),
+        "th_field_read": (i, [C.c_char_p, C.POINTER(C.c_int), i, i, pp, i, dp]),
         "th_last_error": (C.c_char_p, []),
         "th_version": (C.c_char_p, []),
         "th_plan_create": (i, [vp, C.POINTER(vp), i, i, i, i, dp, i, i]),
@@ -149,6 +151,47 @@ def _dbl(a):
     import numpy as np
     a = np.ascontiguousarray(a, dtype=np.float64)
     return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _host_arrays(fields):
+    import numpy as np
+    out = []
+    for f in fields:
+        if hasattr(f, "detach"):
+            f = f.detach().cpu().numpy()
+        out.append(np.ascontiguousarray(f))
+    return out
+
+
+def field_write(path, N, fields, time=0.0, append=False):
+    """th_field_write: one snapshot of the reference's field file (field_out.d of Output_field).
+    fields: complex128 (FP64) or complex64 arrays of shape (N1, N2, N3/2+1), host or device."""
+    import numpy as np
+    arrs = _host_arrays(fields)
+    dt = arrs[0].dtype
+    if dt not in (np.complex128, np.complex64) or any(a.dtype != dt for a in arrs):
+        raise TypeError("field_write: all fields complex128 or all complex64")
+    shape = (N[0], N[1], N[2] // 2 + 1)
+    if any(a.shape != shape for a in arrs):
+        raise ValueError(f"field_write: fields must have shape {shape}")
+    prec = TH_FP64 if dt == np.complex128 else TH_FP32_MIXED
+    n = (C.c_int * 3)(*N)
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    _check(lib().th_field_write(str(path).encode(), n, prec, len(arrs), ptrs, float(time), 1 if append else 0),
+           "th_field_write")
+
+
+def field_read(path, N, ncomp, snapshot=0, precision="fp64"):
+    """th_field_read: snapshot `snapshot` of a field file; returns (list of ncomp host arrays, time)."""
+    import numpy as np
+    dt = np.complex128 if precision == "fp64" else np.complex64
+    arrs = [np.zeros((N[0], N[1], N[2] // 2 + 1), dt) for _ in range(ncomp)]
+    n = (C.c_int * 3)(*N)
+    ptrs = (C.c_void_p * ncomp)(*[a.ctypes.data for a in arrs])
+    t = C.c_double()
+    _check(lib().th_field_read(str(path).encode(), n, TH_FP64 if precision == "fp64" else TH_FP32_MIXED, ncomp, ptrs,
+                               int(snapshot), C.byref(t)), "th_field_read")
+    return arrs, t.value
 
 
 class Context:

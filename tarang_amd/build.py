@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["th_api.hip", "th_generic.hip", "th_reduce.hip", "th_ic.hip", "th_fused.hip", "th_fused_f32.hip",
            "th_fused_f64.hip"]
-HOST_SOURCES = ["th_host.cpp"]
+HOST_SOURCES = ["th_host.cpp", "th_io.cpp"]
 
 COMMON = ["-std=c++17", "-fPIC", "-O3", "--offload-arch=gfx950",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

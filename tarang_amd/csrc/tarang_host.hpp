@@ -67,6 +67,9 @@ public:
     void CV_Compute_totalenergy_diss();              // Cvf.cc:385-397
     void CV_Compute_shell_spectrum(std::vector<double>& result); // Cvf.cc:419-449 (n = 0, summed comps)
     void CV_Dealias();                               // Cvf.cc:699-704
+    // field files (th_field_write / th_field_read): Cvf.cc:715-720 and 739-744
+    void CV_output(const std::string& path, double time, bool append);
+    double CV_input(const std::string& path, int snapshot = 0);   // returns the snapshot's time
     int shell_ek_size() const;
     th_plan* plan() const { return plan_; }
     DField V1, V2, V3;
@@ -129,6 +132,10 @@ public:
     void Compute_rhs();                              // compute_rhs.cc:52-60
     void Time_advance();                             // time_advance.cc:60-163
     double Get_dt();                                 // compute_dt.cc:47-59
+    void Output_field(const std::string& path);      // output_field.cc:43-58 (appends a snapshot at Tnow)
+    // Init_cond() for field_input_proc = 1, init_cond_field.cc:47-86: read V from the file, then
+    // Satisfy_reality_condition_field when asked, and Dealias; returns the snapshot's time (NaN if none)
+    double Init_cond(const std::string& path, bool reality = false, int snapshot = 0);
     double Tnow, Tdt, Tdt_fixed;
     std::string integ_scheme;
 

@@ -2,8 +2,10 @@
 // reference's time loop (trunk/src/IFluid/Ifluid_main.cc:258-301): per step Get_dt, Time_advance,
 // energy + NaN check, then the shell spectrum and (at the end) the energy flux.
 //
-//   tarang_run N steps dt nu [ic=tg|random] [precision=FP64|FP32_MIXED] [path=fused|plain]
+//   tarang_run N steps dt nu [tg|random|file=<field_in.d>] [FP64|FP32_MIXED] [fused|plain] [save=<field_out.d>]
 // Output (stdout): "glob t E D" per step, then "spectrum k E(k)" lines and "flux R Pi(R)" lines.
+// file=: Init_cond from a field file (field_input_proc = 1; its "%% Time" sets Tinit); save=: the
+// final field is appended to a field file (Output_field), the restart point of a later run.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -15,7 +17,8 @@
 int main(int argc, char** argv)
 {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s N steps dt nu [tg|random] [FP64|FP32_MIXED] [fused|plain]\n", argv[0]);
+        fprintf(stderr, "usage: %s N steps dt nu [tg|random|file=PATH] [FP64|FP32_MIXED] [fused|plain] [save=PATH]\n",
+                argv[0]);
         return 2;
     }
     const int N = atoi(argv[1]), steps = atoi(argv[2]);
@@ -23,12 +26,16 @@ int main(int argc, char** argv)
     const std::string ic = argc > 5 ? argv[5] : "tg";
     const std::string prec = argc > 6 ? argv[6] : "FP64";
     const std::string path = argc > 7 ? argv[7] : "fused";
+    const std::string save = (argc > 8 && strncmp(argv[8], "save=", 5) == 0) ? argv[8] + 5 : "";
     try {
         tarang::Context ctx(0);
         tarang::Grid g{{0, N, N, N}, {0, 1.0, 1.0, 1.0}, "FOUR", "DEALIAS", prec};
         tarang::IncFluid U(ctx, g, nu, 0.0, dt, "RK4", path == "fused", N / 3 + 2);
         if (ic == "tg") {
             U.Setup_Taylor_Green_field(1, 1.0);
+        } else if (ic.compare(0, 5, "file=") == 0) {
+            const double t = U.Init_cond(ic.substr(5));   // the snapshot's time becomes Tinit
+            if (!isnan(t)) U.Tnow = t;
         } else {
             void* V[3] = {U.V1.data(), U.V2.data(), U.V3.data()};
             tarang::check(th_ic_random(U.plan(), V, 1, 0.5, 4.0), "th_ic_random");
@@ -49,6 +56,7 @@ int main(int argc, char** argv)
         for (size_t k = 0; k < Ek.size(); k++) printf("spectrum %zu %.15e\n", k, Ek[k]);
         U.Compute_flux();
         for (int s = 1; s <= U.no_spheres; s++) printf("flux %g %.15e\n", U.sphere_radius[s], U.flux_self[s]);
+        if (!save.empty()) U.Output_field(save);
     } catch (const std::exception& e) {
         fprintf(stderr, "tarang_run: %s\n", e.what());
         return 1;

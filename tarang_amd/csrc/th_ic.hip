@@ -285,3 +285,17 @@ extern "C" int th_ic_random_scalar(th_plan* p, void* T, unsigned long long seed,
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
     return p->prec ? ic_scalar<float>(p, T, seed, amp) : ic_scalar<double>(p, T, seed, amp);
 }
+
+template <typename R> static int reality(th_plan* p, void* A)
+{
+    hipLaunchKernelGGL((k_reality<R>), grid_of(2 * (size_t)p->g.N1 * p->g.N2), dim3(256), 0, p->ctx->stream, gi(p->g),
+                       (cplx<R>*)A);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+extern "C" int th_satisfy_reality(th_plan* p, void* A)
+{
+    if (!p || !A) { set_error("th_satisfy_reality: null"); return E_ARG; }
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    return p->prec ? reality<float>(p, A) : reality<double>(p, A);
+}

@@ -27,7 +27,7 @@ void tune_store(th_ctx* c, const std::string& key, const std::vector<int>& v);
 namespace th {
 
 // status codes (mirrored in include/tarang_hip.h)
-enum { OK = 0, E_ARG = -1, E_HIP = -2, E_UNSUPPORTED = -3, E_STATE = -4, E_NOMEM = -5 };
+enum { OK = 0, E_ARG = -1, E_HIP = -2, E_UNSUPPORTED = -3, E_STATE = -4, E_NOMEM = -5, E_IO = -6 };
 
 void set_error(const char* fmt, ...);
 

@@ -46,3 +46,21 @@ def test_driver_taylor_green_matches_oracle(path):
     fo = o.flux_ns(V, radii)
     for i, R in enumerate(radii[1:], start=1):
         assert abs(flux[R] - fo[i]) <= 1e-10 * np.abs(fo).max()
+
+
+def test_driver_restart_from_field_file(tmp_path):
+    """Output_field then Init_cond (field_input_proc = 1): 10 steps, save, 10 more from the file equal
+    20 steps in one run to the 12 printed digits of the restart file; the file's "%% Time" is the
+    restart time."""
+    N, dt, nu = 32, 1e-3, 0.01
+    f = tmp_path / "field_out.d"
+    spec20, flux20, glob20 = run_driver(N, 20, dt, nu, "tg", "FP64", "fused")
+    run_driver(N, 10, dt, nu, "tg", "FP64", "fused", f"save={f}")
+    assert f.read_text().startswith("%% Time = 0.01\n")
+    spec, flux, glob = run_driver(N, 10, dt, nu, f"file={f}", "FP64", "fused")
+    assert abs(glob[-1][0] - glob20[-1][0]) < 1e-12
+    ks = [k for k in spec20 if spec20[k] > 1e-10 * max(spec20.values())]
+    for k in ks:
+        assert abs(spec[k] - spec20[k]) <= 1e-10 * spec20[k], k
+    for r in flux20:
+        assert abs(flux[r] - flux20[r]) <= 1e-10 * max(abs(v) for v in flux20.values()), r
