@@ -117,10 +117,17 @@ int th_step_plain(th_plan *plan, int physics, int scheme, void *const fields[], 
  * th_shell_mult_all    == Shell_mult_all (dot product) universal_ET.cc:160-207
  * th_flux_ns  == IncVF::Compute_flux() compute_flux.cc:52-69 (sphere radii[1..nsph])
  * th_shell_tr_ns == IncVF::Compute_shell_tr() compute_shell_tr.cc:59-79, T is (nsh+1)^2 row-major
- * th_nusselt  == IncVF::Get_Nusselt_no nusselt.cc:53-89 (PRLARGE / ULARGE) */
+ * th_nusselt  == IncVF::Get_Nusselt_no nusselt.cc:53-89 (PRLARGE / ULARGE)
+ * th_helicity == Compute_total_helicity_FOUR four_energy.cc:453-548: out = {H1, H2, DH1, DH2} (FOUR)
+ * th_shell_spectrum_helicity == Compute_shell_spectrum_helicity_FOUR four_energy.cc:551-671: H1k1..3
+ *                of shell_ek_size each (K_c (Vr x Vi)_c parts, partial-shell rescale) (FOUR)
+ * th_entropy  == Get_entropy_FOUR four_energy.cc:687-750, spectral entropy in bits (FOUR) */
 int th_energy(th_plan *plan, const void *A, const void *B, double *out);
 int th_Sn(th_plan *plan, const void *A, double n, double *out);
 int th_shell_spectrum(th_plan *plan, const void *A, const void *B, int n, double *Sk, double *count);
+int th_helicity(th_plan *plan, void *const V[3], double *out);
+int th_shell_spectrum_helicity(th_plan *plan, void *const V[3], double *H1k1, double *H1k2, double *H1k3);
+int th_entropy(th_plan *plan, void *const V[3], double *out);
 int th_fill_array_shell(th_plan *plan, const void *A, void *B, double inner, double outer);
 int th_shell_mult_single(th_plan *plan, void *const A[3], void *const B[3], double inner, double outer, double *out);
 int th_shell_mult_all(th_plan *plan, void *const A[3], void *const B[3], const double *radii, int nr, double *out);

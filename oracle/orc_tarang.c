@@ -1344,6 +1344,88 @@ void orc_shell_tr_scalar(const orc_grid *g, double *const V[3], const double *T,
     free(G); free(N); free(row);
 }
 
+/* ---- helicity and spectral entropy (FOUR basis), the glob.d / spectrum.d columns ---- */
+static void vr_cross_vi(double *const V[3], size_t i, double c[3])
+{
+    double r0 = RE(V[0], i), r1 = RE(V[1], i), r2 = RE(V[2], i);
+    double m0 = IM(V[0], i), m1 = IM(V[1], i), m2 = IM(V[2], i);
+    c[0] = r1 * m2 - r2 * m1;
+    c[1] = r2 * m0 - r0 * m2;
+    c[2] = r0 * m1 - r1 * m0;
+}
+
+/* Compute_local_helicity_FOUR four_energy.cc:453-506: over |K| <= Min_radius_outside,
+ * modal = 2 mult K.(Vr x Vi); out = {H1 = sum modal, H2 = sum modal / K^2 (K^2 > MYEPS = 1e-15),
+ * DH1 = 2 sum K^2 modal, DH2 = 2 H1} */
+void orc_helicity(const orc_grid *g, double *const V[3], double *out)
+{
+    int N1 = g->N1, N2 = g->N2, H = H3(g);
+    int kkmax = orc_min_radius_outside(g);
+    double h1 = 0, h2 = 0, d1 = 0, c[3];
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 < H; l3++) {
+                double km = orc_kmag(g, l1, l2, l3);
+                if (km > kkmax) continue;
+                vr_cross_vi(V, IDX(g, l1, l2, l3), c);
+                double kx = orc_kx(g, l1) * g->f1, ky = orc_ky(g, l2) * g->f2, kz = l3 * g->f3;
+                double ksq = km * km;
+                double modal = 2 * orc_multiplicity(g, l1, l2, l3) * (kx * c[0] + ky * c[1] + kz * c[2]);
+                h1 += modal;
+                if (ksq > 1e-15) h2 += modal / ksq;
+                d1 += ksq * modal;
+            }
+    out[0] = h1; out[1] = h2; out[2] = 2.0 * d1; out[3] = 2.0 * h1;
+}
+
+/* Compute_shell_spectrum_helicity_FOUR four_energy.cc:551-671: bin ceil(|K|) <= Min_radius_outside,
+ * H1k_c += 2 mult K_c (Vr x Vi)_c, counted with 2 mult; shells kmax_inside < k <= kkmax with count >= 1
+ * rescaled by Approx_number_modes_in_shell / count.  Arrays of shell_ek_size. */
+void orc_shell_helicity(const orc_grid *g, double *const V[3], double *Hk1, double *Hk2, double *Hk3)
+{
+    int N1 = g->N1, N2 = g->N2, H = H3(g);
+    int n = orc_shell_ek_size(g), kkmax = orc_min_radius_outside(g), kin = orc_max_radius_inside(g);
+    double *cnt = (double *)calloc(n, sizeof(double)), c[3];
+    for (int i = 0; i < n; i++) Hk1[i] = Hk2[i] = Hk3[i] = 0;
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 < H; l3++) {
+                int index = (int)ceil(orc_kmag(g, l1, l2, l3));
+                if (index > kkmax) continue;
+                double f = 2 * orc_multiplicity(g, l1, l2, l3);
+                vr_cross_vi(V, IDX(g, l1, l2, l3), c);
+                Hk1[index] += f * (orc_kx(g, l1) * g->f1 * c[0]);
+                Hk2[index] += f * (orc_ky(g, l2) * g->f2 * c[1]);
+                Hk3[index] += f * (l3 * g->f3 * c[2]);
+                cnt[index] += f;
+            }
+    for (int index = kin + 1; index <= kkmax; index++)
+        if (cnt[index] >= 1) {
+            double s = orc_approx_modes_in_shell(g, index) / cnt[index];
+            Hk1[index] *= s; Hk2[index] *= s; Hk3[index] *= s;
+        }
+    free(cnt);
+}
+
+/* Get_local_entropy_FOUR four_energy.cc:687-732: p = (sum_c |A_c|^2 / 2) / E (E = Get_total_energy_FOUR of
+ * the three components), entropy = sum over kz <= N3/2 of 2 mult (-p log2 p) for p > MYEPS */
+double orc_entropy(const orc_grid *g, double *const V[3])
+{
+    int N1 = g->N1, N2 = g->N2, maxN3 = g->N3 > 2 ? g->N3 / 2 : 0;
+    double E = orc_energy(g, V[0], NULL) + orc_energy(g, V[1], NULL) + orc_energy(g, V[2], NULL);
+    double S = 0;
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 <= maxN3; l3++) {
+                size_t i = IDX(g, l1, l2, l3);
+                double me = 0;
+                for (int c = 0; c < 3; c++) me += (RE(V[c], i) * RE(V[c], i) + IM(V[c], i) * IM(V[c], i)) / 2;
+                double p = me / E;
+                if (p > 1e-15) S += 2 * orc_multiplicity(g, l1, l2, l3) * (-p * log(p) / log(2.0));
+            }
+    return S;
+}
+
 /* Compute_flux(IncVF& W) compute_flux.cc:148-278 (U = velocity, B = magnetic field) */
 void orc_flux_mhd(const orc_grid *g, double *U[3], double *B[3], const double *radii, int nsph,
                   double *u2u, double *b2b, double *u2b_io, double *u2b_ii,

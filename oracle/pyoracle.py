@@ -75,6 +75,9 @@ def lib():
             "orc_flux_mhd": (None, [p, pp, pp, vp, i] + [vp] * 9),
             "orc_et_radii": (None, [p, i, i, vp]),
             "orc_et_nlin_scalar": (None, [p, pp, vp, vp]),
+            "orc_helicity": (None, [p, pp, vp]),
+            "orc_shell_helicity": (None, [p, pp, vp, vp, vp]),
+            "orc_entropy": (d, [p, pp]),
             "orc_flux_scalar": (None, [p, pp, vp, vp, i, vp, vp]),
             "orc_shell_tr_scalar": (None, [p, pp, vp, vp, i, vp, vp]),
             "orc_nlin_rb": (None, [p, pp, vp, pp, vp]),
@@ -289,6 +292,21 @@ class Oracle:
         N = self.zeros(3)
         self.L.orc_et_nlin(self.gp, _pp(H), _pp(G), _pp(N))
         return N
+
+    def helicity(self, V):
+        """(H1, H2, DH1, DH2) of Compute_total_helicity_FOUR."""
+        out = np.zeros(4)
+        self.L.orc_helicity(self.gp, _pp(V), out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def shell_helicity(self, V):
+        n = self.shell_ek_size()
+        H = np.zeros((3, n))
+        self.L.orc_shell_helicity(self.gp, _pp(V), *[H[c].ctypes.data_as(C.c_void_p) for c in range(3)])
+        return H
+
+    def entropy(self, V):
+        return self.L.orc_entropy(self.gp, _pp(V))
 
     def et_nlin_scalar(self, H, G):
         N = self.zeros()

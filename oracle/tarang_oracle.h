@@ -87,6 +87,9 @@ void   orc_shell_mult_all(const orc_grid *g, double *const A[3], double *const B
 void   orc_et_nlin(const orc_grid *g, double *const H[3], double *const G[3], double *nlin[3]);
 void   orc_flux_ns(const orc_grid *g, double *const V[3], const double *radii, int nsph, double *flux);
 void   orc_shell_tr_ns(const orc_grid *g, double *const V[3], const double *radii, int nsh, double *T);
+void   orc_helicity(const orc_grid *g, double *const V[3], double *out);
+void   orc_shell_helicity(const orc_grid *g, double *const V[3], double *Hk1, double *Hk2, double *Hk3);
+double orc_entropy(const orc_grid *g, double *const V[3]);
 void   orc_et_nlin_scalar(const orc_grid *g, double *const H[3], const double *G, double *nlin);
 void   orc_flux_scalar(const orc_grid *g, double *const V[3], const double *T, const double *radii, int nsph,
                        double *flux_self, double *flux_sf);

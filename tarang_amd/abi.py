@@ -96,6 +96,9 @@ def lib():
         "th_shell_tr_ns": (i, [vp, pp, dp, i, dp]),
         "th_flux_mhd": (i, [vp, pp, pp, dp, i, dp]),
         "th_flux_scalar": (i, [vp, pp, vp, dp, i, dp]),
+        "th_helicity": (i, [vp, pp, dp]),
+        "th_shell_spectrum_helicity": (i, [vp, pp, dp, dp, dp]),
+        "th_entropy": (i, [vp, pp, dp]),
         "th_shell_tr_scalar": (i, [vp, pp, vp, dp, i, dp, dp]),
         "th_nusselt": (i, [vp, pp, vp, d, d, dp]),
         "th_ic_taylor_green": (i, [vp, pp, i, d]),
@@ -435,6 +438,29 @@ class Plan:
         _check(lib().th_flux_mhd(self.h, _pp(U), _pp(B), rp, len(r) - 1,
                                  out.ctypes.data_as(C.POINTER(C.c_double))), "th_flux_mhd")
         return out
+
+    def helicity(self, V):
+        """(H1, H2, DH1, DH2): Compute_total_helicity_FOUR."""
+        import numpy as np
+        out = np.zeros(4)
+        self._pre()
+        _check(lib().th_helicity(self.h, _pp(V), out.ctypes.data_as(C.POINTER(C.c_double))), "th_helicity")
+        return out
+
+    def shell_helicity(self, V):
+        """(3, shell_ek_size) helicity spectra H1k1..3: Compute_shell_spectrum_helicity_FOUR."""
+        import numpy as np
+        H = np.zeros((3, lib().th_shell_ek_size(self.h)))
+        self._pre()
+        _check(lib().th_shell_spectrum_helicity(self.h, _pp(V), *[H[c].ctypes.data_as(C.POINTER(C.c_double))
+                                                                 for c in range(3)]), "th_shell_spectrum_helicity")
+        return H
+
+    def entropy(self, V):
+        out = C.c_double()
+        self._pre()
+        _check(lib().th_entropy(self.h, _pp(V), C.byref(out)), "th_entropy")
+        return out.value
 
     def flux_scalar(self, V, T, radii):
         """(flux_self, flux_SF) over spheres 0..nsph (Compute_flux_scalar / Compute_flux_RB)."""

@@ -12,6 +12,7 @@
 // std::runtime_error carrying th_last_error().  Time_advance uses the fused dealias-pruned step;
 // Compute_nlin & co. use the plain path that keeps the reference's operation sequence.
 #pragma once
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -136,6 +137,15 @@ public:
     // Init_cond() for field_input_proc = 1, init_cond_field.cc:47-86: read V from the file, then
     // Satisfy_reality_condition_field when asked, and Dealias; returns the snapshot's time (NaN if none)
     double Init_cond(const std::string& path, bool reality = false, int snapshot = 0);
+    // Output files, the reference's names and formats (stream precision 12):
+    //   Open_output_files file_operations.cc:163-240 (<data_dir>/out/glob.d, spectrum.d, flux.d)
+    //   Output_global output_energy.cc:43-81, Output_shell_spectrum :368-394, Output_flux output_ET.cc:45-74
+    // Unforced runs: the force-spectrum and force-feed columns are 0, helicity_flux_switch = 0.
+    void Open_output_files(const std::string& data_dir);
+    void Output_global();
+    void Output_shell_spectrum();
+    void Output_flux();
+    std::ofstream global_file, spectrum_file, flux_file;
     double Tnow, Tdt, Tdt_fixed;
     std::string integ_scheme;
 

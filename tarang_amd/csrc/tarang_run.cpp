@@ -6,6 +6,8 @@
 // Output (stdout): "glob t E D" per step, then "spectrum k E(k)" lines and "flux R Pi(R)" lines.
 // file=: Init_cond from a field file (field_input_proc = 1; its "%% Time" sets Tinit); save=: the
 // final field is appended to a field file (Output_field), the restart point of a later run.
+// out=DIR (any position after nu): DIR/out/glob.d every step, DIR/out/spectrum.d and flux.d at the end,
+// in the reference's formats (Output_global, Output_shell_spectrum, Output_flux).
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -26,7 +28,11 @@ int main(int argc, char** argv)
     const std::string ic = argc > 5 ? argv[5] : "tg";
     const std::string prec = argc > 6 ? argv[6] : "FP64";
     const std::string path = argc > 7 ? argv[7] : "fused";
-    const std::string save = (argc > 8 && strncmp(argv[8], "save=", 5) == 0) ? argv[8] + 5 : "";
+    std::string save, outdir;
+    for (int a = 5; a < argc; a++) {
+        if (strncmp(argv[a], "save=", 5) == 0) save = argv[a] + 5;
+        if (strncmp(argv[a], "out=", 4) == 0) outdir = argv[a] + 4;
+    }
     try {
         tarang::Context ctx(0);
         tarang::Grid g{{0, N, N, N}, {0, 1.0, 1.0, 1.0}, "FOUR", "DEALIAS", prec};
@@ -40,6 +46,7 @@ int main(int argc, char** argv)
             void* V[3] = {U.V1.data(), U.V2.data(), U.V3.data()};
             tarang::check(th_ic_random(U.plan(), V, 1, 0.5, 4.0), "th_ic_random");
         }
+        if (!outdir.empty()) U.Open_output_files(outdir);
         for (int it = 0; it < steps; it++) {
             U.Tdt = U.Get_dt();
             U.Tnow += U.Tdt;
@@ -50,6 +57,7 @@ int main(int argc, char** argv)
                 return 1;
             }
             printf("glob %.6f %.15e %.15e\n", U.Tnow, U.CV_total_energy, U.CV_total_dissipation);
+            if (!outdir.empty()) U.Output_global();
         }
         std::vector<double> Ek;
         U.CV_Compute_shell_spectrum(Ek);
@@ -57,6 +65,10 @@ int main(int argc, char** argv)
         U.Compute_flux();
         for (int s = 1; s <= U.no_spheres; s++) printf("flux %g %.15e\n", U.sphere_radius[s], U.flux_self[s]);
         if (!save.empty()) U.Output_field(save);
+        if (!outdir.empty()) {
+            U.Output_shell_spectrum();
+            U.Output_flux();
+        }
     } catch (const std::exception& e) {
         fprintf(stderr, "tarang_run: %s\n", e.what());
         return 1;

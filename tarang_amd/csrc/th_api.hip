@@ -870,6 +870,42 @@ extern "C" int th_shell_spectrum(th_plan* p, const void* A, const void* B, int n
     if (!A || !Sk || !(n == 0 || n == 2 || n == 4)) { set_error("th_shell_spectrum: bad args"); return E_ARG; }
     return DISPATCH(p, do_spec, p, A, B, n, Sk, count);
 }
+template <typename R> static int do_helicity(th_plan* p, void* const V[3], double* out)
+{
+    const cplx<R>* v[3] = {CCP(R, V[0]), CCP(R, V[1]), CCP(R, V[2])};
+    return reduce_helicity<R>(p->g, v, out, p->rs, p->ctx->stream);
+}
+extern "C" int th_helicity(th_plan* p, void* const V[3], double* out)
+{
+    TRY(check_plan(p));
+    if (!V || !out) { set_error("th_helicity: null"); return E_ARG; }
+    if (p->g.basis != TH_FOUR) { set_error("th_helicity: FOUR basis only"); return E_UNSUPPORTED; }
+    return DISPATCH(p, do_helicity, p, V, out);
+}
+template <typename R> static int do_entropy(th_plan* p, void* const V[3], double* out)
+{
+    const cplx<R>* v[3] = {CCP(R, V[0]), CCP(R, V[1]), CCP(R, V[2])};
+    return reduce_entropy<R>(p->g, v, out, p->rs, p->ctx->stream);
+}
+extern "C" int th_entropy(th_plan* p, void* const V[3], double* out)
+{
+    TRY(check_plan(p));
+    if (!V || !out) { set_error("th_entropy: null"); return E_ARG; }
+    if (p->g.basis != TH_FOUR) { set_error("th_entropy: FOUR basis only"); return E_UNSUPPORTED; }
+    return DISPATCH(p, do_entropy, p, V, out);
+}
+template <typename R> static int do_shell_hel(th_plan* p, void* const V[3], double* H1, double* H2, double* H3)
+{
+    const cplx<R>* v[3] = {CCP(R, V[0]), CCP(R, V[1]), CCP(R, V[2])};
+    return reduce_shell_helicity<R>(p->g, v, min_radius_outside(p->g) + 1, H1, H2, H3, p->rs, p->ctx->stream);
+}
+extern "C" int th_shell_spectrum_helicity(th_plan* p, void* const V[3], double* H1k1, double* H1k2, double* H1k3)
+{
+    TRY(check_plan(p));
+    if (!V || !H1k1 || !H1k2 || !H1k3) { set_error("th_shell_spectrum_helicity: null"); return E_ARG; }
+    if (p->g.basis != TH_FOUR) { set_error("th_shell_spectrum_helicity: FOUR basis only"); return E_UNSUPPORTED; }
+    return DISPATCH(p, do_shell_hel, p, V, H1k1, H1k2, H1k3);
+}
 template <typename R> static int do_fill(th_plan* p, const void* A, void* B, double in, double out)
 {
     return k_fill_shell<R>(p->g, CCP(R, A), CP(R, B), in, out, p->ctx->stream);

@@ -264,6 +264,76 @@ double IncFluid::Init_cond(const std::string& path, bool reality, int snapshot)
     return t;
 }
 
+void IncFluid::Open_output_files(const std::string& data_dir)
+{
+    const std::string out = data_dir + "/out/";
+    global_file.open(out + "glob.d");
+    spectrum_file.open(out + "spectrum.d");
+    flux_file.open(out + "flux.d");
+    if (!global_file || !spectrum_file || !flux_file) throw std::runtime_error("Open_output_files: cannot open files in " + out);
+    for (std::ofstream* f : {&global_file, &spectrum_file, &flux_file}) f->precision(12);
+}
+
+void IncFluid::Output_global()
+{
+    CV_Compute_totalenergy_diss();
+    double h[4], entropy;
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    check(th_helicity(plan_, V, h), "CV_Compute_total_helicity");
+    check(th_entropy(plan_, V, &entropy), "CV_Compute_entropy");
+    double mean[3];
+    for (int i = 0; i < 3; i++) {
+        double z[2] = {0, 0};
+        float zf[2] = {0, 0};
+        if (esize_ == 16) hipc(hipMemcpy(z, V[i], 16, hipMemcpyDeviceToHost), "mean mode");
+        else {
+            hipc(hipMemcpy(zf, V[i], 8, hipMemcpyDeviceToHost), "mean mode");
+            z[0] = zf[0];
+        }
+        mean[i] = z[0];
+    }
+    const double kmax = th_max_radius_inside(plan_);
+    const double total_diss = dissipation_coefficient * CV_total_dissipation;
+    const double kolm_scale_u = sqrt(sqrt((pow(dissipation_coefficient, 3) / total_diss)));
+    const double kmax_eta = kmax * kolm_scale_u;
+    const double Rlambda = 2 * CV_total_energy * sqrt(15 / (dissipation_coefficient * total_diss));
+    global_file << Tnow << "\t" << CV_total_energy << " " << CV_total_dissipation << " " << total_diss << "    " << h[0]
+                << " " << h[1] << "    " << h[2] << " " << h[3] << "    " << entropy << "    " << kmax_eta << "    "
+                << dissipation_coefficient << "    " << Rlambda << "    " << Tdt << "    " << mean[0] << " " << mean[1]
+                << " " << mean[2] << std::endl;
+}
+
+void IncFluid::Output_shell_spectrum()
+{
+    spectrum_file << "%% Time = " << Tnow << std::endl;
+    const int n = shell_ek_size();
+    std::vector<double> ek[3], dk(n, 0.0), t(n), hk[3];
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    for (int c = 0; c < 3; c++) {
+        ek[c].resize(n);
+        hk[c].resize(n);
+        check(th_shell_spectrum(plan_, V[c], nullptr, 0, ek[c].data(), nullptr), "CV_Compute_shell_spectrum");
+        check(th_shell_spectrum(plan_, V[c], nullptr, 2, t.data(), nullptr), "CV_Compute_shell_spectrum");
+        for (int i = 0; i < n; i++) dk[i] += t[i];
+    }
+    for (int i = 0; i < n; i++) dk[i] *= 2.0;
+    check(th_shell_spectrum_helicity(plan_, V, hk[0].data(), hk[1].data(), hk[2].data()), "CV_Compute_shell_spectrum_helicity");
+    for (int i = 0; i < n; i++)
+        spectrum_file << i << "\t" << ek[0][i] << " " << ek[1][i] << " " << ek[2][i] << "\t"
+                      << dissipation_coefficient * dk[i] << "\t" << hk[0][i] << "  " << hk[1][i] << "  " << hk[2][i] << "\t"
+                      << 0.0 << "\t" << std::endl;
+    spectrum_file << std::endl;
+}
+
+void IncFluid::Output_flux()
+{
+    flux_file << "%% Time = " << Tnow << std::endl;
+    Compute_flux();
+    for (int i = 1; i <= no_spheres; i++)
+        flux_file << i << " " << sphere_radius[i] << " " << flux_self[i] << "  " << 0.0 << "     " << std::endl;
+    flux_file << std::endl;
+}
+
 void IncFluid::Time_advance()
 {
     th_phys ph{dissipation_coefficient, 0, 0, 0, 0};

@@ -106,6 +106,10 @@ template <typename R> int reduce_shell_mult_single(const Geo& g, const cplx<R>* 
 template <typename R> int reduce_shell_mult_all(const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3],
                                                 const double* radii_dev, int nr, double* out, RedScratch& rs,
                                                 hipStream_t st);
+template <typename R> int reduce_helicity(const Geo& g, const cplx<R>* const V[3], double* out, RedScratch& rs, hipStream_t st);
+template <typename R> int reduce_entropy(const Geo& g, const cplx<R>* const V[3], double* out, RedScratch& rs, hipStream_t st);
+template <typename R> int reduce_shell_helicity(const Geo& g, const cplx<R>* const V[3], int nbins, double* H1, double* H2,
+                                                double* H3, RedScratch& rs, hipStream_t st);
 
 // host-side binning helpers (four_inline.h, scft_inline.h)
 int min_radius_outside(const Geo& g);

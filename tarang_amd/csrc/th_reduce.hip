@@ -10,6 +10,14 @@
 #include <math.h>
 #include "th_internal.hpp"
 
+#ifndef TRY
+#define TRY(x)             \
+    do {                   \
+        int r_ = (x);      \
+        if (r_) return r_; \
+    } while (0)
+#endif
+
 namespace th {
 
 struct GeoR {
@@ -186,6 +194,86 @@ __global__ __launch_bounds__(RBLK) void k_scalar(GeoR g, const cplx<R>* __restri
 }
 
 // out[b] = compensated sum over blocks of partial[blk * stride + b]
+// Helicity and spectral entropy (FOUR basis; the glob.d / spectrum.d columns).
+//   MODE 0: Compute_local_helicity_FOUR four_energy.cc:453-506 over |K| <= kkmax:
+//           modal = 2 mult K.(Vr x Vi); values {sum modal, sum modal/K^2 (K^2 > 1e-15), sum K^2 modal}
+//   MODE 1: Get_local_entropy_FOUR four_energy.cc:687-732: p = (sum_c |V_c|^2 / 2) / E,
+//           value sum 2 mult (-p log2 p) over p > 1e-15
+template <typename R> __device__ __forceinline__ void vr_x_vi(const cplx<R>& a, const cplx<R>& b, const cplx<R>& c,
+                                                             double* x)
+{
+    x[0] = (double)b.x * c.y - (double)c.x * b.y;
+    x[1] = (double)c.x * a.y - (double)a.x * c.y;
+    x[2] = (double)a.x * b.y - (double)b.x * a.y;
+}
+template <typename R, int MODE>
+__global__ __launch_bounds__(RBLK) void k_modal(GeoR g, const cplx<R>* __restrict__ A0, const cplx<R>* __restrict__ A1,
+                                                const cplx<R>* __restrict__ A2, int kkmax, double E,
+                                                double* __restrict__ partial)
+{
+    constexpr int NV = MODE == 0 ? 3 : 1;
+    __shared__ double red[NV][RBLK];
+    double acc[NV] = {};
+    const long n = (long)g.N1 * g.N2 * g.H;
+    for (long i = (long)blockIdx.x * RBLK + threadIdx.x; i < n; i += (long)gridDim.x * RBLK) {
+        int l1, l2, l3;
+        decode_r(g, i, l1, l2, l3);
+        if (MODE == 0) {
+            double km = kmag_r(g, l1, l2, l3);
+            if (km > (double)kkmax) continue;
+            double x[3];
+            vr_x_vi(A0[i], A1[i], A2[i], x);
+            const double kx = kx_four(l1, g.N1) * g.f1, ky = ky_of(l2, g.N2) * g.f2, kz = l3 * g.f3;
+            const double ksq = km * km;
+            const double modal = 2 * mult_r(g, l1, l2, l3) * (kx * x[0] + ky * x[1] + kz * x[2]);
+            acc[0] += modal;
+            if (ksq > 1e-15) acc[NV > 1 ? 1 : 0] += modal / ksq;
+            acc[NV > 2 ? 2 : 0] += ksq * modal;
+        } else {
+            const double p = 0.5 * (sq(A0[i]) + sq(A1[i]) + sq(A2[i])) / E;
+            if (p > 1e-15) acc[0] += 2 * mult_r(g, l1, l2, l3) * (-p * log(p) / log(2.0));
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) red[v][threadIdx.x] = acc[v];
+    __syncthreads();
+    for (int w = RBLK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+#pragma unroll
+            for (int v = 0; v < NV; v++) red[v][threadIdx.x] += red[v][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < NV) partial[(long)blockIdx.x * NV + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Compute_local_shell_spectrum_helicity_FOUR four_energy.cc:551-599: bin ceil(K) <= kkmax,
+// H1k_c += 2 mult K_c (Vr x Vi)_c and count += 2 mult; partial[block][4 nbins]
+template <typename R>
+__global__ __launch_bounds__(RBLK) void k_binned_hel(GeoR g, const cplx<R>* __restrict__ A0, const cplx<R>* __restrict__ A1,
+                                                     const cplx<R>* __restrict__ A2, int kkmax, int nbins,
+                                                     double* __restrict__ partial)
+{
+    extern __shared__ double sh[];
+    for (int b = threadIdx.x; b < 4 * nbins; b += RBLK) sh[b] = 0.0;
+    __syncthreads();
+    const long n = (long)g.N1 * g.N2 * g.H;
+    for (long i = (long)blockIdx.x * RBLK + threadIdx.x; i < n; i += (long)gridDim.x * RBLK) {
+        int l1, l2, l3;
+        decode_r(g, i, l1, l2, l3);
+        const int idx = (int)ceil(kmag_r(g, l1, l2, l3));
+        if (idx > kkmax || idx >= nbins) continue;
+        double x[3];
+        vr_x_vi(A0[i], A1[i], A2[i], x);
+        const double f = 2 * mult_r(g, l1, l2, l3);
+        atomicAdd(&sh[idx], f * (kx_four(l1, g.N1) * g.f1 * x[0]));
+        atomicAdd(&sh[nbins + idx], f * (ky_of(l2, g.N2) * g.f2 * x[1]));
+        atomicAdd(&sh[2 * nbins + idx], f * (l3 * g.f3 * x[2]));
+        atomicAdd(&sh[3 * nbins + idx], f);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 4 * nbins; b += RBLK) partial[(long)blockIdx.x * 4 * nbins + b] = sh[b];
+}
+
 __global__ void k_finalize(const double* __restrict__ partial, int nblk, int stride, int nval, double* __restrict__ out)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -359,6 +447,79 @@ int reduce_shell_mult_all(const Geo& g, const cplx<R>* const A[3], const cplx<R>
     return OK;
 }
 
+template <typename R> static int run_modal(int mode, const Geo& g, const cplx<R>* const V[3], double E, double* out,
+                                           RedScratch& rs, hipStream_t st)
+{
+    const int nv = mode == 0 ? 3 : 1;
+    int nb = nblocks_for(g.ncplx() / 4 + 1);
+    int r = rs.ensure((size_t)nb * nv + nv, nv);
+    if (r) return r;
+    int kkmax = min_radius_outside(g);
+    if (mode == 0)
+        hipLaunchKernelGGL((k_modal<R, 0>), dim3(nb), dim3(RBLK), 0, st, gr(g), V[0], V[1], V[2], kkmax, E, rs.d);
+    else
+        hipLaunchKernelGGL((k_modal<R, 1>), dim3(nb), dim3(RBLK), 0, st, gr(g), V[0], V[1], V[2], kkmax, E, rs.d);
+    TH_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, rs.d, nb, nv, nv, rs.d + (size_t)nb * nv);
+    TH_CHECK_HIP(hipGetLastError());
+    TH_CHECK_HIP(hipMemcpyAsync(rs.h, rs.d + (size_t)nb * nv, nv * sizeof(double), hipMemcpyDeviceToHost, st));
+    TH_CHECK_HIP(hipStreamSynchronize(st));
+    for (int v = 0; v < nv; v++) out[v] = rs.h[v];
+    return OK;
+}
+
+// Compute_total_helicity_FOUR: out = {H1, H2, DH1 = 2 sum K^2 modal, DH2 = 2 H1}
+template <typename R> int reduce_helicity(const Geo& g, const cplx<R>* const V[3], double* out, RedScratch& rs, hipStream_t st)
+{
+    double v[3];
+    TRY(run_modal<R>(0, g, V, 0.0, v, rs, st));
+    out[0] = v[0];
+    out[1] = v[1];
+    out[2] = 2.0 * v[2];
+    out[3] = 2.0 * v[0];
+    return OK;
+}
+
+// Get_entropy_FOUR with E = the three components' Get_total_energy_FOUR
+template <typename R> int reduce_entropy(const Geo& g, const cplx<R>* const V[3], double* out, RedScratch& rs, hipStream_t st)
+{
+    double E = 0, e;
+    for (int c = 0; c < 3; c++) {
+        TRY(reduce_energy<R>(g, V[c], nullptr, &e, rs, st));
+        E += e;
+    }
+    return run_modal<R>(1, g, V, E, out, rs, st);
+}
+
+// Compute_shell_spectrum_helicity_FOUR four_energy.cc:602-671 (with the partial-shell rescale)
+template <typename R>
+int reduce_shell_helicity(const Geo& g, const cplx<R>* const V[3], int nbins, double* H1, double* H2, double* H3,
+                          RedScratch& rs, hipStream_t st)
+{
+    int nb = nblocks_for(g.ncplx() / 4 + 1);
+    int kkmax = min_radius_outside(g);
+    if (nbins != kkmax + 1) { set_error("reduce_shell_helicity: nbins %d != shell_ek_size %d", nbins, kkmax + 1); return E_ARG; }
+    int r = rs.ensure((size_t)nb * 4 * nbins + 4 * nbins, 4 * nbins);
+    if (r) return r;
+    double* out = rs.d + (size_t)nb * 4 * nbins;
+    hipLaunchKernelGGL((k_binned_hel<R>), dim3(nb), dim3(RBLK), 4 * nbins * sizeof(double), st, gr(g), V[0], V[1], V[2],
+                       kkmax, nbins, rs.d);
+    TH_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3((4 * nbins + 255) / 256), dim3(256), 0, st, rs.d, nb, 4 * nbins, 4 * nbins, out);
+    TH_CHECK_HIP(hipGetLastError());
+    TH_CHECK_HIP(hipMemcpyAsync(rs.h, out, 4 * nbins * sizeof(double), hipMemcpyDeviceToHost, st));
+    TH_CHECK_HIP(hipStreamSynchronize(st));
+    int kin = max_radius_inside(g);
+    for (int k = 0; k < nbins; k++) {
+        double c = rs.h[3 * nbins + k], s = 1.0;
+        if (k > kin && k <= kkmax && c >= 1) s = approx_modes_in_shell(g, k) / c;
+        H1[k] = rs.h[k] * s;
+        H2[k] = rs.h[nbins + k] * s;
+        H3[k] = rs.h[2 * nbins + k] * s;
+    }
+    return OK;
+}
+
 #define TH_RINST(R)                                                                                                  \
     template int reduce_shell<R>(const Geo&, const cplx<R>*, const cplx<R>*, double, int, double*, double*,         \
                                  RedScratch&, hipStream_t, bool);                                                    \
@@ -367,7 +528,11 @@ int reduce_shell_mult_all(const Geo& g, const cplx<R>* const A[3], const cplx<R>
     template int reduce_shell_mult_single<R>(const Geo&, const cplx<R>* const*, const cplx<R>* const*, double,     \
                                              double, double*, RedScratch&, hipStream_t);                             \
     template int reduce_shell_mult_all<R>(const Geo&, const cplx<R>* const*, const cplx<R>* const*, const double*, \
-                                          int, double*, RedScratch&, hipStream_t);
+                                          int, double*, RedScratch&, hipStream_t);                                  \
+    template int reduce_helicity<R>(const Geo&, const cplx<R>* const*, double*, RedScratch&, hipStream_t);          \
+    template int reduce_entropy<R>(const Geo&, const cplx<R>* const*, double*, RedScratch&, hipStream_t);           \
+    template int reduce_shell_helicity<R>(const Geo&, const cplx<R>* const*, int, double*, double*, double*,         \
+                                          RedScratch&, hipStream_t);
 TH_RINST(float)
 TH_RINST(double)
 

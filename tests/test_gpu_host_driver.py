@@ -64,3 +64,61 @@ def test_driver_restart_from_field_file(tmp_path):
         assert abs(spec[k] - spec20[k]) <= 1e-10 * spec20[k], k
     for r in flux20:
         assert abs(flux[r] - flux20[r]) <= 1e-10 * max(abs(v) for v in flux20.values()), r
+
+
+def test_driver_output_files_match_oracle(tmp_path):
+    """Output_global / Output_shell_spectrum / Output_flux (output_energy.cc:43-81, 368-394,
+    output_ET.cc:45-74): glob.d, spectrum.d and flux.d in the reference's column layout, every column
+    equal to the oracle's value of the same quantity to the 12 printed digits."""
+    N, steps, dt, nu = 32, 3, 1e-3, 0.01
+    (tmp_path / "out").mkdir()
+    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"out={tmp_path}")
+    o = Oracle((N, N, N))
+    V = o.ic_random(1)
+    o.rk4_ns(V, nu, dt, steps=steps)
+
+    def close(a, b, tol=1e-10):
+        return abs(a - b) <= tol * max(abs(b), 1e-300)
+
+    lines = (tmp_path / "out" / "glob.d").read_text().splitlines()
+    assert len(lines) == steps
+    t, rest = lines[-1].split("\t")
+    g = rest.split("    ")
+    assert len(g) == 9, lines[-1]
+    E, D, Dn = map(float, g[0].split(" "))
+    H1, H2 = map(float, g[1].split(" "))
+    DH1, DH2 = map(float, g[2].split(" "))
+    S, kmax_eta, nu_out, Rl, tdt = float(g[3]), float(g[4]), float(g[5]), float(g[6]), float(g[7])
+    m1, m2, m3 = map(float, g[8].split(" "))
+    assert close(float(t), steps * dt) and nu_out == nu and tdt == dt and (m1, m2, m3) == (0.0, 0.0, 0.0)
+    assert close(E, o.total_energy(V)) and close(D, o.total_dissipation(V)) and close(Dn, nu * D)
+    h = o.helicity(V)
+    for a, b in zip((H1, H2, DH1, DH2), h):
+        assert abs(a - b) <= 1e-10 * np.abs(h).max()
+    assert close(S, o.entropy(V))
+    kmax = o.max_radius_inside()
+    assert close(kmax_eta, kmax * (nu ** 3 / Dn) ** 0.25) and close(Rl, 2 * E * (15 / (nu * Dn)) ** 0.5)
+
+    blocks = (tmp_path / "out" / "spectrum.d").read_text().split("\n")
+    assert blocks[0] == f"%% Time = {steps * dt:.12g}"
+    rows = [r for r in blocks[1:] if r.strip()]
+    ek = [o.shell_spectrum(V[c])[0] for c in range(3)]
+    dk = 2 * sum(o.shell_spectrum(V[c], n=2)[0] for c in range(3))
+    hk = o.shell_helicity(V)
+    assert len(rows) == len(ek[0])
+    for i, r in enumerate(rows):
+        f = r.split("\t")
+        assert int(f[0]) == i and f[-1] == ""
+        e1, e2, e3 = map(float, f[1].split(" "))
+        h1, h2, h3 = map(float, f[3].split("  "))
+        for a, b, sc in ((e1, ek[0][i], ek[0].max()), (e2, ek[1][i], ek[1].max()), (e3, ek[2][i], ek[2].max()),
+                         (float(f[2]), nu * dk[i], nu * dk.max()), (h1, hk[0][i], np.abs(hk).max()),
+                         (h2, hk[1][i], np.abs(hk).max()), (h3, hk[2][i], np.abs(hk).max())):
+            assert abs(a - b) <= 1e-10 * sc, (i, a, b)
+        assert float(f[4]) == 0.0
+
+    frows = [r for r in (tmp_path / "out" / "flux.d").read_text().split("\n")[1:] if r.strip()]
+    assert len(frows) == len(flux)
+    for r in frows:
+        i, R, pi, force = r.split()
+        assert close(float(pi), flux[float(R)], 1e-11) and float(force) == 0.0

@@ -321,6 +321,22 @@ def test_flux_and_shell_tr_scalar_plain(ctx, basis, prec, tol):
         assert np.abs(got - ref).max() < tol * sc, name
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_helicity_and_entropy(ctx, prec, tol):
+    """th_helicity / th_shell_spectrum_helicity / th_entropy (four_energy.cc:453-750) match the oracle."""
+    from tarang_amd import Plan
+    N = (16, 24, 16)
+    o = Oracle(N)
+    V = o.ic_random(12, E0=0.5, kp=3.0)
+    p = Plan(ctx, N, precision=prec)
+    dV = [p.to_dev(a) for a in V]
+    h, ref = p.helicity(dV), o.helicity(V)
+    assert np.abs(h - ref).max() <= tol * np.abs(ref).max()
+    Hk, rHk = p.shell_helicity(dV), o.shell_helicity(V)
+    assert np.abs(Hk - rHk).max() <= tol * np.abs(rHk).max()
+    assert abs(p.entropy(dV) - o.entropy(V)) <= tol * o.entropy(V)
+
+
 def test_config1_taylor_green_64_100_steps(ctx):
     """Config 1: TG 64^3 FP64, RK4, dt = 1e-3, 100 steps, nu = 0.01, 2/3 dealias.  The plain GPU path
     matches the oracle to <= 1e-10 relative in E(k) (shells with E >= 1e-10 max E) and in Pi(R)."""

@@ -354,3 +354,55 @@ def test_scalar_transfer_antisymmetry_conservation_and_flux_identity(basis):
         ou = Oracle(N, kfactor=f, basis="SCFT", dealias=False)
         _, nT = ou.nlin_rb(V, T)
         assert np.abs(ou.et_nlin_scalar(V, T) - nT).max() <= 1e-14 * np.abs(nT).max()
+
+
+@pytest.mark.parametrize("axis,s", [(0, 1), (1, -1), (2, 1), (2, -1)])
+def test_helicity_of_a_helical_mode(axis, s):
+    """Compute_total_helicity_FOUR (four_energy.cc:453-506): a single helical mode V(k) = A (e1 + i s e2),
+    e1 x e2 = k/|k|, has H1 = s |k| E, H2 = H1 / k^2, DH1 = 2 k^2 H1, DH2 = 2 H1 (E in the reference's
+    convention), and its shell helicity sits in shell |k| on the k component only."""
+    N, m, A = (16, 16, 16), 3, 0.7
+    o = Oracle(N)
+    V = o.zeros(3)
+    e = np.eye(3)
+    e1, e2 = e[(axis + 1) % 3], e[(axis + 2) % 3]
+    vec = A * (e1 + 1j * s * e2)
+    idx = [0, 0, 0]
+    idx[axis] = m
+    for c in range(3):
+        V[c][tuple(idx)] = vec[c]
+    if axis != 2:                       # kz = 0: the conjugate partner at -k is stored too
+        jdx = [0, 0, 0]
+        jdx[axis] = N[axis] - m
+        for c in range(3):
+            V[c][tuple(jdx)] = np.conj(vec[c])
+    E = o.total_energy(V)
+    H1, H2, DH1, DH2 = o.helicity(V)
+    assert abs(H1 - s * m * E) < 1e-14 and abs(E - 2 * A * A) < 1e-14
+    assert abs(H2 - H1 / m ** 2) < 1e-15 and abs(DH1 - 2 * m ** 2 * H1) < 1e-12 and abs(DH2 - 2 * H1) < 1e-14
+    Hk = o.shell_helicity(V)
+    assert abs(Hk[axis][m] - H1) < 1e-14
+    assert abs(np.abs(Hk).sum() - abs(H1)) < 1e-14
+
+
+def test_shell_helicity_sums_to_total_and_entropy_closed_form():
+    """Shell helicity (four_energy.cc:551-671) sums over shells and components to H1 for a field inside
+    kmax_inside; the entropy (four_energy.cc:687-732) of n stored kz > 0 modes of equal energy is
+    log2(2n) (each mode and its conjugate carry probability 1/(2n)); a random field's entropy lies
+    below log2 of twice its stored mode count."""
+    o = Oracle((16, 16, 16))
+    V = o.ic_random(8, E0=0.5, kp=2.0)
+    kin = o.max_radius_inside()
+    K = _kmag_grid(o)
+    for a in V:
+        a[K > kin] = 0
+    H1 = o.helicity(V)[0]
+    Hk = o.shell_helicity(V)
+    assert abs(Hk.sum() - H1) < 1e-13 * max(1.0, abs(H1))
+    W = o.zeros(3)
+    n = 7
+    for j in range(n):
+        W[j % 3][j + 1, j, 1 + j] = (0.3 + 0.4j) * (1 if j % 2 else -1)   # off the ky = N2/2 plane (multiplicity 2)
+    assert abs(o.entropy(W) - math.log2(2 * n)) < 1e-13
+    nmodes = int(np.count_nonzero(np.abs(V[0]) + np.abs(V[1]) + np.abs(V[2])))
+    assert 0 < o.entropy(V) < math.log2(2 * nmodes)
