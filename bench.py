@@ -100,6 +100,23 @@ def spectrum_err(plan64, Va, Vb):
     return maxrel, l2
 
 
+def divergence_rel(V, N):
+    """max_k |K . V(k)| / max_k |K| |V(k)| of a reference-layout vector field (device tensors)."""
+    import torch
+    dev = V[0].device
+
+    def signed(n):
+        l = torch.arange(n, device=dev, dtype=torch.float64)
+        return torch.where(l <= n // 2, l, l - n)
+    kx = signed(N[0]).view(-1, 1, 1)
+    ky = signed(N[1]).view(1, -1, 1)
+    kz = torch.arange(N[2] // 2 + 1, device=dev, dtype=torch.float64).view(1, 1, -1)
+    div = (kx * V[0] + ky * V[1] + kz * V[2]).abs().max()
+    mag = (torch.sqrt(kx * kx + ky * ky + kz * kz) *
+           torch.sqrt(V[0].abs() ** 2 + V[1].abs() ** 2 + V[2].abs() ** 2)).max()
+    return float(div / mag)
+
+
 def cpu_baseline(V_host, N, nu, dt, threads):
     """The FP64 oracle (the reference algorithm restated in C, OpenMP over slabs) timed on one RK
     stage of the same 512^3 workload on this host's cores."""
@@ -459,9 +476,17 @@ def main():
         out64 = s64.download()
         v64 = pts * 4 * args.steps / t64 / 1e6
         maxrel, l2 = spectrum_err(p64, [a.to(torch.complex128) for a in out32], out64)
+        # flux Pi(R) and shell-to-shell transfer T(n,m) over 32 unit shells (radii 0..31, inf) from
+        # the fused per-giver-shell pipeline, FP32-mixed against FP64
+        radii = np.concatenate([np.arange(32), [10000.0]]).astype(float)
+        Pi32, T32 = s32.flux_shell_tr(radii)
+        Pi64, T64 = s64.flux_shell_tr(radii)
         result["fp64"] = {"value": round(v64, 1), "ms_per_step": round(1e3 * t64 / args.steps, 3),
                           "speedup_fp32_mixed": round(value / v64, 3),
                           "Ek_rel_err_fp32_vs_fp64_max": maxrel, "Ek_rel_err_fp32_vs_fp64_l2": l2,
+                          "Pi_rel_err_fp32_vs_fp64_max": float(np.abs(Pi32 - Pi64).max() / np.abs(Pi64).max()),
+                          "T_rel_err_fp32_vs_fp64_max": float(np.abs(T32 - T64).max() / np.abs(T64).max()),
+                          "divergence_rel_fp64": divergence_rel(out64, N),
                           "steps_compared": args.warmup + args.steps, "launch_variants": var64}
         del s64, out64
 
@@ -503,6 +528,7 @@ def main():
                    "dt": dt, "cfl": 0.5, "ic": "random-phase k^4 exp(-2(k/4)^2), E=0.5, seed 1",
                    "dealias": "2/3", "precision": "fp32-mixed"},
         "fp64_gpu": result["fp64"],
+        "divergence_rel_fp32": divergence_rel(out32, N),
         "roofline": roof,
         "launch_variants": {"passes": PASS_NAMES, "fp32_mixed": var32},
         "stage": {"ms_per_stage_sum_of_passes": round(stage_ms, 4),
