@@ -99,7 +99,7 @@ __device__ __forceinline__ double ksq_f(double a, double b, double c)
 //     others would spill).
 //   x-forward epilogue: 0 = default lines per workgroup, 1 = half (lines of 128 or more).
 enum { PASS_XINV = 0, PASS_YINV = 1, PASS_Z = 2, PASS_YFWD = 3, PASS_XFWD = 4, NPASS = 5 };
-constexpr int pass_nvar(int pass) { return pass == PASS_Z ? 4 : pass == PASS_XFWD ? 2 : 4; }
+constexpr int pass_nvar(int pass) { return pass == PASS_Z ? 4 : pass == PASS_XFWD ? 3 : 4; }
 
 template <typename R, int MODE>
 int d_xinv(const FG& g, const cplx<R>* S, cplx<R>* A, int nc, const cplx<R>* tw, ShellSel sel, int var, hipStream_t st);

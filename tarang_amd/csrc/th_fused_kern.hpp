@@ -1014,7 +1014,8 @@ static int launch_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, c
                        const StageCoef* sc, hipStream_t st)
 {
     constexpr int E = xfwd_E(L), P = L / E, T0 = xfwd_T<R, L, MODE>();
-    constexpr int T = (V == 1 && L >= 128 && T0 > 1) ? T0 / 2 : T0;
+    // variant 1: half-width tiles; variant 2: double-width tiles (full 128-byte rows at 512 lines)
+    constexpr int T = (V == 1 && L >= 128 && T0 > 1) ? T0 / 2 : (V == 2) ? 2 * T0 : T0;
     const int ntz = (g.nkz + T - 1) / T;
     hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D, S, Vs, acc, tw,
                        sc);
@@ -1107,6 +1108,9 @@ static int xfwd_v(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R
 {
     if constexpr (L >= 128)
         if (var == 1) return launch_xfwd<R, L, MODE, 1>(g, D, S, Vs, acc, tw, sc, st);
+    // 1024-thread workgroups hold a wave to 128 VGPRs: only the FP32 NS kernels fit (124)
+    if constexpr (L >= 128 && sizeof(R) == 4 && MODE == 0 && L % 3 != 0 && 2 * xfwd_T<R, L, MODE>() * (L / xfwd_E(L)) <= 1024)
+        if (var == 2) return launch_xfwd<R, L, MODE, 2>(g, D, S, Vs, acc, tw, sc, st);
     return launch_xfwd<R, L, MODE, 0>(g, D, S, Vs, acc, tw, sc, st);
 }
 

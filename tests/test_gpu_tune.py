@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 from pyoracle import Oracle  # noqa: E402
 
-NVAR = [4, 4, 4, 4, 2]
+NVAR = [4, 4, 4, 4, 3]
 
 
 @pytest.fixture(scope="module")
