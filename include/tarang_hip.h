@@ -51,6 +51,17 @@ typedef struct {
     double Ra, Pr;  /* RB: Ra, Pr; nu = sqrt(Pr/Ra), kappa = 1/sqrt(Pr Ra) (RB_slip_main.cc:238-239) */
 } th_phys;
 
+/* Energy / helicity forcing of the velocity (force_double_para of IncFluid.cc:201-217):
+ * proc 1 == Compute_force_const_energy_helicity_supply compute_force_ek_hk_supply.cc:51-157
+ *           (p3 = energy_supply, p4 = epsh_by_k_epse),
+ * proc 2 == Compute_force_const_energy_helicity compute_force_const_ek_hk.cc:51-140
+ *           (p3 = energy_level, p4 = h_by_k_E). */
+typedef struct th_force {
+    int proc;                       /* force_field_proc: 1 or 2 */
+    double inner_radius, outer_radius;
+    double p3, p4;
+} th_force;
+
 /* ---- context and plans ----
  * replaces: Init_fftw_plan_array universal_tr.cc:48, Init_fftw_plan_FOUR four_tr.cc:44-98,
  *           Init_fftw_plan_SCFT scft_tr.cc:49-74 and the process globals extern_def_vars.h:95-142 */
@@ -106,6 +117,17 @@ int th_et_nlin(th_plan *plan, void *const H[3], void *const G[3], void *nlin[3])
  *    Time_advance(IncVF&) time_advance_VF_MHD.cc:59-180, Time_advance(IncSF&) time_advance_SF.cc:61-233.
  * fields: NS V[0..2]; MHD U[0..2], B[0..2]; RB V[0..2], T.  scheme: 0 EULER, 1 RK2, 2 RK4. */
 int th_step_plain(th_plan *plan, int physics, int scheme, void *const fields[], double dt, const th_phys *p);
+/* NS step with the forcing of `force` recomputed from the stage field at every stage and subtracted
+ * from nlin before the pressure: Compute_force_TO_rhs single_time_step.cc:351-364 (Compute_force
+ * compute_force_main.cc:44-60, Add_force add_force.cc:47-53).  FOUR plans only. */
+int th_step_plain_forced(th_plan *plan, int scheme, void *const V[3], double dt, const th_phys *p,
+                         const th_force *force);
+/* The force field itself (IncFluid::Compute_force for proc 1 / 2): F = alpha V + beta (i K x V) on the
+ * band inner < |K| <= outer, zero elsewhere, dealiased when the band reaches past N/3 and with the
+ * reality condition applied.  nf (may be NULL) receives the mode count of Get_number_modes_in_shell.
+ * Errors: TH_E_UNSUPPORTED for other procs or an SCFT plan; TH_E_ARG for an empty band, or for proc 2
+ * with |h_by_k_E| > 1, where the reference's square roots are not real. */
+int th_compute_force(th_plan *plan, void *const V[3], void *F[3], const th_force *force, long *nf);
 
 /* ---- energies, spectra, transfers (return host doubles; synchronise) ----
  * th_energy   == Get_total_energy four_energy.cc:88-168 (B == NULL: |A|^2; else Re A conj B)

@@ -786,17 +786,17 @@ void orc_rk4_step_ns(const orc_grid *g, double *V[3], double nu, double dt)
 }
 
 /* EULER and RK2 branches of Time_advance (time_advance.cc:64-98) */
-void orc_euler_step_ns(const orc_grid *g, double *V[3], double nu, double dt)
+static void euler_generic(const orc_grid *g, double *V[3], double nu, double dt, rhs_fn rhs, void *ctx)
 {
     stepper_t s; memset(&s, 0, sizeof(s));
     s.nc = 3;
     for (int i = 0; i < 3; i++) { s.F[i] = V[i]; s.N[i] = alloc_field(g); s.diss[i] = nu; }
-    rhs_ns(g, &s, NULL);
+    rhs(g, &s, ctx);
     single_time_step(g, &s, dt, 1, 1, 1);
     for (int i = 0; i < 3; i++) { if (g->dealias) orc_dealias(g, V[i]); free(s.N[i]); }
 }
 
-void orc_rk2_step_ns(const orc_grid *g, double *V[3], double nu, double dt)
+static void rk2_generic(const orc_grid *g, double *V[3], double nu, double dt, rhs_fn rhs, void *ctx)
 {
     stepper_t s; memset(&s, 0, sizeof(s));
     s.nc = 3;
@@ -805,12 +805,157 @@ void orc_rk2_step_ns(const orc_grid *g, double *V[3], double nu, double dt)
         s.F[i] = V[i]; s.N[i] = alloc_field(g); s.diss[i] = nu;
         copy[i] = alloc_field(g); fcopy(g, copy[i], V[i]);
     }
-    rhs_ns(g, &s, NULL);
+    rhs(g, &s, ctx);
     single_time_step(g, &s, dt, 0.5, 0.5, 0.5);
-    rhs_ns(g, &s, NULL);
+    rhs(g, &s, ctx);
     for (int i = 0; i < 3; i++) fcopy(g, V[i], copy[i]);
     single_time_step(g, &s, dt, 1, 0.5, 1);
     for (int i = 0; i < 3; i++) { if (g->dealias) orc_dealias(g, V[i]); free(s.N[i]); free(copy[i]); }
+}
+
+void orc_euler_step_ns(const orc_grid *g, double *V[3], double nu, double dt)
+{
+    euler_generic(g, V, nu, dt, rhs_ns, NULL);
+}
+
+void orc_rk2_step_ns(const orc_grid *g, double *V[3], double nu, double dt)
+{
+    rk2_generic(g, V, nu, dt, rhs_ns, NULL);
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Energy / helicity supply forcing, force_field_proc = 1 on a FOUR box:
+ *   IncFluid::Compute_force_const_energy_helicity_supply compute_force_ek_hk_supply.cc:51-157,
+ *   Get_local_number_modes_in_shell_FOUR four_basic.cc:48-90 (Get_lx_FOUR four_inline.h:103-106,
+ *   Get_ly3D_FOUR :138), IncVF::Force_alpha_beta force_alpha_beta.cc:60-89, CV_Modal_energy
+ *   Cvf.cc:595-600 (Modal_energy_FOUR four_inline.h:367-370), Get_Modal_helicity_FOUR
+ *   four_basic.cc:136-155, Compute_Modal_vorticity_FOUR :166-181, Is_alias_array
+ *   universal_misc.cc:169-181, Dealias_force dealias.cc:78-84, Satisfy_reality_condition_force_field
+ *   reality_cond.cc:90-95; applied by Compute_force_TO_rhs single_time_step.cc:351-364 with
+ *   Add_force add_force.cc:47-53 (nlin -= F before the pressure).
+ * ------------------------------------------------------------------------------------------ */
+#define MYEPS2 1e-5
+
+/* nf: modes with inner <= |K| < outer over kx in [-kx_max, kx_max], ky in [-ky_max, ky_max],
+ * kz in [0, kz_max], each (kx, ky, 0) counted once and each kz > 0 mode twice (its conjugate). */
+int orc_force_supply_modes(const orc_grid *g, double inner, double outer)
+{
+    int kx_max = (int)ceil(outer / g->f1);
+    int ky_max = g->N2 > 1 ? (int)ceil(outer / g->f2) : 0;
+    int kz_max = g->N3 >= 2 ? (int)ceil(outer / g->f3) : 0;
+    int count = 0;
+    for (int kx = -kx_max; kx <= kx_max; kx++)
+        for (int ky = -ky_max; ky <= ky_max; ky++)
+            for (int kz = 0; kz <= kz_max; kz++) {
+                int l1 = kx >= 0 ? kx : kx + g->N1, l2 = ky >= 0 ? ky : ky + g->N2;
+                double km = orc_kmag(g, l1, l2, kz);
+                if (km >= inner && km < outer) count += (kz == 0) ? 1 : 2;
+            }
+    return count;
+}
+
+/* proc 1 (Compute_force_const_energy_helicity_supply, p3 = energy_supply, p4 = epsh_by_k_epse) and
+ * proc 2 (Compute_force_const_energy_helicity compute_force_const_ek_hk.cc:51-140, p3 = energy_level,
+ * p4 = h_by_k_E): F = alpha V + beta (i K x V) on the modes inner < |K| <= outer with modal energy
+ * > MYEPS, zero elsewhere; the vorticity term only when beta > MYEPS.  abs(sk*sk - 1) is the double
+ * overload.  Returns nf (0: no mode in the band, F untouched). */
+int orc_force_ek_hk(const orc_grid *g, int proc, double *const V[3], double *F[3], double inner, double outer,
+                    double p3, double p4)
+{
+    int nf = orc_force_supply_modes(g, inner, outer);
+    if (nf <= 0) return 0;
+    double per_mode = p3 / nf;
+    int N1 = g->N1, N2 = g->N2, H = H3(g);
+    for (int c = 0; c < 3; c++) memset(F[c], 0, 2 * NCPLX(g) * sizeof(double));
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 < H; l3++) {
+                double km = orc_kmag(g, l1, l2, l3);
+                if (!(km > inner && km <= outer)) continue;
+                if (proc == 2 && !(per_mode > MYEPS)) continue;
+                size_t i = IDX(g, l1, l2, l3);
+                double E = 0, vr[3], vi[3];
+                for (int c = 0; c < 3; c++) {
+                    vr[c] = RE(V[c], i); vi[c] = IM(V[c], i);
+                    E += pow2(vr[c]) + pow2(vi[c]);
+                }
+                E = E / 2;
+                if (!(E > MYEPS)) continue;
+                double kk[3] = {orc_kx(g, l1) * g->f1, orc_ky(g, l2) * g->f2, l3 * g->f3};
+                double hel = kk[0] * (vr[1] * vi[2] - vr[2] * vi[1]) + kk[1] * (vr[2] * vi[0] - vr[0] * vi[2]) +
+                             kk[2] * (vr[0] * vi[1] - vr[1] * vi[0]);
+                double sk = hel / (km * E), alpha, beta;
+                if (proc == 1) {
+                    double temp = per_mode / (2 * E);
+                    if (fabs(sk * sk - 1) > MYEPS2) {
+                        alpha = temp * (1 - sk * p4) / (1 - sk * sk);
+                        beta = temp * (p4 - sk) / (1 - sk * sk);
+                    } else {
+                        alpha = temp / 2;
+                        beta = alpha / (sk * km);
+                    }
+                } else {
+                    double temp1 = sqrt(per_mode / E);
+                    if (fabs(sk * sk - 1) > MYEPS2) {
+                        double temp2 = sqrt((1 + p4) / (1 + sk)), temp3 = sqrt((1 - p4) / (1 - sk));
+                        alpha = (temp1 / 2) * (temp2 + temp3);
+                        beta = (temp1 / (2 * km)) * (temp2 - temp3);
+                    } else {
+                        alpha = temp1;
+                        beta = 0;
+                    }
+                }
+                /* omega = i K x V */
+                double wr[3] = {0, 0, 0}, wi[3] = {0, 0, 0};
+                if (beta > MYEPS) {
+                    for (int c = 0; c < 3; c++) {
+                        int a = (c + 1) % 3, b = (c + 2) % 3;
+                        double cr = kk[a] * vr[b] - kk[b] * vr[a], ci = kk[a] * vi[b] - kk[b] * vi[a];
+                        wr[c] = -ci; wi[c] = cr;
+                    }
+                }
+                for (int c = 0; c < 3; c++) {
+                    RE(F[c], i) = alpha * vr[c] + beta * wr[c];
+                    IM(F[c], i) = alpha * vi[c] + beta * wi[c];
+                }
+            }
+    if (g->dealias) {
+        int kx_max = (int)ceil(outer / g->f1), ky_max = (int)ceil(outer / g->f2), kz_max = (int)ceil(outer / g->f3);
+        if (kx_max > N1 / 3 || ky_max > N2 / 3 || kz_max > g->N3 / 3)
+            for (int c = 0; c < 3; c++) orc_dealias(g, F[c]);
+    }
+    for (int c = 0; c < 3; c++) orc_satisfy_reality(g, F[c]);
+    return nf;
+}
+
+typedef struct { int proc; double inner, outer, p3, p4; } force_ctx;
+
+static void rhs_ns_forced(const orc_grid *g, stepper_t *s, void *ctx)
+{
+    const force_ctx *fc = (const force_ctx *)ctx;
+    double *F[3], *p = alloc_field(g);
+    for (int c = 0; c < 3; c++) F[c] = alloc_field(g);
+    orc_force_ek_hk(g, fc->proc, s->F, F, fc->inner, fc->outer, fc->p3, fc->p4);
+    orc_nlin_ns(g, s->F, s->N);
+    for (int c = 0; c < 3; c++) faxpy(g, s->N[c], -1.0, F[c]);
+    orc_pressure(g, s->N, p);
+    orc_rhs(g, s->N, p);
+    for (int c = 0; c < 3; c++) free(F[c]);
+    free(p);
+}
+
+/* scheme 0 EULER, 1 RK2, 2 RK4 */
+void orc_step_ns_forced(const orc_grid *g, double *V[3], double nu, double dt, int scheme, int proc, double inner,
+                        double outer, double p3, double p4)
+{
+    force_ctx fc = {proc, inner, outer, p3, p4};
+    if (scheme == 0) { euler_generic(g, V, nu, dt, rhs_ns_forced, &fc); return; }
+    if (scheme == 1) { rk2_generic(g, V, nu, dt, rhs_ns_forced, &fc); return; }
+    stepper_t s; memset(&s, 0, sizeof(s));
+    s.nvec = 1; s.nc = 3;
+    for (int i = 0; i < 3; i++) { s.F[i] = V[i]; s.N[i] = alloc_field(g); s.diss[i] = nu; }
+    rk4_generic(g, &s, dt, rhs_ns_forced, &fc);
+    for (int i = 0; i < 3; i++) free(s.N[i]);
 }
 
 /* ------------------------------------------------------------------------------------------

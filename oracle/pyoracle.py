@@ -59,6 +59,9 @@ def lib():
             "orc_rk4_step_ns": (None, [p, pp, d, d]),
             "orc_euler_step_ns": (None, [p, pp, d, d]),
             "orc_rk2_step_ns": (None, [p, pp, d, d]),
+            "orc_force_supply_modes": (i, [p, d, d]),
+            "orc_force_ek_hk": (i, [p, i, pp, pp, d, d, d, d]),
+            "orc_step_ns_forced": (None, [p, pp, d, d, i, i, d, d, d, d]),
             "orc_rk4_stage1_ns": (None, [p, pp, d, d]),
             "orc_nlin_mhd": (None, [p, pp, pp, pp, pp]),
             "orc_rk4_step_mhd": (None, [p, pp, pp, d, d, d]),
@@ -217,6 +220,25 @@ class Oracle:
 
     def rk2_ns(self, V, nu, dt):
         self.L.orc_rk2_step_ns(self.gp, _pp(V), nu, dt)
+        return V
+
+    def force_supply_modes(self, inner, outer):
+        """nf of Compute_force_const_energy_helicity_supply (modes with inner <= |K| < outer)."""
+        return self.L.orc_force_supply_modes(self.gp, inner, outer)
+
+    def force(self, V, proc, inner, outer, p3, p4=0.0):
+        """Force field F[3] of force_field_proc 1 (energy/helicity supply: p3 = energy_supply,
+        p4 = epsh_by_k_epse) or 2 (constant energy/helicity: p3 = energy_level, p4 = h_by_k_E)."""
+        F = self.zeros(3)
+        nf = self.L.orc_force_ek_hk(self.gp, proc, _pp(V), _pp(F), inner, outer, p3, p4)
+        if nf <= 0:
+            raise ValueError("no mode in the forcing band")
+        return F
+
+    def step_ns_forced(self, V, nu, dt, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1):
+        """NS steps (scheme 0 EULER, 1 RK2, 2 RK4) with the force recomputed at every stage."""
+        for _ in range(steps):
+            self.L.orc_step_ns_forced(self.gp, _pp(V), nu, dt, scheme, proc, inner, outer, p3, p4)
         return V
 
     def nlin_mhd(self, U, B):

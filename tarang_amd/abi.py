@@ -46,6 +46,13 @@ class Phys(C.Structure):
                 ("Ra", C.c_double), ("Pr", C.c_double)]
 
 
+class Force(C.Structure):
+    """th_force: force_field_proc 1 (p3 = energy_supply, p4 = epsh_by_k_epse) or 2 (p3 = energy_level,
+    p4 = h_by_k_E) on the band inner_radius < |K| <= outer_radius."""
+    _fields_ = [("proc", C.c_int), ("inner_radius", C.c_double), ("outer_radius", C.c_double),
+                ("p3", C.c_double), ("p4", C.c_double)]
+
+
 def lib():
     """Load the gfx950 library.  Raises if it has not been built (no fallback path exists)."""
     global _lib
@@ -86,6 +93,8 @@ def lib():
         "th_nlin_rb": (i, [vp, pp, vp, pp, vp]),
         "th_et_nlin": (i, [vp, pp, pp, pp]),
         "th_step_plain": (i, [vp, i, i, pp, d, C.POINTER(Phys)]),
+        "th_step_plain_forced": (i, [vp, i, pp, d, C.POINTER(Phys), C.POINTER(Force)]),
+        "th_compute_force": (i, [vp, pp, pp, C.POINTER(Force), C.POINTER(C.c_long)]),
         "th_energy": (i, [vp, vp, vp, dp]),
         "th_Sn": (i, [vp, vp, d, dp]),
         "th_shell_spectrum": (i, [vp, vp, vp, i, dp, dp]),
@@ -363,6 +372,26 @@ class Plan:
                    "th_step_plain")
         self._post(0, "")
         return fields
+
+    def step_plain_forced(self, V, dt, nu, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1):
+        """NS steps with the force_field_proc `proc` forcing recomputed at every stage."""
+        ph = Phys(nu, 0.0, 0.0, 0.0, 0.0)
+        fc = Force(proc, inner, outer, p3, p4)
+        self._pre()
+        for _ in range(steps):
+            _check(lib().th_step_plain_forced(self.h, scheme, _pp(V), dt, C.byref(ph), C.byref(fc)),
+                   "th_step_plain_forced")
+        self._post(0, "")
+        return V
+
+    def compute_force(self, V, proc, inner, outer, p3, p4=0.0):
+        """Force field F[3] of force_field_proc `proc` for the velocity V, and nf."""
+        F = self.empty(3)
+        fc = Force(proc, inner, outer, p3, p4)
+        nf = C.c_long()
+        self._pre()
+        self._post(lib().th_compute_force(self.h, _pp(V), _pp(F), C.byref(fc), C.byref(nf)), "th_compute_force")
+        return F, nf.value
 
     # diagnostics
     def energy(self, A, B=None):

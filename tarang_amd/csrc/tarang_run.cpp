@@ -3,11 +3,14 @@
 // energy + NaN check, then the shell spectrum and (at the end) the energy flux.
 //
 //   tarang_run N steps dt nu [tg|random|file=<field_in.d>] [FP64|FP32_MIXED] [fused|plain] [save=<field_out.d>]
+//              [out=DIR] [force=PROC,INNER,OUTER,P3,P4]
 // Output (stdout): "glob t E D" per step, then "spectrum k E(k)" lines and "flux R Pi(R)" lines.
 // file=: Init_cond from a field file (field_input_proc = 1; its "%% Time" sets Tinit); save=: the
 // final field is appended to a field file (Output_field), the restart point of a later run.
 // out=DIR (any position after nu): DIR/out/glob.d every step, DIR/out/spectrum.d and flux.d at the end,
 // in the reference's formats (Output_global, Output_shell_spectrum, Output_flux).
+// force=PROC,INNER,OUTER,P3,P4: force_field_proc 1 (energy/helicity supply) or 2 (constant energy/helicity)
+// with its force_double_para, recomputed at every stage on the plain stepper.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -19,7 +22,8 @@
 int main(int argc, char** argv)
 {
     if (argc < 5) {
-        fprintf(stderr, "usage: %s N steps dt nu [tg|random|file=PATH] [FP64|FP32_MIXED] [fused|plain] [save=PATH]\n",
+        fprintf(stderr, "usage: %s N steps dt nu [tg|random|file=PATH] [FP64|FP32_MIXED] [fused|plain] [save=PATH] "
+                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4]\n",
                 argv[0]);
         return 2;
     }
@@ -29,9 +33,16 @@ int main(int argc, char** argv)
     const std::string prec = argc > 6 ? argv[6] : "FP64";
     const std::string path = argc > 7 ? argv[7] : "fused";
     std::string save, outdir;
+    int fproc = 0;
+    double fpar[4] = {0, 0, 0, 0};
     for (int a = 5; a < argc; a++) {
         if (strncmp(argv[a], "save=", 5) == 0) save = argv[a] + 5;
         if (strncmp(argv[a], "out=", 4) == 0) outdir = argv[a] + 4;
+        if (strncmp(argv[a], "force=", 6) == 0 &&
+            sscanf(argv[a] + 6, "%d,%lf,%lf,%lf,%lf", &fproc, &fpar[0], &fpar[1], &fpar[2], &fpar[3]) != 5) {
+            fprintf(stderr, "tarang_run: force=PROC,INNER,OUTER,P3,P4 expected\n");
+            return 2;
+        }
     }
     try {
         tarang::Context ctx(0);
@@ -46,6 +57,7 @@ int main(int argc, char** argv)
             void* V[3] = {U.V1.data(), U.V2.data(), U.V3.data()};
             tarang::check(th_ic_random(U.plan(), V, 1, 0.5, 4.0), "th_ic_random");
         }
+        if (fproc) U.Set_force(fproc, fpar[0], fpar[1], fpar[2], fpar[3]);
         if (!outdir.empty()) U.Open_output_files(outdir);
         for (int it = 0; it < steps; it++) {
             U.Tdt = U.Get_dt();

@@ -245,12 +245,6 @@ template <typename R> const cplx<R>* th_plan::twd(int L)
 template const cplx<float>* th_plan::twd<float>(int);
 template const cplx<double>* th_plan::twd<double>(int);
 
-#define TRY(x)                  \
-    do {                        \
-        int r_ = (x);           \
-        if (r_) return r_;      \
-    } while (0)
-
 // ------------------------------------------------------------------------------------------
 // basis transforms
 // ------------------------------------------------------------------------------------------
@@ -615,12 +609,16 @@ template <typename R> struct Ops {
     double diss[7];
     int physics = 0;
     double Ra = 0, Pr = 0;
+    const ForceBand* force = nullptr;    // NS forcing (force_field_proc 1 / 2), else none
 
     int compute_rhs()
     {
         cplx<R>* pr = S(SC_P);
         if (physics == TH_NS) {
             TRY(nlin_ns(F, N));
+            // Compute_force_TO_rhs single_time_step.cc:351-364: the force of the stage field (which
+            // Compute_nlin leaves unchanged) is subtracted from nlin before the pressure
+            if (force) TRY(force_subtract<R>(*force, F, N, st));
         } else if (physics == TH_MHD) {
             TRY(nlin_mhd(F, F + 3, N, N + 3));
         } else {
@@ -834,6 +832,34 @@ extern "C" int th_step_plain(th_plan* p, int physics, int scheme, void* const fi
     if (!fields || !ph || physics < 0 || physics > 2 || scheme < 0 || scheme > 2) { set_error("th_step_plain: bad args"); return E_ARG; }
     if (physics == TH_RB && p->g.basis != TH_SCFT) { set_error("th_step_plain: RB needs SCFT"); return E_ARG; }
     return DISPATCH(p, do_step_plain, p, physics, scheme, fields, dt, ph);
+}
+
+template <typename R> static int do_step_plain_forced(th_plan* p, int scheme, void* const V[3], double dt,
+                                                      const th_phys* ph, const ForceBand* fb)
+{
+    Ops<R> o(p);
+    o.physics = TH_NS;
+    o.nc = 3;
+    o.force = fb;
+    for (int q = 0; q < 3; q++) {
+        o.F[q] = CP(R, V[q]);
+        o.N[q] = o.S(SC_N0 + q);
+        if (!o.N[q]) return E_NOMEM;
+        o.diss[q] = ph->nu;
+    }
+    return o.step(scheme, dt);
+}
+extern "C" int th_step_plain_forced(th_plan* p, int scheme, void* const V[3], double dt, const th_phys* ph,
+                                    const th_force* force)
+{
+    TRY(check_plan(p));
+    if (!V || !V[0] || !V[1] || !V[2] || !ph || scheme < 0 || scheme > 2) {
+        set_error("th_step_plain_forced: bad args");
+        return E_ARG;
+    }
+    ForceBand fb;
+    TRY(force_setup(p->g, force, fb));
+    return DISPATCH(p, do_step_plain_forced, p, scheme, V, dt, ph, &fb);
 }
 
 // ------------------------------------------------------------------------------------------

@@ -25,4 +25,18 @@ template <typename R> __host__ __device__ __forceinline__ cplx<R> czero() { retu
 __host__ __device__ __forceinline__ int kx_four(int l1, int N1) { return (l1 <= N1 / 2) ? l1 : l1 - N1; }
 __host__ __device__ __forceinline__ int ky_of(int l2, int N2) { return (l2 <= N2 / 2) ? l2 : l2 - N2; }
 
+// Dealias_array's 2/3 rule (universal_misc.cc:118-159): true when the mode survives.
+__host__ __device__ __forceinline__ bool kept_mode(int basis, int dealias, int N1, int N2, int N3, int l1, int l2, int l3)
+{
+    if (!dealias) return true;
+    int kx = basis == 1 ? l1 : kx_four(l1, N1);
+    if (N1 >= 3) {
+        int ak = kx < 0 ? -kx : kx;
+        if (basis == 0 ? (ak >= N1 / 3) : (ak >= 2 * N1 / 3)) return false;
+    }
+    if (N2 >= 3 && l2 >= N2 / 3 && l2 <= 2 * N2 / 3 + 1) return false;
+    if (N3 >= 3 && l3 >= N3 / 3 && l3 <= N3 / 2) return false;
+    return true;
+}
+
 }  // namespace th

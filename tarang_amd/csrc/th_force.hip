@@ -1,0 +1,253 @@
+// th_force.hip -- energy / helicity forcing of the velocity on a FOUR box (force_field_proc 1 and 2):
+//   proc 1: IncFluid::Compute_force_const_energy_helicity_supply compute_force_ek_hk_supply.cc:51-157
+//   proc 2: IncFluid::Compute_force_const_energy_helicity compute_force_const_ek_hk.cc:51-140
+//   with IncVF::Force_alpha_beta force_alpha_beta.cc:60-89, Get_local_number_modes_in_shell_FOUR
+//   four_basic.cc:48-90, Is_alias_array universal_misc.cc:169-181 + Dealias_force dealias.cc:78-84 and
+//   Satisfy_reality_condition_force_field reality_cond.cc:90-95.
+//
+// The force is nonzero only on the band inner < |K| <= outer.  One thread per mode of the band's bounding
+// box (kx in [-kx_max, kx_max], ky in [-ky_max, ky_max], kz in [0, kz_max], the reference's own loop
+// bounds) computes alpha, beta and F = alpha V + beta (i K x V) in FP64 from the stored velocity, then
+// either stores F (the force field; the rest of the array is cleared first) or subtracts it from the
+// nonlinear term in place (Add_force add_force.cc:47-53).  A forced step therefore touches a few thousand
+// modes instead of making three more whole-field passes.  Dealias_force and the reality condition are
+// folded in: a mode that Satisfy_reality_array overwrites (kz = 0 and kz = N3/2 planes) takes the
+// conjugate of the force computed at its source mode, after the dealias rule at that source.
+#include <math.h>
+#include "../../include/tarang_hip.h"
+#include "th_internal.hpp"
+#include "th_plan.hpp"
+
+using namespace th;
+
+namespace {
+
+constexpr double MYEPS = 1e-15;    // extern_def_vars.h:81
+constexpr double MYEPS2 = 1e-5;    // extern_def_vars.h:82
+
+__host__ __device__ __forceinline__ double kmag3(double a, double b, double c)
+{
+#pragma clang fp contract(off)
+    return sqrt(a * a + b * b + c * c);
+}
+
+// force of one stored mode (FP64), before the reality condition
+template <typename R>
+__device__ void mode_force(const ForceBand& b, const cplx<R>* const* V, int lx, int ly, int lz, double fr[3],
+                           double fi[3])
+{
+    fr[0] = fr[1] = fr[2] = fi[0] = fi[1] = fi[2] = 0.0;
+    const double kk[3] = {kx_four(lx, b.N1) * b.f1, ky_of(ly, b.N2) * b.f2, lz * b.f3};
+    const double km = kmag3(kk[0], kk[1], kk[2]);
+    if (!(km > b.inner && km <= b.outer)) return;
+    if (b.proc == 2 && !(b.per_mode > MYEPS)) return;
+    if (b.dealias_force && !kept_mode(0, 1, b.N1, b.N2, b.N3, lx, ly, lz)) return;
+    const long i = ((long)lx * b.N2 + ly) * b.H + lz;
+    double vr[3], vi[3], E = 0.0;
+    for (int c = 0; c < 3; c++) {
+        const cplx<R> v = V[c][i];
+        vr[c] = (double)v.x;
+        vi[c] = (double)v.y;
+        E += vr[c] * vr[c] + vi[c] * vi[c];
+    }
+    E = E / 2;                                               // CV_Modal_energy Cvf.cc:595-600
+    if (!(E > MYEPS)) return;
+    const double hel = kk[0] * (vr[1] * vi[2] - vr[2] * vi[1]) + kk[1] * (vr[2] * vi[0] - vr[0] * vi[2]) +
+                       kk[2] * (vr[0] * vi[1] - vr[1] * vi[0]);       // Get_Modal_helicity_FOUR
+    const double sk = hel / (km * E);
+    double alpha, beta;
+    if (b.proc == 1) {
+        const double temp = b.per_mode / (2 * E);
+        if (fabs(sk * sk - 1) > MYEPS2) {
+            alpha = temp * (1 - sk * b.p4) / (1 - sk * sk);
+            beta = temp * (b.p4 - sk) / (1 - sk * sk);
+        } else {
+            alpha = temp / 2;
+            beta = alpha / (sk * km);
+        }
+    } else {
+        const double temp1 = sqrt(b.per_mode / E);
+        if (fabs(sk * sk - 1) > MYEPS2) {
+            const double temp2 = sqrt((1 + b.p4) / (1 + sk)), temp3 = sqrt((1 - b.p4) / (1 - sk));
+            alpha = (temp1 / 2) * (temp2 + temp3);
+            beta = (temp1 / (2 * km)) * (temp2 - temp3);
+        } else {
+            alpha = temp1;
+            beta = 0.0;
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        fr[c] = alpha * vr[c];
+        fi[c] = alpha * vi[c];
+    }
+    if (beta > MYEPS) {                                      // vorticity i K x V only for beta > MYEPS
+        for (int c = 0; c < 3; c++) {
+            const int a1 = (c + 1) % 3, a2 = (c + 2) % 3;
+            const double cr = kk[a1] * vr[a2] - kk[a2] * vr[a1], ci = kk[a1] * vi[a2] - kk[a2] * vi[a1];
+            fr[c] += beta * -ci;
+            fi[c] += beta * cr;
+        }
+    }
+}
+
+// MODE 0: Out = F on the box (the array was cleared); MODE 1: Out -= F
+template <typename R, int MODE>
+__global__ void k_force_band(ForceBand b, const cplx<R>* V0, const cplx<R>* V1, const cplx<R>* V2, cplx<R>* O0,
+                             cplx<R>* O1, cplx<R>* O2)
+{
+    const long n = (long)b.nx * b.ny * b.nz;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int tz = (int)(t % b.nz);
+    const long r = t / b.nz;
+    const int ty = (int)(r % b.ny), tx = (int)(r / b.ny);
+    int lx, ly;
+    if (b.fullx) {
+        lx = tx;
+    } else {
+        const int kx = tx - b.kxm;
+        lx = kx >= 0 ? kx : kx + b.N1;                       // Get_lx_FOUR four_inline.h:103-106
+    }
+    if (b.fully) {
+        ly = ty;
+    } else {
+        const int ky = ty - b.kym;
+        ly = ky >= 0 ? ky : ky + b.N2;                       // Get_ly3D_FOUR four_inline.h:138
+    }
+    const int lz = tz;
+    // Satisfy_reality_array on the kz = 0 and kz = N3/2 planes: overwritten modes take conj(F(source))
+    int sx = lx, sy = ly;
+    bool mirror = false;
+    if (lz == 0 || lz == b.N3 / 2) {
+        const int kx = kx_four(lx, b.N1);
+        if (ly > b.N2 / 2) {
+            sx = (-kx >= 0) ? -kx : -kx + b.N1;
+            sy = -ky_of(ly, b.N2);
+            mirror = true;
+        } else if (ly == 0 && kx < 0) {
+            sx = -kx;
+            mirror = true;
+        }
+    }
+    const cplx<R>* V[3] = {V0, V1, V2};
+    double fr[3], fi[3];
+    mode_force<R>(b, V, sx, sy, lz, fr, fi);
+    if (mirror)
+        for (int c = 0; c < 3; c++) fi[c] = -fi[c];
+    const long i = ((long)lx * b.N2 + ly) * b.H + lz;
+    cplx<R>* O[3] = {O0, O1, O2};
+    for (int c = 0; c < 3; c++) {
+        if (MODE == 0) {
+            O[c][i] = mk<R>((R)fr[c], (R)fi[c]);
+        } else {
+            const cplx<R> o = O[c][i];
+            O[c][i] = mk<R>((R)((double)o.x - fr[c]), (R)((double)o.y - fi[c]));
+        }
+    }
+}
+
+template <typename R, int MODE>
+int launch_band(const ForceBand& b, const cplx<R>* const V[3], cplx<R>* const O[3], hipStream_t st)
+{
+    const long n = (long)b.nx * b.ny * b.nz;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL((k_force_band<R, MODE>), dim3(blocks), dim3(256), 0, st, b, V[0], V[1], V[2], O[0], O[1],
+                       O[2]);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+
+}  // namespace
+
+namespace th {
+
+int force_setup(const Geo& g, const th_force* f, ForceBand& b)
+{
+    if (!f) { set_error("force: null parameters"); return E_ARG; }
+    if (f->proc != 1 && f->proc != 2) {
+        set_error("force: force_field_proc %d is not supported (1: energy/helicity supply, 2: constant energy/"
+                  "helicity)", f->proc);
+        return E_UNSUPPORTED;
+    }
+    if (g.basis != TH_FOUR) { set_error("force: only the FOUR basis is supported"); return E_UNSUPPORTED; }
+    if (!(f->outer_radius > f->inner_radius) || !(f->inner_radius >= 0) || !isfinite(f->outer_radius) ||
+        !isfinite(f->p3) || !isfinite(f->p4)) {
+        set_error("force: need 0 <= inner_radius < outer_radius and finite parameters");
+        return E_ARG;
+    }
+    if (f->proc == 2 && !(fabs(f->p4) <= 1.0)) {
+        set_error("force: h_by_k_E = %g outside [-1, 1] (the square roots of the constant-helicity force are "
+                  "not real)", f->p4);
+        return E_ARG;
+    }
+    const double inner = f->inner_radius, outer = f->outer_radius;
+    const int kxm = (int)ceil(outer / g.kf[0]);
+    const int kym = g.N2 > 1 ? (int)ceil(outer / g.kf[1]) : 0;
+    const int kzm = g.N3 > 2 ? (int)ceil(outer / g.kf[2]) : 0;
+    // nf: Get_local_number_modes_in_shell_FOUR four_basic.cc:48-90 (inner <= |K| < outer, kz > 0 twice)
+    long nf = 0;
+    const int kzc = g.N3 >= 2 ? (int)ceil(outer / g.kf[2]) : 0;
+    for (int kx = -kxm; kx <= kxm; kx++)
+        for (int ky = -kym; ky <= kym; ky++)
+            for (int kz = 0; kz <= kzc; kz++) {
+                const int l1 = kx >= 0 ? kx : kx + g.N1, l2 = ky >= 0 ? ky : ky + g.N2;
+                const double km = kmag3(kx_four(l1, g.N1) * g.kf[0], ky_of(l2, g.N2) * g.kf[1], kz * g.kf[2]);
+                if (km >= inner && km < outer) nf += (kz == 0) ? 1 : 2;
+            }
+    if (nf <= 0) {
+        set_error("force: no wavevector with %g <= |K| < %g", inner, outer);
+        return E_ARG;
+    }
+    b.N1 = g.N1; b.N2 = g.N2; b.N3 = g.N3; b.H = g.H;
+    b.f1 = g.kf[0]; b.f2 = g.kf[1]; b.f3 = g.kf[2];
+    b.proc = f->proc;
+    b.inner = inner; b.outer = outer;
+    b.per_mode = f->p3 / (double)nf;
+    b.p4 = f->p4;
+    b.nf = nf;
+    // Is_alias_array universal_misc.cc:169-181: the band reaches past N/3 on some axis
+    const int ax = (int)ceil(outer / g.kf[0]), ay = (int)ceil(outer / g.kf[1]), az = (int)ceil(outer / g.kf[2]);
+    b.dealias_force = g.dealias && (ax > g.N1 / 3 || ay > g.N2 / 3 || az > g.N3 / 3);
+    b.fullx = (2L * kxm + 1 >= g.N1);
+    b.fully = (2L * kym + 1 >= g.N2);
+    b.kxm = kxm; b.kym = kym;
+    b.nx = b.fullx ? g.N1 : 2 * kxm + 1;
+    b.ny = b.fully ? g.N2 : 2 * kym + 1;
+    b.nz = (kzm < g.H - 1 ? kzm : g.H - 1) + 1;
+    return OK;
+}
+
+template <typename R> int force_subtract(const ForceBand& b, const cplx<R>* const V[3], cplx<R>* const nl[3],
+                                         hipStream_t st)
+{
+    return launch_band<R, 1>(b, V, nl, st);
+}
+template int force_subtract<float>(const ForceBand&, const cplx<float>* const[3], cplx<float>* const[3], hipStream_t);
+template int force_subtract<double>(const ForceBand&, const cplx<double>* const[3], cplx<double>* const[3],
+                                    hipStream_t);
+
+}  // namespace th
+
+template <typename R> static int compute_force(th_plan* p, void* const V[3], void* F[3], const th_force* f, long* nf)
+{
+    ForceBand b;
+    TRY(force_setup(p->g, f, b));
+    hipStream_t st = p->ctx->stream;
+    const size_t bytes = p->g.ncplx() * sizeof(cplx<R>);
+    for (int c = 0; c < 3; c++) TH_CHECK_HIP(hipMemsetAsync(F[c], 0, bytes, st));
+    const cplx<R>* v[3] = {(const cplx<R>*)V[0], (const cplx<R>*)V[1], (const cplx<R>*)V[2]};
+    cplx<R>* o[3] = {(cplx<R>*)F[0], (cplx<R>*)F[1], (cplx<R>*)F[2]};
+    TRY((launch_band<R, 0>(b, v, o, st)));
+    if (nf) *nf = b.nf;
+    return OK;
+}
+
+extern "C" int th_compute_force(th_plan* p, void* const V[3], void* F[3], const th_force* f, long* nf)
+{
+    if (!p || !V || !F || !V[0] || !V[1] || !V[2] || !F[0] || !F[1] || !F[2]) {
+        set_error("th_compute_force: null");
+        return E_ARG;
+    }
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    return p->prec ? compute_force<float>(p, V, F, f, nf) : compute_force<double>(p, V, F, f, nf);
+}

@@ -107,12 +107,6 @@ struct th_state {
     }
 };
 
-#define TRY(x)             \
-    do {                   \
-        int r_ = (x);      \
-        if (r_) return r_; \
-    } while (0)
-
 static std::string tune_key(const th_state* s)
 {
     char k[160];

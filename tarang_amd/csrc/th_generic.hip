@@ -324,15 +324,7 @@ __device__ __forceinline__ int kx_g(const GeoK& g, int l1) { return g.basis == 1
 
 __device__ __forceinline__ bool kept_g(const GeoK& g, int l1, int l2, int l3)
 {
-    if (!g.dealias) return true;
-    int kx = kx_g(g, l1);
-    if (g.N1 >= 3) {
-        int ak = kx < 0 ? -kx : kx;
-        if (g.basis == 0 ? (ak >= g.N1 / 3) : (ak >= 2 * g.N1 / 3)) return false;
-    }
-    if (g.N2 >= 3 && l2 >= g.N2 / 3 && l2 <= 2 * g.N2 / 3 + 1) return false;
-    if (g.N3 >= 3 && l3 >= g.N3 / 3 && l3 <= g.N3 / 2) return false;
-    return true;
+    return kept_mode(g.basis, g.dealias, g.N1, g.N2, g.N3, l1, l2, l3);
 }
 
 __device__ __forceinline__ double ksqr_g(const GeoK& g, int l1, int l2, int l3)

@@ -334,9 +334,34 @@ void IncFluid::Output_flux()
     flux_file << std::endl;
 }
 
+void IncFluid::Set_force(int proc, double inner_radius, double outer_radius, double p3, double p4)
+{
+    th_force f{proc, inner_radius, outer_radius, p3, p4};
+    // validate now (th_compute_force's checks) so a bad parameter set fails here, not mid-run
+    DField F1(elems_, esize_), F2(elems_, esize_), F3(elems_, esize_);
+    void* const V[3] = {V1.data(), V2.data(), V3.data()};
+    void* F[3] = {F1.data(), F2.data(), F3.data()};
+    check(th_compute_force(plan_, V, F, &f, nullptr), "Set_force");
+    check(th_sync(plan_), "th_sync");
+    force_ = f;
+    forced_ = true;
+    if (state_) {
+        th_state_destroy(state_);
+        state_ = nullptr;
+    }
+    fused_ = false;
+}
+
 void IncFluid::Time_advance()
 {
     th_phys ph{dissipation_coefficient, 0, 0, 0, 0};
+    if (forced_) {
+        int sch = integ_scheme == "EULER" ? 0 : integ_scheme == "RK2" ? 1 : 2;
+        void* const V[3] = {V1.data(), V2.data(), V3.data()};
+        check(th_step_plain_forced(plan_, sch, V, Tdt, &ph, &force_), "th_step_plain_forced");
+        check(th_sync(plan_), "th_sync");
+        return;
+    }
     if (fused_) {
         void* V[3] = {V1.data(), V2.data(), V3.data()};
         for (int i = 0; i < 3; i++) check(th_state_upload(state_, i, V[i], 0), "th_state_upload");

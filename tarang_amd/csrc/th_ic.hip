@@ -164,12 +164,6 @@ dim3 grid_of(size_t n)
     return dim3((unsigned)(b ? b : 1));
 }
 
-#define TRY(x)             \
-    do {                   \
-        int r_ = (x);      \
-        if (r_) return r_; \
-    } while (0)
-
 template <typename R> int ic_random(th_plan* p, void* Vv[3], uint64_t seed, double E0, double kp)
 {
     const Geo& g = p->g;

@@ -8,6 +8,8 @@
 #include <vector>
 #include "th_common.hpp"
 
+struct th_force;
+
 struct th_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -30,6 +32,13 @@ namespace th {
 enum { OK = 0, E_ARG = -1, E_HIP = -2, E_UNSUPPORTED = -3, E_STATE = -4, E_NOMEM = -5, E_IO = -6 };
 
 void set_error(const char* fmt, ...);
+
+// propagate a nonzero status code
+#define TRY(x)             \
+    do {                   \
+        int r_ = (x);      \
+        if (r_) return r_; \
+    } while (0)
 
 #define TH_CHECK_HIP(expr)                                                                 \
     do {                                                                                   \
@@ -110,6 +119,27 @@ template <typename R> int reduce_helicity(const Geo& g, const cplx<R>* const V[3
 template <typename R> int reduce_entropy(const Geo& g, const cplx<R>* const V[3], double* out, RedScratch& rs, hipStream_t st);
 template <typename R> int reduce_shell_helicity(const Geo& g, const cplx<R>* const V[3], int nbins, double* H1, double* H2,
                                                 double* H3, RedScratch& rs, hipStream_t st);
+
+// ---------------- energy / helicity forcing: th_force.hip ----------------
+// The band inner < |K| <= outer of force_field_proc 1 / 2, its per-mode level and the bounding box of
+// wavevectors one launch covers.
+struct ForceBand {
+    int N1, N2, N3, H;
+    double f1, f2, f3;
+    int proc;                 // 1 energy/helicity supply, 2 constant energy/helicity
+    double inner, outer;
+    double per_mode;          // energy_supply / nf (proc 1) or energy_level / nf (proc 2)
+    double p4;                // epsh_by_k_epse (proc 1) or h_by_k_E (proc 2)
+    long nf;                  // modes in inner <= |K| < outer, conjugates included
+    int dealias_force;        // Is_alias_array: the band reaches past N/3 on some axis
+    int fullx, fully;         // the box spans every x (y) index
+    int kxm, kym;             // box origin: kx = tx - kxm, ky = ty - kym
+    int nx, ny, nz;           // box extents
+};
+int force_setup(const Geo& g, const th_force* f, ForceBand& b);
+// nl[c] -= F[c] over the band (Add_force after Compute_force)
+template <typename R> int force_subtract(const ForceBand& b, const cplx<R>* const V[3], cplx<R>* const nl[3],
+                                         hipStream_t st);
 
 // host-side binning helpers (four_inline.h, scft_inline.h)
 int min_radius_outside(const Geo& g);

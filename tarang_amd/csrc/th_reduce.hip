@@ -10,14 +10,6 @@
 #include <math.h>
 #include "th_internal.hpp"
 
-#ifndef TRY
-#define TRY(x)             \
-    do {                   \
-        int r_ = (x);      \
-        if (r_) return r_; \
-    } while (0)
-#endif
-
 namespace th {
 
 struct GeoR {

@@ -122,3 +122,20 @@ def test_driver_output_files_match_oracle(tmp_path):
     for r in frows:
         i, R, pi, force = r.split()
         assert close(float(pi), flux[float(R)], 1e-11) and float(force) == 0.0
+
+
+def test_driver_forced_run_matches_oracle():
+    """force=1,...: IncFluid::Set_force then Time_advance with the supply force recomputed at every stage
+    (Compute_force_TO_rhs single_time_step.cc:351-364); E(t) and E(k) follow the oracle's forced steps."""
+    N, steps, dt, nu = 16, 5, 1e-3, 0.01
+    spec, flux, glob = run_driver(N, steps, dt, nu, "tg", "FP64", "fused", "force=1,0.5,2.5,0.3,0.2")
+    o = Oracle((N, N, N))
+    V = o.ic_taylor_green(1, 1.0)
+    E = []
+    for _ in range(steps):
+        o.step_ns_forced(V, nu, dt, 1, 0.5, 2.5, 0.3, 0.2)
+        E.append(o.total_energy(V))
+    assert np.abs(glob[:, 1] - np.array(E)).max() <= 1e-12 * max(E)
+    Ek = o.vector_spectrum(V)
+    for k, v in spec.items():
+        assert abs(v - Ek[k]) <= 1e-12 * max(Ek)

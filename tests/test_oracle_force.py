@@ -1,0 +1,213 @@
+"""Pins the oracle's energy/helicity forcing (force_field_proc 1 and 2 on a FOUR box) with a brute-force
+mode count, the energy-injection identity, Beltrami modes (i K x V = s |K| V) and the modal energy and
+helicity of the forced field (CPU only).
+
+Reference: Compute_force_const_energy_helicity_supply compute_force_ek_hk_supply.cc:51-157,
+Compute_force_const_energy_helicity compute_force_const_ek_hk.cc:51-140, Force_alpha_beta
+force_alpha_beta.cc:60-89, Get_local_number_modes_in_shell_FOUR four_basic.cc:48-90."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from pyoracle import Oracle
+
+
+def _kgrid(o):
+    N1, N2, N3 = o.N
+    kx = np.array([o.kx(l) for l in range(N1)], float) * o.g.f1
+    ky = np.array([o.ky(l) for l in range(N2)], float) * o.g.f2
+    kz = np.arange(N3 // 2 + 1, dtype=float) * o.g.f3
+    return np.meshgrid(kx, ky, kz, indexing="ij")
+
+
+def _weights(o):
+    """Full-sphere weight of each stored mode: kz = 0 modes are stored with their conjugates, kz > 0
+    modes stand for two."""
+    w = np.full(o.shape, 2.0)
+    w[:, :, 0] = 1.0
+    return w
+
+
+def _project(o, V):
+    KX, KY, KZ = _kgrid(o)
+    K2 = KX * KX + KY * KY + KZ * KZ
+    K2[0, 0, 0] = 1.0
+    d = (KX * V[0] + KY * V[1] + KZ * V[2]) / K2
+    return [V[0] - KX * d, V[1] - KY * d, V[2] - KZ * d]
+
+
+def _real_coefficient_field(o, seed):
+    """Solenoidal field with real Fourier coefficients: Vr x Vi = 0, so every mode has zero helicity."""
+    rng = np.random.default_rng(seed)
+    V = [rng.standard_normal(o.shape) for _ in range(3)]
+    V = _project(o, V)
+    out = o.zeros(3)
+    for c in range(3):
+        out[c][...] = V[c]
+        o.satisfy_reality(out[c])
+        o.dealias(out[c])
+    return out
+
+
+def _curl(o, V):
+    KX, KY, KZ = _kgrid(o)
+    return [1j * (KY * V[2] - KZ * V[1]), 1j * (KZ * V[0] - KX * V[2]), 1j * (KX * V[1] - KY * V[0])]
+
+
+@pytest.mark.parametrize("kf,inner,outer", [((1, 1, 1), 0.0, 1.5), ((1, 1, 1), 1.0, 2.0), ((1, 1, 1), 2.5, 3.5),
+                                             ((1.0, 0.5, 2.0), 1.2, 2.7), ((0.7, 1.3, 1.0), 0.0, 3.0)])
+def test_mode_count_is_the_full_sphere_count(kf, inner, outer):
+    """nf counts every wavevector (conjugates included) with inner <= |K| < outer; brute force over Z^3."""
+    o = Oracle((32, 32, 32), kfactor=kf)
+    m = int(math.ceil(outer / min(kf))) + 1
+    n = 0
+    for a, b, c in itertools.product(range(-m, m + 1), repeat=3):
+        k = math.sqrt((a * kf[0]) ** 2 + (b * kf[1]) ** 2 + (c * kf[2]) ** 2)
+        n += inner <= k < outer
+    assert o.force_supply_modes(inner, outer) == n
+    if kf == (1, 1, 1) and (inner, outer) == (0.0, 1.5):
+        assert n == 19                    # |K|^2 in {0, 1, 2}: 1 + 6 + 12
+    if kf == (1, 1, 1) and (inner, outer) == (1.0, 2.0):
+        assert n == 26                    # |K|^2 in {1, 2, 3}: 6 + 12 + 8
+
+
+@pytest.mark.parametrize("r", [0.0, 0.5])
+def test_supply_injects_the_energy_supply(r):
+    """Proc 1 on a field whose modes carry no helicity: alpha = eps / (2 nf E_k), so sum_k Re F.V* = eps;
+    the vorticity term (beta = r alpha) adds helicity injection sum_k Re F.(iKxV)* = sum_k beta |K x V|^2
+    and no energy.  F vanishes outside inner < |K| <= outer."""
+    o = Oracle((16, 16, 16))
+    V = _real_coefficient_field(o, 5)
+    inner, outer, eps = 2.5, 3.5, 0.37
+    F = o.force(V, 1, inner, outer, eps, r)
+    KX, KY, KZ = _kgrid(o)
+    K = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    band = (K > inner) & (K <= outer)
+    w = _weights(o)
+    inj = sum((w * (F[c] * np.conj(V[c])).real).sum() for c in range(3))
+    assert abs(inj - eps) < 1e-13 * eps
+    for c in range(3):
+        assert np.abs(F[c][~band]).max() == 0.0
+    nf = o.force_supply_modes(inner, outer)
+    E = 0.5 * sum(np.abs(V[c]) ** 2 for c in range(3))
+    W = _curl(o, V)
+    for c in range(3):
+        ref = np.where(band, eps / nf / (2 * np.where(band, E, 1)) * (V[c] + r * W[c]), 0)
+        assert np.abs(F[c] - ref).max() < 1e-13 * np.abs(ref).max()
+    hinj = sum((w * (F[c] * np.conj(W[c])).real).sum() for c in range(3))
+    href = (w * np.where(band, eps / nf / (2 * np.where(band, E, 1)) * r, 0) *
+            sum(np.abs(W[c]) ** 2 for c in range(3))).sum()
+    assert abs(hinj - href) < 1e-12 * max(1.0, abs(href))
+
+
+def _helical_modes(o, A=0.6):
+    """A positive-helicity mode along z at kz = 3 and a negative-helicity mode along x at kx = +-3:
+    V = A (e1 + i s e2), e1 x e2 = K/|K|, so i K x V = s |K| V."""
+    V = o.zeros(3)
+    V[0][0, 0, 3], V[1][0, 0, 3] = A, 1j * A                     # s = +1, K = 3 z
+    V[1][3, 0, 0], V[2][3, 0, 0] = A, -1j * A                    # s = -1, K = 3 x
+    V[1][-3, 0, 0], V[2][-3, 0, 0] = A, 1j * A                   # its conjugate partner at -K
+    return V
+
+
+def test_beltrami_modes_take_the_degenerate_branches():
+    """|sk| = 1.  Proc 1: alpha = eps_k / (4 E), beta = alpha / (sk |K|); for s = +1 the vorticity term
+    doubles alpha V, for s = -1 beta < 0 and the vorticity is dropped (beta > MYEPS only).  Proc 2:
+    alpha = sqrt(e_k / E), beta = 0 for either sign."""
+    o = Oracle((16, 16, 16))
+    V = _helical_modes(o)
+    inner, outer, eps = 2.5, 3.5, 0.2
+    nf = o.force_supply_modes(inner, outer)
+    E = 0.6 ** 2                                                  # |V|^2 / 2
+    F = o.force(V, 1, inner, outer, eps, 0.0)
+    t = eps / nf / (2 * E)
+    for c in range(3):
+        assert np.allclose(F[c][0, 0, 3], t * V[c][0, 0, 3], rtol=1e-14, atol=0)
+        assert np.allclose(F[c][3, 0, 0], 0.5 * t * V[c][3, 0, 0], rtol=1e-14, atol=0)
+        assert np.allclose(F[c][-3, 0, 0], 0.5 * t * V[c][-3, 0, 0], rtol=1e-14, atol=0)
+    F2 = o.force(V, 2, inner, outer, eps, 0.3)
+    a = math.sqrt(eps / nf / E)
+    for c in range(3):
+        for idx in [(0, 0, 3), (3, 0, 0), (-3, 0, 0)]:
+            assert abs(F2[c][idx] - a * V[c][idx]) <= 1e-14 * abs(a * 0.6)
+    nz = sum(np.count_nonzero(F2[c]) for c in range(3))
+    assert nz == 6
+
+
+@pytest.mark.parametrize("h", [0.0, 0.4, 0.9])
+def test_constant_level_sets_modal_energy_and_relative_helicity(h):
+    """Proc 2 on modes with zero helicity: |F|^2 / 2 = e_k and K.(Fr x Fi) / (|K| |F|^2 / 2) = h on every
+    forced mode (alpha^2 + beta^2 K^2 = e_k / E, 2 alpha beta K^2 E = h |K| e_k)."""
+    o = Oracle((16, 16, 16))
+    V = _real_coefficient_field(o, 9)
+    inner, outer, e = 1.5, 3.2, 0.8
+    F = o.force(V, 2, inner, outer, e, h)
+    nf = o.force_supply_modes(inner, outer)
+    KX, KY, KZ = _kgrid(o)
+    K = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    band = (K > inner) & (K <= outer)
+    EF = 0.5 * sum(np.abs(F[c]) ** 2 for c in range(3))
+    assert np.abs(EF[band] - e / nf).max() < 1e-13 * e / nf
+    Fr, Fi = [F[c].real for c in range(3)], [F[c].imag for c in range(3)]
+    HF = (KX * (Fr[1] * Fi[2] - Fr[2] * Fi[1]) + KY * (Fr[2] * Fi[0] - Fr[0] * Fi[2]) +
+          KZ * (Fr[0] * Fi[1] - Fr[1] * Fi[0]))
+    rel = HF[band] / (K[band] * EF[band])
+    assert np.abs(rel - h).max() < 1e-12
+
+
+def test_constant_level_negative_helicity_drops_the_vorticity():
+    """Proc 2 with h < 0 gives beta < 0, which Force_alpha_beta ignores: F = alpha V with
+    alpha = sqrt(e_k / E) (sqrt(1 + h) + sqrt(1 - h)) / 2, so F carries no helicity."""
+    o = Oracle((16, 16, 16))
+    V = _real_coefficient_field(o, 11)
+    inner, outer, e, h = 1.5, 3.2, 0.8, -0.3
+    F = o.force(V, 2, inner, outer, e, h)
+    nf = o.force_supply_modes(inner, outer)
+    KX, KY, KZ = _kgrid(o)
+    K = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+    band = (K > inner) & (K <= outer)
+    E = 0.5 * sum(np.abs(V[c]) ** 2 for c in range(3))
+    alpha = np.sqrt(e / nf / np.where(band, E, 1)) * (math.sqrt(1 + h) + math.sqrt(1 - h)) / 2
+    for c in range(3):
+        ref = np.where(band, alpha * V[c], 0)
+        assert np.abs(F[c] - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+def test_zero_field_gets_no_force():
+    """Modes with modal energy <= MYEPS are not forced."""
+    o = Oracle((16, 16, 16))
+    F = o.force(o.zeros(3), 1, 1.5, 3.5, 1.0, 0.2)
+    assert all(np.abs(f).max() == 0.0 for f in F)
+    with pytest.raises(ValueError):
+        o.force(o.zeros(3), 1, 0.1, 0.5, 1.0, 0.0)                 # no wavevector in [0.1, 0.5)
+
+
+def test_forced_step_injects_energy_at_the_supply_rate():
+    """One IF-RK4 step with the supply force from a field whose modes carry no helicity (injection
+    exactly eps at t): E_forced(t+dt) - E_free(t+dt) = eps dt + O(dt^2), the remainder halving with dt."""
+    o = Oracle((16, 16, 16))
+    nu, eps = 0.01, 0.5
+    V0 = _real_coefficient_field(o, 4)
+    s = math.sqrt(0.5 / o.total_energy(V0))
+    V0 = [v * s for v in V0]
+    res = []
+    for dt in (2e-3, 1e-3):
+        Vf = o.step_ns_forced([v.copy() for v in V0], nu, dt, 1, 1.5, 3.5, eps, 0.0)
+        Vu = o.rk4_ns([v.copy() for v in V0], nu, dt)
+        res.append(abs((o.total_energy(Vf) - o.total_energy(Vu)) / dt - eps))
+    assert res[1] < 2e-4 * eps
+    assert 0.4 < res[1] / res[0] < 0.6
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+def test_forced_step_with_zero_supply_is_the_free_step(scheme):
+    """energy_supply = 0 gives F = 0, so the forced EULER / RK2 / RK4 steps equal the free ones bit for bit."""
+    o = Oracle((16, 16, 16))
+    V0 = o.ic_random(6, E0=0.5, kp=3.0)
+    Vf = o.step_ns_forced([v.copy() for v in V0], 0.01, 1e-3, 1, 1.5, 3.5, 0.0, 0.0, scheme=scheme)
+    Vu = [v.copy() for v in V0]
+    [o.euler_ns, o.rk2_ns, o.rk4_ns][scheme](Vu, 0.01, 1e-3)
+    for c in range(3):
+        assert np.array_equal(Vf[c], Vu[c])
