@@ -10,6 +10,7 @@ FP32-mixed run against the FP64 run after the same number of steps.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--no-cpu] [--no-fp64]
   python bench.py --workload ensemble [--steps K] [--warmup W] [--no-cpu]
   python bench.py --workload mhd [--n 512]      python bench.py --workload rb [--rb-n 256]
+  python bench.py --workload forced [--n 512]   (the N^3 step with energy-supply forcing)
 Prints one JSON line.  Runs on one GPU; --gpus > 1 exits with a message (single-GPU build).
 """
 import argparse
@@ -327,6 +328,52 @@ def bench_mhd(args):
          "cpu_baseline": None})), flush=True)
 
 
+def bench_forced(args):
+    """N^3 forced turbulence: the decaying-turbulence workload (seed 1, nu = 1e-3, dt from CFL 0.5) with the
+    energy-supply forcing of force_field_proc 1 on the band 1.5 < |K| <= 3.5 (energy_supply 0.1,
+    epsh_by_k_epse 0.2), FP32-mixed fused step, recomputed at every RK stage.  The unforced step of the
+    same state is timed beside it, so the line shows what the forcing costs."""
+    import torch
+    from tarang_amd import Context, Plan, State
+    n = args.n
+    N = (n, n, n)
+    nu = 1e-3
+    ctx = Context(0)
+    p64 = Plan(ctx, N, precision="fp64")
+    p32 = Plan(ctx, N, precision="fp32")
+    V = p64.ic_random(1, 0.5, 4.0)
+    dt, umax = cfl_dt(p64, V)
+    E0 = p64.total_energy(V)
+    s = State(p32)
+    s.upload([v.to(torch.complex64) for v in V])
+    del V
+    var = s.tune(dt, nu=nu, nrep=3) if not args.no_tune else s.variants
+    res = {}
+    for name, force in (("unforced", None), ("forced", (1, 1.5, 3.5, 0.1, 0.2))):
+        s.set_force(*force) if force else s.set_force(0)
+        s.step(dt, nu=nu, steps=args.warmup)
+        torch.cuda.synchronize()
+        res[name] = time_steps(s, args.steps, dt, nu)
+        log(f"bench: {name} {args.steps} steps {res[name]:.4f} s")
+    E1 = p32.total_energy(s.download())
+    pts = float(n) ** 3
+    value = pts * 4 * args.steps / res["forced"] / 1e6
+    print(json.dumps(_line(
+        f"Mpts*RKstage/s at {n}^3 forced turbulence (energy supply), FP32-mixed fused step", value, args,
+        1e3 * res["forced"] / args.steps,
+        {"workload": f"ns_{n}^3_rk4_forced_supply", "N": n, "nu": nu, "dt": dt, "cfl": 0.5,
+         "force": {"force_field_proc": 1, "inner_radius": 1.5, "outer_radius": 3.5, "energy_supply": 0.1,
+                   "epsh_by_k_epse": 0.2},
+         "ic": "seed 1, k^4 exp(-2(k/4)^2), E=0.5", "precision": "fp32-mixed"},
+        {"dtype": "fp32-mixed (fp32 storage/FFT; fp64 integrating factor, twiddles, sums, force)",
+         "data": "synthetic (seeded random-phase IC generated on the GPU)",
+         "ic_check": {"E0": E0, "umax": umax}, "energy_after": E1, "launch_variants": var,
+         "unforced": {"value": round(pts * 4 * args.steps / res["unforced"] / 1e6, 1),
+                      "ms_per_step": round(1e3 * res["unforced"] / args.steps, 3)},
+         "forcing_overhead_ms_per_step": round(1e3 * (res["forced"] - res["unforced"]) / args.steps, 4),
+         "roofline": None, "cpu_baseline": None})), flush=True)
+
+
 def bench_rb(args):
     """Rayleigh-Benard, SCFT basis with the sin/cos axis first, N^3 (default 256^3), kfactor
     (pi, pi/sqrt2, pi/sqrt2), Pr = 1, Ra = 1e7 (PRLARGE/ULARGE scaling), V = 0, temperature
@@ -396,9 +443,10 @@ def bench_rb(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["ns512", "ensemble", "mhd", "rb"], default="ns512",
+    ap.add_argument("--workload", choices=["ns512", "ensemble", "mhd", "rb", "forced"], default="ns512",
                     help="ns512: the N^3 decaying-turbulence step (default, N from --n); "
-                         "ensemble: the uneven 64-box ensemble; mhd: N^3 MHD; rb: Rayleigh-Benard (SCFT)")
+                         "ensemble: the uneven 64-box ensemble; mhd: N^3 MHD; rb: Rayleigh-Benard (SCFT); "
+                         "forced: the N^3 step with energy-supply forcing")
     ap.add_argument("--rb-n", type=int, default=256)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -424,6 +472,9 @@ def main():
         return
     if args.workload == "rb":
         bench_rb(args)
+        return
+    if args.workload == "forced":
+        bench_forced(args)
         return
 
     import numpy as np

@@ -202,6 +202,11 @@ int th_state_download(th_state *state, int comp, void *dst, int dst_on_host);
  * 2 RK4 accumulator, each in the reference layout. */
 int th_state_download_buffer(th_state *state, int which, int comp, void *dst, int dst_on_host);
 int th_step(th_plan *plan, th_state *state, double dt, const th_phys *p, int nsteps);
+/* Forcing of later th_step calls (force_field_proc 1 / 2 recomputed from every stage field, as
+ * Compute_force_TO_rhs single_time_step.cc:351-364); NULL or proc 0 switches it off.  Single-box NS
+ * states; TH_E_UNSUPPORTED when the band reaches a mode the 2/3-dealiased state does not store (then
+ * use th_step_plain_forced), otherwise the errors of th_compute_force. */
+int th_state_set_force(th_state *state, const th_force *force);
 /* Pi(R_i) for spheres radii[1..nsh] and T(n,m) for giver shells n = 1..nsh-1 over the state's
  * velocity (Compute_flux + Compute_shell_tr), from one giver-masked fused pipeline per shell. */
 int th_flux_shell_tr(th_plan *plan, th_state *state, const double *radii, int nsh, double *Pi, double *T);

@@ -118,6 +118,7 @@ def lib():
         "th_state_upload": (i, [vp, i, vp, i]),
         "th_state_download": (i, [vp, i, vp, i]),
         "th_step": (i, [vp, vp, d, C.POINTER(Phys), i]),
+        "th_state_set_force": (i, [vp, C.POINTER(Force)]),
         "th_state_download_buffer": (i, [vp, i, i, vp, i]),
         "th_flux_shell_tr": (i, [vp, vp, dp, i, dp, dp]),
         "th_profile_passes": (i, [vp, vp, d, C.POINTER(Phys), i, i, dp, dp]),
@@ -600,6 +601,11 @@ class State:
         ph = Phys(nu, eta, kappa, Ra, Pr)
         self.plan._pre()
         self.plan._post(lib().th_step(self.plan.h, self.h, dt, C.byref(ph), steps), "th_step")
+
+    def set_force(self, proc=0, inner=0.0, outer=0.0, p3=0.0, p4=0.0):
+        """Forcing (force_field_proc 1 / 2) of later steps; proc 0 switches it off."""
+        fc = Force(proc, inner, outer, p3, p4)
+        _check(lib().th_state_set_force(self.h, C.byref(fc)), "th_state_set_force")
 
     def profile_passes(self, dt, nu=0.0, eta=0.0, stage=2, nrep=5, kappa=0.0, Ra=0.0, Pr=0.0):
         import numpy as np

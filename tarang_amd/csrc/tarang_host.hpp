@@ -136,8 +136,9 @@ public:
     void Output_field(const std::string& path);      // output_field.cc:43-58 (appends a snapshot at Tnow)
     // force_field_proc 1 (energy/helicity supply) or 2 (constant energy/helicity) with
     // force_double_para (inner_radius, outer_radius, p3, p4) (IncFluid.cc:201-217).  Later steps recompute
-    // the force at every stage (Compute_force_TO_rhs single_time_step.cc:351-364) on the plain stepper;
-    // a fused state is released.
+    // the force at every stage (Compute_force_TO_rhs single_time_step.cc:351-364): on the fused step when
+    // every forced mode is one the dealiased state stores, else on the plain stepper (the fused state is
+    // then released).
     void Set_force(int force_field_proc, double inner_radius, double outer_radius, double p3, double p4);
     // Init_cond() for field_input_proc = 1, init_cond_field.cc:47-86: read V from the file, then
     // Satisfy_reality_condition_field when asked, and Dealias; returns the snapshot's time (NaN if none)

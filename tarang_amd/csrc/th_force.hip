@@ -17,78 +17,11 @@
 #include "../../include/tarang_hip.h"
 #include "th_internal.hpp"
 #include "th_plan.hpp"
+#include "th_force.hpp"
 
 using namespace th;
 
 namespace {
-
-constexpr double MYEPS = 1e-15;    // extern_def_vars.h:81
-constexpr double MYEPS2 = 1e-5;    // extern_def_vars.h:82
-
-__host__ __device__ __forceinline__ double kmag3(double a, double b, double c)
-{
-#pragma clang fp contract(off)
-    return sqrt(a * a + b * b + c * c);
-}
-
-// force of one stored mode (FP64), before the reality condition
-template <typename R>
-__device__ void mode_force(const ForceBand& b, const cplx<R>* const* V, int lx, int ly, int lz, double fr[3],
-                           double fi[3])
-{
-    fr[0] = fr[1] = fr[2] = fi[0] = fi[1] = fi[2] = 0.0;
-    const double kk[3] = {kx_four(lx, b.N1) * b.f1, ky_of(ly, b.N2) * b.f2, lz * b.f3};
-    const double km = kmag3(kk[0], kk[1], kk[2]);
-    if (!(km > b.inner && km <= b.outer)) return;
-    if (b.proc == 2 && !(b.per_mode > MYEPS)) return;
-    if (b.dealias_force && !kept_mode(0, 1, b.N1, b.N2, b.N3, lx, ly, lz)) return;
-    const long i = ((long)lx * b.N2 + ly) * b.H + lz;
-    double vr[3], vi[3], E = 0.0;
-    for (int c = 0; c < 3; c++) {
-        const cplx<R> v = V[c][i];
-        vr[c] = (double)v.x;
-        vi[c] = (double)v.y;
-        E += vr[c] * vr[c] + vi[c] * vi[c];
-    }
-    E = E / 2;                                               // CV_Modal_energy Cvf.cc:595-600
-    if (!(E > MYEPS)) return;
-    const double hel = kk[0] * (vr[1] * vi[2] - vr[2] * vi[1]) + kk[1] * (vr[2] * vi[0] - vr[0] * vi[2]) +
-                       kk[2] * (vr[0] * vi[1] - vr[1] * vi[0]);       // Get_Modal_helicity_FOUR
-    const double sk = hel / (km * E);
-    double alpha, beta;
-    if (b.proc == 1) {
-        const double temp = b.per_mode / (2 * E);
-        if (fabs(sk * sk - 1) > MYEPS2) {
-            alpha = temp * (1 - sk * b.p4) / (1 - sk * sk);
-            beta = temp * (b.p4 - sk) / (1 - sk * sk);
-        } else {
-            alpha = temp / 2;
-            beta = alpha / (sk * km);
-        }
-    } else {
-        const double temp1 = sqrt(b.per_mode / E);
-        if (fabs(sk * sk - 1) > MYEPS2) {
-            const double temp2 = sqrt((1 + b.p4) / (1 + sk)), temp3 = sqrt((1 - b.p4) / (1 - sk));
-            alpha = (temp1 / 2) * (temp2 + temp3);
-            beta = (temp1 / (2 * km)) * (temp2 - temp3);
-        } else {
-            alpha = temp1;
-            beta = 0.0;
-        }
-    }
-    for (int c = 0; c < 3; c++) {
-        fr[c] = alpha * vr[c];
-        fi[c] = alpha * vi[c];
-    }
-    if (beta > MYEPS) {                                      // vorticity i K x V only for beta > MYEPS
-        for (int c = 0; c < 3; c++) {
-            const int a1 = (c + 1) % 3, a2 = (c + 2) % 3;
-            const double cr = kk[a1] * vr[a2] - kk[a2] * vr[a1], ci = kk[a1] * vi[a2] - kk[a2] * vi[a1];
-            fr[c] += beta * -ci;
-            fi[c] += beta * cr;
-        }
-    }
-}
 
 // MODE 0: Out = F on the box (the array was cleared); MODE 1: Out -= F
 template <typename R, int MODE>
@@ -98,40 +31,13 @@ __global__ void k_force_band(ForceBand b, const cplx<R>* V0, const cplx<R>* V1, 
     const long n = (long)b.nx * b.ny * b.nz;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const int tz = (int)(t % b.nz);
-    const long r = t / b.nz;
-    const int ty = (int)(r % b.ny), tx = (int)(r / b.ny);
-    int lx, ly;
-    if (b.fullx) {
-        lx = tx;
-    } else {
-        const int kx = tx - b.kxm;
-        lx = kx >= 0 ? kx : kx + b.N1;                       // Get_lx_FOUR four_inline.h:103-106
-    }
-    if (b.fully) {
-        ly = ty;
-    } else {
-        const int ky = ty - b.kym;
-        ly = ky >= 0 ? ky : ky + b.N2;                       // Get_ly3D_FOUR four_inline.h:138
-    }
-    const int lz = tz;
-    // Satisfy_reality_array on the kz = 0 and kz = N3/2 planes: overwritten modes take conj(F(source))
-    int sx = lx, sy = ly;
-    bool mirror = false;
-    if (lz == 0 || lz == b.N3 / 2) {
-        const int kx = kx_four(lx, b.N1);
-        if (ly > b.N2 / 2) {
-            sx = (-kx >= 0) ? -kx : -kx + b.N1;
-            sy = -ky_of(ly, b.N2);
-            mirror = true;
-        } else if (ly == 0 && kx < 0) {
-            sx = -kx;
-            mirror = true;
-        }
-    }
-    const cplx<R>* V[3] = {V0, V1, V2};
+    int lx, ly, lz, sx, sy;
+    band_mode(b, t, lx, ly, lz);
+    const bool mirror = reality_source(b, lx, ly, lz, sx, sy);
+    const long si = ((long)sx * b.N2 + sy) * b.H + lz;
+    const cplx<R> v[3] = {V0[si], V1[si], V2[si]};
     double fr[3], fi[3];
-    mode_force<R>(b, V, sx, sy, lz, fr, fi);
+    mode_force<R>(b, v, sx, sy, lz, fr, fi);
     if (mirror)
         for (int c = 0; c < 3; c++) fi[c] = -fi[c];
     const long i = ((long)lx * b.N2 + ly) * b.H + lz;

@@ -1,12 +1,13 @@
 // th_fused.hip -- host side of the dealias-pruned, fused pseudo-spectral RK stage (FOUR basis):
 // device-resident states (single boxes and batched boxes of one size), the hipGraph-captured RK4
 // step, per-box stage coefficients, energy transfer from shell givers, the per-pass profiler, the
-// launch-shape autotuner and the uneven-box ensemble.  The five passes and their kernels are
-// described in th_fused_kern.hpp.
+// launch-shape autotuner, the uneven-box ensemble and the forced step.  The five passes and their
+// kernels are described in th_fused_kern.hpp.
 #include <stdio.h>
 #include <array>
 #include <string>
 #include <vector>
+#include "th_force.hpp"
 #include "th_fused.hpp"
 
 namespace th {
@@ -92,12 +93,17 @@ struct th_state {
     hipEvent_t cev = nullptr;    // recorded after the last upload from hcoef
     bool cev_live = false;
     int var[NPASS] = {0, 0, 0, 0, 0};   // launch-shape variant per pass (autotuner)
+    bool forced = false;         // energy / helicity forcing (th_state_set_force)
+    ForceBand fb{};
+    double* fbox = nullptr;      // projected force P(F) of the stage input, 3 complex FP64 per band-box mode
+    size_t fbox_cap = 0;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     ~th_state()
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (fbox) (void)hipFree(fbox);
         for (void* q : {S, Vs, acc, X1, X2, X3}) if (q) (void)hipFree(q);
         if (partial) (void)hipFree(partial);
         if (radii) (void)hipFree(radii);
@@ -361,11 +367,165 @@ static void diss_of(const th_state* s, const th_phys* ph, double* d0, double* d1
     }
 }
 
-// One RK4 step of every box in s, as 4 stages x 5 pass launches on stream st.
+// ------------------------------------------------------------------------------------------
+// Forced step (force_field_proc 1 / 2, Compute_force_TO_rhs single_time_step.cc:351-364).  The
+// reference subtracts the force F of the stage field from nlin before the projection, so the stage's
+// rhs gains +P(F), P the projection.  Every stage output (the stage field, the accumulator, the state
+// in stage 4) is linear in the rhs and F lives on a few thousand band modes, so instead of changing
+// the x-forward epilogue, k_force_pre stores P(F) of the stage input for the band box before the
+// stage's passes, and k_force_post adds the stage's share of it afterwards:
+//   stage 1: Vs += dt/2 e_h P(F), acc += dt/6 e_1 P(F)      stage 2: Vs += dt/2 P(F), acc += dt/3 e_h P(F)
+//   stage 3: Vs += dt e_h P(F),   acc += dt/3 e_h P(F)      stage 4: S += dt/6 P(F)
+// with e_h = exp(-nu dt K^2 / 2), e_1 = exp(-nu dt K^2) (the integrating factors of rk_store).
+// ------------------------------------------------------------------------------------------
+__host__ __device__ __forceinline__ bool fused_stored(const FG& g, int lx, int ly, int lz)
+{
+    return lz < g.nkz && lkept(g.mx, lx) && lkept(g.my, ly);
+}
+
+template <typename R>
+__global__ void k_force_pre(ForceBand b, FG g, const cplx<R>* __restrict__ src, double* __restrict__ fbox)
+{
+    const long n = (long)b.nx * b.ny * b.nz;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int lx, ly, lz, sx, sy;
+    band_mode(b, t, lx, ly, lz);
+    const bool mirror = reality_source(b, lx, ly, lz, sx, sy);
+    cplx<R> v[3] = {czero<R>(), czero<R>(), czero<R>()};
+    if (fused_stored(g, sx, sy, lz)) {
+        const long i = (long)lq(g.mx, sx) * g.sx + (long)lq(g.my, sy) * g.KZP + lz;
+        for (int c = 0; c < 3; c++) v[c] = src[c * g.cS + i];
+    }
+    double fr[3], fi[3];
+    mode_force<R>(b, v, sx, sy, lz, fr, fi);
+    if (mirror)
+        for (int c = 0; c < 3; c++) fi[c] = -fi[c];
+    const double K[3] = {kx_four(lx, g.N1) * g.f1, ky_of(ly, g.N2) * g.f2, lz * g.f3};
+    const double k2 = ksq_f(K[0], K[1], K[2]);
+    if (k2 > 0) {
+        const double dr = (K[0] * fr[0] + K[1] * fr[1] + K[2] * fr[2]) / k2;
+        const double di = (K[0] * fi[0] + K[1] * fi[1] + K[2] * fi[2]) / k2;
+        for (int c = 0; c < 3; c++) {
+            fr[c] -= K[c] * dr;
+            fi[c] -= K[c] * di;
+        }
+    }
+    for (int c = 0; c < 3; c++) {
+        fbox[6 * t + 2 * c] = fr[c];
+        fbox[6 * t + 2 * c + 1] = fi[c];
+    }
+}
+
+template <typename R>
+__global__ void k_force_post(ForceBand b, FG g, const StageCoef* __restrict__ coef, cplx<R>* S, cplx<R>* Vs,
+                             cplx<R>* acc, const double* __restrict__ fbox)
+{
+    const long n = (long)b.nx * b.ny * b.nz;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int lx, ly, lz;
+    band_mode(b, t, lx, ly, lz);
+    if (!fused_stored(g, lx, ly, lz)) return;
+    double f[6];
+    bool any = false;
+    for (int j = 0; j < 6; j++) {
+        f[j] = fbox[6 * t + j];
+        any = any || f[j] != 0.0;
+    }
+    if (!any) return;
+    const StageCoef sc = *coef;
+    const double kx = kx_four(lx, g.N1) * g.f1, ky = ky_of(ly, g.N2) * g.f2, kz = lz * g.f3;
+    const double kx2 = ksq_f(kx, 0.0, 0.0), kyz2 = ksq_f(0.0, ky, kz);
+    const double eh = exp(sc.fh[0] * kx2) * exp(sc.fh[0] * kyz2), e1 = exp(sc.f1[0] * kx2) * exp(sc.f1[0] * kyz2);
+    double cv = 0.0, ca = 0.0;
+    if (sc.stage == 1) { cv = 0.5 * sc.dt * eh; ca = sc.dt / 6 * e1; }
+    else if (sc.stage == 2) { cv = 0.5 * sc.dt; ca = sc.dt / 3 * eh; }
+    else if (sc.stage == 3) { cv = sc.dt * eh; ca = sc.dt / 3 * eh; }
+    const long i = (long)lq(g.mx, lx) * g.sx + (long)lq(g.my, ly) * g.KZP + lz;
+    for (int c = 0; c < 3; c++) {
+        const long j = c * g.cS + i;
+        if (sc.stage < 4) {
+            const cplx<R> v = Vs[j], a = acc[j];
+            Vs[j] = mk<R>((R)((double)v.x + cv * f[2 * c]), (R)((double)v.y + cv * f[2 * c + 1]));
+            acc[j] = mk<R>((R)((double)a.x + ca * f[2 * c]), (R)((double)a.y + ca * f[2 * c + 1]));
+        } else {
+            const cplx<R> v = S[j];
+            const double c4 = sc.dt / 6;
+            S[j] = mk<R>((R)((double)v.x + c4 * f[2 * c]), (R)((double)v.y + c4 * f[2 * c + 1]));
+        }
+    }
+}
+
+// One RK4 step of every box in s, as 4 stages x 5 pass launches on stream st (forced: plus the
+// two band kernels around each stage).
 template <typename R> static int enqueue_step(th_state* s, hipStream_t st)
 {
-    for (int stage = 1; stage <= 4; stage++)
+    const long nb = (long)s->fb.nx * s->fb.ny * s->fb.nz;
+    const dim3 fgrid((unsigned)((nb + 255) / 256));
+    for (int stage = 1; stage <= 4; stage++) {
+        if (s->forced) {
+            const cplx<R>* src = (const cplx<R>*)(stage == 1 ? s->S : s->Vs);
+            hipLaunchKernelGGL((k_force_pre<R>), fgrid, dim3(256), 0, st, s->fb, s->fg, src, s->fbox);
+            TH_CHECK_HIP(hipGetLastError());
+        }
         for (int k = 0; k < NPASS; k++) TRY(run_pass<R>(s, k, s->var[k], stage, st));
+        if (s->forced) {
+            hipLaunchKernelGGL((k_force_post<R>), fgrid, dim3(256), 0, st, s->fb, s->fg,
+                               (const StageCoef*)(s->dcoef + (size_t)(stage - 1) * s->nbox), (cplx<R>*)s->S,
+                               (cplx<R>*)s->Vs, (cplx<R>*)s->acc, (const double*)s->fbox);
+            TH_CHECK_HIP(hipGetLastError());
+        }
+    }
+    return OK;
+}
+
+extern "C" int th_state_set_force(th_state* s, const th_force* f)
+{
+    if (!s) { set_error("th_state_set_force: null state"); return E_ARG; }
+    TH_CHECK_HIP(hipSetDevice(s->plan->ctx->device));
+    if (!f || f->proc == 0) {
+        s->forced = false;
+    } else {
+        if (s->physics != TH_NS || s->nbox != 1) {
+            set_error("th_state_set_force: forcing runs on single-box NS states");
+            return E_UNSUPPORTED;
+        }
+        ForceBand fb;
+        TRY(force_setup(s->plan->g, f, fb));
+        // every mode that can carry a force must be stored by the 2/3-dealiased state
+        const long nb = (long)fb.nx * fb.ny * fb.nz;
+        for (long t = 0; t < nb; t++) {
+            int lx, ly, lz, sx, sy;
+            band_mode(fb, t, lx, ly, lz);
+            (void)reality_source(fb, lx, ly, lz, sx, sy);
+            const double km = kmag3(kx_four(sx, fb.N1) * fb.f1, ky_of(sy, fb.N2) * fb.f2, lz * fb.f3);
+            const bool can = km > fb.inner && km <= fb.outer &&
+                             (!fb.dealias_force || kept_mode(0, 1, fb.N1, fb.N2, fb.N3, sx, sy, lz));
+            if (can && !fused_stored(s->fg, lx, ly, lz)) {
+                set_error("th_state_set_force: the band reaches mode (%d, %d, %d), which the 2/3-dealiased fused "
+                          "state does not store; use th_step_plain_forced", kx_four(lx, fb.N1), ky_of(ly, fb.N2), lz);
+                return E_UNSUPPORTED;
+            }
+        }
+        const size_t need = (size_t)nb * 6;
+        if (need > s->fbox_cap) {
+            if (s->fbox) TH_CHECK_HIP(hipFree(s->fbox));
+            s->fbox = nullptr;
+            s->fbox_cap = 0;
+            if (hipMalloc((void**)&s->fbox, need * sizeof(double)) != hipSuccess) {
+                s->fbox = nullptr;
+                set_error("th_state_set_force: out of device memory");
+                return E_NOMEM;
+            }
+            s->fbox_cap = need;
+        }
+        s->fb = fb;
+        s->forced = true;
+    }
+    // the captured step changes
+    if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+    if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
     return OK;
 }
 

@@ -337,6 +337,16 @@ void IncFluid::Output_flux()
 void IncFluid::Set_force(int proc, double inner_radius, double outer_radius, double p3, double p4)
 {
     th_force f{proc, inner_radius, outer_radius, p3, p4};
+    if (fused_ && state_) {
+        const int r = th_state_set_force(state_, &f);
+        if (r == TH_OK) {
+            force_ = f;
+            forced_ = false;           // the fused step carries the force itself
+            return;
+        }
+        if (r != TH_E_UNSUPPORTED) check(r, "Set_force");
+        // the band reaches modes the fused state does not store: run on the plain stepper
+    }
     // validate now (th_compute_force's checks) so a bad parameter set fails here, not mid-run
     DField F1(elems_, esize_), F2(elems_, esize_), F3(elems_, esize_);
     void* const V[3] = {V1.data(), V2.data(), V3.data()};

@@ -132,3 +132,79 @@ def test_force_argument_errors(ctx):
         p.compute_force(dV, 2, 1.5, 3.5, 1.0, 1.5)
     with pytest.raises(RuntimeError, match="FOUR"):
         Plan(ctx, (16, 16, 16), basis="SCFT").compute_force(dV, 1, 1.5, 3.5, 1.0)
+
+
+@pytest.mark.parametrize("N,kf,inner,outer", [((16, 16, 16), (1.0, 1.0, 1.0), 1.5, 3.5),
+                                              ((32, 32, 32), (1.0, 1.0, 1.0), 0.5, 2.5),
+                                              ((48, 32, 24), (1.0, 1.0, 1.0), 2.0, 4.0),
+                                              ((16, 16, 16), (1.0, 1.0, 1.0), 3.0, 6.5)])
+@pytest.mark.parametrize("proc,p3,p4", [(1, 0.5, 0.3), (2, 0.6, 0.2)])
+def test_fused_forced_step_matches_oracle(ctx, N, kf, inner, outer, proc, p3, p4):
+    """th_state_set_force + th_step: the band kernels around each fused stage give the oracle's forced
+    RK4 step (force of each stage field, projected, into the rhs).  The last band reaches past N/3
+    (Dealias_force)."""
+    from tarang_amd import Plan, State
+    log(f"fused forced N={N} band=({inner},{outer}] proc={proc}")
+    o = Oracle(N, kfactor=kf)
+    p = Plan(ctx, N, kfactor=kf)
+    s = State(p)
+    V = o.ic_random(8, E0=0.5, kp=3.0)
+    s.upload([p.to_dev(v) for v in V])
+    s.set_force(proc, inner, outer, p3, p4)
+    nu, dt = 0.01, 1e-3
+    o.step_ns_forced(V, nu, dt, proc, inner, outer, p3, p4, steps=3)
+    s.step(dt, nu=nu, steps=3)
+    out = s.download()
+    for i in range(3):
+        assert rel(H(out[i]), V[i]) < 1e-12
+
+
+def test_fused_forced_fp32_mixed_and_switch_off(ctx):
+    """FP32-mixed fused forced step vs the oracle from the same rounded start (<= 1e-5); with the
+    force switched off again the state steps exactly as a never-forced state; energy_supply = 0
+    leaves the step bit-identical to the unforced one."""
+    from tarang_amd import Plan, State
+    log("fused forced fp32 + off")
+    N = (32, 32, 32)
+    o = Oracle(N)
+    p = Plan(ctx, N, precision="fp32")
+    V = o.ic_random(9, E0=0.5, kp=3.0)
+    d = [p.to_dev(v) for v in V]
+    s = State(p)
+    s.upload(d)
+    V = [H(x) for x in d]
+    s.set_force(1, 1.5, 3.5, 0.5, 0.0)
+    o.step_ns_forced(V, 0.01, 1e-3, 1, 1.5, 3.5, 0.5, 0.0, steps=2)
+    s.step(1e-3, nu=0.01, steps=2)
+    out = s.download()
+    for i in range(3):
+        assert rel(H(out[i]), V[i]) < 1e-5
+    a, b = State(p), State(p)
+    a.upload(d)
+    b.upload(d)
+    a.set_force(1, 1.5, 3.5, 0.5, 0.0)
+    a.set_force(0)
+    a.step(1e-3, nu=0.01, steps=2)
+    b.step(1e-3, nu=0.01, steps=2)
+    c = State(p)
+    c.upload(d)
+    c.set_force(1, 1.5, 3.5, 0.0, 0.0)
+    c.step(1e-3, nu=0.01, steps=2)
+    oa, ob, oc = a.download(), b.download(), c.download()
+    for i in range(3):
+        assert np.array_equal(H(oa[i]), H(ob[i]))
+        assert np.array_equal(H(oc[i]), H(ob[i]))
+
+
+def test_fused_force_band_outside_the_stored_modes(ctx):
+    """A band whose modes the dealiased state does not store (|kx| = N/3 without Dealias_force) is refused
+    by the fused state; the plain stepper runs it."""
+    from tarang_amd import Plan, State
+    log("fused force refused band")
+    p = Plan(ctx, (16, 16, 16))
+    s = State(p)
+    with pytest.raises(RuntimeError, match="does not store"):
+        s.set_force(1, 4.0, 5.0, 0.5, 0.0)
+    with pytest.raises(RuntimeError, match="single-box NS"):
+        from tarang_amd import abi
+        State(p, abi.TH_MHD).set_force(1, 1.5, 3.5, 0.5, 0.0)
