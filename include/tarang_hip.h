@@ -149,6 +149,19 @@ int th_Sn(th_plan *plan, const void *A, double n, double *out);
 int th_shell_spectrum(th_plan *plan, const void *A, const void *B, int n, double *Sk, double *count);
 int th_helicity(th_plan *plan, void *const V[3], double *out);
 int th_shell_spectrum_helicity(th_plan *plan, void *const V[3], double *H1k1, double *H1k2, double *H1k3);
+/* Anisotropic spectra of V about the axis globalvar_anisotropy_switch = axis (1 x, 2 y, 3 z), FOUR 3-D:
+ * th_ring_spectrum == Compute_ring_spectrum_FOUR four_energy.cc:828-992: S1k, S2k of (Max_radius_inside + 1) x (nsect + 1) doubles, row-major; nsect sectors
+ *   spaced linearly over [0, pi/2] (sector_spectrum_input_scheme 0, Cvf.cc:85-106); weight |K|^n, n = 0, 2, 4.
+ *   S1 is the toroidal part |(V x K)_axis|^2 / K_perp^2.  On-axis modes follow the reference: axis 1 adds
+ *   |V_y|^2, |V_z|^2; axis 2 adds |V_z|^2, |V_y|^2; axis 3 adds nothing (four_energy.cc:883-897).
+ * th_cylinder_ring_spectrum == Compute_cylinder_ring_spectrum_FOUR four_energy.cc:1450-1547: S1k, S2k of
+ *   th_cylinder_shell_ek_size(plan, axis) x (nslab + 1), rings ceil(K_perp), nslab slabs of |K_pll| over
+ *   [0, Anis_max_Kpll] (Cvf.cc:166-190); S1 = |V_axis|^2 (0 for axis 3, four_energy.cc:1493-1500).
+ * Bin indices past the last column land on the next row's column 0, as the reference's unchecked
+ * stores do.  rows x (bins + 1) is limited to 4096.  Synchronises. */
+int th_ring_spectrum(th_plan *plan, void *const V[3], int nsect, int n, int axis, double *S1k, double *S2k);
+int th_cylinder_ring_spectrum(th_plan *plan, void *const V[3], int nslab, int n, int axis, double *S1k, double *S2k);
+int th_cylinder_shell_ek_size(const th_plan *plan, int axis);   /* Anis_max_Krho_radius_inside + 1 */
 int th_entropy(th_plan *plan, void *const V[3], double *out);
 int th_fill_array_shell(th_plan *plan, const void *A, void *B, double inner, double outer);
 int th_shell_mult_single(th_plan *plan, void *const A[3], void *const B[3], double inner, double outer, double *out);

@@ -1499,6 +1499,144 @@ static void vr_cross_vi(double *const V[3], size_t i, double c[3])
     c[2] = r0 * m1 - r1 * m0;
 }
 
+/* ------------------------------------------------------------------------------------------
+ * Anisotropic spectra (FOUR, 3-D), axis a = globalvar_anisotropy_switch (1, 2, 3):
+ *   AnisKpll_FOUR / AnisKperp_FOUR four_inline.h:382-412, Get_polar_angle basis_basicfn_inline.h:89-112,
+ *   Get_sector_index basis_basicfn.cc:304-322 (sectors i dtheta, dtheta = (pi/2)/nsect, Cvf.cc:85-106),
+ *   Get_slab_index basis_basicfn.cc:373-390 (slabs i dkpll up to Anis_max_Kpll four_inline.h:484-514,
+ *   Cvf.cc:166-190), Anis_max_Krho_radius_inside_FOUR four_inline.h:517-546.
+ * S arrays are 2-D row-major (rows x (nbin + 1)); an index past the last column lands, as in the
+ * reference's unchecked Blitz store, on the next row's column 0, and past the array end it is dropped.
+ * ------------------------------------------------------------------------------------------ */
+static double aniso_kpll(const orc_grid *g, int a, int l1, int l2, int l3)
+{
+    return a == 1 ? orc_kx(g, l1) * g->f1 : a == 2 ? orc_ky(g, l2) * g->f2 : l3 * g->f3;
+}
+static double aniso_kperp(const orc_grid *g, int a, int l1, int l2, int l3)
+{
+    double x = orc_kx(g, l1) * g->f1, y = orc_ky(g, l2) * g->f2, z = l3 * g->f3;
+    return a == 1 ? sqrt(pow2(y) + pow2(z)) : a == 2 ? sqrt(pow2(x) + pow2(z)) : sqrt(pow2(x) + pow2(y));
+}
+static double polar_angle(double kperp, double kpll)
+{
+    double t;
+    if (fabs(kperp) > MYEPS) t = (fabs(kpll) > MYEPS) ? atan(kperp / kpll) : M_PI / 2;
+    else t = (kpll >= 0) ? 0.0 : M_PI;
+    return (t >= 0) ? t : t + M_PI;
+}
+static int sector_index(double theta, const double *arr, int len)
+{
+    if (theta > M_PI / 2) theta = M_PI - theta;
+    if (fabs(theta) < MYEPS) return 1;
+    int i;
+    for (i = 0; i < len; i++)
+        if (theta <= arr[i]) break;
+    return i;
+}
+static int slab_index(double kpll, const double *arr, int len)
+{
+    kpll = fabs(kpll);
+    if (kpll < MYEPS) return 1;
+    int i;
+    for (i = 1; i < len; i++)
+        if (kpll <= arr[i]) break;
+    return i;
+}
+int orc_cylinder_shell_ek_size(const orc_grid *g, int a)
+{
+    double f[3] = {g->f1, g->f2, g->f3};
+    int N[3] = {g->N1, g->N2, g->N3}, d = g->dealias ? 3 : 2;
+    int p = a == 1 ? 1 : 0, q = a == 3 ? 1 : 2;
+    double k1 = (N[p] / d) * f[p], k2 = (N[q] / d) * f[q];
+    return (int)(k1 < k2 ? k1 : k2) + 1;
+}
+static double aniso_max_kpll(const orc_grid *g, int a)
+{
+    double f[3] = {g->f1, g->f2, g->f3};
+    int N[3] = {g->N1, g->N2, g->N3}, d = g->dealias ? 3 : 2;
+    return (N[a - 1] / d) * f[a - 1];
+}
+
+static void add2(double *S, int rows, int cols, int r, int c, double v)
+{
+    long flat = (long)r * cols + c;
+    if (flat >= 0 && flat < (long)rows * cols) S[flat] += v;
+}
+
+/* Compute_local_ring_spectrum_FOUR four_energy.cc:828-954 (3-D branch): modes 0 < |K|, ceil|K| <=
+ * Max_radius_inside; off the axis S1 += f |K|^n |(V x K)_a / K_perp|^2 and S2 += f |K|^n (|V|^2 - that);
+ * on the axis S1 += f |K|^n |V_b|^2, S2 += f |K|^n |V_c|^2 with (b, c) = (2, 3) for a = 1 and (3, 2) for
+ * a = 2, while for a = 3 the anisotropic components are never assigned and keep their initial zero
+ * (four_energy.cc:883-897 tests switch 2 twice), so on-axis modes add nothing.  f = Multiplicity. */
+void orc_ring_spectrum(const orc_grid *g, double *const V[3], int nsect, int n, int a, double *S1, double *S2)
+{
+    int N1 = g->N1, N2 = g->N2, kkmax = orc_max_radius_inside(g), rows = kkmax + 1, cols = nsect + 1;
+    double *arr = (double *)malloc(sizeof(double) * cols);
+    double dth = (M_PI / 2) / nsect;
+    for (int i = 0; i < nsect; i++) arr[i] = i * dth;
+    arr[nsect] = M_PI / 2;
+    for (int i = 0; i < rows * cols; i++) S1[i] = S2[i] = 0;
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 <= g->N3 / 2; l3++) {
+                double km = orc_kmag(g, l1, l2, l3);
+                int sh = (int)ceil(km);
+                if (!(km > MYEPS && sh <= kkmax)) continue;
+                double kpll = aniso_kpll(g, a, l1, l2, l3), kperp = aniso_kperp(g, a, l1, l2, l3);
+                int sec = sector_index(polar_angle(kperp, kpll), arr, cols);
+                double w = orc_multiplicity(g, l1, l2, l3) * my_pow(km, n);
+                size_t i = IDX(g, l1, l2, l3);
+                double vr[3], vi[3], v2 = 0;
+                for (int c = 0; c < 3; c++) { vr[c] = RE(V[c], i); vi[c] = IM(V[c], i); v2 += pow2(vr[c]) + pow2(vi[c]); }
+                if (kperp < MYEPS) {
+                    if (a == 3) { add2(S1, rows, cols, sh, sec, 0.0); add2(S2, rows, cols, sh, sec, 0.0); continue; }
+                    int b = a == 1 ? 1 : 2, c = a == 1 ? 2 : 1;
+                    add2(S1, rows, cols, sh, sec, w * (pow2(vr[b]) + pow2(vi[b])));
+                    add2(S2, rows, cols, sh, sec, w * (pow2(vr[c]) + pow2(vi[c])));
+                } else {
+                    double kk[3] = {orc_kx(g, l1) * g->f1, orc_ky(g, l2) * g->f2, l3 * g->f3};
+                    int p = a % 3, q = (a + 1) % 3;     /* (V x K)_(a-1) = V_p K_q - V_q K_p */
+                    double xr = (vr[p] * kk[q] - vr[q] * kk[p]) / kperp, xi = (vi[p] * kk[q] - vi[q] * kk[p]) / kperp;
+                    double s1 = pow2(xr) + pow2(xi);
+                    add2(S1, rows, cols, sh, sec, w * s1);
+                    add2(S2, rows, cols, sh, sec, w * (v2 - s1));
+                }
+            }
+    free(arr);
+}
+
+/* Compute_local_cylinder_ring_spectrum_FOUR four_energy.cc:1450-1515: every mode with
+ * ceil(K_perp) <= Anis_max_Krho_radius_inside, slab of |K_pll|; S1 += f |K|^n |V_a|^2 and
+ * S2 += f |K|^n (|V|^2 - |V_a|^2), where for a = 3 V_a is never assigned (switch 2 tested twice,
+ * four_energy.cc:1493-1500) and stays zero. */
+void orc_cylinder_ring_spectrum(const orc_grid *g, double *const V[3], int nslab, int n, int a, double *S1,
+                                double *S2)
+{
+    int N1 = g->N1, N2 = g->N2, rows = orc_cylinder_shell_ek_size(g, a), cols = nslab + 1, kpm = rows - 1;
+    double *arr = (double *)malloc(sizeof(double) * cols), kmax = aniso_max_kpll(g, a);
+    double dk = kmax / nslab;
+    arr[0] = 0.0;
+    for (int i = 1; i < nslab; i++) arr[i] = i * dk;
+    arr[nslab] = kmax;
+    for (int i = 0; i < rows * cols; i++) S1[i] = S2[i] = 0;
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 <= g->N3 / 2; l3++) {
+                double kperp = aniso_kperp(g, a, l1, l2, l3);
+                int sh = (int)ceil(kperp);
+                if (sh > kpm) continue;
+                int sl = slab_index(aniso_kpll(g, a, l1, l2, l3), arr, cols);
+                double w = orc_multiplicity(g, l1, l2, l3) * my_pow(orc_kmag(g, l1, l2, l3), n);
+                size_t i = IDX(g, l1, l2, l3);
+                double v2 = 0, va = 0;
+                for (int c = 0; c < 3; c++) v2 += pow2(RE(V[c], i)) + pow2(IM(V[c], i));
+                if (a != 3) va = pow2(RE(V[a - 1], i)) + pow2(IM(V[a - 1], i));
+                add2(S1, rows, cols, sh, sl, w * va);
+                add2(S2, rows, cols, sh, sl, w * (v2 - va));
+            }
+    free(arr);
+}
+
 /* Compute_local_helicity_FOUR four_energy.cc:453-506: over |K| <= Min_radius_outside,
  * modal = 2 mult K.(Vr x Vi); out = {H1 = sum modal, H2 = sum modal / K^2 (K^2 > MYEPS = 1e-15),
  * DH1 = 2 sum K^2 modal, DH2 = 2 H1} */

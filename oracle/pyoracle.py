@@ -79,6 +79,9 @@ def lib():
             "orc_et_radii": (None, [p, i, i, vp]),
             "orc_et_nlin_scalar": (None, [p, pp, vp, vp]),
             "orc_helicity": (None, [p, pp, vp]),
+            "orc_ring_spectrum": (None, [p, pp, i, i, i, vp, vp]),
+            "orc_cylinder_shell_ek_size": (i, [p, i]),
+            "orc_cylinder_ring_spectrum": (None, [p, pp, i, i, i, vp, vp]),
             "orc_shell_helicity": (None, [p, pp, vp, vp, vp]),
             "orc_entropy": (d, [p, pp]),
             "orc_flux_scalar": (None, [p, pp, vp, vp, i, vp, vp]),
@@ -320,6 +323,22 @@ class Oracle:
         out = np.zeros(4)
         self.L.orc_helicity(self.gp, _pp(V), out.ctypes.data_as(C.c_void_p))
         return out
+
+    def ring_spectrum(self, V, nsect, n=0, axis=3):
+        """(S1k, S2k) of Compute_ring_spectrum_FOUR, shape (Max_radius_inside+1, nsect+1)."""
+        shape = (self.max_radius_inside() + 1, nsect + 1)
+        S1, S2 = np.zeros(shape), np.zeros(shape)
+        self.L.orc_ring_spectrum(self.gp, _pp(V), nsect, n, axis, S1.ctypes.data_as(C.c_void_p),
+                                 S2.ctypes.data_as(C.c_void_p))
+        return S1, S2
+
+    def cylinder_ring_spectrum(self, V, nslab, n=0, axis=3):
+        """(S1k, S2k) of Compute_cylinder_ring_spectrum_FOUR, shape (Anis_max_Krho_radius_inside+1, nslab+1)."""
+        shape = (self.L.orc_cylinder_shell_ek_size(self.gp, axis), nslab + 1)
+        S1, S2 = np.zeros(shape), np.zeros(shape)
+        self.L.orc_cylinder_ring_spectrum(self.gp, _pp(V), nslab, n, axis, S1.ctypes.data_as(C.c_void_p),
+                                          S2.ctypes.data_as(C.c_void_p))
+        return S1, S2
 
     def shell_helicity(self, V):
         n = self.shell_ek_size()

@@ -94,6 +94,12 @@ void   orc_et_nlin(const orc_grid *g, double *const H[3], double *const G[3], do
 void   orc_flux_ns(const orc_grid *g, double *const V[3], const double *radii, int nsph, double *flux);
 void   orc_shell_tr_ns(const orc_grid *g, double *const V[3], const double *radii, int nsh, double *T);
 void   orc_helicity(const orc_grid *g, double *const V[3], double *out);
+/* anisotropic spectra, axis a = 1, 2, 3: ring (Max_radius_inside+1) x (nsect+1), cylinder
+ * orc_cylinder_shell_ek_size(g, a) x (nslab+1), both row-major */
+void   orc_ring_spectrum(const orc_grid *g, double *const V[3], int nsect, int n, int a, double *S1, double *S2);
+int    orc_cylinder_shell_ek_size(const orc_grid *g, int a);
+void   orc_cylinder_ring_spectrum(const orc_grid *g, double *const V[3], int nslab, int n, int a, double *S1,
+                                  double *S2);
 void   orc_shell_helicity(const orc_grid *g, double *const V[3], double *Hk1, double *Hk2, double *Hk3);
 double orc_entropy(const orc_grid *g, double *const V[3]);
 void   orc_et_nlin_scalar(const orc_grid *g, double *const H[3], const double *G, double *nlin);

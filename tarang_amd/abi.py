@@ -119,6 +119,9 @@ def lib():
         "th_state_download": (i, [vp, i, vp, i]),
         "th_step": (i, [vp, vp, d, C.POINTER(Phys), i]),
         "th_state_set_force": (i, [vp, C.POINTER(Force)]),
+        "th_ring_spectrum": (i, [vp, pp, i, i, i, dp, dp]),
+        "th_cylinder_ring_spectrum": (i, [vp, pp, i, i, i, dp, dp]),
+        "th_cylinder_shell_ek_size": (i, [vp, i]),
         "th_state_download_buffer": (i, [vp, i, i, vp, i]),
         "th_flux_shell_tr": (i, [vp, vp, dp, i, dp, dp]),
         "th_profile_passes": (i, [vp, vp, d, C.POINTER(Phys), i, i, dp, dp]),
@@ -384,6 +387,27 @@ class Plan:
                    "th_step_plain_forced")
         self._post(0, "")
         return V
+
+    def ring_spectrum(self, V, nsect, n=0, axis=3):
+        """(S1k, S2k) of Compute_ring_spectrum_FOUR, shape (Max_radius_inside+1, nsect+1)."""
+        import numpy as np
+        shape = (lib().th_max_radius_inside(self.h) + 1, nsect + 1)
+        S1, S2 = np.zeros(shape), np.zeros(shape)
+        self._pre()
+        _check(lib().th_ring_spectrum(self.h, _pp(V), nsect, n, axis, S1.ctypes.data_as(C.POINTER(C.c_double)),
+                                      S2.ctypes.data_as(C.POINTER(C.c_double))), "th_ring_spectrum")
+        return S1, S2
+
+    def cylinder_ring_spectrum(self, V, nslab, n=0, axis=3):
+        """(S1k, S2k) of Compute_cylinder_ring_spectrum_FOUR, shape (Anis_max_Krho_radius_inside+1, nslab+1)."""
+        import numpy as np
+        rows = lib().th_cylinder_shell_ek_size(self.h, axis)
+        _check(0 if rows > 0 else rows, "th_cylinder_shell_ek_size")
+        S1, S2 = np.zeros((rows, nslab + 1)), np.zeros((rows, nslab + 1))
+        self._pre()
+        _check(lib().th_cylinder_ring_spectrum(self.h, _pp(V), nslab, n, axis, S1.ctypes.data_as(C.POINTER(C.c_double)),
+                                               S2.ctypes.data_as(C.POINTER(C.c_double))), "th_cylinder_ring_spectrum")
+        return S1, S2
 
     def compute_force(self, V, proc, inner, outer, p3, p4=0.0):
         """Force field F[3] of force_field_proc `proc` for the velocity V, and nf."""

@@ -932,6 +932,30 @@ extern "C" int th_shell_spectrum_helicity(th_plan* p, void* const V[3], double* 
     if (p->g.basis != TH_FOUR) { set_error("th_shell_spectrum_helicity: FOUR basis only"); return E_UNSUPPORTED; }
     return DISPATCH(p, do_shell_hel, p, V, H1k1, H1k2, H1k3);
 }
+template <typename R> static int do_aniso(th_plan* p, void* const V[3], int mode, int nbin, int n, int axis, double* S1,
+                                          double* S2)
+{
+    const cplx<R>* v[3] = {CCP(R, V[0]), CCP(R, V[1]), CCP(R, V[2])};
+    return reduce_aniso<R>(p->g, v, mode, nbin, n, axis, S1, S2, p->rs, p->ctx->stream);
+}
+extern "C" int th_ring_spectrum(th_plan* p, void* const V[3], int nsect, int n, int axis, double* S1k, double* S2k)
+{
+    TRY(check_plan(p));
+    if (!V || !V[0] || !V[1] || !V[2] || !S1k || !S2k) { set_error("th_ring_spectrum: null"); return E_ARG; }
+    return DISPATCH(p, do_aniso, p, V, 0, nsect, n, axis, S1k, S2k);
+}
+extern "C" int th_cylinder_ring_spectrum(th_plan* p, void* const V[3], int nslab, int n, int axis, double* S1k,
+                                         double* S2k)
+{
+    TRY(check_plan(p));
+    if (!V || !V[0] || !V[1] || !V[2] || !S1k || !S2k) { set_error("th_cylinder_ring_spectrum: null"); return E_ARG; }
+    return DISPATCH(p, do_aniso, p, V, 1, nslab, n, axis, S1k, S2k);
+}
+extern "C" int th_cylinder_shell_ek_size(const th_plan* p, int axis)
+{
+    if (!p || axis < 1 || axis > 3) { set_error("th_cylinder_shell_ek_size: bad args"); return E_ARG; }
+    return cylinder_shell_ek_size(p->g, axis);
+}
 template <typename R> static int do_fill(th_plan* p, const void* A, void* B, double in, double out)
 {
     return k_fill_shell<R>(p->g, CCP(R, A), CP(R, B), in, out, p->ctx->stream);

@@ -141,6 +141,12 @@ int force_setup(const Geo& g, const th_force* f, ForceBand& b);
 template <typename R> int force_subtract(const ForceBand& b, const cplx<R>* const V[3], cplx<R>* const nl[3],
                                          hipStream_t st);
 
+// anisotropic spectra: mode 0 ring (rows Max_radius_inside+1), 1 cylinder (rows cylinder_shell_ek_size);
+// S1, S2 host arrays of rows x (nbin+1)
+template <typename R> int reduce_aniso(const Geo& g, const cplx<R>* const V[3], int mode, int nbin, int n, int axis,
+                                       double* S1, double* S2, RedScratch& rs, hipStream_t st);
+int cylinder_shell_ek_size(const Geo& g, int axis);
+
 // host-side binning helpers (four_inline.h, scft_inline.h)
 int min_radius_outside(const Geo& g);
 int max_radius_inside(const Geo& g);

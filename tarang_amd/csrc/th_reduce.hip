@@ -266,6 +266,107 @@ __global__ __launch_bounds__(RBLK) void k_binned_hel(GeoR g, const cplx<R>* __re
     for (int b = threadIdx.x; b < 4 * nbins; b += RBLK) partial[(long)blockIdx.x * 4 * nbins + b] = sh[b];
 }
 
+// Anisotropic spectra (FOUR, 3-D) about axis a = globalvar_anisotropy_switch (1, 2, 3), bins
+// (row, col) flattened row-major into rows x cols; a column past the last lands on the next row's
+// column 0 and past the array it is dropped (the reference's unchecked Blitz store).
+//   MODE 0, ring: Compute_local_ring_spectrum_FOUR four_energy.cc:828-954; row ceil|K| <= kmaxi for
+//     |K| > MYEPS, column Get_sector_index basis_basicfn.cc:304-322 of the polar angle
+//     (Get_polar_angle basis_basicfn_inline.h:89-112; sector edges i dpar, the last pmax).
+//     Off the axis S1 += w |(V x K)_a / K_perp|^2, S2 += w (|V|^2 - that); on the axis (a = 1) S1 += w |V_y|^2,
+//     S2 += w |V_z|^2, (a = 2) S1 += w |V_z|^2, S2 += w |V_y|^2, (a = 3) nothing (the components are never set).
+//   MODE 1, cylinder: Compute_local_cylinder_ring_spectrum_FOUR four_energy.cc:1450-1515; row ceil(K_perp)
+//     <= kmaxi, column Get_slab_index basis_basicfn.cc:373-390 of |K_pll| (slab edges i dpar, the last pmax);
+//     S1 += w |V_a|^2 (0 for a = 3), S2 += w (|V|^2 - |V_a|^2).
+//   w = Multiplicity_factor_FOUR |K|^n.
+struct AnisoP {
+    int axis, nbin, rows, cols, n, kmaxi;
+    double dpar, pmax;
+};
+
+template <typename R, int MODE>
+__global__ __launch_bounds__(RBLK) void k_aniso(GeoR g, const cplx<R>* __restrict__ A0, const cplx<R>* __restrict__ A1,
+                                                const cplx<R>* __restrict__ A2, AnisoP ap, double* __restrict__ partial)
+{
+    extern __shared__ double sh[];
+    const int nb = ap.rows * ap.cols;
+    for (int b = threadIdx.x; b < 2 * nb; b += RBLK) sh[b] = 0.0;
+    __syncthreads();
+    const long n = (long)g.N1 * g.N2 * g.H;
+    const double eps = 1e-15;
+    for (long i = (long)blockIdx.x * RBLK + threadIdx.x; i < n; i += (long)gridDim.x * RBLK) {
+        int l1, l2, l3;
+        decode_r(g, i, l1, l2, l3);
+        const double kk[3] = {kx_four(l1, g.N1) * g.f1, ky_of(l2, g.N2) * g.f2, l3 * g.f3};
+        const double km = kmag_r(g, l1, l2, l3);
+        const int a = ap.axis - 1, p = ap.axis % 3, q = (ap.axis + 1) % 3;
+        const double kpll = kk[a];
+        double kperp;
+        {
+#pragma clang fp contract(off)
+            kperp = sqrt(kk[p] * kk[p] + kk[q] * kk[q]);
+        }
+        int row, col;
+        if (MODE == 0) {
+            row = (int)ceil(km);
+            if (!(km > eps && row <= ap.kmaxi)) continue;
+            double t;
+            if (fabs(kperp) > eps) {
+                if (fabs(kpll) > eps) {
+                    const double r = kperp / kpll;
+                    t = (r == 1.0) ? M_PI / 4 : atan(r);       // atan(1) correctly rounded, as libm gives it
+                } else {
+                    t = M_PI / 2;
+                }
+            } else {
+                t = (kpll >= 0) ? 0.0 : M_PI;
+            }
+            if (t < 0) t += M_PI;
+            if (t > M_PI / 2) t = M_PI - t;
+            if (fabs(t) < eps) {
+                col = 1;
+            } else {
+                for (col = 0; col < ap.cols; col++)
+                    if (t <= (col < ap.nbin ? col * ap.dpar : ap.pmax)) break;
+            }
+        } else {
+            row = (int)ceil(kperp);
+            if (row > ap.kmaxi) continue;
+            const double ak = fabs(kpll);
+            if (ak < eps) {
+                col = 1;
+            } else {
+                for (col = 1; col < ap.cols; col++)
+                    if (ak <= (col < ap.nbin ? col * ap.dpar : ap.pmax)) break;
+            }
+        }
+        const long flat = (long)row * ap.cols + col;
+        if (flat >= nb) continue;
+        const cplx<R> v[3] = {A0[i], A1[i], A2[i]};
+        const double w = mult_r(g, l1, l2, l3) * my_pow_r(km, ap.n);
+        const double v2 = sq(v[0]) + sq(v[1]) + sq(v[2]);
+        double s1, s2;
+        if (MODE == 0) {
+            if (kperp < eps) {
+                if (ap.axis == 3) continue;
+                s1 = sq(v[ap.axis == 1 ? 1 : 2]);
+                s2 = sq(v[ap.axis == 1 ? 2 : 1]);
+            } else {
+                const double xr = ((double)v[p].x * kk[q] - (double)v[q].x * kk[p]) / kperp;
+                const double xi = ((double)v[p].y * kk[q] - (double)v[q].y * kk[p]) / kperp;
+                s1 = xr * xr + xi * xi;
+                s2 = v2 - s1;
+            }
+        } else {
+            s1 = ap.axis == 3 ? 0.0 : sq(v[a]);
+            s2 = v2 - s1;
+        }
+        atomicAdd(&sh[flat], w * s1);
+        atomicAdd(&sh[nb + flat], w * s2);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 2 * nb; b += RBLK) partial[(long)blockIdx.x * 2 * nb + b] = sh[b];
+}
+
 __global__ void k_finalize(const double* __restrict__ partial, int nblk, int stride, int nval, double* __restrict__ out)
 {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -512,6 +613,71 @@ int reduce_shell_helicity(const Geo& g, const cplx<R>* const V[3], int nbins, do
     return OK;
 }
 
+// Anis_max_Krho_radius_inside_FOUR four_inline.h:517-546 (+1: the rows of the cylindrical spectrum)
+int cylinder_shell_ek_size(const Geo& g, int axis)
+{
+    const int d = g.dealias ? 3 : 2;
+    const int N[3] = {g.N1, g.N2, g.N3};
+    const int p = axis == 1 ? 1 : 0, q = axis == 3 ? 1 : 2;
+    const double k1 = (N[p] / d) * g.kf[p], k2 = (N[q] / d) * g.kf[q];
+    return (int)(k1 < k2 ? k1 : k2) + 1;
+}
+
+template <typename R>
+int reduce_aniso(const Geo& g, const cplx<R>* const V[3], int mode, int nbin, int n, int axis, double* S1, double* S2,
+                 RedScratch& rs, hipStream_t st)
+{
+    if (g.basis != 0 || g.N2 < 2 || g.N3 < 4) { set_error("anisotropic spectra: 3-D FOUR boxes only"); return E_UNSUPPORTED; }
+    if (axis < 1 || axis > 3 || nbin < 1 || (n != 0 && n != 2 && n != 4)) {
+        set_error("anisotropic spectra: axis must be 1..3, bins >= 1, n in {0, 2, 4}");
+        return E_ARG;
+    }
+    AnisoP ap;
+    ap.axis = axis;
+    ap.nbin = nbin;
+    ap.cols = nbin + 1;
+    ap.n = n;
+    if (mode == 0) {
+        ap.kmaxi = max_radius_inside(g);
+        ap.rows = ap.kmaxi + 1;
+        ap.pmax = M_PI / 2;                               // Get_max_polar_angle_FOUR four_inline.h:549-554
+        ap.dpar = ap.pmax / nbin;                         // Cvf.cc:85-106, linear sectors
+    } else {
+        ap.rows = cylinder_shell_ek_size(g, axis);
+        ap.kmaxi = ap.rows - 1;
+        const int d = g.dealias ? 3 : 2;
+        const int N[3] = {g.N1, g.N2, g.N3};
+        ap.pmax = (N[axis - 1] / d) * g.kf[axis - 1];     // Anis_max_Kpll_FOUR four_inline.h:484-514
+        ap.dpar = (ap.pmax - 0.0) / nbin;                 // Cvf.cc:166-190, linear slabs from 0
+    }
+    const int nb2 = 2 * ap.rows * ap.cols;
+    if ((size_t)nb2 * sizeof(double) > 64 * 1024) {
+        set_error("anisotropic spectra: %d x %d bins exceed one workgroup's 64 KiB of LDS", ap.rows, ap.cols);
+        return E_ARG;
+    }
+    const int nblk = nblocks_for(g.ncplx() / 4 + 1);
+    int r = rs.ensure((size_t)nblk * nb2 + nb2, nb2);
+    if (r) return r;
+    double* out = rs.d + (size_t)nblk * nb2;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_aniso<R, 0>), dim3(nblk), dim3(RBLK), nb2 * sizeof(double), st, gr(g), V[0], V[1], V[2],
+                           ap, rs.d);
+    else
+        hipLaunchKernelGGL((k_aniso<R, 1>), dim3(nblk), dim3(RBLK), nb2 * sizeof(double), st, gr(g), V[0], V[1], V[2],
+                           ap, rs.d);
+    TH_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3((nb2 + 255) / 256), dim3(256), 0, st, rs.d, nblk, nb2, nb2, out);
+    TH_CHECK_HIP(hipGetLastError());
+    TH_CHECK_HIP(hipMemcpyAsync(rs.h, out, nb2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    TH_CHECK_HIP(hipStreamSynchronize(st));
+    const int nb = ap.rows * ap.cols;
+    for (int k = 0; k < nb; k++) {
+        S1[k] = rs.h[k];
+        S2[k] = rs.h[nb + k];
+    }
+    return OK;
+}
+
 #define TH_RINST(R)                                                                                                  \
     template int reduce_shell<R>(const Geo&, const cplx<R>*, const cplx<R>*, double, int, double*, double*,         \
                                  RedScratch&, hipStream_t, bool);                                                    \
@@ -524,7 +690,9 @@ int reduce_shell_helicity(const Geo& g, const cplx<R>* const V[3], int nbins, do
     template int reduce_helicity<R>(const Geo&, const cplx<R>* const*, double*, RedScratch&, hipStream_t);          \
     template int reduce_entropy<R>(const Geo&, const cplx<R>* const*, double*, RedScratch&, hipStream_t);           \
     template int reduce_shell_helicity<R>(const Geo&, const cplx<R>* const*, int, double*, double*, double*,         \
-                                          RedScratch&, hipStream_t);
+                                          RedScratch&, hipStream_t);                                                \
+    template int reduce_aniso<R>(const Geo&, const cplx<R>* const*, int, int, int, int, double*, double*,             \
+                                 RedScratch&, hipStream_t);
 TH_RINST(float)
 TH_RINST(double)
 
