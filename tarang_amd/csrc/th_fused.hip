@@ -560,20 +560,6 @@ extern "C" int th_step(th_plan* p, th_state* s, double dt, const th_phys* ph, in
     return OK;
 }
 
-static __global__ void th_finalize_bins(const double* __restrict__ partial, int nblk, int stride, int nval,
-                                        double* __restrict__ out)
-{
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nval) return;
-    double s = 0.0, c = 0.0;
-    for (int k = 0; k < nblk; k++) {
-        double v = partial[(long)k * stride + b];
-        double t = s + v;
-        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
-        s = t;
-    }
-    out[b] = s + c;
-}
 
 // Energy transfer from the state's velocity: one giver-masked fused pipeline per shell n gives
 // T(n, m) (Shell_mult_all, |K| <= kmax_inside) and the same sums over |K| > kmax_inside ("corner",
@@ -624,8 +610,7 @@ template <typename R> static int flux_impl(th_state* s, const double* radii, int
         TRY((d_z<R, 2>(g, X2, X3, twz, twn, 0, st)));
         TRY(d_yfwd<R>(g, X3, X1, 9, twy, 0, st));
         TRY(d_xfwd_et<R>(g, X1, S, twx, s->radii, nr, kin, s->partial, &nblk, st));
-        hipLaunchKernelGGL(th_finalize_bins, dim3((2 * nr + 255) / 256), dim3(256), 0, st, (const double*)s->partial,
-                           nblk, 2 * nr, 2 * nr, outd);
+        launch_finalize((const double*)s->partial, nblk, 2 * nr, 2 * nr, outd, st);
         TH_CHECK_HIP(hipGetLastError());
         TH_CHECK_HIP(hipMemcpyAsync(hostbuf.data(), outd, 2 * nr * sizeof(double), hipMemcpyDeviceToHost, st));
         TH_CHECK_HIP(hipStreamSynchronize(st));

@@ -147,6 +147,43 @@ template <typename R> int reduce_aniso(const Geo& g, const cplx<R>* const V[3], 
                                        double* S1, double* S2, RedScratch& rs, hipStream_t st);
 int cylinder_shell_ek_size(const Geo& g, int axis);
 
+#ifdef __HIP__
+// out[b] = sum over k < nblk of partial[k * stride + b], b < nval: one workgroup per value, each thread
+// a compensated (Neumaier) sum over a strided subset of the k, then the (sum, compensation) pairs
+// combined in a fixed tree, so the result does not depend on scheduling.
+constexpr int FIN_T = 256;
+static __global__ __launch_bounds__(FIN_T) void k_finalize_bins(const double* __restrict__ partial, int nblk, long stride,
+                                                                double* __restrict__ out)
+{
+    __shared__ double ss[FIN_T], cc[FIN_T];
+    const int b = blockIdx.x, t = threadIdx.x;
+    double s = 0.0, c = 0.0;
+    for (int k = t; k < nblk; k += FIN_T) {
+        const double v = partial[(long)k * stride + b];
+        const double u = s + v;
+        c += (fabs(s) >= fabs(v)) ? ((s - u) + v) : ((v - u) + s);
+        s = u;
+    }
+    ss[t] = s;
+    cc[t] = c;
+    __syncthreads();
+    for (int w = FIN_T / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            const double s1 = ss[t], s2 = ss[t + w], u = s1 + s2;
+            const double e = (fabs(s1) >= fabs(s2)) ? ((s1 - u) + s2) : ((s2 - u) + s1);
+            ss[t] = u;
+            cc[t] = cc[t] + cc[t + w] + e;
+        }
+        __syncthreads();
+    }
+    if (t == 0) out[b] = ss[0] + cc[0];
+}
+static inline void launch_finalize(const double* partial, int nblk, long stride, int nval, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_finalize_bins, dim3((unsigned)nval), dim3(FIN_T), 0, st, partial, nblk, stride, out);
+}
+#endif
+
 // host-side binning helpers (four_inline.h, scft_inline.h)
 int min_radius_outside(const Geo& g);
 int max_radius_inside(const Geo& g);

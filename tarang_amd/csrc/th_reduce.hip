@@ -367,19 +367,6 @@ __global__ __launch_bounds__(RBLK) void k_aniso(GeoR g, const cplx<R>* __restric
     for (int b = threadIdx.x; b < 2 * nb; b += RBLK) partial[(long)blockIdx.x * 2 * nb + b] = sh[b];
 }
 
-__global__ void k_finalize(const double* __restrict__ partial, int nblk, int stride, int nval, double* __restrict__ out)
-{
-    int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nval) return;
-    double s = 0.0, c = 0.0;
-    for (int k = 0; k < nblk; k++) {
-        double v = partial[(long)k * stride + b];
-        double t = s + v;
-        c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
-        s = t;
-    }
-    out[b] = s + c;
-}
 
 int RedScratch::ensure(size_t n_dev, size_t n_host)
 {
@@ -460,7 +447,7 @@ int reduce_shell(const Geo& g, const cplx<R>* A, const cplx<R>* B, double n, int
                        (const cplx<R>*)nullptr, (const cplx<R>*)nullptr, B, (const cplx<R>*)nullptr,
                        (const cplx<R>*)nullptr, (int)n, kkmax, (const double*)nullptr, 0, nbins, rs.d);
     TH_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3((2 * nbins + 255) / 256), dim3(256), 0, st, rs.d, nb, 2 * nbins, 2 * nbins, out);
+    launch_finalize(rs.d, nb, 2 * nbins, 2 * nbins, out, st);
     TH_CHECK_HIP(hipGetLastError());
     TH_CHECK_HIP(hipMemcpyAsync(rs.h, out, 2 * nbins * sizeof(double), hipMemcpyDeviceToHost, st));
     TH_CHECK_HIP(hipStreamSynchronize(st));
@@ -490,7 +477,7 @@ static int run_scalar(int mode, const Geo& g, const cplx<R>* const A[3], const c
     else if (mode == 1) launch(k_scalar<R, 1>);
     else launch(k_scalar<R, 2>);
     TH_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, rs.d, nb, 1, 1, rs.d + nb);
+    launch_finalize(rs.d, nb, 1, 1, rs.d + nb, st);
     TH_CHECK_HIP(hipGetLastError());
     TH_CHECK_HIP(hipMemcpyAsync(rs.h, rs.d + nb, sizeof(double), hipMemcpyDeviceToHost, st));
     TH_CHECK_HIP(hipStreamSynchronize(st));
@@ -532,7 +519,7 @@ int reduce_shell_mult_all(const Geo& g, const cplx<R>* const A[3], const cplx<R>
     hipLaunchKernelGGL((k_binned<R, 1>), dim3(nb), dim3(RBLK), 2 * nbins * sizeof(double), st, gr(g), A[0], A[1], A[2],
                        B[0], B[1], B[2], 0, kin, radii_dev, nr, nbins, rs.d);
     TH_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3((nbins + 255) / 256), dim3(256), 0, st, rs.d, nb, 2 * nbins, nbins, dout);
+    launch_finalize(rs.d, nb, 2 * nbins, nbins, dout, st);
     TH_CHECK_HIP(hipGetLastError());
     TH_CHECK_HIP(hipMemcpyAsync(rs.h, dout, nbins * sizeof(double), hipMemcpyDeviceToHost, st));
     TH_CHECK_HIP(hipStreamSynchronize(st));
@@ -553,7 +540,7 @@ template <typename R> static int run_modal(int mode, const Geo& g, const cplx<R>
     else
         hipLaunchKernelGGL((k_modal<R, 1>), dim3(nb), dim3(RBLK), 0, st, gr(g), V[0], V[1], V[2], kkmax, E, rs.d);
     TH_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(64), 0, st, rs.d, nb, nv, nv, rs.d + (size_t)nb * nv);
+    launch_finalize(rs.d, nb, nv, nv, rs.d + (size_t)nb * nv, st);
     TH_CHECK_HIP(hipGetLastError());
     TH_CHECK_HIP(hipMemcpyAsync(rs.h, rs.d + (size_t)nb * nv, nv * sizeof(double), hipMemcpyDeviceToHost, st));
     TH_CHECK_HIP(hipStreamSynchronize(st));
@@ -598,7 +585,7 @@ int reduce_shell_helicity(const Geo& g, const cplx<R>* const V[3], int nbins, do
     hipLaunchKernelGGL((k_binned_hel<R>), dim3(nb), dim3(RBLK), 4 * nbins * sizeof(double), st, gr(g), V[0], V[1], V[2],
                        kkmax, nbins, rs.d);
     TH_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3((4 * nbins + 255) / 256), dim3(256), 0, st, rs.d, nb, 4 * nbins, 4 * nbins, out);
+    launch_finalize(rs.d, nb, 4 * nbins, 4 * nbins, out, st);
     TH_CHECK_HIP(hipGetLastError());
     TH_CHECK_HIP(hipMemcpyAsync(rs.h, out, 4 * nbins * sizeof(double), hipMemcpyDeviceToHost, st));
     TH_CHECK_HIP(hipStreamSynchronize(st));
@@ -666,7 +653,7 @@ int reduce_aniso(const Geo& g, const cplx<R>* const V[3], int mode, int nbin, in
         hipLaunchKernelGGL((k_aniso<R, 1>), dim3(nblk), dim3(RBLK), nb2 * sizeof(double), st, gr(g), V[0], V[1], V[2],
                            ap, rs.d);
     TH_CHECK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3((nb2 + 255) / 256), dim3(256), 0, st, rs.d, nblk, nb2, nb2, out);
+    launch_finalize(rs.d, nblk, nb2, nb2, out, st);
     TH_CHECK_HIP(hipGetLastError());
     TH_CHECK_HIP(hipMemcpyAsync(rs.h, out, nb2 * sizeof(double), hipMemcpyDeviceToHost, st));
     TH_CHECK_HIP(hipStreamSynchronize(st));
