@@ -146,7 +146,8 @@ public:
     // Output files, the reference's names and formats (stream precision 12):
     //   Open_output_files file_operations.cc:163-240 (<data_dir>/out/glob.d, spectrum.d, flux.d)
     //   Output_global output_energy.cc:43-81, Output_shell_spectrum :368-394, Output_flux output_ET.cc:45-74
-    // Unforced runs: the force-spectrum and force-feed columns are 0, helicity_flux_switch = 0.
+    // The force-spectrum (spectrum.d) and force-feed (flux.d) columns come from the force of the current
+    // velocity (0 for unforced runs); helicity_flux_switch = 0.
     void Open_output_files(const std::string& data_dir);
     void Output_global();
     void Output_shell_spectrum();
@@ -156,8 +157,13 @@ public:
     std::string integ_scheme;
 
 private:
+    struct Force_field {           // F[3] of the current velocity (th_compute_force)
+        explicit Force_field(IncFluid& u);
+        DField f[3];
+    };
     bool fused_;
-    bool forced_ = false;
+    bool forced_ = false;          // the plain stepper carries the force
+    bool has_force_ = false;       // Set_force was called (fused or plain)
     th_force force_{};
     th_state* state_ = nullptr;
     bool state_fresh_ = false;

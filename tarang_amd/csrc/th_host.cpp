@@ -318,10 +318,20 @@ void IncFluid::Output_shell_spectrum()
     }
     for (int i = 0; i < n; i++) dk[i] *= 2.0;
     check(th_shell_spectrum_helicity(plan_, V, hk[0].data(), hk[1].data(), hk[2].data()), "CV_Compute_shell_spectrum_helicity");
+    // Compute_force_shell_spectrum force_spectrum.cc:48-53: 2 sum_c Re(V_c conj F_c) per shell (0 unforced)
+    std::vector<double> fk(n, 0.0);
+    if (has_force_) {
+        Force_field F(*this);
+        for (int c = 0; c < 3; c++) {
+            check(th_shell_spectrum(plan_, V[c], F.f[c].data(), 0, t.data(), nullptr), "Compute_force_shell_spectrum");
+            for (int i = 0; i < n; i++) fk[i] += t[i];
+        }
+        for (int i = 0; i < n; i++) fk[i] *= 2.0;
+    }
     for (int i = 0; i < n; i++)
         spectrum_file << i << "\t" << ek[0][i] << " " << ek[1][i] << " " << ek[2][i] << "\t"
                       << dissipation_coefficient * dk[i] << "\t" << hk[0][i] << "  " << hk[1][i] << "  " << hk[2][i] << "\t"
-                      << 0.0 << "\t" << std::endl;
+                      << fk[i] << "\t" << std::endl;
     spectrum_file << std::endl;
 }
 
@@ -329,9 +339,28 @@ void IncFluid::Output_flux()
 {
     flux_file << "%% Time = " << Tnow << std::endl;
     Compute_flux();
+    // Compute_force_feed_shell compute_force_feed.cc:70-76: Shell_mult_all(F, V) over shell_radius
+    std::vector<double> feed(shell_radius.size(), 0.0);
+    if (has_force_) {
+        Force_field F(*this);
+        void* Fp[3] = {F.f[0].data(), F.f[1].data(), F.f[2].data()};
+        void* V[3] = {V1.data(), V2.data(), V3.data()};
+        check(th_shell_mult_all(plan_, Fp, V, shell_radius.data(), (int)shell_radius.size(), feed.data()),
+              "Compute_force_feed_shell");
+    }
     for (int i = 1; i <= no_spheres; i++)
-        flux_file << i << " " << sphere_radius[i] << " " << flux_self[i] << "  " << 0.0 << "     " << std::endl;
+        flux_file << i << " " << sphere_radius[i] << " " << flux_self[i] << "  " << (i < (int)feed.size() ? feed[i] : 0.0)
+                  << "     " << std::endl;
     flux_file << std::endl;
+}
+
+// The force of the current velocity (Compute_force, as the time loop leaves Force1..3 before output).
+IncFluid::Force_field::Force_field(IncFluid& u)
+{
+    for (int c = 0; c < 3; c++) f[c] = DField(u.elems_, u.esize_);
+    void* const V[3] = {u.V1.data(), u.V2.data(), u.V3.data()};
+    void* F[3] = {f[0].data(), f[1].data(), f[2].data()};
+    check(th_compute_force(u.plan_, V, F, &u.force_, nullptr), "Compute_force");
 }
 
 void IncFluid::Set_force(int proc, double inner_radius, double outer_radius, double p3, double p4)
@@ -342,6 +371,7 @@ void IncFluid::Set_force(int proc, double inner_radius, double outer_radius, dou
         if (r == TH_OK) {
             force_ = f;
             forced_ = false;           // the fused step carries the force itself
+            has_force_ = true;
             return;
         }
         if (r != TH_E_UNSUPPORTED) check(r, "Set_force");
@@ -355,6 +385,7 @@ void IncFluid::Set_force(int proc, double inner_radius, double outer_radius, dou
     check(th_sync(plan_), "th_sync");
     force_ = f;
     forced_ = true;
+    has_force_ = true;
     if (state_) {
         th_state_destroy(state_);
         state_ = nullptr;

@@ -124,18 +124,49 @@ def test_driver_output_files_match_oracle(tmp_path):
         assert close(float(pi), flux[float(R)], 1e-11) and float(force) == 0.0
 
 
-def test_driver_forced_run_matches_oracle():
+@pytest.mark.parametrize("band", [(0.5, 2.5), (4.0, 5.0)])
+def test_driver_forced_run_matches_oracle(band):
     """force=1,...: IncFluid::Set_force then Time_advance with the supply force recomputed at every stage
-    (Compute_force_TO_rhs single_time_step.cc:351-364); E(t) and E(k) follow the oracle's forced steps."""
+    (Compute_force_TO_rhs single_time_step.cc:351-364); E(t) and E(k) follow the oracle's forced steps.
+    The first band runs on the fused step, the second (|k| = N/3 modes, not stored) on the plain one."""
     N, steps, dt, nu = 16, 5, 1e-3, 0.01
-    spec, flux, glob = run_driver(N, steps, dt, nu, "tg", "FP64", "fused", "force=1,0.5,2.5,0.3,0.2")
+    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"force=1,{band[0]},{band[1]},0.3,0.2")
     o = Oracle((N, N, N))
-    V = o.ic_taylor_green(1, 1.0)
+    V = o.ic_random(1)
     E = []
     for _ in range(steps):
-        o.step_ns_forced(V, nu, dt, 1, 0.5, 2.5, 0.3, 0.2)
+        o.step_ns_forced(V, nu, dt, 1, band[0], band[1], 0.3, 0.2)
         E.append(o.total_energy(V))
     assert np.abs(glob[:, 1] - np.array(E)).max() <= 1e-12 * max(E)
     Ek = o.vector_spectrum(V)
     for k, v in spec.items():
         assert abs(v - Ek[k]) <= 1e-12 * max(Ek)
+
+
+def test_driver_forced_output_columns(tmp_path):
+    """Forced run with out=DIR: the force-spectrum column of spectrum.d (Compute_force_shell_spectrum
+    force_spectrum.cc:48-53, 2 sum_c Re V_c.F_c* per shell) and the force-feed column of flux.d
+    (Compute_force_feed_shell compute_force_feed.cc:70-76, Shell_mult_all(F, V) over shell_radius) hold
+    the force of the final velocity, as the oracle computes them."""
+    N, steps, dt, nu = 32, 3, 1e-3, 0.01
+    fp = (1, 1.5, 3.5, 0.3, 0.2)
+    (tmp_path / "out").mkdir()
+    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"out={tmp_path}",
+                                  "force=" + ",".join(str(v) for v in fp))
+    o = Oracle((N, N, N))
+    V = o.ic_random(1)
+    o.step_ns_forced(V, nu, dt, *fp, steps=steps)
+    assert abs(glob[-1][1] - o.total_energy(V)) <= 1e-12 * o.total_energy(V)
+    F = o.force(V, *fp)
+    fk = 2 * sum(o.shell_spectrum(V[c], F[c])[0] for c in range(3))
+    rows = [r for r in (tmp_path / "out" / "spectrum.d").read_text().split("\n")[1:] if r.strip()]
+    assert len(rows) == len(fk) and np.abs(fk).max() > 0
+    for i, r in enumerate(rows):
+        assert abs(float(r.split("\t")[4]) - fk[i]) <= 1e-10 * np.abs(fk).max(), (i, r)
+    radii = np.concatenate([np.arange(0, N // 3 + 2, dtype=float), [10000.0]])
+    feed = o.shell_mult_all(F, V, radii)
+    frows = [r for r in (tmp_path / "out" / "flux.d").read_text().split("\n")[1:] if r.strip()]
+    assert np.abs(feed).max() > 0
+    for r in frows:
+        i, R, pi, force = r.split()
+        assert abs(float(force) - feed[int(i)]) <= 1e-10 * np.abs(feed).max(), r
