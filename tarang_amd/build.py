@@ -57,7 +57,7 @@ def build(jobs=8, force=False, verbose=False):
         if force or _stale(src, obj, headers):
             cmd = [HIPCC] + COMMON + ["-c", "-o", obj, src]
             if s.endswith(".cpp"):
-                cmd = [HIPCC, "-std=c++17", "-fPIC", "-O2", "-I" + os.path.join(ROOT, "include"),
+                cmd = [HIPCC, "-std=c++17", "-fPIC", "-O2", "--offload-arch=gfx950", "-I" + os.path.join(ROOT, "include"),
                        "-I" + CSRC, "-c", "-o", obj, src]
             jobs_list.append(cmd)
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
@@ -69,7 +69,7 @@ def build(jobs=8, force=False, verbose=False):
     os.replace(tmp, LIB)
     # C++ driver over the host classes (links the library in place, rpath $ORIGIN)
     drv = os.path.join(HERE, "tarang_run")
-    _run([HIPCC, "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+    _run([HIPCC, "-std=c++17", "-O2", "--offload-arch=gfx950", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
           "-o", drv + ".tmp", os.path.join(CSRC, "tarang_run.cpp"), "-L" + HERE, "-ltarang_hip",
           "-Wl,-rpath,$ORIGIN"])
     os.replace(drv + ".tmp", drv)

@@ -152,6 +152,16 @@ public:
     void Output_global();
     void Output_shell_spectrum();
     void Output_flux();
+    using IncVF::Compute_flux;
+    // Incompressible MHD with the magnetic field W (W.dissipation_coefficient = eta), the IMHD program's
+    // step and outputs (src/IMHD, output_main.cc:311-351):
+    void Time_advance(IncVF& W);                     // time_advance_VF_MHD.cc:59-180 (fused or plain)
+    double Get_dt(IncVF& W);                         // compute_dt.cc:75-101
+    void Compute_flux(IncVF& W);                     // compute_flux.cc:148-262 -> flux_mhd
+    void Output_global(IncVF& W);                    // output_energy.cc:217-286
+    void Output_shell_spectrum(IncVF& W);            // output_energy.cc:463-500
+    void Output_flux(IncVF& W);                      // output_ET.cc:118-165
+    std::vector<double> flux_mhd;                    // 9 x (no_spheres + 1), the rows of th_flux_mhd
     std::ofstream global_file, spectrum_file, flux_file;
     double Tnow, Tdt, Tdt_fixed;
     std::string integ_scheme;
@@ -166,6 +176,7 @@ private:
     bool has_force_ = false;       // Set_force was called (fused or plain)
     th_force force_{};
     th_state* state_ = nullptr;
+    th_state* mstate_ = nullptr;   // fused MHD state (U, B), created by the first Time_advance(W)
     bool state_fresh_ = false;
 };
 

@@ -3,14 +3,17 @@
 // energy + NaN check, then the shell spectrum and (at the end) the energy flux.
 //
 //   tarang_run N steps dt nu [tg|random|file=<field_in.d>] [FP64|FP32_MIXED] [fused|plain] [save=<field_out.d>]
-//              [out=DIR] [force=PROC,INNER,OUTER,P3,P4]
+//              [out=DIR] [force=PROC,INNER,OUTER,P3,P4] [mhd=ETA]
 // Output (stdout): "glob t E D" per step, then "spectrum k E(k)" lines and "flux R Pi(R)" lines.
 // file=: Init_cond from a field file (field_input_proc = 1; its "%% Time" sets Tinit); save=: the
 // final field is appended to a field file (Output_field), the restart point of a later run.
 // out=DIR (any position after nu): DIR/out/glob.d every step, DIR/out/spectrum.d and flux.d at the end,
 // in the reference's formats (Output_global, Output_shell_spectrum, Output_flux).
 // force=PROC,INNER,OUTER,P3,P4: force_field_proc 1 (energy/helicity supply) or 2 (constant energy/helicity)
-// with its force_double_para, recomputed at every stage on the plain stepper.
+// with its force_double_para, recomputed at every stage.
+// mhd=ETA: incompressible MHD (the IMHD program, src/IMHD/IMHD_main.cc) with magnetic diffusivity ETA; U
+// from the IC choice, B random-phase (seed 2); Get_dt(W), Time_advance(W) and the W outputs; "glob t Eu Eb"
+// lines, the spectrum of U and the kinetic flux.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -23,7 +26,7 @@ int main(int argc, char** argv)
 {
     if (argc < 5) {
         fprintf(stderr, "usage: %s N steps dt nu [tg|random|file=PATH] [FP64|FP32_MIXED] [fused|plain] [save=PATH] "
-                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4]\n",
+                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4] [mhd=ETA]\n",
                 argv[0]);
         return 2;
     }
@@ -35,9 +38,11 @@ int main(int argc, char** argv)
     std::string save, outdir;
     int fproc = 0;
     double fpar[4] = {0, 0, 0, 0};
+    double eta = -1;
     for (int a = 5; a < argc; a++) {
         if (strncmp(argv[a], "save=", 5) == 0) save = argv[a] + 5;
         if (strncmp(argv[a], "out=", 4) == 0) outdir = argv[a] + 4;
+        if (strncmp(argv[a], "mhd=", 4) == 0) eta = atof(argv[a] + 4);
         if (strncmp(argv[a], "force=", 6) == 0 &&
             sscanf(argv[a] + 6, "%d,%lf,%lf,%lf,%lf", &fproc, &fpar[0], &fpar[1], &fpar[2], &fpar[3]) != 5) {
             fprintf(stderr, "tarang_run: force=PROC,INNER,OUTER,P3,P4 expected\n");
@@ -59,6 +64,36 @@ int main(int argc, char** argv)
         }
         if (fproc) U.Set_force(fproc, fpar[0], fpar[1], fpar[2], fpar[3]);
         if (!outdir.empty()) U.Open_output_files(outdir);
+        if (eta >= 0) {
+            if (fproc) {
+                fprintf(stderr, "tarang_run: forcing of MHD runs is not supported\n");
+                return 2;
+            }
+            tarang::IncVF W(ctx, g, eta, N / 3 + 2, 1);
+            void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+            tarang::check(th_ic_random(W.plan(), B, 2, 0.5, 4.0), "th_ic_random");
+            for (int it = 0; it < steps; it++) {
+                U.Tdt = U.Get_dt(W);
+                U.Tnow += U.Tdt;
+                U.Time_advance(W);
+                U.CV_Compute_totalenergy_diss();
+                W.CV_Compute_totalenergy_diss();
+                if (isnan(U.CV_total_energy) || isnan(W.CV_total_energy)) {
+                    printf("ERROR: Numerical Overflow\n");
+                    return 1;
+                }
+                printf("glob %.6f %.15e %.15e\n", U.Tnow, U.CV_total_energy, W.CV_total_energy);
+                if (!outdir.empty()) U.Output_global(W);
+            }
+            std::vector<double> Ek;
+            U.CV_Compute_shell_spectrum(Ek);
+            for (size_t k = 0; k < Ek.size(); k++) printf("spectrum %zu %.15e\n", k, Ek[k]);
+            if (!outdir.empty()) {
+                U.Output_shell_spectrum(W);
+                U.Output_flux(W);
+            }
+            return 0;
+        }
         for (int it = 0; it < steps; it++) {
             U.Tdt = U.Get_dt();
             U.Tnow += U.Tdt;

@@ -1,5 +1,6 @@
 // th_host.cpp -- the C++ host classes of tarang_host.hpp over the C-ABI.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <math.h>
 #include <string.h>
 #include <string>
@@ -16,6 +17,24 @@ static void hipc(hipError_t e, const char* what)
 {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
+
+namespace {
+// real part of the k = 0 mode of each component (the mean field)
+void mean_modes(void* const V[3], size_t esize, double out[3])
+{
+    for (int i = 0; i < 3; i++) {
+        double z[2] = {0, 0};
+        float zf[2] = {0, 0};
+        if (esize == 16) {
+            hipc(hipMemcpy(z, V[i], 16, hipMemcpyDeviceToHost), "mean mode");
+        } else {
+            hipc(hipMemcpy(zf, V[i], 8, hipMemcpyDeviceToHost), "mean mode");
+            z[0] = zf[0];
+        }
+        out[i] = z[0];
+    }
+}
+}  // namespace
 
 Context::Context(int device) { check(th_ctx_create(&h, device, nullptr), "th_ctx_create"); }
 Context::~Context() { th_ctx_destroy(h); }
@@ -239,6 +258,7 @@ IncFluid::IncFluid(Context& ctx, const Grid& g, double nu, double Tinit, double 
 IncFluid::~IncFluid()
 {
     if (state_) th_state_destroy(state_);
+    if (mstate_) th_state_destroy(mstate_);
 }
 void IncFluid::Compute_rhs()
 {
@@ -282,16 +302,7 @@ void IncFluid::Output_global()
     check(th_helicity(plan_, V, h), "CV_Compute_total_helicity");
     check(th_entropy(plan_, V, &entropy), "CV_Compute_entropy");
     double mean[3];
-    for (int i = 0; i < 3; i++) {
-        double z[2] = {0, 0};
-        float zf[2] = {0, 0};
-        if (esize_ == 16) hipc(hipMemcpy(z, V[i], 16, hipMemcpyDeviceToHost), "mean mode");
-        else {
-            hipc(hipMemcpy(zf, V[i], 8, hipMemcpyDeviceToHost), "mean mode");
-            z[0] = zf[0];
-        }
-        mean[i] = z[0];
-    }
+    mean_modes(V, esize_, mean);
     const double kmax = th_max_radius_inside(plan_);
     const double total_diss = dissipation_coefficient * CV_total_dissipation;
     const double kolm_scale_u = sqrt(sqrt((pow(dissipation_coefficient, 3) / total_diss)));
@@ -351,6 +362,132 @@ void IncFluid::Output_flux()
     for (int i = 1; i <= no_spheres; i++)
         flux_file << i << " " << sphere_radius[i] << " " << flux_self[i] << "  " << (i < (int)feed.size() ? feed[i] : 0.0)
                   << "     " << std::endl;
+    flux_file << std::endl;
+}
+
+// ---------------------------------------------------------------- IncFluid with W (MHD)
+
+double IncFluid::Get_dt(IncVF& W)
+{
+    CV_Compute_totalenergy_diss();
+    W.CV_Compute_totalenergy_diss();
+    const double kmax = th_max_radius_inside(plan_);
+    const double dx = (2 * M_PI) / kmax;
+    double dt = dx / (sqrt(CV_total_energy + W.CV_total_energy) * 20);
+    void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+    double b0[3];
+    mean_modes(B, esize_, b0);
+    const double b0sqr = b0[0] * b0[0] + b0[1] * b0[1] + b0[2] * b0[2];
+    if (b0sqr > 1e-15) dt = std::min(dt, dx / (sqrt(b0sqr) * 20));
+    return std::min(dt, Tdt_fixed);
+}
+
+void IncFluid::Time_advance(IncVF& W)
+{
+    th_phys ph{dissipation_coefficient, W.dissipation_coefficient, 0, 0, 0};
+    void* f[6] = {V1.data(), V2.data(), V3.data(), W.V1.data(), W.V2.data(), W.V3.data()};
+    if (fused_) {
+        if (!mstate_) check(th_state_create(plan_, &mstate_, TH_MHD), "th_state_create");
+        for (int i = 0; i < 6; i++) check(th_state_upload(mstate_, i, f[i], 0), "th_state_upload");
+        check(th_step(plan_, mstate_, Tdt, &ph, 1), "th_step");
+        for (int i = 0; i < 6; i++) check(th_state_download(mstate_, i, f[i], 0), "th_state_download");
+        check(th_sync(plan_), "th_sync");
+        return;
+    }
+    const int sch = integ_scheme == "EULER" ? 0 : integ_scheme == "RK2" ? 1 : 2;
+    check(th_step_plain(plan_, TH_MHD, sch, f, Tdt, &ph), "th_step_plain");
+    check(th_sync(plan_), "th_sync");
+}
+
+void IncFluid::Compute_flux(IncVF& W)
+{
+    void* U[3] = {V1.data(), V2.data(), V3.data()};
+    void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+    flux_mhd.assign((size_t)9 * (no_spheres + 1), 0.0);
+    check(th_flux_mhd(plan_, U, B, sphere_radius.data(), no_spheres, flux_mhd.data()), "Compute_flux(W)");
+}
+
+void IncFluid::Output_global(IncVF& W)
+{
+    CV_Compute_totalenergy_diss();
+    W.CV_Compute_totalenergy_diss();
+    void* U[3] = {V1.data(), V2.data(), V3.data()};
+    void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+    double Hc = 0.0;
+    for (int c = 0; c < 3; c++) {        // Get_cross_helicity hc.cc:48-53
+        double e;
+        check(th_energy(plan_, U[c], B[c], &e), "Get_cross_helicity");
+        Hc += e;
+    }
+    double hu[4], hw[4], su, sw;
+    check(th_helicity(plan_, U, hu), "CV_Compute_total_helicity");
+    check(th_helicity(plan_, B, hw), "W.CV_Compute_total_helicity");
+    check(th_entropy(plan_, U, &su), "CV_Compute_entropy");
+    check(th_entropy(plan_, B, &sw), "W.CV_Compute_entropy");
+    double mu[3], mb[3];
+    mean_modes(U, esize_, mu);
+    mean_modes(B, esize_, mb);
+    const double nu = dissipation_coefficient, eta = W.dissipation_coefficient;
+    const double kmax = th_max_radius_inside(plan_);
+    const double total_diss = nu * CV_total_dissipation;
+    const double kmax_eta1 = kmax * sqrt(sqrt(pow(nu, 3) / total_diss));
+    const double tempvar = nu / eta;
+    const double kmax_eta2 = kmax_eta1 * sqrt(sqrt(tempvar)) / tempvar;
+    const double Rlambda = 2 * CV_total_energy * sqrt(15 / (nu * total_diss));
+    global_file << Tnow << "\t" << CV_total_energy << " " << W.CV_total_energy << "   " << CV_total_dissipation << " "
+                << W.CV_total_dissipation << " " << total_diss << " " << eta * W.CV_total_dissipation << "    " << Hc
+                << "    " << hu[0] << " " << hu[1] << " " << hw[0] << " " << hw[1] << "    " << hu[2] << " " << hu[3]
+                << "    " << hw[2] << " " << hw[3] << "    " << su << " " << sw << "   " << kmax_eta1 << " " << kmax_eta2
+                << "    " << nu << " " << eta << "    " << Rlambda << "    " << Tdt << "    " << mu[0] << " " << mu[1]
+                << " " << mu[2] << "\t" << mb[0] << " " << mb[1] << " " << mb[2] << std::endl;
+}
+
+// Columns as output_energy.cc:478-497.  shell_ek_cross is printed but never filled on this path in the
+// reference (Compute_cross_helicity_shell_spectrum hc.cc:67-71 has no caller), so it is 0, and so are the
+// force spectra of an unforced MHD run.
+void IncFluid::Output_shell_spectrum(IncVF& W)
+{
+    spectrum_file << "%% Time = " << Tnow << std::endl;
+    const int n = shell_ek_size();
+    void* U[3] = {V1.data(), V2.data(), V3.data()};
+    void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+    std::vector<double> eu[3], ew[3], hu[3], hw[3], du(n, 0.0), dw(n, 0.0), t(n);
+    for (int c = 0; c < 3; c++) {
+        eu[c].resize(n);
+        ew[c].resize(n);
+        hu[c].resize(n);
+        hw[c].resize(n);
+        check(th_shell_spectrum(plan_, U[c], nullptr, 0, eu[c].data(), nullptr), "CV_Compute_shell_spectrum");
+        check(th_shell_spectrum(plan_, B[c], nullptr, 0, ew[c].data(), nullptr), "W.CV_Compute_shell_spectrum");
+        check(th_shell_spectrum(plan_, U[c], nullptr, 2, t.data(), nullptr), "CV_Compute_shell_spectrum");
+        for (int i = 0; i < n; i++) du[i] += 2.0 * t[i];
+        check(th_shell_spectrum(plan_, B[c], nullptr, 2, t.data(), nullptr), "W.CV_Compute_shell_spectrum");
+        for (int i = 0; i < n; i++) dw[i] += 2.0 * t[i];
+    }
+    check(th_shell_spectrum_helicity(plan_, U, hu[0].data(), hu[1].data(), hu[2].data()), "CV_Compute_shell_spectrum_helicity");
+    check(th_shell_spectrum_helicity(plan_, B, hw[0].data(), hw[1].data(), hw[2].data()), "W.CV_Compute_shell_spectrum_helicity");
+    const double nu = dissipation_coefficient, eta = W.dissipation_coefficient;
+    for (int i = 0; i < n; i++)
+        spectrum_file << i << "\t" << eu[0][i] << " " << eu[1][i] << " " << eu[2][i] << "\t" << ew[0][i] << " " << ew[1][i]
+                      << " " << ew[2][i] << "\t" << nu * du[i] << " " << eta * dw[i] << "    " << 0.0 << "    " << hu[0][i]
+                      << " " << hu[1][i] << " " << hu[2][i] << "\t" << hw[0][i] << " " << hw[1][i] << " " << hw[2][i]
+                      << "\t" << 0.0 << " " << 0.0 << std::endl;
+    spectrum_file << std::endl;
+}
+
+// Columns as output_ET.cc:139-156: flux_self, flux_VF_in_out, flux_VF_in_in of U, then of W and
+// W.flux_VF_out_out, flux_Elsasser of U and W, the force feeds.  Compute_flux(W) stores VF_out_out in U
+// (compute_flux.cc:232), so W's printed value stays 0; the force feeds of an unforced run are 0.
+void IncFluid::Output_flux(IncVF& W)
+{
+    flux_file << "%% Time = " << Tnow << std::endl;
+    Compute_flux(W);
+    const size_t s = (size_t)no_spheres + 1;
+    auto F = [&](int row, int i) { return flux_mhd[row * s + i]; };
+    for (int i = 1; i <= no_spheres; i++)
+        flux_file << i << " " << sphere_radius[i] << " " << F(0, i) << " " << F(2, i) << " " << F(3, i) << "   "
+                  << F(1, i) << " " << F(4, i) << " " << F(5, i) << " " << 0.0 << "    " << F(7, i) << " " << F(8, i)
+                  << "    " << 0.0 << " " << 0.0 << "    " << std::endl;
     flux_file << std::endl;
 }
 

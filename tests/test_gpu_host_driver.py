@@ -170,3 +170,72 @@ def test_driver_forced_output_columns(tmp_path):
     for r in frows:
         i, R, pi, force = r.split()
         assert abs(float(force) - feed[int(i)]) <= 1e-10 * np.abs(feed).max(), r
+
+
+def test_driver_mhd_output_files_match_oracle(tmp_path):
+    """mhd=ETA: Get_dt(W), Time_advance(W) (fused MHD step) and Output_global(W) / Output_shell_spectrum(W) /
+    Output_flux(W) (output_energy.cc:217-286, 463-500, output_ET.cc:118-165) in the reference's column
+    layout, every number equal to the oracle's value of the same quantity to the printed digits.  The
+    cross-helicity spectrum, W.flux_VF_out_out and the force columns are printed as 0, as the reference
+    leaves them on this path."""
+    N, steps, dt, nu, eta = 32, 3, 1e-3, 0.01, 0.02
+    (tmp_path / "out").mkdir()
+    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"out={tmp_path}", f"mhd={eta}")
+    o = Oracle((N, N, N))
+    U, B = o.ic_random(1), o.ic_random(2)
+    o.rk4_mhd(U, B, nu, eta, dt, steps=steps)
+    Eu, Eb = o.total_energy(U), o.total_energy(B)
+    assert abs(glob[-1][1] - Eu) <= 1e-12 * Eu and abs(glob[-1][2] - Eb) <= 1e-12 * Eb
+
+    def close(a, b, tol=1e-10, sc=None):
+        return abs(a - b) <= tol * (sc if sc is not None else max(abs(b), 1e-300))
+
+    line = (tmp_path / "out" / "glob.d").read_text().splitlines()[-1]
+    t, mid, mb = line.split("\t")
+    assert mid.count("    ") == 9 and mb.split(" ") == ["0", "0", "0"]
+    v = [float(x) for x in mid.split()]
+    assert len(v) == 26
+    Du, Db = o.total_dissipation(U), o.total_dissipation(B)
+    Hc = sum(o.energy(U[c], B[c]) for c in range(3))
+    hu, hb = o.helicity(U), o.helicity(B)
+    kmax = o.max_radius_inside()
+    ke1 = kmax * (nu ** 3 / (nu * Du)) ** 0.25
+    ref = [Eu, Eb, Du, Db, nu * Du, eta * Db, Hc, hu[0], hu[1], hb[0], hb[1], hu[2], hu[3], hb[2], hb[3],
+           o.entropy(U), o.entropy(B), ke1, ke1 * (nu / eta) ** 0.25 / (nu / eta), nu, eta,
+           2 * Eu * (15 / (nu * nu * Du)) ** 0.5, dt, 0.0, 0.0, 0.0]
+    for i, (a, b) in enumerate(zip(v, ref)):
+        assert close(a, b, sc=max(abs(b), 1e-12 * max(abs(x) for x in ref[7:15]))), (i, a, b)
+    assert close(float(t), steps * dt)
+
+    rows = [r for r in (tmp_path / "out" / "spectrum.d").read_text().split("\n")[1:] if r.strip()]
+    eu = [o.shell_spectrum(U[c])[0] for c in range(3)]
+    eb = [o.shell_spectrum(B[c])[0] for c in range(3)]
+    du = 2 * sum(o.shell_spectrum(U[c], n=2)[0] for c in range(3))
+    db = 2 * sum(o.shell_spectrum(B[c], n=2)[0] for c in range(3))
+    hku, hkb = o.shell_helicity(U), o.shell_helicity(B)
+    assert len(rows) == len(eu[0])
+    for i, r in enumerate(rows):
+        f = r.split("\t")
+        assert int(f[0]) == i and len(f) == 6 and f[5] == "0 0"
+        got = [float(x) for x in " ".join(f[1:5]).split()]
+        ref = [eu[0][i], eu[1][i], eu[2][i], eb[0][i], eb[1][i], eb[2][i], nu * du[i], eta * db[i], 0.0,
+               hku[0][i], hku[1][i], hku[2][i], hkb[0][i], hkb[1][i], hkb[2][i]]
+        scale = [max(np.abs(eu).max(), 1e-300)] * 3 + [np.abs(eb).max()] * 3 + [nu * du.max(), eta * db.max(), 1.0] + \
+                [np.abs(hku).max()] * 3 + [np.abs(hkb).max()] * 3
+        for j, (a, b, sc) in enumerate(zip(got, ref, scale)):
+            assert abs(a - b) <= 1e-10 * sc, (i, j, a, b)
+
+    radii = np.concatenate([np.arange(0, N // 3 + 2, dtype=float), [10000.0]])
+    fl = o.flux_mhd([a.copy() for a in U], [a.copy() for a in B], radii, elsasser=True)
+    frows = [r for r in (tmp_path / "out" / "flux.d").read_text().split("\n")[1:] if r.strip()]
+    assert len(frows) == len(radii) - 1
+    order = ["u2u", "u2b_in_out", "u2b_in_in", "b2b", "b2u_in_out", "b2u_in_in", None, "zp", "zm", None, None]
+    for r in frows:
+        x = r.split()
+        i = int(x[0])
+        assert float(x[1]) == radii[i] and len(x) == 13
+        for tok, name in zip(x[2:], order):
+            if name is None:
+                assert float(tok) == 0.0
+            else:
+                assert abs(float(tok) - fl[name][i]) <= 1e-10 * max(np.abs(fl[name]).max(), 1e-300), (r, name)
