@@ -1,6 +1,7 @@
 // th_host.cpp -- the C++ host classes of tarang_host.hpp over the C-ABI.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <stdexcept>
 #include <math.h>
 #include <string.h>
 #include <string>
@@ -384,6 +385,7 @@ double IncFluid::Get_dt(IncVF& W)
 
 void IncFluid::Time_advance(IncVF& W)
 {
+    if (has_force_) throw std::runtime_error("Time_advance(W): forcing of MHD runs is not supported");
     th_phys ph{dissipation_coefficient, W.dissipation_coefficient, 0, 0, 0};
     void* f[6] = {V1.data(), V2.data(), V3.data(), W.V1.data(), W.V2.data(), W.V3.data()};
     if (fused_) {
