@@ -140,10 +140,13 @@ int th_compute_force(th_plan *plan, void *const V[3], void *F[3], const th_force
  * th_flux_ns  == IncVF::Compute_flux() compute_flux.cc:52-69 (sphere radii[1..nsph])
  * th_shell_tr_ns == IncVF::Compute_shell_tr() compute_shell_tr.cc:59-79, T is (nsh+1)^2 row-major
  * th_nusselt  == IncVF::Get_Nusselt_no nusselt.cc:53-89 (PRLARGE / ULARGE)
- * th_helicity == Compute_total_helicity_FOUR four_energy.cc:453-548: out = {H1, H2, DH1, DH2} (FOUR)
- * th_shell_spectrum_helicity == Compute_shell_spectrum_helicity_FOUR four_energy.cc:551-671: H1k1..3
- *                of shell_ek_size each (K_c (Vr x Vi)_c parts, partial-shell rescale) (FOUR)
- * th_entropy  == Get_entropy_FOUR four_energy.cc:687-750, spectral entropy in bits (FOUR) */
+ * th_helicity == Compute_total_helicity_FOUR four_energy.cc:453-548 / _SCFT scft_energy.cc:448-543 (V1 taken
+ *                times -i): out = {H1, H2, DH1, DH2}
+ * th_shell_spectrum_helicity == Compute_shell_spectrum_helicity_FOUR four_energy.cc:551-671 / _SCFT
+ *                scft_energy.cc:546-677: H1k1..3 of shell_ek_size each (K_c (Vr x Vi)_c parts,
+ *                partial-shell rescale)
+ * th_entropy  == Get_entropy_FOUR four_energy.cc:687-750 / Get_entropy_SCFT scft_energy.cc:679-748,
+ *                spectral entropy in bits */
 int th_energy(th_plan *plan, const void *A, const void *B, double *out);
 int th_Sn(th_plan *plan, const void *A, double n, double *out);
 int th_shell_spectrum(th_plan *plan, const void *A, const void *B, int n, double *Sk, double *count);

@@ -1489,11 +1489,14 @@ void orc_shell_tr_scalar(const orc_grid *g, double *const V[3], const double *T,
     free(G); free(N); free(row);
 }
 
-/* ---- helicity and spectral entropy (FOUR basis), the glob.d / spectrum.d columns ---- */
-static void vr_cross_vi(double *const V[3], size_t i, double c[3])
+/* ---- helicity and spectral entropy (FOUR and SCFT), the glob.d / spectrum.d columns ---- */
+/* Vr x Vi of mode i; on SCFT boxes V1 is first multiplied by -i, its sin -> Fourier conversion
+ * (Compute_local_helicity_SCFT scft_energy.cc:448-500) */
+static void vr_cross_vi_b(const orc_grid *g, double *const V[3], size_t i, double c[3])
 {
     double r0 = RE(V[0], i), r1 = RE(V[1], i), r2 = RE(V[2], i);
     double m0 = IM(V[0], i), m1 = IM(V[1], i), m2 = IM(V[2], i);
+    if (g->basis == 1) { double t = r0; r0 = m0; m0 = -t; }
     c[0] = r1 * m2 - r2 * m1;
     c[1] = r2 * m0 - r0 * m2;
     c[2] = r0 * m1 - r1 * m0;
@@ -1637,7 +1640,7 @@ void orc_cylinder_ring_spectrum(const orc_grid *g, double *const V[3], int nslab
     free(arr);
 }
 
-/* Compute_local_helicity_FOUR four_energy.cc:453-506: over |K| <= Min_radius_outside,
+/* Compute_local_helicity_FOUR four_energy.cc:453-506 (SCFT scft_energy.cc:448-500, V1 times -i): over |K| <= Min_radius_outside,
  * modal = 2 mult K.(Vr x Vi); out = {H1 = sum modal, H2 = sum modal / K^2 (K^2 > MYEPS = 1e-15),
  * DH1 = 2 sum K^2 modal, DH2 = 2 H1} */
 void orc_helicity(const orc_grid *g, double *const V[3], double *out)
@@ -1650,7 +1653,7 @@ void orc_helicity(const orc_grid *g, double *const V[3], double *out)
             for (int l3 = 0; l3 < H; l3++) {
                 double km = orc_kmag(g, l1, l2, l3);
                 if (km > kkmax) continue;
-                vr_cross_vi(V, IDX(g, l1, l2, l3), c);
+                vr_cross_vi_b(g, V, IDX(g, l1, l2, l3), c);
                 double kx = orc_kx(g, l1) * g->f1, ky = orc_ky(g, l2) * g->f2, kz = l3 * g->f3;
                 double ksq = km * km;
                 double modal = 2 * orc_multiplicity(g, l1, l2, l3) * (kx * c[0] + ky * c[1] + kz * c[2]);
@@ -1661,7 +1664,8 @@ void orc_helicity(const orc_grid *g, double *const V[3], double *out)
     out[0] = h1; out[1] = h2; out[2] = 2.0 * d1; out[3] = 2.0 * h1;
 }
 
-/* Compute_shell_spectrum_helicity_FOUR four_energy.cc:551-671: bin ceil(|K|) <= Min_radius_outside,
+/* Compute_shell_spectrum_helicity_FOUR four_energy.cc:551-671 (SCFT scft_energy.cc:546-677: V1 times -i, count
+ * factor/2): bin ceil(|K|) <= Min_radius_outside,
  * H1k_c += 2 mult K_c (Vr x Vi)_c, counted with 2 mult; shells kmax_inside < k <= kkmax with count >= 1
  * rescaled by Approx_number_modes_in_shell / count.  Arrays of shell_ek_size. */
 void orc_shell_helicity(const orc_grid *g, double *const V[3], double *Hk1, double *Hk2, double *Hk3)
@@ -1676,11 +1680,11 @@ void orc_shell_helicity(const orc_grid *g, double *const V[3], double *Hk1, doub
                 int index = (int)ceil(orc_kmag(g, l1, l2, l3));
                 if (index > kkmax) continue;
                 double f = 2 * orc_multiplicity(g, l1, l2, l3);
-                vr_cross_vi(V, IDX(g, l1, l2, l3), c);
+                vr_cross_vi_b(g, V, IDX(g, l1, l2, l3), c);
                 Hk1[index] += f * (orc_kx(g, l1) * g->f1 * c[0]);
                 Hk2[index] += f * (orc_ky(g, l2) * g->f2 * c[1]);
                 Hk3[index] += f * (l3 * g->f3 * c[2]);
-                cnt[index] += f;
+                cnt[index] += g->basis == 1 ? f / 2 : f;      /* SCFT counts factor/2 */
             }
     for (int index = kin + 1; index <= kkmax; index++)
         if (cnt[index] >= 1) {
@@ -1691,7 +1695,9 @@ void orc_shell_helicity(const orc_grid *g, double *const V[3], double *Hk1, doub
 }
 
 /* Get_local_entropy_FOUR four_energy.cc:687-732: p = (sum_c |A_c|^2 / 2) / E (E = Get_total_energy_FOUR of
- * the three components), entropy = sum over kz <= N3/2 of 2 mult (-p log2 p) for p > MYEPS */
+ * the three components), entropy = sum over kz <= N3/2 of 2 mult (-p log2 p) for p > MYEPS.
+ * SCFT (Get_local_entropy_SCFT scft_energy.cc:679-723): Modal_energy_SCFT is |A|^2 for kx > 0, and the
+ * weight is mult (not 2 mult). */
 double orc_entropy(const orc_grid *g, double *const V[3])
 {
     int N1 = g->N1, N2 = g->N2, maxN3 = g->N3 > 2 ? g->N3 / 2 : 0;
@@ -1703,8 +1709,10 @@ double orc_entropy(const orc_grid *g, double *const V[3])
                 size_t i = IDX(g, l1, l2, l3);
                 double me = 0;
                 for (int c = 0; c < 3; c++) me += (RE(V[c], i) * RE(V[c], i) + IM(V[c], i) * IM(V[c], i)) / 2;
+                if (g->basis == 1 && l1 > 0) me *= 2;       /* Modal_energy_SCFT scft_inline.h:347-354 */
                 double p = me / E;
-                if (p > 1e-15) S += 2 * orc_multiplicity(g, l1, l2, l3) * (-p * log(p) / log(2.0));
+                double w = (g->basis == 1 ? 1 : 2) * orc_multiplicity(g, l1, l2, l3);
+                if (p > 1e-15) S += w * (-p * log(p) / log(2.0));
             }
     return S;
 }

@@ -905,7 +905,6 @@ extern "C" int th_helicity(th_plan* p, void* const V[3], double* out)
 {
     TRY(check_plan(p));
     if (!V || !out) { set_error("th_helicity: null"); return E_ARG; }
-    if (p->g.basis != TH_FOUR) { set_error("th_helicity: FOUR basis only"); return E_UNSUPPORTED; }
     return DISPATCH(p, do_helicity, p, V, out);
 }
 template <typename R> static int do_entropy(th_plan* p, void* const V[3], double* out)
@@ -917,7 +916,6 @@ extern "C" int th_entropy(th_plan* p, void* const V[3], double* out)
 {
     TRY(check_plan(p));
     if (!V || !out) { set_error("th_entropy: null"); return E_ARG; }
-    if (p->g.basis != TH_FOUR) { set_error("th_entropy: FOUR basis only"); return E_UNSUPPORTED; }
     return DISPATCH(p, do_entropy, p, V, out);
 }
 template <typename R> static int do_shell_hel(th_plan* p, void* const V[3], double* H1, double* H2, double* H3)
@@ -929,7 +927,6 @@ extern "C" int th_shell_spectrum_helicity(th_plan* p, void* const V[3], double* 
 {
     TRY(check_plan(p));
     if (!V || !H1k1 || !H1k2 || !H1k3) { set_error("th_shell_spectrum_helicity: null"); return E_ARG; }
-    if (p->g.basis != TH_FOUR) { set_error("th_shell_spectrum_helicity: FOUR basis only"); return E_UNSUPPORTED; }
     return DISPATCH(p, do_shell_hel, p, V, H1k1, H1k2, H1k3);
 }
 template <typename R> static int do_aniso(th_plan* p, void* const V[3], int mode, int nbin, int n, int axis, double* S1,

@@ -191,6 +191,12 @@ __global__ __launch_bounds__(RBLK) void k_scalar(GeoR g, const cplx<R>* __restri
 //           modal = 2 mult K.(Vr x Vi); values {sum modal, sum modal/K^2 (K^2 > 1e-15), sum K^2 modal}
 //   MODE 1: Get_local_entropy_FOUR four_energy.cc:687-732: p = (sum_c |V_c|^2 / 2) / E,
 //           value sum 2 mult (-p log2 p) over p > 1e-15
+// SCFT boxes hold V1 in the sin basis along x; -i V1 is its Fourier counterpart (scft_energy.cc:470-474)
+template <typename R> __device__ __forceinline__ cplx<R> rot_sin(const GeoR& g, const cplx<R>& a)
+{
+    return g.basis == 1 ? mk<R>(a.y, -a.x) : a;
+}
+__device__ __forceinline__ int kx_r(const GeoR& g, int l1) { return g.basis == 1 ? l1 : kx_four(l1, g.N1); }
 template <typename R> __device__ __forceinline__ void vr_x_vi(const cplx<R>& a, const cplx<R>& b, const cplx<R>& c,
                                                              double* x)
 {
@@ -214,16 +220,18 @@ __global__ __launch_bounds__(RBLK) void k_modal(GeoR g, const cplx<R>* __restric
             double km = kmag_r(g, l1, l2, l3);
             if (km > (double)kkmax) continue;
             double x[3];
-            vr_x_vi(A0[i], A1[i], A2[i], x);
-            const double kx = kx_four(l1, g.N1) * g.f1, ky = ky_of(l2, g.N2) * g.f2, kz = l3 * g.f3;
+            vr_x_vi(rot_sin(g, A0[i]), A1[i], A2[i], x);
+            const double kx = kx_r(g, l1) * g.f1, ky = ky_of(l2, g.N2) * g.f2, kz = l3 * g.f3;
             const double ksq = km * km;
             const double modal = 2 * mult_r(g, l1, l2, l3) * (kx * x[0] + ky * x[1] + kz * x[2]);
             acc[0] += modal;
             if (ksq > 1e-15) acc[NV > 1 ? 1 : 0] += modal / ksq;
             acc[NV > 2 ? 2 : 0] += ksq * modal;
         } else {
-            const double p = 0.5 * (sq(A0[i]) + sq(A1[i]) + sq(A2[i])) / E;
-            if (p > 1e-15) acc[0] += 2 * mult_r(g, l1, l2, l3) * (-p * log(p) / log(2.0));
+            // Modal_energy_SCFT (scft_inline.h:347-354) is |A|^2 for kx > 0; SCFT weighs by mult, FOUR by 2 mult
+            const double me = 0.5 * (sq(A0[i]) + sq(A1[i]) + sq(A2[i])) * ((g.basis == 1 && l1 > 0) ? 2.0 : 1.0);
+            const double p = me / E;
+            if (p > 1e-15) acc[0] += (g.basis == 1 ? 1.0 : 2.0) * mult_r(g, l1, l2, l3) * (-p * log(p) / log(2.0));
         }
     }
 #pragma unroll
@@ -238,7 +246,7 @@ __global__ __launch_bounds__(RBLK) void k_modal(GeoR g, const cplx<R>* __restric
     if (threadIdx.x < NV) partial[(long)blockIdx.x * NV + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Compute_local_shell_spectrum_helicity_FOUR four_energy.cc:551-599: bin ceil(K) <= kkmax,
+// Compute_local_shell_spectrum_helicity_FOUR four_energy.cc:551-599 (_SCFT scft_energy.cc:546-599): bin ceil(K) <= kkmax,
 // H1k_c += 2 mult K_c (Vr x Vi)_c and count += 2 mult; partial[block][4 nbins]
 template <typename R>
 __global__ __launch_bounds__(RBLK) void k_binned_hel(GeoR g, const cplx<R>* __restrict__ A0, const cplx<R>* __restrict__ A1,
@@ -255,12 +263,12 @@ __global__ __launch_bounds__(RBLK) void k_binned_hel(GeoR g, const cplx<R>* __re
         const int idx = (int)ceil(kmag_r(g, l1, l2, l3));
         if (idx > kkmax || idx >= nbins) continue;
         double x[3];
-        vr_x_vi(A0[i], A1[i], A2[i], x);
+        vr_x_vi(rot_sin(g, A0[i]), A1[i], A2[i], x);
         const double f = 2 * mult_r(g, l1, l2, l3);
-        atomicAdd(&sh[idx], f * (kx_four(l1, g.N1) * g.f1 * x[0]));
+        atomicAdd(&sh[idx], f * (kx_r(g, l1) * g.f1 * x[0]));
         atomicAdd(&sh[nbins + idx], f * (ky_of(l2, g.N2) * g.f2 * x[1]));
         atomicAdd(&sh[2 * nbins + idx], f * (l3 * g.f3 * x[2]));
-        atomicAdd(&sh[3 * nbins + idx], f);
+        atomicAdd(&sh[3 * nbins + idx], g.basis == 1 ? f / 2 : f);      // SCFT counts factor/2
     }
     __syncthreads();
     for (int b = threadIdx.x; b < 4 * nbins; b += RBLK) partial[(long)blockIdx.x * 4 * nbins + b] = sh[b];
@@ -560,7 +568,7 @@ template <typename R> int reduce_helicity(const Geo& g, const cplx<R>* const V[3
     return OK;
 }
 
-// Get_entropy_FOUR with E = the three components' Get_total_energy_FOUR
+// Get_entropy_FOUR / _SCFT with E = the three components' Get_total_energy
 template <typename R> int reduce_entropy(const Geo& g, const cplx<R>* const V[3], double* out, RedScratch& rs, hipStream_t st)
 {
     double E = 0, e;
@@ -571,7 +579,8 @@ template <typename R> int reduce_entropy(const Geo& g, const cplx<R>* const V[3]
     return run_modal<R>(1, g, V, E, out, rs, st);
 }
 
-// Compute_shell_spectrum_helicity_FOUR four_energy.cc:602-671 (with the partial-shell rescale)
+// Compute_shell_spectrum_helicity_FOUR four_energy.cc:602-671 / _SCFT scft_energy.cc:601-677 (with the
+// partial-shell rescale)
 template <typename R>
 int reduce_shell_helicity(const Geo& g, const cplx<R>* const V[3], int nbins, double* H1, double* H2, double* H3,
                           RedScratch& rs, hipStream_t st)

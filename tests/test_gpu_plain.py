@@ -321,14 +321,16 @@ def test_flux_and_shell_tr_scalar_plain(ctx, basis, prec, tol):
         assert np.abs(got - ref).max() < tol * sc, name
 
 
+@pytest.mark.parametrize("basis", ["FOUR", "SCFT"])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
-def test_helicity_and_entropy(ctx, prec, tol):
-    """th_helicity / th_shell_spectrum_helicity / th_entropy (four_energy.cc:453-750) match the oracle."""
+def test_helicity_and_entropy(ctx, basis, prec, tol):
+    """th_helicity / th_shell_spectrum_helicity / th_entropy (four_energy.cc:453-750, scft_energy.cc:448-748)
+    match the oracle."""
     from tarang_amd import Plan
     N = (16, 24, 16)
-    o = Oracle(N)
+    o = Oracle(N, basis=basis)
     V = o.ic_random(12, E0=0.5, kp=3.0)
-    p = Plan(ctx, N, precision=prec)
+    p = Plan(ctx, N, basis=basis, precision=prec)
     dV = [p.to_dev(a) for a in V]
     h, ref = p.helicity(dV), o.helicity(V)
     assert np.abs(h - ref).max() <= tol * np.abs(ref).max()

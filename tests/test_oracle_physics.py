@@ -406,3 +406,46 @@ def test_shell_helicity_sums_to_total_and_entropy_closed_form():
     assert abs(o.entropy(W) - math.log2(2 * n)) < 1e-13
     nmodes = int(np.count_nonzero(np.abs(V[0]) + np.abs(V[1]) + np.abs(V[2])))
     assert 0 < o.entropy(V) < math.log2(2 * nmodes)
+
+
+def test_scft_helicity_is_the_positive_kx_half_of_a_fourier_field():
+    """Compute_total_helicity_SCFT / Compute_shell_spectrum_helicity_SCFT (scft_energy.cc:448-677) evaluate the
+    sin/cos field through its e^{+i kx x} half: V1 sin(kx x) contributes -i V1 / 2 there and V2, V3 cos(kx x)
+    contribute V / 2.  A FOUR field holding only those +kx modes therefore has H1, H2, DH1, DH2 and the shell
+    helicities (shells <= kmax_inside) in the same ratio to the SCFT ones as its energy to the SCFT energy.
+    (With the -kx half added the doubled field is mirror-symmetric and its FOUR helicity vanishes.)"""
+    N1, N2, N3 = 16, 16, 16
+    os_ = Oracle((N1, N2, N3), basis="SCFT")
+    of = Oracle((2 * N1, N2, N3))
+    rng = np.random.default_rng(1)
+    W = os_.zeros(3)
+    for c in range(3):
+        W[c][...] = rng.standard_normal(os_.shape) + 1j * rng.standard_normal(os_.shape)
+        W[c][0] = 0
+        os_.satisfy_reality(W[c])
+        os_.dealias(W[c])
+    G, F = of.zeros(3), of.zeros(3)
+    for l1 in range(1, N1):
+        for X in (G, F):
+            X[0][l1], X[1][l1], X[2][l1] = -0.5j * W[0][l1], 0.5 * W[1][l1], 0.5 * W[2][l1]
+        F[0][2 * N1 - l1], F[1][2 * N1 - l1], F[2][2 * N1 - l1] = 0.5j * W[0][l1], 0.5 * W[1][l1], 0.5 * W[2][l1]
+    ratio = sum(of.energy(G[c]) for c in range(3)) / sum(os_.energy(W[c]) for c in range(3))
+    hs, hg = os_.helicity(W), of.helicity(G)
+    assert np.abs(hg - ratio * hs).max() < 1e-13 * np.abs(hs).max()
+    kin = os_.max_radius_inside()
+    Hs, Hg = np.asarray(os_.shell_helicity(W)), np.asarray(of.shell_helicity(G))
+    assert np.abs(Hg[:, :kin + 1] - ratio * Hs[:, :kin + 1]).max() < 1e-13 * np.abs(Hs).max()
+    assert abs(of.helicity(F)[0]) < 1e-13 * abs(hs[0])
+
+
+def test_scft_entropy_closed_form():
+    """Get_entropy_SCFT (scft_energy.cc:679-748): n stored modes with kx > 0, kz > 0 and equal |V|^2 have
+    Modal_energy_SCFT = |V|^2 and weight mult = 2, so S = -2 n p log2 p with p = |V|^2 / E."""
+    o = Oracle((16, 16, 16), basis="SCFT")
+    W = o.zeros(3)
+    n = 5
+    for j in range(n):
+        W[j % 3][j + 1, j, 1 + j] = 0.4 - 0.3j
+    E = sum(o.energy(W[c]) for c in range(3))
+    p = 0.25 / E
+    assert abs(o.entropy(W) - (-2 * n * p * math.log2(p))) < 1e-13
