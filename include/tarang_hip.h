@@ -166,6 +166,9 @@ int th_ring_spectrum(th_plan *plan, void *const V[3], int nsect, int n, int axis
 int th_cylinder_ring_spectrum(th_plan *plan, void *const V[3], int nslab, int n, int axis, double *S1k, double *S2k);
 int th_cylinder_shell_ek_size(const th_plan *plan, int axis);   /* Anis_max_Krho_radius_inside + 1 */
 int th_entropy(th_plan *plan, void *const V[3], double *out);
+/* == Get_entropy_scalar_FOUR four_energy.cc:761-826 / _SCFT scft_energy.cc:750-800 (CSF::CS_Compute_entropy
+ * Csf.cc:306-309): the spectral entropy of one field in bits */
+int th_entropy_scalar(th_plan *plan, const void *A, double *out);
 int th_fill_array_shell(th_plan *plan, const void *A, void *B, double inner, double outer);
 int th_shell_mult_single(th_plan *plan, void *const A[3], void *const B[3], double inner, double outer, double *out);
 int th_shell_mult_all(th_plan *plan, void *const A[3], void *const B[3], const double *radii, int nr, double *out);

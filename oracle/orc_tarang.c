@@ -1640,6 +1640,25 @@ void orc_cylinder_ring_spectrum(const orc_grid *g, double *const V[3], int nslab
     free(arr);
 }
 
+/* Get_local_entropy_scalar_FOUR four_energy.cc:761-798 / _SCFT scft_energy.cc:750-777: the entropy of one
+ * field, p = modal energy / Get_total_energy of that field, weights as orc_entropy */
+double orc_entropy_scalar(const orc_grid *g, const double *A)
+{
+    int N1 = g->N1, N2 = g->N2, maxN3 = g->N3 > 2 ? g->N3 / 2 : 0;
+    double E = orc_energy(g, A, NULL), S = 0;
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 <= maxN3; l3++) {
+                size_t i = IDX(g, l1, l2, l3);
+                double me = (RE(A, i) * RE(A, i) + IM(A, i) * IM(A, i)) / 2;
+                if (g->basis == 1 && l1 > 0) me *= 2;
+                double p = me / E;
+                double w = (g->basis == 1 ? 1 : 2) * orc_multiplicity(g, l1, l2, l3);
+                if (p > 1e-15) S += w * (-p * log(p) / log(2.0));
+            }
+    return S;
+}
+
 /* Compute_local_helicity_FOUR four_energy.cc:453-506 (SCFT scft_energy.cc:448-500, V1 times -i): over |K| <= Min_radius_outside,
  * modal = 2 mult K.(Vr x Vi); out = {H1 = sum modal, H2 = sum modal / K^2 (K^2 > MYEPS = 1e-15),
  * DH1 = 2 sum K^2 modal, DH2 = 2 H1} */

@@ -84,6 +84,7 @@ def lib():
             "orc_cylinder_ring_spectrum": (None, [p, pp, i, i, i, vp, vp]),
             "orc_shell_helicity": (None, [p, pp, vp, vp, vp]),
             "orc_entropy": (d, [p, pp]),
+            "orc_entropy_scalar": (d, [p, vp]),
             "orc_flux_scalar": (None, [p, pp, vp, vp, i, vp, vp]),
             "orc_shell_tr_scalar": (None, [p, pp, vp, vp, i, vp, vp]),
             "orc_nlin_rb": (None, [p, pp, vp, pp, vp]),
@@ -348,6 +349,10 @@ class Oracle:
 
     def entropy(self, V):
         return self.L.orc_entropy(self.gp, _pp(V))
+
+    def entropy_scalar(self, A):
+        """Get_entropy_scalar of one field (bits)."""
+        return self.L.orc_entropy_scalar(self.gp, _ptr(A))
 
     def et_nlin_scalar(self, H, G):
         N = self.zeros()

@@ -102,6 +102,7 @@ void   orc_cylinder_ring_spectrum(const orc_grid *g, double *const V[3], int nsl
                                   double *S2);
 void   orc_shell_helicity(const orc_grid *g, double *const V[3], double *Hk1, double *Hk2, double *Hk3);
 double orc_entropy(const orc_grid *g, double *const V[3]);
+double orc_entropy_scalar(const orc_grid *g, const double *A);
 void   orc_et_nlin_scalar(const orc_grid *g, double *const H[3], const double *G, double *nlin);
 void   orc_flux_scalar(const orc_grid *g, double *const V[3], const double *T, const double *radii, int nsph,
                        double *flux_self, double *flux_sf);

@@ -119,6 +119,7 @@ def lib():
         "th_state_download": (i, [vp, i, vp, i]),
         "th_step": (i, [vp, vp, d, C.POINTER(Phys), i]),
         "th_state_set_force": (i, [vp, C.POINTER(Force)]),
+        "th_entropy_scalar": (i, [vp, vp, dp]),
         "th_ring_spectrum": (i, [vp, pp, i, i, i, dp, dp]),
         "th_cylinder_ring_spectrum": (i, [vp, pp, i, i, i, dp, dp]),
         "th_cylinder_shell_ek_size": (i, [vp, i]),
@@ -387,6 +388,12 @@ class Plan:
                    "th_step_plain_forced")
         self._post(0, "")
         return V
+
+    def entropy_scalar(self, A):
+        out = C.c_double()
+        self._pre()
+        _check(lib().th_entropy_scalar(self.h, _ptr(A), C.byref(out)), "th_entropy_scalar")
+        return out.value
 
     def ring_spectrum(self, V, nsect, n=0, axis=3):
         """(S1k, S2k) of Compute_ring_spectrum_FOUR, shape (Max_radius_inside+1, nsect+1)."""

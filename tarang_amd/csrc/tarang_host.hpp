@@ -125,6 +125,24 @@ public:
     double dissipation_coefficient;
 };
 
+// Scalar field T (CSF / IncSF, Csf.cc, IncSF.h): the temperature of Rayleigh-Benard runs.
+class IncSF {
+public:
+    IncSF(Context& ctx, const Grid& g, double diffusion_coefficient);
+    ~IncSF();
+    IncSF(const IncSF&) = delete;
+    IncSF& operator=(const IncSF&) = delete;
+    void CS_Compute_totalenergy_diss();              // Csf.cc:293-300
+    void CS_Compute_entropy();                       // Csf.cc:306-309
+    th_plan* plan() const { return plan_; }
+    DField F;
+    double diffusion_coefficient;
+    double CS_total_energy = 0, CS_total_dissipation = 0, CS_entropy = 0;
+
+private:
+    th_plan* plan_ = nullptr;
+};
+
 class IncFluid : public IncVF {
 public:
     IncFluid(Context& ctx, const Grid& g, double nu, double Tinit, double Tdt_fixed, const std::string& scheme = "RK4",
@@ -162,6 +180,13 @@ public:
     void Output_shell_spectrum(IncVF& W);            // output_energy.cc:463-500
     void Output_flux(IncVF& W);                      // output_ET.cc:118-165
     std::vector<double> flux_mhd;                    // 9 x (no_spheres + 1), the rows of th_flux_mhd
+    // Rayleigh-Benard (SCFT, PRLARGE with ULARGE scaling: dissipation_coefficient = sqrt(Pr/Ra),
+    // T.diffusion_coefficient = 1/sqrt(Pr Ra), RB_slip_main.cc:238-239), the RB_SLIP program's step and glob.d:
+    void Time_advance(IncSF& T);                     // time_advance_SF.cc:61-233 (fused or plain)
+    double Get_dt(IncSF& T);                         // compute_dt.cc:66-69 (= Get_dt())
+    double Get_Nusselt_no(IncSF& T);                 // nusselt.cc:53-89
+    void Output_global(IncSF& T);                    // Output_global_RB output_energy.cc:156-215
+    double Rayleigh = 0, Prandtl = 0;                // Ra, Pr of the RB run
     std::ofstream global_file, spectrum_file, flux_file;
     double Tnow, Tdt, Tdt_fixed;
     std::string integ_scheme;
@@ -177,6 +202,7 @@ private:
     th_force force_{};
     th_state* state_ = nullptr;
     th_state* mstate_ = nullptr;   // fused MHD state (U, B), created by the first Time_advance(W)
+    th_state* rstate_ = nullptr;   // fused RB state (V, T), created by the first Time_advance(T)
     bool state_fresh_ = false;
 };
 

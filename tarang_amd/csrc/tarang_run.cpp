@@ -3,7 +3,7 @@
 // energy + NaN check, then the shell spectrum and (at the end) the energy flux.
 //
 //   tarang_run N steps dt nu [tg|random|file=<field_in.d>] [FP64|FP32_MIXED] [fused|plain] [save=<field_out.d>]
-//              [out=DIR] [force=PROC,INNER,OUTER,P3,P4] [mhd=ETA]
+//              [out=DIR] [force=PROC,INNER,OUTER,P3,P4] [mhd=ETA] [rb=RA,PR]
 // Output (stdout): "glob t E D" per step, then "spectrum k E(k)" lines and "flux R Pi(R)" lines.
 // file=: Init_cond from a field file (field_input_proc = 1; its "%% Time" sets Tinit); save=: the
 // final field is appended to a field file (Output_field), the restart point of a later run.
@@ -14,6 +14,10 @@
 // mhd=ETA: incompressible MHD (the IMHD program, src/IMHD/IMHD_main.cc) with magnetic diffusivity ETA; U
 // from the IC choice, B random-phase (seed 2); Get_dt(W), Time_advance(W) and the W outputs; "glob t Eu Eb"
 // lines, the spectrum of U and the kinetic flux.
+// rb=RA,PR: Rayleigh-Benard on the SCFT basis (the RB_SLIP program, PRLARGE with ULARGE scaling: nu =
+// sqrt(PR/RA), kappa = 1/sqrt(PR RA)), kfactor (pi, pi/sqrt2, pi/sqrt2), V = 0 and a temperature
+// perturbation of amplitude 1e-3 (seed 4); "glob t E_V E_T" lines, glob.d (Output_global_RB) with out=,
+// and "nusselt Nu" at the end.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -26,7 +30,7 @@ int main(int argc, char** argv)
 {
     if (argc < 5) {
         fprintf(stderr, "usage: %s N steps dt nu [tg|random|file=PATH] [FP64|FP32_MIXED] [fused|plain] [save=PATH] "
-                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4] [mhd=ETA]\n",
+                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4] [mhd=ETA] [rb=RA,PR]\n",
                 argv[0]);
         return 2;
     }
@@ -38,11 +42,15 @@ int main(int argc, char** argv)
     std::string save, outdir;
     int fproc = 0;
     double fpar[4] = {0, 0, 0, 0};
-    double eta = -1;
+    double eta = -1, Ra = 0, Pr = 0;
     for (int a = 5; a < argc; a++) {
         if (strncmp(argv[a], "save=", 5) == 0) save = argv[a] + 5;
         if (strncmp(argv[a], "out=", 4) == 0) outdir = argv[a] + 4;
         if (strncmp(argv[a], "mhd=", 4) == 0) eta = atof(argv[a] + 4);
+        if (strncmp(argv[a], "rb=", 3) == 0 && sscanf(argv[a] + 3, "%lf,%lf", &Ra, &Pr) != 2) {
+            fprintf(stderr, "tarang_run: rb=RA,PR expected\n");
+            return 2;
+        }
         if (strncmp(argv[a], "force=", 6) == 0 &&
             sscanf(argv[a] + 6, "%d,%lf,%lf,%lf,%lf", &fproc, &fpar[0], &fpar[1], &fpar[2], &fpar[3]) != 5) {
             fprintf(stderr, "tarang_run: force=PROC,INNER,OUTER,P3,P4 expected\n");
@@ -51,6 +59,31 @@ int main(int argc, char** argv)
     }
     try {
         tarang::Context ctx(0);
+        if (Ra > 0 && Pr > 0) {
+            const double kfx = M_PI, kfy = M_PI / sqrt(2.0);
+            tarang::Grid g{{0, N, N, N}, {0, kfx, kfy, kfy}, "SCFT", "DEALIAS", prec};
+            tarang::IncFluid U(ctx, g, sqrt(Pr / Ra), 0.0, dt, "RK4", true, N / 3 + 2);
+            U.Rayleigh = Ra;
+            U.Prandtl = Pr;
+            tarang::IncSF T(ctx, g, 1.0 / sqrt(Pr * Ra));
+            tarang::check(th_ic_random_scalar(T.plan(), T.F.data(), 4, 1e-3), "th_ic_random_scalar");
+            if (!outdir.empty()) U.Open_output_files(outdir);
+            for (int it = 0; it < steps; it++) {
+                U.Tdt = U.Get_dt(T);
+                U.Tnow += U.Tdt;
+                U.Time_advance(T);
+                U.CV_Compute_totalenergy_diss();
+                T.CS_Compute_totalenergy_diss();
+                if (isnan(U.CV_total_energy) || isnan(T.CS_total_energy)) {
+                    printf("ERROR: Numerical Overflow\n");
+                    return 1;
+                }
+                printf("glob %.6f %.15e %.15e\n", U.Tnow, U.CV_total_energy, T.CS_total_energy);
+                if (!outdir.empty()) U.Output_global(T);
+            }
+            printf("nusselt %.15e\n", U.Get_Nusselt_no(T));
+            return 0;
+        }
         tarang::Grid g{{0, N, N, N}, {0, 1.0, 1.0, 1.0}, "FOUR", "DEALIAS", prec};
         tarang::IncFluid U(ctx, g, nu, 0.0, dt, "RK4", path == "fused", N / 3 + 2);
         if (ic == "tg") {

@@ -918,6 +918,16 @@ extern "C" int th_entropy(th_plan* p, void* const V[3], double* out)
     if (!V || !out) { set_error("th_entropy: null"); return E_ARG; }
     return DISPATCH(p, do_entropy, p, V, out);
 }
+template <typename R> static int do_entropy_scalar(th_plan* p, const void* A, double* out)
+{
+    return reduce_entropy_scalar<R>(p->g, CCP(R, A), out, p->rs, p->ctx->stream);
+}
+extern "C" int th_entropy_scalar(th_plan* p, const void* A, double* out)
+{
+    TRY(check_plan(p));
+    if (!A || !out) { set_error("th_entropy_scalar: null"); return E_ARG; }
+    return DISPATCH(p, do_entropy_scalar, p, A, out);
+}
 template <typename R> static int do_shell_hel(th_plan* p, void* const V[3], double* H1, double* H2, double* H3)
 {
     const cplx<R>* v[3] = {CCP(R, V[0]), CCP(R, V[1]), CCP(R, V[2])};

@@ -248,6 +248,26 @@ void IncVF::Setup_Taylor_Green_field(int k0, double amp)
     check(th_ic_taylor_green(plan_, V, k0, amp), "Setup_Taylor_Green_field");
 }
 
+// ---------------------------------------------------------------- IncSF
+IncSF::IncSF(Context& ctx, const Grid& g, double kappa) : diffusion_coefficient(kappa)
+{
+    const int basis = g.basis_type == "SCFT" ? TH_SCFT : TH_FOUR;
+    const int prec = g.precision == "FP32_MIXED" ? TH_FP32_MIXED : TH_FP64;
+    const double kf[3] = {g.kfactor[1], g.kfactor[2], g.kfactor[3]};
+    check(th_plan_create(ctx.h, &plan_, basis, g.N[1], g.N[2], g.N[3], kf, prec, g.alias_switch != "ALIAS"),
+          "th_plan_create");
+    F = DField(th_plan_field_elems(plan_), th_plan_elem_bytes(plan_));
+}
+IncSF::~IncSF() { th_plan_destroy(plan_); }
+void IncSF::CS_Compute_totalenergy_diss()
+{
+    double s;
+    check(th_energy(plan_, F.data(), nullptr, &CS_total_energy), "CS_Compute_totalenergy_diss");
+    check(th_Sn(plan_, F.data(), 2, &s), "CS_Compute_totalenergy_diss");
+    CS_total_dissipation = 2 * s;
+}
+void IncSF::CS_Compute_entropy() { check(th_entropy_scalar(plan_, F.data(), &CS_entropy), "CS_Compute_entropy"); }
+
 // ---------------------------------------------------------------- IncFluid
 IncFluid::IncFluid(Context& ctx, const Grid& g, double nu, double Tinit, double tdt_fixed, const std::string& scheme,
                    bool fused, int nsh)
@@ -260,6 +280,7 @@ IncFluid::~IncFluid()
 {
     if (state_) th_state_destroy(state_);
     if (mstate_) th_state_destroy(mstate_);
+    if (rstate_) th_state_destroy(rstate_);
 }
 void IncFluid::Compute_rhs()
 {
@@ -491,6 +512,64 @@ void IncFluid::Output_flux(IncVF& W)
                   << F(1, i) << " " << F(4, i) << " " << F(5, i) << " " << 0.0 << "    " << F(7, i) << " " << F(8, i)
                   << "    " << 0.0 << " " << 0.0 << "    " << std::endl;
     flux_file << std::endl;
+}
+
+// ---------------------------------------------------------------- IncFluid with T (RB)
+double IncFluid::Get_dt(IncSF&) { return Get_dt(); }
+
+void IncFluid::Time_advance(IncSF& T)
+{
+    if (has_force_) throw std::runtime_error("Time_advance(T): Set_force applies to NS runs only");
+    th_phys ph{dissipation_coefficient, 0, T.diffusion_coefficient, Rayleigh, Prandtl};
+    void* f[4] = {V1.data(), V2.data(), V3.data(), T.F.data()};
+    if (grid.basis_type == "SCFT" && integ_scheme == "RK4") {
+        if (!rstate_) check(th_state_create(plan_, &rstate_, TH_RB), "th_state_create");
+        for (int i = 0; i < 4; i++) check(th_state_upload(rstate_, i, f[i], 0), "th_state_upload");
+        check(th_step(plan_, rstate_, Tdt, &ph, 1), "th_step");
+        for (int i = 0; i < 4; i++) check(th_state_download(rstate_, i, f[i], 0), "th_state_download");
+        check(th_sync(plan_), "th_sync");
+        return;
+    }
+    const int sch = integ_scheme == "EULER" ? 0 : integ_scheme == "RK2" ? 1 : 2;
+    check(th_step_plain(plan_, TH_RB, sch, f, Tdt, &ph), "th_step_plain");
+    check(th_sync(plan_), "th_sync");
+}
+
+double IncFluid::Get_Nusselt_no(IncSF& T)
+{
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    double nu;
+    check(th_nusselt(plan_, V, T.F.data(), Rayleigh, Prandtl, &nu), "Get_Nusselt_no");
+    return nu;
+}
+
+void IncFluid::Output_global(IncSF& T)
+{
+    CV_Compute_totalenergy_diss();
+    T.CS_Compute_totalenergy_diss();
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    double h[4], sv;
+    check(th_entropy(plan_, V, &sv), "CV_Compute_entropy");
+    T.CS_Compute_entropy();
+    check(th_helicity(plan_, V, h), "CV_Compute_total_helicity");
+    const double nusselt_no = Get_Nusselt_no(T);
+    double mv[3], mt[3];
+    void* Tp[3] = {T.F.data(), T.F.data(), T.F.data()};
+    mean_modes(V, esize_, mv);
+    mean_modes(Tp, esize_, mt);
+    const double nu = dissipation_coefficient, kappa = T.diffusion_coefficient;
+    const double kmax = th_max_radius_inside(plan_);
+    const double total_diss = nu * CV_total_dissipation;
+    const double kmax_eta1 = kmax * sqrt(sqrt(pow(nu, 3) / total_diss));
+    const double tempvar = nu / kappa;
+    const double kmax_eta2 = kmax_eta1 * sqrt(sqrt(tempvar)) / tempvar;
+    const double Rlambda = 2 * CV_total_energy * sqrt(15 / (nu * total_diss));
+    global_file << Tnow << "\t" << CV_total_energy << " " << T.CS_total_energy << "\t" << CV_total_dissipation << " "
+                << T.CS_total_dissipation << "\t" << total_diss << " " << kappa * T.CS_total_dissipation << "    "
+                << nusselt_no << "    " << h[0] << " " << h[1] << "    " << h[2] << " " << h[3] << "    " << sv << " "
+                << T.CS_entropy << "   " << kmax_eta1 << " " << kmax_eta2 << "    " << nu << " " << kappa << "    "
+                << Rlambda << "    " << Tdt << "    " << mv[0] << " " << mv[1] << " " << mv[2] << "\t" << mt[0]
+                << std::endl;
 }
 
 // The force of the current velocity (Compute_force, as the time loop leaves Force1..3 before output).

@@ -117,6 +117,7 @@ template <typename R> int reduce_shell_mult_all(const Geo& g, const cplx<R>* con
                                                 hipStream_t st);
 template <typename R> int reduce_helicity(const Geo& g, const cplx<R>* const V[3], double* out, RedScratch& rs, hipStream_t st);
 template <typename R> int reduce_entropy(const Geo& g, const cplx<R>* const V[3], double* out, RedScratch& rs, hipStream_t st);
+template <typename R> int reduce_entropy_scalar(const Geo& g, const cplx<R>* A, double* out, RedScratch& rs, hipStream_t st);
 template <typename R> int reduce_shell_helicity(const Geo& g, const cplx<R>* const V[3], int nbins, double* H1, double* H2,
                                                 double* H3, RedScratch& rs, hipStream_t st);
 

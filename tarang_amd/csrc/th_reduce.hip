@@ -229,7 +229,8 @@ __global__ __launch_bounds__(RBLK) void k_modal(GeoR g, const cplx<R>* __restric
             acc[NV > 2 ? 2 : 0] += ksq * modal;
         } else {
             // Modal_energy_SCFT (scft_inline.h:347-354) is |A|^2 for kx > 0; SCFT weighs by mult, FOUR by 2 mult
-            const double me = 0.5 * (sq(A0[i]) + sq(A1[i]) + sq(A2[i])) * ((g.basis == 1 && l1 > 0) ? 2.0 : 1.0);
+            const double v2 = sq(A0[i]) + (A1 ? sq(A1[i]) + sq(A2[i]) : 0.0);     // A1 == nullptr: one field
+            const double me = 0.5 * v2 * ((g.basis == 1 && l1 > 0) ? 2.0 : 1.0);
             const double p = me / E;
             if (p > 1e-15) acc[0] += (g.basis == 1 ? 1.0 : 2.0) * mult_r(g, l1, l2, l3) * (-p * log(p) / log(2.0));
         }
@@ -579,6 +580,15 @@ template <typename R> int reduce_entropy(const Geo& g, const cplx<R>* const V[3]
     return run_modal<R>(1, g, V, E, out, rs, st);
 }
 
+// Get_entropy_scalar_FOUR four_energy.cc:761-826 / _SCFT scft_energy.cc:750-800: one field
+template <typename R> int reduce_entropy_scalar(const Geo& g, const cplx<R>* A, double* out, RedScratch& rs, hipStream_t st)
+{
+    double E;
+    TRY(reduce_energy<R>(g, A, nullptr, &E, rs, st));
+    const cplx<R>* V[3] = {A, nullptr, nullptr};
+    return run_modal<R>(1, g, V, E, out, rs, st);
+}
+
 // Compute_shell_spectrum_helicity_FOUR four_energy.cc:602-671 / _SCFT scft_energy.cc:601-677 (with the
 // partial-shell rescale)
 template <typename R>
@@ -685,6 +695,7 @@ int reduce_aniso(const Geo& g, const cplx<R>* const V[3], int mode, int nbin, in
                                           int, double*, RedScratch&, hipStream_t);                                  \
     template int reduce_helicity<R>(const Geo&, const cplx<R>* const*, double*, RedScratch&, hipStream_t);          \
     template int reduce_entropy<R>(const Geo&, const cplx<R>* const*, double*, RedScratch&, hipStream_t);           \
+    template int reduce_entropy_scalar<R>(const Geo&, const cplx<R>*, double*, RedScratch&, hipStream_t);         \
     template int reduce_shell_helicity<R>(const Geo&, const cplx<R>* const*, int, double*, double*, double*,         \
                                           RedScratch&, hipStream_t);                                                \
     template int reduce_aniso<R>(const Geo&, const cplx<R>* const*, int, int, int, int, double*, double*,             \

@@ -239,3 +239,41 @@ def test_driver_mhd_output_files_match_oracle(tmp_path):
                 assert float(tok) == 0.0
             else:
                 assert abs(float(tok) - fl[name][i]) <= 1e-10 * max(np.abs(fl[name]).max(), 1e-300), (r, name)
+
+
+def test_driver_rb_glob_matches_oracle(tmp_path):
+    """rb=RA,PR: the RB_SLIP program's loop (Get_dt(T), Time_advance(T) on the fused SCFT step) and
+    Output_global_RB (output_energy.cc:156-215): every glob.d number (energies, dissipations, Nusselt number,
+    SCFT helicities, vector and scalar entropies, Kolmogorov and Taylor-scale figures, means) equals the
+    oracle's value to the printed digits."""
+    import math
+    N, steps, dt, Ra, Pr = 32, 4, 1e-3, 1e6, 1.0
+    (tmp_path / "out").mkdir()
+    spec, flux, glob = run_driver(N, steps, dt, 0.0, "random", "FP64", "fused", f"out={tmp_path}", f"rb={Ra},{Pr}")
+    kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
+    o = Oracle((N, N, N), kfactor=kf, basis="SCFT")
+    V, T = o.zeros(3), o.ic_random_scalar(4, 1e-3)
+    o.rk4_rb(V, T, Ra, Pr, dt, steps=steps)
+    nu, kappa = math.sqrt(Pr / Ra), 1 / math.sqrt(Pr * Ra)
+    Ev, Et = o.total_energy(V), o.energy(T)
+    assert abs(glob[-1][1] - Ev) <= 1e-12 * Ev and abs(glob[-1][2] - Et) <= 1e-12 * Et
+    line = (tmp_path / "out" / "glob.d").read_text().splitlines()[-1]
+    assert line.count("\t") == 4 and line.count("    ") == 8
+    v = [float(x) for x in line.split()]
+    assert len(v) == 24
+    Dv, Dt = o.total_dissipation(V), 2 * o.Sn(T, 2)
+    h = o.helicity(V)
+    kmax = o.max_radius_inside()
+    ke1 = kmax * (nu ** 3 / (nu * Dv)) ** 0.25
+    ref = [steps * dt, Ev, Et, Dv, Dt, nu * Dv, kappa * Dt, o.nusselt(V, T, Ra, Pr), h[0], h[1], h[2], h[3],
+           o.entropy(V), o.entropy_scalar(T), ke1, ke1 * (nu / kappa) ** 0.25 / (nu / kappa), nu, kappa,
+           2 * Ev * (15 / (nu * nu * Dv)) ** 0.5, dt, 0.0, 0.0, 0.0, 0.0]
+    # the RB field carries no helicity beyond rounding: judge H1, H2, DH1, DH2 on their natural scale
+    # kmax^2 sqrt(E D) (Cauchy-Schwarz bound) instead of their own roundoff-sized values
+    hs = kmax ** 2 * math.sqrt(Ev * Dv)
+    for i, (a, b) in enumerate(zip(v, ref)):
+        if i >= 20:                                  # mean modes: zero up to rounding
+            assert abs(a) <= 1e-15, (i, a)
+            continue
+        sc = max(abs(b), hs) if 8 <= i <= 11 else max(abs(b), 1e-300)
+        assert abs(a - b) <= 1e-10 * sc, (i, a, b)

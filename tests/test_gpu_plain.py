@@ -337,6 +337,7 @@ def test_helicity_and_entropy(ctx, basis, prec, tol):
     Hk, rHk = p.shell_helicity(dV), o.shell_helicity(V)
     assert np.abs(Hk - rHk).max() <= tol * np.abs(rHk).max()
     assert abs(p.entropy(dV) - o.entropy(V)) <= tol * o.entropy(V)
+    assert abs(p.entropy_scalar(dV[1]) - o.entropy_scalar(V[1])) <= tol * o.entropy_scalar(V[1])
 
 
 def test_config1_taylor_green_64_100_steps(ctx):

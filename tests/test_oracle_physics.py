@@ -449,3 +449,21 @@ def test_scft_entropy_closed_form():
     E = sum(o.energy(W[c]) for c in range(3))
     p = 0.25 / E
     assert abs(o.entropy(W) - (-2 * n * p * math.log2(p))) < 1e-13
+
+
+def test_entropy_scalar_closed_forms():
+    """Get_entropy_scalar_FOUR / _SCFT (four_energy.cc:761-826, scft_energy.cc:750-800): one field with n stored
+    kz > 0 modes of equal |A|^2 has entropy log2(2n) on a FOUR box (each mode and its conjugate at 1/(2n)) and
+    -2 n p log2 p, p = |A|^2 / E, on a SCFT box (Modal_energy_SCFT = |A|^2 for kx > 0, weight mult = 2)."""
+    n = 6
+    for basis in ("FOUR", "SCFT"):
+        o = Oracle((16, 16, 16), basis=basis)
+        A = o.zeros()
+        for j in range(n):
+            A[j + 1, j, 1 + j] = 0.3 + 0.2j * (1 if j % 2 else -1)
+        S = o.entropy_scalar(A)
+        if basis == "FOUR":
+            assert abs(S - math.log2(2 * n)) < 1e-13
+        else:
+            p = abs(0.3 + 0.2j) ** 2 / o.energy(A)
+            assert abs(S - (-2 * n * p * math.log2(p))) < 1e-13
