@@ -119,9 +119,11 @@ int th_et_nlin(th_plan *plan, void *const H[3], void *const G[3], void *nlin[3])
 int th_step_plain(th_plan *plan, int physics, int scheme, void *const fields[], double dt, const th_phys *p);
 /* NS step with the forcing of `force` recomputed from the stage field at every stage and subtracted
  * from nlin before the pressure: Compute_force_TO_rhs single_time_step.cc:351-364 (Compute_force
- * compute_force_main.cc:44-60, Add_force add_force.cc:47-53).  FOUR plans only. */
+ * compute_force_main.cc:44-60, Add_force add_force.cc:47-53).  FOUR plans only.  last_stage (may be
+ * NULL) receives the field of the step's last stage, whose force the reference's Force1..3 still hold
+ * when the time loop writes its outputs (Ifluid_main.cc:278-298). */
 int th_step_plain_forced(th_plan *plan, int scheme, void *const V[3], double dt, const th_phys *p,
-                         const th_force *force);
+                         const th_force *force, void *const last_stage[3]);
 /* The force field itself (IncFluid::Compute_force for proc 1 / 2): F = alpha V + beta (i K x V) on the
  * band inner < |K| <= outer, zero elsewhere, dealiased when the band reaches past N/3 and with the
  * reality condition applied.  nf (may be NULL) receives the mode count of Get_number_modes_in_shell.

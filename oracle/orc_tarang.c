@@ -928,7 +928,7 @@ int orc_force_ek_hk(const orc_grid *g, int proc, double *const V[3], double *F[3
     return nf;
 }
 
-typedef struct { int proc; double inner, outer, p3, p4; } force_ctx;
+typedef struct { int proc; double inner, outer, p3, p4; double **keep; } force_ctx;
 
 static void rhs_ns_forced(const orc_grid *g, stepper_t *s, void *ctx)
 {
@@ -936,6 +936,8 @@ static void rhs_ns_forced(const orc_grid *g, stepper_t *s, void *ctx)
     double *F[3], *p = alloc_field(g);
     for (int c = 0; c < 3; c++) F[c] = alloc_field(g);
     orc_force_ek_hk(g, fc->proc, s->F, F, fc->inner, fc->outer, fc->p3, fc->p4);
+    if (fc->keep)                       /* the field this (so far last) force was computed from */
+        for (int c = 0; c < 3; c++) fcopy(g, fc->keep[c], s->F[c]);
     orc_nlin_ns(g, s->F, s->N);
     for (int c = 0; c < 3; c++) faxpy(g, s->N[c], -1.0, F[c]);
     orc_pressure(g, s->N, p);
@@ -944,11 +946,12 @@ static void rhs_ns_forced(const orc_grid *g, stepper_t *s, void *ctx)
     free(p);
 }
 
-/* scheme 0 EULER, 1 RK2, 2 RK4 */
+/* scheme 0 EULER, 1 RK2, 2 RK4; keep (may be NULL) receives the field of the last stage, whose force the
+ * reference's Force1..3 hold after the step */
 void orc_step_ns_forced(const orc_grid *g, double *V[3], double nu, double dt, int scheme, int proc, double inner,
-                        double outer, double p3, double p4)
+                        double outer, double p3, double p4, double *keep[3])
 {
-    force_ctx fc = {proc, inner, outer, p3, p4};
+    force_ctx fc = {proc, inner, outer, p3, p4, keep};
     if (scheme == 0) { euler_generic(g, V, nu, dt, rhs_ns_forced, &fc); return; }
     if (scheme == 1) { rk2_generic(g, V, nu, dt, rhs_ns_forced, &fc); return; }
     stepper_t s; memset(&s, 0, sizeof(s));

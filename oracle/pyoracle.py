@@ -61,7 +61,7 @@ def lib():
             "orc_rk2_step_ns": (None, [p, pp, d, d]),
             "orc_force_supply_modes": (i, [p, d, d]),
             "orc_force_ek_hk": (i, [p, i, pp, pp, d, d, d, d]),
-            "orc_step_ns_forced": (None, [p, pp, d, d, i, i, d, d, d, d]),
+            "orc_step_ns_forced": (None, [p, pp, d, d, i, i, d, d, d, d, pp]),
             "orc_rk4_stage1_ns": (None, [p, pp, d, d]),
             "orc_nlin_mhd": (None, [p, pp, pp, pp, pp]),
             "orc_rk4_step_mhd": (None, [p, pp, pp, d, d, d]),
@@ -239,10 +239,12 @@ class Oracle:
             raise ValueError("no mode in the forcing band")
         return F
 
-    def step_ns_forced(self, V, nu, dt, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1):
-        """NS steps (scheme 0 EULER, 1 RK2, 2 RK4) with the force recomputed at every stage."""
+    def step_ns_forced(self, V, nu, dt, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1, last_stage=None):
+        """NS steps (scheme 0 EULER, 1 RK2, 2 RK4) with the force recomputed at every stage; last_stage
+        (3 arrays, optional) receives the field of the last stage of the last step."""
         for _ in range(steps):
-            self.L.orc_step_ns_forced(self.gp, _pp(V), nu, dt, scheme, proc, inner, outer, p3, p4)
+            self.L.orc_step_ns_forced(self.gp, _pp(V), nu, dt, scheme, proc, inner, outer, p3, p4,
+                                      _pp(last_stage) if last_stage is not None else None)
         return V
 
     def nlin_mhd(self, U, B):

@@ -93,7 +93,7 @@ def lib():
         "th_nlin_rb": (i, [vp, pp, vp, pp, vp]),
         "th_et_nlin": (i, [vp, pp, pp, pp]),
         "th_step_plain": (i, [vp, i, i, pp, d, C.POINTER(Phys)]),
-        "th_step_plain_forced": (i, [vp, i, pp, d, C.POINTER(Phys), C.POINTER(Force)]),
+        "th_step_plain_forced": (i, [vp, i, pp, d, C.POINTER(Phys), C.POINTER(Force), pp]),
         "th_compute_force": (i, [vp, pp, pp, C.POINTER(Force), C.POINTER(C.c_long)]),
         "th_energy": (i, [vp, vp, vp, dp]),
         "th_Sn": (i, [vp, vp, d, dp]),
@@ -378,13 +378,15 @@ class Plan:
         self._post(0, "")
         return fields
 
-    def step_plain_forced(self, V, dt, nu, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1):
-        """NS steps with the force_field_proc `proc` forcing recomputed at every stage."""
+    def step_plain_forced(self, V, dt, nu, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1, last_stage=None):
+        """NS steps with the force_field_proc `proc` forcing recomputed at every stage; last_stage (3 device
+        arrays, optional) receives the field of the last stage."""
         ph = Phys(nu, 0.0, 0.0, 0.0, 0.0)
         fc = Force(proc, inner, outer, p3, p4)
         self._pre()
         for _ in range(steps):
-            _check(lib().th_step_plain_forced(self.h, scheme, _pp(V), dt, C.byref(ph), C.byref(fc)),
+            _check(lib().th_step_plain_forced(self.h, scheme, _pp(V), dt, C.byref(ph), C.byref(fc),
+                                              _pp(last_stage) if last_stage is not None else None),
                    "th_step_plain_forced")
         self._post(0, "")
         return V

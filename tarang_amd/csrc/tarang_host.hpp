@@ -192,13 +192,15 @@ public:
     std::string integ_scheme;
 
 private:
-    struct Force_field {           // F[3] of the current velocity (th_compute_force)
+    struct Force_field {           // Force1..3 as the reference holds them when the time loop writes outputs
         explicit Force_field(IncFluid& u);
         DField f[3];
     };
     bool fused_;
     bool forced_ = false;          // the plain stepper carries the force
     bool has_force_ = false;       // Set_force was called (fused or plain)
+    bool stepped_ = false;         // a forced step has run since Set_force
+    DField last_[3];               // plain forced stepper: the field of the last stage
     th_force force_{};
     th_state* state_ = nullptr;
     th_state* mstate_ = nullptr;   // fused MHD state (U, B), created by the first Time_advance(W)

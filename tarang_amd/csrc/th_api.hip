@@ -610,6 +610,7 @@ template <typename R> struct Ops {
     int physics = 0;
     double Ra = 0, Pr = 0;
     const ForceBand* force = nullptr;    // NS forcing (force_field_proc 1 / 2), else none
+    cplx<R>* keep[3] = {nullptr, nullptr, nullptr};   // receives each forced stage's field (the last stays)
 
     int compute_rhs()
     {
@@ -619,6 +620,8 @@ template <typename R> struct Ops {
             // Compute_force_TO_rhs single_time_step.cc:351-364: the force of the stage field (which
             // Compute_nlin leaves unchanged) is subtracted from nlin before the pressure
             if (force) TRY(force_subtract<R>(*force, F, N, st));
+            if (force && keep[0])
+                for (int c = 0; c < 3; c++) TRY(copy(keep[c], F[c]));
         } else if (physics == TH_MHD) {
             TRY(nlin_mhd(F, F + 3, N, N + 3));
         } else {
@@ -835,12 +838,14 @@ extern "C" int th_step_plain(th_plan* p, int physics, int scheme, void* const fi
 }
 
 template <typename R> static int do_step_plain_forced(th_plan* p, int scheme, void* const V[3], double dt,
-                                                      const th_phys* ph, const ForceBand* fb)
+                                                      const th_phys* ph, const ForceBand* fb, void* const last[3])
 {
     Ops<R> o(p);
     o.physics = TH_NS;
     o.nc = 3;
     o.force = fb;
+    if (last)
+        for (int c = 0; c < 3; c++) o.keep[c] = CP(R, last[c]);
     for (int q = 0; q < 3; q++) {
         o.F[q] = CP(R, V[q]);
         o.N[q] = o.S(SC_N0 + q);
@@ -850,16 +855,17 @@ template <typename R> static int do_step_plain_forced(th_plan* p, int scheme, vo
     return o.step(scheme, dt);
 }
 extern "C" int th_step_plain_forced(th_plan* p, int scheme, void* const V[3], double dt, const th_phys* ph,
-                                    const th_force* force)
+                                    const th_force* force, void* const last_stage[3])
 {
     TRY(check_plan(p));
-    if (!V || !V[0] || !V[1] || !V[2] || !ph || scheme < 0 || scheme > 2) {
+    if (!V || !V[0] || !V[1] || !V[2] || !ph || scheme < 0 || scheme > 2 ||
+        (last_stage && (!last_stage[0] || !last_stage[1] || !last_stage[2]))) {
         set_error("th_step_plain_forced: bad args");
         return E_ARG;
     }
     ForceBand fb;
     TRY(force_setup(p->g, force, fb));
-    return DISPATCH(p, do_step_plain_forced, p, scheme, V, dt, ph, &fb);
+    return DISPATCH(p, do_step_plain_forced, p, scheme, V, dt, ph, &fb, last_stage);
 }
 
 // ------------------------------------------------------------------------------------------

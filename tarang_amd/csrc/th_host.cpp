@@ -572,18 +572,33 @@ void IncFluid::Output_global(IncSF& T)
                 << std::endl;
 }
 
-// The force of the current velocity (Compute_force, as the time loop leaves Force1..3 before output).
+// Force1..3 as the reference holds them when its time loop writes the outputs (Ifluid_main.cc:278-298):
+// Time_advance recomputes the force at every RK stage (Compute_force_TO_rhs), so they hold the force of the
+// last stage's field; before the first step they are still zero.
 IncFluid::Force_field::Force_field(IncFluid& u)
 {
     for (int c = 0; c < 3; c++) f[c] = DField(u.elems_, u.esize_);
-    void* const V[3] = {u.V1.data(), u.V2.data(), u.V3.data()};
+    if (!u.stepped_) return;                                   // DField starts zeroed
+    DField stage[3];
+    void* src[3];
+    if (u.forced_) {
+        for (int c = 0; c < 3; c++) src[c] = u.last_[c].data();
+    } else {
+        // fused step: the stage field buffer still holds the last stage's input
+        for (int c = 0; c < 3; c++) {
+            stage[c] = DField(u.elems_, u.esize_);
+            check(th_state_download_buffer(u.state_, 1, c, stage[c].data(), 0), "th_state_download_buffer");
+            src[c] = stage[c].data();
+        }
+    }
     void* F[3] = {f[0].data(), f[1].data(), f[2].data()};
-    check(th_compute_force(u.plan_, V, F, &u.force_, nullptr), "Compute_force");
+    check(th_compute_force(u.plan_, src, F, &u.force_, nullptr), "Compute_force");
 }
 
 void IncFluid::Set_force(int proc, double inner_radius, double outer_radius, double p3, double p4)
 {
     th_force f{proc, inner_radius, outer_radius, p3, p4};
+    stepped_ = false;
     if (fused_ && state_) {
         const int r = th_state_set_force(state_, &f);
         if (r == TH_OK) {
@@ -617,7 +632,11 @@ void IncFluid::Time_advance()
     if (forced_) {
         int sch = integ_scheme == "EULER" ? 0 : integ_scheme == "RK2" ? 1 : 2;
         void* const V[3] = {V1.data(), V2.data(), V3.data()};
-        check(th_step_plain_forced(plan_, sch, V, Tdt, &ph, &force_), "th_step_plain_forced");
+        if (!last_[0].data())
+            for (int c = 0; c < 3; c++) last_[c] = DField(elems_, esize_);
+        void* const last[3] = {last_[0].data(), last_[1].data(), last_[2].data()};
+        check(th_step_plain_forced(plan_, sch, V, Tdt, &ph, &force_, last), "th_step_plain_forced");
+        stepped_ = true;
         check(th_sync(plan_), "th_sync");
         return;
     }
@@ -627,6 +646,7 @@ void IncFluid::Time_advance()
         check(th_step(plan_, state_, Tdt, &ph, 1), "th_step");
         for (int i = 0; i < 3; i++) check(th_state_download(state_, i, V[i], 0), "th_state_download");
         check(th_sync(plan_), "th_sync");
+        stepped_ = has_force_;
         return;
     }
     int sch = integ_scheme == "EULER" ? 0 : integ_scheme == "RK2" ? 1 : 2;

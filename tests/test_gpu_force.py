@@ -104,6 +104,25 @@ def test_forced_steps_match_oracle(ctx, proc, p3, p4, scheme):
         assert rel(H(gV[i]), V[i]) < 1e-12
 
 
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+def test_forced_step_reports_its_last_stage(ctx, scheme):
+    """th_step_plain_forced's last_stage is the field of the step's last stage (the one whose force the
+    reference's Force1..3 hold after Time_advance), equal to the oracle's."""
+    from tarang_amd import Plan
+    log(f"forced last stage scheme={scheme}")
+    N = (16, 16, 16)
+    o = Oracle(N)
+    V = o.ic_random(6, E0=0.5, kp=3.0)
+    p = Plan(ctx, N)
+    gV = [p.to_dev(a) for a in V]
+    last, glast = o.zeros(3), p.empty(3)
+    o.step_ns_forced(V, 0.01, 1e-3, 1, 1.5, 3.5, 0.5, 0.2, scheme=scheme, steps=2, last_stage=last)
+    p.step_plain_forced(gV, 1e-3, 0.01, 1, 1.5, 3.5, 0.5, 0.2, scheme=scheme, steps=2, last_stage=glast)
+    for c in range(3):
+        assert rel(H(glast[c]), last[c]) < 1e-12
+        assert rel(H(gV[c]), V[c]) < 1e-12
+
+
 def test_forced_rk4_fp32(ctx):
     from tarang_amd import Plan
     log("forced rk4 fp32")

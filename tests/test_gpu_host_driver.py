@@ -143,21 +143,25 @@ def test_driver_forced_run_matches_oracle(band):
         assert abs(v - Ek[k]) <= 1e-12 * max(Ek)
 
 
-def test_driver_forced_output_columns(tmp_path):
+@pytest.mark.parametrize("path", ["fused", "plain"])
+def test_driver_forced_output_columns(tmp_path, path):
     """Forced run with out=DIR: the force-spectrum column of spectrum.d (Compute_force_shell_spectrum
     force_spectrum.cc:48-53, 2 sum_c Re V_c.F_c* per shell) and the force-feed column of flux.d
-    (Compute_force_feed_shell compute_force_feed.cc:70-76, Shell_mult_all(F, V) over shell_radius) hold
-    the force of the final velocity, as the oracle computes them."""
+    (Compute_force_feed_shell compute_force_feed.cc:70-76, Shell_mult_all(F, V) over shell_radius).  The
+    reference writes them after Time_advance, whose stages recompute the force (time_advance.cc:100-155), so
+    F is the force of the last RK stage's field, V the new velocity; the oracle's forced step reports that
+    field."""
     N, steps, dt, nu = 32, 3, 1e-3, 0.01
     fp = (1, 1.5, 3.5, 0.3, 0.2)
     (tmp_path / "out").mkdir()
-    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"out={tmp_path}",
+    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", path, f"out={tmp_path}",
                                   "force=" + ",".join(str(v) for v in fp))
     o = Oracle((N, N, N))
     V = o.ic_random(1)
-    o.step_ns_forced(V, nu, dt, *fp, steps=steps)
+    last = o.zeros(3)
+    o.step_ns_forced(V, nu, dt, *fp, steps=steps, last_stage=last)
     assert abs(glob[-1][1] - o.total_energy(V)) <= 1e-12 * o.total_energy(V)
-    F = o.force(V, *fp)
+    F = o.force(last, *fp)
     fk = 2 * sum(o.shell_spectrum(V[c], F[c])[0] for c in range(3))
     rows = [r for r in (tmp_path / "out" / "spectrum.d").read_text().split("\n")[1:] if r.strip()]
     assert len(rows) == len(fk) and np.abs(fk).max() > 0
@@ -170,6 +174,10 @@ def test_driver_forced_output_columns(tmp_path):
     for r in frows:
         i, R, pi, force = r.split()
         assert abs(float(force) - feed[int(i)]) <= 1e-10 * np.abs(feed).max(), r
+    # the force of the new velocity differs from the last stage's at O(dt): the columns tell them apart
+    F_new = o.force(V, *fp)
+    fk_new = 2 * sum(o.shell_spectrum(V[c], F_new[c])[0] for c in range(3))
+    assert np.abs(fk_new - fk).max() > 1e-6 * np.abs(fk).max()
 
 
 def test_driver_mhd_output_files_match_oracle(tmp_path):
