@@ -1107,7 +1107,9 @@ void orc_nlin_rb(const orc_grid *g, double *V[3], double *T, double *nV[3], doub
 
 static void rhs_rb(const orc_grid *g, stepper_t *s, void *ctx)
 {
-    (void)ctx;
+    double **keep = (double **)ctx;      /* NULL, or 4 fields receiving this stage's V1..V3, T */
+    if (keep)
+        for (int c = 0; c < 4; c++) fcopy(g, keep[c], s->F[c]);
     double *p = alloc_field(g);
     double *F1 = alloc_field(g), *FT = alloc_field(g);
     /* Compute_force_RB: Force1 = T.F, T.Force = V1, dealiased */
@@ -1123,14 +1125,16 @@ static void rhs_rb(const orc_grid *g, stepper_t *s, void *ctx)
     free(p); free(F1); free(FT);
 }
 
-void orc_rk4_step_rb(const orc_grid *g, double *V[3], double *T, double Ra, double Pr, double dt)
+/* keep (may be NULL): 4 fields receiving the last stage's V1..V3, T (whose force the reference's Force1 and
+ * T.Force hold after Time_advance(T)) */
+void orc_rk4_step_rb(const orc_grid *g, double *V[3], double *T, double Ra, double Pr, double dt, double *keep[4])
 {
     stepper_t s; memset(&s, 0, sizeof(s));
     s.nvec = 1; s.nsc = 1; s.nc = 4;
     double nu = sqrt(Pr / Ra), kappa = 1 / sqrt(Pr * Ra);
     for (int i = 0; i < 3; i++) { s.F[i] = V[i]; s.N[i] = alloc_field(g); s.diss[i] = nu; }
     s.F[3] = T; s.N[3] = alloc_field(g); s.diss[3] = kappa;
-    rk4_generic(g, &s, dt, rhs_rb, NULL);
+    rk4_generic(g, &s, dt, rhs_rb, keep);
     for (int i = 0; i < 4; i++) free(s.N[i]);
 }
 

@@ -88,7 +88,7 @@ def lib():
             "orc_flux_scalar": (None, [p, pp, vp, vp, i, vp, vp]),
             "orc_shell_tr_scalar": (None, [p, pp, vp, vp, i, vp, vp]),
             "orc_nlin_rb": (None, [p, pp, vp, pp, vp]),
-            "orc_rk4_step_rb": (None, [p, pp, vp, d, d, d]),
+            "orc_rk4_step_rb": (None, [p, pp, vp, d, d, d, pp]),
             "orc_nusselt": (d, [p, pp, vp, d, d]),
             "orc_ic_taylor_green": (None, [p, pp, i, d]),
             "orc_ic_random": (None, [p, pp, C.c_ulonglong, d, d, i]),
@@ -262,9 +262,11 @@ class Oracle:
         self.L.orc_nlin_rb(self.gp, _pp(V), _ptr(T), _pp(nV), _ptr(nT))
         return nV, nT
 
-    def rk4_rb(self, V, T, Ra, Pr, dt, steps=1):
+    def rk4_rb(self, V, T, Ra, Pr, dt, steps=1, last_stage=None):
+        """RB RK4 steps; last_stage (4 arrays, optional) receives the last stage's V1..V3, T."""
         for _ in range(steps):
-            self.L.orc_rk4_step_rb(self.gp, _pp(V), _ptr(T), Ra, Pr, dt)
+            self.L.orc_rk4_step_rb(self.gp, _pp(V), _ptr(T), Ra, Pr, dt,
+                                   _pp(last_stage) if last_stage is not None else None)
         return V, T
 
     def nusselt(self, V, T, Ra, Pr):

@@ -116,7 +116,8 @@ void   orc_et_radii(const orc_grid *g, int scheme, int no_shells, double *radii)
 
 /* ---- RB convection with SCFT (compute_nlin_SF.cc, time_advance_SF.cc, nusselt.cc) ---- */
 void   orc_nlin_rb(const orc_grid *g, double *V[3], double *T, double *nV[3], double *nT);
-void   orc_rk4_step_rb(const orc_grid *g, double *V[3], double *T, double Ra, double Pr, double dt);
+void   orc_rk4_step_rb(const orc_grid *g, double *V[3], double *T, double Ra, double Pr, double dt,
+                       double *keep[4]);   /* keep (may be NULL): the last stage V1..V3, T */
 double orc_nusselt(const orc_grid *g, double *const V[3], const double *T, double Ra, double Pr);
 
 /* ---- initial conditions ---- */

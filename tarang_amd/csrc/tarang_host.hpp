@@ -186,6 +186,10 @@ public:
     double Get_dt(IncSF& T);                         // compute_dt.cc:66-69 (= Get_dt())
     double Get_Nusselt_no(IncSF& T);                 // nusselt.cc:53-89
     void Output_global(IncSF& T);                    // Output_global_RB output_energy.cc:156-215
+    void Output_shell_spectrum(IncSF& T);            // Output_shell_spectrum_scalar output_energy.cc:410-449
+    void Compute_flux(IncSF& T);                     // Compute_flux_RB compute_flux.cc:113-141 -> flux_self, flux_SF
+    void Output_flux(IncSF& T);                      // output_ET.cc:81-115
+    std::vector<double> flux_SF;                     // (no_spheres + 1)
     double Rayleigh = 0, Prandtl = 0;                // Ra, Pr of the RB run
     std::ofstream global_file, spectrum_file, flux_file;
     double Tnow, Tdt, Tdt_fixed;
@@ -200,6 +204,8 @@ private:
     bool forced_ = false;          // the plain stepper carries the force
     bool has_force_ = false;       // Set_force was called (fused or plain)
     bool stepped_ = false;         // a forced step has run since Set_force
+    bool rb_stepped_ = false;      // a fused RB step has run (its stage buffer holds the last stage)
+    void rb_forces(DField fv[3], DField& ft);   // Force1..3 and T.Force after Time_advance(T)
     DField last_[3];               // plain forced stepper: the field of the last stage
     th_force force_{};
     th_state* state_ = nullptr;

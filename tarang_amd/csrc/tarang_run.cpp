@@ -16,8 +16,9 @@
 // lines, the spectrum of U and the kinetic flux.
 // rb=RA,PR: Rayleigh-Benard on the SCFT basis (the RB_SLIP program, PRLARGE with ULARGE scaling: nu =
 // sqrt(PR/RA), kappa = 1/sqrt(PR RA)), kfactor (pi, pi/sqrt2, pi/sqrt2), V = 0 and a temperature
-// perturbation of amplitude 1e-3 (seed 4); "glob t E_V E_T" lines, glob.d (Output_global_RB) with out=,
-// and "nusselt Nu" at the end.
+// perturbation of amplitude 1e-3 (seed 4); "glob t E_V E_T" lines, "nusselt Nu" at the end, and with out=
+// glob.d (Output_global_RB) every step, spectrum.d and flux.d (Output_shell_spectrum_scalar, Output_flux(T))
+// at the end.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -82,6 +83,10 @@ int main(int argc, char** argv)
                 if (!outdir.empty()) U.Output_global(T);
             }
             printf("nusselt %.15e\n", U.Get_Nusselt_no(T));
+            if (!outdir.empty()) {
+                U.Output_shell_spectrum(T);
+                U.Output_flux(T);
+            }
             return 0;
         }
         tarang::Grid g{{0, N, N, N}, {0, 1.0, 1.0, 1.0}, "FOUR", "DEALIAS", prec};

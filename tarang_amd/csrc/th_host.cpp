@@ -528,6 +528,7 @@ void IncFluid::Time_advance(IncSF& T)
         check(th_step(plan_, rstate_, Tdt, &ph, 1), "th_step");
         for (int i = 0; i < 4; i++) check(th_state_download(rstate_, i, f[i], 0), "th_state_download");
         check(th_sync(plan_), "th_sync");
+        rb_stepped_ = true;
         return;
     }
     const int sch = integ_scheme == "EULER" ? 0 : integ_scheme == "RK2" ? 1 : 2;
@@ -570,6 +571,85 @@ void IncFluid::Output_global(IncSF& T)
                 << T.CS_entropy << "   " << kmax_eta1 << " " << kmax_eta2 << "    " << nu << " " << kappa << "    "
                 << Rlambda << "    " << Tdt << "    " << mv[0] << " " << mv[1] << " " << mv[2] << "\t" << mt[0]
                 << std::endl;
+}
+
+// Compute_force_RB compute_force_RB.cc:51-105 (PRLARGE, ULARGE) of the last RK stage of Time_advance(T), which is
+// what Force1..3 and T.Force hold when RB_slip_main.cc:363-402 writes its outputs: Force1 = T, Force2 = Force3 = 0,
+// T.Force = V1, all of the stage field and dealiased.  The fused step keeps that field in its stage buffer; before
+// the first step (and on the plain stepper, which does not report it) the forces are left zero.
+void IncFluid::rb_forces(DField fv[3], DField& ft)
+{
+    for (int c = 0; c < 3; c++) fv[c] = DField(elems_, esize_);
+    ft = DField(elems_, esize_);
+    if (!rb_stepped_ || !rstate_) return;
+    check(th_state_download_buffer(rstate_, 1, 3, fv[0].data(), 0), "th_state_download_buffer");
+    check(th_state_download_buffer(rstate_, 1, 0, ft.data(), 0), "th_state_download_buffer");
+}
+
+// Columns as output_energy.cc:429-446.  The scalar dissipation column prints IncFluid's own CS_shell_dissk
+// (inherited through NLIN : CSF, Nlin.h:51), which nothing computes, so it is 0 (Csf.cc:120-121).
+void IncFluid::Output_shell_spectrum(IncSF& T)
+{
+    spectrum_file << "%% Time = " << Tnow << std::endl;
+    const int n = shell_ek_size();
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    std::vector<double> ek[3], hk[3], cr[3], et(n), dk(n, 0.0), t(n), fvk(n), ftk(n);
+    for (int c = 0; c < 3; c++) {
+        ek[c].resize(n);
+        hk[c].resize(n);
+        cr[c].resize(n);
+        check(th_shell_spectrum(plan_, V[c], nullptr, 0, ek[c].data(), nullptr), "CV_Compute_shell_spectrum");
+        check(th_shell_spectrum(plan_, V[c], nullptr, 2, t.data(), nullptr), "CV_Compute_shell_spectrum");
+        for (int i = 0; i < n; i++) dk[i] += 2.0 * t[i];
+        // Compute_cross_vT_shell_spectrum hc.cc:112-124
+        check(th_shell_spectrum(plan_, V[c], T.F.data(), 0, cr[c].data(), nullptr), "Compute_cross_vT_shell_spectrum");
+    }
+    check(th_shell_spectrum(plan_, T.F.data(), nullptr, 0, et.data(), nullptr), "T.CS_Compute_shell_spectrum");
+    check(th_shell_spectrum_helicity(plan_, V, hk[0].data(), hk[1].data(), hk[2].data()), "CV_Compute_shell_spectrum_helicity");
+    DField fv[3], ft;
+    rb_forces(fv, ft);
+    // Compute_force_shell_spectrum force_spectrum.cc:48-53 (Force2 = Force3 = 0) and IncSF's :64-68
+    check(th_shell_spectrum(plan_, V[0], fv[0].data(), 0, fvk.data(), nullptr), "Compute_force_shell_spectrum");
+    check(th_shell_spectrum(plan_, T.F.data(), ft.data(), 0, ftk.data(), nullptr), "T.Compute_force_shell_spectrum");
+    const double nu = dissipation_coefficient;
+    for (int i = 0; i < n; i++)
+        spectrum_file << i << "\t" << ek[0][i] << " " << ek[1][i] << " " << ek[2][i] << "\t" << et[i] << "\t" << nu * dk[i]
+                      << " " << T.diffusion_coefficient * 0.0 << "\t" << cr[0][i] << " " << cr[1][i] << " " << cr[2][i]
+                      << "\t" << hk[0][i] << "  " << hk[1][i] << "  " << hk[2][i] << "\t" << 2 * fvk[i] << " " << 2 * ftk[i]
+                      << std::endl;
+    spectrum_file << std::endl;
+}
+
+void IncFluid::Compute_flux(IncSF& T)
+{
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    std::vector<double> out((size_t)2 * (no_spheres + 1), 0.0);
+    check(th_flux_scalar(plan_, V, T.F.data(), sphere_radius.data(), no_spheres, out.data()), "Compute_flux_RB");
+    flux_self.assign(out.begin(), out.begin() + no_spheres + 1);
+    flux_SF.assign(out.begin() + no_spheres + 1, out.end());
+}
+
+// Columns as output_ET.cc:96-104: flux_self, flux_SF, then the force feeds of Compute_force_feed_shell_scalar
+// compute_force_feed.cc:94-100 (Shell_mult_all(Force, V) and Shell_mult_all(T.Force, T.F) over shell_radius).
+void IncFluid::Output_flux(IncSF& T)
+{
+    flux_file << "%% Time = " << Tnow << std::endl;
+    Compute_flux(T);
+    DField fv[3], ft;
+    rb_forces(fv, ft);
+    DField z(elems_, esize_);
+    std::vector<double> feedV(shell_radius.size(), 0.0), feedT(shell_radius.size(), 0.0);
+    void* F[3] = {fv[0].data(), fv[1].data(), fv[2].data()};
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    void* A[3] = {ft.data(), z.data(), z.data()};
+    void* B[3] = {T.F.data(), z.data(), z.data()};
+    check(th_shell_mult_all(plan_, F, V, shell_radius.data(), (int)shell_radius.size(), feedV.data()), "Compute_force_feed_shell");
+    check(th_shell_mult_all(plan_, A, B, shell_radius.data(), (int)shell_radius.size(), feedT.data()), "Compute_force_feed_shell_scalar");
+    for (int i = 1; i <= no_spheres; i++)
+        flux_file << i << " " << sphere_radius[i] << " " << flux_self[i] << " " << flux_SF[i] << "  "
+                  << (i < (int)feedV.size() ? feedV[i] : 0.0) << " " << (i < (int)feedT.size() ? feedT[i] : 0.0) << "     "
+                  << std::endl;
+    flux_file << std::endl;
 }
 
 // Force1..3 as the reference holds them when its time loop writes the outputs (Ifluid_main.cc:278-298):

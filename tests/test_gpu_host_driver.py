@@ -261,7 +261,8 @@ def test_driver_rb_glob_matches_oracle(tmp_path):
     kf = (math.pi, math.pi / math.sqrt(2), math.pi / math.sqrt(2))
     o = Oracle((N, N, N), kfactor=kf, basis="SCFT")
     V, T = o.zeros(3), o.ic_random_scalar(4, 1e-3)
-    o.rk4_rb(V, T, Ra, Pr, dt, steps=steps)
+    last = o.zeros(4)
+    o.rk4_rb(V, T, Ra, Pr, dt, steps=steps, last_stage=last)
     nu, kappa = math.sqrt(Pr / Ra), 1 / math.sqrt(Pr * Ra)
     Ev, Et = o.total_energy(V), o.energy(T)
     assert abs(glob[-1][1] - Ev) <= 1e-12 * Ev and abs(glob[-1][2] - Et) <= 1e-12 * Et
@@ -285,3 +286,43 @@ def test_driver_rb_glob_matches_oracle(tmp_path):
             continue
         sc = max(abs(b), hs) if 8 <= i <= 11 else max(abs(b), 1e-300)
         assert abs(a - b) <= 1e-10 * sc, (i, a, b)
+
+    # spectrum.d (Output_shell_spectrum_scalar output_energy.cc:410-449) and flux.d (output_ET.cc:81-115);
+    # the forces are those of the last RK stage: Force1 = T_s, T.Force = V1_s, dealiased
+    Ts, V1s = last[3].copy(), last[0].copy()
+    o.dealias(Ts)
+    o.dealias(V1s)
+    Z = o.zeros()
+    ek = [o.shell_spectrum(V[c])[0] for c in range(3)]
+    et = o.shell_spectrum(T)[0]
+    dk = 2 * sum(o.shell_spectrum(V[c], n=2)[0] for c in range(3))
+    cr = [o.shell_spectrum(V[c], T)[0] for c in range(3)]
+    hk = o.shell_helicity(V)
+    fv = 2 * o.shell_spectrum(V[0], Ts)[0]
+    ft = 2 * o.shell_spectrum(T, V1s)[0]
+    cols = [ek[0], ek[1], ek[2], et, nu * dk, np.zeros_like(et), cr[0], cr[1], cr[2], hk[0], hk[1], hk[2], fv, ft]
+    rows = [r for r in (tmp_path / "out" / "spectrum.d").read_text().split("\n")[1:] if r.strip()]
+    assert len(rows) == len(et)
+    for i, r in enumerate(rows):
+        f = r.split("\t")
+        assert int(f[0]) == i and len(f) == 7
+        got = [float(x) for x in " ".join(f[1:]).split()]
+        assert len(got) == 14
+        for j, (a, col) in enumerate(zip(got, cols)):
+            sc = np.abs(col).max()
+            if j in (9, 10, 11):                     # helicity: natural scale, the field carries none
+                sc = max(sc, kmax ** 2 * math.sqrt(Ev * Dv))
+            assert abs(a - col[i]) <= 1e-10 * max(sc, 1e-300), (i, j, a, col[i])
+    assert np.abs(fv).max() > 0 and np.abs(ft).max() > 0
+    radii = np.concatenate([np.arange(0, N // 3 + 2, dtype=float), [10000.0]])
+    fs, fsf = o.flux_scalar(V, T, radii)
+    feedV = o.shell_mult_all([Ts, Z, Z], V, radii)
+    feedT = o.shell_mult_all([V1s, Z, Z], [T, Z, Z], radii)
+    frows = [r for r in (tmp_path / "out" / "flux.d").read_text().split("\n")[1:] if r.strip()]
+    assert len(frows) == len(radii) - 1
+    for r in frows:
+        x = r.split()
+        i = int(x[0])
+        assert len(x) == 6 and float(x[1]) == radii[i]
+        for tok, col in zip(x[2:], (fs, fsf, feedV, feedT)):
+            assert abs(float(tok) - col[i]) <= 1e-10 * max(np.abs(col).max(), 1e-300), (r, tok, col[i])
