@@ -484,6 +484,8 @@ extern "C" int th_state_set_force(th_state* s, const th_force* f)
 {
     if (!s) { set_error("th_state_set_force: null state"); return E_ARG; }
     TH_CHECK_HIP(hipSetDevice(s->plan->ctx->device));
+    // steps already enqueued read the force buffer and the captured graph: let them finish first
+    TH_CHECK_HIP(hipStreamSynchronize(s->plan->ctx->stream));
     if (!f || f->proc == 0) {
         s->forced = false;
     } else {
@@ -510,6 +512,10 @@ extern "C" int th_state_set_force(th_state* s, const th_force* f)
         }
         const size_t need = (size_t)nb * 6;
         if (need > s->fbox_cap) {
+            // the old buffer goes away: the state is unforced (and its captured step void) until this succeeds
+            s->forced = false;
+            if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+            if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
             if (s->fbox) TH_CHECK_HIP(hipFree(s->fbox));
             s->fbox = nullptr;
             s->fbox_cap = 0;
