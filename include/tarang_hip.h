@@ -167,6 +167,11 @@ int th_shell_spectrum_helicity(th_plan *plan, void *const V[3], double *H1k1, do
 int th_ring_spectrum(th_plan *plan, void *const V[3], int nsect, int n, int axis, double *S1k, double *S2k);
 int th_cylinder_ring_spectrum(th_plan *plan, void *const V[3], int nslab, int n, int axis, double *S1k, double *S2k);
 int th_cylinder_shell_ek_size(const th_plan *plan, int axis);   /* Anis_max_Krho_radius_inside + 1 */
+/* Helicity tables on the same bins: th_ring_spectrum_helicity == Compute_ring_spectrum_helicity_FOUR
+ * four_energy.cc:1357-1447, th_cylinder_ring_spectrum_helicity == Compute_cylinder_ring_spectrum_helicity_FOUR
+ * :1825-1912; H1k[row][col] = sum of 2 mult K.(Vr x Vi). */
+int th_ring_spectrum_helicity(th_plan *plan, void *const V[3], int nsect, int axis, double *H1k);
+int th_cylinder_ring_spectrum_helicity(th_plan *plan, void *const V[3], int nslab, int axis, double *H1k);
 int th_entropy(th_plan *plan, void *const V[3], double *out);
 /* == Get_entropy_scalar_FOUR four_energy.cc:761-826 / _SCFT scft_energy.cc:750-800 (CSF::CS_Compute_entropy
  * Csf.cc:306-309): the spectral entropy of one field in bits */

@@ -1647,6 +1647,49 @@ void orc_cylinder_ring_spectrum(const orc_grid *g, double *const V[3], int nslab
     free(arr);
 }
 
+/* Compute_local_ring_spectrum_helicity_FOUR four_energy.cc:1357-1412 (mode 0, bins of orc_ring_spectrum) and
+ * Compute_local_cylinder_ring_spectrum_helicity_FOUR :1825-1880 (mode 1, bins of orc_cylinder_ring_spectrum):
+ * H[row][col] += 2 mult K.(Vr x Vi) */
+void orc_aniso_helicity(const orc_grid *g, double *const V[3], int mode, int nbin, int a, double *Hk)
+{
+    int N1 = g->N1, N2 = g->N2, cols = nbin + 1, rows;
+    double *arr = (double *)malloc(sizeof(double) * cols), c3[3];
+    if (mode == 0) {
+        rows = orc_max_radius_inside(g) + 1;
+        double dth = (M_PI / 2) / nbin;
+        for (int i = 0; i < nbin; i++) arr[i] = i * dth;
+        arr[nbin] = M_PI / 2;
+    } else {
+        rows = orc_cylinder_shell_ek_size(g, a);
+        double kmax = aniso_max_kpll(g, a), dk = kmax / nbin;
+        arr[0] = 0.0;
+        for (int i = 1; i < nbin; i++) arr[i] = i * dk;
+        arr[nbin] = kmax;
+    }
+    for (int i = 0; i < rows * cols; i++) Hk[i] = 0;
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 <= g->N3 / 2; l3++) {
+                double km = orc_kmag(g, l1, l2, l3), kpll = aniso_kpll(g, a, l1, l2, l3);
+                double kperp = aniso_kperp(g, a, l1, l2, l3);
+                int row, col;
+                if (mode == 0) {
+                    row = (int)ceil(km);
+                    if (!(km > MYEPS && row <= rows - 1)) continue;
+                    col = sector_index(polar_angle(kperp, kpll), arr, cols);
+                } else {
+                    row = (int)ceil(kperp);
+                    if (row > rows - 1) continue;
+                    col = slab_index(kpll, arr, cols);
+                }
+                vr_cross_vi_b(g, V, IDX(g, l1, l2, l3), c3);
+                double kk[3] = {orc_kx(g, l1) * g->f1, orc_ky(g, l2) * g->f2, l3 * g->f3};
+                double h = 2 * orc_multiplicity(g, l1, l2, l3) * (kk[0] * c3[0] + kk[1] * c3[1] + kk[2] * c3[2]);
+                add2(Hk, rows, cols, row, col, h);
+            }
+    free(arr);
+}
+
 /* Get_local_entropy_scalar_FOUR four_energy.cc:761-798 / _SCFT scft_energy.cc:750-777: the entropy of one
  * field, p = modal energy / Get_total_energy of that field, weights as orc_entropy */
 double orc_entropy_scalar(const orc_grid *g, const double *A)

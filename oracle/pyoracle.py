@@ -81,6 +81,7 @@ def lib():
             "orc_helicity": (None, [p, pp, vp]),
             "orc_ring_spectrum": (None, [p, pp, i, i, i, vp, vp]),
             "orc_cylinder_shell_ek_size": (i, [p, i]),
+            "orc_aniso_helicity": (None, [p, pp, i, i, i, vp]),
             "orc_cylinder_ring_spectrum": (None, [p, pp, i, i, i, vp, vp]),
             "orc_shell_helicity": (None, [p, pp, vp, vp, vp]),
             "orc_entropy": (d, [p, pp]),
@@ -344,6 +345,18 @@ class Oracle:
         self.L.orc_cylinder_ring_spectrum(self.gp, _pp(V), nslab, n, axis, S1.ctypes.data_as(C.c_void_p),
                                           S2.ctypes.data_as(C.c_void_p))
         return S1, S2
+
+    def ring_spectrum_helicity(self, V, nsect, axis=3):
+        """H1k of Compute_ring_spectrum_helicity_FOUR, shape (Max_radius_inside+1, nsect+1)."""
+        H = np.zeros((self.max_radius_inside() + 1, nsect + 1))
+        self.L.orc_aniso_helicity(self.gp, _pp(V), 0, nsect, axis, H.ctypes.data_as(C.c_void_p))
+        return H
+
+    def cylinder_ring_spectrum_helicity(self, V, nslab, axis=3):
+        """H1k of Compute_cylinder_ring_spectrum_helicity_FOUR, shape (Anis_max_Krho_radius_inside+1, nslab+1)."""
+        H = np.zeros((self.L.orc_cylinder_shell_ek_size(self.gp, axis), nslab + 1))
+        self.L.orc_aniso_helicity(self.gp, _pp(V), 1, nslab, axis, H.ctypes.data_as(C.c_void_p))
+        return H
 
     def shell_helicity(self, V):
         n = self.shell_ek_size()

@@ -98,6 +98,7 @@ void   orc_helicity(const orc_grid *g, double *const V[3], double *out);
  * orc_cylinder_shell_ek_size(g, a) x (nslab+1), both row-major */
 void   orc_ring_spectrum(const orc_grid *g, double *const V[3], int nsect, int n, int a, double *S1, double *S2);
 int    orc_cylinder_shell_ek_size(const orc_grid *g, int a);
+void   orc_aniso_helicity(const orc_grid *g, double *const V[3], int mode, int nbin, int a, double *Hk);
 void   orc_cylinder_ring_spectrum(const orc_grid *g, double *const V[3], int nslab, int n, int a, double *S1,
                                   double *S2);
 void   orc_shell_helicity(const orc_grid *g, double *const V[3], double *Hk1, double *Hk2, double *Hk3);

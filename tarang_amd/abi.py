@@ -123,6 +123,8 @@ def lib():
         "th_ring_spectrum": (i, [vp, pp, i, i, i, dp, dp]),
         "th_cylinder_ring_spectrum": (i, [vp, pp, i, i, i, dp, dp]),
         "th_cylinder_shell_ek_size": (i, [vp, i]),
+        "th_ring_spectrum_helicity": (i, [vp, pp, i, i, dp]),
+        "th_cylinder_ring_spectrum_helicity": (i, [vp, pp, i, i, dp]),
         "th_state_download_buffer": (i, [vp, i, i, vp, i]),
         "th_flux_shell_tr": (i, [vp, vp, dp, i, dp, dp]),
         "th_profile_passes": (i, [vp, vp, d, C.POINTER(Phys), i, i, dp, dp]),
@@ -417,6 +419,27 @@ class Plan:
         _check(lib().th_cylinder_ring_spectrum(self.h, _pp(V), nslab, n, axis, S1.ctypes.data_as(C.POINTER(C.c_double)),
                                                S2.ctypes.data_as(C.POINTER(C.c_double))), "th_cylinder_ring_spectrum")
         return S1, S2
+
+    def ring_spectrum_helicity(self, V, nsect, axis=3):
+        """H1k of Compute_ring_spectrum_helicity_FOUR, shape (Max_radius_inside+1, nsect+1)."""
+        import numpy as np
+        H = np.zeros((lib().th_max_radius_inside(self.h) + 1, nsect + 1))
+        self._pre()
+        _check(lib().th_ring_spectrum_helicity(self.h, _pp(V), nsect, axis, H.ctypes.data_as(C.POINTER(C.c_double))),
+               "th_ring_spectrum_helicity")
+        return H
+
+    def cylinder_ring_spectrum_helicity(self, V, nslab, axis=3):
+        """H1k of Compute_cylinder_ring_spectrum_helicity_FOUR, shape (Anis_max_Krho_radius_inside+1, nslab+1)."""
+        import numpy as np
+        rows = lib().th_cylinder_shell_ek_size(self.h, axis)
+        _check(0 if rows > 0 else rows, "th_cylinder_shell_ek_size")
+        H = np.zeros((rows, nslab + 1))
+        self._pre()
+        _check(lib().th_cylinder_ring_spectrum_helicity(self.h, _pp(V), nslab, axis,
+                                                        H.ctypes.data_as(C.POINTER(C.c_double))),
+               "th_cylinder_ring_spectrum_helicity")
+        return H
 
     def compute_force(self, V, proc, inner, outer, p3, p4=0.0):
         """Force field F[3] of force_field_proc `proc` for the velocity V, and nf."""

@@ -964,6 +964,18 @@ extern "C" int th_cylinder_ring_spectrum(th_plan* p, void* const V[3], int nslab
     if (!V || !V[0] || !V[1] || !V[2] || !S1k || !S2k) { set_error("th_cylinder_ring_spectrum: null"); return E_ARG; }
     return DISPATCH(p, do_aniso, p, V, 1, nslab, n, axis, S1k, S2k);
 }
+extern "C" int th_ring_spectrum_helicity(th_plan* p, void* const V[3], int nsect, int axis, double* H1k)
+{
+    TRY(check_plan(p));
+    if (!V || !V[0] || !V[1] || !V[2] || !H1k) { set_error("th_ring_spectrum_helicity: null"); return E_ARG; }
+    return DISPATCH(p, do_aniso, p, V, 2, nsect, 0, axis, H1k, (double*)nullptr);
+}
+extern "C" int th_cylinder_ring_spectrum_helicity(th_plan* p, void* const V[3], int nslab, int axis, double* H1k)
+{
+    TRY(check_plan(p));
+    if (!V || !V[0] || !V[1] || !V[2] || !H1k) { set_error("th_cylinder_ring_spectrum_helicity: null"); return E_ARG; }
+    return DISPATCH(p, do_aniso, p, V, 3, nslab, 0, axis, H1k, (double*)nullptr);
+}
 extern "C" int th_cylinder_shell_ek_size(const th_plan* p, int axis)
 {
     if (!p || axis < 1 || axis > 3) { set_error("th_cylinder_shell_ek_size: bad args"); return E_ARG; }

@@ -292,7 +292,9 @@ struct AnisoP {
     double dpar, pmax;
 };
 
-template <typename R, int MODE>
+//   HEL: the helicity tables instead (Compute_local_ring_spectrum_helicity_FOUR four_energy.cc:1357-1412,
+//   Compute_local_cylinder_ring_spectrum_helicity_FOUR :1825-1880): S1 += 2 mult K.(Vr x Vi), S2 untouched.
+template <typename R, int MODE, bool HEL>
 __global__ __launch_bounds__(RBLK) void k_aniso(GeoR g, const cplx<R>* __restrict__ A0, const cplx<R>* __restrict__ A1,
                                                 const cplx<R>* __restrict__ A2, AnisoP ap, double* __restrict__ partial)
 {
@@ -350,6 +352,12 @@ __global__ __launch_bounds__(RBLK) void k_aniso(GeoR g, const cplx<R>* __restric
         }
         const long flat = (long)row * ap.cols + col;
         if (flat >= nb) continue;
+        if (HEL) {
+            double x[3];
+            vr_x_vi(A0[i], A1[i], A2[i], x);
+            atomicAdd(&sh[flat], 2 * mult_r(g, l1, l2, l3) * (kk[0] * x[0] + kk[1] * x[1] + kk[2] * x[2]));
+            continue;
+        }
         const cplx<R> v[3] = {A0[i], A1[i], A2[i]};
         const double w = mult_r(g, l1, l2, l3) * my_pow_r(km, ap.n);
         const double v2 = sq(v[0]) + sq(v[1]) + sq(v[2]);
@@ -629,10 +637,13 @@ int cylinder_shell_ek_size(const Geo& g, int axis)
     return (int)(k1 < k2 ? k1 : k2) + 1;
 }
 
+// mode 0 ring / 1 cylinder energy spectra (S1, S2); mode 2 / 3 the ring / cylinder helicity tables (S1 only)
 template <typename R>
 int reduce_aniso(const Geo& g, const cplx<R>* const V[3], int mode, int nbin, int n, int axis, double* S1, double* S2,
                  RedScratch& rs, hipStream_t st)
 {
+    const bool hel = mode >= 2;
+    mode &= 1;
     if (g.basis != 0 || g.N2 < 2 || g.N3 < 4) { set_error("anisotropic spectra: 3-D FOUR boxes only"); return E_UNSUPPORTED; }
     if (axis < 1 || axis > 3 || nbin < 1 || (n != 0 && n != 2 && n != 4)) {
         set_error("anisotropic spectra: axis must be 1..3, bins >= 1, n in {0, 2, 4}");
@@ -665,12 +676,15 @@ int reduce_aniso(const Geo& g, const cplx<R>* const V[3], int mode, int nbin, in
     int r = rs.ensure((size_t)nblk * nb2 + nb2, nb2);
     if (r) return r;
     double* out = rs.d + (size_t)nblk * nb2;
-    if (mode == 0)
-        hipLaunchKernelGGL((k_aniso<R, 0>), dim3(nblk), dim3(RBLK), nb2 * sizeof(double), st, gr(g), V[0], V[1], V[2],
-                           ap, rs.d);
+    const size_t lds = nb2 * sizeof(double);
+    if (mode == 0 && !hel)
+        hipLaunchKernelGGL((k_aniso<R, 0, false>), dim3(nblk), dim3(RBLK), lds, st, gr(g), V[0], V[1], V[2], ap, rs.d);
+    else if (mode == 1 && !hel)
+        hipLaunchKernelGGL((k_aniso<R, 1, false>), dim3(nblk), dim3(RBLK), lds, st, gr(g), V[0], V[1], V[2], ap, rs.d);
+    else if (mode == 0)
+        hipLaunchKernelGGL((k_aniso<R, 0, true>), dim3(nblk), dim3(RBLK), lds, st, gr(g), V[0], V[1], V[2], ap, rs.d);
     else
-        hipLaunchKernelGGL((k_aniso<R, 1>), dim3(nblk), dim3(RBLK), nb2 * sizeof(double), st, gr(g), V[0], V[1], V[2],
-                           ap, rs.d);
+        hipLaunchKernelGGL((k_aniso<R, 1, true>), dim3(nblk), dim3(RBLK), lds, st, gr(g), V[0], V[1], V[2], ap, rs.d);
     TH_CHECK_HIP(hipGetLastError());
     launch_finalize(rs.d, nblk, nb2, nb2, out, st);
     TH_CHECK_HIP(hipGetLastError());
@@ -679,7 +693,7 @@ int reduce_aniso(const Geo& g, const cplx<R>* const V[3], int mode, int nbin, in
     const int nb = ap.rows * ap.cols;
     for (int k = 0; k < nb; k++) {
         S1[k] = rs.h[k];
-        S2[k] = rs.h[nb + k];
+        if (S2) S2[k] = rs.h[nb + k];
     }
     return OK;
 }

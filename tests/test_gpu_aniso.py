@@ -54,6 +54,11 @@ def test_ring_and_cylinder_spectra_match_oracle(ctx, N, kf, axis, prec, tol):
             assert h1.shape == c1.shape
             m = max(np.abs(c1).max(), np.abs(c2).max())
             assert np.abs(h1 - c1).max() <= tol * m and np.abs(h2 - c2).max() <= tol * m
+        # helicity tables on the same bins (four_energy.cc:1357-1412, 1825-1880)
+        rh, gh = o.ring_spectrum_helicity(V, nb, axis), p.ring_spectrum_helicity(dV, nb, axis)
+        assert gh.shape == rh.shape and np.abs(gh - rh).max() <= tol * np.abs(rh).max()
+        ch, gc = o.cylinder_ring_spectrum_helicity(V, nb, axis), p.cylinder_ring_spectrum_helicity(dV, nb, axis)
+        assert gc.shape == ch.shape and np.abs(gc - ch).max() <= tol * np.abs(ch).max()
 
 
 def test_aniso_exact_boundary_modes(ctx):

@@ -114,3 +114,23 @@ def test_cylinder_slab_location_and_overflow():
     R[3, 0] += 0.01
     assert np.abs(S1).max() == 0.0
     assert np.abs(S2 - R).max() < 1e-15
+
+
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_ring_and_cylinder_helicity_sum_to_the_pinned_helicities(axis):
+    """Compute_ring_spectrum_helicity_FOUR / Compute_cylinder_ring_spectrum_helicity_FOUR (four_energy.cc:1357-1412,
+    1825-1880) bin the modal helicity 2 mult K.(Vr x Vi): summed over sectors the ring table is the shell
+    helicity spectrum (shells <= kmax_inside), and for a field inside every cylindrical ring the cylinder table
+    sums to the total helicity H1."""
+    o = Oracle((16, 16, 16))
+    V = o.ic_random(7)
+    R = o.ring_spectrum_helicity(V, 5, axis)
+    kin = o.max_radius_inside()
+    Hk = np.asarray(o.shell_helicity(V)).sum(axis=0)
+    assert np.abs(R.sum(axis=1) - Hk[:kin + 1]).max() < 1e-13 * np.abs(Hk).max()
+    KX, KY, KZ = np.meshgrid(np.array([o.kx(l) for l in range(16)], float), np.array([o.ky(l) for l in range(16)], float),
+                             np.arange(9, dtype=float), indexing="ij")
+    W = [np.where(np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2) <= 3.0, v, 0) for v in V]
+    C = o.cylinder_ring_spectrum_helicity(W, 4, axis)
+    H1 = o.helicity(W)[0]
+    assert abs(C.sum() - H1) < 1e-13 * max(abs(H1), 1e-300) and abs(H1) > 0
