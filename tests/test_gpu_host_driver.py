@@ -277,7 +277,7 @@ def test_driver_rb_glob_matches_oracle(tmp_path):
     ref = [steps * dt, Ev, Et, Dv, Dt, nu * Dv, kappa * Dt, o.nusselt(V, T, Ra, Pr), h[0], h[1], h[2], h[3],
            o.entropy(V), o.entropy_scalar(T), ke1, ke1 * (nu / kappa) ** 0.25 / (nu / kappa), nu, kappa,
            2 * Ev * (15 / (nu * nu * Dv)) ** 0.5, dt, 0.0, 0.0, 0.0, 0.0]
-    # the RB field carries no helicity beyond rounding: judge H1, H2, DH1, DH2 on their natural scale
+    # the RB field carries no helicity beyond rounding: compare H1, H2, DH1, DH2 on their natural scale
     # kmax^2 sqrt(E D) (Cauchy-Schwarz bound) instead of their own roundoff-sized values
     hs = kmax ** 2 * math.sqrt(Ev * Dv)
     for i, (a, b) in enumerate(zip(v, ref)):
