@@ -605,16 +605,17 @@ template <typename R> static int flux_impl(th_state* s, const double* radii, int
     ShellSel all{-1.0, 1e300};
     // helper H = U: inverse x, y once, kept in X2 comps 0..2 (B layout)
     TRY((d_xinv<R, 0>(g, S, X1, 3, twx, all, 0, st)));
-    TRY(d_yinv<R>(g, X1, X2, 3, twy, 0, st));
+    // the y passes take the step's tuned launch shapes (the same kernels, any component count)
+    TRY(d_yinv<R>(g, X1, X2, 3, twy, s->var[PASS_YINV], st));
     std::vector<double> hostbuf(2 * nr);
     std::vector<double> Tm((size_t)nr * nr, 0.0), corner((size_t)nr * nr, 0.0);
     double* outd = s->partial + (size_t)nblk * (2 * nr);
     for (int n = 1; n < nsh; n++) {
         ShellSel sel{radii[n - 1], radii[n]};
         TRY((d_xinv<R, 2>(g, S, X1, 3, twx, sel, 0, st)));
-        TRY(d_yinv<R>(g, X1, X2 + 3 * g.cB, 3, twy, 0, st));
+        TRY(d_yinv<R>(g, X1, X2 + 3 * g.cB, 3, twy, s->var[PASS_YINV], st));
         TRY((d_z<R, 2>(g, X2, X3, twz, twn, 0, st)));
-        TRY(d_yfwd<R>(g, X3, X1, 9, twy, 0, st));
+        TRY(d_yfwd<R>(g, X3, X1, 9, twy, s->var[PASS_YFWD], st));
         TRY(d_xfwd_et<R>(g, X1, S, twx, s->radii, nr, kin, s->partial, &nblk, st));
         launch_finalize((const double*)s->partial, nblk, 2 * nr, 2 * nr, outd, st);
         TH_CHECK_HIP(hipGetLastError());

@@ -176,3 +176,25 @@ def test_variants_rb_scft(ctx, ref_rb128, v, prec):
     tol = 1e-12 if prec == "fp64" else 2e-5
     for i in range(4):
         assert rel(H(out[i]), V1[i]) < tol, (var, i)
+
+
+def test_flux_shell_tr_with_tuned_y_variants(ctx):
+    """th_flux_shell_tr runs its y passes with the state's launch shapes: every y-variant pair gives the
+    default-shape result to 1e-12 (128^3, lines long enough for all four y variants)."""
+    import numpy as np
+    from tarang_amd import Plan, State
+    from pyoracle import Oracle
+    N = (128, 128, 64)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    s = State(p)
+    V = o.ic_random(5, kp=3.0)
+    s.upload([p.to_dev(v) for v in V])
+    radii = np.concatenate([np.arange(8, dtype=float), [10000.0]])
+    s.variants = [0, 0, 0, 0, 0]
+    Pi0, T0 = s.flux_shell_tr(radii)
+    for vy in (1, 2, 3):
+        s.variants = [0, vy, 0, vy, 0]
+        Pi, T = s.flux_shell_tr(radii)
+        assert np.abs(Pi - Pi0).max() <= 1e-12 * np.abs(Pi0).max()
+        assert np.abs(T - T0).max() <= 1e-12 * np.abs(T0).max()
