@@ -60,6 +60,7 @@ typedef struct th_force {
     int proc;                       /* force_field_proc: 1 or 2 */
     double inner_radius, outer_radius;
     double p3, p4;
+    double p5, p6;                  /* MHD, proc 1 (IncVF& W version): energy_supply_W, epsh_by_k_epse_W */
 } th_force;
 
 /* ---- context and plans ----
@@ -124,6 +125,13 @@ int th_step_plain(th_plan *plan, int physics, int scheme, void *const fields[], 
  * when the time loop writes its outputs (Ifluid_main.cc:278-298). */
 int th_step_plain_forced(th_plan *plan, int scheme, void *const V[3], double dt, const th_phys *p,
                          const th_force *force, void *const last_stage[3]);
+/* MHD step with the energy/helicity supply of both fields (force_field_proc 1,
+ * Compute_force_const_energy_helicity_supply(IncVF& W) compute_force_ek_hk_supply.cc:294-420: U forced
+ * with p3, p4 and W with p5, p6 on the same band, each part only when its supply per mode exceeds
+ * MYEPS), computed from U and B before the nonlinear term and subtracted from both nlin
+ * (Add_force(IncVF& W) add_force.cc:67-76) at every stage.  FOUR plans, proc 1 only. */
+int th_step_plain_forced_mhd(th_plan *plan, int scheme, void *const U[3], void *const B[3], double dt,
+                             const th_phys *p, const th_force *force);
 /* The force field itself (IncFluid::Compute_force for proc 1 / 2): F = alpha V + beta (i K x V) on the
  * band inner < |K| <= outer, zero elsewhere, dealiased when the band reaches past N/3 and with the
  * reality condition applied.  nf (may be NULL) receives the mode count of Get_number_modes_in_shell.

@@ -1045,6 +1045,44 @@ static void rhs_mhd(const orc_grid *g, stepper_t *s, void *ctx)
     free(p);
 }
 
+/* Compute_force_TO_rhs(IncVF& W) with Compute_force_const_energy_helicity_supply(IncVF& W)
+ * compute_force_ek_hk_supply.cc:294-420: the forces of U (p3, p4) and of W (p5, p6) on one band, each only
+ * when its supply per mode exceeds MYEPS, taken before Compute_nlin and subtracted after it (Add_force(W)
+ * add_force.cc:67-76). */
+typedef struct { double inner, outer, p3, p4, p5, p6; } mhd_force_ctx;
+
+static void rhs_mhd_forced(const orc_grid *g, stepper_t *s, void *ctx)
+{
+    const mhd_force_ctx *fc = (const mhd_force_ctx *)ctx;
+    int nf = orc_force_supply_modes(g, fc->inner, fc->outer);
+    double *FU[3], *FW[3], *p = alloc_field(g);
+    for (int c = 0; c < 3; c++) { FU[c] = alloc_field(g); FW[c] = alloc_field(g); }
+    if (fc->p3 / nf > MYEPS) orc_force_ek_hk(g, 1, s->F, FU, fc->inner, fc->outer, fc->p3, fc->p4);
+    if (fc->p5 / nf > MYEPS) orc_force_ek_hk(g, 1, s->F + 3, FW, fc->inner, fc->outer, fc->p5, fc->p6);
+    orc_nlin_mhd(g, s->F, s->F + 3, s->N, s->N + 3);
+    for (int c = 0; c < 3; c++) { fsub(g, s->N[c], FU[c]); fsub(g, s->N[3 + c], FW[c]); }
+    orc_pressure(g, s->N, p);
+    orc_rhs(g, s->N, p);
+    for (int i = 3; i < 6; i++) fneg(g, s->N[i]);
+    for (int c = 0; c < 3; c++) { free(FU[c]); free(FW[c]); }
+    free(p);
+}
+
+void orc_rk4_step_mhd_forced(const orc_grid *g, double *U[3], double *B[3], double nu, double eta, double dt,
+                             double inner, double outer, double p3, double p4, double p5, double p6)
+{
+    mhd_force_ctx fc = {inner, outer, p3, p4, p5, p6};
+    stepper_t s; memset(&s, 0, sizeof(s));
+    s.nvec = 2; s.nc = 6;
+    for (int i = 0; i < 3; i++) {
+        s.F[i] = U[i]; s.F[3 + i] = B[i];
+        s.N[i] = alloc_field(g); s.N[3 + i] = alloc_field(g);
+        s.diss[i] = nu; s.diss[3 + i] = eta;
+    }
+    rk4_generic(g, &s, dt, rhs_mhd_forced, &fc);
+    for (int i = 0; i < 6; i++) free(s.N[i]);
+}
+
 void orc_rk4_step_mhd(const orc_grid *g, double *U[3], double *B[3], double nu, double eta, double dt)
 {
     stepper_t s; memset(&s, 0, sizeof(s));

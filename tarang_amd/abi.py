@@ -48,9 +48,10 @@ class Phys(C.Structure):
 
 class Force(C.Structure):
     """th_force: force_field_proc 1 (p3 = energy_supply, p4 = epsh_by_k_epse) or 2 (p3 = energy_level,
-    p4 = h_by_k_E) on the band inner_radius < |K| <= outer_radius."""
+    p4 = h_by_k_E) on the band inner_radius < |K| <= outer_radius; p5, p6 are the magnetic field's
+    energy_supply_W and epsh_by_k_epse_W of the MHD step."""
     _fields_ = [("proc", C.c_int), ("inner_radius", C.c_double), ("outer_radius", C.c_double),
-                ("p3", C.c_double), ("p4", C.c_double)]
+                ("p3", C.c_double), ("p4", C.c_double), ("p5", C.c_double), ("p6", C.c_double)]
 
 
 def lib():
@@ -94,6 +95,7 @@ def lib():
         "th_et_nlin": (i, [vp, pp, pp, pp]),
         "th_step_plain": (i, [vp, i, i, pp, d, C.POINTER(Phys)]),
         "th_step_plain_forced": (i, [vp, i, pp, d, C.POINTER(Phys), C.POINTER(Force), pp]),
+        "th_step_plain_forced_mhd": (i, [vp, i, pp, pp, d, C.POINTER(Phys), C.POINTER(Force)]),
         "th_compute_force": (i, [vp, pp, pp, C.POINTER(Force), C.POINTER(C.c_long)]),
         "th_energy": (i, [vp, vp, vp, dp]),
         "th_Sn": (i, [vp, vp, d, dp]),
@@ -392,6 +394,19 @@ class Plan:
                    "th_step_plain_forced")
         self._post(0, "")
         return V
+
+    def step_plain_forced_mhd(self, U, B, dt, nu, eta, inner, outer, p3, p4=0.0, p5=0.0, p6=0.0, scheme=2,
+                              steps=1):
+        """MHD steps with the energy/helicity supply of U (p3, p4) and B (p5, p6) recomputed at every
+        stage (force_field_proc 1)."""
+        ph = Phys(nu, eta, 0.0, 0.0, 0.0)
+        fc = Force(1, inner, outer, p3, p4, p5, p6)
+        self._pre()
+        for _ in range(steps):
+            _check(lib().th_step_plain_forced_mhd(self.h, scheme, _pp(U), _pp(B), dt, C.byref(ph), C.byref(fc)),
+                   "th_step_plain_forced_mhd")
+        self._post(0, "")
+        return U, B
 
     def entropy_scalar(self, A):
         out = C.c_double()

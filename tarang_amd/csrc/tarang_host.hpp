@@ -174,6 +174,11 @@ public:
     // Incompressible MHD with the magnetic field W (W.dissipation_coefficient = eta), the IMHD program's
     // step and outputs (src/IMHD, output_main.cc:311-351):
     void Time_advance(IncVF& W);                     // time_advance_VF_MHD.cc:59-180 (fused or plain)
+    // force_field_proc 1 of an MHD run (Compute_force_const_energy_helicity_supply(IncVF& W)
+    // compute_force_ek_hk_supply.cc:294-420): U takes energy_supply p3 / epsh_by_k_epse p4 and W takes p5 / p6
+    // on the one band; later Time_advance(W) calls recompute both forces at every stage on the plain stepper.
+    // The force columns of the MHD output files are not filled in for forced runs.
+    void Set_force(IncVF& W, double inner_radius, double outer_radius, double p3, double p4, double p5, double p6);
     double Get_dt(IncVF& W);                         // compute_dt.cc:75-101
     void Compute_flux(IncVF& W);                     // compute_flux.cc:148-262 -> flux_mhd
     void Output_global(IncVF& W);                    // output_energy.cc:217-286
@@ -208,6 +213,8 @@ private:
     void rb_forces(DField fv[3], DField& ft);   // Force1..3 and T.Force after Time_advance(T)
     DField last_[3];               // plain forced stepper: the field of the last stage
     th_force force_{};
+    th_force mhd_force_{};
+    bool mhd_forced_ = false;      // Set_force(W, ...) was called
     th_state* state_ = nullptr;
     th_state* mstate_ = nullptr;   // fused MHD state (U, B), created by the first Time_advance(W)
     th_state* rstate_ = nullptr;   // fused RB state (V, T), created by the first Time_advance(T)

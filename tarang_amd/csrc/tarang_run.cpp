@@ -3,14 +3,15 @@
 // energy + NaN check, then the shell spectrum and (at the end) the energy flux.
 //
 //   tarang_run N steps dt nu [tg|random|file=<field_in.d>] [FP64|FP32_MIXED] [fused|plain] [save=<field_out.d>]
-//              [out=DIR] [force=PROC,INNER,OUTER,P3,P4] [mhd=ETA] [rb=RA,PR]
+//              [out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR]
 // Output (stdout): "glob t E D" per step, then "spectrum k E(k)" lines and "flux R Pi(R)" lines.
 // file=: Init_cond from a field file (field_input_proc = 1; its "%% Time" sets Tinit); save=: the
 // final field is appended to a field file (Output_field), the restart point of a later run.
 // out=DIR (any position after nu): DIR/out/glob.d every step, DIR/out/spectrum.d and flux.d at the end,
 // in the reference's formats (Output_global, Output_shell_spectrum, Output_flux).
 // force=PROC,INNER,OUTER,P3,P4: force_field_proc 1 (energy/helicity supply) or 2 (constant energy/helicity)
-// with its force_double_para, recomputed at every stage.
+// with its force_double_para, recomputed at every stage; with mhd=, proc 1 and P5, P6 the magnetic field's
+// energy_supply_W and epsh_by_k_epse_W (0 when left out).
 // mhd=ETA: incompressible MHD (the IMHD program, src/IMHD/IMHD_main.cc) with magnetic diffusivity ETA; U
 // from the IC choice, B random-phase (seed 2); Get_dt(W), Time_advance(W) and the W outputs; "glob t Eu Eb"
 // lines, the spectrum of U and the kinetic flux.
@@ -31,7 +32,7 @@ int main(int argc, char** argv)
 {
     if (argc < 5) {
         fprintf(stderr, "usage: %s N steps dt nu [tg|random|file=PATH] [FP64|FP32_MIXED] [fused|plain] [save=PATH] "
-                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4] [mhd=ETA] [rb=RA,PR]\n",
+                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR]\n",
                 argv[0]);
         return 2;
     }
@@ -42,7 +43,7 @@ int main(int argc, char** argv)
     const std::string path = argc > 7 ? argv[7] : "fused";
     std::string save, outdir;
     int fproc = 0;
-    double fpar[4] = {0, 0, 0, 0};
+    double fpar[6] = {0, 0, 0, 0, 0, 0};
     double eta = -1, Ra = 0, Pr = 0;
     for (int a = 5; a < argc; a++) {
         if (strncmp(argv[a], "save=", 5) == 0) save = argv[a] + 5;
@@ -52,10 +53,13 @@ int main(int argc, char** argv)
             fprintf(stderr, "tarang_run: rb=RA,PR expected\n");
             return 2;
         }
-        if (strncmp(argv[a], "force=", 6) == 0 &&
-            sscanf(argv[a] + 6, "%d,%lf,%lf,%lf,%lf", &fproc, &fpar[0], &fpar[1], &fpar[2], &fpar[3]) != 5) {
-            fprintf(stderr, "tarang_run: force=PROC,INNER,OUTER,P3,P4 expected\n");
-            return 2;
+        if (strncmp(argv[a], "force=", 6) == 0) {
+            const int k = sscanf(argv[a] + 6, "%d,%lf,%lf,%lf,%lf,%lf,%lf", &fproc, &fpar[0], &fpar[1], &fpar[2],
+                                 &fpar[3], &fpar[4], &fpar[5]);
+            if (k != 5 && k != 7) {
+                fprintf(stderr, "tarang_run: force=PROC,INNER,OUTER,P3,P4[,P5,P6] expected\n");
+                return 2;
+            }
         }
     }
     try {
@@ -100,16 +104,17 @@ int main(int argc, char** argv)
             void* V[3] = {U.V1.data(), U.V2.data(), U.V3.data()};
             tarang::check(th_ic_random(U.plan(), V, 1, 0.5, 4.0), "th_ic_random");
         }
-        if (fproc) U.Set_force(fproc, fpar[0], fpar[1], fpar[2], fpar[3]);
+        if (fproc && eta < 0) U.Set_force(fproc, fpar[0], fpar[1], fpar[2], fpar[3]);
         if (!outdir.empty()) U.Open_output_files(outdir);
         if (eta >= 0) {
-            if (fproc) {
-                fprintf(stderr, "tarang_run: forcing of MHD runs is not supported\n");
+            if (fproc && fproc != 1) {
+                fprintf(stderr, "tarang_run: MHD runs take force_field_proc 1 only\n");
                 return 2;
             }
             tarang::IncVF W(ctx, g, eta, N / 3 + 2, 1);
             void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
             tarang::check(th_ic_random(W.plan(), B, 2, 0.5, 4.0), "th_ic_random");
+            if (fproc) U.Set_force(W, fpar[0], fpar[1], fpar[2], fpar[3], fpar[4], fpar[5]);
             for (int it = 0; it < steps; it++) {
                 U.Tdt = U.Get_dt(W);
                 U.Tnow += U.Tdt;

@@ -610,6 +610,7 @@ template <typename R> struct Ops {
     int physics = 0;
     double Ra = 0, Pr = 0;
     const ForceBand* force = nullptr;    // NS forcing (force_field_proc 1 / 2), else none
+    const ForceBand* force_w = nullptr;  // MHD: the band of W's supply (force of U in `force`)
     cplx<R>* keep[3] = {nullptr, nullptr, nullptr};   // receives each forced stage's field (the last stays)
 
     int compute_rhs()
@@ -623,7 +624,19 @@ template <typename R> struct Ops {
             if (force && keep[0])
                 for (int c = 0; c < 3; c++) TRY(copy(keep[c], F[c]));
         } else if (physics == TH_MHD) {
+            // Compute_force(IncVF& W) precedes Compute_nlin, which round-trips U and B through the Elsasser
+            // variables: the forces are taken from U and B first and subtracted afterwards (Add_force(W))
+            cplx<R>* FU[3] = {S(SC_FU0), S(SC_FU0 + 1), S(SC_FU0 + 2)};
+            cplx<R>* FW[3] = {S(SC_FW0), S(SC_FW0 + 1), S(SC_FW0 + 2)};
+            const bool fu = force && force->per_mode > 1e-15, fw = force_w && force_w->per_mode > 1e-15;
+            if ((fu || fw) && (!FU[0] || !FU[1] || !FU[2] || !FW[0] || !FW[1] || !FW[2])) return E_NOMEM;
+            if (fu) TRY(force_write<R>(*force, F, FU, n, st));
+            if (fw) TRY(force_write<R>(*force_w, F + 3, FW, n, st));
             TRY(nlin_mhd(F, F + 3, N, N + 3));
+            for (int c = 0; c < 3; c++) {
+                if (fu) TRY(axpy(N[c], -1.0, FU[c]));
+                if (fw) TRY(axpy(N[3 + c], -1.0, FW[c]));
+            }
         } else {
             // Compute_force_RB compute_force_RB.cc:51-105 (PRLARGE, ULARGE): Force1 = T, T.Force = V1
             cplx<R>* F1 = S(SC_F1);
@@ -854,6 +867,47 @@ template <typename R> static int do_step_plain_forced(th_plan* p, int scheme, vo
     }
     return o.step(scheme, dt);
 }
+template <typename R> static int do_step_plain_forced_mhd(th_plan* p, int scheme, void* const U[3], void* const B[3],
+                                                          double dt, const th_phys* ph, const ForceBand* fu,
+                                                          const ForceBand* fw)
+{
+    Ops<R> o(p);
+    o.physics = TH_MHD;
+    o.nc = 6;
+    o.force = fu;
+    o.force_w = fw;
+    for (int q = 0; q < 6; q++) {
+        o.F[q] = CP(R, q < 3 ? U[q] : B[q - 3]);
+        o.N[q] = o.S(SC_N0 + q);
+        if (!o.N[q]) return E_NOMEM;
+        o.diss[q] = q < 3 ? ph->nu : ph->eta;
+    }
+    return o.step(scheme, dt);
+}
+extern "C" int th_step_plain_forced_mhd(th_plan* p, int scheme, void* const U[3], void* const B[3], double dt,
+                                        const th_phys* ph, const th_force* force)
+{
+    TRY(check_plan(p));
+    if (!U || !B || !U[0] || !U[1] || !U[2] || !B[0] || !B[1] || !B[2] || !ph || scheme < 0 || scheme > 2) {
+        set_error("th_step_plain_forced_mhd: bad args");
+        return E_ARG;
+    }
+    if (force && force->proc != 1) {
+        set_error("th_step_plain_forced_mhd: force_field_proc %d is not supported for MHD (1 only)", force->proc);
+        return E_UNSUPPORTED;
+    }
+    ForceBand fu, fw;
+    TRY(force_setup(p->g, force, fu));
+    if (!isfinite(force->p5) || !isfinite(force->p6)) {
+        set_error("th_step_plain_forced_mhd: energy_supply_W and epsh_by_k_epse_W must be finite");
+        return E_ARG;
+    }
+    fw = fu;                                       // same band and nf; W's own supply and helicity ratio
+    fw.per_mode = force->p5 / (double)fu.nf;
+    fw.p4 = force->p6;
+    return DISPATCH(p, do_step_plain_forced_mhd, p, scheme, U, B, dt, ph, &fu, &fw);
+}
+
 extern "C" int th_step_plain_forced(th_plan* p, int scheme, void* const V[3], double dt, const th_phys* ph,
                                     const th_force* force, void* const last_stage[3])
 {

@@ -128,6 +128,15 @@ template <typename R> int force_subtract(const ForceBand& b, const cplx<R>* cons
 {
     return launch_band<R, 1>(b, V, nl, st);
 }
+template <typename R> int force_write(const ForceBand& b, const cplx<R>* const V[3], cplx<R>* const F[3], size_t n,
+                                      hipStream_t st)
+{
+    for (int c = 0; c < 3; c++) TH_CHECK_HIP(hipMemsetAsync(F[c], 0, n * sizeof(cplx<R>), st));
+    return launch_band<R, 0>(b, V, F, st);
+}
+template int force_write<float>(const ForceBand&, const cplx<float>* const[3], cplx<float>* const[3], size_t, hipStream_t);
+template int force_write<double>(const ForceBand&, const cplx<double>* const[3], cplx<double>* const[3], size_t,
+                                 hipStream_t);
 template int force_subtract<float>(const ForceBand&, const cplx<float>* const[3], cplx<float>* const[3], hipStream_t);
 template int force_subtract<double>(const ForceBand&, const cplx<double>* const[3], cplx<double>* const[3],
                                     hipStream_t);
@@ -138,12 +147,9 @@ template <typename R> static int compute_force(th_plan* p, void* const V[3], voi
 {
     ForceBand b;
     TRY(force_setup(p->g, f, b));
-    hipStream_t st = p->ctx->stream;
-    const size_t bytes = p->g.ncplx() * sizeof(cplx<R>);
-    for (int c = 0; c < 3; c++) TH_CHECK_HIP(hipMemsetAsync(F[c], 0, bytes, st));
     const cplx<R>* v[3] = {(const cplx<R>*)V[0], (const cplx<R>*)V[1], (const cplx<R>*)V[2]};
     cplx<R>* o[3] = {(cplx<R>*)F[0], (cplx<R>*)F[1], (cplx<R>*)F[2]};
-    TRY((launch_band<R, 0>(b, v, o, st)));
+    TRY(force_write<R>(b, v, o, p->g.ncplx(), p->ctx->stream));
     if (nf) *nf = b.nf;
     return OK;
 }

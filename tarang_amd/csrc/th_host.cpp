@@ -406,9 +406,17 @@ double IncFluid::Get_dt(IncVF& W)
 
 void IncFluid::Time_advance(IncVF& W)
 {
-    if (has_force_) throw std::runtime_error("Time_advance(W): forcing of MHD runs is not supported");
+    if (has_force_) throw std::runtime_error("Time_advance(W): Set_force(proc, ...) forces NS runs; use Set_force(W, ...)");
     th_phys ph{dissipation_coefficient, W.dissipation_coefficient, 0, 0, 0};
     void* f[6] = {V1.data(), V2.data(), V3.data(), W.V1.data(), W.V2.data(), W.V3.data()};
+    const int sch = integ_scheme == "EULER" ? 0 : integ_scheme == "RK2" ? 1 : 2;
+    if (mhd_forced_) {
+        void* const U[3] = {f[0], f[1], f[2]};
+        void* const B[3] = {f[3], f[4], f[5]};
+        check(th_step_plain_forced_mhd(plan_, sch, U, B, Tdt, &ph, &mhd_force_), "th_step_plain_forced_mhd");
+        check(th_sync(plan_), "th_sync");
+        return;
+    }
     if (fused_) {
         if (!mstate_) check(th_state_create(plan_, &mstate_, TH_MHD), "th_state_create");
         for (int i = 0; i < 6; i++) check(th_state_upload(mstate_, i, f[i], 0), "th_state_upload");
@@ -417,7 +425,6 @@ void IncFluid::Time_advance(IncVF& W)
         check(th_sync(plan_), "th_sync");
         return;
     }
-    const int sch = integ_scheme == "EULER" ? 0 : integ_scheme == "RK2" ? 1 : 2;
     check(th_step_plain(plan_, TH_MHD, sch, f, Tdt, &ph), "th_step_plain");
     check(th_sync(plan_), "th_sync");
 }
@@ -673,6 +680,26 @@ IncFluid::Force_field::Force_field(IncFluid& u)
     }
     void* F[3] = {f[0].data(), f[1].data(), f[2].data()};
     check(th_compute_force(u.plan_, src, F, &u.force_, nullptr), "Compute_force");
+}
+
+void IncFluid::Set_force(IncVF& W, double inner_radius, double outer_radius, double p3, double p4, double p5,
+                         double p6)
+{
+    th_force f{1, inner_radius, outer_radius, p3, p4, p5, p6};
+    // validate on a zero-length step (the entry point's argument checks) so a bad set fails here
+    th_phys ph{dissipation_coefficient, W.dissipation_coefficient, 0, 0, 0};
+    DField U1(elems_, esize_), U2(elems_, esize_), U3(elems_, esize_), B1(elems_, esize_), B2(elems_, esize_),
+        B3(elems_, esize_);
+    void* const U[3] = {U1.data(), U2.data(), U3.data()};
+    void* const B[3] = {B1.data(), B2.data(), B3.data()};
+    check(th_step_plain_forced_mhd(plan_, 2, U, B, 0.0, &ph, &f), "Set_force(W)");
+    check(th_sync(plan_), "th_sync");
+    mhd_force_ = f;
+    mhd_forced_ = true;
+    if (mstate_) {
+        th_state_destroy(mstate_);
+        mstate_ = nullptr;
+    }
 }
 
 void IncFluid::Set_force(int proc, double inner_radius, double outer_radius, double p3, double p4)

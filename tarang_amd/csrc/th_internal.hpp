@@ -138,6 +138,9 @@ struct ForceBand {
     int nx, ny, nz;           // box extents
 };
 int force_setup(const Geo& g, const th_force* f, ForceBand& b);
+// F = the force of V (zero outside the band)
+template <typename R> int force_write(const ForceBand& b, const cplx<R>* const V[3], cplx<R>* const F[3], size_t n,
+                                      hipStream_t st);
 // nl[c] -= F[c] over the band (Add_force after Compute_force)
 template <typename R> int force_subtract(const ForceBand& b, const cplx<R>* const V[3], cplx<R>* const nl[3],
                                          hipStream_t st);

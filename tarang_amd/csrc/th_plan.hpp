@@ -20,7 +20,8 @@ enum {
     SC_FT = 33,      // T.Force (RB)
     SC_G0 = 34,      // ET giver G1..G3 34..36
     SC_NE0 = 37,     // ET nlin 37..39
-    SC_IC0 = 40,     // IC temporaries
+    SC_FU0 = 40,     // Force1..3 of U (forced MHD) 40..42
+    SC_FW0 = 43,     // W.Force1..3 (forced MHD) 43..45
 };
 struct Fused;        // dealias-pruned pipeline (th_fused.hip)
 }  // namespace th

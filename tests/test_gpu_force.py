@@ -227,3 +227,40 @@ def test_fused_force_band_outside_the_stored_modes(ctx):
     with pytest.raises(RuntimeError, match="single-box NS"):
         from tarang_amd import abi
         State(p, abi.TH_MHD).set_force(1, 1.5, 3.5, 0.5, 0.0)
+
+
+@pytest.mark.parametrize("p3,p4,p5,p6", [(0.5, 0.3, 0.4, -0.2), (0.5, 0.0, 0.0, 0.0), (0.0, 0.0, 0.3, 0.5)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_forced_mhd_steps_match_oracle(ctx, p3, p4, p5, p6, prec, tol):
+    """th_step_plain_forced_mhd: the supply of U (p3, p4) and of W (p5, p6) recomputed at every RK4 stage
+    matches the oracle's forced MHD step; a zero supply leaves its field unforced."""
+    from tarang_amd import Plan
+    log(f"forced mhd p3={p3} p4={p4} p5={p5} p6={p6} {prec}")
+    N = (16, 16, 16)
+    o = Oracle(N)
+    U, B = o.ic_random(3, E0=0.5, kp=3.0), o.ic_random(4, E0=0.3, kp=3.0)
+    p = Plan(ctx, N, precision=prec)
+    gU, gB = [p.to_dev(a) for a in U], [p.to_dev(a) for a in B]
+    if prec == "fp32":
+        U, B = [H(d) for d in gU], [H(d) for d in gB]
+    nu, eta, dt = 0.01, 0.02, 1e-3
+    o.rk4_mhd_forced(U, B, nu, eta, dt, 1.5, 3.5, p3, p4, p5, p6, steps=3)
+    p.step_plain_forced_mhd(gU, gB, dt, nu, eta, 1.5, 3.5, p3, p4, p5, p6, steps=3)
+    for i in range(3):
+        assert rel(H(gU[i]), U[i]) < tol
+        assert rel(H(gB[i]), B[i]) < tol
+
+
+def test_forced_mhd_argument_errors(ctx):
+    from tarang_amd import Plan
+    log("forced mhd errors")
+    p = Plan(ctx, (16, 16, 16))
+    U, B = p.empty(3), p.empty(3)
+    with pytest.raises(RuntimeError, match="not supported for MHD"):
+        from tarang_amd import abi
+        fc = abi.Force(2, 1.5, 3.5, 0.5, 0.2, 0.3, 0.1)
+        ph = abi.Phys(0.01, 0.02, 0.0, 0.0, 0.0)
+        abi._check(abi.lib().th_step_plain_forced_mhd(p.h, 2, abi._pp(U), abi._pp(B), 1e-3, abi.C.byref(ph),
+                                                      abi.C.byref(fc)), "th_step_plain_forced_mhd")
+    with pytest.raises(RuntimeError, match="must be finite"):
+        p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.0, float("nan"), 0.0)

@@ -180,6 +180,22 @@ def test_driver_forced_output_columns(tmp_path, path):
     assert np.abs(fk_new - fk).max() > 1e-6 * np.abs(fk).max()
 
 
+def test_driver_forced_mhd_matches_oracle():
+    """mhd=ETA with force=1,INNER,OUTER,P3,P4,P5,P6: Set_force(W, ...) then Time_advance(W) steps U and B with
+    both supplies recomputed at every stage, the energies equal to the oracle's forced MHD steps."""
+    N, steps, dt, nu, eta = 32, 3, 1e-3, 0.01, 0.02
+    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"mhd={eta}",
+                                  "force=1,1.5,3.5,0.5,0.3,0.4,-0.2")
+    o = Oracle((N, N, N))
+    U, B = o.ic_random(1), o.ic_random(2)
+    o.rk4_mhd_forced(U, B, nu, eta, dt, 1.5, 3.5, 0.5, 0.3, 0.4, -0.2, steps=steps)
+    Eu, Eb = o.total_energy(U), o.total_energy(B)
+    assert abs(glob[-1][1] - Eu) <= 1e-12 * Eu and abs(glob[-1][2] - Eb) <= 1e-12 * Eb
+    U0, B0 = o.ic_random(1), o.ic_random(2)
+    o.rk4_mhd(U0, B0, nu, eta, dt, steps=steps)
+    assert abs(glob[-1][1] - o.total_energy(U0)) > 1e-6       # the force acted
+
+
 def test_driver_mhd_output_files_match_oracle(tmp_path):
     """mhd=ETA: Get_dt(W), Time_advance(W) (fused MHD step) and Output_global(W) / Output_shell_spectrum(W) /
     Output_flux(W) (output_energy.cc:217-286, 463-500, output_ET.cc:118-165) in the reference's column

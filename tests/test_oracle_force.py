@@ -211,3 +211,39 @@ def test_forced_step_with_zero_supply_is_the_free_step(scheme):
     [o.euler_ns, o.rk2_ns, o.rk4_ns][scheme](Vu, 0.01, 1e-3)
     for c in range(3):
         assert np.array_equal(Vf[c], Vu[c])
+
+
+@pytest.mark.parametrize("epsU,epsW", [(0.5, 0.0), (0.0, 0.4), (0.3, 0.2)])
+def test_forced_mhd_step_injects_each_supply_into_its_field(epsU, epsW):
+    """MHD step with the supply of both fields (compute_force_ek_hk_supply.cc:294-420) from U and B with
+    real coefficients: E_U grows by epsU dt and E_B by epsW dt over the free step, up to O(dt^2) (the
+    remainder of a forced field halving with dt); a supply sent to the wrong field fails the split."""
+    o = Oracle((16, 16, 16))
+    nu, eta = 0.01, 0.02
+    U0, B0 = _real_coefficient_field(o, 4), _real_coefficient_field(o, 9)
+    U0 = [u * math.sqrt(0.5 / o.total_energy(U0)) for u in U0]
+    B0 = [b * math.sqrt(0.3 / o.total_energy(B0)) for b in B0]
+    res = []
+    for dt in (2e-3, 1e-3):
+        Uf, Bf = o.rk4_mhd_forced([u.copy() for u in U0], [b.copy() for b in B0], nu, eta, dt, 1.5, 3.5,
+                                  epsU, 0.0, epsW, 0.0)
+        Uu, Bu = o.rk4_mhd([u.copy() for u in U0], [b.copy() for b in B0], nu, eta, dt)
+        res.append((abs((o.total_energy(Uf) - o.total_energy(Uu)) / dt - epsU),
+                    abs((o.total_energy(Bf) - o.total_energy(Bu)) / dt - epsW)))
+    for f, eps in enumerate((epsU, epsW)):
+        assert res[1][f] < 2e-4 * 0.5
+        if eps > 0:                       # the unforced field's change is of higher order
+            assert 0.4 < res[1][f] / res[0][f] < 0.6
+
+
+def test_forced_mhd_step_with_zero_supply_is_the_free_step():
+    """Both supplies zero (below MYEPS per mode): no force is added and the step equals the free MHD step
+    bit for bit."""
+    o = Oracle((16, 16, 16))
+    U0, B0 = o.ic_random(6, E0=0.5, kp=3.0), o.ic_random(7, E0=0.3, kp=3.0)
+    Uf, Bf = o.rk4_mhd_forced([u.copy() for u in U0], [b.copy() for b in B0], 0.01, 0.02, 1e-3, 1.5, 3.5,
+                              0.0, 0.3, 0.0, 0.2)
+    Uu, Bu = o.rk4_mhd([u.copy() for u in U0], [b.copy() for b in B0], 0.01, 0.02, 1e-3)
+    for c in range(3):
+        assert np.array_equal(Uf[c], Uu[c])
+        assert np.array_equal(Bf[c], Bu[c])
