@@ -129,9 +129,11 @@ int th_step_plain_forced(th_plan *plan, int scheme, void *const V[3], double dt,
  * Compute_force_const_energy_helicity_supply(IncVF& W) compute_force_ek_hk_supply.cc:294-420: U forced
  * with p3, p4 and W with p5, p6 on the same band, each part only when its supply per mode exceeds
  * MYEPS), computed from U and B before the nonlinear term and subtracted from both nlin
- * (Add_force(IncVF& W) add_force.cc:67-76) at every stage.  FOUR plans, proc 1 only. */
+ * (Add_force(IncVF& W) add_force.cc:67-76) at every stage.  FOUR plans, proc 1 only.  last_stage (may be
+ * NULL) receives U1..U3, B1..B3 of the step's last stage, the fields whose forces the reference's Force1..3
+ * and W.Force1..3 hold when the MHD outputs are written (output_energy.cc:463-500, output_ET.cc:118-165). */
 int th_step_plain_forced_mhd(th_plan *plan, int scheme, void *const U[3], void *const B[3], double dt,
-                             const th_phys *p, const th_force *force);
+                             const th_phys *p, const th_force *force, void *const last_stage[6]);
 /* The force field itself (IncFluid::Compute_force for proc 1 / 2): F = alpha V + beta (i K x V) on the
  * band inner < |K| <= outer, zero elsewhere, dealiased when the band reaches past N/3 and with the
  * reality condition applied.  nf (may be NULL) receives the mode count of Get_number_modes_in_shell.

@@ -1049,7 +1049,7 @@ static void rhs_mhd(const orc_grid *g, stepper_t *s, void *ctx)
  * compute_force_ek_hk_supply.cc:294-420: the forces of U (p3, p4) and of W (p5, p6) on one band, each only
  * when its supply per mode exceeds MYEPS, taken before Compute_nlin and subtracted after it (Add_force(W)
  * add_force.cc:67-76). */
-typedef struct { double inner, outer, p3, p4, p5, p6; } mhd_force_ctx;
+typedef struct { double inner, outer, p3, p4, p5, p6; double **keep; } mhd_force_ctx;
 
 static void rhs_mhd_forced(const orc_grid *g, stepper_t *s, void *ctx)
 {
@@ -1059,6 +1059,8 @@ static void rhs_mhd_forced(const orc_grid *g, stepper_t *s, void *ctx)
     for (int c = 0; c < 3; c++) { FU[c] = alloc_field(g); FW[c] = alloc_field(g); }
     if (fc->p3 / nf > MYEPS) orc_force_ek_hk(g, 1, s->F, FU, fc->inner, fc->outer, fc->p3, fc->p4);
     if (fc->p5 / nf > MYEPS) orc_force_ek_hk(g, 1, s->F + 3, FW, fc->inner, fc->outer, fc->p5, fc->p6);
+    if (fc->keep)                       /* the fields these (so far last) forces were computed from */
+        for (int c = 0; c < 6; c++) fcopy(g, fc->keep[c], s->F[c]);
     orc_nlin_mhd(g, s->F, s->F + 3, s->N, s->N + 3);
     for (int c = 0; c < 3; c++) { fsub(g, s->N[c], FU[c]); fsub(g, s->N[3 + c], FW[c]); }
     orc_pressure(g, s->N, p);
@@ -1068,10 +1070,11 @@ static void rhs_mhd_forced(const orc_grid *g, stepper_t *s, void *ctx)
     free(p);
 }
 
+/* keep (may be NULL): 6 fields receiving the last stage's U1..U3, B1..B3 */
 void orc_rk4_step_mhd_forced(const orc_grid *g, double *U[3], double *B[3], double nu, double eta, double dt,
-                             double inner, double outer, double p3, double p4, double p5, double p6)
+                             double inner, double outer, double p3, double p4, double p5, double p6, double *keep[6])
 {
-    mhd_force_ctx fc = {inner, outer, p3, p4, p5, p6};
+    mhd_force_ctx fc = {inner, outer, p3, p4, p5, p6, keep};
     stepper_t s; memset(&s, 0, sizeof(s));
     s.nvec = 2; s.nc = 6;
     for (int i = 0; i < 3; i++) {

@@ -95,7 +95,7 @@ def lib():
         "th_et_nlin": (i, [vp, pp, pp, pp]),
         "th_step_plain": (i, [vp, i, i, pp, d, C.POINTER(Phys)]),
         "th_step_plain_forced": (i, [vp, i, pp, d, C.POINTER(Phys), C.POINTER(Force), pp]),
-        "th_step_plain_forced_mhd": (i, [vp, i, pp, pp, d, C.POINTER(Phys), C.POINTER(Force)]),
+        "th_step_plain_forced_mhd": (i, [vp, i, pp, pp, d, C.POINTER(Phys), C.POINTER(Force), pp]),
         "th_compute_force": (i, [vp, pp, pp, C.POINTER(Force), C.POINTER(C.c_long)]),
         "th_energy": (i, [vp, vp, vp, dp]),
         "th_Sn": (i, [vp, vp, d, dp]),
@@ -396,14 +396,16 @@ class Plan:
         return V
 
     def step_plain_forced_mhd(self, U, B, dt, nu, eta, inner, outer, p3, p4=0.0, p5=0.0, p6=0.0, scheme=2,
-                              steps=1):
+                              steps=1, last_stage=None):
         """MHD steps with the energy/helicity supply of U (p3, p4) and B (p5, p6) recomputed at every
-        stage (force_field_proc 1)."""
+        stage (force_field_proc 1); last_stage (6 device arrays U1..U3, B1..B3, optional) receives the
+        fields of the last stage."""
         ph = Phys(nu, eta, 0.0, 0.0, 0.0)
         fc = Force(1, inner, outer, p3, p4, p5, p6)
         self._pre()
         for _ in range(steps):
-            _check(lib().th_step_plain_forced_mhd(self.h, scheme, _pp(U), _pp(B), dt, C.byref(ph), C.byref(fc)),
+            _check(lib().th_step_plain_forced_mhd(self.h, scheme, _pp(U), _pp(B), dt, C.byref(ph), C.byref(fc),
+                                                  _pp(last_stage) if last_stage is not None else None),
                    "th_step_plain_forced_mhd")
         self._post(0, "")
         return U, B

@@ -177,7 +177,7 @@ public:
     // force_field_proc 1 of an MHD run (Compute_force_const_energy_helicity_supply(IncVF& W)
     // compute_force_ek_hk_supply.cc:294-420): U takes energy_supply p3 / epsh_by_k_epse p4 and W takes p5 / p6
     // on the one band; later Time_advance(W) calls recompute both forces at every stage on the plain stepper.
-    // The force columns of the MHD output files are not filled in for forced runs.
+    // The force columns of the MHD outputs come from the forces of the last RK stage's U and W.
     void Set_force(IncVF& W, double inner_radius, double outer_radius, double p3, double p4, double p5, double p6);
     double Get_dt(IncVF& W);                         // compute_dt.cc:75-101
     void Compute_flux(IncVF& W);                     // compute_flux.cc:148-262 -> flux_mhd
@@ -215,6 +215,9 @@ private:
     th_force force_{};
     th_force mhd_force_{};
     bool mhd_forced_ = false;      // Set_force(W, ...) was called
+    bool mhd_stepped_ = false;     // a forced MHD step has run since Set_force(W, ...)
+    DField mlast_[6];              // forced MHD stepper: U1..U3, B1..B3 of the last stage
+    void mhd_forces(DField fu[3], DField fw[3]);   // Force1..3 and W.Force1..3 after Time_advance(W)
     th_state* state_ = nullptr;
     th_state* mstate_ = nullptr;   // fused MHD state (U, B), created by the first Time_advance(W)
     th_state* rstate_ = nullptr;   // fused RB state (V, T), created by the first Time_advance(T)

@@ -611,7 +611,7 @@ template <typename R> struct Ops {
     double Ra = 0, Pr = 0;
     const ForceBand* force = nullptr;    // NS forcing (force_field_proc 1 / 2), else none
     const ForceBand* force_w = nullptr;  // MHD: the band of W's supply (force of U in `force`)
-    cplx<R>* keep[3] = {nullptr, nullptr, nullptr};   // receives each forced stage's field (the last stays)
+    cplx<R>* keep[6] = {};   // receives each forced stage's fields (NS: V, MHD: U then B; the last stays)
 
     int compute_rhs()
     {
@@ -632,6 +632,8 @@ template <typename R> struct Ops {
             if ((fu || fw) && (!FU[0] || !FU[1] || !FU[2] || !FW[0] || !FW[1] || !FW[2])) return E_NOMEM;
             if (fu) TRY(force_write<R>(*force, F, FU, n, st));
             if (fw) TRY(force_write<R>(*force_w, F + 3, FW, n, st));
+            if (keep[0])
+                for (int c = 0; c < 6; c++) TRY(copy(keep[c], F[c]));
             TRY(nlin_mhd(F, F + 3, N, N + 3));
             for (int c = 0; c < 3; c++) {
                 if (fu) TRY(axpy(N[c], -1.0, FU[c]));
@@ -869,13 +871,15 @@ template <typename R> static int do_step_plain_forced(th_plan* p, int scheme, vo
 }
 template <typename R> static int do_step_plain_forced_mhd(th_plan* p, int scheme, void* const U[3], void* const B[3],
                                                           double dt, const th_phys* ph, const ForceBand* fu,
-                                                          const ForceBand* fw)
+                                                          const ForceBand* fw, void* const last[6])
 {
     Ops<R> o(p);
     o.physics = TH_MHD;
     o.nc = 6;
     o.force = fu;
     o.force_w = fw;
+    if (last)
+        for (int c = 0; c < 6; c++) o.keep[c] = CP(R, last[c]);
     for (int q = 0; q < 6; q++) {
         o.F[q] = CP(R, q < 3 ? U[q] : B[q - 3]);
         o.N[q] = o.S(SC_N0 + q);
@@ -885,13 +889,16 @@ template <typename R> static int do_step_plain_forced_mhd(th_plan* p, int scheme
     return o.step(scheme, dt);
 }
 extern "C" int th_step_plain_forced_mhd(th_plan* p, int scheme, void* const U[3], void* const B[3], double dt,
-                                        const th_phys* ph, const th_force* force)
+                                        const th_phys* ph, const th_force* force, void* const last_stage[6])
 {
     TRY(check_plan(p));
     if (!U || !B || !U[0] || !U[1] || !U[2] || !B[0] || !B[1] || !B[2] || !ph || scheme < 0 || scheme > 2) {
         set_error("th_step_plain_forced_mhd: bad args");
         return E_ARG;
     }
+    if (last_stage)
+        for (int c = 0; c < 6; c++)
+            if (!last_stage[c]) { set_error("th_step_plain_forced_mhd: null last_stage field"); return E_ARG; }
     if (force && force->proc != 1) {
         set_error("th_step_plain_forced_mhd: force_field_proc %d is not supported for MHD (1 only)", force->proc);
         return E_UNSUPPORTED;
@@ -905,7 +912,7 @@ extern "C" int th_step_plain_forced_mhd(th_plan* p, int scheme, void* const U[3]
     fw = fu;                                       // same band and nf; W's own supply and helicity ratio
     fw.per_mode = force->p5 / (double)fu.nf;
     fw.p4 = force->p6;
-    return DISPATCH(p, do_step_plain_forced_mhd, p, scheme, U, B, dt, ph, &fu, &fw);
+    return DISPATCH(p, do_step_plain_forced_mhd, p, scheme, U, B, dt, ph, &fu, &fw, last_stage);
 }
 
 extern "C" int th_step_plain_forced(th_plan* p, int scheme, void* const V[3], double dt, const th_phys* ph,

@@ -413,7 +413,12 @@ void IncFluid::Time_advance(IncVF& W)
     if (mhd_forced_) {
         void* const U[3] = {f[0], f[1], f[2]};
         void* const B[3] = {f[3], f[4], f[5]};
-        check(th_step_plain_forced_mhd(plan_, sch, U, B, Tdt, &ph, &mhd_force_), "th_step_plain_forced_mhd");
+        if (!mlast_[0].data())
+            for (int c = 0; c < 6; c++) mlast_[c] = DField(elems_, esize_);
+        void* const last[6] = {mlast_[0].data(), mlast_[1].data(), mlast_[2].data(),
+                               mlast_[3].data(), mlast_[4].data(), mlast_[5].data()};
+        check(th_step_plain_forced_mhd(plan_, sch, U, B, Tdt, &ph, &mhd_force_, last), "th_step_plain_forced_mhd");
+        mhd_stepped_ = true;
         check(th_sync(plan_), "th_sync");
         return;
     }
@@ -473,8 +478,8 @@ void IncFluid::Output_global(IncVF& W)
 }
 
 // Columns as output_energy.cc:478-497.  shell_ek_cross is printed but never filled on this path in the
-// reference (Compute_cross_helicity_shell_spectrum hc.cc:67-71 has no caller), so it is 0, and so are the
-// force spectra of an unforced MHD run.
+// reference (Compute_cross_helicity_shell_spectrum hc.cc:67-71 has no caller), so it is 0; the force spectra
+// of an unforced MHD run are 0.
 void IncFluid::Output_shell_spectrum(IncVF& W)
 {
     spectrum_file << "%% Time = " << Tnow << std::endl;
@@ -496,12 +501,24 @@ void IncFluid::Output_shell_spectrum(IncVF& W)
     }
     check(th_shell_spectrum_helicity(plan_, U, hu[0].data(), hu[1].data(), hu[2].data()), "CV_Compute_shell_spectrum_helicity");
     check(th_shell_spectrum_helicity(plan_, B, hw[0].data(), hw[1].data(), hw[2].data()), "W.CV_Compute_shell_spectrum_helicity");
+    // Compute_force_shell_spectrum force_spectrum.cc:48-53 of U and of W (0 for an unforced run)
+    std::vector<double> fku(n, 0.0), fkw(n, 0.0);
+    if (mhd_forced_) {
+        DField fu[3], fw[3];
+        mhd_forces(fu, fw);
+        for (int c = 0; c < 3; c++) {
+            check(th_shell_spectrum(plan_, U[c], fu[c].data(), 0, t.data(), nullptr), "Compute_force_shell_spectrum");
+            for (int i = 0; i < n; i++) fku[i] += 2.0 * t[i];
+            check(th_shell_spectrum(plan_, B[c], fw[c].data(), 0, t.data(), nullptr), "W.Compute_force_shell_spectrum");
+            for (int i = 0; i < n; i++) fkw[i] += 2.0 * t[i];
+        }
+    }
     const double nu = dissipation_coefficient, eta = W.dissipation_coefficient;
     for (int i = 0; i < n; i++)
         spectrum_file << i << "\t" << eu[0][i] << " " << eu[1][i] << " " << eu[2][i] << "\t" << ew[0][i] << " " << ew[1][i]
                       << " " << ew[2][i] << "\t" << nu * du[i] << " " << eta * dw[i] << "    " << 0.0 << "    " << hu[0][i]
                       << " " << hu[1][i] << " " << hu[2][i] << "\t" << hw[0][i] << " " << hw[1][i] << " " << hw[2][i]
-                      << "\t" << 0.0 << " " << 0.0 << std::endl;
+                      << "\t" << fku[i] << " " << fkw[i] << std::endl;
     spectrum_file << std::endl;
 }
 
@@ -514,10 +531,25 @@ void IncFluid::Output_flux(IncVF& W)
     Compute_flux(W);
     const size_t s = (size_t)no_spheres + 1;
     auto F = [&](int row, int i) { return flux_mhd[row * s + i]; };
+    // Compute_force_feed_shell(W) compute_force_feed.cc:130-139: Shell_mult_all(F, U) and (W.Force, W)
+    std::vector<double> feu(shell_radius.size(), 0.0), few(shell_radius.size(), 0.0);
+    if (mhd_forced_) {
+        DField fu[3], fw[3];
+        mhd_forces(fu, fw);
+        void* Fu[3] = {fu[0].data(), fu[1].data(), fu[2].data()};
+        void* Fw[3] = {fw[0].data(), fw[1].data(), fw[2].data()};
+        void* U[3] = {V1.data(), V2.data(), V3.data()};
+        void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+        check(th_shell_mult_all(plan_, Fu, U, shell_radius.data(), (int)shell_radius.size(), feu.data()),
+              "Compute_force_feed_shell");
+        check(th_shell_mult_all(plan_, Fw, B, shell_radius.data(), (int)shell_radius.size(), few.data()),
+              "Compute_force_feed_shell(W)");
+    }
+    auto at = [](const std::vector<double>& v, int i) { return i < (int)v.size() ? v[i] : 0.0; };
     for (int i = 1; i <= no_spheres; i++)
         flux_file << i << " " << sphere_radius[i] << " " << F(0, i) << " " << F(2, i) << " " << F(3, i) << "   "
                   << F(1, i) << " " << F(4, i) << " " << F(5, i) << " " << 0.0 << "    " << F(7, i) << " " << F(8, i)
-                  << "    " << 0.0 << " " << 0.0 << "    " << std::endl;
+                  << "    " << at(feu, i) << " " << at(few, i) << "    " << std::endl;
     flux_file << std::endl;
 }
 
@@ -682,6 +714,31 @@ IncFluid::Force_field::Force_field(IncFluid& u)
     check(th_compute_force(u.plan_, src, F, &u.force_, nullptr), "Compute_force");
 }
 
+// Compute_force_const_energy_helicity_supply(IncVF& W) compute_force_ek_hk_supply.cc:294-420 on the fields of
+// the last RK stage (the stage loop recomputes both forces, so that is what Force1..3 and W.Force1..3 hold
+// when the outputs are written); a part whose supply per mode is below MYEPS is never computed and stays 0,
+// as before the first step.
+void IncFluid::mhd_forces(DField fu[3], DField fw[3])
+{
+    for (int c = 0; c < 3; c++) {
+        fu[c] = DField(elems_, esize_);
+        fw[c] = DField(elems_, esize_);
+    }
+    if (!mhd_stepped_) return;
+    const double supply[2] = {mhd_force_.p3, mhd_force_.p5}, ratio[2] = {mhd_force_.p4, mhd_force_.p6};
+    for (int part = 0; part < 2; part++) {
+        DField* out = part ? fw : fu;
+        th_force f{1, mhd_force_.inner_radius, mhd_force_.outer_radius, supply[part], ratio[part], 0, 0};
+        void* const src[3] = {mlast_[3 * part].data(), mlast_[3 * part + 1].data(), mlast_[3 * part + 2].data()};
+        void* F[3] = {out[0].data(), out[1].data(), out[2].data()};
+        long nf = 0;
+        check(th_compute_force(plan_, src, F, &f, &nf), "Compute_force(W)");
+        check(th_sync(plan_), "th_sync");
+        if (!(supply[part] / (double)nf > 1e-15))
+            for (int c = 0; c < 3; c++) out[c] = DField(elems_, esize_);
+    }
+}
+
 void IncFluid::Set_force(IncVF& W, double inner_radius, double outer_radius, double p3, double p4, double p5,
                          double p6)
 {
@@ -692,10 +749,11 @@ void IncFluid::Set_force(IncVF& W, double inner_radius, double outer_radius, dou
         B3(elems_, esize_);
     void* const U[3] = {U1.data(), U2.data(), U3.data()};
     void* const B[3] = {B1.data(), B2.data(), B3.data()};
-    check(th_step_plain_forced_mhd(plan_, 2, U, B, 0.0, &ph, &f), "Set_force(W)");
+    check(th_step_plain_forced_mhd(plan_, 2, U, B, 0.0, &ph, &f, nullptr), "Set_force(W)");
     check(th_sync(plan_), "th_sync");
     mhd_force_ = f;
     mhd_forced_ = true;
+    mhd_stepped_ = false;
     if (mstate_) {
         th_state_destroy(mstate_);
         mstate_ = nullptr;

@@ -261,6 +261,6 @@ def test_forced_mhd_argument_errors(ctx):
         fc = abi.Force(2, 1.5, 3.5, 0.5, 0.2, 0.3, 0.1)
         ph = abi.Phys(0.01, 0.02, 0.0, 0.0, 0.0)
         abi._check(abi.lib().th_step_plain_forced_mhd(p.h, 2, abi._pp(U), abi._pp(B), 1e-3, abi.C.byref(ph),
-                                                      abi.C.byref(fc)), "th_step_plain_forced_mhd")
+                                                      abi.C.byref(fc), None), "th_step_plain_forced_mhd")
     with pytest.raises(RuntimeError, match="must be finite"):
         p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.0, float("nan"), 0.0)

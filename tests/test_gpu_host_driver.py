@@ -180,20 +180,49 @@ def test_driver_forced_output_columns(tmp_path, path):
     assert np.abs(fk_new - fk).max() > 1e-6 * np.abs(fk).max()
 
 
-def test_driver_forced_mhd_matches_oracle():
+@pytest.mark.parametrize("fp", [(1.5, 3.5, 0.5, 0.3, 0.4, -0.2), (1.5, 3.5, 0.5, 0.3, 0.0, 0.0)])
+def test_driver_forced_mhd_matches_oracle(tmp_path, fp):
     """mhd=ETA with force=1,INNER,OUTER,P3,P4,P5,P6: Set_force(W, ...) then Time_advance(W) steps U and B with
-    both supplies recomputed at every stage, the energies equal to the oracle's forced MHD steps."""
+    both supplies recomputed at every stage, the energies equal to the oracle's forced MHD steps.  The force
+    spectra of spectrum.d (force_spectrum.cc:48-53 for U and W) and the force feeds of flux.d
+    (compute_force_feed.cc:130-139) come from the forces of the last RK stage's U and B (Force1..3 and
+    W.Force1..3 after Time_advance(W)); a zero W supply leaves W's columns 0."""
     N, steps, dt, nu, eta = 32, 3, 1e-3, 0.01, 0.02
-    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"mhd={eta}",
-                                  "force=1,1.5,3.5,0.5,0.3,0.4,-0.2")
+    (tmp_path / "out").mkdir()
+    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"mhd={eta}", f"out={tmp_path}",
+                                  "force=1," + ",".join(str(v) for v in fp))
     o = Oracle((N, N, N))
     U, B = o.ic_random(1), o.ic_random(2)
-    o.rk4_mhd_forced(U, B, nu, eta, dt, 1.5, 3.5, 0.5, 0.3, 0.4, -0.2, steps=steps)
+    last = o.zeros(6)
+    o.rk4_mhd_forced(U, B, nu, eta, dt, *fp, steps=steps, last_stage=last)
     Eu, Eb = o.total_energy(U), o.total_energy(B)
     assert abs(glob[-1][1] - Eu) <= 1e-12 * Eu and abs(glob[-1][2] - Eb) <= 1e-12 * Eb
     U0, B0 = o.ic_random(1), o.ic_random(2)
     o.rk4_mhd(U0, B0, nu, eta, dt, steps=steps)
     assert abs(glob[-1][1] - o.total_energy(U0)) > 1e-6       # the force acted
+    inner, outer, p3, p4, p5, p6 = fp
+    FU = o.force(last[:3], 1, inner, outer, p3, p4)
+    FW = o.force(last[3:], 1, inner, outer, p5, p6) if p5 > 0 else o.zeros(3)
+    fku = 2 * sum(o.shell_spectrum(U[c], FU[c])[0] for c in range(3))
+    fkw = 2 * sum(o.shell_spectrum(B[c], FW[c])[0] for c in range(3))
+    rows = [r for r in (tmp_path / "out" / "spectrum.d").read_text().split("\n")[1:] if r.strip()]
+    assert len(rows) == len(fku) and np.abs(fku).max() > 0
+    sc = np.abs(fku).max()
+    for i, r in enumerate(rows):
+        a, b = (float(x) for x in r.split("\t")[-1].split())
+        assert abs(a - fku[i]) <= 1e-10 * sc and abs(b - fkw[i]) <= 1e-10 * sc, (i, r)
+    radii = np.concatenate([np.arange(0, N // 3 + 2, dtype=float), [10000.0]])
+    feu, few = o.shell_mult_all(FU, U, radii), o.shell_mult_all(FW, B, radii)
+    frows = [r for r in (tmp_path / "out" / "flux.d").read_text().split("\n")[1:] if r.strip()]
+    sc = np.abs(feu).max()
+    assert sc > 0 and len(frows) > 0
+    for r in frows:
+        t = r.split()
+        assert len(t) == 13, r
+        i = int(t[0])
+        assert abs(float(t[11]) - feu[i]) <= 1e-10 * sc and abs(float(t[12]) - few[i]) <= 1e-10 * sc, r
+    if p5 > 0:
+        assert np.abs(fkw).max() > 0 and np.abs(few).max() > 0
 
 
 def test_driver_mhd_output_files_match_oracle(tmp_path):
