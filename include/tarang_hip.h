@@ -60,7 +60,8 @@ typedef struct th_force {
     int proc;                       /* force_field_proc: 1 or 2 */
     double inner_radius, outer_radius;
     double p3, p4;
-    double p5, p6;                  /* MHD, proc 1 (IncVF& W version): energy_supply_W, epsh_by_k_epse_W */
+    double p5, p6;                  /* MHD (IncVF& W versions): proc 1 energy_supply_W, epsh_by_k_epse_W;
+                                       proc 2 energy_level_W, h_by_k_E_W */
 } th_force;
 
 /* ---- context and plans ----
@@ -125,11 +126,12 @@ int th_step_plain(th_plan *plan, int physics, int scheme, void *const fields[], 
  * when the time loop writes its outputs (Ifluid_main.cc:278-298). */
 int th_step_plain_forced(th_plan *plan, int scheme, void *const V[3], double dt, const th_phys *p,
                          const th_force *force, void *const last_stage[3]);
-/* MHD step with the energy/helicity supply of both fields (force_field_proc 1,
- * Compute_force_const_energy_helicity_supply(IncVF& W) compute_force_ek_hk_supply.cc:294-420: U forced
- * with p3, p4 and W with p5, p6 on the same band, each part only when its supply per mode exceeds
- * MYEPS), computed from U and B before the nonlinear term and subtracted from both nlin
- * (Add_force(IncVF& W) add_force.cc:67-76) at every stage.  FOUR plans, proc 1 only.  last_stage (may be
+/* MHD step with the forcing of both fields (force_field_proc 1,
+ * Compute_force_const_energy_helicity_supply(IncVF& W) compute_force_ek_hk_supply.cc:294-420, or 2,
+ * Compute_force_const_energy_helicity(IncVF& W) compute_force_const_ek_hk.cc:296-437: U forced with p3, p4
+ * and W with p5, p6 on the same band, each part only when its level per mode exceeds MYEPS), computed from
+ * U and B before the nonlinear term and subtracted from both nlin (Add_force(IncVF& W)
+ * add_force.cc:67-76) at every stage.  FOUR plans.  last_stage (may be
  * NULL) receives U1..U3, B1..B3 of the step's last stage, the fields whose forces the reference's Force1..3
  * and W.Force1..3 hold when the MHD outputs are written (output_energy.cc:463-500, output_ET.cc:118-165). */
 int th_step_plain_forced_mhd(th_plan *plan, int scheme, void *const U[3], void *const B[3], double dt,
@@ -140,6 +142,13 @@ int th_step_plain_forced_mhd(th_plan *plan, int scheme, void *const U[3], void *
  * Errors: TH_E_UNSUPPORTED for other procs or an SCFT plan; TH_E_ARG for an empty band, or for proc 2
  * with |h_by_k_E| > 1, where the reference's square roots are not real. */
 int th_compute_force(th_plan *plan, void *const V[3], void *F[3], const th_force *force, long *nf);
+/* Both forces of an MHD run (IncFluid::Compute_force(IncVF& W) compute_force_main.cc:88-105, proc 1 / 2): FU
+ * from U with p3, p4 and FW from B with p5, p6, each zero when its level per mode is not above MYEPS.
+ * Under proc 2 a mode of B is forced when its modal energy is > 0, a mode of U when it is > MYEPS
+ * (compute_force_const_ek_hk.cc:376-402).  Errors as th_compute_force, and TH_E_ARG for non-finite p5, p6
+ * or, under proc 2, |h_by_k_E_W| > 1. */
+int th_compute_force_mhd(th_plan *plan, void *const U[3], void *const B[3], void *FU[3], void *FW[3],
+                         const th_force *force, long *nf);
 
 /* ---- energies, spectra, transfers (return host doubles; synchronise) ----
  * th_energy   == Get_total_energy four_energy.cc:88-168 (B == NULL: |A|^2; else Re A conj B)

@@ -859,8 +859,34 @@ int orc_force_supply_modes(const orc_grid *g, double inner, double outer)
  * p4 = h_by_k_E): F = alpha V + beta (i K x V) on the modes inner < |K| <= outer with modal energy
  * > MYEPS, zero elsewhere; the vorticity term only when beta > MYEPS.  abs(sk*sk - 1) is the double
  * overload.  Returns nf (0: no mode in the band, F untouched). */
+static int force_ek_hk_e(const orc_grid *g, int proc, double *const V[3], double *F[3], double inner,
+                         double outer, double p3, double p4, double emin);
+
 int orc_force_ek_hk(const orc_grid *g, int proc, double *const V[3], double *F[3], double inner, double outer,
                     double p3, double p4)
+{
+    return force_ek_hk_e(g, proc, V, F, inner, outer, p3, p4, MYEPS);
+}
+
+/* Compute_force(IncVF& W) compute_force_main.cc:88-105 for proc 1 / 2: FU from U (p3, p4), FW from B (p5, p6),
+ * each left zero when its level per mode is not above MYEPS; under proc 2 a mode of B is forced when its modal
+ * energy is > 0 (compute_force_const_ek_hk.cc:376-402), every other part when it is > MYEPS.  Returns nf. */
+int orc_force_mhd(const orc_grid *g, int proc, double *const U[3], double *const B[3], double *FU[3], double *FW[3],
+                  double inner, double outer, double p3, double p4, double p5, double p6)
+{
+    int nf = orc_force_supply_modes(g, inner, outer);
+    for (int c = 0; c < 3; c++) {
+        memset(FU[c], 0, 2 * NCPLX(g) * sizeof(double));
+        memset(FW[c], 0, 2 * NCPLX(g) * sizeof(double));
+    }
+    if (nf <= 0) return 0;
+    if (p3 / nf > MYEPS) force_ek_hk_e(g, proc, U, FU, inner, outer, p3, p4, MYEPS);
+    if (p5 / nf > MYEPS) force_ek_hk_e(g, proc, B, FW, inner, outer, p5, p6, proc == 2 ? 0.0 : MYEPS);
+    return nf;
+}
+
+static int force_ek_hk_e(const orc_grid *g, int proc, double *const V[3], double *F[3], double inner,
+                         double outer, double p3, double p4, double emin)
 {
     int nf = orc_force_supply_modes(g, inner, outer);
     if (nf <= 0) return 0;
@@ -880,7 +906,7 @@ int orc_force_ek_hk(const orc_grid *g, int proc, double *const V[3], double *F[3
                     E += pow2(vr[c]) + pow2(vi[c]);
                 }
                 E = E / 2;
-                if (!(E > MYEPS)) continue;
+                if (!(E > emin)) continue;
                 double kk[3] = {orc_kx(g, l1) * g->f1, orc_ky(g, l2) * g->f2, l3 * g->f3};
                 double hel = kk[0] * (vr[1] * vi[2] - vr[2] * vi[1]) + kk[1] * (vr[2] * vi[0] - vr[0] * vi[2]) +
                              kk[2] * (vr[0] * vi[1] - vr[1] * vi[0]);
@@ -1045,11 +1071,10 @@ static void rhs_mhd(const orc_grid *g, stepper_t *s, void *ctx)
     free(p);
 }
 
-/* Compute_force_TO_rhs(IncVF& W) with Compute_force_const_energy_helicity_supply(IncVF& W)
- * compute_force_ek_hk_supply.cc:294-420: the forces of U (p3, p4) and of W (p5, p6) on one band, each only
- * when its supply per mode exceeds MYEPS, taken before Compute_nlin and subtracted after it (Add_force(W)
- * add_force.cc:67-76). */
-typedef struct { double inner, outer, p3, p4, p5, p6; double **keep; } mhd_force_ctx;
+/* Compute_force_TO_rhs(IncVF& W) with Compute_force(IncVF& W) for proc 1 (compute_force_ek_hk_supply.cc:294-420)
+ * or 2 (compute_force_const_ek_hk.cc:296-437): the forces of U (p3, p4) and of W (p5, p6) on one band, taken
+ * before Compute_nlin and subtracted after it (Add_force(W) add_force.cc:67-76). */
+typedef struct { int proc; double inner, outer, p3, p4, p5, p6; double **keep; } mhd_force_ctx;
 
 static void rhs_mhd_forced(const orc_grid *g, stepper_t *s, void *ctx)
 {
@@ -1057,8 +1082,8 @@ static void rhs_mhd_forced(const orc_grid *g, stepper_t *s, void *ctx)
     int nf = orc_force_supply_modes(g, fc->inner, fc->outer);
     double *FU[3], *FW[3], *p = alloc_field(g);
     for (int c = 0; c < 3; c++) { FU[c] = alloc_field(g); FW[c] = alloc_field(g); }
-    if (fc->p3 / nf > MYEPS) orc_force_ek_hk(g, 1, s->F, FU, fc->inner, fc->outer, fc->p3, fc->p4);
-    if (fc->p5 / nf > MYEPS) orc_force_ek_hk(g, 1, s->F + 3, FW, fc->inner, fc->outer, fc->p5, fc->p6);
+    (void)nf;
+    orc_force_mhd(g, fc->proc, s->F, s->F + 3, FU, FW, fc->inner, fc->outer, fc->p3, fc->p4, fc->p5, fc->p6);
     if (fc->keep)                       /* the fields these (so far last) forces were computed from */
         for (int c = 0; c < 6; c++) fcopy(g, fc->keep[c], s->F[c]);
     orc_nlin_mhd(g, s->F, s->F + 3, s->N, s->N + 3);
@@ -1072,9 +1097,10 @@ static void rhs_mhd_forced(const orc_grid *g, stepper_t *s, void *ctx)
 
 /* keep (may be NULL): 6 fields receiving the last stage's U1..U3, B1..B3 */
 void orc_rk4_step_mhd_forced(const orc_grid *g, double *U[3], double *B[3], double nu, double eta, double dt,
-                             double inner, double outer, double p3, double p4, double p5, double p6, double *keep[6])
+                             int proc, double inner, double outer, double p3, double p4, double p5, double p6,
+                             double *keep[6])
 {
-    mhd_force_ctx fc = {inner, outer, p3, p4, p5, p6, keep};
+    mhd_force_ctx fc = {proc, inner, outer, p3, p4, p5, p6, keep};
     stepper_t s; memset(&s, 0, sizeof(s));
     s.nvec = 2; s.nc = 6;
     for (int i = 0; i < 3; i++) {

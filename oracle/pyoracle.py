@@ -65,7 +65,8 @@ def lib():
             "orc_rk4_stage1_ns": (None, [p, pp, d, d]),
             "orc_nlin_mhd": (None, [p, pp, pp, pp, pp]),
             "orc_rk4_step_mhd": (None, [p, pp, pp, d, d, d]),
-            "orc_rk4_step_mhd_forced": (None, [p, pp, pp, d, d, d, d, d, d, d, d, d, pp]),
+            "orc_rk4_step_mhd_forced": (None, [p, pp, pp, d, d, d, i, d, d, d, d, d, d, pp]),
+            "orc_force_mhd": (i, [p, i, pp, pp, pp, pp, d, d, d, d, d, d]),
             "orc_energy": (d, [p, vp, vp]),
             "orc_Sn": (d, [p, vp, d]),
             "orc_shell_spectrum": (None, [p, vp, vp, d, vp, vp]),
@@ -259,13 +260,19 @@ class Oracle:
             self.L.orc_rk4_step_mhd(self.gp, _pp(U), _pp(B), nu, eta, dt)
         return U, B
 
-    def rk4_mhd_forced(self, U, B, nu, eta, dt, inner, outer, p3, p4, p5, p6, steps=1, last_stage=None):
-        """MHD RK4 steps with the energy/helicity supply of U (p3, p4) and W (p5, p6); last_stage (6 arrays,
-        optional) receives U1..U3, B1..B3 of the last stage."""
+    def rk4_mhd_forced(self, U, B, nu, eta, dt, inner, outer, p3, p4, p5, p6, steps=1, last_stage=None, proc=1):
+        """MHD RK4 steps with force_field_proc `proc` forcing of U (p3, p4) and W (p5, p6); last_stage (6
+        arrays, optional) receives U1..U3, B1..B3 of the last stage."""
         for _ in range(steps):
-            self.L.orc_rk4_step_mhd_forced(self.gp, _pp(U), _pp(B), nu, eta, dt, inner, outer, p3, p4, p5, p6,
+            self.L.orc_rk4_step_mhd_forced(self.gp, _pp(U), _pp(B), nu, eta, dt, proc, inner, outer, p3, p4, p5, p6,
                                            _pp(last_stage) if last_stage is not None else None)
         return U, B
+
+    def force_mhd(self, U, B, proc, inner, outer, p3, p4, p5, p6):
+        """Compute_force(IncVF& W): (FU, FW)."""
+        FU, FW = self.zeros(3), self.zeros(3)
+        self.L.orc_force_mhd(self.gp, proc, _pp(U), _pp(B), _pp(FU), _pp(FW), inner, outer, p3, p4, p5, p6)
+        return FU, FW
 
     def nlin_rb(self, V, T):
         nV, nT = self.zeros(3), self.zeros()

@@ -78,7 +78,10 @@ void orc_rk4_stage1_ns(const orc_grid *g, double *V[3], double nu, double dt);
 void orc_nlin_mhd(const orc_grid *g, double *U[3], double *B[3], double *nU[3], double *nB[3]);
 void orc_rk4_step_mhd(const orc_grid *g, double *U[3], double *B[3], double nu, double eta, double dt);
 void orc_rk4_step_mhd_forced(const orc_grid *g, double *U[3], double *B[3], double nu, double eta, double dt,
-                             double inner, double outer, double p3, double p4, double p5, double p6, double *keep[6]);
+                             int proc, double inner, double outer, double p3, double p4, double p5, double p6,
+                             double *keep[6]);
+int orc_force_mhd(const orc_grid *g, int proc, double *const U[3], double *const B[3], double *FU[3], double *FW[3],
+                  double inner, double outer, double p3, double p4, double p5, double p6);
 
 /* ---- energies and spectra (four_energy.cc, scft_energy.cc, Cvf.cc:385-449) ---- */
 double orc_energy(const orc_grid *g, const double *A, const double *B);   /* B==NULL -> A */

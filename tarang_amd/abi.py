@@ -97,6 +97,7 @@ def lib():
         "th_step_plain_forced": (i, [vp, i, pp, d, C.POINTER(Phys), C.POINTER(Force), pp]),
         "th_step_plain_forced_mhd": (i, [vp, i, pp, pp, d, C.POINTER(Phys), C.POINTER(Force), pp]),
         "th_compute_force": (i, [vp, pp, pp, C.POINTER(Force), C.POINTER(C.c_long)]),
+        "th_compute_force_mhd": (i, [vp, pp, pp, pp, pp, C.POINTER(Force), C.POINTER(C.c_long)]),
         "th_energy": (i, [vp, vp, vp, dp]),
         "th_Sn": (i, [vp, vp, d, dp]),
         "th_shell_spectrum": (i, [vp, vp, vp, i, dp, dp]),
@@ -396,12 +397,11 @@ class Plan:
         return V
 
     def step_plain_forced_mhd(self, U, B, dt, nu, eta, inner, outer, p3, p4=0.0, p5=0.0, p6=0.0, scheme=2,
-                              steps=1, last_stage=None):
-        """MHD steps with the energy/helicity supply of U (p3, p4) and B (p5, p6) recomputed at every
-        stage (force_field_proc 1); last_stage (6 device arrays U1..U3, B1..B3, optional) receives the
-        fields of the last stage."""
+                              steps=1, last_stage=None, proc=1):
+        """MHD steps with the force_field_proc `proc` forcing of U (p3, p4) and B (p5, p6) recomputed at every
+        stage; last_stage (6 device arrays U1..U3, B1..B3, optional) receives the fields of the last stage."""
         ph = Phys(nu, eta, 0.0, 0.0, 0.0)
-        fc = Force(1, inner, outer, p3, p4, p5, p6)
+        fc = Force(proc, inner, outer, p3, p4, p5, p6)
         self._pre()
         for _ in range(steps):
             _check(lib().th_step_plain_forced_mhd(self.h, scheme, _pp(U), _pp(B), dt, C.byref(ph), C.byref(fc),
@@ -466,6 +466,16 @@ class Plan:
         self._pre()
         self._post(lib().th_compute_force(self.h, _pp(V), _pp(F), C.byref(fc), C.byref(nf)), "th_compute_force")
         return F, nf.value
+
+    def compute_force_mhd(self, U, B, proc, inner, outer, p3, p4, p5, p6):
+        """Forces (FU[3], FW[3]) of an MHD run (force_field_proc `proc`), and nf."""
+        FU, FW = self.empty(3), self.empty(3)
+        fc = Force(proc, inner, outer, p3, p4, p5, p6)
+        nf = C.c_long()
+        self._pre()
+        self._post(lib().th_compute_force_mhd(self.h, _pp(U), _pp(B), _pp(FU), _pp(FW), C.byref(fc), C.byref(nf)),
+                   "th_compute_force_mhd")
+        return FU, FW, nf.value
 
     # diagnostics
     def energy(self, A, B=None):

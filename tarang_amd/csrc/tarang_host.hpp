@@ -174,11 +174,12 @@ public:
     // Incompressible MHD with the magnetic field W (W.dissipation_coefficient = eta), the IMHD program's
     // step and outputs (src/IMHD, output_main.cc:311-351):
     void Time_advance(IncVF& W);                     // time_advance_VF_MHD.cc:59-180 (fused or plain)
-    // force_field_proc 1 of an MHD run (Compute_force_const_energy_helicity_supply(IncVF& W)
-    // compute_force_ek_hk_supply.cc:294-420): U takes energy_supply p3 / epsh_by_k_epse p4 and W takes p5 / p6
-    // on the one band; later Time_advance(W) calls recompute both forces at every stage on the plain stepper.
-    // The force columns of the MHD outputs come from the forces of the last RK stage's U and W.
-    void Set_force(IncVF& W, double inner_radius, double outer_radius, double p3, double p4, double p5, double p6);
+    // force_field_proc 1 or 2 of an MHD run (Compute_force(IncVF& W) compute_force_main.cc:88-105): U takes
+    // p3 / p4 and W takes p5 / p6 on the one band; later Time_advance(W) calls recompute both forces at every
+    // stage on the plain stepper.  The force columns of the MHD outputs come from the forces of the last RK
+    // stage's U and W.
+    void Set_force(IncVF& W, int force_field_proc, double inner_radius, double outer_radius, double p3, double p4,
+                   double p5, double p6);
     double Get_dt(IncVF& W);                         // compute_dt.cc:75-101
     void Compute_flux(IncVF& W);                     // compute_flux.cc:148-262 -> flux_mhd
     void Output_global(IncVF& W);                    // output_energy.cc:217-286

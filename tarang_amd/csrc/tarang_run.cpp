@@ -10,8 +10,8 @@
 // out=DIR (any position after nu): DIR/out/glob.d every step, DIR/out/spectrum.d and flux.d at the end,
 // in the reference's formats (Output_global, Output_shell_spectrum, Output_flux).
 // force=PROC,INNER,OUTER,P3,P4: force_field_proc 1 (energy/helicity supply) or 2 (constant energy/helicity)
-// with its force_double_para, recomputed at every stage; with mhd=, proc 1 and P5, P6 the magnetic field's
-// energy_supply_W and epsh_by_k_epse_W (0 when left out).
+// with its force_double_para, recomputed at every stage; with mhd=, P5 and P6 are the magnetic field's
+// parameters (energy_supply_W, epsh_by_k_epse_W or energy_level_W, h_by_k_E_W; 0 when left out).
 // mhd=ETA: incompressible MHD (the IMHD program, src/IMHD/IMHD_main.cc) with magnetic diffusivity ETA; U
 // from the IC choice, B random-phase (seed 2); Get_dt(W), Time_advance(W) and the W outputs; "glob t Eu Eb"
 // lines, the spectrum of U and the kinetic flux.
@@ -107,14 +107,10 @@ int main(int argc, char** argv)
         if (fproc && eta < 0) U.Set_force(fproc, fpar[0], fpar[1], fpar[2], fpar[3]);
         if (!outdir.empty()) U.Open_output_files(outdir);
         if (eta >= 0) {
-            if (fproc && fproc != 1) {
-                fprintf(stderr, "tarang_run: MHD runs take force_field_proc 1 only\n");
-                return 2;
-            }
             tarang::IncVF W(ctx, g, eta, N / 3 + 2, 1);
             void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
             tarang::check(th_ic_random(W.plan(), B, 2, 0.5, 4.0), "th_ic_random");
-            if (fproc) U.Set_force(W, fpar[0], fpar[1], fpar[2], fpar[3], fpar[4], fpar[5]);
+            if (fproc) U.Set_force(W, fproc, fpar[0], fpar[1], fpar[2], fpar[3], fpar[4], fpar[5]);
             for (int it = 0; it < steps; it++) {
                 U.Tdt = U.Get_dt(W);
                 U.Tnow += U.Tdt;

@@ -899,19 +899,8 @@ extern "C" int th_step_plain_forced_mhd(th_plan* p, int scheme, void* const U[3]
     if (last_stage)
         for (int c = 0; c < 6; c++)
             if (!last_stage[c]) { set_error("th_step_plain_forced_mhd: null last_stage field"); return E_ARG; }
-    if (force && force->proc != 1) {
-        set_error("th_step_plain_forced_mhd: force_field_proc %d is not supported for MHD (1 only)", force->proc);
-        return E_UNSUPPORTED;
-    }
     ForceBand fu, fw;
-    TRY(force_setup(p->g, force, fu));
-    if (!isfinite(force->p5) || !isfinite(force->p6)) {
-        set_error("th_step_plain_forced_mhd: energy_supply_W and epsh_by_k_epse_W must be finite");
-        return E_ARG;
-    }
-    fw = fu;                                       // same band and nf; W's own supply and helicity ratio
-    fw.per_mode = force->p5 / (double)fu.nf;
-    fw.p4 = force->p6;
+    TRY(force_setup_mhd(p->g, force, fu, fw));
     return DISPATCH(p, do_step_plain_forced_mhd, p, scheme, U, B, dt, ph, &fu, &fw, last_stage);
 }
 

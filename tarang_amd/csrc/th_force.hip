@@ -120,6 +120,28 @@ int force_setup(const Geo& g, const th_force* f, ForceBand& b)
     b.nx = b.fullx ? g.N1 : 2 * kxm + 1;
     b.ny = b.fully ? g.N2 : 2 * kym + 1;
     b.nz = (kzm < g.H - 1 ? kzm : g.H - 1) + 1;
+    b.emin = MYEPS;
+    return OK;
+}
+
+int force_setup_mhd(const Geo& g, const th_force* f, ForceBand& bu, ForceBand& bw)
+{
+    TRY(force_setup(g, f, bu));
+    if (!isfinite(f->p5) || !isfinite(f->p6)) {
+        set_error("force: energy_supply_W / energy_level_W and the W helicity ratio must be finite");
+        return E_ARG;
+    }
+    if (f->proc == 2 && !(fabs(f->p6) <= 1.0)) {
+        set_error("force: h_by_k_E_W = %g outside [-1, 1] (the square roots of the constant-helicity force are "
+                  "not real)", f->p6);
+        return E_ARG;
+    }
+    bw = bu;                                  // same band and nf; W's own level and helicity ratio
+    bw.per_mode = f->p5 / (double)bu.nf;
+    bw.p4 = f->p6;
+    // compute_force_const_ek_hk.cc:376-402: W's modes are forced when W.CV_Modal_energy > 0, U's (and both
+    // in the supply scheme, compute_force_ek_hk_supply.cc:377-408) when it exceeds MYEPS
+    bw.emin = f->proc == 2 ? 0.0 : MYEPS;
     return OK;
 }
 
@@ -152,6 +174,39 @@ template <typename R> static int compute_force(th_plan* p, void* const V[3], voi
     TRY(force_write<R>(b, v, o, p->g.ncplx(), p->ctx->stream));
     if (nf) *nf = b.nf;
     return OK;
+}
+
+// Compute_force(IncVF& W): both forces, each zero when its level per mode is not above MYEPS
+template <typename R> static int compute_force_mhd(th_plan* p, void* const U[3], void* const B[3], void* FU[3],
+                                                  void* FW[3], const th_force* f, long* nf)
+{
+    ForceBand bu, bw;
+    TRY(force_setup_mhd(p->g, f, bu, bw));
+    const ForceBand* b[2] = {&bu, &bw};
+    void* const* src[2] = {U, B};
+    void** dst[2] = {FU, FW};
+    for (int part = 0; part < 2; part++) {
+        const cplx<R>* v[3] = {(const cplx<R>*)src[part][0], (const cplx<R>*)src[part][1], (const cplx<R>*)src[part][2]};
+        cplx<R>* o[3] = {(cplx<R>*)dst[part][0], (cplx<R>*)dst[part][1], (cplx<R>*)dst[part][2]};
+        if (b[part]->per_mode > MYEPS) {
+            TRY(force_write<R>(*b[part], v, o, p->g.ncplx(), p->ctx->stream));
+        } else {
+            for (int c = 0; c < 3; c++)
+                TH_CHECK_HIP(hipMemsetAsync(o[c], 0, p->g.ncplx() * sizeof(cplx<R>), p->ctx->stream));
+        }
+    }
+    if (nf) *nf = bu.nf;
+    return OK;
+}
+
+extern "C" int th_compute_force_mhd(th_plan* p, void* const U[3], void* const B[3], void* FU[3], void* FW[3],
+                                    const th_force* f, long* nf)
+{
+    if (!p || !U || !B || !FU || !FW) { set_error("th_compute_force_mhd: null"); return E_ARG; }
+    for (int c = 0; c < 3; c++)
+        if (!U[c] || !B[c] || !FU[c] || !FW[c]) { set_error("th_compute_force_mhd: null"); return E_ARG; }
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    return p->prec ? compute_force_mhd<float>(p, U, B, FU, FW, f, nf) : compute_force_mhd<double>(p, U, B, FU, FW, f, nf);
 }
 
 extern "C" int th_compute_force(th_plan* p, void* const V[3], void* F[3], const th_force* f, long* nf)

@@ -36,7 +36,7 @@ __device__ __forceinline__ void mode_force(const ForceBand& b, const cplx<R> v[3
         E += vr[c] * vr[c] + vi[c] * vi[c];
     }
     E = E / 2;                                               // CV_Modal_energy Cvf.cc:595-600
-    if (!(E > MYEPS)) return;
+    if (!(E > b.emin)) return;
     const double hel = kk[0] * (vr[1] * vi[2] - vr[2] * vi[1]) + kk[1] * (vr[2] * vi[0] - vr[0] * vi[2]) +
                        kk[2] * (vr[0] * vi[1] - vr[1] * vi[0]);       // Get_Modal_helicity_FOUR
     const double sk = hel / (km * E);

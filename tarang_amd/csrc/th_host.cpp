@@ -714,10 +714,9 @@ IncFluid::Force_field::Force_field(IncFluid& u)
     check(th_compute_force(u.plan_, src, F, &u.force_, nullptr), "Compute_force");
 }
 
-// Compute_force_const_energy_helicity_supply(IncVF& W) compute_force_ek_hk_supply.cc:294-420 on the fields of
-// the last RK stage (the stage loop recomputes both forces, so that is what Force1..3 and W.Force1..3 hold
-// when the outputs are written); a part whose supply per mode is below MYEPS is never computed and stays 0,
-// as before the first step.
+// Compute_force(IncVF& W) compute_force_main.cc:88-105 on the fields of the last RK stage (the stage loop
+// recomputes both forces, so that is what Force1..3 and W.Force1..3 hold when the outputs are written); a
+// part whose level per mode is not above MYEPS is never computed and stays 0, as before the first step.
 void IncFluid::mhd_forces(DField fu[3], DField fw[3])
 {
     for (int c = 0; c < 3; c++) {
@@ -725,24 +724,18 @@ void IncFluid::mhd_forces(DField fu[3], DField fw[3])
         fw[c] = DField(elems_, esize_);
     }
     if (!mhd_stepped_) return;
-    const double supply[2] = {mhd_force_.p3, mhd_force_.p5}, ratio[2] = {mhd_force_.p4, mhd_force_.p6};
-    for (int part = 0; part < 2; part++) {
-        DField* out = part ? fw : fu;
-        th_force f{1, mhd_force_.inner_radius, mhd_force_.outer_radius, supply[part], ratio[part], 0, 0};
-        void* const src[3] = {mlast_[3 * part].data(), mlast_[3 * part + 1].data(), mlast_[3 * part + 2].data()};
-        void* F[3] = {out[0].data(), out[1].data(), out[2].data()};
-        long nf = 0;
-        check(th_compute_force(plan_, src, F, &f, &nf), "Compute_force(W)");
-        check(th_sync(plan_), "th_sync");
-        if (!(supply[part] / (double)nf > 1e-15))
-            for (int c = 0; c < 3; c++) out[c] = DField(elems_, esize_);
-    }
+    void* const U[3] = {mlast_[0].data(), mlast_[1].data(), mlast_[2].data()};
+    void* const B[3] = {mlast_[3].data(), mlast_[4].data(), mlast_[5].data()};
+    void* FU[3] = {fu[0].data(), fu[1].data(), fu[2].data()};
+    void* FW[3] = {fw[0].data(), fw[1].data(), fw[2].data()};
+    check(th_compute_force_mhd(plan_, U, B, FU, FW, &mhd_force_, nullptr), "Compute_force(W)");
+    check(th_sync(plan_), "th_sync");
 }
 
-void IncFluid::Set_force(IncVF& W, double inner_radius, double outer_radius, double p3, double p4, double p5,
-                         double p6)
+void IncFluid::Set_force(IncVF& W, int proc, double inner_radius, double outer_radius, double p3, double p4,
+                         double p5, double p6)
 {
-    th_force f{1, inner_radius, outer_radius, p3, p4, p5, p6};
+    th_force f{proc, inner_radius, outer_radius, p3, p4, p5, p6};
     // validate on a zero-length step (the entry point's argument checks) so a bad set fails here
     th_phys ph{dissipation_coefficient, W.dissipation_coefficient, 0, 0, 0};
     DField U1(elems_, esize_), U2(elems_, esize_), U3(elems_, esize_), B1(elems_, esize_), B2(elems_, esize_),

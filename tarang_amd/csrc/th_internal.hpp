@@ -136,8 +136,12 @@ struct ForceBand {
     int fullx, fully;         // the box spans every x (y) index
     int kxm, kym;             // box origin: kx = tx - kxm, ky = ty - kym
     int nx, ny, nz;           // box extents
+    double emin;              // a mode is forced when its modal energy exceeds this (MYEPS; 0 for W, proc 2)
 };
 int force_setup(const Geo& g, const th_force* f, ForceBand& b);
+// the bands of U (p3, p4) and W (p5, p6) of Compute_force(IncVF& W) (compute_force_ek_hk_supply.cc:294-420,
+// compute_force_const_ek_hk.cc:296-437)
+int force_setup_mhd(const Geo& g, const th_force* f, ForceBand& bu, ForceBand& bw);
 // F = the force of V (zero outside the band)
 template <typename R> int force_write(const ForceBand& b, const cplx<R>* const V[3], cplx<R>* const F[3], size_t n,
                                       hipStream_t st);

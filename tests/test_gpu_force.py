@@ -251,16 +251,60 @@ def test_forced_mhd_steps_match_oracle(ctx, p3, p4, p5, p6, prec, tol):
         assert rel(H(gB[i]), B[i]) < tol
 
 
+@pytest.mark.parametrize("proc,p6", [(1, -0.2), (2, -0.5)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-5)])
+def test_compute_force_mhd_matches_oracle(ctx, proc, p6, prec, tol):
+    """th_compute_force_mhd vs the oracle's Compute_force(IncVF& W), with a band mode of energy 1e-17 in both
+    fields (forced in B under proc 2 only)."""
+    from tarang_amd import Plan
+    log(f"compute_force_mhd proc={proc} {prec}")
+    N = (16, 16, 16)
+    o = Oracle(N)
+    U, B = o.ic_random(3, E0=0.5, kp=3.0), o.ic_random(4, E0=0.3, kp=3.0)
+    a = np.sqrt(2e-17)
+    for V in (U, B):
+        V[0][0, 3, 0] = V[0][0, -3, 0] = a
+        V[1][0, 3, 0] = V[1][0, -3, 0] = 0.0
+        V[2][0, 3, 0] = V[2][0, -3, 0] = 0.0
+    p = Plan(ctx, N, precision=prec)
+    dU, dB = [p.to_dev(x) for x in U], [p.to_dev(x) for x in B]
+    if prec == "fp32":
+        U, B = [H(d) for d in dU], [H(d) for d in dB]
+    RU, RW = o.force_mhd(U, B, proc, 1.5, 3.5, 0.5, 0.3, 0.4, p6)
+    FU, FW, nf = p.compute_force_mhd(dU, dB, proc, 1.5, 3.5, 0.5, 0.3, 0.4, p6)
+    assert nf == o.force_supply_modes(1.5, 3.5)
+    m = max(np.abs(r).max() for r in RU + RW)
+    for c in range(3):
+        assert np.abs(H(FU[c]) - RU[c]).max() <= tol * m
+        assert np.abs(H(FW[c]) - RW[c]).max() <= tol * m
+    if proc == 2 and prec == "fp64":
+        assert np.abs(RW[0][0, 3, 0]) > 0 and RU[0][0, 3, 0] == 0
+
+
+@pytest.mark.parametrize("p6", [-0.5, 0.0])
+def test_forced_mhd_constant_level_steps_match_oracle(ctx, p6):
+    from tarang_amd import Plan
+    log(f"forced mhd proc 2 p6={p6}")
+    N = (16, 16, 16)
+    o = Oracle(N)
+    U, B = o.ic_random(3, E0=0.5, kp=3.0), o.ic_random(4, E0=0.3, kp=3.0)
+    p = Plan(ctx, N)
+    gU, gB = [p.to_dev(a) for a in U], [p.to_dev(a) for a in B]
+    o.rk4_mhd_forced(U, B, 0.01, 0.02, 1e-3, 1.5, 3.5, 0.6, 0.2, 0.4, p6, steps=3, proc=2)
+    p.step_plain_forced_mhd(gU, gB, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.6, 0.2, 0.4, p6, steps=3, proc=2)
+    for i in range(3):
+        assert rel(H(gU[i]), U[i]) < 1e-12
+        assert rel(H(gB[i]), B[i]) < 1e-12
+
+
 def test_forced_mhd_argument_errors(ctx):
     from tarang_amd import Plan
     log("forced mhd errors")
     p = Plan(ctx, (16, 16, 16))
     U, B = p.empty(3), p.empty(3)
-    with pytest.raises(RuntimeError, match="not supported for MHD"):
-        from tarang_amd import abi
-        fc = abi.Force(2, 1.5, 3.5, 0.5, 0.2, 0.3, 0.1)
-        ph = abi.Phys(0.01, 0.02, 0.0, 0.0, 0.0)
-        abi._check(abi.lib().th_step_plain_forced_mhd(p.h, 2, abi._pp(U), abi._pp(B), 1e-3, abi.C.byref(ph),
-                                                      abi.C.byref(fc), None), "th_step_plain_forced_mhd")
+    with pytest.raises(RuntimeError, match="not supported"):
+        p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.2, 0.3, 0.1, proc=3)
     with pytest.raises(RuntimeError, match="must be finite"):
         p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.0, float("nan"), 0.0)
+    with pytest.raises(RuntimeError, match="h_by_k_E_W"):
+        p.compute_force_mhd(U, B, 2, 1.5, 3.5, 0.5, 0.0, 0.3, 1.5)

@@ -180,8 +180,9 @@ def test_driver_forced_output_columns(tmp_path, path):
     assert np.abs(fk_new - fk).max() > 1e-6 * np.abs(fk).max()
 
 
-@pytest.mark.parametrize("fp", [(1.5, 3.5, 0.5, 0.3, 0.4, -0.2), (1.5, 3.5, 0.5, 0.3, 0.0, 0.0)])
-def test_driver_forced_mhd_matches_oracle(tmp_path, fp):
+@pytest.mark.parametrize("proc,fp", [(1, (1.5, 3.5, 0.5, 0.3, 0.4, -0.2)), (1, (1.5, 3.5, 0.5, 0.3, 0.0, 0.0)),
+                                     (2, (1.5, 3.5, 0.6, 0.2, 0.4, -0.5))])
+def test_driver_forced_mhd_matches_oracle(tmp_path, proc, fp):
     """mhd=ETA with force=1,INNER,OUTER,P3,P4,P5,P6: Set_force(W, ...) then Time_advance(W) steps U and B with
     both supplies recomputed at every stage, the energies equal to the oracle's forced MHD steps.  The force
     spectra of spectrum.d (force_spectrum.cc:48-53 for U and W) and the force feeds of flux.d
@@ -190,19 +191,18 @@ def test_driver_forced_mhd_matches_oracle(tmp_path, fp):
     N, steps, dt, nu, eta = 32, 3, 1e-3, 0.01, 0.02
     (tmp_path / "out").mkdir()
     spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"mhd={eta}", f"out={tmp_path}",
-                                  "force=1," + ",".join(str(v) for v in fp))
+                                  f"force={proc}," + ",".join(str(v) for v in fp))
     o = Oracle((N, N, N))
     U, B = o.ic_random(1), o.ic_random(2)
     last = o.zeros(6)
-    o.rk4_mhd_forced(U, B, nu, eta, dt, *fp, steps=steps, last_stage=last)
+    o.rk4_mhd_forced(U, B, nu, eta, dt, *fp, steps=steps, last_stage=last, proc=proc)
     Eu, Eb = o.total_energy(U), o.total_energy(B)
     assert abs(glob[-1][1] - Eu) <= 1e-12 * Eu and abs(glob[-1][2] - Eb) <= 1e-12 * Eb
     U0, B0 = o.ic_random(1), o.ic_random(2)
     o.rk4_mhd(U0, B0, nu, eta, dt, steps=steps)
     assert abs(glob[-1][1] - o.total_energy(U0)) > 1e-6       # the force acted
     inner, outer, p3, p4, p5, p6 = fp
-    FU = o.force(last[:3], 1, inner, outer, p3, p4)
-    FW = o.force(last[3:], 1, inner, outer, p5, p6) if p5 > 0 else o.zeros(3)
+    FU, FW = o.force_mhd(last[:3], last[3:], proc, inner, outer, p3, p4, p5, p6)
     fku = 2 * sum(o.shell_spectrum(U[c], FU[c])[0] for c in range(3))
     fkw = 2 * sum(o.shell_spectrum(B[c], FW[c])[0] for c in range(3))
     rows = [r for r in (tmp_path / "out" / "spectrum.d").read_text().split("\n")[1:] if r.strip()]

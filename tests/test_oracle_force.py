@@ -247,3 +247,38 @@ def test_forced_mhd_step_with_zero_supply_is_the_free_step():
     for c in range(3):
         assert np.array_equal(Uf[c], Uu[c])
         assert np.array_equal(Bf[c], Bu[c])
+
+
+@pytest.mark.parametrize("proc,p4,p6", [(1, 0.3, -0.2), (2, 0.3, -0.5)])
+def test_mhd_force_is_the_field_force_of_each_part(proc, p4, p6):
+    """Compute_force(IncVF& W) (compute_force_ek_hk_supply.cc:294-420, compute_force_const_ek_hk.cc:296-437):
+    on fields without sub-MYEPS modes FU is the pinned force of U with (p3, p4) and FW that of B with (p5, p6);
+    a zero level gives a zero part."""
+    o = Oracle((16, 16, 16))
+    U, B = o.ic_random(3, E0=0.5, kp=3.0), o.ic_random(4, E0=0.3, kp=3.0)
+    FU, FW = o.force_mhd(U, B, proc, 1.5, 3.5, 0.5, p4, 0.4, p6)
+    RU, RW = o.force(U, proc, 1.5, 3.5, 0.5, p4), o.force(B, proc, 1.5, 3.5, 0.4, p6)
+    for c in range(3):
+        assert np.array_equal(FU[c], RU[c]) and np.array_equal(FW[c], RW[c])
+    assert max(np.abs(f).max() for f in FU) > 0 and max(np.abs(f).max() for f in FW) > 0
+    FU, FW = o.force_mhd(U, B, proc, 1.5, 3.5, 0.0, p4, 0.4, p6)
+    assert max(np.abs(f).max() for f in FU) == 0 and max(np.abs(f).max() for f in FW) > 0
+
+
+def test_constant_level_forces_weak_magnetic_modes():
+    """proc 2 forces a mode of B whose modal energy is above 0 (compute_force_const_ek_hk.cc:401), a mode of U
+    only above MYEPS (:377): a band mode with energy 1e-17 in both fields is set to |FW|^2/2 = energy_level_W
+    / nf (the constant-level closed form, no helicity) and FU stays 0 there; under proc 1 neither is forced."""
+    o = Oracle((16, 16, 16))
+    U, B = o.zeros(3), o.zeros(3)
+    a = math.sqrt(2e-17)                  # |V|^2 / 2 = 1e-17, K = (3, 0, 0), V along y (solenoidal, real)
+    for V in (U, B):
+        V[1][3, 0, 0] = a
+        V[1][-3, 0, 0] = a
+    nf = o.force_supply_modes(2.5, 3.5)
+    FU, FW = o.force_mhd(U, B, 2, 2.5, 3.5, 0.5, 0.0, 0.4, 0.0)
+    assert max(np.abs(f).max() for f in FU) == 0
+    e = 0.5 * sum(abs(FW[c][3, 0, 0]) ** 2 for c in range(3))
+    assert abs(e - 0.4 / nf) < 1e-13 * (0.4 / nf)
+    FU, FW = o.force_mhd(U, B, 1, 2.5, 3.5, 0.5, 0.0, 0.4, 0.0)
+    assert max(np.abs(f).max() for f in FU) == 0 and max(np.abs(f).max() for f in FW) == 0
