@@ -55,7 +55,11 @@ typedef struct {
  * proc 1 == Compute_force_const_energy_helicity_supply compute_force_ek_hk_supply.cc:51-157
  *           (p3 = energy_supply, p4 = epsh_by_k_epse),
  * proc 2 == Compute_force_const_energy_helicity compute_force_const_ek_hk.cc:51-140
- *           (p3 = energy_level, p4 = h_by_k_E). */
+ *           (p3 = energy_level, p4 = h_by_k_E),
+ * proc 3 == Compute_force_Taylor_Green compute_force_TG.cc:50-104: inner_radius = k0 (truncated to int, as
+ *           force_double_para(1)), outer_radius = force_amp, p3 = forceW_amp (MHD); the fixed field
+ *           amp (sin k0x cos k0y cos k0z, -cos k0x sin k0y cos k0z, 0) of Setup_Taylor_Green_force_field
+ *           force_model.cc:57-88, 1 <= k0 < N/2 inside the dealiased modes; runs on the plain steppers. */
 typedef struct th_force {
     int proc;                       /* force_field_proc: 1 or 2 */
     double inner_radius, outer_radius;

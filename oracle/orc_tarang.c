@@ -862,9 +862,24 @@ int orc_force_supply_modes(const orc_grid *g, double inner, double outer)
 static int force_ek_hk_e(const orc_grid *g, int proc, double *const V[3], double *F[3], double inner,
                          double outer, double p3, double p4, double emin);
 
+/* Setup_Taylor_Green_force_field force_model.cc:57-88 (FOUR): the four modes (+-k0, +-k0, k0) of
+ * amp (sin k0x cos k0y cos k0z, -cos k0x sin k0y cos k0z, 0), written in the reference's order. */
+static void force_tg(const orc_grid *g, double *F[3], int k0, double amp)
+{
+    for (int c = 0; c < 3; c++) memset(F[c], 0, 2 * NCPLX(g) * sizeof(double));
+    int lxp = k0, lxm = g->N1 - k0, lyp = k0, lym = g->N2 - k0;
+    double a = amp / 8;
+    IM(F[0], IDX(g, lxp, lyp, k0)) = -a; IM(F[0], IDX(g, lxp, lym, k0)) = -a;
+    IM(F[1], IDX(g, lxp, lyp, k0)) = a;  IM(F[1], IDX(g, lxp, lym, k0)) = -a;
+    IM(F[0], IDX(g, lxm, lyp, k0)) = a;  IM(F[0], IDX(g, lxm, lym, k0)) = a;
+    IM(F[1], IDX(g, lxm, lyp, k0)) = a;  IM(F[1], IDX(g, lxm, lym, k0)) = -a;
+}
+
+/* proc 3: inner = k0 (truncated like force_double_para(1)), outer = force_amp; returns 0 (no mode count) */
 int orc_force_ek_hk(const orc_grid *g, int proc, double *const V[3], double *F[3], double inner, double outer,
                     double p3, double p4)
 {
+    if (proc == 3) { force_tg(g, F, (int)inner, outer); return 0; }
     return force_ek_hk_e(g, proc, V, F, inner, outer, p3, p4, MYEPS);
 }
 
@@ -874,6 +889,11 @@ int orc_force_ek_hk(const orc_grid *g, int proc, double *const V[3], double *F[3
 int orc_force_mhd(const orc_grid *g, int proc, double *const U[3], double *const B[3], double *FU[3], double *FW[3],
                   double inner, double outer, double p3, double p4, double p5, double p6)
 {
+    if (proc == 3) {                    /* compute_force_TG.cc:90-104: force_amp for U, forceW_amp = p3 for W */
+        force_tg(g, FU, (int)inner, outer);
+        force_tg(g, FW, (int)inner, p3);
+        return 0;
+    }
     int nf = orc_force_supply_modes(g, inner, outer);
     for (int c = 0; c < 3; c++) {
         memset(FU[c], 0, 2 * NCPLX(g) * sizeof(double));

@@ -238,7 +238,7 @@ class Oracle:
         p4 = epsh_by_k_epse) or 2 (constant energy/helicity: p3 = energy_level, p4 = h_by_k_E)."""
         F = self.zeros(3)
         nf = self.L.orc_force_ek_hk(self.gp, proc, _pp(V), _pp(F), inner, outer, p3, p4)
-        if nf <= 0:
+        if nf <= 0 and proc != 3:             # proc 3 (Taylor-Green: inner = k0, outer = amp) counts no modes
             raise ValueError("no mode in the forcing band")
         return F
 

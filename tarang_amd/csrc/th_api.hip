@@ -628,7 +628,8 @@ template <typename R> struct Ops {
             // variables: the forces are taken from U and B first and subtracted afterwards (Add_force(W))
             cplx<R>* FU[3] = {S(SC_FU0), S(SC_FU0 + 1), S(SC_FU0 + 2)};
             cplx<R>* FW[3] = {S(SC_FW0), S(SC_FW0 + 1), S(SC_FW0 + 2)};
-            const bool fu = force && force->per_mode > 1e-15, fw = force_w && force_w->per_mode > 1e-15;
+            const bool fu = force && (force->proc == 3 || force->per_mode > 1e-15),
+                       fw = force_w && (force_w->proc == 3 || force_w->per_mode > 1e-15);
             if ((fu || fw) && (!FU[0] || !FU[1] || !FU[2] || !FW[0] || !FW[1] || !FW[2])) return E_NOMEM;
             if (fu) TRY(force_write<R>(*force, F, FU, n, st));
             if (fw) TRY(force_write<R>(*force_w, F + 3, FW, n, st));

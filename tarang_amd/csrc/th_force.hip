@@ -1,6 +1,8 @@
 // th_force.hip -- energy / helicity forcing of the velocity on a FOUR box (force_field_proc 1 and 2):
 //   proc 1: IncFluid::Compute_force_const_energy_helicity_supply compute_force_ek_hk_supply.cc:51-157
 //   proc 2: IncFluid::Compute_force_const_energy_helicity compute_force_const_ek_hk.cc:51-140
+//   proc 3: IncFluid::Compute_force_Taylor_Green compute_force_TG.cc:50-104 (Setup_Taylor_Green_force_field
+//           force_model.cc:57-88), a fixed four-mode field
 //   with IncVF::Force_alpha_beta force_alpha_beta.cc:60-89, Get_local_number_modes_in_shell_FOUR
 //   four_basic.cc:48-90, Is_alias_array universal_misc.cc:169-181 + Dealias_force dealias.cc:78-84 and
 //   Satisfy_reality_condition_force_field reality_cond.cc:90-95.
@@ -70,12 +72,41 @@ namespace th {
 int force_setup(const Geo& g, const th_force* f, ForceBand& b)
 {
     if (!f) { set_error("force: null parameters"); return E_ARG; }
-    if (f->proc != 1 && f->proc != 2) {
+    if (f->proc < 1 || f->proc > 3) {
         set_error("force: force_field_proc %d is not supported (1: energy/helicity supply, 2: constant energy/"
-                  "helicity)", f->proc);
+                  "helicity, 3: Taylor-Green)", f->proc);
         return E_UNSUPPORTED;
     }
     if (g.basis != TH_FOUR) { set_error("force: only the FOUR basis is supported"); return E_UNSUPPORTED; }
+    if (f->proc == 3) {
+        // Compute_force_Taylor_Green compute_force_TG.cc:50-63: k0 = (int) force_double_para(1) (inner_radius),
+        // force_amp = force_double_para(2) (outer_radius); set once, the same at every stage
+        if (!isfinite(f->inner_radius) || !isfinite(f->outer_radius)) {
+            set_error("force: Taylor-Green k0 and amplitude must be finite");
+            return E_ARG;
+        }
+        const int k0 = (int)f->inner_radius;
+        if (k0 < 1 || 2 * k0 >= g.N1 || 2 * k0 >= g.N2 || 2 * k0 >= g.N3 ||
+            (g.dealias && (!kept_mode(0, 1, g.N1, g.N2, g.N3, k0, k0, k0) ||
+                           !kept_mode(0, 1, g.N1, g.N2, g.N3, g.N1 - k0, g.N2 - k0, k0)))) {
+            set_error("force: Taylor-Green k0 = %d must satisfy 1 <= k0 < N/2 on every axis and lie in the "
+                      "dealiased modes", k0);
+            return E_ARG;
+        }
+        b = ForceBand{};
+        b.N1 = g.N1; b.N2 = g.N2; b.N3 = g.N3; b.H = g.H;
+        b.f1 = g.kf[0]; b.f2 = g.kf[1]; b.f3 = g.kf[2];
+        b.proc = 3;
+        b.k0 = k0;
+        b.per_mode = f->outer_radius;
+        b.nf = 0;
+        b.dealias_force = 0;
+        b.emin = MYEPS;
+        b.kxm = k0; b.kym = k0;
+        b.fullx = b.fully = 0;
+        b.nx = 2 * k0 + 1; b.ny = 2 * k0 + 1; b.nz = k0 + 1;
+        return OK;
+    }
     if (!(f->outer_radius > f->inner_radius) || !(f->inner_radius >= 0) || !isfinite(f->outer_radius) ||
         !isfinite(f->p3) || !isfinite(f->p4)) {
         set_error("force: need 0 <= inner_radius < outer_radius and finite parameters");
@@ -137,6 +168,11 @@ int force_setup_mhd(const Geo& g, const th_force* f, ForceBand& bu, ForceBand& b
         return E_ARG;
     }
     bw = bu;                                  // same band and nf; W's own level and helicity ratio
+    if (f->proc == 3) {                       // compute_force_TG.cc:90-104: forceW_amp = force_double_para(3)
+        if (!isfinite(f->p3)) { set_error("force: Taylor-Green forceW_amp must be finite"); return E_ARG; }
+        bw.per_mode = f->p3;
+        return OK;
+    }
     bw.per_mode = f->p5 / (double)bu.nf;
     bw.p4 = f->p6;
     // compute_force_const_ek_hk.cc:376-402: W's modes are forced when W.CV_Modal_energy > 0, U's (and both
@@ -188,7 +224,7 @@ template <typename R> static int compute_force_mhd(th_plan* p, void* const U[3],
     for (int part = 0; part < 2; part++) {
         const cplx<R>* v[3] = {(const cplx<R>*)src[part][0], (const cplx<R>*)src[part][1], (const cplx<R>*)src[part][2]};
         cplx<R>* o[3] = {(cplx<R>*)dst[part][0], (cplx<R>*)dst[part][1], (cplx<R>*)dst[part][2]};
-        if (b[part]->per_mode > MYEPS) {
+        if (b[part]->proc == 3 || b[part]->per_mode > MYEPS) {
             TRY(force_write<R>(*b[part], v, o, p->g.ncplx(), p->ctx->stream));
         } else {
             for (int c = 0; c < 3; c++)

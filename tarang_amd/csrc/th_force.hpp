@@ -24,6 +24,17 @@ __device__ __forceinline__ void mode_force(const ForceBand& b, const cplx<R> v[3
                                            double fr[3], double fi[3])
 {
     fr[0] = fr[1] = fr[2] = fi[0] = fi[1] = fi[2] = 0.0;
+    if (b.proc == 3) {
+        // Setup_Taylor_Green_force_field force_model.cc:57-88 (FOUR): F1 = -i s_x amp/8, F2 = i s_y amp/8 on
+        // (s_x k0, s_y k0, k0), the field amp (sin k0x cos k0y cos k0z, -cos k0x sin k0y cos k0z, 0)
+        const int kx = kx_four(lx, b.N1), ky = ky_of(ly, b.N2);
+        if (lz == b.k0 && (kx == b.k0 || kx == -b.k0) && (ky == b.k0 || ky == -b.k0)) {
+            const double a = b.per_mode / 8;
+            fi[0] = kx > 0 ? -a : a;
+            fi[1] = ky > 0 ? a : -a;
+        }
+        return;
+    }
     const double kk[3] = {kx_four(lx, b.N1) * b.f1, ky_of(ly, b.N2) * b.f2, lz * b.f3};
     const double km = kmag3(kk[0], kk[1], kk[2]);
     if (!(km > b.inner && km <= b.outer)) return;

@@ -493,6 +493,10 @@ extern "C" int th_state_set_force(th_state* s, const th_force* f)
             set_error("th_state_set_force: forcing runs on single-box NS states");
             return E_UNSUPPORTED;
         }
+        if (f->proc == 3) {
+            set_error("th_state_set_force: the Taylor-Green force runs on the plain stepper");
+            return E_UNSUPPORTED;
+        }
         ForceBand fb;
         TRY(force_setup(s->plan->g, f, fb));
         // every mode that can carry a force must be stored by the 2/3-dealiased state

@@ -144,7 +144,7 @@ def test_force_argument_errors(ctx):
     p = Plan(ctx, (16, 16, 16))
     dV = p.empty(3)
     with pytest.raises(RuntimeError, match="not supported"):
-        p.compute_force(dV, 3, 1.5, 3.5, 1.0)
+        p.compute_force(dV, 4, 1.5, 3.5, 1.0)
     with pytest.raises(RuntimeError, match="no wavevector"):
         p.compute_force(dV, 1, 0.1, 0.5, 1.0)
     with pytest.raises(RuntimeError, match="h_by_k_E"):
@@ -303,8 +303,50 @@ def test_forced_mhd_argument_errors(ctx):
     p = Plan(ctx, (16, 16, 16))
     U, B = p.empty(3), p.empty(3)
     with pytest.raises(RuntimeError, match="not supported"):
-        p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.2, 0.3, 0.1, proc=3)
+        p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.2, 0.3, 0.1, proc=4)
     with pytest.raises(RuntimeError, match="must be finite"):
         p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.0, float("nan"), 0.0)
     with pytest.raises(RuntimeError, match="h_by_k_E_W"):
         p.compute_force_mhd(U, B, 2, 1.5, 3.5, 0.5, 0.0, 0.3, 1.5)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-15), ("fp32", 1e-7)])
+def test_taylor_green_force_and_steps_match_oracle(ctx, prec, tol):
+    """force_field_proc 3 (compute_force_TG.cc:50-104): the fixed four-mode field, the forced EULER / RK2 / RK4
+    NS steps and the MHD step (forceW_amp in p3) against the oracle; the fused state leaves it to the plain
+    stepper; k0 outside 1 <= k0 < N/2 or the dealiased modes is refused."""
+    from tarang_amd import Plan, State
+    log(f"taylor-green force {prec}")
+    N = (16, 16, 16)
+    o = Oracle(N)
+    p = Plan(ctx, N, precision=prec)
+    V = o.ic_random(5, E0=0.5, kp=3.0)
+    dV = [p.to_dev(a) for a in V]
+    if prec == "fp32":
+        V = [H(d) for d in dV]
+    for k0, amp in [(1, 0.8), (2, -0.5), (4, 0.3)]:
+        ref = o.force(V, 3, float(k0), amp, 0.0)
+        F, nf = p.compute_force(dV, 3, float(k0), amp, 0.0)
+        assert nf == 0
+        for c in range(3):
+            assert np.abs(H(F[c]) - ref[c]).max() <= tol * abs(amp)
+    if prec == "fp64":
+        for scheme in (0, 1, 2):
+            W = [v.copy() for v in V]
+            gW = [p.to_dev(a) for a in W]
+            o.step_ns_forced(W, 0.01, 1e-3, 3, 1.0, 0.8, 0.0, 0.0, scheme=scheme, steps=3)
+            p.step_plain_forced(gW, 1e-3, 0.01, 3, 1.0, 0.8, 0.0, 0.0, scheme=scheme, steps=3)
+            for i in range(3):
+                assert rel(H(gW[i]), W[i]) < 1e-12
+        U, B = o.ic_random(3, E0=0.5, kp=3.0), o.ic_random(4, E0=0.3, kp=3.0)
+        gU, gB = [p.to_dev(a) for a in U], [p.to_dev(a) for a in B]
+        o.rk4_mhd_forced(U, B, 0.01, 0.02, 1e-3, 1.0, 0.8, -0.4, 0.0, 0.0, 0.0, steps=3, proc=3)
+        p.step_plain_forced_mhd(gU, gB, 1e-3, 0.01, 0.02, 1.0, 0.8, -0.4, 0.0, 0.0, 0.0, steps=3, proc=3)
+        for i in range(3):
+            assert rel(H(gU[i]), U[i]) < 1e-12
+            assert rel(H(gB[i]), B[i]) < 1e-12
+        with pytest.raises(RuntimeError, match="plain stepper"):
+            State(p).set_force(3, 1.0, 0.8, 0.0, 0.0)
+        for k0 in (0.0, 5.0, 6.0, 8.0):
+            with pytest.raises(RuntimeError, match="Taylor-Green k0"):
+                p.compute_force(dV, 3, k0, 0.8, 0.0)

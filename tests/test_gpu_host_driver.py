@@ -181,7 +181,7 @@ def test_driver_forced_output_columns(tmp_path, path):
 
 
 @pytest.mark.parametrize("proc,fp", [(1, (1.5, 3.5, 0.5, 0.3, 0.4, -0.2)), (1, (1.5, 3.5, 0.5, 0.3, 0.0, 0.0)),
-                                     (2, (1.5, 3.5, 0.6, 0.2, 0.4, -0.5))])
+                                     (2, (1.5, 3.5, 0.6, 0.2, 0.4, -0.5)), (3, (1.0, 0.8, -0.4, 0.0, 0.0, 0.0))])
 def test_driver_forced_mhd_matches_oracle(tmp_path, proc, fp):
     """mhd=ETA with force=1,INNER,OUTER,P3,P4,P5,P6: Set_force(W, ...) then Time_advance(W) steps U and B with
     both supplies recomputed at every stage, the energies equal to the oracle's forced MHD steps.  The force
@@ -200,7 +200,8 @@ def test_driver_forced_mhd_matches_oracle(tmp_path, proc, fp):
     assert abs(glob[-1][1] - Eu) <= 1e-12 * Eu and abs(glob[-1][2] - Eb) <= 1e-12 * Eb
     U0, B0 = o.ic_random(1), o.ic_random(2)
     o.rk4_mhd(U0, B0, nu, eta, dt, steps=steps)
-    assert abs(glob[-1][1] - o.total_energy(U0)) > 1e-6       # the force acted
+    if proc != 3:                                               # the supply forces act at O(1e-3)
+        assert abs(glob[-1][1] - o.total_energy(U0)) > 1e-6
     inner, outer, p3, p4, p5, p6 = fp
     FU, FW = o.force_mhd(last[:3], last[3:], proc, inner, outer, p3, p4, p5, p6)
     fku = 2 * sum(o.shell_spectrum(U[c], FU[c])[0] for c in range(3))
@@ -221,7 +222,7 @@ def test_driver_forced_mhd_matches_oracle(tmp_path, proc, fp):
         assert len(t) == 13, r
         i = int(t[0])
         assert abs(float(t[11]) - feu[i]) <= 1e-10 * sc and abs(float(t[12]) - few[i]) <= 1e-10 * sc, r
-    if p5 > 0:
+    if p5 > 0 or proc == 3:
         assert np.abs(fkw).max() > 0 and np.abs(few).max() > 0
 
 

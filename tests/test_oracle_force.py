@@ -282,3 +282,27 @@ def test_constant_level_forces_weak_magnetic_modes():
     assert abs(e - 0.4 / nf) < 1e-13 * (0.4 / nf)
     FU, FW = o.force_mhd(U, B, 1, 2.5, 3.5, 0.5, 0.0, 0.4, 0.0)
     assert max(np.abs(f).max() for f in FU) == 0 and max(np.abs(f).max() for f in FW) == 0
+
+
+@pytest.mark.parametrize("k0,amp", [(1, 0.8), (2, -0.5)])
+def test_taylor_green_force_is_the_closed_form_field(k0, amp):
+    """force_field_proc 3 (Setup_Taylor_Green_force_field force_model.cc:57-88): the inverse transform of the
+    force is amp (sin k0x cos k0y cos k0z, -cos k0x sin k0y cos k0z, 0) on the grid, whatever the velocity;
+    the MHD W part takes forceW_amp (compute_force_TG.cc:90-104)."""
+    N = (16, 16, 16)
+    o = Oracle(N)
+    V = o.ic_random(3, E0=0.5, kp=3.0)
+    F = o.force(V, 3, float(k0) + 0.7, amp, 0.0)           # k0 is truncated to int, as the reference does
+    x = 2 * np.pi * np.arange(16) / 16
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    ref = [amp * np.sin(k0 * X) * np.cos(k0 * Y) * np.cos(k0 * Z),
+           -amp * np.cos(k0 * X) * np.sin(k0 * Y) * np.cos(k0 * Z), np.zeros_like(X)]
+    for c in range(3):
+        A = F[c].copy()
+        o.inverse(A)
+        got = A.view(np.float64).reshape(16, 16, -1)[:, :, :16]
+        assert np.abs(got - ref[c]).max() < 1e-13
+    FU, FW = o.force_mhd(V, V, 3, float(k0), amp, 0.3, 0.0, 0.0, 0.0)
+    for c in range(3):
+        assert np.array_equal(FU[c], F[c])
+        assert np.allclose(FW[c], F[c] * (0.3 / amp), rtol=0, atol=1e-15)
