@@ -59,7 +59,11 @@ typedef struct {
  * proc 3 == Compute_force_Taylor_Green compute_force_TG.cc:50-104: inner_radius = k0 (truncated to int, as
  *           force_double_para(1)), outer_radius = force_amp, p3 = forceW_amp (MHD); the fixed field
  *           amp (sin k0x cos k0y cos k0z, -cos k0x sin k0y cos k0z, 0) of Setup_Taylor_Green_force_field
- *           force_model.cc:57-88, 1 <= k0 < N/2 inside the dealiased modes; runs on the plain steppers. */
+ *           force_model.cc:57-88, 1 <= k0 < N/2 inside the dealiased modes; runs on the plain steppers,
+ * proc 4 == Compute_force_ABC compute_force_ABC.cc:52-68: inner_radius = k0, outer_radius = A, p3 = B, p4 = C,
+ *           p5 = force_amp; the fixed field amp (B cos k0y + C sin k0z, A sin k0x + C cos k0z,
+ *           A cos k0x + B sin k0y) of Setup_ABC_force_field force_model.cc:122-160, same k0 limits, NS plain
+ *           steppers only. */
 typedef struct th_force {
     int proc;                       /* force_field_proc: 1 or 2 */
     double inner_radius, outer_radius;

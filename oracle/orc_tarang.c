@@ -974,14 +974,32 @@ static int force_ek_hk_e(const orc_grid *g, int proc, double *const V[3], double
     return nf;
 }
 
-typedef struct { int proc; double inner, outer, p3, p4; double **keep; } force_ctx;
+/* Setup_ABC_force_field force_model.cc:122-160 (FOUR), written in the reference's order: amp (B cos k0y +
+ * C sin k0z, A sin k0x + C cos k0z, A cos k0x + B sin k0y) on (0, +-k0, 0), (0, 0, k0) and (+-k0, 0, 0). */
+void orc_force_abc(const orc_grid *g, double *F[3], int k0, double A, double B, double C, double amp)
+{
+    for (int c = 0; c < 3; c++) memset(F[c], 0, 2 * NCPLX(g) * sizeof(double));
+    size_t yp = IDX(g, 0, k0, 0), ym = IDX(g, 0, g->N2 - k0, 0), zp = IDX(g, 0, 0, k0);
+    size_t xp = IDX(g, k0, 0, 0), xm = IDX(g, g->N1 - k0, 0, 0);
+    RE(F[0], yp) = B * amp / 2; RE(F[0], ym) = B * amp / 2;
+    IM(F[0], zp) = -(C * amp / 2);
+    RE(F[1], zp) = C * amp / 2;
+    IM(F[2], yp) = -(B * amp / 2); IM(F[2], ym) = B * amp / 2;
+    IM(F[1], xp) = -(A * amp / 2); RE(F[2], xp) = A * amp / 2;
+    IM(F[1], xm) = A * amp / 2;    RE(F[2], xm) = A * amp / 2;
+}
+
+typedef struct { int proc; double inner, outer, p3, p4, p5; double **keep; } force_ctx;
 
 static void rhs_ns_forced(const orc_grid *g, stepper_t *s, void *ctx)
 {
     const force_ctx *fc = (const force_ctx *)ctx;
     double *F[3], *p = alloc_field(g);
     for (int c = 0; c < 3; c++) F[c] = alloc_field(g);
-    orc_force_ek_hk(g, fc->proc, s->F, F, fc->inner, fc->outer, fc->p3, fc->p4);
+    if (fc->proc == 4)                  /* k0, A, B, C, amp = inner, outer, p3, p4, p5 */
+        orc_force_abc(g, F, (int)fc->inner, fc->outer, fc->p3, fc->p4, fc->p5);
+    else
+        orc_force_ek_hk(g, fc->proc, s->F, F, fc->inner, fc->outer, fc->p3, fc->p4);
     if (fc->keep)                       /* the field this (so far last) force was computed from */
         for (int c = 0; c < 3; c++) fcopy(g, fc->keep[c], s->F[c]);
     orc_nlin_ns(g, s->F, s->N);
@@ -995,9 +1013,9 @@ static void rhs_ns_forced(const orc_grid *g, stepper_t *s, void *ctx)
 /* scheme 0 EULER, 1 RK2, 2 RK4; keep (may be NULL) receives the field of the last stage, whose force the
  * reference's Force1..3 hold after the step */
 void orc_step_ns_forced(const orc_grid *g, double *V[3], double nu, double dt, int scheme, int proc, double inner,
-                        double outer, double p3, double p4, double *keep[3])
+                        double outer, double p3, double p4, double p5, double *keep[3])
 {
-    force_ctx fc = {proc, inner, outer, p3, p4, keep};
+    force_ctx fc = {proc, inner, outer, p3, p4, p5, keep};
     if (scheme == 0) { euler_generic(g, V, nu, dt, rhs_ns_forced, &fc); return; }
     if (scheme == 1) { rk2_generic(g, V, nu, dt, rhs_ns_forced, &fc); return; }
     stepper_t s; memset(&s, 0, sizeof(s));

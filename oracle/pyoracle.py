@@ -61,7 +61,8 @@ def lib():
             "orc_rk2_step_ns": (None, [p, pp, d, d]),
             "orc_force_supply_modes": (i, [p, d, d]),
             "orc_force_ek_hk": (i, [p, i, pp, pp, d, d, d, d]),
-            "orc_step_ns_forced": (None, [p, pp, d, d, i, i, d, d, d, d, pp]),
+            "orc_step_ns_forced": (None, [p, pp, d, d, i, i, d, d, d, d, d, pp]),
+            "orc_force_abc": (None, [p, pp, i, d, d, d, d]),
             "orc_rk4_stage1_ns": (None, [p, pp, d, d]),
             "orc_nlin_mhd": (None, [p, pp, pp, pp, pp]),
             "orc_rk4_step_mhd": (None, [p, pp, pp, d, d, d]),
@@ -233,20 +234,24 @@ class Oracle:
         """nf of Compute_force_const_energy_helicity_supply (modes with inner <= |K| < outer)."""
         return self.L.orc_force_supply_modes(self.gp, inner, outer)
 
-    def force(self, V, proc, inner, outer, p3, p4=0.0):
+    def force(self, V, proc, inner, outer, p3, p4=0.0, p5=0.0):
         """Force field F[3] of force_field_proc 1 (energy/helicity supply: p3 = energy_supply,
-        p4 = epsh_by_k_epse) or 2 (constant energy/helicity: p3 = energy_level, p4 = h_by_k_E)."""
+        p4 = epsh_by_k_epse), 2 (constant energy/helicity: p3 = energy_level, p4 = h_by_k_E), 3 (Taylor-Green:
+        inner = k0, outer = amp) or 4 (ABC: inner = k0, outer = A, p3 = B, p4 = C, p5 = amp)."""
         F = self.zeros(3)
+        if proc == 4:
+            self.L.orc_force_abc(self.gp, _pp(F), int(inner), outer, p3, p4, p5)
+            return F
         nf = self.L.orc_force_ek_hk(self.gp, proc, _pp(V), _pp(F), inner, outer, p3, p4)
         if nf <= 0 and proc != 3:             # proc 3 (Taylor-Green: inner = k0, outer = amp) counts no modes
             raise ValueError("no mode in the forcing band")
         return F
 
-    def step_ns_forced(self, V, nu, dt, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1, last_stage=None):
+    def step_ns_forced(self, V, nu, dt, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1, last_stage=None, p5=0.0):
         """NS steps (scheme 0 EULER, 1 RK2, 2 RK4) with the force recomputed at every stage; last_stage
         (3 arrays, optional) receives the field of the last stage of the last step."""
         for _ in range(steps):
-            self.L.orc_step_ns_forced(self.gp, _pp(V), nu, dt, scheme, proc, inner, outer, p3, p4,
+            self.L.orc_step_ns_forced(self.gp, _pp(V), nu, dt, scheme, proc, inner, outer, p3, p4, p5,
                                       _pp(last_stage) if last_stage is not None else None)
         return V
 

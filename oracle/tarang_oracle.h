@@ -71,7 +71,8 @@ int  orc_force_supply_modes(const orc_grid *g, double inner, double outer);
 int  orc_force_ek_hk(const orc_grid *g, int proc, double *const V[3], double *F[3], double inner, double outer,
                      double p3, double p4);
 void orc_step_ns_forced(const orc_grid *g, double *V[3], double nu, double dt, int scheme, int proc, double inner,
-                        double outer, double p3, double p4, double *keep[3]);   /* scheme 0 EULER, 1 RK2, 2 RK4 */
+                        double outer, double p3, double p4, double p5, double *keep[3]);   /* scheme 0 EULER, 1 RK2, 2 RK4 */
+void orc_force_abc(const orc_grid *g, double *F[3], int k0, double A, double B, double C, double amp);
 void orc_rk4_stage1_ns(const orc_grid *g, double *V[3], double nu, double dt);
 
 /* ---- MHD (compute_nlin_VF_MHD.cc, time_advance_VF_MHD.cc) ---- */

@@ -383,11 +383,12 @@ class Plan:
         self._post(0, "")
         return fields
 
-    def step_plain_forced(self, V, dt, nu, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1, last_stage=None):
+    def step_plain_forced(self, V, dt, nu, proc, inner, outer, p3, p4=0.0, scheme=2, steps=1, last_stage=None,
+                          p5=0.0):
         """NS steps with the force_field_proc `proc` forcing recomputed at every stage; last_stage (3 device
         arrays, optional) receives the field of the last stage."""
         ph = Phys(nu, 0.0, 0.0, 0.0, 0.0)
-        fc = Force(proc, inner, outer, p3, p4)
+        fc = Force(proc, inner, outer, p3, p4, p5)
         self._pre()
         for _ in range(steps):
             _check(lib().th_step_plain_forced(self.h, scheme, _pp(V), dt, C.byref(ph), C.byref(fc),
@@ -458,10 +459,10 @@ class Plan:
                "th_cylinder_ring_spectrum_helicity")
         return H
 
-    def compute_force(self, V, proc, inner, outer, p3, p4=0.0):
+    def compute_force(self, V, proc, inner, outer, p3, p4=0.0, p5=0.0):
         """Force field F[3] of force_field_proc `proc` for the velocity V, and nf."""
         F = self.empty(3)
-        fc = Force(proc, inner, outer, p3, p4)
+        fc = Force(proc, inner, outer, p3, p4, p5)
         nf = C.c_long()
         self._pre()
         self._post(lib().th_compute_force(self.h, _pp(V), _pp(F), C.byref(fc), C.byref(nf)), "th_compute_force")

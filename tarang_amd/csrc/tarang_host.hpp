@@ -157,7 +157,9 @@ public:
     // the force at every stage (Compute_force_TO_rhs single_time_step.cc:351-364): on the fused step when
     // every forced mode is one the dealiased state stores, else on the plain stepper (the fused state is
     // then released).
-    void Set_force(int force_field_proc, double inner_radius, double outer_radius, double p3, double p4);
+    // p5: force_double_para(5) (the amplitude of the ABC force, force_field_proc 4).
+    void Set_force(int force_field_proc, double inner_radius, double outer_radius, double p3, double p4,
+                   double p5 = 0.0);
     // Init_cond() for field_input_proc = 1, init_cond_field.cc:47-86: read V from the file, then
     // Satisfy_reality_condition_field when asked, and Dealias; returns the snapshot's time (NaN if none)
     double Init_cond(const std::string& path, bool reality = false, int snapshot = 0);

@@ -104,7 +104,7 @@ int main(int argc, char** argv)
             void* V[3] = {U.V1.data(), U.V2.data(), U.V3.data()};
             tarang::check(th_ic_random(U.plan(), V, 1, 0.5, 4.0), "th_ic_random");
         }
-        if (fproc && eta < 0) U.Set_force(fproc, fpar[0], fpar[1], fpar[2], fpar[3]);
+        if (fproc && eta < 0) U.Set_force(fproc, fpar[0], fpar[1], fpar[2], fpar[3], fpar[4]);
         if (!outdir.empty()) U.Open_output_files(outdir);
         if (eta >= 0) {
             tarang::IncVF W(ctx, g, eta, N / 3 + 2, 1);

@@ -3,6 +3,8 @@
 //   proc 2: IncFluid::Compute_force_const_energy_helicity compute_force_const_ek_hk.cc:51-140
 //   proc 3: IncFluid::Compute_force_Taylor_Green compute_force_TG.cc:50-104 (Setup_Taylor_Green_force_field
 //           force_model.cc:57-88), a fixed four-mode field
+//   proc 4: IncFluid::Compute_force_ABC compute_force_ABC.cc:52-68 (Setup_ABC_force_field force_model.cc:122-160),
+//           a fixed six-mode field (NS only)
 //   with IncVF::Force_alpha_beta force_alpha_beta.cc:60-89, Get_local_number_modes_in_shell_FOUR
 //   four_basic.cc:48-90, Is_alias_array universal_misc.cc:169-181 + Dealias_force dealias.cc:78-84 and
 //   Satisfy_reality_condition_force_field reality_cond.cc:90-95.
@@ -72,33 +74,46 @@ namespace th {
 int force_setup(const Geo& g, const th_force* f, ForceBand& b)
 {
     if (!f) { set_error("force: null parameters"); return E_ARG; }
-    if (f->proc < 1 || f->proc > 3) {
+    if (f->proc < 1 || f->proc > 4) {
         set_error("force: force_field_proc %d is not supported (1: energy/helicity supply, 2: constant energy/"
-                  "helicity, 3: Taylor-Green)", f->proc);
+                  "helicity, 3: Taylor-Green, 4: ABC)", f->proc);
         return E_UNSUPPORTED;
     }
     if (g.basis != TH_FOUR) { set_error("force: only the FOUR basis is supported"); return E_UNSUPPORTED; }
-    if (f->proc == 3) {
+    if (f->proc == 3 || f->proc == 4) {
         // Compute_force_Taylor_Green compute_force_TG.cc:50-63: k0 = (int) force_double_para(1) (inner_radius),
-        // force_amp = force_double_para(2) (outer_radius); set once, the same at every stage
-        if (!isfinite(f->inner_radius) || !isfinite(f->outer_radius)) {
-            set_error("force: Taylor-Green k0 and amplitude must be finite");
+        // force_amp = force_double_para(2) (outer_radius).  Compute_force_ABC compute_force_ABC.cc:52-68: k0,
+        // A, B, C, force_amp = force_double_para(1..5) (inner_radius, outer_radius, p3, p4, p5).  Set once, the
+        // same at every stage.
+        const char* name = f->proc == 3 ? "Taylor-Green" : "ABC";
+        if (!isfinite(f->inner_radius) || !isfinite(f->outer_radius) ||
+            (f->proc == 4 && (!isfinite(f->p3) || !isfinite(f->p4) || !isfinite(f->p5)))) {
+            set_error("force: %s parameters must be finite", name);
             return E_ARG;
         }
         const int k0 = (int)f->inner_radius;
-        if (k0 < 1 || 2 * k0 >= g.N1 || 2 * k0 >= g.N2 || 2 * k0 >= g.N3 ||
-            (g.dealias && (!kept_mode(0, 1, g.N1, g.N2, g.N3, k0, k0, k0) ||
-                           !kept_mode(0, 1, g.N1, g.N2, g.N3, g.N1 - k0, g.N2 - k0, k0)))) {
-            set_error("force: Taylor-Green k0 = %d must satisfy 1 <= k0 < N/2 on every axis and lie in the "
-                      "dealiased modes", k0);
+        bool kept = true;
+        if (g.dealias) {
+            if (f->proc == 3)
+                kept = kept_mode(0, 1, g.N1, g.N2, g.N3, k0, k0, k0) &&
+                       kept_mode(0, 1, g.N1, g.N2, g.N3, g.N1 - k0, g.N2 - k0, k0);
+            else
+                kept = kept_mode(0, 1, g.N1, g.N2, g.N3, k0, 0, 0) && kept_mode(0, 1, g.N1, g.N2, g.N3, g.N1 - k0, 0, 0) &&
+                       kept_mode(0, 1, g.N1, g.N2, g.N3, 0, k0, 0) && kept_mode(0, 1, g.N1, g.N2, g.N3, 0, g.N2 - k0, 0) &&
+                       kept_mode(0, 1, g.N1, g.N2, g.N3, 0, 0, k0);
+        }
+        if (k0 < 1 || 2 * k0 >= g.N1 || 2 * k0 >= g.N2 || 2 * k0 >= g.N3 || !kept) {
+            set_error("force: %s k0 = %d must satisfy 1 <= k0 < N/2 on every axis and lie in the dealiased modes",
+                      name, k0);
             return E_ARG;
         }
         b = ForceBand{};
         b.N1 = g.N1; b.N2 = g.N2; b.N3 = g.N3; b.H = g.H;
         b.f1 = g.kf[0]; b.f2 = g.kf[1]; b.f3 = g.kf[2];
-        b.proc = 3;
+        b.proc = f->proc;
         b.k0 = k0;
-        b.per_mode = f->outer_radius;
+        b.per_mode = f->proc == 3 ? f->outer_radius : f->p5;
+        b.abc[0] = f->outer_radius; b.abc[1] = f->p3; b.abc[2] = f->p4;
         b.nf = 0;
         b.dealias_force = 0;
         b.emin = MYEPS;
@@ -157,6 +172,10 @@ int force_setup(const Geo& g, const th_force* f, ForceBand& b)
 
 int force_setup_mhd(const Geo& g, const th_force* f, ForceBand& bu, ForceBand& bw)
 {
+    if (f && f->proc == 4) {
+        set_error("force: the ABC force (force_field_proc 4) is not supported for MHD");
+        return E_UNSUPPORTED;
+    }
     TRY(force_setup(g, f, bu));
     if (!isfinite(f->p5) || !isfinite(f->p6)) {
         set_error("force: energy_supply_W / energy_level_W and the W helicity ratio must be finite");

@@ -35,6 +35,16 @@ __device__ __forceinline__ void mode_force(const ForceBand& b, const cplx<R> v[3
         }
         return;
     }
+    if (b.proc == 4) {
+        // Setup_ABC_force_field force_model.cc:122-160 (FOUR): amp (B cos k0y + C sin k0z, A sin k0x + C cos k0z,
+        // A cos k0x + B sin k0y) on (0, +-k0, 0), (0, 0, k0) and (+-k0, 0, 0), each value X * amp / 2
+        const int kx = kx_four(lx, b.N1), ky = ky_of(ly, b.N2), k0 = b.k0;
+        const double amp = b.per_mode, A = b.abc[0] * amp / 2, B = b.abc[1] * amp / 2, C = b.abc[2] * amp / 2;
+        if (kx == 0 && lz == 0 && (ky == k0 || ky == -k0)) { fr[0] = B; fi[2] = ky > 0 ? -B : B; }
+        if (kx == 0 && ky == 0 && lz == k0) { fi[0] = -C; fr[1] = C; }
+        if (ky == 0 && lz == 0 && (kx == k0 || kx == -k0)) { fi[1] = kx > 0 ? -A : A; fr[2] = A; }
+        return;
+    }
     const double kk[3] = {kx_four(lx, b.N1) * b.f1, ky_of(ly, b.N2) * b.f2, lz * b.f3};
     const double km = kmag3(kk[0], kk[1], kk[2]);
     if (!(km > b.inner && km <= b.outer)) return;

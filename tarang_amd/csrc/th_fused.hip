@@ -493,8 +493,8 @@ extern "C" int th_state_set_force(th_state* s, const th_force* f)
             set_error("th_state_set_force: forcing runs on single-box NS states");
             return E_UNSUPPORTED;
         }
-        if (f->proc == 3) {
-            set_error("th_state_set_force: the Taylor-Green force runs on the plain stepper");
+        if (f->proc == 3 || f->proc == 4) {
+            set_error("th_state_set_force: the Taylor-Green and ABC forces run on the plain stepper");
             return E_UNSUPPORTED;
         }
         ForceBand fb;

@@ -753,9 +753,9 @@ void IncFluid::Set_force(IncVF& W, int proc, double inner_radius, double outer_r
     }
 }
 
-void IncFluid::Set_force(int proc, double inner_radius, double outer_radius, double p3, double p4)
+void IncFluid::Set_force(int proc, double inner_radius, double outer_radius, double p3, double p4, double p5)
 {
-    th_force f{proc, inner_radius, outer_radius, p3, p4};
+    th_force f{proc, inner_radius, outer_radius, p3, p4, p5, 0.0};
     stepped_ = false;
     if (fused_ && state_) {
         const int r = th_state_set_force(state_, &f);

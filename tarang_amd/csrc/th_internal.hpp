@@ -137,7 +137,8 @@ struct ForceBand {
     int kxm, kym;             // box origin: kx = tx - kxm, ky = ty - kym
     int nx, ny, nz;           // box extents
     double emin;              // a mode is forced when its modal energy exceeds this (MYEPS; 0 for W, proc 2)
-    int k0;                   // proc 3 (Taylor-Green): the forced wavenumber; per_mode holds the amplitude
+    int k0;                   // proc 3 / 4 (Taylor-Green, ABC): the forced wavenumber; per_mode holds the amplitude
+    double abc[3];            // proc 4: A, B, C
 };
 int force_setup(const Geo& g, const th_force* f, ForceBand& b);
 // the bands of U (p3, p4) and W (p5, p6) of Compute_force(IncVF& W) (compute_force_ek_hk_supply.cc:294-420,

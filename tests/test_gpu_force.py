@@ -144,7 +144,7 @@ def test_force_argument_errors(ctx):
     p = Plan(ctx, (16, 16, 16))
     dV = p.empty(3)
     with pytest.raises(RuntimeError, match="not supported"):
-        p.compute_force(dV, 4, 1.5, 3.5, 1.0)
+        p.compute_force(dV, 5, 1.5, 3.5, 1.0)
     with pytest.raises(RuntimeError, match="no wavevector"):
         p.compute_force(dV, 1, 0.1, 0.5, 1.0)
     with pytest.raises(RuntimeError, match="h_by_k_E"):
@@ -303,7 +303,7 @@ def test_forced_mhd_argument_errors(ctx):
     p = Plan(ctx, (16, 16, 16))
     U, B = p.empty(3), p.empty(3)
     with pytest.raises(RuntimeError, match="not supported"):
-        p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.2, 0.3, 0.1, proc=4)
+        p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.2, 0.3, 0.1, proc=5)
     with pytest.raises(RuntimeError, match="must be finite"):
         p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.5, 3.5, 0.5, 0.0, float("nan"), 0.0)
     with pytest.raises(RuntimeError, match="h_by_k_E_W"):
@@ -350,3 +350,35 @@ def test_taylor_green_force_and_steps_match_oracle(ctx, prec, tol):
         for k0 in (0.0, 5.0, 6.0, 8.0):
             with pytest.raises(RuntimeError, match="Taylor-Green k0"):
                 p.compute_force(dV, 3, k0, 0.8, 0.0)
+
+
+def test_abc_force_and_steps_match_oracle(ctx):
+    """force_field_proc 4 (compute_force_ABC.cc:52-68): k0, A, B, C, amp in inner_radius, outer_radius, p3, p4,
+    p5; the field and the forced EULER / RK2 / RK4 NS steps against the oracle; refused for MHD and by the
+    fused state."""
+    from tarang_amd import Plan, State
+    log("abc force")
+    N = (16, 16, 16)
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    V = o.ic_random(5, E0=0.5, kp=3.0)
+    dV = [p.to_dev(a) for a in V]
+    for k0, A, B, C, amp in [(1, 1.0, 0.7, -0.4, 0.5), (4, -0.3, 1.2, 0.9, 2.0)]:
+        ref = o.force(V, 4, float(k0), A, B, C, amp)
+        F, nf = p.compute_force(dV, 4, float(k0), A, B, C, amp)
+        for c in range(3):
+            assert np.array_equal(H(F[c]), ref[c])
+    for scheme in (0, 1, 2):
+        W = [v.copy() for v in V]
+        gW = [p.to_dev(a) for a in W]
+        o.step_ns_forced(W, 0.01, 1e-3, 4, 1.0, 1.0, 0.7, -0.4, scheme=scheme, steps=3, p5=0.5)
+        p.step_plain_forced(gW, 1e-3, 0.01, 4, 1.0, 1.0, 0.7, -0.4, scheme=scheme, steps=3, p5=0.5)
+        for i in range(3):
+            assert rel(H(gW[i]), W[i]) < 1e-12
+    with pytest.raises(RuntimeError, match="plain stepper"):
+        State(p).set_force(4, 1.0, 1.0, 0.7, -0.4)
+    U, B = p.empty(3), p.empty(3)
+    with pytest.raises(RuntimeError, match="not supported for MHD"):
+        p.step_plain_forced_mhd(U, B, 1e-3, 0.01, 0.02, 1.0, 1.0, 0.7, -0.4, 0.5, 0.0, proc=4)
+    with pytest.raises(RuntimeError, match="ABC k0"):
+        p.compute_force(dV, 4, 6.0, 1.0, 0.7, -0.4, 0.5)

@@ -124,6 +124,20 @@ def test_driver_output_files_match_oracle(tmp_path):
         assert close(float(pi), flux[float(R)], 1e-11) and float(force) == 0.0
 
 
+def test_driver_abc_forced_run_matches_oracle():
+    """force=4,K0,A,B,C,AMP,0: Set_force(4, k0, A, B, C, amp) leaves the fused state for the plain stepper and
+    each stage subtracts the fixed ABC field (compute_force_ABC.cc:52-68); E(t) follows the oracle."""
+    N, steps, dt, nu = 16, 5, 1e-3, 0.01
+    spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", "fused", "force=4,1,1.0,0.7,-0.4,0.5,0")
+    o = Oracle((N, N, N))
+    V = o.ic_random(1)
+    E = []
+    for _ in range(steps):
+        o.step_ns_forced(V, nu, dt, 4, 1.0, 1.0, 0.7, -0.4, p5=0.5)
+        E.append(o.total_energy(V))
+    assert np.abs(glob[:, 1] - np.array(E)).max() <= 1e-12 * max(E)
+
+
 @pytest.mark.parametrize("band", [(0.5, 2.5), (4.0, 5.0)])
 def test_driver_forced_run_matches_oracle(band):
     """force=1,...: IncFluid::Set_force then Time_advance with the supply force recomputed at every stage

@@ -306,3 +306,26 @@ def test_taylor_green_force_is_the_closed_form_field(k0, amp):
     for c in range(3):
         assert np.array_equal(FU[c], F[c])
         assert np.allclose(FW[c], F[c] * (0.3 / amp), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("k0,A,B,C,amp", [(1, 1.0, 0.7, -0.4, 0.5), (3, -0.3, 1.2, 0.9, 2.0)])
+def test_abc_force_is_the_closed_form_field(k0, A, B, C, amp):
+    """force_field_proc 4 (Setup_ABC_force_field force_model.cc:122-160): the inverse transform of the force is
+    amp (B cos k0y + C sin k0z, A sin k0x + C cos k0z, A cos k0x + B sin k0y); the field is solenoidal and
+    satisfies the reality condition."""
+    N = (16, 16, 16)
+    o = Oracle(N)
+    F = o.force(o.zeros(3), 4, float(k0), A, B, C, amp)
+    x = 2 * np.pi * np.arange(16) / 16
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    ref = [amp * (B * np.cos(k0 * Y) + C * np.sin(k0 * Z)), amp * (A * np.sin(k0 * X) + C * np.cos(k0 * Z)),
+           amp * (A * np.cos(k0 * X) + B * np.sin(k0 * Y))]
+    for c in range(3):
+        R = F[c].copy()
+        o.satisfy_reality(R)
+        assert np.array_equal(R, F[c])
+        o.inverse(R)
+        got = R.view(np.float64).reshape(16, 16, -1)[:, :, :16]
+        assert np.abs(got - ref[c]).max() < 1e-13
+    KX, KY, KZ = _kgrid(o)
+    assert np.abs(KX * F[0] + KY * F[1] + KZ * F[2]).max() == 0
