@@ -13,7 +13,10 @@ These vectors pin the oracle independently of the oracle's own code:
   * nlin_triad.npz -- exact truncated convolution N_i(k) = sum_j i k_j sum_{p+q=k} u_i(p) u_j(q)
                       for a sparse seeded field, computed by explicit double loops over modes
                       (brute force, no FFT), for the divergence form of compute_nlin.cc:58-108.
-Usage: python tests/golden/make_golden.py
+  * et_triad.npz   -- energy fluxes and shell-to-shell transfers (NS, MHD with Elsasser fluxes,
+                      passive scalar) of band-limited seeded fields at 12^3 as explicit triad sums over
+                      the full wavenumber lattice (et_triad below).
+Usage: python tests/golden/make_golden.py  (rewrites every fixture; each generator is deterministic)
 """
 import os
 
@@ -125,9 +128,150 @@ def nlin_triad():
                         V=np.stack(V), nlin=np.stack(Nk))
 
 
+def _band_field(rng, N, kmax, vector):
+    """Real random field band-limited to |kx|, |kz| <= kmax[0], kmax[2] and |ky| <= kmax[1] (a set that is
+    symmetric under k -> -k, so the half spectrum stays Hermitian on kz = 0), k = 0 removed, vector fields
+    projected to K.V = 0; half spectrum (N1, N2, N3/2+1) with Tarang's forward scaling 1/(N1 N2 N3)."""
+    N1, N2, N3 = N
+    kx = np.fft.fftfreq(N1, 1.0 / N1).reshape(-1, 1, 1)
+    ky = np.fft.fftfreq(N2, 1.0 / N2).reshape(1, -1, 1)
+    kz = np.arange(N3 // 2 + 1).reshape(1, 1, -1)
+    sup = (np.abs(kx) <= kmax[0]) & (np.abs(ky) <= kmax[1]) & (kz <= kmax[2])
+    sup = sup & ~((kx == 0) & (ky == 0) & (kz == 0))
+    comps = [np.fft.rfftn(rng.standard_normal(N)) / (N1 * N2 * N3) * sup for _ in range(3 if vector else 1)]
+    if vector:
+        K2 = (kx * kx + ky * ky + kz * kz).astype(float)
+        K2[0, 0, 0] = 1.0
+        div = kx * comps[0] + ky * comps[1] + kz * comps[2]
+        comps = [c - k * div / K2 for c, k in zip(comps, (kx, ky, kz))]
+    return comps
+
+
+def _full_grid(A, N, M):
+    """Half spectrum -> full-lattice array on an M^3 grid indexed by k mod M (M > twice the support, so
+    sums of two support vectors never wrap)."""
+    N1, N2, N3 = N
+    F = np.zeros((M, M, M), complex)
+    for l1 in range(N1):
+        kx = l1 if l1 <= N1 // 2 else l1 - N1
+        for l2 in range(N2):
+            ky = l2 if l2 <= N2 // 2 else l2 - N2
+            for l3 in range(N3 // 2 + 1):
+                v = A[l1, l2, l3]
+                if v == 0:
+                    continue
+                F[kx % M, ky % M, l3 % M] = v
+                F[-kx % M, -ky % M, -l3 % M] = np.conj(v)
+    return F
+
+
+def et_triad():
+    """Energy transfer by explicit triad sums over the full wavenumber lattice (no FFT, no multiplicity
+    factors): the receiver-resolved transfer S_R(k | G) = Re[ sum_{p+q=k} i (k . H(q)) (G(p) . R(k)^*) ] for
+    helper H, giver G (a sphere / shell / outside-sphere truncation of a field) and receiver R, the form of
+    EnergyTr_Compute_nlin (ET_compute_nlin.cc:85-120, N_i = D_j FFT(H_j G_i)) paired with
+    Shell_mult_single / Shell_mult_all (four_ET.cc:49-80, 112-151).  Every flux and shell-to-shell entry of
+    compute_flux.cc:52-262 and compute_shell_tr.cc:59-117 is then a signed sum of S_R over receiver modes:
+      Compute_flux:            flux_self(i)      = - sum_{|k|>r_i} S_U(k | U<)         (helper U)
+      Compute_shell_tr:        T(m, n)           = - sum_{k in shell n, |k|<=kin} S_U(k | U_m)
+      Compute_flux(W):         U<->B> in_out     = + sum_{|k|>r_i}  S_B(k | U<)        (helper B)
+                               in_in             = + sum_{0<|k|<=r_i} S_B(k | U<)
+                               B<->B> (W.self)   = - sum_{|k|>r_i}  S_B(k | B<)        (helper U)
+                               B<->U> in_out     = + sum_{|k|>r_i}  S_U(k | B<)        (helper U)
+                               in_in             = + sum_{0<|k|<=r_i} S_U(k | B<)
+                               U>->B> out_out    = + sum_{|k|>r_i}  S_B(k | U>)        (helper B)
+                               Zp, Zm (Z+- = U +- B): - sum_{|k|>r_i} S_{Z+}(k | Z+<) with helper Z-, and
+                               with the roles exchanged
+      Compute_flux_scalar:     flux_SF(i)        = - sum_{|k|>r_i} S_T(k | T<)         (helper U)
+      Compute_shell_tr_scalar: T_SF(m, n)        = - sum_{k in shell n, |k|<=kin} S_T(k | T_m)
+    Shells follow Get_shell_index (basis_basicfn.cc:288-299): shell n holds r_{n-1} < |k| <= r_n; the giver
+    truncations follow Fill_array_shell (universal_ET.cc:45-65): inner < |k| <= outer.  kin is
+    Max_radius_inside_dealiased_FOUR (four_inline.h:233-245) = min(N_i / 3) = 4 at 12^3."""
+    N = (12, 12, 12)
+    M = 16
+    rng = np.random.default_rng(14)
+    kmax = (3, 2, 3)
+    U = _band_field(rng, N, kmax, True)
+    B = _band_field(rng, N, kmax, True)
+    T = _band_field(rng, N, kmax, False)
+    radii = np.array([0.0, 1, 2, 3, 4, 5, 10000.0])
+    kin = 4
+    FU = [_full_grid(a, N, M) for a in U]
+    FB = [_full_grid(a, N, M) for a in B]
+    FT = [_full_grid(T[0], N, M)]
+    k1 = np.fft.fftfreq(M, 1.0 / M)
+    KX, KY, KZ = np.meshgrid(k1, k1, k1, indexing="ij")
+    Kv = (KX, KY, KZ)
+    Kmag = np.sqrt(KX * KX + KY * KY + KZ * KZ)
+
+    def conv_nlin(H, G):
+        # N_i(k) = i sum_j k_j sum_p H_j(k - p) G_i(p), exact on the M-grid (no wrap)
+        out = [np.zeros((M, M, M), complex) for _ in G]
+        idx = np.argwhere(np.abs(sum(np.abs(g) for g in G)) > 0)
+        for (a, b, c) in idx:
+            Hs = [np.roll(h, (a, b, c), axis=(0, 1, 2)) for h in H]
+            kh = 1j * (KX * Hs[0] + KY * Hs[1] + KZ * Hs[2])
+            for i, g in enumerate(G):
+                out[i] += g[a, b, c] * kh
+        return out
+
+    def S(H, G, R):
+        Nl = conv_nlin(H, G)
+        return sum((n * np.conj(r)).real for n, r in zip(Nl, R))
+
+    def trunc(F, lo, hi):
+        m = (Kmag > lo) & (Kmag <= hi)
+        return [f * m for f in F]
+
+    nsph = len(radii) - 1
+    out = {"N": np.array(N), "radii": radii, "kin": np.array(kin),
+           "U": np.stack(U), "B": np.stack(B), "T": T[0]}
+    names = ["u2u", "b2b", "u2b_in_out", "u2b_in_in", "b2u_in_out", "b2u_in_in", "u_out_b_out", "zp", "zm",
+             "flux_ns", "flux_sf"]
+    res = {k: np.zeros(nsph + 1) for k in names}
+    FZp = [u + b for u, b in zip(FU, FB)]
+    FZm = [u - b for u, b in zip(FU, FB)]
+    for i in range(1, nsph + 1):
+        r = radii[i]
+        out_m = Kmag > r
+        in_m = (Kmag > 0) & (Kmag <= r)
+        s = S(FU, trunc(FU, -1, r), FU)
+        res["flux_ns"][i] = res["u2u"][i] = -s[out_m].sum()
+        s = S(FB, trunc(FU, -1, r), FB)
+        res["u2b_in_out"][i] = s[out_m].sum()
+        res["u2b_in_in"][i] = s[in_m].sum()
+        s = S(FU, trunc(FB, -1, r), FB)
+        res["b2b"][i] = -s[out_m].sum()
+        s = S(FU, trunc(FB, -1, r), FU)
+        res["b2u_in_out"][i] = s[out_m].sum()
+        res["b2u_in_in"][i] = s[in_m].sum()
+        s = S(FB, trunc(FU, r, 1e300), FB)
+        res["u_out_b_out"][i] = s[out_m].sum()
+        res["zp"][i] = -S(FZm, trunc(FZp, -1, r), FZp)[out_m].sum()
+        res["zm"][i] = -S(FZp, trunc(FZm, -1, r), FZm)[out_m].sum()
+        res["flux_sf"][i] = -S(FU, trunc(FT, -1, r), FT)[out_m].sum()
+    out.update(res)
+    nsh = nsph
+    shell_of = np.searchsorted(radii, Kmag, side="left")      # first index with |k| <= r_index
+    recv = Kmag <= kin
+    Tns = np.zeros((nsh + 1, nsh + 1))
+    Tsf = np.zeros((nsh + 1, nsh + 1))
+    for m in range(1, nsh):
+        s = S(FU, trunc(FU, radii[m - 1], radii[m]), FU)
+        st = S(FU, trunc(FT, radii[m - 1], radii[m]), FT)
+        for n in range(nsh + 1):
+            sel = recv & (shell_of == n)
+            Tns[m, n] = -s[sel].sum()
+            Tsf[m, n] = -st[sel].sum()
+    out["shell_tr_ns"] = Tns
+    out["shell_tr_sf"] = Tsf
+    np.savez_compressed(os.path.join(HERE, "et_triad.npz"), **out)
+
+
 if __name__ == "__main__":
     four_fft()
     scft_fft()
     kat_out1()
     nlin_triad()
+    et_triad()
     print("golden fixtures written to", HERE)

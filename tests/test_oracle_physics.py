@@ -133,8 +133,15 @@ def test_spectrum_parseval_random_field():
     for a in V:
         Sk, cnt = o.shell_spectrum(a)
         raw += Sk
-    # no partial shells are populated for a dealiased field beyond kmax_inside rescale: compare
-    # the real-space energy instead
+    # shells above kmax_inside are rescaled by Approx_number_modes_in_shell / count; undoing that gives
+    # the plain binned energy, which sums to the total
+    kin = o.max_radius_inside()
+    assert raw[kin + 1:].sum() > 0
+    plain = raw.copy()
+    for k in range(kin + 1, len(raw)):
+        if cnt[k] >= 1:
+            plain[k] = raw[k] * cnt[k] / o.approx_modes(k)
+    assert abs(plain.sum() - E) < 1e-14
     usq = 0.0
     for a in V:
         b = a.copy()
