@@ -48,16 +48,37 @@ def pass_times(state, dt, nrep=3, **phys):
 
 
 def roofline(ms, nbytes, N, esize, physics, traffic=None):
+    """Roofline of the dominant (longest) pass: `achieved` = the bytes that kernel must move per launch (its
+    dealias-pruned reads and writes of whole compacted arrays, each once) / its HIP-event duration.  The
+    canonical bytes of the unpruned pipeline (every pass reading and writing whole (N1, N2, N3/2+1) arrays)
+    divided by the same duration are reported beside it as an effective rate, not as a fraction of peak."""
     import numpy as np
     S = float(N[0]) * N[1] * (N[2] // 2 + 1) * esize
     k = int(np.argmax(ms))
+    alg = float(nbytes[k])
+    ach = alg / (ms[k] * 1e-3) / 1e9
     canon = CANON_S[physics][k] * S
-    ach = canon / (ms[k] * 1e-3) / 1e9
     return {"bound": "hbm", "kernel": PASS_NAMES[k], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "bytes_per_launch_canonical": canon, "bytes_per_launch_pruned": float(nbytes[k]),
-            "achieved_pruned_GB_s": round(nbytes[k] / (ms[k] * 1e-3) / 1e9, 1),
-            "launch_ms": round(float(ms[k]), 4)}, sum(CANON_S[physics]) * S
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_launch": alg,
+            "launch_ms": round(float(ms[k]), 4),
+            "bytes_model": "compacted kept-mode arrays, each read and written once per launch",
+            "canonical_bytes_per_launch": canon,
+            "effective_canonical_GB_s": round(canon / (ms[k] * 1e-3) / 1e9, 1)}, sum(CANON_S[physics]) * S
+
+
+def host_info(threads):
+    """nproc, the OpenMP threads used, OMP_NUM_THREADS and the CPU model of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "threads": threads, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "cpu_model": model}
 
 
 def log(msg):
@@ -92,13 +113,16 @@ def time_steps(state, steps, dt, nu):
 
 
 def spectrum_err(plan64, Va, Vb):
+    """Max relative E(k) error of Va against Vb over shells with E_b(k) >= 1e-10 max E_b, the l2 relative error
+    over all shells, and the shell that sets the maximum (k, E_b(k) / max E_b, its error)."""
     import numpy as np
     Ea = plan64.vector_spectrum(Va)
     Eb = plan64.vector_spectrum(Vb)
     sel = Eb >= 1e-10 * Eb.max()
-    maxrel = float((np.abs(Ea - Eb)[sel] / Eb[sel]).max())
+    rel = np.where(sel, np.abs(Ea - Eb) / np.where(Eb > 0, Eb, 1.0), 0.0)
+    k = int(np.argmax(rel))
     l2 = float(np.linalg.norm(Ea - Eb) / np.linalg.norm(Eb))
-    return maxrel, l2
+    return float(rel[k]), l2, {"k": k, "E_over_max": float(Eb[k] / Eb.max()), "rel_err": float(rel[k])}
 
 
 def divergence_rel(V, N):
@@ -119,23 +143,79 @@ def divergence_rel(V, N):
 
 
 def cpu_baseline(V_host, N, nu, dt, threads):
-    """The FP64 oracle (the reference algorithm restated in C, OpenMP over slabs) timed on one RK
-    stage of the same 512^3 workload on this host's cores."""
+    """The FP64 oracle (the reference algorithm restated in C: split transforms, 9 whole-array transforms per
+    stage, separate array passes, OpenMP over x-slabs) timed on one full RK4 step (4 stages) of the same
+    workload on this host's cores; returns (seconds, threads used, the oracle's result fields)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from pyoracle import Oracle
     o = Oracle(N, threads=threads)
     V = [v.copy() for v in V_host]
     t0 = time.perf_counter()
-    o.rk4_stage1_ns(V, nu, dt)
+    o.rk4_ns(V, nu, dt, steps=1)
     el = time.perf_counter() - t0
-    return el, o.L.orc_get_threads()
+    return el, o.L.orc_get_threads(), V
+
+
+def config1(ctx, threads):
+    """The CPU-runnable configuration: Taylor-Green 64^3, FP64, RK4, dt = 1e-3, nu = 0.01, 100 steps.  The
+    fused FP64 GPU step and the oracle each run it in full; E(k) over unit shells and Pi(R) over spheres
+    R = 1..21, INF of the two runs are compared (max relative error over |ref| >= 1e-10 max |ref|)."""
+    import numpy as np
+    import torch
+    from tarang_amd import Plan, State
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle
+    N = (64, 64, 64)
+    nu, dt, steps = 0.01, 1e-3, 100
+    p = Plan(ctx, N, precision="fp64")
+    s = State(p)
+    s.upload(p.ic_taylor_green(1, 1.0))
+    s.step(dt, nu=nu, steps=1)                   # graph capture outside the timed run (then restart from TG)
+    s.upload(p.ic_taylor_green(1, 1.0))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(dt, nu=nu, steps=steps)
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - t0
+    g = s.download()
+    o = Oracle(N, threads=threads)
+    V = o.ic_taylor_green(1, 1.0)
+    t0 = time.perf_counter()
+    o.rk4_ns(V, nu, dt, steps=steps)
+    tc = time.perf_counter() - t0
+    Er, Eg = o.vector_spectrum(V), p.vector_spectrum(g)
+    sel = Er >= 1e-10 * Er.max()
+    radii = np.concatenate([np.arange(0, 22, dtype=float), [10000.0]])
+    fo = o.flux_ns(V, radii)
+    s2 = State(p)
+    s2.upload(g)
+    fg, _ = s2.flux_shell_tr(radii)
+    fsel = np.abs(fo) >= 1e-10 * np.abs(fo).max()
+    pts = 64.0 ** 3 * 4 * steps
+    return {"workload": "taylor_green_64^3_fp64_rk4_dt1e-3_nu0.01_100_steps",
+            "gpu_fp64_fused_s": round(tg, 4), "gpu_value": round(pts / tg / 1e6, 1),
+            "cpu_oracle_s": round(tc, 3), "cpu_value": round(pts / tc / 1e6, 3), "cpu_threads": o.L.orc_get_threads(),
+            "Ek_rel_err_gpu_vs_cpu_max": float((np.abs(Eg - Er)[sel] / Er[sel]).max()),
+            "Pi_rel_err_gpu_vs_cpu_max": float((np.abs(fg - fo)[fsel] / np.abs(fo)[fsel]).max()),
+            "unit": "Mpts*RKstage/s"}
 
 
 def load_traffic(workload):
+    """Measured HBM bytes per launch of each pass (profiles/traffic_<workload>.json: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs scaled by a copy kernel of known size), keyed by pass, with the launch variant the
+    counters were read on."""
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f)
+    return None
+
+
+def traffic_of(table, name, variant):
+    """The measured bytes of pass `name` when the counters were read on the same launch variant, else None."""
+    e = (table or {}).get(name)
+    if isinstance(e, dict) and e.get("variant") == variant:
+        return e.get("bytes")
     return None
 
 
@@ -216,8 +296,19 @@ def bench_ensemble(args):
     pts = float(sum(N[0] * N[1] * N[2] for N in shapes))
     value = pts * 4 * args.steps / res["ensemble"] / 1e6
     seqv = pts * 4 * args.steps / res["sequential"] / 1e6
-    # algorithmic bytes per RK stage: 51 S_b per box, S_b = N1 N2 (N3/2+1) x 8 B (FP32 complex)
-    stage_bytes = sum(51.0 * N[0] * N[1] * (N[2] // 2 + 1) * 8 for N in shapes)
+    # bytes the pruned kernels must move per RK stage (compacted arrays, each read and written once; mean over
+    # stages 1-4 of one box per size class, from th_profile_passes), summed over the boxes; the canonical model
+    # (51 S_b per box, S_b = N1 N2 (N3/2+1) x 8 B) beside it
+    per_class = {}
+    for b, N in enumerate(shapes):
+        if N[0] not in per_class:
+            nbytes = 0.0
+            for stage in (1, 2, 3, 4):
+                _, by = singles[b].profile_passes(float(dt[b]), nu=float(nu[b]), stage=stage, nrep=1)
+                nbytes += float(by.sum()) / 4
+            per_class[N[0]] = nbytes
+    stage_bytes = sum(per_class[N[0]] for N in shapes)
+    canon_bytes = sum(51.0 * N[0] * N[1] * (N[2] // 2 + 1) * 8 for N in shapes)
     ach = stage_bytes * 4 * args.steps / res["ensemble"] / 1e9
     cpu = None
     if not args.no_cpu and host_ic:
@@ -259,8 +350,12 @@ def bench_ensemble(args):
                        "max_abs_diff_sampled_boxes": worst},
         "roofline": {"bound": "hbm", "kernel": "whole_ensemble_step", "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_model": "canonical 51 S per box per stage; the 64^3 and 96^3 "
-                     "working sets fit in the 256 MB last-level cache"},
+                     "traffic": None, "bytes_per_stage": stage_bytes,
+                     "bytes_model": "pruned kernels' compacted arrays, each read and written once per pass, summed "
+                                    "over the 5 passes and 64 boxes (the 64^3 and 96^3 working sets fit in the "
+                                    "256 MB last-level cache, so HBM traffic can be lower)",
+                     "canonical_bytes_per_stage": canon_bytes,
+                     "effective_canonical_GB_s": round(canon_bytes * 4 * args.steps / res["ensemble"] / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -526,7 +621,7 @@ def main():
         log(f"bench: fp64 {args.steps} steps {t64:.4f} s")
         out64 = s64.download()
         v64 = pts * 4 * args.steps / t64 / 1e6
-        maxrel, l2 = spectrum_err(p64, [a.to(torch.complex128) for a in out32], out64)
+        maxrel, l2, worst = spectrum_err(p64, [a.to(torch.complex128) for a in out32], out64)
         # flux Pi(R) and shell-to-shell transfer T(n,m) over 32 unit shells (radii 0..31, inf) from
         # the fused per-giver-shell pipeline, FP32-mixed against FP64
         radii = np.concatenate([np.arange(32), [10000.0]]).astype(float)
@@ -535,6 +630,9 @@ def main():
         result["fp64"] = {"value": round(v64, 1), "ms_per_step": round(1e3 * t64 / args.steps, 3),
                           "speedup_fp32_mixed": round(value / v64, 3),
                           "Ek_rel_err_fp32_vs_fp64_max": maxrel, "Ek_rel_err_fp32_vs_fp64_l2": l2,
+                          "Ek_gate": {"max_rel_err_le": 1e-5, "pass": bool(maxrel <= 1e-5), "worst_shell": worst,
+                                      "shells": "E_fp64(k) >= 1e-10 max E_fp64",
+                                      "horizon_steps": args.warmup + args.steps},
                           "Pi_rel_err_fp32_vs_fp64_max": float(np.abs(Pi32 - Pi64).max() / np.abs(Pi64).max()),
                           "T_rel_err_fp32_vs_fp64_max": float(np.abs(T32 - T64).max() / np.abs(T64).max()),
                           "divergence_rel_fp64": divergence_rel(out64, N),
@@ -544,7 +642,9 @@ def main():
     # roofline of the dominant kernel (the longest pass)
     traffic = load_traffic(f"ns{args.n}_fp32mixed")
     k = int(np.argmax(ms))
-    roof, canon_stage = roofline(ms, nbytes, N, 8, "ns", (traffic or {}).get(PASS_NAMES[k]))
+    roof, canon_stage = roofline(ms, nbytes, N, 8, "ns", traffic_of(traffic, PASS_NAMES[k], var32[k]))
+    roof["traffic_source"] = ("profiles/traffic_ns512_fp32mixed.md (same launch variant)" if roof["traffic"] is not None
+                              else "no counter reading for this launch variant")
     stage_bytes = float(sum(nbytes))
     stage_ms = float(sum(ms))
     passes = {PASS_NAMES[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
@@ -552,15 +652,23 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        Vh = [a.cpu().numpy() for a in V]
+        # the GPU box gives one GPU's share of the host to a job: 16 threads unless --cpu-threads says otherwise
         threads = args.cpu_threads if args.cpu_threads > 0 else min(os.cpu_count() or 1, 16)
-        log(f"bench: CPU oracle, one RK stage at {N}, {threads} threads")
-        el, used = cpu_baseline(Vh, N, nu, dt, threads)
-        cpu = {"value": round(pts / el / 1e6, 3), "unit": "Mpts*RKstage/s", "cores": int(used),
-               "kind": "port", "seconds": round(el, 2),
-               "sample": f"1 RK4 stage (rhs: 9 whole-array transforms + projection + IF update) of the "
-                         f"same {N[0]}^3 FP64 workload by the oracle (reference algorithm restated in C, "
-                         f"own FFT, OpenMP)"}
+        Vh = [a.cpu().numpy() for a in V]
+        del V
+        log(f"bench: CPU oracle, one RK4 step at {N}, {threads} threads")
+        el, used, _ = cpu_baseline(Vh, N, nu, dt, threads)
+        del Vh
+        log(f"bench: CPU oracle step {el:.2f} s; config 1 (TG 64^3, 100 steps) on GPU and CPU")
+        c1 = config1(ctx, threads)
+        cpu = {"value": round(pts * 4 / el / 1e6, 3), "unit": "Mpts*RKstage/s", "cores": int(used),
+               "kind": "port", "seconds": round(el, 2), "steps_measured": 1,
+               "sample": f"1 full RK4 step (4 stages, each: 9 whole-array transforms + projection + IF update) of "
+                         f"the same {N[0]}^3 FP64 workload by the oracle (reference algorithm restated in C, own "
+                         f"FFT, OpenMP over x-slabs)",
+               "host": host_info(int(used)), "config1": c1}
+    else:
+        del V
 
     line = {
         "metric": "Mpts*RKstage/s at 512^3 NS: FP32-mixed vs FP64 GPU; % HBM roofline; E(k) err",

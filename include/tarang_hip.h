@@ -222,6 +222,11 @@ int th_flux_mhd(th_plan *plan, void *const U[3], void *const B[3], const double 
 int th_flux_scalar(th_plan *plan, void *const V[3], void *T, const double *radii, int nsph, double *out);
 int th_shell_tr_scalar(th_plan *plan, void *const V[3], void *T, const double *radii, int nsh, double *Tself,
                        double *Tsf);
+/* th_shell_tr_mhd == IncVF::Compute_shell_tr(IncVF& W) compute_shell_tr.cc:154-260: out holds five
+ * (nsh+1)^2 row-major matrices, rows n = 1..nsh-1 the giver shell: 0 U to U (shelltoshell_self),
+ * 1 W to W (W.shelltoshell_self), 2 U to W (shelltoshell_VF), 3 Zp to Zp (shelltoshell_Elsasser),
+ * 4 Zm to Zm (W.shelltoshell_Elsasser), Z+- = U +- B. */
+int th_shell_tr_mhd(th_plan *plan, void *const U[3], void *const B[3], const double *radii, int nsh, double *out);
 int th_nusselt(th_plan *plan, void *const V[3], const void *T, double Ra, double Pr, double *Nu);
 
 /* ---- initial conditions on reference-layout device fields ----

@@ -1601,6 +1601,44 @@ void orc_shell_tr_scalar(const orc_grid *g, double *const V[3], const double *T,
     free(G); free(N); free(row);
 }
 
+/* IncVF::Compute_shell_tr(IncVF& W) compute_shell_tr.cc:154-260, five (nsh+1)^2 row-major matrices, rows
+ * n = 1..nsh-1 (giver shell), columns m (receiver shell of Shell_mult_all):
+ *   uu[n][m] = -(U.grad U_n).U over m          (Compute_shell_tr)
+ *   ww[n][m] = -(U.grad W_n).W over m          (helper U, giver W_n)
+ *   uw[n][m] = +(W.grad U_n).W over m          (helper W, giver U_n: U shell n to W shell m)
+ *   zp[n][m] = -(Zm.grad Zp_n).Zp over m, zm[n][m] = -(Zp.grad Zm_n).Zm over m, Z+- = U +- B
+ *   (UB_to_Elsasser_field elsasser.cc:53-60) */
+void orc_shell_tr_mhd(const orc_grid *g, double *const U[3], double *const B[3], const double *radii, int nsh,
+                      double *uu, double *ww, double *uw, double *zp, double *zm)
+{
+    double *G[3], *N[3], *Zp[3], *Zm[3];
+    double *row = (double *)calloc(nsh + 1, sizeof(double));
+    size_t n2 = 2 * NCPLX(g);
+    for (int i = 0; i < 3; i++) {
+        G[i] = alloc_field(g); N[i] = alloc_field(g); Zp[i] = alloc_field(g); Zm[i] = alloc_field(g);
+        for (size_t q = 0; q < n2; q++) { Zp[i][q] = U[i][q] + B[i][q]; Zm[i][q] = U[i][q] - B[i][q]; }
+    }
+    orc_shell_tr_ns(g, U, radii, nsh, uu);
+    double *mats[4] = {ww, uw, zp, zm};
+    for (int k = 0; k < 4; k++)
+        for (int i = 0; i < (nsh + 1) * (nsh + 1); i++) mats[k][i] = 0;
+    /* giver, helper, receiver and sign of each matrix */
+    double *const *giv[4] = {B, U, Zp, Zm};
+    double *const *hel[4] = {U, B, Zm, Zp};
+    double *const *rec[4] = {B, B, Zp, Zm};
+    const double sgn[4] = {-1.0, 1.0, -1.0, -1.0};
+    for (int k = 0; k < 4; k++) {
+        for (int n = 1; n < nsh; n++) {
+            for (int i = 0; i < 3; i++) orc_fill_array_shell(g, giv[k][i], G[i], radii[n - 1], radii[n]);
+            orc_et_nlin(g, hel[k], G, N);
+            orc_shell_mult_all(g, N, rec[k], radii, nsh + 1, row);
+            for (int m = 0; m <= nsh; m++) mats[k][n * (nsh + 1) + m] = sgn[k] * row[m];
+        }
+    }
+    for (int i = 0; i < 3; i++) { free(G[i]); free(N[i]); free(Zp[i]); free(Zm[i]); }
+    free(row);
+}
+
 /* ---- helicity and spectral entropy (FOUR and SCFT), the glob.d / spectrum.d columns ---- */
 /* Vr x Vi of mode i; on SCFT boxes V1 is first multiplied by -i, its sin -> Fourier conversion
  * (Compute_local_helicity_SCFT scft_energy.cc:448-500) */

@@ -79,6 +79,7 @@ def lib():
             "orc_flux_ns": (None, [p, pp, vp, i, vp]),
             "orc_shell_tr_ns": (None, [p, pp, vp, i, vp]),
             "orc_flux_mhd": (None, [p, pp, pp, vp, i] + [vp] * 9),
+            "orc_shell_tr_mhd": (None, [p, pp, pp, vp, i] + [vp] * 5),
             "orc_et_radii": (None, [p, i, i, vp]),
             "orc_et_nlin_scalar": (None, [p, pp, vp, vp]),
             "orc_helicity": (None, [p, pp, vp]),
@@ -426,6 +427,14 @@ class Oracle:
         self.L.orc_shell_tr_ns(self.gp, _pp(V), radii.ctypes.data_as(C.c_void_p), nsh,
                                T.ctypes.data_as(C.c_void_p))
         return T
+
+    def shell_tr_mhd(self, U, B, radii):
+        """The five (nsh+1)^2 matrices of Compute_shell_tr(IncVF& W): uu, ww, uw, zp, zm."""
+        nsh = len(radii) - 1
+        outs = [np.zeros((nsh + 1, nsh + 1)) for _ in range(5)]
+        self.L.orc_shell_tr_mhd(self.gp, _pp(U), _pp(B), radii.ctypes.data_as(C.c_void_p), nsh,
+                                *[o.ctypes.data_as(C.c_void_p) for o in outs])
+        return dict(zip(["uu", "ww", "uw", "zp", "zm"], outs))
 
     def flux_mhd(self, U, B, radii, elsasser=False):
         nsph = len(radii) - 1

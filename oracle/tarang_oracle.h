@@ -115,6 +115,8 @@ void   orc_flux_scalar(const orc_grid *g, double *const V[3], const double *T, c
                        double *flux_self, double *flux_sf);
 void   orc_shell_tr_scalar(const orc_grid *g, double *const V[3], const double *T, const double *radii, int nsh,
                            double *Tself, double *Tsf);
+void   orc_shell_tr_mhd(const orc_grid *g, double *const U[3], double *const B[3], const double *radii, int nsh,
+                        double *uu, double *ww, double *uw, double *zp, double *zm);
 void   orc_flux_mhd(const orc_grid *g, double *U[3], double *B[3], const double *radii, int nsph,
                     double *u2u, double *b2b, double *u2b_in_out, double *u2b_in_in,
                     double *b2u_in_out, double *b2u_in_in, double *u_out_b_out,

@@ -112,6 +112,7 @@ def lib():
         "th_shell_spectrum_helicity": (i, [vp, pp, dp, dp, dp]),
         "th_entropy": (i, [vp, pp, dp]),
         "th_shell_tr_scalar": (i, [vp, pp, vp, dp, i, dp, dp]),
+        "th_shell_tr_mhd": (i, [vp, pp, pp, dp, i, dp]),
         "th_nusselt": (i, [vp, pp, vp, d, d, dp]),
         "th_ic_taylor_green": (i, [vp, pp, i, d]),
         "th_ic_random": (i, [vp, pp, C.c_ulonglong, d, d]),
@@ -596,6 +597,17 @@ class Plan:
         _check(lib().th_shell_tr_scalar(self.h, _pp(V), _ptr(T), rp, nsh, Ts.ctypes.data_as(C.POINTER(C.c_double)),
                                         Tt.ctypes.data_as(C.POINTER(C.c_double))), "th_shell_tr_scalar")
         return Ts, Tt
+
+    def shell_tr_mhd(self, U, B, radii):
+        """The five (nsh+1)^2 matrices of Compute_shell_tr(IncVF& W): uu, ww, uw, zp, zm."""
+        import numpy as np
+        r, rp = _dbl(radii)
+        nsh = len(r) - 1
+        out = np.zeros((5, nsh + 1, nsh + 1))
+        self._pre()
+        _check(lib().th_shell_tr_mhd(self.h, _pp(U), _pp(B), rp, nsh, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "th_shell_tr_mhd")
+        return dict(zip(["uu", "ww", "uw", "zp", "zm"], out))
 
     def shell_tr_ns(self, V, radii):
         import numpy as np

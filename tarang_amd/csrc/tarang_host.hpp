@@ -108,6 +108,9 @@ private:
 class EnergyTr {
 public:
     EnergyTr(const CVF& f, int no_shells, int shell_input_scheme);   // EnergyTr.cc:44-170
+    // shell_input_scheme != 0: the radii are ET_shell_radii_sector_array's entries r(0..no_shells-1), the last
+    // radius INF_RADIUS (EnergyTr.cc:162-170); the scheme-1 default is r(i) = i.
+    void Set_shell_radii(const std::vector<double>& r);
     std::vector<double> shell_radius, sphere_radius;
     std::vector<double> flux_self;                   // (no_spheres+1)
     std::vector<double> shelltoshell_self;           // (no_shells+1)^2 row-major
@@ -146,7 +149,7 @@ private:
 class IncFluid : public IncVF {
 public:
     IncFluid(Context& ctx, const Grid& g, double nu, double Tinit, double Tdt_fixed, const std::string& scheme = "RK4",
-             bool fused = true, int no_shells = 32);
+             bool fused = true, int no_shells = 32, int shell_input_scheme = 1);
     ~IncFluid() override;
     void Compute_rhs();                              // compute_rhs.cc:52-60
     void Time_advance();                             // time_advance.cc:60-163
@@ -168,10 +171,23 @@ public:
     //   Output_global output_energy.cc:43-81, Output_shell_spectrum :368-394, Output_flux output_ET.cc:45-74
     // The force-spectrum (spectrum.d) and force-feed (flux.d) columns come from the force of the current
     // velocity (0 for unforced runs); helicity_flux_switch = 0.
-    void Open_output_files(const std::string& data_dir);
+    // The streams keep the reference's formats: glob.d fixed with 12 decimals (MY_PRECISION), the other
+    // files the default stream format (6 significant digits).  Every file opens with Output_prefix
+    // (file_operations.cc:108-121); shell.d then holds "%% Shellradius = " and the radii 1..no_shells.
+    void Open_output_files(const std::string& data_dir, const std::string& prefix_str = "");
     void Output_global();
     void Output_shell_spectrum();
     void Output_flux();
+    // Output_shell_to_shell output_ET.cc:220-254 (U to U), :258-290 (U to U, T to T), :293-352 (U to U, W to W,
+    // U to W, Zp to Zp, Zm to Zm): rows 1..no_shells-1, columns 1..no_shells of each matrix, printed as a
+    // Blitz++ 2-D array (helicity_shell_ET_switch = 0; no mean field B0).
+    void Output_shell_to_shell();
+    void Output_shell_to_shell(IncSF& T);
+    void Output_shell_to_shell(IncVF& W);
+    std::vector<double> shelltoshell_SF;             // T to T, (no_shells+1)^2 (Compute_shell_tr(T))
+    std::vector<double> shelltoshell_mhd;            // 5 x (no_shells+1)^2: th_shell_tr_mhd's matrices
+    double Tfinal = 0;                               // printed by Output_prefix
+    int anisotropy_switch = 3, waveno_switch = 0;    // globalvar_anisotropy_switch / globalvar_waveno_switch
     using IncVF::Compute_flux;
     // Incompressible MHD with the magnetic field W (W.dissipation_coefficient = eta), the IMHD program's
     // step and outputs (src/IMHD, output_main.cc:311-351):
@@ -199,7 +215,7 @@ public:
     void Output_flux(IncSF& T);                      // output_ET.cc:81-115
     std::vector<double> flux_SF;                     // (no_spheres + 1)
     double Rayleigh = 0, Prandtl = 0;                // Ra, Pr of the RB run
-    std::ofstream global_file, spectrum_file, flux_file;
+    std::ofstream global_file, spectrum_file, flux_file, shell_to_shell_file;
     double Tnow, Tdt, Tdt_fixed;
     std::string integ_scheme;
 
@@ -225,6 +241,7 @@ private:
     th_state* mstate_ = nullptr;   // fused MHD state (U, B), created by the first Time_advance(W)
     th_state* rstate_ = nullptr;   // fused RB state (V, T), created by the first Time_advance(T)
     bool state_fresh_ = false;
+    double Tinit_;
 };
 
 }  // namespace tarang

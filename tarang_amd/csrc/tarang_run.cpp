@@ -3,12 +3,15 @@
 // energy + NaN check, then the shell spectrum and (at the end) the energy flux.
 //
 //   tarang_run N steps dt nu [tg|random|file=<field_in.d>] [FP64|FP32_MIXED] [fused|plain] [save=<field_out.d>]
-//              [out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR]
+//              [out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR] [et=NSH,SCHEME]
 // Output (stdout): "glob t E D" per step, then "spectrum k E(k)" lines and "flux R Pi(R)" lines.
 // file=: Init_cond from a field file (field_input_proc = 1; its "%% Time" sets Tinit); save=: the
 // final field is appended to a field file (Output_field), the restart point of a later run.
 // out=DIR (any position after nu): DIR/out/glob.d every step, DIR/out/spectrum.d and flux.d at the end,
-// in the reference's formats (Output_global, Output_shell_spectrum, Output_flux).
+// in the reference's formats (Output_global, Output_shell_spectrum, Output_flux), and DIR/out/shell.d at the end
+// (Output_shell_to_shell).
+// et=NSH,SCHEME: NSH energy-transfer shells and spheres with shell_input_scheme SCHEME (0: the 2^(n/4)-like
+// radii of EnergyTr.cc:136-160; else unit radii); the radii are printed as "radius i r" lines.
 // force=PROC,INNER,OUTER,P3,P4: force_field_proc 1 (energy/helicity supply) or 2 (constant energy/helicity)
 // with its force_double_para, recomputed at every stage; with mhd=, P5 and P6 are the magnetic field's
 // parameters (energy_supply_W, epsh_by_k_epse_W or energy_level_W, h_by_k_E_W; 0 when left out).
@@ -32,7 +35,7 @@ int main(int argc, char** argv)
 {
     if (argc < 5) {
         fprintf(stderr, "usage: %s N steps dt nu [tg|random|file=PATH] [FP64|FP32_MIXED] [fused|plain] [save=PATH] "
-                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR]\n",
+                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR] [et=NSH,SCHEME]\n",
                 argv[0]);
         return 2;
     }
@@ -45,10 +48,15 @@ int main(int argc, char** argv)
     int fproc = 0;
     double fpar[6] = {0, 0, 0, 0, 0, 0};
     double eta = -1, Ra = 0, Pr = 0;
+    int et_nsh = N / 3 + 2, et_scheme = 1;
     for (int a = 5; a < argc; a++) {
         if (strncmp(argv[a], "save=", 5) == 0) save = argv[a] + 5;
         if (strncmp(argv[a], "out=", 4) == 0) outdir = argv[a] + 4;
         if (strncmp(argv[a], "mhd=", 4) == 0) eta = atof(argv[a] + 4);
+        if (strncmp(argv[a], "et=", 3) == 0 && sscanf(argv[a] + 3, "%d,%d", &et_nsh, &et_scheme) != 2) {
+            fprintf(stderr, "tarang_run: et=NSH,SCHEME expected\n");
+            return 2;
+        }
         if (strncmp(argv[a], "rb=", 3) == 0 && sscanf(argv[a] + 3, "%lf,%lf", &Ra, &Pr) != 2) {
             fprintf(stderr, "tarang_run: rb=RA,PR expected\n");
             return 2;
@@ -67,7 +75,8 @@ int main(int argc, char** argv)
         if (Ra > 0 && Pr > 0) {
             const double kfx = M_PI, kfy = M_PI / sqrt(2.0);
             tarang::Grid g{{0, N, N, N}, {0, kfx, kfy, kfy}, "SCFT", "DEALIAS", prec};
-            tarang::IncFluid U(ctx, g, sqrt(Pr / Ra), 0.0, dt, "RK4", true, N / 3 + 2);
+            tarang::IncFluid U(ctx, g, sqrt(Pr / Ra), 0.0, dt, "RK4", true, et_nsh, et_scheme);
+            U.Tfinal = steps * dt;
             U.Rayleigh = Ra;
             U.Prandtl = Pr;
             tarang::IncSF T(ctx, g, 1.0 / sqrt(Pr * Ra));
@@ -90,11 +99,13 @@ int main(int argc, char** argv)
             if (!outdir.empty()) {
                 U.Output_shell_spectrum(T);
                 U.Output_flux(T);
+                U.Output_shell_to_shell(T);
             }
             return 0;
         }
         tarang::Grid g{{0, N, N, N}, {0, 1.0, 1.0, 1.0}, "FOUR", "DEALIAS", prec};
-        tarang::IncFluid U(ctx, g, nu, 0.0, dt, "RK4", path == "fused", N / 3 + 2);
+        tarang::IncFluid U(ctx, g, nu, 0.0, dt, "RK4", path == "fused", et_nsh, et_scheme);
+        for (int s = 0; s <= U.no_shells; s++) printf("radius %d %.17g\n", s, U.shell_radius[s]);
         if (ic == "tg") {
             U.Setup_Taylor_Green_field(1, 1.0);
         } else if (ic.compare(0, 5, "file=") == 0) {
@@ -105,9 +116,10 @@ int main(int argc, char** argv)
             tarang::check(th_ic_random(U.plan(), V, 1, 0.5, 4.0), "th_ic_random");
         }
         if (fproc && eta < 0) U.Set_force(fproc, fpar[0], fpar[1], fpar[2], fpar[3], fpar[4]);
+        U.Tfinal = U.Tnow + steps * dt;
         if (!outdir.empty()) U.Open_output_files(outdir);
         if (eta >= 0) {
-            tarang::IncVF W(ctx, g, eta, N / 3 + 2, 1);
+            tarang::IncVF W(ctx, g, eta, et_nsh, et_scheme);
             void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
             tarang::check(th_ic_random(W.plan(), B, 2, 0.5, 4.0), "th_ic_random");
             if (fproc) U.Set_force(W, fproc, fpar[0], fpar[1], fpar[2], fpar[3], fpar[4], fpar[5]);
@@ -130,6 +142,7 @@ int main(int argc, char** argv)
             if (!outdir.empty()) {
                 U.Output_shell_spectrum(W);
                 U.Output_flux(W);
+                U.Output_shell_to_shell(W);
             }
             return 0;
         }
@@ -154,6 +167,7 @@ int main(int argc, char** argv)
         if (!outdir.empty()) {
             U.Output_shell_spectrum();
             U.Output_flux();
+            U.Output_shell_to_shell();
         }
     } catch (const std::exception& e) {
         fprintf(stderr, "tarang_run: %s\n", e.what());

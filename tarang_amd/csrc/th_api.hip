@@ -1260,6 +1260,54 @@ extern "C" int th_shell_tr_scalar(th_plan* p, void* const V[3], void* T, const d
     return DISPATCH(p, do_shell_tr_scalar, p, V, T, radii, nsh, Tself, Tsf);
 }
 
+// IncVF::Compute_shell_tr(IncVF& W) compute_shell_tr.cc:154-260: out holds five (nsh+1)^2 row-major
+// matrices, rows n = 1..nsh-1 the giver shell: 0 U to U (Compute_shell_tr), 1 W to W = -(U.grad W_n).W,
+// 2 U to W = +(W.grad U_n).W, 3 Zp to Zp = -(Zm.grad Zp_n).Zp, 4 Zm to Zm = -(Zp.grad Zm_n).Zm,
+// Z+- = U +- B (UB_to_Elsasser_field elsasser.cc:53-60).
+template <typename R> static int do_shell_tr_mhd(th_plan* p, void* const U[3], void* const B[3], const double* radii,
+                                                 int nsh, double* out)
+{
+    const size_t M = (size_t)(nsh + 1) * (nsh + 1);
+    TRY(do_shell_tr<R>(p, U, radii, nsh, out));
+    Ops<R> o(p);
+    cplx<R>* u[3] = {CP(R, U[0]), CP(R, U[1]), CP(R, U[2])};
+    cplx<R>* b[3] = {CP(R, B[0]), CP(R, B[1]), CP(R, B[2])};
+    cplx<R>* G[3] = {o.S(SC_G0), o.S(SC_G0 + 1), o.S(SC_G0 + 2)};
+    cplx<R>* Nn[3] = {o.S(SC_NE0), o.S(SC_NE0 + 1), o.S(SC_NE0 + 2)};
+    cplx<R>* zp[3] = {o.S(SC_COPY0), o.S(SC_COPY0 + 1), o.S(SC_COPY0 + 2)};
+    cplx<R>* zm[3] = {o.S(SC_COPY0 + 3), o.S(SC_COPY0 + 4), o.S(SC_COPY0 + 5)};
+    for (int i = 0; i < 3; i++) {
+        TRY(o.copy(zp[i], u[i])); TRY(o.axpy(zp[i], 1.0, b[i]));
+        TRY(o.copy(zm[i], u[i])); TRY(o.axpy(zm[i], -1.0, b[i]));
+    }
+    TRY(upload_radii(p, radii, nsh + 1));
+    // giver, helper, receiver and sign of matrices 1..4
+    cplx<R>* const* giv[4] = {b, u, zp, zm};
+    cplx<R>* const* hel[4] = {u, b, zm, zp};
+    cplx<R>* const* rec[4] = {b, b, zp, zm};
+    const double sgn[4] = {-1.0, 1.0, -1.0, -1.0};
+    std::vector<double> row(nsh + 1);
+    for (int k = 0; k < 4; k++) {
+        double* T = out + (k + 1) * M;
+        for (size_t i = 0; i < M; i++) T[i] = 0;
+        for (int nn = 1; nn < nsh; nn++) {
+            for (int i = 0; i < 3; i++) TRY(k_fill_shell<R>(p->g, giv[k][i], G[i], radii[nn - 1], radii[nn], o.st));
+            TRY(o.et_nlin(hel[k], G, Nn));
+            const cplx<R>* a[3] = {Nn[0], Nn[1], Nn[2]};
+            const cplx<R>* r[3] = {rec[k][0], rec[k][1], rec[k][2]};
+            TRY(reduce_shell_mult_all<R>(p->g, a, r, p->radii_dev, nsh + 1, row.data(), p->rs, o.st));
+            for (int m = 0; m <= nsh; m++) T[nn * (nsh + 1) + m] = sgn[k] * row[m];
+        }
+    }
+    return OK;
+}
+extern "C" int th_shell_tr_mhd(th_plan* p, void* const U[3], void* const B[3], const double* radii, int nsh, double* out)
+{
+    TRY(check_plan(p));
+    if (!U || !B || !radii || nsh < 2 || !out) { set_error("th_shell_tr_mhd: bad args"); return E_ARG; }
+    return DISPATCH(p, do_shell_tr_mhd, p, U, B, radii, nsh, out);
+}
+
 extern "C" int th_nusselt(th_plan* p, void* const V[3], const void* T, double Ra, double Pr, double* Nu)
 {
     TRY(check_plan(p));

@@ -5,6 +5,7 @@
 #include <math.h>
 #include <string.h>
 #include <string>
+#include <iomanip>
 #include "tarang_host.hpp"
 
 namespace tarang {
@@ -213,6 +214,13 @@ EnergyTr::EnergyTr(const CVF& f, int nsh, int scheme) : no_shells(nsh), no_spher
     flux_self.assign(nsh + 1, 0.0);
     shelltoshell_self.assign((size_t)(nsh + 1) * (nsh + 1), 0.0);
 }
+void EnergyTr::Set_shell_radii(const std::vector<double>& r)
+{
+    if ((int)r.size() < no_shells) throw std::runtime_error("Set_shell_radii: need no_shells radii");
+    for (int i = 0; i < no_shells; i++) shell_radius[i] = r[i];
+    shell_radius[no_shells] = 10000;
+    sphere_radius = shell_radius;
+}
 
 // ---------------------------------------------------------------- IncVF
 IncVF::IncVF(Context& ctx, const Grid& g, double nu, int nsh, int scheme)
@@ -270,8 +278,9 @@ void IncSF::CS_Compute_entropy() { check(th_entropy_scalar(plan_, F.data(), &CS_
 
 // ---------------------------------------------------------------- IncFluid
 IncFluid::IncFluid(Context& ctx, const Grid& g, double nu, double Tinit, double tdt_fixed, const std::string& scheme,
-                   bool fused, int nsh)
-    : IncVF(ctx, g, nu, nsh, 1), Tnow(Tinit), Tdt(tdt_fixed), Tdt_fixed(tdt_fixed), integ_scheme(scheme),
+                   bool fused, int nsh, int shell_scheme)
+    : IncVF(ctx, g, nu, nsh, shell_scheme), Tfinal(Tinit), Tnow(Tinit), Tdt(tdt_fixed), Tdt_fixed(tdt_fixed),
+      integ_scheme(scheme), Tinit_(Tinit),
       fused_(fused && g.basis_type == "FOUR" && scheme == "RK4")
 {
     if (fused_) check(th_state_create(plan_, &state_, TH_NS), "th_state_create");
@@ -306,14 +315,99 @@ double IncFluid::Init_cond(const std::string& path, bool reality, int snapshot)
     return t;
 }
 
-void IncFluid::Open_output_files(const std::string& data_dir)
+// Blitz++'s stream output of arrays (blitz/array/io.cc), as the reference's shell.d prints them: a 1-D array as
+// its extent, a newline, " [ ", every value in setw(9) followed by a space, a newline and two spaces after
+// every 7th value, then " ]"; a 2-D array as "rows x columns", a newline, "[ ", the rows (each value in setw(9)
+// and a space, a newline and two spaces after every 7th value of a row, a newline and two spaces between rows)
+// and "]" with a newline.
+static void blitz_out_1d(std::ostream& os, const double* a, int n)
+{
+    os << n << std::endl << " [ ";
+    for (int i = 0; i < n; i++) {
+        os << std::setw(9) << a[i] << " ";
+        if (!((i + 1) % 7)) os << std::endl << "  ";
+    }
+    os << " ]";
+}
+static void blitz_out_2d(std::ostream& os, const double* a, int ld, int r0, int r1, int c0, int c1)
+{
+    os << (r1 - r0 + 1) << " x " << (c1 - c0 + 1) << std::endl << "[ ";
+    for (int i = r0; i <= r1; i++) {
+        for (int j = c0; j <= c1; j++) {
+            os << std::setw(9) << a[(size_t)i * ld + j] << " ";
+            if (!((j + 1 - c0) % 7)) os << std::endl << "  ";
+        }
+        if (i != r1) os << std::endl << "  ";
+    }
+    os << "]" << std::endl;
+}
+
+void IncFluid::Open_output_files(const std::string& data_dir, const std::string& prefix_str)
 {
     const std::string out = data_dir + "/out/";
     global_file.open(out + "glob.d");
     spectrum_file.open(out + "spectrum.d");
     flux_file.open(out + "flux.d");
-    if (!global_file || !spectrum_file || !flux_file) throw std::runtime_error("Open_output_files: cannot open files in " + out);
-    for (std::ofstream* f : {&global_file, &spectrum_file, &flux_file}) f->precision(12);
+    shell_to_shell_file.open(out + "shell.d");
+    if (!global_file || !spectrum_file || !flux_file || !shell_to_shell_file)
+        throw std::runtime_error("Open_output_files: cannot open files in " + out);
+    global_file.setf(std::ios::fixed, std::ios::floatfield);
+    global_file.precision(12);                        // MY_PRECISION extern_def_vars.h:78
+    std::ofstream& X = shell_to_shell_file;            // Output_prefix file_operations.cc:108-121
+    X << prefix_str << std::endl;
+    X << "%% Grid size " << grid.N[1] << " " << grid.N[2] << " " << grid.N[3] << std::endl;
+    X << "%% Basis, Integration scheme:  " << grid.basis_type << " " << integ_scheme << std::endl;
+    X << "%% Tinit, Tfinal, Tdt_fixed:  " << Tinit_ << "    " << Tfinal << "    " << Tdt_fixed << std::endl << std::endl;
+    X << "%% globalvar_anisotropy_switch: " << anisotropy_switch << std::endl;
+    X << "%% globalvar_waveno_switch: " << waveno_switch << std::endl;
+    X << "%% Shellradius = ";
+    blitz_out_1d(X, shell_radius.data() + 1, no_shells);
+    X << std::endl;
+}
+
+void IncFluid::Output_shell_to_shell()
+{
+    Compute_shell_tr();
+    shell_to_shell_file << "%% Time = " << Tnow << std::endl;
+    shell_to_shell_file << "%% U to U: " << std::endl;
+    blitz_out_2d(shell_to_shell_file, shelltoshell_self.data(), no_shells + 1, 1, no_shells - 1, 1, no_shells);
+    shell_to_shell_file << std::endl << std::endl << std::endl;
+}
+
+void IncFluid::Output_shell_to_shell(IncSF& T)
+{
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    shelltoshell_SF.assign(shelltoshell_self.size(), 0.0);
+    check(th_shell_tr_scalar(plan_, V, T.F.data(), shell_radius.data(), no_shells, shelltoshell_self.data(),
+                             shelltoshell_SF.data()), "Compute_shell_tr(T)");
+    const int ld = no_shells + 1;
+    shell_to_shell_file << "%% Time = " << Tnow << std::endl;
+    shell_to_shell_file << "%% U to U: " << std::endl;
+    blitz_out_2d(shell_to_shell_file, shelltoshell_self.data(), ld, 1, no_shells - 1, 1, no_shells);
+    shell_to_shell_file << std::endl << "%% T to T: " << std::endl;
+    blitz_out_2d(shell_to_shell_file, shelltoshell_SF.data(), ld, 1, no_shells - 1, 1, no_shells);
+    shell_to_shell_file << std::endl << std::endl << std::endl;
+}
+
+void IncFluid::Output_shell_to_shell(IncVF& W)
+{
+    void* U[3] = {V1.data(), V2.data(), V3.data()};
+    void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+    const int ld = no_shells + 1;
+    const size_t M = (size_t)ld * ld;
+    shelltoshell_mhd.assign(5 * M, 0.0);
+    check(th_shell_tr_mhd(plan_, U, B, shell_radius.data(), no_shells, shelltoshell_mhd.data()), "Compute_shell_tr(W)");
+    std::copy(shelltoshell_mhd.begin(), shelltoshell_mhd.begin() + M, shelltoshell_self.begin());
+    if (W.shelltoshell_self.size() == M)
+        std::copy(shelltoshell_mhd.begin() + M, shelltoshell_mhd.begin() + 2 * M, W.shelltoshell_self.begin());
+    const char* names[5] = {"%% U to U: ", "%% W to W: ", "%% U to W: ", "%% Zp to Zp: ", "%% Zm to Zm: "};
+    shell_to_shell_file << "%% Time = " << Tnow << std::endl;
+    for (int k = 0; k < 5; k++) {
+        shell_to_shell_file << names[k] << std::endl;
+        blitz_out_2d(shell_to_shell_file, shelltoshell_mhd.data() + k * M, ld, 1, no_shells - 1, 1, no_shells);
+        shell_to_shell_file << std::endl;
+    }
+    shell_to_shell_file << std::endl << std::endl << std::endl << std::endl;
 }
 
 void IncFluid::Output_global()

@@ -184,6 +184,9 @@ def et_triad():
                                with the roles exchanged
       Compute_flux_scalar:     flux_SF(i)        = - sum_{|k|>r_i} S_T(k | T<)         (helper U)
       Compute_shell_tr_scalar: T_SF(m, n)        = - sum_{k in shell n, |k|<=kin} S_T(k | T_m)
+      Compute_shell_tr(W):     W to W (m, n)     = - sum_{k in shell n, |k|<=kin} S_B(k | B_m)   (helper U)
+                               U to W (m, n)     = + sum_{k in shell n, |k|<=kin} S_B(k | U_m)   (helper B)
+                               Zp, Zm (m, n)     = - sum S_{Z+}(k | Z+_m) with helper Z-, and exchanged
     Shells follow Get_shell_index (basis_basicfn.cc:288-299): shell n holds r_{n-1} < |k| <= r_n; the giver
     truncations follow Fill_array_shell (universal_ET.cc:45-65): inner < |k| <= outer.  kin is
     Max_radius_inside_dealiased_FOUR (four_inline.h:233-245) = min(N_i / 3) = 4 at 12^3."""
@@ -256,15 +259,26 @@ def et_triad():
     recv = Kmag <= kin
     Tns = np.zeros((nsh + 1, nsh + 1))
     Tsf = np.zeros((nsh + 1, nsh + 1))
+    # Compute_shell_tr(IncVF& W) (compute_shell_tr.cc:154-260): W to W (helper U), U to W (helper W, + sign),
+    # Zp to Zp (helper Zm), Zm to Zm (helper Zp)
+    mhd = {k: np.zeros((nsh + 1, nsh + 1)) for k in ("ww", "uw", "zp", "zm")}
     for m in range(1, nsh):
         s = S(FU, trunc(FU, radii[m - 1], radii[m]), FU)
         st = S(FU, trunc(FT, radii[m - 1], radii[m]), FT)
+        sm = {"ww": -S(FU, trunc(FB, radii[m - 1], radii[m]), FB),
+              "uw": S(FB, trunc(FU, radii[m - 1], radii[m]), FB),
+              "zp": -S(FZm, trunc(FZp, radii[m - 1], radii[m]), FZp),
+              "zm": -S(FZp, trunc(FZm, radii[m - 1], radii[m]), FZm)}
         for n in range(nsh + 1):
             sel = recv & (shell_of == n)
             Tns[m, n] = -s[sel].sum()
             Tsf[m, n] = -st[sel].sum()
+            for k in mhd:
+                mhd[k][m, n] = sm[k][sel].sum()
     out["shell_tr_ns"] = Tns
     out["shell_tr_sf"] = Tsf
+    for k, v in mhd.items():
+        out["shell_tr_mhd_" + k] = v
     np.savez_compressed(os.path.join(HERE, "et_triad.npz"), **out)
 
 

@@ -287,6 +287,24 @@ def test_flux_mhd_plain(ctx, N):
         assert np.abs(got[r] - ref[k]).max() < 1e-12 * sc, k
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_shell_tr_mhd_plain(ctx, prec, tol):
+    """th_shell_tr_mhd (IncVF::Compute_shell_tr(IncVF& W), compute_shell_tr.cc:154-260): U to U, W to W,
+    U to W and the Elsasser shell-to-shell matrices match the oracle over unit shells."""
+    from tarang_amd import Plan
+    N = (24, 24, 24)
+    o = Oracle(N)
+    p = Plan(ctx, N, precision=prec)
+    U, B = o.ic_random(2), o.ic_random(3)
+    radii = np.concatenate([np.arange(9), [10000.0]])
+    ref = o.shell_tr_mhd(U, B, radii)
+    got = p.shell_tr_mhd([p.to_dev(a) for a in U], [p.to_dev(a) for a in B], radii)
+    for k in ("uu", "ww", "uw", "zp", "zm"):
+        sc = np.abs(ref[k]).max()
+        assert sc > 0
+        assert np.abs(got[k] - ref[k]).max() < tol * sc, k
+
+
 @pytest.mark.parametrize("basis,prec,tol", [("SCFT", "fp64", 1e-12), ("FOUR", "fp64", 1e-12), ("SCFT", "fp32", 2e-5)])
 def test_flux_and_shell_tr_scalar_plain(ctx, basis, prec, tol):
     """th_flux_scalar / th_shell_tr_scalar (Compute_flux_scalar compute_flux.cc:89-109, Compute_flux_RB

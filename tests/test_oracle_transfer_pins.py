@@ -54,6 +54,17 @@ def test_flux_mhd_triads(triad):
         _close(f[k][1:], G[k][1:])
 
 
+def test_shell_tr_mhd_triads(triad):
+    """Compute_shell_tr(IncVF& W) (compute_shell_tr.cc:154-260): U to U as Compute_shell_tr, W to W with
+    helper U and the minus sign, U to W with helper W and the plus sign, and the Elsasser pair with
+    Z+- = U +- B, each helper the other variable."""
+    G, o, U, B, T = triad
+    m = o.shell_tr_mhd(U, B, G["radii"])
+    _close(m["uu"], G["shell_tr_ns"])
+    for k in ("ww", "uw", "zp", "zm"):
+        _close(m[k], G["shell_tr_mhd_" + k])
+
+
 def test_flux_and_shell_tr_scalar_triads(triad):
     """Compute_flux_scalar / Compute_shell_tr_scalar (compute_flux.cc:89-111, compute_shell_tr.cc:98-117):
     flux_SF(i) = -sum_{|k|>r_i} Re(N[U, T<] T*), T_SF(m, n) = -sum_{k in shell n} Re(N[U, T_m] T*)."""
