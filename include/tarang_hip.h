@@ -227,6 +227,26 @@ int th_shell_tr_scalar(th_plan *plan, void *const V[3], void *T, const double *r
  * 1 W to W (W.shelltoshell_self), 2 U to W (shelltoshell_VF), 3 Zp to Zp (shelltoshell_Elsasser),
  * 4 Zm to Zm (W.shelltoshell_Elsasser), Z+- = U +- B. */
 int th_shell_tr_mhd(th_plan *plan, void *const U[3], void *const B[3], const double *radii, int nsh, double *out);
+/* th_ring_tr == IncVF::Compute_ring_tr compute_ring_tr.cc:60-291 (cyl = 0: shells x polar sectors about
+ * axis 1..3) and IncVF::Compute_cylinder_ring_tr compute_cylin_ring_tr.cc:54-133 (cyl = 1: K_perp shells x
+ * K_pll slabs), FOUR 3-D.  radii[0..nr] (radii[nr] = INF_RADIUS), edges[0..nsec] (sector angles up to pi/2 or
+ * K_pll edges up to Anis_max_Kpll), 2 <= nr, nsec <= 32.  For giver ring (gs, gq), gs = 1..ngiv (<= nr),
+ * gq = 1..nsec: G = F on the ring (Fill_array_ring / Fill_array_cylinder_ring universal_ET.cc:70-127),
+ * N = EnergyTr_Compute_nlin with helper H (ncomp 3), or its scalar form (ncomp 1: F[0] = T, Rv[0] = T), and
+ * out[(gs (nsec+1) + gq) M + (rs (nsec+1) + rq)] = sign * Ring_mult_all(N, Rv) over receiver ring (rs, rq),
+ * M = (nr+1)(nsec+1) (four_ET.cc:259-460).  The reference's matrices: ring_to_ring_self (H = F = Rv = U,
+ * sign -1, ngiv = nr - 1), W.ring_to_ring_self (H = U, F = Rv = B, -1), ring_to_ring_VF (H = B, F = U,
+ * Rv = B, +1), the Elsasser pair (H = Z-+, F = Rv = Z+-, -1) and ring_to_ring_SF (H = U, F = Rv = T, -1). */
+int th_ring_tr(th_plan *plan, int cyl, int axis, void *const H[3], void *const F[3], void *const Rv[3], int ncomp,
+               const double *radii, int nr, const double *edges, int nsec, int ngiv, double sign, double *out);
+/* th_ring_mult_all == Ring_mult_all four_ET.cc:259-340 (cyl = 0) / Cyl_ring_mult_all :422-500 (cyl = 1):
+ * out[(nr+1)(nsec+1)] = 2 mult Re(A . B*) over each receiver ring (the force feeds of compute_force_feed.cc:
+ * 162-250 with A = Force, B = V). */
+int th_ring_mult_all(th_plan *plan, int cyl, int axis, void *const A[3], void *const B[3], int ncomp,
+                     const double *radii, int nr, const double *edges, int nsec, double *out);
+/* th_ub_to_elsasser == IncVF::UB_to_Elsasser_field elsasser.cc:53-60 (back = 0: U <- U + B, B <- U - B) and
+ * Elsasser_to_UB_field :62-75 (back = 1: U <- (Zp + Zm) / 2, B <- (Zp - Zm) / 2), in place; asynchronous. */
+int th_ub_to_elsasser(th_plan *plan, void *const U[3], void *const B[3], int back);
 int th_nusselt(th_plan *plan, void *const V[3], const void *T, double Ra, double Pr, double *Nu);
 
 /* ---- initial conditions on reference-layout device fields ----

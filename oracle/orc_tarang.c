@@ -2128,3 +2128,97 @@ void orc_rk4_stage1_ns(const orc_grid *g, double *V[3], double nu, double dt)
     rk4_parts(g, &s, tot, dt, 1.0, dt / 6);
     for (int i = 0; i < 3; i++) { free(s.N[i]); free(tot[i]); }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * Ring-to-ring and cylinder-ring-to-ring energy transfers (FOUR, 3-D, axis a):
+ *   IncVF::Compute_ring_tr compute_ring_tr.cc:60-173 (U to U; scalar T to T; MHD W to W, U to W, Zp, Zm
+ *   :174-291), Compute_cylinder_ring_tr compute_cylin_ring_tr.cc:54-133; givers from Fill_array_ring /
+ *   Fill_array_cylinder_ring universal_ET.cc:70-127 (inner < |K| <= outer or inner < K_perp <= outer, and
+ *   left < theta <= right with the unfolded polar angle theta in [0, pi] (so K_pll < 0 lies in no giver
+ *   sector whose right edge is pi/2), or h_lower < K_pll <= h_upper signed; the theta = 0 / K_pll = 0 line
+ *   joins the first sector / slab); receivers from Local_ring_mult_all_FOUR four_ET.cc:259-299 (0 < |K| <=
+ *   Max_radius_inside, Compute_ring_index basis_basicfn.cc:329-366 with theta folded into [0, pi/2]) and
+ *   Local_cyl_ring_mult_all_FOUR four_ET.cc:422-460 (K_perp <= Anis_max_Krho_radius_inside,
+ *   Compute_cylinder_ring_index :413-450).  Radii r[0..nr] (r[nr] = INF_RADIUS), sector angles or K_pll
+ *   edges e[0..nsec].  out is (nr+1)(nsec+1) x (nr+1)(nsec+1) row-major, row (giver shell, giver sector),
+ *   column (receiver shell, receiver sector); givers shell 1..ngiv, sector 1..nsec; entry = sign *
+ *   2 mult Re(N[H, G] . R*) summed over the receiver ring, N = EnergyTr_Compute_nlin (ncomp 3) or its
+ *   scalar form (ncomp 1: G = T ring, R = T).
+ * ------------------------------------------------------------------------------------------ */
+static int ring_giver(const orc_grid *g, int cyl, int a, int l1, int l2, int l3, double lo, double hi, double e0,
+                      double e1)
+{
+    double kpll = aniso_kpll(g, a, l1, l2, l3), kperp = aniso_kperp(g, a, l1, l2, l3);
+    if (cyl) {
+        if (!((kperp > lo) && (kperp <= hi))) return 0;
+        return ((kpll > e0) && (kpll <= e1)) || ((fabs(kpll) < MYEPS) && (e0 < MYEPS));
+    }
+    double km = orc_kmag(g, l1, l2, l3);
+    if (!((km > lo) && (km <= hi))) return 0;
+    double th = polar_angle(kperp, kpll);
+    return ((th > e0) && (th <= e1)) || ((fabs(th) < MYEPS) && (e0 < MYEPS));
+}
+
+static void ring_mult_all(const orc_grid *g, int cyl, int a, double *const A[3], double *const B[3], int ncomp,
+                          const double *radii, int nr, const double *edges, int nsec, double *res)
+{
+    int N1 = g->N1, N2 = g->N2, H = H3(g);
+    int rows = nr + 1, cols = nsec + 1;
+    int kin = orc_max_radius_inside(g), kpm = orc_cylinder_shell_ek_size(g, a) - 1;
+    for (int i = 0; i < rows * cols; i++) res[i] = 0;
+    for (int l1 = 0; l1 < N1; l1++)
+        for (int l2 = 0; l2 < N2; l2++)
+            for (int l3 = 0; l3 < H; l3++) {
+                double kpll = aniso_kpll(g, a, l1, l2, l3), kperp = aniso_kperp(g, a, l1, l2, l3);
+                int si, sj;
+                if (cyl) {
+                    if (!(kperp <= kpm)) continue;
+                    for (si = 0; si < rows; si++)
+                        if (kperp <= radii[si]) break;
+                    sj = slab_index(kpll, edges, cols);
+                } else {
+                    double km = orc_kmag(g, l1, l2, l3);
+                    if (!((km <= kin) && (km > MYEPS))) continue;
+                    for (si = 0; si < rows; si++)
+                        if (km <= radii[si]) break;
+                    sj = sector_index(polar_angle(kperp, kpll), edges, cols);
+                }
+                size_t i = IDX(g, l1, l2, l3);
+                double d = 0;
+                for (int c = 0; c < ncomp; c++) d += re_prod(A[c], B[c], i);
+                add2(res, rows, cols, si, sj, 2 * orc_multiplicity(g, l1, l2, l3) * d);
+            }
+}
+
+void orc_ring_tr(const orc_grid *g, int cyl, int a, double *const H[3], double *const F[3], double *const R[3],
+                 int ncomp, const double *radii, int nr, const double *edges, int nsec, int ngiv, double sign,
+                 double *out)
+{
+    int rows = nr + 1, cols = nsec + 1;
+    size_t M = (size_t)rows * cols;
+    int N1 = g->N1, N2 = g->N2, Hh = H3(g);
+    double *G[3], *N[3];
+    double *row = (double *)calloc(M, sizeof(double));
+    for (int c = 0; c < 3; c++) { G[c] = alloc_field(g); N[c] = alloc_field(g); }
+    for (size_t i = 0; i < M * M; i++) out[i] = 0;
+    for (int gs = 1; gs <= ngiv && gs < rows; gs++)
+        for (int gq = 1; gq <= nsec; gq++) {
+            for (int c = 0; c < ncomp; c++) {
+                memset(G[c], 0, sizeof(double) * 2 * NCPLX(g));
+                for (int l1 = 0; l1 < N1; l1++)
+                    for (int l2 = 0; l2 < N2; l2++)
+                        for (int l3 = 0; l3 < Hh; l3++)
+                            if (ring_giver(g, cyl, a, l1, l2, l3, radii[gs - 1], radii[gs], edges[gq - 1], edges[gq])) {
+                                size_t i = IDX(g, l1, l2, l3);
+                                RE(G[c], i) = RE(F[c], i);
+                                IM(G[c], i) = IM(F[c], i);
+                            }
+            }
+            if (ncomp == 3) orc_et_nlin(g, H, G, N);
+            else orc_et_nlin_scalar(g, H, G[0], N[0]);
+            ring_mult_all(g, cyl, a, N, R, ncomp, radii, nr, edges, nsec, row);
+            for (size_t j = 0; j < M; j++) out[(size_t)(gs * cols + gq) * M + j] = sign * row[j];
+        }
+    for (int c = 0; c < 3; c++) { free(G[c]); free(N[c]); }
+    free(row);
+}

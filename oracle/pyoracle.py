@@ -80,6 +80,7 @@ def lib():
             "orc_shell_tr_ns": (None, [p, pp, vp, i, vp]),
             "orc_flux_mhd": (None, [p, pp, pp, vp, i] + [vp] * 9),
             "orc_shell_tr_mhd": (None, [p, pp, pp, vp, i] + [vp] * 5),
+            "orc_ring_tr": (None, [p, i, i, pp, pp, pp, i, vp, i, vp, i, i, d, vp]),
             "orc_et_radii": (None, [p, i, i, vp]),
             "orc_et_nlin_scalar": (None, [p, pp, vp, vp]),
             "orc_helicity": (None, [p, pp, vp]),
@@ -435,6 +436,24 @@ class Oracle:
         self.L.orc_shell_tr_mhd(self.gp, _pp(U), _pp(B), radii.ctypes.data_as(C.c_void_p), nsh,
                                 *[o.ctypes.data_as(C.c_void_p) for o in outs])
         return dict(zip(["uu", "ww", "uw", "zp", "zm"], outs))
+
+    def ring_tr(self, H, F, R, radii, edges, ngiv=None, sign=-1.0, cyl=False, axis=3):
+        """Ring-to-ring (cyl=False: shells x polar sectors) or cylinder-ring-to-ring (cyl=True: K_perp shells x
+        K_pll slabs) transfer of Compute_ring_tr / Compute_cylinder_ring_tr: helper H (3 comps), giver source F
+        and receiver R (3 comps, or 1 for the scalar form); returns a 4-D array
+        [giver shell, giver sector, receiver shell, receiver sector]."""
+        radii = np.ascontiguousarray(radii, dtype=float)
+        edges = np.ascontiguousarray(edges, dtype=float)
+        nr, nsec = len(radii) - 1, len(edges) - 1
+        ngiv = nr - 1 if ngiv is None else ngiv
+        M = (nr + 1) * (nsec + 1)
+        out = np.zeros((nr + 1, nsec + 1, nr + 1, nsec + 1))
+        nc = len(F)
+        pad = lambda X: list(X) + [X[0]] * (3 - len(X))
+        self.L.orc_ring_tr(self.gp, int(cyl), axis, _pp(H), _pp(pad(F)), _pp(pad(R)), nc,
+                           radii.ctypes.data_as(C.c_void_p), nr, edges.ctypes.data_as(C.c_void_p), nsec, ngiv,
+                           float(sign), out.ctypes.data_as(C.c_void_p))
+        return out
 
     def flux_mhd(self, U, B, radii, elsasser=False):
         nsph = len(radii) - 1

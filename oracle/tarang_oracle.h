@@ -103,6 +103,9 @@ void   orc_helicity(const orc_grid *g, double *const V[3], double *out);
 /* anisotropic spectra, axis a = 1, 2, 3: ring (Max_radius_inside+1) x (nsect+1), cylinder
  * orc_cylinder_shell_ek_size(g, a) x (nslab+1), both row-major */
 void   orc_ring_spectrum(const orc_grid *g, double *const V[3], int nsect, int n, int a, double *S1, double *S2);
+void   orc_ring_tr(const orc_grid *g, int cyl, int a, double *const H[3], double *const F[3], double *const R[3],
+                   int ncomp, const double *radii, int nr, const double *edges, int nsec, int ngiv, double sign,
+                   double *out);
 int    orc_cylinder_shell_ek_size(const orc_grid *g, int a);
 void   orc_aniso_helicity(const orc_grid *g, double *const V[3], int mode, int nbin, int a, double *Hk);
 void   orc_cylinder_ring_spectrum(const orc_grid *g, double *const V[3], int nslab, int n, int a, double *S1,

@@ -113,6 +113,9 @@ def lib():
         "th_entropy": (i, [vp, pp, dp]),
         "th_shell_tr_scalar": (i, [vp, pp, vp, dp, i, dp, dp]),
         "th_shell_tr_mhd": (i, [vp, pp, pp, dp, i, dp]),
+        "th_ring_tr": (i, [vp, i, i, pp, pp, pp, i, dp, i, dp, i, i, d, dp]),
+        "th_ring_mult_all": (i, [vp, i, i, pp, pp, i, dp, i, dp, i, dp]),
+        "th_ub_to_elsasser": (i, [vp, pp, pp, i]),
         "th_nusselt": (i, [vp, pp, vp, d, d, dp]),
         "th_ic_taylor_green": (i, [vp, pp, i, d]),
         "th_ic_random": (i, [vp, pp, C.c_ulonglong, d, d]),
@@ -608,6 +611,20 @@ class Plan:
         _check(lib().th_shell_tr_mhd(self.h, _pp(U), _pp(B), rp, nsh, out.ctypes.data_as(C.POINTER(C.c_double))),
                "th_shell_tr_mhd")
         return dict(zip(["uu", "ww", "uw", "zp", "zm"], out))
+
+    def ring_tr(self, H, F, R, radii, edges, ngiv=None, sign=-1.0, cyl=False, axis=3):
+        """th_ring_tr: [giver shell, giver sector, receiver shell, receiver sector] (see include/tarang_hip.h)."""
+        import numpy as np
+        r, rp = _dbl(radii)
+        e, ep = _dbl(edges)
+        nr, nsec = len(r) - 1, len(e) - 1
+        ngiv = nr - 1 if ngiv is None else ngiv
+        out = np.zeros((nr + 1, nsec + 1, nr + 1, nsec + 1))
+        pad = lambda X: list(X) + [X[0]] * (3 - len(X))
+        self._pre()
+        _check(lib().th_ring_tr(self.h, int(cyl), axis, _pp(H), _pp(pad(F)), _pp(pad(R)), len(F), rp, nr, ep, nsec,
+                                ngiv, float(sign), out.ctypes.data_as(C.POINTER(C.c_double))), "th_ring_tr")
+        return out
 
     def shell_tr_ns(self, V, radii):
         import numpy as np

@@ -15,6 +15,7 @@
 #include <fstream>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 #include "tarang_hip.h"
 
@@ -115,6 +116,21 @@ public:
     std::vector<double> flux_self;                   // (no_spheres+1)
     std::vector<double> shelltoshell_self;           // (no_shells+1)^2 row-major
     int no_shells, no_spheres;
+    // Anisotropic transfers (EnergyTr.cc:176-360) about axis globalvar_anisotropy_switch: ring shells x polar
+    // sectors (ET_anisotropic_ring_switch) and cylinder shells x K_pll slabs (ET_anisotropic_cylinder_switch).
+    // Shell scheme 0: radii 0, 4, 8, log-spaced, kmax/2, kmax, INF (kmax = Max_radius_inside, or
+    // Anis_max_Krho_radius_inside for cylinders); otherwise r(i) = i (ET_shell_radii_sector_array's role).
+    // Sectors split [0, pi/2] and slabs [0, Anis_max_Kpll] linearly (input schemes 0).
+    void Init_ring_tr(int no_ring_shells, int no_sectors, int ring_scheme, int no_cylinder_shells, int no_slabs,
+                      int cylinder_scheme, int axis);
+    std::vector<double> ring_shell_radius, sector_angle_ring_tr, cylinder_shell_radius, cylinder_kpll_array_tr;
+    int no_ring_shells = 0, no_sectors_ring_tr = 0, no_cylinder_shells = 0, no_cylinder_kpll_slabs = 0;
+    int anisotropy_axis = 3;
+    // 4-D tables [giver shell][giver sector][receiver shell][receiver sector], row-major
+    std::vector<double> ring_to_ring_self, cylinder_ring_to_ring_self;
+
+protected:
+    const CVF* et_f_;
 };
 
 class IncVF : public CVF, public NLIN, public EnergyTr {
@@ -125,7 +141,19 @@ public:
     void Compute_flux();                             // compute_flux.cc:52-69
     void Compute_shell_tr();                         // compute_shell_tr.cc:59-79
     void Setup_Taylor_Green_field(int k0, double amp);   // ic_model.cc:58-107
+    // compute_ring_tr.cc:60-291 and compute_cylin_ring_tr.cc:54-133 (Init_ring_tr sets the rings):
+    void Compute_ring_tr();                          // ring_to_ring_self
+    void Compute_ring_tr(IncVF& W);                  // + W.ring_to_ring_self, ring_to_ring_VF, Elsasser pair
+    void Compute_cylinder_ring_tr();                 // cylinder_ring_to_ring_self
+    void Compute_cylinder_ring_tr(IncVF& W);
+    std::vector<double> ring_to_ring_VF, ring_to_ring_Elsasser, cylinder_ring_to_ring_VF,
+        cylinder_ring_to_ring_Elsasser;
     double dissipation_coefficient;
+
+protected:
+    // one 4-D transfer table: helper H, giver source F, receiver R (ncomp 3 or 1), sign; giver shells 1..ngiv
+    void ring_table(bool cyl, void* const H[3], void* const F[3], void* const R[3], int ncomp, double sign, int ngiv,
+                    std::vector<double>& out);
 };
 
 // Scalar field T (CSF / IncSF, Csf.cc, IncSF.h): the temperature of Rayleigh-Benard runs.
@@ -184,6 +212,22 @@ public:
     void Output_shell_to_shell();
     void Output_shell_to_shell(IncSF& T);
     void Output_shell_to_shell(IncVF& W);
+    // Output_ring_to_ring output_ET.cc:427-555 and Output_cylinder_ring_to_ring :627-700 (ring_to_ring.d,
+    // cyl_ring_to_ring.d; opened by Open_output_files when Init_ring_tr set rings): the 4-D tables over giver and
+    // receiver shells 1..n-1 and sectors (slabs) 1..m, as Blitz++ prints a 4-D array, then the force feed
+    // (Compute_force_feed_ring compute_force_feed.cc:162-250: Ring_mult_all(Force, V), 0 unforced).
+    void Output_ring_to_ring();
+    void Output_ring_to_ring(IncSF& T);
+    void Output_ring_to_ring(IncVF& W);
+    void Output_cylinder_ring_to_ring();
+    void Output_cylinder_ring_to_ring(IncSF& T);
+    void Output_cylinder_ring_to_ring(IncVF& W);
+    void Compute_ring_tr(IncSF& T);                  // ring_to_ring_self and ring_to_ring_SF
+    void Compute_cylinder_ring_tr(IncSF& T);
+    using IncVF::Compute_ring_tr;
+    using IncVF::Compute_cylinder_ring_tr;
+    std::vector<double> ring_to_ring_SF, cylinder_ring_to_ring_SF;
+    std::ofstream ring_to_ring_file, cylinder_ring_to_ring_file;
     std::vector<double> shelltoshell_SF;             // T to T, (no_shells+1)^2 (Compute_shell_tr(T))
     std::vector<double> shelltoshell_mhd;            // 5 x (no_shells+1)^2: th_shell_tr_mhd's matrices
     double Tfinal = 0;                               // printed by Output_prefix
@@ -220,6 +264,18 @@ public:
     std::string integ_scheme;
 
 private:
+    struct RingFeed {
+        const char* name;
+        void* const* A;        // nullptr: zeros
+        void* const* B;
+        int ncomp;
+    };
+    void ring_feed(bool cyl, std::ostream& os, void* const* A, void* const* B, int ncomp);
+    void ring_out(bool cyl, const std::vector<std::pair<const char*, const std::vector<double>*>>& tables,
+                  const std::vector<RingFeed>& feeds);
+    void ring_out_ns(bool cyl);
+    void ring_out_scalar(bool cyl, IncSF& T);
+    void ring_out_mhd(bool cyl, IncVF& W);
     struct Force_field {           // Force1..3 as the reference holds them when the time loop writes outputs
         explicit Force_field(IncFluid& u);
         DField f[3];

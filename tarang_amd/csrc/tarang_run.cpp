@@ -4,6 +4,7 @@
 //
 //   tarang_run N steps dt nu [tg|random|file=<field_in.d>] [FP64|FP32_MIXED] [fused|plain] [save=<field_out.d>]
 //              [out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR] [et=NSH,SCHEME]
+//              [ring=NRING,NSECT,NCYL,NSLAB,AXIS,SCHEME]
 // Output (stdout): "glob t E D" per step, then "spectrum k E(k)" lines and "flux R Pi(R)" lines.
 // file=: Init_cond from a field file (field_input_proc = 1; its "%% Time" sets Tinit); save=: the
 // final field is appended to a field file (Output_field), the restart point of a later run.
@@ -12,6 +13,10 @@
 // (Output_shell_to_shell).
 // et=NSH,SCHEME: NSH energy-transfer shells and spheres with shell_input_scheme SCHEME (0: the 2^(n/4)-like
 // radii of EnergyTr.cc:136-160; else unit radii); the radii are printed as "radius i r" lines.
+// ring=...: ring-to-ring transfers over NRING shells x NSECT polar sectors and cylinder-ring transfers over NCYL
+// K_perp shells x NSLAB K_pll slabs about AXIS (Init_ring_tr; SCHEME 0: the 0, 4, 8, log-spaced radii, else unit
+// radii; a count of 0 switches that table off); with out=DIR, DIR/out/ring_to_ring.d and cyl_ring_to_ring.d at
+// the end (Output_ring_to_ring, Output_cylinder_ring_to_ring).
 // force=PROC,INNER,OUTER,P3,P4: force_field_proc 1 (energy/helicity supply) or 2 (constant energy/helicity)
 // with its force_double_para, recomputed at every stage; with mhd=, P5 and P6 are the magnetic field's
 // parameters (energy_supply_W, epsh_by_k_epse_W or energy_level_W, h_by_k_E_W; 0 when left out).
@@ -35,7 +40,8 @@ int main(int argc, char** argv)
 {
     if (argc < 5) {
         fprintf(stderr, "usage: %s N steps dt nu [tg|random|file=PATH] [FP64|FP32_MIXED] [fused|plain] [save=PATH] "
-                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR] [et=NSH,SCHEME]\n",
+                        "[out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR] [et=NSH,SCHEME] "
+                        "[ring=NRING,NSECT,NCYL,NSLAB,AXIS,SCHEME]\n",
                 argv[0]);
         return 2;
     }
@@ -49,10 +55,16 @@ int main(int argc, char** argv)
     double fpar[6] = {0, 0, 0, 0, 0, 0};
     double eta = -1, Ra = 0, Pr = 0;
     int et_nsh = N / 3 + 2, et_scheme = 1;
+    int ring[6] = {0, 0, 0, 0, 3, 1};
     for (int a = 5; a < argc; a++) {
         if (strncmp(argv[a], "save=", 5) == 0) save = argv[a] + 5;
         if (strncmp(argv[a], "out=", 4) == 0) outdir = argv[a] + 4;
         if (strncmp(argv[a], "mhd=", 4) == 0) eta = atof(argv[a] + 4);
+        if (strncmp(argv[a], "ring=", 5) == 0 &&
+            sscanf(argv[a] + 5, "%d,%d,%d,%d,%d,%d", &ring[0], &ring[1], &ring[2], &ring[3], &ring[4], &ring[5]) != 6) {
+            fprintf(stderr, "tarang_run: ring=NRING,NSECT,NCYL,NSLAB,AXIS,SCHEME expected\n");
+            return 2;
+        }
         if (strncmp(argv[a], "et=", 3) == 0 && sscanf(argv[a] + 3, "%d,%d", &et_nsh, &et_scheme) != 2) {
             fprintf(stderr, "tarang_run: et=NSH,SCHEME expected\n");
             return 2;
@@ -77,6 +89,7 @@ int main(int argc, char** argv)
             tarang::Grid g{{0, N, N, N}, {0, kfx, kfy, kfy}, "SCFT", "DEALIAS", prec};
             tarang::IncFluid U(ctx, g, sqrt(Pr / Ra), 0.0, dt, "RK4", true, et_nsh, et_scheme);
             U.Tfinal = steps * dt;
+            if (ring[0] || ring[2]) U.Init_ring_tr(ring[0], ring[1], ring[5], ring[2], ring[3], ring[5], ring[4]);
             U.Rayleigh = Ra;
             U.Prandtl = Pr;
             tarang::IncSF T(ctx, g, 1.0 / sqrt(Pr * Ra));
@@ -100,6 +113,8 @@ int main(int argc, char** argv)
                 U.Output_shell_spectrum(T);
                 U.Output_flux(T);
                 U.Output_shell_to_shell(T);
+                U.Output_ring_to_ring(T);
+                U.Output_cylinder_ring_to_ring(T);
             }
             return 0;
         }
@@ -117,6 +132,7 @@ int main(int argc, char** argv)
         }
         if (fproc && eta < 0) U.Set_force(fproc, fpar[0], fpar[1], fpar[2], fpar[3], fpar[4]);
         U.Tfinal = U.Tnow + steps * dt;
+        if (ring[0] || ring[2]) U.Init_ring_tr(ring[0], ring[1], ring[5], ring[2], ring[3], ring[5], ring[4]);
         if (!outdir.empty()) U.Open_output_files(outdir);
         if (eta >= 0) {
             tarang::IncVF W(ctx, g, eta, et_nsh, et_scheme);
@@ -143,6 +159,8 @@ int main(int argc, char** argv)
                 U.Output_shell_spectrum(W);
                 U.Output_flux(W);
                 U.Output_shell_to_shell(W);
+                U.Output_ring_to_ring(W);
+                U.Output_cylinder_ring_to_ring(W);
             }
             return 0;
         }
@@ -168,6 +186,8 @@ int main(int argc, char** argv)
             U.Output_shell_spectrum();
             U.Output_flux();
             U.Output_shell_to_shell();
+            U.Output_ring_to_ring();
+            U.Output_cylinder_ring_to_ring();
         }
     } catch (const std::exception& e) {
         fprintf(stderr, "tarang_run: %s\n", e.what());

@@ -1308,6 +1308,75 @@ extern "C" int th_shell_tr_mhd(th_plan* p, void* const U[3], void* const B[3], c
     return DISPATCH(p, do_shell_tr_mhd, p, U, B, radii, nsh, out);
 }
 
+// Ring-to-ring / cylinder-ring-to-ring transfer (IncVF::Compute_ring_tr compute_ring_tr.cc:60-291,
+// Compute_cylinder_ring_tr compute_cylin_ring_tr.cc:54-133): for giver ring (gs, gq), gs = 1..ngiv,
+// gq = 1..nsec, G = F on the ring, N = EnergyTr_Compute_nlin with helper H (ncomp 3) or its scalar form
+// (ncomp 1, G = T ring), and row (gs, gq) of out = sign * Ring_mult_all(N, Rv) over the receiver rings.
+template <typename R>
+static int do_ring_tr(th_plan* p, int cyl, int axis, void* const H[3], void* const F[3], void* const Rv[3], int ncomp,
+                      const double* radii, int nr, const double* edges, int nsec, int ngiv, double sign, double* out)
+{
+    RingP rp;
+    TRY(ring_params(p->g, cyl, axis, radii, nr, edges, nsec, &rp));
+    if (ngiv < 1 || ngiv > nr) { set_error("th_ring_tr: giver shells must be 1..%d", nr); return E_ARG; }
+    Ops<R> o(p);
+    cplx<R>* h[3] = {CP(R, H[0]), CP(R, H[1]), CP(R, H[2])};
+    cplx<R>* G[3] = {o.S(SC_G0), o.S(SC_G0 + 1), o.S(SC_G0 + 2)};
+    cplx<R>* Nn[3] = {o.S(SC_NE0), o.S(SC_NE0 + 1), o.S(SC_NE0 + 2)};
+    const size_t M = (size_t)(nr + 1) * (nsec + 1);
+    std::vector<double> row(M);
+    for (size_t i = 0; i < M * M; i++) out[i] = 0;
+    const cplx<R>* a[3] = {Nn[0], ncomp == 3 ? Nn[1] : nullptr, ncomp == 3 ? Nn[2] : nullptr};
+    const cplx<R>* b[3] = {CCP(R, Rv[0]), ncomp == 3 ? CCP(R, Rv[1]) : nullptr, ncomp == 3 ? CCP(R, Rv[2]) : nullptr};
+    for (int gs = 1; gs <= ngiv; gs++)
+        for (int gq = 1; gq <= nsec; gq++) {
+            for (int c = 0; c < ncomp; c++) TRY(fill_ring<R>(p->g, CCP(R, F[c]), G[c], rp, gs, gq, o.st));
+            if (ncomp == 3) TRY(o.et_nlin(h, G, Nn));
+            else TRY(o.et_nlin_scalar(h, G[0], Nn[0]));
+            TRY(reduce_ring_mult<R>(p->g, a, b, rp, row.data(), p->rs, o.st));
+            double* dst = out + (size_t)(gs * (nsec + 1) + gq) * M;
+            for (size_t j = 0; j < M; j++) dst[j] = sign * row[j];
+        }
+    return OK;
+}
+extern "C" int th_ring_tr(th_plan* p, int cyl, int axis, void* const H[3], void* const F[3], void* const Rv[3], int ncomp,
+                          const double* radii, int nr, const double* edges, int nsec, int ngiv, double sign, double* out)
+{
+    TRY(check_plan(p));
+    if (!H || !F || !Rv || !out || (ncomp != 1 && ncomp != 3)) { set_error("th_ring_tr: bad args"); return E_ARG; }
+    return DISPATCH(p, do_ring_tr, p, cyl, axis, H, F, Rv, ncomp, radii, nr, edges, nsec, ngiv, sign, out);
+}
+
+template <typename R>
+static int do_ring_mult_all(th_plan* p, int cyl, int axis, void* const A[3], void* const B[3], int ncomp,
+                            const double* radii, int nr, const double* edges, int nsec, double* out)
+{
+    RingP rp;
+    TRY(ring_params(p->g, cyl, axis, radii, nr, edges, nsec, &rp));
+    const cplx<R>* a[3] = {CCP(R, A[0]), ncomp == 3 ? CCP(R, A[1]) : nullptr, ncomp == 3 ? CCP(R, A[2]) : nullptr};
+    const cplx<R>* b[3] = {CCP(R, B[0]), ncomp == 3 ? CCP(R, B[1]) : nullptr, ncomp == 3 ? CCP(R, B[2]) : nullptr};
+    return reduce_ring_mult<R>(p->g, a, b, rp, out, p->rs, p->ctx->stream);
+}
+extern "C" int th_ring_mult_all(th_plan* p, int cyl, int axis, void* const A[3], void* const B[3], int ncomp,
+                                const double* radii, int nr, const double* edges, int nsec, double* out)
+{
+    TRY(check_plan(p));
+    if (!A || !B || !out || (ncomp != 1 && ncomp != 3)) { set_error("th_ring_mult_all: bad args"); return E_ARG; }
+    return DISPATCH(p, do_ring_mult_all, p, cyl, axis, A, B, ncomp, radii, nr, edges, nsec, out);
+}
+
+template <typename R> static int do_elsasser(th_plan* p, void* const U[3], void* const B[3], int back)
+{
+    for (int c = 0; c < 3; c++) TRY(k_elsasser<R>(p->g.ncplx(), CP(R, U[c]), CP(R, B[c]), back, p->ctx->stream));
+    return OK;
+}
+extern "C" int th_ub_to_elsasser(th_plan* p, void* const U[3], void* const B[3], int back)
+{
+    TRY(check_plan(p));
+    if (!U || !B) { set_error("th_ub_to_elsasser: null"); return E_ARG; }
+    return DISPATCH(p, do_elsasser, p, U, B, back);
+}
+
 extern "C" int th_nusselt(th_plan* p, void* const V[3], const void* T, double Ra, double Pr, double* Nu)
 {
     TRY(check_plan(p));

@@ -425,6 +425,29 @@ int k_real_mult(size_t n, const cplx<R>* A, const cplx<R>* B, cplx<R>* C, hipStr
     return OK;
 }
 
+// UB_to_Elsasser_field / Elsasser_to_UB_field elsasser.cc:53-75: (U, B) -> (U + B, U - B) and back with 1/2
+template <typename R>
+__global__ void k_elsasser_kernel(long n, cplx<R>* __restrict__ U, cplx<R>* __restrict__ B, int back)
+{
+    GRID_LOOP(i, n) {
+        const cplx<R> u = U[i], b = B[i];
+        cplx<R> p = mk<R>(u.x + b.x, u.y + b.y), m = mk<R>(u.x - b.x, u.y - b.y);
+        if (back) {
+            p = mk<R>(R(0.5) * p.x, R(0.5) * p.y);
+            m = mk<R>(R(0.5) * m.x, R(0.5) * m.y);
+        }
+        U[i] = p;
+        B[i] = m;
+    }
+}
+template <typename R>
+int k_elsasser(size_t n, cplx<R>* U, cplx<R>* B, int back, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_elsasser_kernel<R>), ew_grid(n), dim3(256), 0, st, (long)n, U, B, back);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+
 template <typename R>
 __global__ void k_divide_ksqr_kernel(GeoK g, cplx<R>* __restrict__ A)
 {
@@ -572,6 +595,7 @@ int k_zero_plane_x0(const Geo& g, cplx<R>* A, hipStream_t st)
     template int k_add_deriv<R>(const Geo&, int, int, const cplx<R>*, cplx<R>*, hipStream_t);                       \
     template int k_dealias<R>(const Geo&, cplx<R>*, hipStream_t);                                                    \
     template int k_real_mult<R>(size_t, const cplx<R>*, const cplx<R>*, cplx<R>*, hipStream_t);                     \
+    template int k_elsasser<R>(size_t, cplx<R>*, cplx<R>*, int, hipStream_t);                                         \
     template int k_divide_ksqr<R>(const Geo&, cplx<R>*, hipStream_t);                                                \
     template int k_exp_ksqr<R>(const Geo&, cplx<R>*, double, hipStream_t);                                           \
     template int k_axpy<R>(size_t, cplx<R>*, double, const cplx<R>*, hipStream_t);                                   \

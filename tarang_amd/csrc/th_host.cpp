@@ -191,7 +191,7 @@ void NLIN::NLIN_diag_Forward_transform_derivative()
 }
 
 // ---------------------------------------------------------------- EnergyTr
-EnergyTr::EnergyTr(const CVF& f, int nsh, int scheme) : no_shells(nsh), no_spheres(nsh)
+EnergyTr::EnergyTr(const CVF& f, int nsh, int scheme) : no_shells(nsh), no_spheres(nsh), et_f_(&f)
 {
     shell_radius.assign(nsh + 1, 0.0);
     if (scheme == 0) {
@@ -214,6 +214,62 @@ EnergyTr::EnergyTr(const CVF& f, int nsh, int scheme) : no_shells(nsh), no_spher
     flux_self.assign(nsh + 1, 0.0);
     shelltoshell_self.assign((size_t)(nsh + 1) * (nsh + 1), 0.0);
 }
+// EnergyTr.cc:176-360 (ring_shell_input_scheme / cylinder_shell_input_scheme 0 and the linear sector and slab
+// schemes 0); the cylinder radii of scheme 0 take their log-spaced middle when no_ring_shells > 5, as the
+// reference tests that count there (EnergyTr.cc:318).
+void EnergyTr::Init_ring_tr(int nring, int nsect, int ring_scheme, int ncyl, int nslab, int cyl_scheme, int axis)
+{
+    if (axis < 1 || axis > 3) throw std::runtime_error("Init_ring_tr: anisotropy axis must be 1, 2 or 3");
+    anisotropy_axis = axis;
+    auto radii = [](int n, int scheme, double kmax) {
+        std::vector<double> r(n + 1, 0.0);
+        if (scheme == 0) {
+            r[0] = 0;
+            r[1] = 4;
+            r[2] = 8;
+            r[n - 2] = kmax / 2;
+            r[n - 1] = kmax;
+        } else {
+            for (int i = 0; i < n; i++) r[i] = i;
+        }
+        r[n] = 10000;
+        return r;
+    };
+    if (nring > 0) {
+        if (nring < 3 || nsect < 1) throw std::runtime_error("Init_ring_tr: need >= 3 ring shells and >= 1 sector");
+        no_ring_shells = nring;
+        no_sectors_ring_tr = nsect;
+        const double kmax = 1.0 * th_max_radius_inside(et_f_->plan());
+        ring_shell_radius = radii(nring, ring_scheme, kmax);
+        if (ring_scheme == 0 && nring > 5) {
+            const double sc = log2(kmax / 16.0) / (nring - 4);
+            for (int i = 3; i <= nring - 3; i++) ring_shell_radius[i] = 8 * pow(2, sc * (i - 2));
+        }
+        sector_angle_ring_tr.assign(nsect + 1, 0.0);
+        const double dth = (M_PI / 2) / nsect;              // Get_max_polar_angle = pi/2
+        for (int i = 0; i < nsect; i++) sector_angle_ring_tr[i] = i * dth;
+        sector_angle_ring_tr[nsect] = M_PI / 2;
+    }
+    if (ncyl > 0) {
+        if (ncyl < 3 || nslab < 1) throw std::runtime_error("Init_ring_tr: need >= 3 cylinder shells and >= 1 slab");
+        no_cylinder_shells = ncyl;
+        no_cylinder_kpll_slabs = nslab;
+        const double kmax = 1.0 * (th_cylinder_shell_ek_size(et_f_->plan(), axis) - 1);   // Anis_max_Krho_radius_inside
+        cylinder_shell_radius = radii(ncyl, cyl_scheme, kmax);
+        if (cyl_scheme == 0 && no_ring_shells > 5) {
+            const double sc = log2(kmax / 16.0) / (ncyl - 4);
+            for (int i = 3; i <= ncyl - 3; i++) cylinder_shell_radius[i] = 8 * pow(2, sc * (i - 2));
+        }
+        // Anis_max_Kpll four_inline.h:484-514: (N_axis / 3) kfactor_axis on a dealiased grid, N_axis / 2 otherwise
+        const Grid& g = et_f_->grid;
+        const double kpmax = (g.N[axis] / (g.alias_switch == "DEALIAS" ? 3 : 2)) * g.kfactor[axis];
+        cylinder_kpll_array_tr.assign(nslab + 1, 0.0);
+        const double dk = (kpmax - 0.0) / nslab;
+        for (int i = 1; i < nslab; i++) cylinder_kpll_array_tr[i] = 0.0 + i * dk;
+        cylinder_kpll_array_tr[nslab] = kpmax;
+    }
+}
+
 void EnergyTr::Set_shell_radii(const std::vector<double>& r)
 {
     if ((int)r.size() < no_shells) throw std::runtime_error("Set_shell_radii: need no_shells radii");
@@ -250,6 +306,66 @@ void IncVF::Compute_shell_tr()
     void* V[3] = {V1.data(), V2.data(), V3.data()};
     check(th_shell_tr_ns(plan_, V, shell_radius.data(), no_shells, shelltoshell_self.data()), "Compute_shell_tr");
 }
+void IncVF::ring_table(bool cyl, void* const H[3], void* const F[3], void* const R[3], int ncomp, double sign, int ngiv,
+                       std::vector<double>& out)
+{
+    const std::vector<double>& r = cyl ? cylinder_shell_radius : ring_shell_radius;
+    const std::vector<double>& e = cyl ? cylinder_kpll_array_tr : sector_angle_ring_tr;
+    const int nr = cyl ? no_cylinder_shells : no_ring_shells, ns = cyl ? no_cylinder_kpll_slabs : no_sectors_ring_tr;
+    if (nr < 3) throw std::runtime_error("ring transfers: Init_ring_tr has not set these rings");
+    const size_t M = (size_t)(nr + 1) * (ns + 1);
+    out.assign(M * M, 0.0);
+    check(th_ring_tr(plan_, cyl ? 1 : 0, anisotropy_axis, H, F, R, ncomp, r.data(), nr, e.data(), ns, ngiv, sign,
+                     out.data()),
+          cyl ? "Compute_cylinder_ring_tr" : "Compute_ring_tr");
+}
+void IncVF::Compute_ring_tr()
+{
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    ring_table(false, V, V, V, 3, -1.0, no_ring_shells - 1, ring_to_ring_self);
+}
+void IncVF::Compute_cylinder_ring_tr()
+{
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    ring_table(true, V, V, V, 3, -1.0, no_cylinder_shells - 1, cylinder_ring_to_ring_self);
+}
+// compute_ring_tr.cc:174-291 / the cylinder counterpart: W to W (helper U), U to W (helper W, + sign), and the
+// Elsasser pair after UB_to_Elsasser_field (converted back at the end, as the reference does).  The reference
+// loops its MHD giver shells up to no_shells - 1 (the shell-to-shell count); here up to no_ring_shells - 1,
+// the extent of the ring radii.
+void IncVF::Compute_ring_tr(IncVF& W)
+{
+    W.no_ring_shells = no_ring_shells;
+    W.no_sectors_ring_tr = no_sectors_ring_tr;
+    Compute_ring_tr();
+    void* U[3] = {V1.data(), V2.data(), V3.data()};
+    void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+    const int ng = no_ring_shells - 1;
+    ring_table(false, U, B, B, 3, -1.0, ng, W.ring_to_ring_self);
+    ring_table(false, B, U, B, 3, 1.0, ng, ring_to_ring_VF);
+    check(th_ub_to_elsasser(plan_, U, B, 0), "UB_to_Elsasser_field");
+    ring_table(false, B, U, U, 3, -1.0, ng, ring_to_ring_Elsasser);
+    ring_table(false, U, B, B, 3, -1.0, ng, W.ring_to_ring_Elsasser);
+    check(th_ub_to_elsasser(plan_, U, B, 1), "Elsasser_to_UB_field");
+    check(th_sync(plan_), "th_sync");
+}
+void IncVF::Compute_cylinder_ring_tr(IncVF& W)
+{
+    W.no_cylinder_shells = no_cylinder_shells;
+    W.no_cylinder_kpll_slabs = no_cylinder_kpll_slabs;
+    Compute_cylinder_ring_tr();
+    void* U[3] = {V1.data(), V2.data(), V3.data()};
+    void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+    const int ng = no_cylinder_shells - 1;
+    ring_table(true, U, B, B, 3, -1.0, ng, W.cylinder_ring_to_ring_self);
+    ring_table(true, B, U, B, 3, 1.0, ng, cylinder_ring_to_ring_VF);
+    check(th_ub_to_elsasser(plan_, U, B, 0), "UB_to_Elsasser_field");
+    ring_table(true, B, U, U, 3, -1.0, ng, cylinder_ring_to_ring_Elsasser);
+    ring_table(true, U, B, B, 3, -1.0, ng, W.cylinder_ring_to_ring_Elsasser);
+    check(th_ub_to_elsasser(plan_, U, B, 1), "Elsasser_to_UB_field");
+    check(th_sync(plan_), "th_sync");
+}
+
 void IncVF::Setup_Taylor_Green_field(int k0, double amp)
 {
     void* V[3] = {V1.data(), V2.data(), V3.data()};
@@ -363,6 +479,142 @@ void IncFluid::Open_output_files(const std::string& data_dir, const std::string&
     X << "%% Shellradius = ";
     blitz_out_1d(X, shell_radius.data() + 1, no_shells);
     X << std::endl;
+    if (no_ring_shells >= 3) {
+        ring_to_ring_file.open(out + "ring_to_ring.d");
+        if (!ring_to_ring_file) throw std::runtime_error("Open_output_files: cannot open ring_to_ring.d in " + out);
+    }
+    if (no_cylinder_shells >= 3) {
+        cylinder_ring_to_ring_file.open(out + "cyl_ring_to_ring.d");
+        if (!cylinder_ring_to_ring_file) throw std::runtime_error("Open_output_files: cannot open cyl_ring_to_ring.d in " + out);
+    }
+}
+
+// Blitz++'s stream output of an N-d array: the extents joined by " x ", a newline, "[ ", the elements in
+// row-major order, each in setw(9) followed by a space, a newline and two spaces after every 7th, then "]" and
+// a newline.  Here the 4-D sub-table [1..n-1][1..m][1..n-1][1..m] of a (n+1, m+1, n+1, m+1) table.
+static void blitz_out_4d(std::ostream& os, const std::vector<double>& t, int n, int m)
+{
+    const size_t M = (size_t)(n + 1) * (m + 1);
+    os << (n - 1) << " x " << m << " x " << (n - 1) << " x " << m << std::endl << "[ ";
+    long p = 0;
+    for (int a = 1; a < n; a++)
+        for (int b = 1; b <= m; b++)
+            for (int c = 1; c < n; c++)
+                for (int d = 1; d <= m; d++) {
+                    os << std::setw(9) << t[(size_t)(a * (m + 1) + b) * M + (size_t)(c * (m + 1) + d)] << " ";
+                    if (!(++p % 7)) os << std::endl << "  ";
+                }
+    os << "]" << std::endl;
+}
+
+// the receiver-ring force feed Ring_mult_all(A, B) (compute_force_feed.cc:162-250), rows 1..n-1 x cols 1..m;
+// A = nullptr prints zeros (an unforced run)
+void IncFluid::ring_feed(bool cyl, std::ostream& os, void* const* A, void* const* B, int ncomp)
+{
+    const std::vector<double>& r = cyl ? cylinder_shell_radius : ring_shell_radius;
+    const std::vector<double>& e = cyl ? cylinder_kpll_array_tr : sector_angle_ring_tr;
+    const int n = cyl ? no_cylinder_shells : no_ring_shells, m = cyl ? no_cylinder_kpll_slabs : no_sectors_ring_tr;
+    std::vector<double> f((size_t)(n + 1) * (m + 1), 0.0);
+    if (A) check(th_ring_mult_all(plan_, cyl ? 1 : 0, anisotropy_axis, A, B, ncomp, r.data(), n, e.data(), m, f.data()),
+                 "Compute_force_feed_ring");
+    blitz_out_2d(os, f.data(), m + 1, 1, n - 1, 1, m);
+}
+
+// One output time of ring_to_ring.d / cyl_ring_to_ring.d: "%% Time = t", each named 4-D table, then each named
+// force feed (output_ET.cc:427-760).
+void IncFluid::ring_out(bool cyl, const std::vector<std::pair<const char*, const std::vector<double>*>>& tables,
+                        const std::vector<RingFeed>& feeds)
+{
+    std::ofstream& os = cyl ? cylinder_ring_to_ring_file : ring_to_ring_file;
+    const int n = cyl ? no_cylinder_shells : no_ring_shells, m = cyl ? no_cylinder_kpll_slabs : no_sectors_ring_tr;
+    os << "%% Time = " << Tnow << std::endl;
+    for (size_t k = 0; k < tables.size(); k++) {
+        os << tables[k].first << std::endl;
+        blitz_out_4d(os, *tables[k].second, n, m);
+        os << std::endl;
+        if (k + 1 == tables.size()) os << std::endl;
+    }
+    for (const RingFeed& f : feeds) {
+        os << f.name << std::endl;
+        ring_feed(cyl, os, f.A, f.B, f.ncomp);
+        os << std::endl << std::endl;
+    }
+}
+
+void IncFluid::Output_ring_to_ring() { ring_out_ns(false); }
+void IncFluid::Output_cylinder_ring_to_ring() { ring_out_ns(true); }
+void IncFluid::ring_out_ns(bool cyl)
+{
+    if ((cyl ? no_cylinder_shells : no_ring_shells) < 3) return;   // ET_anisotropic_*_switch = 0
+    if (cyl) Compute_cylinder_ring_tr();
+    else Compute_ring_tr();
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    Force_field F(*this);                               // zero before a forced step and for unforced runs
+    void* Fp[3] = {F.f[0].data(), F.f[1].data(), F.f[2].data()};
+    ring_out(cyl, {{"%% U to U: ", cyl ? &cylinder_ring_to_ring_self : &ring_to_ring_self}},
+             {RingFeed{"%% Force feed: ", has_force_ ? Fp : nullptr, V, 3}});
+}
+
+// Compute_ring_tr_RB / _scalar compute_ring_tr.cc:90-170 (PRLARGE / ULARGE): U to U, then T to T with helper U
+void IncFluid::Compute_ring_tr(IncSF& T)
+{
+    Compute_ring_tr();
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    void* Tf[3] = {T.F.data(), nullptr, nullptr};
+    ring_table(false, V, Tf, Tf, 1, -1.0, no_ring_shells - 1, ring_to_ring_SF);
+}
+void IncFluid::Compute_cylinder_ring_tr(IncSF& T)
+{
+    Compute_cylinder_ring_tr();
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    void* Tf[3] = {T.F.data(), nullptr, nullptr};
+    ring_table(true, V, Tf, Tf, 1, -1.0, no_cylinder_shells - 1, cylinder_ring_to_ring_SF);
+}
+
+// Output_ring_to_ring_scalar output_ET.cc:468-499 and its cylinder form: U to U, T to T, then the force feeds
+// (Force, V) and (T.Force, T) with the last RK stage's buoyancy forces of the RB step
+void IncFluid::Output_ring_to_ring(IncSF& T) { ring_out_scalar(false, T); }
+void IncFluid::Output_cylinder_ring_to_ring(IncSF& T) { ring_out_scalar(true, T); }
+void IncFluid::ring_out_scalar(bool cyl, IncSF& T)
+{
+    if ((cyl ? no_cylinder_shells : no_ring_shells) < 3) return;
+    if (cyl) Compute_cylinder_ring_tr(T);
+    else Compute_ring_tr(T);
+    DField fv[3], ft;
+    rb_forces(fv, ft);
+    DField z(elems_, esize_);
+    void* F[3] = {fv[0].data(), fv[1].data(), fv[2].data()};
+    void* V[3] = {V1.data(), V2.data(), V3.data()};
+    void* A[3] = {ft.data(), z.data(), z.data()};
+    void* B[3] = {T.F.data(), z.data(), z.data()};
+    ring_out(cyl, {{"%% U to U: ", cyl ? &cylinder_ring_to_ring_self : &ring_to_ring_self},
+                   {"%% T to T: ", cyl ? &cylinder_ring_to_ring_SF : &ring_to_ring_SF}},
+             {RingFeed{"%% Force feed (Fv * V): ", F, V, 3}, RingFeed{"%% Force feed (Ftheta * theta): ", A, B, 1}});
+}
+
+// Output_ring_to_ring(IncVF& W) output_ET.cc:512-555 and its cylinder form: U to U, W to W, U to W, Zp to Zp,
+// Zm to Zm, then the force feeds (Force, U) and (W.Force, W) (no mean field B0, so no B0 transfer block)
+void IncFluid::Output_ring_to_ring(IncVF& W) { ring_out_mhd(false, W); }
+void IncFluid::Output_cylinder_ring_to_ring(IncVF& W) { ring_out_mhd(true, W); }
+void IncFluid::ring_out_mhd(bool cyl, IncVF& W)
+{
+    if ((cyl ? no_cylinder_shells : no_ring_shells) < 3) return;
+    if (cyl) Compute_cylinder_ring_tr(W);
+    else Compute_ring_tr(W);
+    void* U[3] = {V1.data(), V2.data(), V3.data()};
+    void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
+    DField fu[3], fw[3];
+    if (mhd_forced_) mhd_forces(fu, fw);
+    void* Fu[3] = {fu[0].data(), fu[1].data(), fu[2].data()};
+    void* Fw[3] = {fw[0].data(), fw[1].data(), fw[2].data()};
+    ring_out(cyl,
+             {{"%% U to U: ", cyl ? &cylinder_ring_to_ring_self : &ring_to_ring_self},
+              {"%% W to W: ", cyl ? &W.cylinder_ring_to_ring_self : &W.ring_to_ring_self},
+              {"%% U to W: ", cyl ? &cylinder_ring_to_ring_VF : &ring_to_ring_VF},
+              {"%% Zp to Zp: ", cyl ? &cylinder_ring_to_ring_Elsasser : &ring_to_ring_Elsasser},
+              {"%% Zm to Zm: ", cyl ? &W.cylinder_ring_to_ring_Elsasser : &W.ring_to_ring_Elsasser}},
+             {RingFeed{"%% Force feed (Fv * V): ", mhd_forced_ ? Fu : nullptr, U, 3},
+              RingFeed{"%% Force feed (Fw * W): ", mhd_forced_ ? Fw : nullptr, B, 3}});
 }
 
 void IncFluid::Output_shell_to_shell()

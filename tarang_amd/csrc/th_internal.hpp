@@ -82,6 +82,7 @@ template <typename R> int k_add_deriv(const Geo& g, int axis, int parity, const 
                                       hipStream_t st);       // B += D(A)
 template <typename R> int k_dealias(const Geo& g, cplx<R>* A, hipStream_t st);
 template <typename R> int k_real_mult(size_t n, const cplx<R>* A, const cplx<R>* B, cplx<R>* C, hipStream_t st);
+template <typename R> int k_elsasser(size_t n, cplx<R>* U, cplx<R>* B, int back, hipStream_t st);
 template <typename R> int k_divide_ksqr(const Geo& g, cplx<R>* A, hipStream_t st);
 template <typename R> int k_exp_ksqr(const Geo& g, cplx<R>* A, double factor, hipStream_t st);
 template <typename R> int k_axpy(size_t n, cplx<R>* Y, double a, const cplx<R>* X, hipStream_t st);   // Y = Y + a X
@@ -105,6 +106,20 @@ struct RedScratch {
 // mode selects the weight/binning rule (spectrum, shell_mult_all, shell_mult_single ...)
 template <typename R> int reduce_shell(const Geo& g, const cplx<R>* A, const cplx<R>* B, double n, int nbins,
                                        double* Sk, double* cnt, RedScratch& rs, hipStream_t st, bool rescale = true);
+// Rings of the ring-to-ring (shells x polar sectors) and cylinder-ring-to-ring (K_perp shells x K_pll slabs)
+// transfers (compute_ring_tr.cc, compute_cylin_ring_tr.cc): radii r[0..nr], sector angles / slab edges e[0..nsec]
+constexpr int RING_MAX = 32;
+struct RingP {
+    int cyl, axis, nr, nsec, kin, kpm;
+    double r[RING_MAX + 1], e[RING_MAX + 1];
+};
+int ring_params(const Geo& g, int cyl, int axis, const double* radii, int nr, const double* edges, int nsec, RingP* rp);
+// G = F on giver ring (gs, gq) (Fill_array_ring / Fill_array_cylinder_ring universal_ET.cc:70-127), else 0
+template <typename R> int fill_ring(const Geo& g, const cplx<R>* F, cplx<R>* G, const RingP& rp, int gs, int gq,
+                                    hipStream_t st);
+// out[(nr+1)(nsec+1)] = Ring_mult_all / Cyl_ring_mult_all of A, B (1 or 3 components; A[1] == nullptr: one)
+template <typename R> int reduce_ring_mult(const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3],
+                                           const RingP& rp, double* out, RedScratch& rs, hipStream_t st);
 template <typename R> int reduce_energy(const Geo& g, const cplx<R>* A, const cplx<R>* B, double* out,
                                         RedScratch& rs, hipStream_t st);
 template <typename R> int reduce_Sn(const Geo& g, const cplx<R>* A, double n, double* out, RedScratch& rs,

@@ -385,6 +385,119 @@ __global__ __launch_bounds__(RBLK) void k_aniso(GeoR g, const cplx<R>* __restric
 }
 
 
+// ---- ring-to-ring and cylinder-ring-to-ring transfers (FOUR, 3-D) ----
+// Polar angle of K about the axis (Get_polar_angle basis_basicfn_inline.h:89-112), in [0, pi]; a ratio of
+// exactly +-1 takes +-pi/4, what a correctly rounded atan returns.
+__device__ __forceinline__ double polar_angle_r(double kperp, double kpll)
+{
+    const double eps = 1e-15;
+    double t;
+    if (fabs(kperp) > eps) {
+        if (fabs(kpll) > eps) {
+            const double r = kperp / kpll;
+            t = (r == 1.0) ? M_PI / 4 : (r == -1.0) ? -(M_PI / 4) : atan(r);
+        } else {
+            t = M_PI / 2;
+        }
+    } else {
+        t = (kpll >= 0) ? 0.0 : M_PI;
+    }
+    return (t >= 0) ? t : t + M_PI;
+}
+__device__ __forceinline__ void pll_perp_r(const GeoR& g, int axis, int l1, int l2, int l3, double& kpll, double& kperp)
+{
+#pragma clang fp contract(off)
+    const double kk[3] = {kx_four(l1, g.N1) * g.f1, ky_of(l2, g.N2) * g.f2, l3 * g.f3};
+    const int p = axis % 3, q = (axis + 1) % 3;
+    kpll = kk[axis - 1];
+    kperp = sqrt(kk[p] * kk[p] + kk[q] * kk[q]);
+}
+
+// Fill_array_ring / Fill_array_cylinder_ring universal_ET.cc:70-127: inner < |K| (K_perp) <= outer and
+// left < theta <= right with theta unfolded in [0, pi] (or h_lower < K_pll <= h_upper, K_pll signed); the
+// theta = 0 (K_pll = 0) line belongs to the first sector (slab).
+template <typename R>
+__global__ void k_fill_ring(GeoR g, const cplx<R>* __restrict__ F, cplx<R>* __restrict__ G, RingP rp, double lo,
+                            double hi, double e0, double e1)
+{
+    const long n = (long)g.N1 * g.N2 * g.H;
+    const double eps = 1e-15;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int l1, l2, l3;
+        decode_r(g, i, l1, l2, l3);
+        double kpll, kperp;
+        pll_perp_r(g, rp.axis, l1, l2, l3, kpll, kperp);
+        bool in;
+        if (rp.cyl) {
+            in = (kperp > lo) && (kperp <= hi) && (((kpll > e0) && (kpll <= e1)) || ((fabs(kpll) < eps) && (e0 < eps)));
+        } else {
+            const double km = kmag_r(g, l1, l2, l3);
+            const double th = polar_angle_r(kperp, kpll);
+            in = (km > lo) && (km <= hi) && (((th > e0) && (th <= e1)) || ((fabs(th) < eps) && (e0 < eps)));
+        }
+        G[i] = in ? F[i] : mk<R>(R(0), R(0));
+    }
+}
+
+// Local_ring_mult_all_FOUR four_ET.cc:259-299 (receivers 0 < |K| <= Max_radius_inside, Compute_ring_index
+// basis_basicfn.cc:329-366: the first radius >= |K|, the folded angle's sector, theta = 0 in sector 1) and
+// Local_cyl_ring_mult_all_FOUR :422-460 (K_perp <= Anis_max_Krho_radius_inside, Compute_cylinder_ring_index
+// :413-450: the first K_perp radius, the slab of |K_pll|, K_pll = 0 in slab 1); value 2 mult Re(A . B*).
+// A bin past the last column lands on the next row's column 0 and past the table it is dropped, as the
+// reference's unchecked Blitz store.
+template <typename R, bool CYL>
+__global__ __launch_bounds__(RBLK) void k_ring_mult(GeoR g, const cplx<R>* __restrict__ A0, const cplx<R>* __restrict__ A1,
+                                                    const cplx<R>* __restrict__ A2, const cplx<R>* __restrict__ B0,
+                                                    const cplx<R>* __restrict__ B1, const cplx<R>* __restrict__ B2,
+                                                    RingP rp, double* __restrict__ partial)
+{
+    extern __shared__ double sh[];
+    const int rows = rp.nr + 1, cols = rp.nsec + 1, nb = rows * cols;
+    for (int b = threadIdx.x; b < nb; b += RBLK) sh[b] = 0.0;
+    __syncthreads();
+    const long n = (long)g.N1 * g.N2 * g.H;
+    const double eps = 1e-15;
+    for (long i = (long)blockIdx.x * RBLK + threadIdx.x; i < n; i += (long)gridDim.x * RBLK) {
+        int l1, l2, l3;
+        decode_r(g, i, l1, l2, l3);
+        double kpll, kperp;
+        pll_perp_r(g, rp.axis, l1, l2, l3, kpll, kperp);
+        int row, col;
+        if (CYL) {
+            if (!(kperp <= rp.kpm)) continue;
+            for (row = 0; row < rows; row++)
+                if (kperp <= rp.r[row]) break;
+            const double ak = fabs(kpll);
+            if (ak < eps) {
+                col = 1;
+            } else {
+                for (col = 1; col < cols; col++)
+                    if (ak <= rp.e[col]) break;
+            }
+        } else {
+            const double km = kmag_r(g, l1, l2, l3);
+            if (!((km <= rp.kin) && (km > eps))) continue;
+            for (row = 0; row < rows; row++)
+                if (km <= rp.r[row]) break;
+            double th = polar_angle_r(kperp, kpll);
+            if (th > M_PI / 2) th = M_PI - th;
+            if (fabs(th) < eps) {
+                col = 1;
+            } else {
+                for (col = 0; col < cols; col++)
+                    if (th <= rp.e[col]) break;
+            }
+        }
+        const long flat = (long)row * cols + col;
+        if (flat >= nb) continue;
+        double d = rdot(A0[i], B0[i]);
+        if (A1) d += rdot(A1[i], B1[i]) + rdot(A2[i], B2[i]);
+        atomicAdd(&sh[flat], 2 * mult_r(g, l1, l2, l3) * d);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nb; b += RBLK) partial[(long)blockIdx.x * nb + b] = sh[b];
+}
+
 int RedScratch::ensure(size_t n_dev, size_t n_host)
 {
     if (n_dev > cap) {
@@ -415,6 +528,61 @@ static int nblocks_for(size_t n)
     size_t b = (n + RBLK * RSEG - 1) / (RBLK * RSEG);
     if (b > 1024) b = 1024;
     return (int)(b ? b : 1);
+}
+
+int ring_params(const Geo& g, int cyl, int axis, const double* radii, int nr, const double* edges, int nsec, RingP* rp)
+{
+    if (g.basis != 0 || g.N2 < 2 || g.N3 < 4) { set_error("ring transfers: 3-D FOUR boxes only"); return E_UNSUPPORTED; }
+    if (axis < 1 || axis > 3 || nr < 2 || nr > RING_MAX || nsec < 1 || nsec > RING_MAX || !radii || !edges) {
+        set_error("ring transfers: axis 1..3, 2 <= shells <= %d, 1 <= sectors <= %d", RING_MAX, RING_MAX);
+        return E_ARG;
+    }
+    rp->cyl = cyl ? 1 : 0;
+    rp->axis = axis;
+    rp->nr = nr;
+    rp->nsec = nsec;
+    rp->kin = max_radius_inside(g);
+    rp->kpm = cylinder_shell_ek_size(g, axis) - 1;     // Anis_max_Krho_radius_inside four_inline.h:517-546
+    for (int i = 0; i <= RING_MAX; i++) {
+        rp->r[i] = i <= nr ? radii[i] : 0.0;
+        rp->e[i] = i <= nsec ? edges[i] : 0.0;
+    }
+    return OK;
+}
+
+template <typename R>
+int fill_ring(const Geo& g, const cplx<R>* F, cplx<R>* G, const RingP& rp, int gs, int gq, hipStream_t st)
+{
+    const int nblk = nblocks_for(g.ncplx() / 4 + 1);
+    hipLaunchKernelGGL((k_fill_ring<R>), dim3(nblk), dim3(RBLK), 0, st, gr(g), F, G, rp, rp.r[gs - 1], rp.r[gs],
+                       rp.e[gq - 1], rp.e[gq]);
+    TH_CHECK_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename R>
+int reduce_ring_mult(const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3], const RingP& rp, double* out,
+                     RedScratch& rs, hipStream_t st)
+{
+    const int nb = (rp.nr + 1) * (rp.nsec + 1);
+    const int nblk = nblocks_for(g.ncplx() / 4 + 1);
+    int r = rs.ensure((size_t)nblk * nb + nb, nb);
+    if (r) return r;
+    double* dout = rs.d + (size_t)nblk * nb;
+    const size_t lds = nb * sizeof(double);
+    if (rp.cyl)
+        hipLaunchKernelGGL((k_ring_mult<R, true>), dim3(nblk), dim3(RBLK), lds, st, gr(g), A[0], A[1], A[2], B[0], B[1],
+                           B[2], rp, rs.d);
+    else
+        hipLaunchKernelGGL((k_ring_mult<R, false>), dim3(nblk), dim3(RBLK), lds, st, gr(g), A[0], A[1], A[2], B[0], B[1],
+                           B[2], rp, rs.d);
+    TH_CHECK_HIP(hipGetLastError());
+    launch_finalize(rs.d, nblk, nb, nb, dout, st);
+    TH_CHECK_HIP(hipGetLastError());
+    TH_CHECK_HIP(hipMemcpyAsync(rs.h, dout, nb * sizeof(double), hipMemcpyDeviceToHost, st));
+    TH_CHECK_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < nb; k++) out[k] = rs.h[k];
+    return OK;
 }
 
 int min_radius_outside(const Geo& g)
@@ -713,7 +881,10 @@ int reduce_aniso(const Geo& g, const cplx<R>* const V[3], int mode, int nbin, in
     template int reduce_shell_helicity<R>(const Geo&, const cplx<R>* const*, int, double*, double*, double*,         \
                                           RedScratch&, hipStream_t);                                                \
     template int reduce_aniso<R>(const Geo&, const cplx<R>* const*, int, int, int, int, double*, double*,             \
-                                 RedScratch&, hipStream_t);
+                                 RedScratch&, hipStream_t);                                                          \
+    template int fill_ring<R>(const Geo&, const cplx<R>*, cplx<R>*, const RingP&, int, int, hipStream_t);             \
+    template int reduce_ring_mult<R>(const Geo&, const cplx<R>* const*, const cplx<R>* const*, const RingP&, double*, \
+                                     RedScratch&, hipStream_t);
 TH_RINST(float)
 TH_RINST(double)
 

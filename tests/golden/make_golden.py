@@ -282,10 +282,112 @@ def et_triad():
     np.savez_compressed(os.path.join(HERE, "et_triad.npz"), **out)
 
 
+def ring_triad():
+    """Ring-to-ring and cylinder-ring-to-ring transfers (compute_ring_tr.cc:60-173, compute_cylin_ring_tr.cc:54-133)
+    by explicit triad sums over the full lattice, as et_triad: T(giver ring, receiver ring) = sign * sum over the
+    receiver ring of S_R(k | G) with G the giver field restricted to the giver ring.  Giver rings are selected on
+    the half spectrum (Fill_array_ring / Fill_array_cylinder_ring universal_ET.cc:70-127: inner < |k| <= outer
+    or inner < k_perp <= outer, and left < theta <= right with theta = Get_polar_angle(k_perp, k_pll) in
+    [0, pi] unfolded, or h_lower < k_pll <= h_upper signed; theta = 0 or k_pll = 0 joins the first sector /
+    slab), their conjugates following, and on the kz = 0 plane (G(k) + conj G(-k)) / 2; receivers are lattice modes with 0 < |k| <= kin (rings) or
+    k_perp <= kperp_max (cylinders) in the ring of their folded angle (Compute_ring_index basis_basicfn.cc:
+    329-366) or of |k_pll| (Compute_cylinder_ring_index :413-450)."""
+    N = (12, 12, 12)
+    M = 16
+    rng = np.random.default_rng(15)
+    kmax = (3, 3, 3)
+    U = _band_field(rng, N, kmax, True)
+    T = _band_field(rng, N, kmax, False)[0]
+    kin, kpm = 4, 4
+    radii = np.array([0.0, 2.0, 3.0, 10000.0])
+    sect = np.array([0.0, np.pi / 4, np.pi / 2])
+    slabs = np.array([0.0, 2.0, 4.0])
+    N1, N2, N3 = N
+    kxh = np.fft.fftfreq(N1, 1.0 / N1).reshape(-1, 1, 1) * np.ones((1, N2, N3 // 2 + 1))
+    kyh = np.fft.fftfreq(N2, 1.0 / N2).reshape(1, -1, 1) * np.ones((N1, 1, N3 // 2 + 1))
+    kzh = np.arange(N3 // 2 + 1).reshape(1, 1, -1) * np.ones((N1, N2, 1))
+    k1 = np.fft.fftfreq(M, 1.0 / M)
+    KX, KY, KZ = np.meshgrid(k1, k1, k1, indexing="ij")
+
+    def polar(kperp, kpll):
+        t = np.where(np.abs(kpll) > 1e-15, np.arctan(kperp / np.where(kpll == 0, 1, kpll)), np.pi / 2)
+        t = np.where(np.abs(kperp) > 1e-15, t, np.where(kpll >= 0, 0.0, np.pi))
+        return np.where(t >= 0, t, t + np.pi)
+
+    def pll_perp(kx, ky, kz, a):
+        c = (kx, ky, kz)
+        pll = c[a - 1]
+        perp = np.sqrt(sum(v * v for i, v in enumerate(c) if i != a - 1))
+        return pll, perp
+
+    def conv_nlin(H, G):
+        out = [np.zeros((M, M, M), complex) for _ in G]
+        idx = np.argwhere(np.abs(sum(np.abs(g) for g in G)) > 0)
+        for (a, b, c) in idx:
+            Hs = [np.roll(h, (a, b, c), axis=(0, 1, 2)) for h in H]
+            kh = 1j * (KX * Hs[0] + KY * Hs[1] + KZ * Hs[2])
+            for i, g in enumerate(G):
+                out[i] += g[a, b, c] * kh
+        return out
+
+    FU = [_full_grid(a, N, M) for a in U]
+    FT = [_full_grid(T, N, M)]
+    out = {"N": np.array(N), "U": np.stack(U), "T": T, "radii": radii, "sectors": sect, "slabs": slabs}
+    for cyl in (False, True):
+        for a in ((3, 1) if not cyl else (3, 2)):
+            hp, hq = pll_perp(kxh, kyh, kzh, a)
+            lp, lq = pll_perp(KX, KY, KZ, a)
+            Kh = np.sqrt(kxh ** 2 + kyh ** 2 + kzh ** 2)
+            Kl = np.sqrt(KX ** 2 + KY ** 2 + KZ ** 2)
+            edges = slabs if cyl else sect
+            if cyl:
+                rsel = lq <= kpm
+                rsh = np.searchsorted(radii, lq, side="left")
+                al = np.abs(lp)
+                rse = np.where(al < 1e-15, 1, np.searchsorted(edges, al, side="left"))
+                rse = np.maximum(rse, 1)
+            else:
+                rsel = (Kl <= kin) & (Kl > 1e-15)
+                rsh = np.searchsorted(radii, Kl, side="left")
+                th = polar(lq, lp)
+                th = np.where(th > np.pi / 2, np.pi - th, th)
+                rse = np.where(np.abs(th) < 1e-15, 1, np.searchsorted(edges, th, side="left"))
+            for name, Fsrc, Ffull, Rfull in (("ns", U, FU, FU), ("sf", [T], FT, FT)):
+                res = np.zeros((len(radii), len(edges), len(radii), len(edges)))
+                for gs in range(1, len(radii) - 1):
+                    for gq in range(1, len(edges)):
+                        if cyl:
+                            m = (hq > radii[gs - 1]) & (hq <= radii[gs])
+                            m &= ((hp > edges[gq - 1]) & (hp <= edges[gq])) | ((np.abs(hp) < 1e-15) & (edges[gq - 1] < 1e-15))
+                        else:
+                            m = (Kh > radii[gs - 1]) & (Kh <= radii[gs])
+                            t = polar(hq, hp)
+                            m &= ((t > edges[gq - 1]) & (t <= edges[gq])) | ((np.abs(t) < 1e-15) & (edges[gq - 1] < 1e-15))
+                        # A giver that breaks the kz = 0 plane's Hermitian pairs (axis 1 or 2: k_pll < 0 lies in
+                        # no sector) enters the inverse transform as FFTW's c2r takes it: the z line's DC
+                        # coefficient keeps its real part, i.e. the plane acts as (G(k) + conj G(-k)) / 2.
+                        G = []
+                        for f in Fsrc:
+                            h = f * m
+                            p0 = h[:, :, 0]
+                            mirror = np.conj(p0[(-np.arange(N1)) % N1][:, (-np.arange(N2)) % N2])
+                            h[:, :, 0] = 0.5 * (p0 + mirror)
+                            G.append(_full_grid(h, N, M))
+                        Nl = conv_nlin(FU, G)
+                        Sr = sum((n * np.conj(r)).real for n, r in zip(Nl, Rfull))
+                        for rs in range(len(radii)):
+                            for rq in range(len(edges)):
+                                sel = rsel & (rsh == rs) & (rse == rq)
+                                res[gs, gq, rs, rq] = -Sr[sel].sum()
+                out[f"{'cyl' if cyl else 'ring'}_{name}_axis{a}"] = res
+    np.savez_compressed(os.path.join(HERE, "ring_triad.npz"), **out)
+
+
 if __name__ == "__main__":
     four_fft()
     scft_fft()
     kat_out1()
     nlin_triad()
     et_triad()
+    ring_triad()
     print("golden fixtures written to", HERE)

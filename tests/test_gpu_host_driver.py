@@ -42,8 +42,9 @@ def f12(a, b, sc=0.0, rel=1e-10):
 
 
 def blitz_2d(lines, i):
-    """Parse one Blitz++ 2-D array printed as 'R x C', '[ v v ... ]' from lines[i]; returns (array, next i)."""
-    r, c = (int(v) for v in lines[i].split(" x "))
+    """Parse one Blitz++ array printed as its extents 'R x C [x ...]', then '[ v v ... ]', from lines[i];
+    returns (array, next i)."""
+    dims = [int(v) for v in lines[i].split(" x ")]
     vals = []
     i += 1
     while True:
@@ -52,8 +53,8 @@ def blitz_2d(lines, i):
         if "]" in lines[i]:
             break
         i += 1
-    assert len(vals) == r * c, (r, c, len(vals))
-    return np.array(vals).reshape(r, c), i + 1
+    assert len(vals) == int(np.prod(dims)), (dims, len(vals))
+    return np.array(vals).reshape(dims), i + 1
 
 
 def read_shell_d(path):
@@ -195,6 +196,67 @@ def test_driver_output_files_match_oracle(tmp_path):
             assert g6(Tm[n - 1, m - 1], Tref[n, m], sc), (n, m, Tm[n - 1, m - 1], Tref[n, m])
 
 
+def read_blocks(path):
+    """A ring_to_ring.d-like file: '%% Time = t', then '%% NAME: ' lines each followed by a Blitz++ array."""
+    lines = path.read_text().split("\n")
+    out, i = [], 0
+    while i < len(lines):
+        if lines[i].startswith("%% Time = "):
+            out.append((float(lines[i][10:]), {}))
+            i += 1
+        elif lines[i].startswith("%% ") and lines[i].endswith(": "):
+            name = lines[i][3:-2]
+            a, i = blitz_2d(lines, i + 1)
+            out[-1][1][name] = a
+        else:
+            i += 1
+    return out
+
+
+@pytest.mark.parametrize("physics", ["ns", "mhd"])
+def test_driver_ring_to_ring_files(tmp_path, physics):
+    """ring=NRING,NSECT,NCYL,NSLAB,AXIS,SCHEME with out=DIR: Init_ring_tr (unit radii, sectors over [0, pi/2],
+    slabs over [0, Anis_max_Kpll]) and Output_ring_to_ring / Output_cylinder_ring_to_ring (output_ET.cc:427-760):
+    every table of ring_to_ring.d and cyl_ring_to_ring.d (U to U; for MHD also W to W, U to W and the Elsasser
+    pair) equals the oracle's ring transfer over giver and receiver rings 1..n-1 x 1..m to the printed digits;
+    the force feeds of the unforced run are 0."""
+    N, steps, dt, nu, eta = 32, 2, 1e-3, 0.01, 0.02
+    (tmp_path / "out").mkdir()
+    extra = [f"mhd={eta}"] if physics == "mhd" else []
+    run_driver(N, steps, dt, nu, "random", "FP64", "fused", f"out={tmp_path}", "ring=5,3,5,3,1,1", *extra)
+    o = Oracle((N, N, N))
+    U, B = o.ic_random(1), o.ic_random(2)
+    if physics == "ns":
+        o.rk4_ns(U, nu, dt, steps=steps)
+    else:
+        o.rk4_mhd(U, B, nu, eta, dt, steps=steps)
+    radii = np.array([0.0, 1, 2, 3, 4, 10000.0])
+    sect = np.array([0.0, np.pi / 6, np.pi / 3, np.pi / 2])
+    kp = (N // 3) * 1.0
+    slabs = np.array([0.0, kp / 3, 2 * kp / 3, kp])
+    Zp = [u + b for u, b in zip(U, B)]
+    Zm = [u - b for u, b in zip(U, B)]
+    for fname, cyl, edges in (("ring_to_ring.d", False, sect), ("cyl_ring_to_ring.d", True, slabs)):
+        blocks = read_blocks(tmp_path / "out" / fname)
+        assert len(blocks) == 1 and abs(blocks[0][0] - steps * dt) < 1e-9
+        got = blocks[0][1]
+        refs = {"U to U": o.ring_tr(U, U, U, radii, edges, cyl=cyl, axis=1)}
+        if physics == "mhd":
+            refs["W to W"] = o.ring_tr(U, B, B, radii, edges, cyl=cyl, axis=1)
+            refs["U to W"] = o.ring_tr(B, U, B, radii, edges, sign=1.0, cyl=cyl, axis=1)
+            refs["Zp to Zp"] = o.ring_tr(Zm, Zp, Zp, radii, edges, cyl=cyl, axis=1)
+            refs["Zm to Zm"] = o.ring_tr(Zp, Zm, Zm, radii, edges, cyl=cyl, axis=1)
+        for name, ref in refs.items():
+            t = got[name]
+            sub = ref[1:5, 1:4, 1:5, 1:4]
+            assert t.shape == (4, 3, 4, 3) and np.abs(sub).max() > 0
+            sc = np.abs(sub).max()
+            for a, b in zip(t.ravel(), sub.ravel()):
+                assert g6(a, b, sc), (fname, name, a, b)
+        feeds = [k for k in got if k.startswith("Force feed")]
+        assert feeds and all(np.all(got[k] == 0) and got[k].shape == (4, 3) for k in feeds)
+
+
 def test_driver_et_scheme0_radii_and_flux():
     """EnergyTr with shell_input_scheme 0 (EnergyTr.cc:136-160): radii 0, 2, 4, 8, log-spaced, kmax/2, kmax,
     INF_RADIUS for no_shells = 8 at 64^3 (kmax_inside 21 > 16), equal to the pinned oracle radii, and
@@ -248,8 +310,8 @@ def test_driver_get_dt_energy_branch(tmp_path, physics):
             tdt = float(line.split("\t")[1].split("    ")[7])
         else:
             tdt = float(line.split("\t")[1].split()[22])
-        assert abs(tdt - dts[i]) <= 1e-11 * dts[i], (i, tdt, dts[i])
-        assert abs(float(line.split("\t")[0]) - sum(dts[:i + 1])) <= 1e-11
+        assert f12(tdt, dts[i], rel=1e-11), (i, tdt, dts[i])
+        assert f12(float(line.split("\t")[0]), sum(dts[:i + 1]), rel=1e-11)
     assert np.abs(glob[:, 1] - np.array(E)).max() <= 1e-12 * max(E)
 
 

@@ -270,6 +270,29 @@ def test_flux_and_shell_transfer_plain(ctx):
     assert np.abs(T - T_ref).max() < 1e-12 * np.abs(T_ref).max()
 
 
+@pytest.mark.parametrize("cyl,axis", [(False, 3), (False, 1), (True, 3), (True, 2)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_ring_tr_plain(ctx, cyl, axis, prec, tol):
+    """th_ring_tr (Compute_ring_tr compute_ring_tr.cc:60-291, Compute_cylinder_ring_tr
+    compute_cylin_ring_tr.cc:54-133): U to U, the scalar T to T and the MHD U to W matrices (helper, giver,
+    receiver and sign of each) match the oracle over shells x sectors (slabs)."""
+    from tarang_amd import Plan
+    N = (24, 24, 24)
+    o = Oracle(N)
+    p = Plan(ctx, N, precision=prec)
+    U, B = o.ic_random(2), o.ic_random(3)
+    T = o.dealias(o.ic_random_scalar(4, 0.3))
+    radii = np.array([0.0, 2.0, 4.0, 6.0, 10000.0])
+    edges = np.array([0.0, 3.0, 5.0, 8.0]) if cyl else np.array([0.0, np.pi / 6, np.pi / 3, np.pi / 2])
+    d = lambda X: [p.to_dev(a) for a in X]
+    for H, F, R, sign in ((U, U, U, -1.0), (U, [T], [T], -1.0), (B, U, B, 1.0)):
+        ref = o.ring_tr(H, F, R, radii, edges, sign=sign, cyl=cyl, axis=axis)
+        got = p.ring_tr(d(H), d(F), d(R), radii, edges, sign=sign, cyl=cyl, axis=axis)
+        sc = np.abs(ref).max()
+        assert sc > 0
+        assert np.abs(got - ref).max() <= tol * sc, (len(F), sign)
+
+
 @pytest.mark.parametrize("N", [(16, 16, 16), (24, 16, 32)])
 def test_flux_mhd_plain(ctx, N):
     """th_flux_mhd (IncVF::Compute_flux(IncVF& W), compute_flux.cc:148-262): the nine MHD and
