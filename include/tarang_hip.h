@@ -179,6 +179,12 @@ int th_compute_force_mhd(th_plan *plan, void *const U[3], void *const B[3], void
 int th_energy(th_plan *plan, const void *A, const void *B, double *out);
 int th_Sn(th_plan *plan, const void *A, double n, double *out);
 int th_shell_spectrum(th_plan *plan, const void *A, const void *B, int n, double *Sk, double *count);
+/* th_shell_spectrum_noalloc: no counterpart in the reference.  The same shell spectrum as th_shell_spectrum,
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): it allocates no device or host memory.  Its partial sums live in a static device array of
+ * 8.4 MB that each GPU holds until the process ends; shell_ek_size above 512 is refused (E_ARG).  Calls that
+ * overlap on one GPU share that array and race.  Synchronises. */
+int th_shell_spectrum_noalloc(th_plan *plan, const void *A, const void *B, int n, double *Sk, double *count);
 int th_helicity(th_plan *plan, void *const V[3], double *out);
 int th_shell_spectrum_helicity(th_plan *plan, void *const V[3], double *H1k1, double *H1k2, double *H1k3);
 /* Anisotropic spectra of V about the axis globalvar_anisotropy_switch = axis (1 x, 2 y, 3 z), FOUR 3-D:

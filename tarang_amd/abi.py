@@ -113,6 +113,7 @@ def lib():
         "th_entropy": (i, [vp, pp, dp]),
         "th_shell_tr_scalar": (i, [vp, pp, vp, dp, i, dp, dp]),
         "th_shell_tr_mhd": (i, [vp, pp, pp, dp, i, dp]),
+        "th_shell_spectrum_noalloc": (i, [vp, vp, vp, i, dp, dp]),
         "th_ring_tr": (i, [vp, i, i, pp, pp, pp, i, dp, i, dp, i, i, d, dp]),
         "th_ring_mult_all": (i, [vp, i, i, pp, pp, i, dp, i, dp, i, dp]),
         "th_ub_to_elsasser": (i, [vp, pp, pp, i]),

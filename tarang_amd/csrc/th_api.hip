@@ -953,6 +953,17 @@ extern "C" int th_shell_spectrum(th_plan* p, const void* A, const void* B, int n
     if (!A || !Sk || !(n == 0 || n == 2 || n == 4)) { set_error("th_shell_spectrum: bad args"); return E_ARG; }
     return DISPATCH(p, do_spec, p, A, B, n, Sk, count);
 }
+template <typename R> static int do_spec_noalloc(th_plan* p, const void* A, const void* B, int n, double* Sk, double* cnt)
+{
+    return reduce_shell_noalloc<R>(p->g, CCP(R, A), CCP(R, B), (double)n, min_radius_outside(p->g) + 1, Sk, cnt,
+                                   p->ctx->stream);
+}
+extern "C" int th_shell_spectrum_noalloc(th_plan* p, const void* A, const void* B, int n, double* Sk, double* count)
+{
+    TRY(check_plan(p));
+    if (!A || !Sk || !(n == 0 || n == 2 || n == 4)) { set_error("th_shell_spectrum_noalloc: bad args"); return E_ARG; }
+    return DISPATCH(p, do_spec_noalloc, p, A, B, n, Sk, count);
+}
 template <typename R> static int do_helicity(th_plan* p, void* const V[3], double* out)
 {
     const cplx<R>* v[3] = {CCP(R, V[0]), CCP(R, V[1]), CCP(R, V[2])};

@@ -120,6 +120,9 @@ template <typename R> int fill_ring(const Geo& g, const cplx<R>* F, cplx<R>* G, 
 // out[(nr+1)(nsec+1)] = Ring_mult_all / Cyl_ring_mult_all of A, B (1 or 3 components; A[1] == nullptr: one)
 template <typename R> int reduce_ring_mult(const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3],
                                            const RingP& rp, double* out, RedScratch& rs, hipStream_t st);
+// reduce_shell with no allocation: partials and results in a file-scope device array (th_reduce.hip)
+template <typename R> int reduce_shell_noalloc(const Geo& g, const cplx<R>* A, const cplx<R>* B, double n, int nbins,
+                                               double* Sk, double* cnt, hipStream_t st);
 template <typename R> int reduce_energy(const Geo& g, const cplx<R>* A, const cplx<R>* B, double* out,
                                         RedScratch& rs, hipStream_t st);
 template <typename R> int reduce_Sn(const Geo& g, const cplx<R>* A, double n, double* out, RedScratch& rs,

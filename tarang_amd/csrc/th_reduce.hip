@@ -8,6 +8,7 @@
 // shell index ceil(K) is bit-identical to the CPU's.  Per-block partial sums are FP64 (LDS atomics
 // for binned sums, tree reduction for scalars); blocks are merged with Neumaier-compensated sums.
 #include <math.h>
+#include <mutex>
 #include "th_internal.hpp"
 
 namespace th {
@@ -646,6 +647,85 @@ int reduce_shell(const Geo& g, const cplx<R>* A, const cplx<R>* B, double n, int
     return OK;
 }
 
+// ---- th_shell_spectrum_noalloc: the shell spectrum with no device or host allocation ----
+// Its per-block partials (nblocks_for(n) <= 1024 blocks x 2 NOALLOC_BINS values) and finalized sums and
+// counts (2 NOALLOC_BINS values) live in this file-scope array, one copy per device for the life of the
+// process (8.4 MB).  Host code never passes the array's name to a kernel: each device's own address comes
+// from hipGetSymbolAddress, looked up once per device with that device current.  Calls that overlap on one
+// device share the array and race.
+constexpr int NOALLOC_BINS = 512;
+constexpr size_t NOALLOC_DOUBLES = (size_t)1024 * 2 * NOALLOC_BINS + 2 * NOALLOC_BINS;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+static int noalloc_scratch(double** out)
+{
+    int dev = -1;
+    TH_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) { set_error("noalloc scratch: device %d out of range", dev); return E_ARG; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error("hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            return E_HIP;
+        }
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return OK;
+}
+
+// The partial-shell rescale of four_energy.cc:324-336 on the device, with the host formula's operation order
+// (Sk * Approx_number_modes_in_shell / Sk_count), so the values equal those of reduce_shell bit for bit.
+__global__ void k_rescale_shells(double* __restrict__ sums, int nbins, int kin, int kkmax, int basis, double kf0,
+                                 double kf1, double kf2)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbins) return;
+    const double c = sums[nbins + k];
+    if (k > kin && k <= kkmax && c >= 1) {
+        double v = M_PI * k * k / (kf0 * kf1 * kf2);
+        double a = basis == 0 ? 2 * v : v;
+        sums[k] = sums[k] * a / c;
+    }
+}
+
+template <typename R>
+int reduce_shell_noalloc(const Geo& g, const cplx<R>* A, const cplx<R>* B, double n, int nbins, double* Sk, double* cnt,
+                         hipStream_t st)
+{
+    int nb = nblocks_for(g.ncplx());
+    int kkmax = min_radius_outside(g);
+    if (nbins != kkmax + 1) { set_error("reduce_shell_noalloc: nbins %d != shell_ek_size %d", nbins, kkmax + 1); return E_ARG; }
+    if (nbins > NOALLOC_BINS || (size_t)nb * 2 * nbins + 2 * nbins > NOALLOC_DOUBLES) {
+        set_error("th_shell_spectrum_noalloc: shell_ek_size %d exceeds the maximum %d of its static scratch", nbins,
+                  NOALLOC_BINS);
+        return E_ARG;
+    }
+    double* scr = nullptr;
+    TRY(noalloc_scratch(&scr));
+    double* out = scr + (size_t)nb * 2 * nbins;
+    // every block writes all 2 nbins of its partial row, and the finalize writes all of `out`: nothing here
+    // reads what an earlier call left in the array
+    hipLaunchKernelGGL((k_binned<R, 0>), dim3(nb), dim3(RBLK), 2 * nbins * sizeof(double), st, gr(g), A,
+                       (const cplx<R>*)nullptr, (const cplx<R>*)nullptr, B, (const cplx<R>*)nullptr,
+                       (const cplx<R>*)nullptr, (int)n, kkmax, (const double*)nullptr, 0, nbins, scr);
+    TH_CHECK_HIP(hipGetLastError());
+    launch_finalize(scr, nb, 2 * nbins, 2 * nbins, out, st);
+    TH_CHECK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_rescale_shells, dim3((nbins + 255) / 256), dim3(256), 0, st, out, nbins, max_radius_inside(g),
+                       kkmax, g.basis, g.kf[0], g.kf[1], g.kf[2]);
+    TH_CHECK_HIP(hipGetLastError());
+    TH_CHECK_HIP(hipMemcpyAsync(Sk, out, nbins * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (cnt) TH_CHECK_HIP(hipMemcpyAsync(cnt, out + nbins, nbins * sizeof(double), hipMemcpyDeviceToHost, st));
+    TH_CHECK_HIP(hipStreamSynchronize(st));
+    return OK;
+}
+
 template <typename R>
 static int run_scalar(int mode, const Geo& g, const cplx<R>* const A[3], const cplx<R>* const B[3], int npow,
                       double inner, double outer, double* out, RedScratch& rs, hipStream_t st)
@@ -882,6 +962,8 @@ int reduce_aniso(const Geo& g, const cplx<R>* const V[3], int mode, int nbin, in
                                           RedScratch&, hipStream_t);                                                \
     template int reduce_aniso<R>(const Geo&, const cplx<R>* const*, int, int, int, int, double*, double*,             \
                                  RedScratch&, hipStream_t);                                                          \
+    template int reduce_shell_noalloc<R>(const Geo&, const cplx<R>*, const cplx<R>*, double, int, double*, double*, \
+                                         hipStream_t);                                                               \
     template int fill_ring<R>(const Geo&, const cplx<R>*, cplx<R>*, const RingP&, int, int, hipStream_t);             \
     template int reduce_ring_mult<R>(const Geo&, const cplx<R>* const*, const cplx<R>* const*, const RingP&, double*, \
                                      RedScratch&, hipStream_t);
