@@ -197,6 +197,14 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
     cplx<R> twk[E];   // W_N3^(t + P m)
 #pragma unroll
     for (int m = 0; m < E; m++) twk[m] = tws[LZ + t + P * m];
+    // The state is 2/3-dealiased (th_state_create refuses other plans): a line holds kz < NKZ = N3/3, so
+    // register m (points k = t + P m) is zero for every lane when P m >= NKZ, and its mirror LZ - k is
+    // zero for every lane when LZ - (P - 1) - P m >= NKZ (LZ itself, the kz = N3/2 coefficient, included).
+    // Both are known at compile time: those loads, stores and terms are left out.
+    constexpr int NKZ = (2 * LZ) / 3;
+    auto zin = [](int m) constexpr { return P * m >= NKZ; };
+    auto zmir = [](int m) constexpr { return LZ - (P - 1) - P * m >= NKZ; };
+    static_assert(LZ - (P - 1) >= NKZ, "register 0's mirrors (position LZ included) must be dealiased zeros");
     cplx<R> u[ZP::NIN][E];
 #pragma unroll
     for (int a = 0; a < ZP::NIN; a++) {
@@ -205,23 +213,30 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const int k = t + P * m;
-            xa[m] = (ok && k < g.nkz) ? X[k] : czero<R>();
+            xa[m] = (!zin(m) && ok && k < NKZ) ? X[k] : czero<R>();
         }
-        const cplx<R> xny = (ok && t == 0 && LZ < g.nkz) ? X[LZ] : czero<R>();   // kz = N3/2 (undealiased)
         wave_sync();
+        // every position is written (the zeros too): mixed registers read mirrors up to LZ - NKZ + P
 #pragma unroll
         for (int m = 0; m < E; m++) Z[t + P * m] = xa[m];
-        if (t == 0) Z[LZ] = xny;
         wave_sync();
 #pragma unroll
         for (int m = 0; m < E; m++) {
-            cplx<R> xb = Z[LZ - t - P * m];
-            cplx<R> xv = xa[m];
-            if (t == 0 && m == 0) { xv.y = R(0); xb.y = R(0); }
-            xb.y = -xb.y;
-            cplx<R> s = xv + xb, d = xv - xb;
             cplx<R> w = conjg(twk[m]);
-            u[a][zreg<E, FI::R0>(m)] = s + mul_i(cmul(w, d));
+            if (zmir(m)) {                    // s = d = x_k
+                cplx<R> xv = xa[m];
+                if (t == 0 && m == 0) xv.y = R(0);
+                u[a][zreg<E, FI::R0>(m)] = xv + mul_i(cmul(w, xv));
+            } else if (zin(m)) {              // s = conj(x_{LZ-k}), d = -s
+                cplx<R> xb = conjg(Z[LZ - t - P * m]);
+                u[a][zreg<E, FI::R0>(m)] = xb - mul_i(cmul(w, xb));
+            } else {
+                cplx<R> xb = Z[LZ - t - P * m];
+                cplx<R> xv = xa[m];
+                xb.y = -xb.y;
+                cplx<R> s = xv + xb, d = xv - xb;
+                u[a][zreg<E, FI::R0>(m)] = s + mul_i(cmul(w, d));
+            }
         }
         FI::template run<1, true, Lay>(u[a], Z, t, tws);
     }
@@ -241,8 +256,10 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
         const cplx<R>* twp = tw + LZ;   // W_N3^(t + P m)
         cplx<R>* X = Cout + p * g.cB + off;
         wave_sync();
+        // only mirrors LZ - k of kept k < NKZ are read: positions above LZ - NKZ
 #pragma unroll
-        for (int m = 0; m < E; m++) Z[t + P * m] = w[zreg<E, FF::RL>(m)];
+        for (int m = 0; m < E; m++)
+            if (P * m + (P - 1) > LZ - NKZ) Z[t + P * m] = w[zreg<E, FF::RL>(m)];
         if (t == 0) Z[LZ] = w[zreg<E, FF::RL>(0)];
         wave_sync();
         auto unpack = [&](int m) {
@@ -252,28 +269,27 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
             cplx<R> tt = cmul(twp[t + P * m], zk - zc);
             return mk<R>(e.x + R(0.5) * tt.y, e.y - R(0.5) * tt.x);
         };
+        // registers whose points all lie at kz >= NKZ are neither unpacked nor stored
         if constexpr (sizeof(R) == 4) {
             // unpack every point first, then store: no LDS read waits between the stores
             cplx<R> o[E];
 #pragma unroll
-            for (int m = 0; m < E; m++) o[m] = unpack(m);
+            for (int m = 0; m < E; m++)
+                if (!zin(m)) o[m] = unpack(m);
 #pragma unroll
             for (int m = 0; m < E; m++) {
                 const int k = t + P * m;
-                if (ok && k < g.nkz) X[k] = o[m];
+                if (!zin(m) && ok && k < NKZ) X[k] = o[m];
             }
         } else {
             // FP64: holding every unpacked point would cost 4 E VGPRs
 #pragma unroll
             for (int m = 0; m < E; m++) {
+                if (zin(m)) continue;
                 const int k = t + P * m;
                 const cplx<R> o = unpack(m);
-                if (ok && k < g.nkz) X[k] = o;
+                if (ok && k < NKZ) X[k] = o;
             }
-        }
-        if (ok && t == 0 && LZ < g.nkz) {   // kz = N3/2 (undealiased): Y[0] - ... with W_N3^(N3/2) = -1
-            cplx<R> z0 = w[zreg<E, FF::RL>(0)];
-            X[LZ] = mk<R>(z0.x - z0.y, R(0));
         }
     };
     if constexpr (sizeof(R) == 4) {
@@ -984,6 +1000,10 @@ static int launch_z(const FG& g, const cplx<R>* B, cplx<R>* Cc, const cplx<R>* t
     constexpr int E = (V == 3) ? 16 : pts_per_thread(LZ), P = LZ / E;
     constexpr int NT = (V == 1) ? 128 : (V == 2) ? 512 : 256;
     constexpr int TZ = (NT / P) < 1 ? 1 : NT / P;
+    if (g.nkz != (2 * LZ) / 3) {   // k_fz takes the kept kz < N3/3 of a dealiased state at compile time
+        set_error("fused z pass: %d kept kz for N3 = %d (a dealiased state keeps %d)", g.nkz, 2 * LZ, (2 * LZ) / 3);
+        return E_UNSUPPORTED;
+    }
     const long lines = (long)g.N1 * g.N2;
     hipLaunchKernelGGL((k_fz<R, LZ, P, TZ, MODE>), dim3((unsigned)((lines + TZ - 1) / TZ), g.nbox), dim3(TZ * P), 0, st, g, B, Cc,
                        twL, twN);
