@@ -295,6 +295,11 @@ int th_state_set_force(th_state *state, const th_force *force);
 /* Pi(R_i) for spheres radii[1..nsh] and T(n,m) for giver shells n = 1..nsh-1 over the state's
  * velocity (Compute_flux + Compute_shell_tr), from one giver-masked fused pipeline per shell. */
 int th_flux_shell_tr(th_plan *plan, th_state *state, const double *radii, int nsh, double *Pi, double *T);
+/* th_flux_mhd_fused == IncVF::Compute_flux(IncVF& W) compute_flux.cc:148-262 on a fused MHD state (U, B): the
+ * nine rows of th_flux_mhd (9 x (nsph+1), same order) for spheres radii[1..nsph] (radii[nsph] = INF_RADIUS),
+ * built from giver shells through the fused pipeline: four helper x giver pipelines (H, G in {U, B}), each
+ * binned against both receivers, the Elsasser rows from the bilinearity of N[H, G].  Synchronises. */
+int th_flux_mhd_fused(th_plan *plan, th_state *state, const double *radii, int nsph, double *out);
 
 /* Measurement only (no counterpart in the reference): runs RK stage `stage` (1..4) of the fused
  * pipeline nrep times pass by pass, timing each of the 5 passes with HIP events on the plan's

@@ -117,6 +117,7 @@ def lib():
         "th_ring_tr": (i, [vp, i, i, pp, pp, pp, i, dp, i, dp, i, i, d, dp]),
         "th_ring_mult_all": (i, [vp, i, i, pp, pp, i, dp, i, dp, i, dp]),
         "th_ub_to_elsasser": (i, [vp, pp, pp, i]),
+        "th_flux_mhd_fused": (i, [vp, vp, dp, i, dp]),
         "th_nusselt": (i, [vp, pp, vp, d, d, dp]),
         "th_ic_taylor_green": (i, [vp, pp, i, d]),
         "th_ic_random": (i, [vp, pp, C.c_ulonglong, d, d]),
@@ -751,6 +752,16 @@ class State:
         v = (C.c_int * 5)(*[int(x) for x in var])
         self.plan._pre()
         _check(lib().th_state_set_variants(self.h, v), "th_state_set_variants")
+
+    def flux_mhd(self, radii):
+        """th_flux_mhd_fused: the nine MHD flux rows (th_flux_mhd's order) of the state's U and B."""
+        import numpy as np
+        r, rp = _dbl(radii)
+        out = np.zeros((9, len(r)))
+        self.plan._pre()
+        _check(lib().th_flux_mhd_fused(self.plan.h, self.h, rp, len(r) - 1, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "th_flux_mhd_fused")
+        return out
 
     def flux_shell_tr(self, radii):
         import numpy as np

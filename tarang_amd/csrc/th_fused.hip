@@ -575,7 +575,15 @@ extern "C" int th_step(th_plan* p, th_state* s, double dt, const th_phys* ph, in
 // T(n, m) (Shell_mult_all, |K| <= kmax_inside) and the same sums over |K| > kmax_inside ("corner",
 // which Shell_mult_single counts); then Pi(R_i) = sum_{n<=i} sum_{m>i} [T(n,m) - corner(n,m)]
 // (sphere i = union of shells 1..i and N[G] is linear in G).
-template <typename R> static int flux_impl(th_state* s, const double* radii, int nsh, double* Pi, double* T)
+// Per giver shell n (inner radii[n-1] < |K| <= radii[n], n = 1..ngiv) of the energy-transfer pipeline with
+// helper H = state components hoff..hoff+2, giver G = components goff..goff+2 and each receiver R_r =
+// components roff[r]..roff[r]+2 of the state, the sums over the modes of receiver shell m (Get_shell_index
+// basis_basicfn.cc:288-299 on |K|) of 2 mult Re(N[H, G_n] . R_r*), N = EnergyTr_Compute_nlin
+// (ET_compute_nlin.cc:85-120), split as Shell_mult_all splits them: Yb over |K| <= kmax_inside
+// (four_ET.cc:142) and Yc over the corners above it.  Rows are [nr], nr = nsh + 1, indexed n * nr + m.
+template <typename R>
+static int et_shell_rows(th_state* s, const double* radii, int nsh, int ngiv, int hoff, int goff, const int* roff, int nrecv,
+                         std::vector<double>* Yb, std::vector<double>* Yc)
 {
     th_plan* p = s->plan;
     hipStream_t st = p->ctx->stream;
@@ -607,29 +615,49 @@ template <typename R> static int flux_impl(th_state* s, const double* radii, int
     }
     const int kin = max_radius_inside(p->g);
     ShellSel all{-1.0, 1e300};
-    // helper H = U: inverse x, y once, kept in X2 comps 0..2 (B layout)
-    TRY((d_xinv<R, 0>(g, S, X1, 3, twx, all, 0, st)));
+    // helper H: inverse x, y once, kept in X2 comps 0..2 (B layout)
+    TRY((d_xinv<R, 0>(g, S + (long)hoff * g.cS, X1, 3, twx, all, 0, st)));
     // the y passes take the step's tuned launch shapes (the same kernels, any component count)
     TRY(d_yinv<R>(g, X1, X2, 3, twy, s->var[PASS_YINV], st));
     std::vector<double> hostbuf(2 * nr);
-    std::vector<double> Tm((size_t)nr * nr, 0.0), corner((size_t)nr * nr, 0.0);
+    for (int r = 0; r < nrecv; r++) {
+        Yb[r].assign((size_t)(ngiv + 1) * nr, 0.0);
+        Yc[r].assign((size_t)(ngiv + 1) * nr, 0.0);
+    }
     double* outd = s->partial + (size_t)nblk * (2 * nr);
-    for (int n = 1; n < nsh; n++) {
+    for (int n = 1; n <= ngiv; n++) {
         ShellSel sel{radii[n - 1], radii[n]};
-        TRY((d_xinv<R, 2>(g, S, X1, 3, twx, sel, 0, st)));
+        TRY((d_xinv<R, 2>(g, S + (long)goff * g.cS, X1, 3, twx, sel, 0, st)));
         TRY(d_yinv<R>(g, X1, X2 + 3 * g.cB, 3, twy, s->var[PASS_YINV], st));
         TRY((d_z<R, 2>(g, X2, X3, twz, twn, 0, st)));
         TRY(d_yfwd<R>(g, X3, X1, 9, twy, s->var[PASS_YFWD], st));
-        TRY(d_xfwd_et<R>(g, X1, S, twx, s->radii, nr, kin, s->partial, &nblk, st));
-        launch_finalize((const double*)s->partial, nblk, 2 * nr, 2 * nr, outd, st);
-        TH_CHECK_HIP(hipGetLastError());
-        TH_CHECK_HIP(hipMemcpyAsync(hostbuf.data(), outd, 2 * nr * sizeof(double), hipMemcpyDeviceToHost, st));
-        TH_CHECK_HIP(hipStreamSynchronize(st));
-        for (int m = 0; m < nr; m++) {
-            Tm[(size_t)n * nr + m] = -hostbuf[m];
-            corner[(size_t)n * nr + m] = hostbuf[nr + m];
+        for (int r = 0; r < nrecv; r++) {
+            TRY(d_xfwd_et<R>(g, X1, S + (long)roff[r] * g.cS, twx, s->radii, nr, kin, s->partial, &nblk, st));
+            launch_finalize((const double*)s->partial, nblk, 2 * nr, 2 * nr, outd, st);
+            TH_CHECK_HIP(hipGetLastError());
+            TH_CHECK_HIP(hipMemcpyAsync(hostbuf.data(), outd, 2 * nr * sizeof(double), hipMemcpyDeviceToHost, st));
+            TH_CHECK_HIP(hipStreamSynchronize(st));
+            for (int m = 0; m < nr; m++) {
+                Yb[r][(size_t)n * nr + m] = hostbuf[m];
+                Yc[r][(size_t)n * nr + m] = hostbuf[nr + m];
+            }
         }
     }
+    return OK;
+}
+
+// Compute_flux / Compute_shell_tr (compute_flux.cc:52-69, compute_shell_tr.cc:59-79) from shell givers:
+// T(n,m) = -Yb(n,m) (Shell_mult_all, |K| <= kmax_inside) and Pi(R_i) = sum_{n<=i} sum_{m>i} [T(n,m) - Yc(n,m)]
+// (Shell_mult_single over |K| > R_i counts the corners too), Pi(INF) = 0.
+template <typename R> static int flux_impl(th_state* s, const double* radii, int nsh, double* Pi, double* T)
+{
+    const int nr = nsh + 1;
+    std::vector<double> Yb, Yc;
+    const int roff = 0;
+    TRY(et_shell_rows<R>(s, radii, nsh, nsh - 1, 0, 0, &roff, 1, &Yb, &Yc));
+    std::vector<double> Tm((size_t)nr * nr, 0.0);
+    for (int n = 1; n < nsh; n++)
+        for (int m = 0; m < nr; m++) Tm[(size_t)n * nr + m] = -Yb[(size_t)n * nr + m];
     if (T)
         for (size_t i = 0; i < (size_t)nr * nr; i++) T[i] = Tm[i];
     if (Pi) {
@@ -637,11 +665,73 @@ template <typename R> static int flux_impl(th_state* s, const double* radii, int
         for (int i = 1; i <= nsh; i++) {
             double acc_ = 0;
             for (int n = 1; n <= i && n < nsh; n++)
-                for (int m = i + 1; m < nr; m++) acc_ += Tm[(size_t)n * nr + m] - corner[(size_t)n * nr + m];
+                for (int m = i + 1; m < nr; m++) acc_ += Tm[(size_t)n * nr + m] - Yc[(size_t)n * nr + m];
             Pi[i] = (i >= nsh) ? 0.0 : acc_;
         }
     }
     return OK;
+}
+
+// IncVF::Compute_flux(IncVF& W) compute_flux.cc:148-262 on the fused MHD state (U = comps 0..2, B = 3..5):
+// for helper H and giver G in {U, B} and receiver R in {U, B}, Y_HGR(n, m) = Yb + Yc of et_shell_rows, and
+// out_HGR(i) = sum_{n<=i} sum_{m>i} Y (giver inside R_i, receiver |K| > R_i: Prod_out_sphere_nlinV),
+// in_HGR(i) = sum_{n<=i} sum_{0<m<=i} Y (Prod_in_sphere_nlinV), outout(i) = sum_{n>i} sum_{m>i} Y (giver U>).
+// Rows as th_flux_mhd: u2u = -out_UUU, b2b = -out_UBB, u2b_in_out = out_BUB, u2b_in_in = in_BUB,
+// b2u_in_out = out_UBU, b2u_in_in = in_UBU, u_out_b_out = outout_BUB; the Elsasser fluxes from N's
+// bilinearity: Zp = -out over (Z- helper, Z+ giver, Z+ receiver) = -sum s_H out_HGR (s_U = 1, s_B = -1), and
+// Zm = -sum s_G s_R out_HGR.
+template <typename R> static int flux_mhd_impl(th_state* s, const double* radii, int nsh, double* out)
+{
+    const int nr = nsh + 1, W = nsh + 1;
+    const int roff[2] = {0, 3};
+    std::vector<double> Yb[4][2], Yc[4][2];
+    // pipelines (H, G): 0 (U, U), 1 (U, B), 2 (B, U), 3 (B, B); givers up to the infinite shell where the
+    // in-sphere sums of the last sphere or the outside givers need it
+    const int hg[4][2] = {{0, 0}, {0, 3}, {3, 0}, {3, 3}};
+    const int ngiv[4] = {nsh - 1, nsh, nsh, nsh - 1};
+    for (int k = 0; k < 4; k++) TRY(et_shell_rows<R>(s, radii, nsh, ngiv[k], hg[k][0], hg[k][1], roff, 2, Yb[k], Yc[k]));
+    auto Y = [&](int k, int r, int n, int m) {
+        if (n > ngiv[k]) return 0.0;
+        return Yb[k][r][(size_t)n * nr + m] + Yc[k][r][(size_t)n * nr + m];
+    };
+    auto sum = [&](int k, int r, int n0, int n1, int m0, int m1) {
+        double a = 0;
+        for (int n = n0; n <= n1; n++)
+            for (int m = m0; m <= m1; m++) a += Y(k, r, n, m);
+        return a;
+    };
+    for (int i = 0; i < 9 * W; i++) out[i] = 0.0;
+    for (int i = 1; i <= nsh; i++) {
+        double o[4][2];
+        for (int k = 0; k < 4; k++)
+            for (int r = 0; r < 2; r++) o[k][r] = sum(k, r, 1, i, i + 1, nsh);
+        out[0 * W + i] = -o[0][0];                          // U<->U>  flux_self
+        out[1 * W + i] = -o[1][1];                          // B<->B>  W.flux_self
+        out[2 * W + i] = o[2][1];                           // U<->B>  flux_VF_in_out (helper B)
+        out[3 * W + i] = sum(2, 1, 1, i, 1, i);             // U<->B<  flux_VF_in_in
+        out[4 * W + i] = o[1][0];                           // B<->U>  W.flux_VF_in_out (helper U)
+        out[5 * W + i] = sum(1, 0, 1, i, 1, i);             // B<->U<  W.flux_VF_in_in
+        out[6 * W + i] = sum(2, 1, i + 1, nsh, i + 1, nsh); // U>->B>  flux_VF_out_out
+        const double sH[4] = {1, 1, -1, -1}, sG[4] = {1, -1, 1, -1}, sR[2] = {1, -1};
+        double zp = 0, zm = 0;
+        for (int k = 0; k < 4; k++)
+            for (int r = 0; r < 2; r++) {
+                zp += sH[k] * o[k][r];
+                zm += sG[k] * sR[r] * o[k][r];
+            }
+        out[7 * W + i] = -zp;
+        out[8 * W + i] = -zm;
+    }
+    return OK;
+}
+
+extern "C" int th_flux_mhd_fused(th_plan* p, th_state* s, const double* radii, int nsph, double* out)
+{
+    if (!p || !s || s->plan != p || !radii || nsph < 2 || !out) { set_error("th_flux_mhd_fused: bad args"); return E_ARG; }
+    if (s->physics != TH_MHD) { set_error("th_flux_mhd_fused: MHD state required"); return E_UNSUPPORTED; }
+    if (radii[nsph] < 1e3) { set_error("th_flux_mhd_fused: radii[nsph] must be the infinite radius"); return E_ARG; }
+    TH_CHECK_HIP(hipSetDevice(p->ctx->device));
+    return p->prec ? flux_mhd_impl<float>(s, radii, nsph, out) : flux_mhd_impl<double>(s, radii, nsph, out);
 }
 
 extern "C" int th_flux_shell_tr(th_plan* p, th_state* s, const double* radii, int nsh, double* Pi, double* T)

@@ -780,11 +780,19 @@ void IncFluid::Time_advance(IncVF& W)
     check(th_sync(plan_), "th_sync");
 }
 
+// Runs that step on the fused MHD state (its fields are 2/3-dealiased after every step) take the fused
+// giver-shell pipeline (th_flux_mhd_fused); otherwise the plain sphere loop of the reference (th_flux_mhd).
 void IncFluid::Compute_flux(IncVF& W)
 {
     void* U[3] = {V1.data(), V2.data(), V3.data()};
     void* B[3] = {W.V1.data(), W.V2.data(), W.V3.data()};
     flux_mhd.assign((size_t)9 * (no_spheres + 1), 0.0);
+    if (mstate_ && no_spheres >= 2) {
+        void* f[6] = {U[0], U[1], U[2], B[0], B[1], B[2]};
+        for (int i = 0; i < 6; i++) check(th_state_upload(mstate_, i, f[i], 0), "th_state_upload");
+        check(th_flux_mhd_fused(plan_, mstate_, sphere_radius.data(), no_spheres, flux_mhd.data()), "Compute_flux(W)");
+        return;
+    }
     check(th_flux_mhd(plan_, U, B, sphere_radius.data(), no_spheres, flux_mhd.data()), "Compute_flux(W)");
 }
 

@@ -122,6 +122,28 @@ def test_fused_flux_and_shell_transfer(ctx, prec, tol):
     assert np.abs(T - T_ref).max() < tol * np.abs(T_ref).max()
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 2e-5)])
+def test_fused_flux_mhd(ctx, prec, tol):
+    """th_flux_mhd_fused: the nine MHD fluxes of Compute_flux(IncVF& W) (compute_flux.cc:148-262) from giver
+    shells through the fused pipeline equal the reference's sphere loop (the oracle), every row including
+    the in-sphere and outside-giver ones and the Elsasser pair."""
+    from tarang_amd import Plan, State, abi
+    N = (32, 32, 32)
+    o = Oracle(N)
+    p = Plan(ctx, N, precision=prec)
+    s = State(p, abi.TH_MHD)
+    U, B = o.ic_random(2, kp=3.0), o.ic_random(3, kp=3.0)
+    s.upload([p.to_dev(a) for a in U + B])
+    radii = np.concatenate([np.arange(12, dtype=float), [10000.0]])
+    got = s.flux_mhd(radii)
+    ref = o.flux_mhd(U, B, radii, elsasser=True)
+    names = ["u2u", "b2b", "u2b_in_out", "u2b_in_in", "b2u_in_out", "b2u_in_in", "u_out_b_out", "zp", "zm"]
+    for r, k in enumerate(names):
+        sc = np.abs(ref[k]).max()
+        assert sc > 0, k
+        assert np.abs(got[r] - ref[k]).max() <= tol * sc, (k, np.abs(got[r] - ref[k]).max() / sc)
+
+
 def test_config1_fused_taylor_green_64(ctx):
     """Config 1 through the fused path: TG 64^3 FP64, RK4, dt = 1e-3, 100 steps, nu = 0.01."""
     from tarang_amd import Plan, State
