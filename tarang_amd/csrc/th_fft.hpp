@@ -225,13 +225,19 @@ template <typename R, int L, int P, int SIGN, bool R16 = false> struct LineFFT {
 
     // WAVE: the P threads of every line sit in one wavefront (P divides 64 and each line's
     // threads are consecutive lanes), so LDS exchanges need no workgroup barrier.
-    template <int LS, bool WAVE = false, class Lay = NoPad>
+    // DBUF: exchange s uses LDS buffer s % 2 (lds and lds + L * LS; the caller provides both), so an exchange
+    // never overwrites the buffer the previous exchange is still being read from, and only the barrier after
+    // each exchange's writes remains.  Across consecutive FFTs on the same buffers: with an even number of
+    // exchanges the next FFT's first exchange writes buffer 0 after the barrier that follows this FFT's last
+    // writes (to buffer 1), which every thread reaches after its reads of buffer 0; with an odd number both
+    // would use buffer 0 back to back, so the first exchange keeps its barrier before writing.
+    template <int LS, bool WAVE = false, class Lay = NoPad, bool DBUF = false>
     static __device__ __forceinline__ void run(cplx<R> (&v)[E], cplx<R>* lds, int t, const cplx<R>* __restrict__ tw)
     {
-        stage<0, 1, LS, WAVE, Lay>(v, lds, t, tw);
+        stage<0, 1, LS, WAVE, Lay, DBUF>(v, lds, t, tw);
     }
 
-    template <int s, int Ns, int LS, bool WAVE, class Lay>
+    template <int s, int Ns, int LS, bool WAVE, class Lay, bool DBUF = false>
     static __device__ __forceinline__ void stage(cplx<R> (&v)[E], cplx<R>* lds, int t, const cplx<R>* __restrict__ tw)
     {
         constexpr int Rr = S.r[s];
@@ -251,19 +257,20 @@ template <typename R, int L, int P, int SIGN, bool R16 = false> struct LineFFT {
             DFT<Rr, SIGN, R>::run(&v[b * Rr]);
         }
         if constexpr (s + 1 < NST) {
-            line_sync<WAVE>();
+            cplx<R>* buf = DBUF ? lds + (s & 1) * (L * LS) : lds;
+            if constexpr (!DBUF || (s == 0 && (NST - 1) % 2 == 1)) line_sync<WAVE>();
 #pragma unroll
             for (int b = 0; b < B; b++) {
                 const int j = t + P * b;
                 const int base = (j / Ns) * Ns * Rr + (j % Ns);
 #pragma unroll
-                for (int r = 0; r < Rr; r++) lds[Lay::map(s, base + r * Ns) * LS] = v[b * Rr + r];
+                for (int r = 0; r < Rr; r++) buf[Lay::map(s, base + r * Ns) * LS] = v[b * Rr + r];
             }
             line_sync<WAVE>();
             constexpr int Rn = S.r[s + 1];
 #pragma unroll
-            for (int i = 0; i < E; i++) v[i] = lds[Lay::map(s, t + P * (i / Rn) + (i % Rn) * (L / Rn)) * LS];
-            stage<s + 1, Ns * Rr, LS, WAVE, Lay>(v, lds, t, tw);
+            for (int i = 0; i < E; i++) v[i] = buf[Lay::map(s, t + P * (i / Rn) + (i % Rn) * (L / Rn)) * LS];
+            stage<s + 1, Ns * Rr, LS, WAVE, Lay, DBUF>(v, lds, t, tw);
         }
     }
 };

@@ -437,7 +437,7 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     constexpr int NP = (MODE == 0) ? 6 : 9;
     constexpr int NA = (MODE == 0) ? 3 : 6;
     constexpr int NX = (MODE == 1) ? 4 : 2;
-    __shared__ cplx<R> lds[L * T];
+    __shared__ cplx<R> lds[2 * L * T];   // two exchange buffers (LineFFT DBUF)
     __shared__ cplx<R> twl[L];
     // exp(c Kx^2) per row for the integrating factors c = fh[0], f1[0] (MHD: and fh[1], f1[1]):
     // exp(c K^2) = exp(c Kx^2) exp(c (Ky^2 + Kz^2)), the second factor once per thread
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
             const cplx<R>* row = src + (long)F::in_pos(0, i) * g.sx;
             v[i] = ok ? row[lane_off] : czero<R>();
         }
-        F::template run<T>(v, lds + c, t, twl + opaque_zero());
+        F::template run<T, false, NoPad, true>(v, lds + c, t, twl + opaque_zero());
         // contributions (a1 <- K_k1) and (a2 <- K_k2) of product p; k = 0 means Kx (per row)
         int a1, k1, a2, k2;
         if (MODE == 0) {           // P_ij, (i,j) = (0,0),(1,1),(2,2),(0,1),(1,2),(2,0): N_i += iK_j P, N_j += iK_i P
