@@ -231,13 +231,20 @@ template <typename R, int L, int P, int SIGN, bool R16 = false> struct LineFFT {
     // exchanges the next FFT's first exchange writes buffer 0 after the barrier that follows this FFT's last
     // writes (to buffer 1), which every thread reaches after its reads of buffer 0; with an odd number both
     // would use buffer 0 back to back, so the first exchange keeps its barrier before writing.
-    template <int LS, bool WAVE = false, class Lay = NoPad, bool DBUF = false>
+    // PRE0 overrides that first barrier: -1 as above (always taken without DBUF), 0 never (the caller
+    // guarantees that no thread still reads the first exchange's buffer, e.g. nothing read it before this
+    // FFT), 1 always (the caller read buffer 0 after the previous FFT).
+    template <int LS, bool WAVE = false, class Lay = NoPad, bool DBUF = false, int PRE0 = -1>
     static __device__ __forceinline__ void run(cplx<R> (&v)[E], cplx<R>* lds, int t, const cplx<R>* __restrict__ tw)
     {
-        stage<0, 1, LS, WAVE, Lay, DBUF>(v, lds, t, tw);
+        stage<0, 1, LS, WAVE, Lay, DBUF, PRE0>(v, lds, t, tw);
     }
+    // With DBUF, the buffer (0 or 1) that the last exchange did not use; after run() returns every thread
+    // has finished reading it (the barrier after the last exchange's writes ensures that), so a caller may
+    // write it at once.  Without exchanges (NST == 1) there is no such barrier and this is buffer 0.
+    static constexpr int FREEBUF = (NST >= 2) ? 1 - ((NST - 2) & 1) : 0;
 
-    template <int s, int Ns, int LS, bool WAVE, class Lay, bool DBUF = false>
+    template <int s, int Ns, int LS, bool WAVE, class Lay, bool DBUF = false, int PRE0 = -1>
     static __device__ __forceinline__ void stage(cplx<R> (&v)[E], cplx<R>* lds, int t, const cplx<R>* __restrict__ tw)
     {
         constexpr int Rr = S.r[s];
@@ -258,7 +265,8 @@ template <typename R, int L, int P, int SIGN, bool R16 = false> struct LineFFT {
         }
         if constexpr (s + 1 < NST) {
             cplx<R>* buf = DBUF ? lds + (s & 1) * (L * LS) : lds;
-            if constexpr (!DBUF || (s == 0 && (NST - 1) % 2 == 1)) line_sync<WAVE>();
+            constexpr bool pre = (s == 0 && PRE0 >= 0) ? (PRE0 == 1) : (!DBUF || (s == 0 && (NST - 1) % 2 == 1));
+            if constexpr (pre) line_sync<WAVE>();
 #pragma unroll
             for (int b = 0; b < B; b++) {
                 const int j = t + P * b;
@@ -270,7 +278,7 @@ template <typename R, int L, int P, int SIGN, bool R16 = false> struct LineFFT {
             constexpr int Rn = S.r[s + 1];
 #pragma unroll
             for (int i = 0; i < E; i++) v[i] = buf[Lay::map(s, t + P * (i / Rn) + (i % Rn) * (L / Rn)) * LS];
-            stage<s + 1, Ns * Rr, LS, WAVE, Lay, DBUF>(v, lds, t, tw);
+            stage<s + 1, Ns * Rr, LS, WAVE, Lay, DBUF, PRE0>(v, lds, t, tw);
         }
     }
 };

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restric
         }
         v[i] = a;
     }
-    F::template run<T>(v, lds + c, t, twl);
+    F::template run<T, false, NoPad, false, 0>(v, lds + c, t, twl);   // lds unused before: no first pre-write barrier
     if (ok) {
         cplx<R>* out = A + oc * g.cA + m;
 #pragma unroll
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(T * P) void k_fy_inv(FG g, const cplx<R>* __restric
         const int l = F::in_pos(t, i);
         v[i] = (ok && lkept(g.my, l)) ? a[(long)lq(g.my, l) * g.KZP] : czero<R>();
     }
-    F::template run<T>(v, lds + c, t, twl);
+    F::template run<T, false, NoPad, false, 0>(v, lds + c, t, twl);   // lds unused before: no first pre-write barrier
     if (ok) {
         cplx<R>* b = B + comp * g.cB + (long)x * g.sB + cz;
 #pragma unroll
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(T * P) void k_fy_fwd(FG g, const cplx<R>* __restric
     cplx<R> v[F::E];
 #pragma unroll
     for (int i = 0; i < F::E; i++) v[i] = ok ? a[(long)F::in_pos(t, i) * g.KZP] : czero<R>();
-    F::template run<T>(v, lds + c, t, twl);
+    F::template run<T, false, NoPad, false, 0>(v, lds + c, t, twl);   // lds unused before: no first pre-write barrier
     if (ok) {
         cplx<R>* d = D + comp * g.cA + (long)x * g.sx + cz;
 #pragma unroll
@@ -653,7 +653,7 @@ __global__ __launch_bounds__(T * P) void k_sx_inv(FG g, const cplx<R>* __restric
         const cplx<R> x = mk<R>(a.x + b.y, a.y - b.x);
         v[i] = cmul(conjg(tw4l[k]), x);
     }
-    F::template run<T>(v, lds + c, t, twl);
+    F::template run<T, false, NoPad, false, 0>(v, lds + c, t, twl);   // lds unused before: no first pre-write barrier
     if (ok) {
         cplx<R>* out = A + oc * g.cA + m;
 #pragma unroll
@@ -692,7 +692,10 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
 {
     using F = LineFFT<R, L, P, -1>;
     constexpr int E = F::E;
-    __shared__ cplx<R> lds[L * T];
+    // two exchange buffers (LineFFT DBUF); the output table of each product goes to the one the FFT's
+    // last exchange left free, and the next product's first exchange waits for its mirror reads (PRE0 = 1)
+    __shared__ cplx<R> lds[2 * L * T];
+    cplx<R>* const ob = lds + F::FREEBUF * (L * T);
     __shared__ cplx<R> twl[L];
     // exp(c Kx^2) per row for c = fh[0], f1[0] (velocity) and fh[1], f1[1] (temperature); the
     // (Ky, Kz) factor of exp(c K^2) is computed once per thread
@@ -744,10 +747,10 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
             const R sg = (psin && j0 + t >= L / 2) ? R(-1) : R(1);
             v[i] = cscale(x, sg);
         }
-        F::template run<T>(v, lds + c, t, twl + opaque_zero());
-        __syncthreads();
+        F::template run<T, false, NoPad, true, 1>(v, lds + c, t, twl + opaque_zero());
+        if constexpr (F::NST < 2) __syncthreads();    // no exchange: the previous product's mirror reads
 #pragma unroll
-        for (int i = 0; i < E; i++) lds[F::out_pos(t, i) * T + c] = v[i];
+        for (int i = 0; i < E; i++) ob[F::out_pos(t, i) * T + c] = v[i];
         __syncthreads();
         // product p adds d_j F(p) to N_a: Xderiv_Cos (-Kx) / Xderiv_Sin (+Kx) along x, i Ky, i Kz
         const int a1 = (p == 0 || p == 3 || p == 5) ? 0 : (p == 1 || p == 4) ? 1 : (p == 2) ? 2 : 3;
@@ -763,7 +766,7 @@ __global__ __launch_bounds__(T * P) void k_sx_fwd_rb(FG g, const cplx<R>* __rest
             const int l = F::out_pos(t, i);
             const cplx<R> w = w4[l];
             const cplx<R> Pv = cmul(w, v[i]);
-            const cplx<R> Qv = cmul(conjg(w), lds[((L - l) % L) * T + c]);
+            const cplx<R> Qv = cmul(conjg(w), ob[((L - l) % L) * T + c]);
             const cplx<R> dv = Pv - Qv, sv = Pv + Qv;
             cplx<R> Av = psin ? mk<R>(-dv.y, dv.x) : sv;                        // i (P - Q) or P + Q
             if (psin && l == 0) Av = czero<R>();
@@ -861,7 +864,7 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __rest
 {
     using F = LineFFT<R, L, P, -1>;
     constexpr int E = F::E;
-    __shared__ cplx<R> lds[L * T];
+    __shared__ cplx<R> lds[2 * L * T];   // two exchange buffers (LineFFT DBUF)
     __shared__ cplx<R> twl[L];
     stage_tw<R, L>(twl, tw);
     __syncthreads();
@@ -886,7 +889,7 @@ __global__ __launch_bounds__(T * P) void k_fx_fwd_et(FG g, const cplx<R>* __rest
         const cplx<R>* src = D + p * g.cA + m;
 #pragma unroll
         for (int i = 0; i < E; i++) v[i] = ok ? src[(long)F::in_pos(t, i) * g.sx] : czero<R>();
-        F::template run<T>(v, lds + c, t, twl + opaque_zero());
+        F::template run<T, false, NoPad, true>(v, lds + c, t, twl + opaque_zero());
         const int pi = p / 3, pj = p % 3;        // P(pi,pj) = G_pi H_pj -> N_pi += i K_pj P
         R al[3], be[3];
 #pragma unroll
