@@ -202,8 +202,8 @@ def config1(ctx, threads):
 
 def load_traffic(workload):
     """Measured HBM bytes per launch of each pass (profiles/traffic_<workload>.json: rocprofv3 FETCH_SIZE and
-    WRITE_SIZE runs scaled by a copy kernel of known size), keyed by pass, with the launch variant the
-    counters were read on."""
+    WRITE_SIZE runs scaled by a copy kernel of known size), keyed by pass and then by the launch variant the
+    counters were read on: {pass: {"<variant>": {"kernel": ..., "bytes": ..., ...}}}."""
     p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if os.path.exists(p):
         with open(p) as f:
@@ -214,9 +214,8 @@ def load_traffic(workload):
 def traffic_of(table, name, variant):
     """The measured bytes of pass `name` when the counters were read on the same launch variant, else None."""
     e = (table or {}).get(name)
-    if isinstance(e, dict) and e.get("variant") == variant:
-        return e.get("bytes")
-    return None
+    e = e.get(str(variant)) if isinstance(e, dict) else None
+    return e.get("bytes") if isinstance(e, dict) else None
 
 
 ENSEMBLE_SHAPES = [64] * 24 + [96] * 24 + [128] * 16
@@ -570,6 +569,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
+    ap.add_argument("--variants", default="",
+                    help="ns512: launch variants of the five FP32-mixed passes, comma-separated (no autotuning; "
+                         "counter runs use this to read each variant)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size kernel "
@@ -617,7 +619,11 @@ def main():
     s32 = State(p32)
     s32.upload([v.to(torch.complex64) for v in V])
     # launch shapes chosen by the autotuner on this workload (outside the timed region)
-    var32 = s32.tune(dt, nu=nu, nrep=3) if not args.no_tune else s32.variants
+    if args.variants:
+        s32.variants = [int(x) for x in args.variants.split(",")]
+        var32 = s32.variants
+    else:
+        var32 = s32.tune(dt, nu=nu, nrep=3) if not args.no_tune else s32.variants
     log(f"bench: fp32-mixed launch variants {var32}")
     # FP32-mixed timing
     s32.step(dt, nu=nu, steps=args.warmup)

@@ -18,6 +18,11 @@ template <typename R> __host__ __device__ __forceinline__ cplx<R> cmulc(cplx<R> 
 template <typename R> __host__ __device__ __forceinline__ cplx<R> cscale(cplx<R> a, R s) { return mk<R>(a.x * s, a.y * s); }
 template <typename R> __host__ __device__ __forceinline__ cplx<R> conjg(cplx<R> a) { return mk<R>(a.x, -a.y); }
 template <typename R> __host__ __device__ __forceinline__ cplx<R> mul_i(cplx<R> a) { return mk<R>(-a.y, a.x); }      // i*a
+// s + i w d in four fused multiply-adds (s + mul_i(cmul(w, d)) rounds the product first: two more instructions)
+template <typename R> __device__ __forceinline__ cplx<R> add_iwd(cplx<R> s, cplx<R> w, cplx<R> d)
+{
+    return mk<R>(fma(-w.x, d.y, fma(-w.y, d.x, s.x)), fma(w.x, d.x, fma(-w.y, d.y, s.y)));
+}
 template <typename R> __host__ __device__ __forceinline__ cplx<R> mul_mi(cplx<R> a) { return mk<R>(a.y, -a.x); }     // -i*a
 template <typename R> __host__ __device__ __forceinline__ cplx<R> czero() { return mk<R>(R(0), R(0)); }
 

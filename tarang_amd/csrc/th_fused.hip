@@ -59,8 +59,9 @@ FG make_fg(const Geo& geo, int prec)
     g.f1 = geo.kf[0];
     g.f2 = geo.kf[1];
     g.f3 = geo.kf[2];
-    g.norm = 1.0 / ((double)geo.N1 * (double)geo.N2 * (double)geo.N3);
-    if (g.scft) g.norm *= 0.5;                                          // Norm_SCFT: 1 / (2 N1 N2 N3)
+    // 1 / (N1 N2 N3) times the 1/2 that the z pass's r2c unpacking leaves out (k_fz writes 2 X_k)
+    g.norm = 0.5 / ((double)geo.N1 * (double)geo.N2 * (double)geo.N3);
+    if (g.scft) g.norm *= 0.5;                                          // Norm_SCFT: transforms scale by 1 / (2 N1 N2 N3)
     g.nbox = 1;
     return g;
 }

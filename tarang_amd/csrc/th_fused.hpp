@@ -62,7 +62,7 @@ struct FG {
     int nbox;              // boxes of this size advanced by one launch per pass
     int scft;              // sin/cos along x (SCFT basis), else Fourier
     double f1, f2, f3;
-    double norm; // 1 / (N1 N2 N3)
+    double norm; // 1 / (2 N1 N2 N3): the line transforms' 1 / (N1 N2 N3) and the z pass's unpacking 1/2 (SCFT: 1/2 more)
 };
 
 struct StageCoef {

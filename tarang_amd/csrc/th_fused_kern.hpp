@@ -226,16 +226,16 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
             if (zmir(m)) {                    // s = d = x_k
                 cplx<R> xv = xa[m];
                 if (t == 0 && m == 0) xv.y = R(0);
-                u[a][zreg<E, FI::R0>(m)] = xv + mul_i(cmul(w, xv));
+                u[a][zreg<E, FI::R0>(m)] = add_iwd(xv, w, xv);
             } else if (zin(m)) {              // s = conj(x_{LZ-k}), d = -s
                 cplx<R> xb = conjg(Z[LZ - t - P * m]);
-                u[a][zreg<E, FI::R0>(m)] = xb - mul_i(cmul(w, xb));
+                u[a][zreg<E, FI::R0>(m)] = add_iwd(xb, mk<R>(-w.x, -w.y), xb);
             } else {
                 cplx<R> xb = Z[LZ - t - P * m];
                 cplx<R> xv = xa[m];
                 xb.y = -xb.y;
                 cplx<R> s = xv + xb, d = xv - xb;
-                u[a][zreg<E, FI::R0>(m)] = s + mul_i(cmul(w, d));
+                u[a][zreg<E, FI::R0>(m)] = add_iwd(s, w, d);
             }
         }
         FI::template run<1, true, Lay>(u[a], Z, t, tws);
@@ -262,12 +262,13 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
             if (P * m + (P - 1) > LZ - NKZ) Z[t + P * m] = w[zreg<E, FF::RL>(m)];
         if (t == 0) Z[LZ] = w[zreg<E, FF::RL>(0)];
         wave_sync();
+        // 2 X_k = (z_k + conj z_{LZ-k}) - i W^k (z_k - conj z_{LZ-k}); the factor 1/2 is left to the
+        // consumers of the product lines, whose normalisation FG::norm carries it
         auto unpack = [&](int m) {
             cplx<R> zk = w[zreg<E, FF::RL>(m)];
             cplx<R> zc = conjg(Z[LZ - t - P * m]);
-            cplx<R> e = cscale(zk + zc, R(0.5));
-            cplx<R> tt = cmul(twp[t + P * m], zk - zc);
-            return mk<R>(e.x + R(0.5) * tt.y, e.y - R(0.5) * tt.x);
+            const cplx<R> tw = twp[t + P * m];
+            return add_iwd(zk + zc, mk<R>(-tw.x, -tw.y), zk - zc);
         };
         // registers whose points all lie at kz >= NKZ are neither unpacked nor stored
         if constexpr (sizeof(R) == 4) {
@@ -293,31 +294,21 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
         }
     };
     if constexpr (sizeof(R) == 4) {
-        // FP32: a rolled loop over the 3 rotation steps keeps the compiler from interleaving
-        // products and hoisting their twiddles (registers, occupancy); rotating the group's
-        // registers by one between steps puts every operand in a fixed register, so no per-product
-        // operand selects are needed.
+        // FP32: products one after another, each fenced off for the scheduler, so that the compiler
+        // neither interleaves products nor holds two products' lines and twiddles at once (registers,
+        // occupancy); every operand is a compile-time register.
         constexpr int G0 = ZP::G0;
-#pragma unroll 1
+#pragma unroll
         for (int r = 0; r < 3; r++) {
 #pragma unroll
             for (int j = 0; j < ZP::NPR; j++) {
                 const int yo = ZP::yop(j);
+                const int xi = G0 + r, yi = yo < 0 ? G0 + (r - 1 - yo) % 3 : yo;
                 cplx<R> w[E];
 #pragma unroll
-                for (int i = 0; i < E; i++) {
-                    const cplx<R> x = u[G0][i];
-                    const cplx<R> y = yo < 0 ? u[G0 - 1 - yo][i] : u[yo][i];
-                    w[i] = mk<R>(x.x * y.x, x.y * y.y);
-                }
+                for (int i = 0; i < E; i++) w[i] = mk<R>(u[xi][i].x * u[yi][i].x, u[xi][i].y * u[yi][i].y);
                 product_out(w, ZP::pout(r, j));
-            }
-#pragma unroll
-            for (int i = 0; i < E; i++) {
-                const cplx<R> t0 = u[G0][i];
-                u[G0][i] = u[G0 + 1][i];
-                u[G0 + 1][i] = u[G0 + 2][i];
-                u[G0 + 2][i] = t0;
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     } else {
