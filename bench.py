@@ -11,6 +11,7 @@ FP32-mixed run against the FP64 run after the same number of steps.
   python bench.py --workload ensemble [--steps K] [--warmup W] [--no-cpu]
   python bench.py --workload mhd [--n 512]      python bench.py --workload rb [--rb-n 256]
   python bench.py --workload forced [--n 512]   (the N^3 step with energy-supply forcing)
+  python bench.py --workload dft16              (16-point DFTs: VALU codelet vs MFMA DFT-as-GEMM)
 Prints one JSON line.  Runs on one GPU; --gpus > 1 exits with a message (single-GPU build).
 """
 import argparse
@@ -555,9 +556,60 @@ def bench_rb(args):
          "cpu_baseline": None})), flush=True)
 
 
+def bench_dft16(args):
+    """The DFT-as-GEMM measurement: the 16-point DFTs of the radix-16 line-FFT stages on 2^22 FP32 complex
+    vectors (512 MiB in, 512 MiB out, resident in HBM) by the vector-ALU codelet the line FFTs use and by the
+    FP32 matrix cores (th_dft16_probe; the DFT as a real 32 x 32 matrix times 32 vectors, 16
+    v_mfma_f32_32x32x2f32 per 32 vectors).  Timed at 1, 8 and 32 applications per launch (the data read
+    and written once, the applications in registers), HIP events over 10 launches each; the cost of one
+    application is the increment from 8 to 32 (memory traffic identical).  Parity of both against
+    numpy's FFT on the first 65536 vectors."""
+    import numpy as np
+    import torch
+    from tarang_amd import Context
+    ctx = Context(0)
+    nvec = 1 << 22
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(5)
+    x = torch.randn((16, nvec), dtype=torch.complex64, device="cuda:0", generator=g)
+    y = torch.empty_like(x)
+    xs = x[:, :65536].cpu().numpy().astype(np.complex128)
+    res = {}
+    for method, name in ((0, "valu"), (1, "mfma")):
+        r = {}
+        for reps in (1, 8, 32):
+            ms = ctx.dft16_probe(x, y, method, sign=-1, reps=reps, ntimed=10)
+            ref = xs.copy()
+            for _ in range(reps):
+                ref = np.fft.fft(ref, axis=0) / 4
+            got = y[:, :65536].cpu().numpy().astype(np.complex128)
+            err = float(np.abs(got - ref).max() / np.abs(ref).max())
+            r[f"reps{reps}"] = {"ms": round(ms, 4), "GDFT16_s": round(nvec * reps / (ms * 1e-3) / 1e9, 2),
+                                "GB_s": round(2 * 16 * 8 * nvec / (ms * 1e-3) / 1e9, 1), "max_rel_err": err}
+            log(f"bench: dft16 {name} reps={reps} {ms:.4f} ms err {err:.2e}")
+        # one application in registers: the 8 -> 32 increment
+        inc = (r["reps32"]["ms"] - r["reps8"]["ms"]) / 24
+        r["per_application_us"] = round(1e3 * inc, 3)
+        r["GDFT16_s_compute"] = round(nvec / (inc * 1e-3) / 1e9, 1)
+        res[name] = r
+    # executed MFMA work: 16 instructions x 4096 flops per 32 vectors per application
+    mfma_tflops = 16 * 4096 * (nvec / 32) / (res["mfma"]["per_application_us"] * 1e-6) / 1e12
+    ratio = res["mfma"]["per_application_us"] / res["valu"]["per_application_us"]
+    roof = {"bound": "mfma", "kernel": "k_dft16_mfma (one application, 8 -> 32 increment)",
+            "achieved": round(mfma_tflops, 1), "peak": 157.3, "unit": "TFLOP/s", "frac": round(mfma_tflops / 157.3, 4),
+            "traffic": None, "flops_model": "16 v_mfma_f32_32x32x2f32 x 4096 flops per 32 vectors per application"}
+    print(json.dumps(_line(
+        "16-point FP32 complex DFTs per second in registers: VALU codelet vs MFMA DFT-as-GEMM (2^22 vectors)",
+        res["valu"]["GDFT16_s_compute"], args, res["valu"]["reps8"]["ms"],
+        {"workload": "dft16_2^22_vectors_fp32", "nvec": nvec, "layout": "point-major", "sign": -1},
+        {"unit": "GDFT16/s", "dtype": "fp32", "data": "synthetic (torch.randn on the GPU, seed 5)",
+         "valu": res["valu"], "mfma": res["mfma"], "mfma_over_valu_time_per_application": round(ratio, 2),
+         "winner": "valu" if ratio >= 1 else "mfma", "roofline": roof, "cpu_baseline": None})), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["ns512", "ensemble", "mhd", "rb", "forced"], default="ns512",
+    ap.add_argument("--workload", choices=["ns512", "ensemble", "mhd", "rb", "forced", "dft16"], default="ns512",
                     help="ns512: the N^3 decaying-turbulence step (default, N from --n); "
                          "ensemble: the uneven 64-box ensemble; mhd: N^3 MHD; rb: Rayleigh-Benard (SCFT); "
                          "forced: the N^3 step with energy-supply forcing")
@@ -592,6 +644,9 @@ def main():
         return
     if args.workload == "forced":
         bench_forced(args)
+        return
+    if args.workload == "dft16":
+        bench_dft16(args)
         return
 
     import numpy as np

@@ -309,6 +309,17 @@ int th_flux_mhd_fused(th_plan *plan, th_state *state, const double *radii, int n
 int th_profile_passes(th_plan *plan, th_state *state, double dt, const th_phys *p, int stage, int nrep,
                       double ms[5], double bytes[5]);
 
+/* Measurement only (no counterpart in the reference, whose FFTW codelets are planned in
+ * Init_fftw_plan_FOUR four_tr.cc:44-98): the 16-point DFT of the line FFTs' radix-16 stages on nvec
+ * FP32 complex vectors, stored point-major (x[j * nvec + v] = point j of vector v), by
+ * method 0 = the vector-ALU codelet or method 1 = a DFT-as-GEMM on the FP32 matrix cores
+ * (v_mfma_f32_32x32x2f32, the DFT as a real 32 x 32 matrix times 32 vectors).  y = (W / 4)^reps x with
+ * W[k][n] = exp(sign 2 pi i k n / 16).  nvec must be a positive multiple of 256, 1 <= reps <= 32 (4^32 |x| stays inside the FP32 range).  The call
+ * launches once (asynchronously on the context's stream); with ntimed > 0 it then times ntimed more
+ * launches with HIP events and returns their mean duration in *ms (synchronises). */
+int th_dft16_probe(th_ctx *ctx, int method, int sign, const void *x, void *y, long nvec, int reps, int ntimed,
+                   double *ms);
+
 /* ---- launch-shape autotuner (no counterpart in the reference, whose transforms are planned by
  * FFTW_MEASURE in Init_fftw_plan_FOUR four_tr.cc:44-98) ----
  * th_tune times every launch-shape variant of the five fused passes on the state's own buffers

@@ -138,6 +138,7 @@ def lib():
         "th_flux_shell_tr": (i, [vp, vp, dp, i, dp, dp]),
         "th_profile_passes": (i, [vp, vp, d, C.POINTER(Phys), i, i, dp, dp]),
         "th_tune": (i, [vp, vp, d, C.POINTER(Phys), i, C.POINTER(C.c_int)]),
+        "th_dft16_probe": (i, [vp, i, i, vp, vp, C.c_long, i, i, dp]),
         "th_state_set_variants": (i, [vp, C.POINTER(C.c_int)]),
         "th_state_variants": (i, [vp, C.POINTER(C.c_int)]),
         "th_ensemble_tune": (i, [vp, dp, dp, i]),
@@ -240,6 +241,18 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def dft16_probe(self, x, y, method, sign=-1, reps=1, ntimed=0):
+        """th_dft16_probe on complex64 device tensors of shape (16, nvec): y = (W / 4)^reps x by the VALU
+        codelet (method 0) or the MFMA DFT-as-GEMM (method 1); returns the mean ms of ntimed launches."""
+        assert x.shape == y.shape and x.shape[0] == 16 and x.is_contiguous() and y.is_contiguous()
+        t = _torch()
+        t.cuda.synchronize(self.device)
+        ms = C.c_double(0.0)
+        _check(lib().th_dft16_probe(self.h, int(method), int(sign), _ptr(x), _ptr(y), int(x.shape[1]), int(reps),
+                                    int(ntimed), C.byref(ms)), "th_dft16_probe")
+        t.cuda.synchronize(self.device)
+        return ms.value if ntimed > 0 else None
 
 
 class Plan:

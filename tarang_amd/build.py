@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libtarang_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["th_api.hip", "th_generic.hip", "th_reduce.hip", "th_ic.hip", "th_fused.hip", "th_fused_f32.hip",
-           "th_fused_f64.hip", "th_force.hip"]
+           "th_fused_f64.hip", "th_force.hip", "th_probe.hip"]
 HOST_SOURCES = ["th_host.cpp", "th_io.cpp"]
 
 COMMON = ["-std=c++17", "-fPIC", "-O3", "--offload-arch=gfx950",
