@@ -102,6 +102,42 @@ def test_fused_mhd_fp64(ctx):
         assert rel(H(out[i]), ref) < 1e-12
 
 
+@pytest.mark.parametrize("N", [(48, 32, 24), (32, 32, 48), (512, 16, 16)])
+def test_fused_mhd_fp64_spilling_shapes(ctx, N):
+    """The FP64 MHD kernels that spill registers to scratch (z pass at N3 = 3 * 2^k, x-forward at N1 = 512)
+    still equal the oracle's Time_advance(IncVF&) steps."""
+    from tarang_amd import Plan, State, abi
+    log(f"fused mhd fp64 N={N}")
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    s = State(p, abi.TH_MHD)
+    U, B = o.ic_random(4), o.ic_random(6)
+    s.upload([p.to_dev(a) for a in U + B])
+    o.rk4_mhd(U, B, 0.01, 0.02, 2e-3, steps=2)
+    s.step(2e-3, nu=0.01, eta=0.02, steps=2)
+    out = s.download()
+    for i, ref in enumerate(U + B):
+        assert rel(H(out[i]), ref) < 1e-12, i
+
+
+@pytest.mark.parametrize("N", [(48, 32, 24), (32, 32, 48)])
+def test_fused_flux_mhd_fp64_spilling_shapes(ctx, N):
+    """th_flux_mhd_fused through the energy-transfer z pass (MODE 2) at N3 = 3 * 2^k, where its FP64 build
+    spills, equals the oracle's sphere loop."""
+    from tarang_amd import Plan, State, abi
+    o = Oracle(N)
+    p = Plan(ctx, N)
+    s = State(p, abi.TH_MHD)
+    U, B = o.ic_random(2, kp=3.0), o.ic_random(3, kp=3.0)
+    s.upload([p.to_dev(a) for a in U + B])
+    radii = np.concatenate([np.arange(10, dtype=float), [10000.0]])
+    got = s.flux_mhd(radii)
+    ref = o.flux_mhd(U, B, radii, elsasser=True)
+    for r, k in enumerate(["u2u", "b2b", "u2b_in_out", "u2b_in_in", "b2u_in_out", "b2u_in_in", "u_out_b_out", "zp", "zm"]):
+        sc = np.abs(ref[k]).max()
+        assert np.abs(got[r] - ref[k]).max() <= 1e-11 * sc, k
+
+
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 2e-5)])
 def test_fused_flux_and_shell_transfer(ctx, prec, tol):
     """Pi(R) from shell givers + corner bins equals the reference's sphere loop (Compute_flux) and
