@@ -110,10 +110,20 @@ template <int L> struct ZPad {
     static_assert(line >= need(), "ZPad line stride too short");
 };
 
+// Workgroup barrier that orders LDS accesses only: the fences are scoped to the local address space, so
+// the compiler waits for this wave's LDS operations (lgkmcnt) but not for its outstanding global-memory
+// operations, which keeps loads into an LDS staging buffer (global_load_lds) in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <bool WAVE> __device__ __forceinline__ void line_sync()
 {
     if constexpr (WAVE) wave_sync();
-    else __syncthreads();
+    else lds_barrier();
 }
 
 template <int RADIX, int SIGN, typename R> struct DFT;
