@@ -355,8 +355,9 @@ int th_ensemble_tune(th_ensemble *ens, const double *dt, const double *nu, int n
 /* ---- field files (host arrays only; no GPU is touched) ----
  * th_field_write == IncFluid::Output_field output_field.cc:43-58 (CV_output Cvf.cc:715-720,
  *   Write_data_MPI array_basic.cc:814-858): writes (append = 0) or appends one snapshot, a
- *   "%% Time = t" line and ncomp components of N1 x N2 x (N3/2+1) values "(re,im) " at 12 significant
- *   digits (MY_PRECISION, extern_def_vars.h:78), each component followed by a blank line.
+ *   "%% Time = t" line and ncomp components of N1 x N2 x (N3/2+1) values "(re,im) " in fixed notation with
+ *   12 decimals (field_out_file: ios::fixed, MY_PRECISION = 12, file_operations.cc:229-230,
+ *   extern_def_vars.h:78), each component followed by a blank line.
  * th_field_read == the reading half of IncFluid::Init_cond init_cond_field.cc:47-86 with
  *   field_input_proc = 1 (CV_input Cvf.cc:739-744, Read_data_MPI array_basic.cc:294-335): snapshot
  *   number `snapshot` (0 = first) of such a file into fields; *time = its "%% Time" value, NaN when the
@@ -367,6 +368,20 @@ int th_field_write(const char *path, const int N[3], int precision, int ncomp, c
                    double time, int append);
 int th_field_read(const char *path, const int N[3], int precision, int ncomp, void *const fields[], int snapshot,
                   double *time);
+/* Reduced-size files (field_out_reduced.d; field_input_proc = 2), Nr = {Nr1, Nr2, Nr3} even, 2 <= Nr[d] <= N[d]:
+ * th_field_write_reduced == IncFluid::Output_field_reduced output_field.cc:505-520 (CV_output(file, Nreduced)
+ *   Cvf.cc:732-737, Write_data_MPI(..., Nreduced, ...) array_basic.cc:997-1170): the Nr1 x Nr2 x (Nr3/2+1) modes
+ *   of the reduced grid in its own layout (x rows kx = 0..Nr1/2 then -Nr1/2+1..-1 for FOUR, kx = 0..Nr1-1 for
+ *   SCFT; ky = 0..Nr2/2 then -Nr2/2+1..-1; kz = 0..Nr3/2) taken from the N-grid arrays, each value "(re,im)" with
+ *   no separator at the stream's default 6 significant digits, a blank line after each component.
+ * th_field_read_reduced == the reading half of IncFluid::Init_cond_reduced init_cond_field.cc:292-330
+ *   (CV_input(file, Nreduced) Cvf.cc:747-752, Read_data_MPI(..., Nreduced, ...) array_basic.cc:388-590): those
+ *   modes into zeroed N-grid arrays, the reduced grid's Nyquist planes on both sides (ky = Nr2/2 also at
+ *   -Nr2/2; FOUR: kx = Nr1/2 also at -Nr1/2), as the reference places them.  basis: TH_FOUR or TH_SCFT. */
+int th_field_write_reduced(const char *path, const int N[3], const int Nr[3], int basis, int precision, int ncomp,
+                           const void *const fields[], double time, int append);
+int th_field_read_reduced(const char *path, const int N[3], const int Nr[3], int basis, int precision, int ncomp,
+                          void *const fields[], int snapshot, double *time);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,8 @@ def lib():
         "th_ctx_destroy": (i, [vp]),
         "th_field_write": (i, [C.c_char_p, C.POINTER(C.c_int), i, i, pp, d, i]),
         "th_field_read": (i, [C.c_char_p, C.POINTER(C.c_int), i, i, pp, i, dp]),
+        "th_field_write_reduced": (i, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), i, i, i, pp, d, i]),
+        "th_field_read_reduced": (i, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), i, i, i, pp, i, dp]),
         "th_last_error": (C.c_char_p, []),
         "th_version": (C.c_char_p, []),
         "th_plan_create": (i, [vp, C.POINTER(vp), i, i, i, i, dp, i, i]),
@@ -220,6 +222,39 @@ def field_read(path, N, ncomp, snapshot=0, precision="fp64"):
     t = C.c_double()
     _check(lib().th_field_read(str(path).encode(), n, TH_FP64 if precision == "fp64" else TH_FP32_MIXED, ncomp, ptrs,
                                int(snapshot), C.byref(t)), "th_field_read")
+    return arrs, t.value
+
+
+def field_write_reduced(path, N, Nr, fields, time=0.0, append=False, basis="FOUR"):
+    """th_field_write_reduced: one snapshot of the reference's reduced field file (field_out_reduced.d of
+    Output_field_reduced): the modes of the Nr grid out of fields of shape (N1, N2, N3/2+1)."""
+    import numpy as np
+    arrs = _host_arrays(fields)
+    dt = arrs[0].dtype
+    if dt not in (np.complex128, np.complex64) or any(a.dtype != dt for a in arrs):
+        raise TypeError("field_write_reduced: all fields complex128 or all complex64")
+    shape = (N[0], N[1], N[2] // 2 + 1)
+    if any(a.shape != shape for a in arrs):
+        raise ValueError(f"field_write_reduced: fields must have shape {shape}")
+    prec = TH_FP64 if dt == np.complex128 else TH_FP32_MIXED
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    _check(lib().th_field_write_reduced(str(path).encode(), (C.c_int * 3)(*N), (C.c_int * 3)(*Nr),
+                                        TH_FOUR if basis == "FOUR" else TH_SCFT, prec, len(arrs), ptrs, float(time),
+                                        1 if append else 0), "th_field_write_reduced")
+
+
+def field_read_reduced(path, N, Nr, ncomp, snapshot=0, precision="fp64", basis="FOUR"):
+    """th_field_read_reduced: snapshot `snapshot` of a reduced field file of grid Nr into arrays of grid N;
+    returns (list of ncomp host arrays, time)."""
+    import numpy as np
+    dt = np.complex128 if precision == "fp64" else np.complex64
+    arrs = [np.zeros((N[0], N[1], N[2] // 2 + 1), dt) for _ in range(ncomp)]
+    ptrs = (C.c_void_p * ncomp)(*[a.ctypes.data for a in arrs])
+    t = C.c_double()
+    _check(lib().th_field_read_reduced(str(path).encode(), (C.c_int * 3)(*N), (C.c_int * 3)(*Nr),
+                                       TH_FOUR if basis == "FOUR" else TH_SCFT,
+                                       TH_FP64 if precision == "fp64" else TH_FP32_MIXED, ncomp, ptrs, int(snapshot),
+                                       C.byref(t)), "th_field_read_reduced")
     return arrs, t.value
 
 

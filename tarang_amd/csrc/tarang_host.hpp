@@ -72,6 +72,9 @@ public:
     // field files (th_field_write / th_field_read): Cvf.cc:715-720 and 739-744
     void CV_output(const std::string& path, double time, bool append);
     double CV_input(const std::string& path, int snapshot = 0);   // returns the snapshot's time
+    // reduced-size files (th_field_write_reduced / th_field_read_reduced): Cvf.cc:732-737 and 747-752
+    void CV_output(const std::string& path, const int Nreduced[3], double time, bool append);
+    double CV_input(const std::string& path, const int Nreduced[3], int snapshot = 0);
     int shell_ek_size() const;
     th_plan* plan() const { return plan_; }
     DField V1, V2, V3;
@@ -183,6 +186,8 @@ public:
     void Time_advance();                             // time_advance.cc:60-163
     double Get_dt();                                 // compute_dt.cc:47-59
     void Output_field(const std::string& path);      // output_field.cc:43-58 (appends a snapshot at Tnow)
+    // output_field.cc:505-520: appends the modes of the Nreduced grid (N_out_reduced) at Tnow
+    void Output_field_reduced(const std::string& path, const int Nreduced[3]);
     // force_field_proc 1 (energy/helicity supply) or 2 (constant energy/helicity) with
     // force_double_para (inner_radius, outer_radius, p3, p4) (IncFluid.cc:201-217).  Later steps recompute
     // the force at every stage (Compute_force_TO_rhs single_time_step.cc:351-364): on the fused step when
@@ -194,6 +199,9 @@ public:
     // Init_cond() for field_input_proc = 1, init_cond_field.cc:47-86: read V from the file, then
     // Satisfy_reality_condition_field when asked, and Dealias; returns the snapshot's time (NaN if none)
     double Init_cond(const std::string& path, bool reality = false, int snapshot = 0);
+    // Init_cond_reduced() for field_input_proc = 2, init_cond_field.cc:292-330: V from a file of the Nreduced grid
+    // (N_in_reduced; the other modes 0), then as Init_cond
+    double Init_cond_reduced(const std::string& path, const int Nreduced[3], bool reality = false, int snapshot = 0);
     // Output files, the reference's names and formats (stream precision 12):
     //   Open_output_files file_operations.cc:163-240 (<data_dir>/out/glob.d, spectrum.d, flux.d)
     //   Output_global output_energy.cc:43-81, Output_shell_spectrum :368-394, Output_flux output_ET.cc:45-74

@@ -4,10 +4,12 @@
 //
 //   tarang_run N steps dt nu [tg|random|file=<field_in.d>] [FP64|FP32_MIXED] [fused|plain] [save=<field_out.d>]
 //              [out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR] [et=NSH,SCHEME]
-//              [ring=NRING,NSECT,NCYL,NSLAB,AXIS,SCHEME]
+//              [ring=NRING,NSECT,NCYL,NSLAB,AXIS,SCHEME] [file_reduced=NR1,NR2,NR3] [save_reduced=NR1,NR2,NR3]
 // Output (stdout): "glob t E D" per step, then "spectrum k E(k)" lines and "flux R Pi(R)" lines.
 // file=: Init_cond from a field file (field_input_proc = 1; its "%% Time" sets Tinit); save=: the
 // final field is appended to a field file (Output_field), the restart point of a later run.
+// file_reduced=NR1,NR2,NR3: file= holds a field of that grid (Init_cond_reduced, field_input_proc = 2);
+// save_reduced=NR1,NR2,NR3: save= writes the modes of that grid (Output_field_reduced, field_out_reduced.d).
 // out=DIR (any position after nu): DIR/out/glob.d every step, DIR/out/spectrum.d and flux.d at the end,
 // in the reference's formats (Output_global, Output_shell_spectrum, Output_flux), and DIR/out/shell.d at the end
 // (Output_shell_to_shell).
@@ -41,7 +43,7 @@ int main(int argc, char** argv)
     if (argc < 5) {
         fprintf(stderr, "usage: %s N steps dt nu [tg|random|file=PATH] [FP64|FP32_MIXED] [fused|plain] [save=PATH] "
                         "[out=DIR] [force=PROC,INNER,OUTER,P3,P4[,P5,P6]] [mhd=ETA] [rb=RA,PR] [et=NSH,SCHEME] "
-                        "[ring=NRING,NSECT,NCYL,NSLAB,AXIS,SCHEME]\n",
+                        "[ring=NRING,NSECT,NCYL,NSLAB,AXIS,SCHEME] [file_reduced=NR1,NR2,NR3] [save_reduced=NR1,NR2,NR3]\n",
                 argv[0]);
         return 2;
     }
@@ -56,7 +58,18 @@ int main(int argc, char** argv)
     double eta = -1, Ra = 0, Pr = 0;
     int et_nsh = N / 3 + 2, et_scheme = 1;
     int ring[6] = {0, 0, 0, 0, 3, 1};
+    int nin_red[3] = {0, 0, 0}, nout_red[3] = {0, 0, 0};
     for (int a = 5; a < argc; a++) {
+        if (strncmp(argv[a], "file_reduced=", 13) == 0 &&
+            sscanf(argv[a] + 13, "%d,%d,%d", &nin_red[0], &nin_red[1], &nin_red[2]) != 3) {
+            fprintf(stderr, "tarang_run: file_reduced=NR1,NR2,NR3 expected\n");
+            return 2;
+        }
+        if (strncmp(argv[a], "save_reduced=", 13) == 0 &&
+            sscanf(argv[a] + 13, "%d,%d,%d", &nout_red[0], &nout_red[1], &nout_red[2]) != 3) {
+            fprintf(stderr, "tarang_run: save_reduced=NR1,NR2,NR3 expected\n");
+            return 2;
+        }
         if (strncmp(argv[a], "save=", 5) == 0) save = argv[a] + 5;
         if (strncmp(argv[a], "out=", 4) == 0) outdir = argv[a] + 4;
         if (strncmp(argv[a], "mhd=", 4) == 0) eta = atof(argv[a] + 4);
@@ -124,7 +137,8 @@ int main(int argc, char** argv)
         if (ic == "tg") {
             U.Setup_Taylor_Green_field(1, 1.0);
         } else if (ic.compare(0, 5, "file=") == 0) {
-            const double t = U.Init_cond(ic.substr(5));   // the snapshot's time becomes Tinit
+            // the snapshot's time becomes Tinit
+            const double t = nin_red[0] ? U.Init_cond_reduced(ic.substr(5), nin_red) : U.Init_cond(ic.substr(5));
             if (!isnan(t)) U.Tnow = t;
         } else {
             void* V[3] = {U.V1.data(), U.V2.data(), U.V3.data()};
@@ -181,7 +195,10 @@ int main(int argc, char** argv)
         for (size_t k = 0; k < Ek.size(); k++) printf("spectrum %zu %.15e\n", k, Ek[k]);
         U.Compute_flux();
         for (int s = 1; s <= U.no_spheres; s++) printf("flux %g %.15e\n", U.sphere_radius[s], U.flux_self[s]);
-        if (!save.empty()) U.Output_field(save);
+        if (!save.empty()) {
+            if (nout_red[0]) U.Output_field_reduced(save, nout_red);
+            else U.Output_field(save);
+        }
         if (!outdir.empty()) {
             U.Output_shell_spectrum();
             U.Output_flux();

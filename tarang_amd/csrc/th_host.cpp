@@ -135,6 +135,39 @@ double CVF::CV_input(const std::string& path, int snapshot)
     V3.upload(h[2]);
     return t;
 }
+void CVF::CV_output(const std::string& path, const int Nreduced[3], double time, bool append)
+{
+    std::vector<char> host(3 * elems_ * esize_);
+    const void* h[3];
+    for (int i = 0; i < 3; i++) {
+        DField const& f = i == 0 ? V1 : i == 1 ? V2 : V3;
+        f.download(host.data() + i * elems_ * esize_);
+        h[i] = host.data() + i * elems_ * esize_;
+    }
+    const int N[3] = {grid.N[1], grid.N[2], grid.N[3]};
+    const int prec = esize_ == 16 ? TH_FP64 : TH_FP32_MIXED;
+    check(th_field_write_reduced(path.c_str(), N, Nreduced, grid.basis_type == "SCFT" ? TH_SCFT : TH_FOUR, prec, 3, h, time,
+                                 append ? 1 : 0),
+          "CV_output(Nreduced)");
+}
+
+double CVF::CV_input(const std::string& path, const int Nreduced[3], int snapshot)
+{
+    std::vector<char> host(3 * elems_ * esize_);
+    void* h[3];
+    for (int i = 0; i < 3; i++) h[i] = host.data() + i * elems_ * esize_;
+    const int N[3] = {grid.N[1], grid.N[2], grid.N[3]};
+    const int prec = esize_ == 16 ? TH_FP64 : TH_FP32_MIXED;
+    double t = 0;
+    check(th_field_read_reduced(path.c_str(), N, Nreduced, grid.basis_type == "SCFT" ? TH_SCFT : TH_FOUR, prec, 3, h, snapshot,
+                                &t),
+          "CV_input(Nreduced)");
+    V1.upload(h[0]);
+    V2.upload(h[1]);
+    V3.upload(h[2]);
+    return t;
+}
+
 void CVF::CV_Compute_shell_spectrum(std::vector<double>& result)
 {
     int n = shell_ek_size();
@@ -421,6 +454,16 @@ double IncFluid::Get_dt()
     return dt < Tdt_fixed ? dt : Tdt_fixed;
 }
 void IncFluid::Output_field(const std::string& path) { CV_output(path, Tnow, true); }
+void IncFluid::Output_field_reduced(const std::string& path, const int Nreduced[3]) { CV_output(path, Nreduced, Tnow, true); }
+
+double IncFluid::Init_cond_reduced(const std::string& path, const int Nreduced[3], bool reality, int snapshot)
+{
+    const double t = CV_input(path, Nreduced, snapshot);
+    if (reality)
+        for (DField* f : {&V1, &V2, &V3}) check(th_satisfy_reality(plan_, f->data()), "Satisfy_reality_condition_field");
+    if (grid.alias_switch == "DEALIAS") CV_Dealias();
+    return t;
+}
 
 double IncFluid::Init_cond(const std::string& path, bool reality, int snapshot)
 {

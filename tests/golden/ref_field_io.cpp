@@ -1,10 +1,14 @@
 // Field-file writer and reader built only from the C++ standard library operations the reference
-// uses for field_out.d / field_in.d: a stream with precision 12 (MY_PRECISION), "%% Time = " << t << endl,
-// then per component every value as  stream << std::complex<double> << " "  and endl endl
-// (Write_data_MPI array_basic.cc:814-858); reading is  stream >> std::complex<double>
-// (Read_data_MPI array_basic.cc:294-335).  Values travel to and from this program as raw doubles.
-//   ref_field_io write <values.bin> <out.txt> <n_values_per_comp> <ncomp> <time>
-//   ref_field_io read  <in.txt> <values.bin> <n_values_per_comp> <ncomp>
+// uses for field_out.d / field_in.d: a stream set to ios::fixed with precision 12 (MY_PRECISION,
+// file_operations.cc:229-230), "%% Time = " << t << endl, then per component every value as
+// stream << std::complex<double> << " "  and endl endl (Write_data_MPI array_basic.cc:814-858); reading is
+// stream >> std::complex<double> (Read_data_MPI array_basic.cc:294-335).  The reduced file
+// (field_out_reduced.d) is a stream left at its defaults, every value as  stream << std::complex<double>
+// with no separator, and endl endl per component (Write_data_MPI(..., Nreduced, ...) array_basic.cc:997-1170);
+// the values arrive here already in file order.  Values travel to and from this program as raw doubles.
+//   ref_field_io write    <values.bin> <out.txt> <n_values_per_comp> <ncomp> <time>
+//   ref_field_io writered <values.bin> <out.txt> <n_values_per_comp> <ncomp> <time>
+//   ref_field_io read     <in.txt> <values.bin> <n_values_per_comp> <ncomp>
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
@@ -20,15 +24,22 @@ int main(int argc, char** argv)
     const long n = atol(argv[4]);
     const int ncomp = atoi(argv[5]);
     std::vector<double> v(2 * n * ncomp);
-    if (mode == "write") {
+    if (mode == "write" || mode == "writered") {
+        const bool red = mode == "writered";
         FILE* f = fopen(argv[2], "rb");
         if (!f || fread(v.data(), sizeof(double), v.size(), f) != v.size()) return 3;
         fclose(f);
         std::ofstream out(argv[3]);
-        out.precision(12);
+        if (!red) {
+            out.setf(std::ios::fixed, std::ios::floatfield);
+            out.precision(12);
+        }
         out << "%% Time = " << atof(argv[6]) << std::endl;
         for (int c = 0; c < ncomp; c++) {
-            for (long i = 0; i < n; i++) out << std::complex<double>(v[2 * (c * n + i)], v[2 * (c * n + i) + 1]) << " ";
+            for (long i = 0; i < n; i++) {
+                out << std::complex<double>(v[2 * (c * n + i)], v[2 * (c * n + i) + 1]);
+                if (!red) out << " ";
+            }
             out << std::endl << std::endl;
         }
         return out.good() ? 0 : 4;
