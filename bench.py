@@ -12,6 +12,8 @@ FP32-mixed run against the FP64 run after the same number of steps.
   python bench.py --workload mhd [--n 512]      python bench.py --workload rb [--rb-n 256]
   python bench.py --workload forced [--n 512]   (the N^3 step with energy-supply forcing)
   python bench.py --workload dft16              (16-point DFTs: VALU codelet vs MFMA DFT-as-GEMM)
+The default run also times the MHD 512^3, RB 256^3 and 64-box ensemble workloads and reports the main fields
+of their own lines under other_workloads (--no-extra skips them).
 Prints one JSON line.  Runs on one GPU; --gpus > 1 exits with a message (single-GPU build).
 """
 import argparse
@@ -222,7 +224,7 @@ def traffic_of(table, name, variant):
 ENSEMBLE_SHAPES = [64] * 24 + [96] * 24 + [128] * 16
 
 
-def bench_ensemble(args):
+def bench_ensemble(args, emit=True):
     """The uneven 64-box ensemble (24 x 64^3, 24 x 96^3, 16 x 128^3), FP32-mixed, box b seeded
     100 + b, nu_b ~ U[5e-4, 5e-3] (numpy default_rng(5)), dt_b from CFL 0.5 at t = 0 (FP64).
     value = sum_b N_b^3 x 4 stages per step / wall of the batched, graph-captured ensemble step;
@@ -358,7 +360,13 @@ def bench_ensemble(args):
                      "effective_canonical_GB_s": round(canon_bytes * 4 * args.steps / res["ensemble"] / 1e9, 1)},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    return _emit(line, emit)
+
+
+def _emit(line, emit):
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def _line(metric, value, args, ms_step, config, extra):
@@ -370,7 +378,7 @@ def _line(metric, value, args, ms_step, config, extra):
     return line
 
 
-def bench_mhd(args):
+def bench_mhd(args, emit=True):
     """512^3 incompressible MHD (Elsasser products), FP32-mixed fused step, nu = eta = 1e-3, U seed 2 and
     B seed 3 with the random-phase k^4 exp(-2(k/4)^2) spectrum (E = 0.5 each; B's component along U removed
     so the cross helicity is 0), dt from CFL 0.5 on max(|u|, |b|) at t = 0.  Algorithmic bytes per stage: 90 S (S = N^2 (N/2+1) x 8 B)."""
@@ -423,7 +431,7 @@ def bench_mhd(args):
     roof, canon_stage = roofline(ms, nbytes, N, 8, "mhd")
     passes = {PASS_NAMES[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
                               "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
-    print(json.dumps(_line(
+    return _emit(_line(
         f"Mpts*RKstage/s at {n}^3 MHD (Elsasser), FP32-mixed fused step", value, args, 1e3 * el / args.steps,
         {"workload": f"mhd_{n}^3_rk4_elsasser", "N": n, "nu": nu, "eta": eta, "dt": dt, "cfl": 0.5,
          "ic": "U seed 2, B seed 3 minus its component along U, k^4 exp(-2(k/4)^2), E=0.5 each",
@@ -440,7 +448,7 @@ def bench_mhd(args):
          "stage": {"canonical_GB_per_stage": round(canon_stage / 1e9, 3),
                    "pruned_GB_per_stage": round(float(sum(nbytes)) / 1e9, 3),
                    "GB_s_canonical": round(canon_stage / (float(sum(ms)) * 1e-3) / 1e9, 1), "passes": passes},
-         "cpu_baseline": None})), flush=True)
+         "cpu_baseline": None}), emit)
 
 
 def bench_forced(args):
@@ -489,7 +497,7 @@ def bench_forced(args):
          "roofline": None, "cpu_baseline": None})), flush=True)
 
 
-def bench_rb(args):
+def bench_rb(args, emit=True):
     """Rayleigh-Benard, SCFT basis with the sin/cos axis first, N^3 (default 256^3), kfactor
     (pi, pi/sqrt2, pi/sqrt2), Pr = 1, Ra = 1e7 (PRLARGE/ULARGE scaling), V = 0, temperature
     perturbation amplitude 1e-3 (seed 4), dt = 1e-3 fixed.  The fused RB step (DCT/DST x passes, 9
@@ -537,7 +545,7 @@ def bench_rb(args):
     roof, canon_stage = roofline(ms, nbytes, N, 8, "rb")
     passes = {PASS_NAMES[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
                               "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
-    print(json.dumps(_line(
+    return _emit(_line(
         f"Mpts*RKstage/s at {n}^3 Rayleigh-Benard (SCFT), FP32-mixed fused step", value, args,
         1e3 * el / args.steps,
         {"workload": f"rb_scft_{n}^3_rk4", "N": n, "kfactor": "pi, pi/sqrt2, pi/sqrt2", "Ra": Ra, "Pr": Pr, "dt": dt,
@@ -553,7 +561,7 @@ def bench_rb(args):
          "stage": {"canonical_GB_per_stage": round(canon_stage / 1e9, 3),
                    "pruned_GB_per_stage": round(float(sum(nbytes)) / 1e9, 3),
                    "GB_s_canonical": round(canon_stage / (float(sum(ms)) * 1e-3) / 1e9, 1), "passes": passes},
-         "cpu_baseline": None})), flush=True)
+         "cpu_baseline": None}), emit)
 
 
 def bench_dft16(args):
@@ -621,6 +629,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes (no autotuning)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="ns512: skip the MHD 512^3, RB 256^3 and 64-box ensemble runs summarised under other_workloads")
     ap.add_argument("--variants", default="",
                     help="ns512: launch variants of the five FP32-mixed passes, comma-separated (no autotuning; "
                          "counter runs use this to read each variant)")
@@ -778,7 +788,37 @@ def main():
                   "GB_s_pruned": round(stage_bytes / (stage_ms * 1e-3) / 1e9, 1), "passes": passes},
         "cpu_baseline": cpu,
     }
+    if not args.no_extra and not args.variants:
+        line["other_workloads"] = other_workloads(args)
     print(json.dumps(line), flush=True)
+
+
+def other_workloads(args):
+    """The MHD 512^3, RB 256^3 and 64-box ensemble benchmarks (`--workload mhd | rb | ensemble`, same steps and
+    warm-up, no CPU legs), summarised so that the default run measures every configuration."""
+    import copy
+    import gc
+    import torch
+    gc.collect()
+    torch.cuda.empty_cache()
+    ea = copy.copy(args)
+    ea.no_cpu = True
+    out = {}
+    for name, fn in (("mhd_512", bench_mhd), ("rb_256", bench_rb), ("ensemble_64_boxes", bench_ensemble)):
+        log(f"bench: other workload {name}")
+        d = fn(ea, emit=False)
+        r = d.get("roofline") or {}
+        e = {"metric": d["metric"], "value": d["value"], "unit": d["unit"], "ms_per_step": d["ms_per_step"],
+             "workload": d["config"]["workload"],
+             "roofline": {k: r.get(k) for k in ("kernel", "achieved", "frac", "unit")}}
+        for k in ("flux_mhd_fused", "energy_after", "Nu_fp32", "Nu_fp64", "Nu_plain_fp32", "fp64_gpu", "plain_fp32_gpu",
+                  "sequential", "launch_variants"):
+            if k in d:
+                e[k] = d[k]
+        out[name] = e
+        gc.collect()
+        torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":
