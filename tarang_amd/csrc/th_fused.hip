@@ -347,7 +347,7 @@ template <typename R> static int run_pass(th_state* s, int pass, int var, int st
     }
     case PASS_YFWD: return d_yfwd<R>(g, X3, X1, nout, twy, var, st);
     default:
-        if (ph == TH_RB) return d_sx_fwd_rb<R>(g, X1, S, Vs, acc, twx, p->twd<R>(4 * g.N1), sc, st);
+        if (ph == TH_RB) return d_sx_fwd_rb<R>(g, X1, S, Vs, acc, twx, p->twd<R>(4 * g.N1), sc, var, st);
         return ph == TH_MHD ? d_xfwd<R, 1>(g, X1, S, Vs, acc, twx, sc, var, st)
                             : d_xfwd<R, 0>(g, X1, S, Vs, acc, twx, sc, var, st);
     }

@@ -119,7 +119,7 @@ template <typename R>
 int d_sx_inv(const FG& g, const cplx<R>* S, cplx<R>* A, const cplx<R>* tw, const cplx<R>* tw4, hipStream_t st);
 template <typename R>
 int d_sx_fwd_rb(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
-                const cplx<R>* tw4, const StageCoef* sc, hipStream_t st);
+                const cplx<R>* tw4, const StageCoef* sc, int var, hipStream_t st);
 template <typename R> int d_gather(const FG& g, const cplx<R>* ref, cplx<R>* S, hipStream_t st);
 template <typename R> int d_scatter(const FG& g, const cplx<R>* S, cplx<R>* ref, hipStream_t st);
 

@@ -1059,11 +1059,14 @@ static int launch_sxinv(const FG& g, const cplx<R>* S, cplx<R>* A, const cplx<R>
     TH_CHECK_HIP(hipGetLastError());
     return OK;
 }
-template <typename R, int L>
+// variant 1: half-width tiles (more, smaller workgroups: less idle tail at 256^3, where full tiles leave
+// 1026 workgroups for 256 CUs holding one each)
+template <typename R, int L, int V>
 static int launch_sxfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
                         const cplx<R>* tw4, const StageCoef* sc, hipStream_t st)
 {
-    constexpr int E = xfwd_E(L), P = L / E, T = xfwd_T<R, L, 0>();
+    constexpr int E = xfwd_E(L), P = L / E, T0 = xfwd_T<R, L, 0>();
+    constexpr int T = (V == 1 && L >= 128 && T0 > 1) ? T0 / 2 : T0;
     const int ntz = (g.nkz + T - 1) / T;
     hipLaunchKernelGGL((k_sx_fwd_rb<R, L, P, T>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D, S, Vs, acc, tw,
                        tw4, sc);
@@ -1182,9 +1185,12 @@ int d_sx_inv(const FG& g, const cplx<R>* S, cplx<R>* A, const cplx<R>* tw, const
 }
 template <typename R>
 int d_sx_fwd_rb(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
-                const cplx<R>* tw4, const StageCoef* sc, hipStream_t st)
+                const cplx<R>* tw4, const StageCoef* sc, int var, hipStream_t st)
 {
-#define C_(n) case n: return launch_sxfwd<R, n>(g, D, S, Vs, acc, tw, tw4, sc, st);
+#define C_(n)                                                                                   \
+    case n:                                                                                     \
+        return var == 1 ? launch_sxfwd<R, n, 1>(g, D, S, Vs, acc, tw, tw4, sc, st)             \
+                        : launch_sxfwd<R, n, 0>(g, D, S, Vs, acc, tw, tw4, sc, st);
     switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
 #undef C_
 }
@@ -1219,7 +1225,7 @@ template <typename R> int d_scatter(const FG& g, const cplx<R>* S, cplx<R>* ref,
     template int d_z<R, 3>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, int, hipStream_t);   \
     template int d_sx_inv<R>(const FG&, const cplx<R>*, cplx<R>*, const cplx<R>*, const cplx<R>*, hipStream_t);      \
     template int d_sx_fwd_rb<R>(const FG&, const cplx<R>*, cplx<R>*, cplx<R>*, cplx<R>*, const cplx<R>*,            \
-                                const cplx<R>*, const StageCoef*, hipStream_t);                                       \
+                                const cplx<R>*, const StageCoef*, int, hipStream_t);                                  \
     template int d_yfwd<R>(const FG&, const cplx<R>*, cplx<R>*, int, const cplx<R>*, int, hipStream_t);              \
     template int d_xfwd<R, 0>(const FG&, const cplx<R>*, cplx<R>*, cplx<R>*, cplx<R>*, const cplx<R>*,              \
                               const StageCoef*, int, hipStream_t);                                                     \
