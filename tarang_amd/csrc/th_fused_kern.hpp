@@ -58,6 +58,38 @@ __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restric
         Kz = cz * g.f3;
     }
     cplx<R> v[F::E];
+    if constexpr (MODE == 1) {
+        // component cc of U and B read once: Z+ = U + B into output cc, then Z- = U - B into output 3 + cc
+        const int cc = oc;
+        cplx<R> u[F::E], b[F::E];
+#pragma unroll
+        for (int i = 0; i < F::E; i++) {
+            const int l = F::in_pos(t, i);
+            u[i] = b[i] = czero<R>();
+            if (ok && lkept(g.mx, l)) {
+                const long off = (long)lq(g.mx, l) * g.sx + m;
+                u[i] = S[cc * g.cS + off];
+                b[i] = S[(3 + cc) * g.cS + off];
+            }
+            v[i] = u[i] + b[i];
+        }
+        F::template run<T, false, NoPad, false, 0>(v, lds + c, t, twl);   // lds unused before
+        if (ok) {
+            cplx<R>* out = A + cc * g.cA + m;
+#pragma unroll
+            for (int i = 0; i < F::E; i++) out[(long)F::out_pos(t, i) * g.sx] = v[i];
+        }
+#pragma unroll
+        for (int i = 0; i < F::E; i++) v[i] = u[i] - b[i];
+        // the first transform's last exchange reads precede this one's first writes: barrier first
+        F::template run<T, false, NoPad, false, 1>(v, lds + c, t, twl);
+        if (ok) {
+            cplx<R>* out = A + (3 + cc) * g.cA + m;
+#pragma unroll
+            for (int i = 0; i < F::E; i++) out[(long)F::out_pos(t, i) * g.sx] = v[i];
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < F::E; i++) {
         const int l = F::in_pos(t, i);
@@ -66,10 +98,6 @@ __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restric
             const long off = (long)lq(g.mx, l) * g.sx + m;
             if (MODE == 0) {
                 a = S[oc * g.cS + off];
-            } else if (MODE == 1) {
-                const int cc = oc % 3;
-                cplx<R> u = S[cc * g.cS + off], b = S[(3 + cc) * g.cS + off];
-                a = (oc < 3) ? (u + b) : (u - b);
             } else {
                 double km = kmag_f(kx_four(l, g.N1) * g.f1, Ky, Kz);
                 if ((km > sel.inner) && (km <= sel.outer)) a = S[oc * g.cS + off];
@@ -975,7 +1003,9 @@ static int launch_xinv(const FG& g, const cplx<R>* S, cplx<R>* A, int ncomp, con
 {
     constexpr int E = var_E<L, V>(), P = L / E, T = var_T<R, L, V>();
     const int ntz = (g.nkz + T - 1) / T;
-    hipLaunchKernelGGL((k_fx_inv<R, L, P, T, MODE>), dim3(g.my.nk * ntz, ncomp, g.nbox), dim3(T * P), 0, st, g, S, A, tw, sel);
+    // MHD: one workgroup per component pair (U_c, B_c) writes both Elsasser components
+    const int ny = (MODE == 1) ? ncomp / 2 : ncomp;
+    hipLaunchKernelGGL((k_fx_inv<R, L, P, T, MODE>), dim3(g.my.nk * ntz, ny, g.nbox), dim3(T * P), 0, st, g, S, A, tw, sel);
     TH_CHECK_HIP(hipGetLastError());
     return OK;
 }
