@@ -321,7 +321,34 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
             }
         }
     };
-    if constexpr (sizeof(R) == 4) {
+    if constexpr (sizeof(R) == 4 && E <= 8) {
+        // FP32, 8 or fewer points per thread: a rolled loop over the 3 rotation steps (fewer live
+        // registers than the unrolled form below: more waves per SIMD); rotating the group's registers
+        // by one between steps puts every operand in a fixed register, so no per-product selects.
+        constexpr int G0 = ZP::G0;
+#pragma unroll 1
+        for (int r = 0; r < 3; r++) {
+#pragma unroll
+            for (int j = 0; j < ZP::NPR; j++) {
+                const int yo = ZP::yop(j);
+                cplx<R> w[E];
+#pragma unroll
+                for (int i = 0; i < E; i++) {
+                    const cplx<R> x = u[G0][i];
+                    const cplx<R> y = yo < 0 ? u[G0 - 1 - yo][i] : u[yo][i];
+                    w[i] = mk<R>(x.x * y.x, x.y * y.y);
+                }
+                product_out(w, ZP::pout(r, j));
+            }
+#pragma unroll
+            for (int i = 0; i < E; i++) {
+                const cplx<R> t0 = u[G0][i];
+                u[G0][i] = u[G0 + 1][i];
+                u[G0 + 1][i] = u[G0 + 2][i];
+                u[G0 + 2][i] = t0;
+            }
+        }
+    } else if constexpr (sizeof(R) == 4) {
         // FP32: products one after another, each fenced off for the scheduler, so that the compiler
         // neither interleaves products nor holds two products' lines and twiddles at once (registers,
         // occupancy); every operand is a compile-time register.
