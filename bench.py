@@ -408,7 +408,9 @@ def bench_mhd(args, emit=True):
     s = State(p32, abi.TH_MHD)
     s.upload([a.to(torch.complex64) for a in U + B])
     del U, B
-    var = s.tune(dt, nu=nu, eta=eta, nrep=3) if not args.no_tune else s.variants
+    if args.variants:
+        s.variants = [int(x) for x in args.variants.split(",")]
+    var = s.tune(dt, nu=nu, eta=eta, nrep=3) if not (args.no_tune or args.variants) else s.variants
     s.step(dt, nu=nu, eta=eta, steps=args.warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -632,7 +634,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="ns512: skip the MHD 512^3, RB 256^3 and 64-box ensemble runs summarised under other_workloads")
     ap.add_argument("--variants", default="",
-                    help="ns512: launch variants of the five FP32-mixed passes, comma-separated (no autotuning; "
+                    help="ns512 and mhd: launch variants of the five FP32-mixed passes, comma-separated (no autotuning; "
                          "counter runs use this to read each variant)")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
     ap.add_argument("--calib-copy-mib", type=int, default=0,
