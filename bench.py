@@ -50,6 +50,15 @@ def pass_times(state, dt, nrep=3, **phys):
     return ms, nb
 
 
+def pass_names(nbytes):
+    """The five pass names; a chained step (the x-inverse done inside the previous stage's x-forward, so no bytes
+    of its own) names the x-forward after both."""
+    names = list(PASS_NAMES)
+    if nbytes[0] == 0:
+        names[4] = "x_forward_epilogue_chained_x_inverse"
+    return names
+
+
 def roofline(ms, nbytes, N, esize, physics, traffic=None):
     """Roofline of the dominant (longest) pass: `achieved` = the bytes that kernel must move per launch (its
     dealias-pruned reads and writes of whole compacted arrays, each once) / its HIP-event duration.  The
@@ -61,7 +70,7 @@ def roofline(ms, nbytes, N, esize, physics, traffic=None):
     alg = float(nbytes[k])
     ach = alg / (ms[k] * 1e-3) / 1e9
     canon = CANON_S[physics][k] * S
-    return {"bound": "hbm", "kernel": PASS_NAMES[k], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    return {"bound": "hbm", "kernel": pass_names(nbytes)[k], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_launch": alg,
             "launch_ms": round(float(ms[k]), 4),
             "bytes_model": "compacted kept-mode arrays, each read and written once per launch",
@@ -431,8 +440,8 @@ def bench_mhd(args, emit=True):
     pts = float(n) ** 3
     value = pts * 4 * args.steps / el / 1e6
     roof, canon_stage = roofline(ms, nbytes, N, 8, "mhd")
-    passes = {PASS_NAMES[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
-                              "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
+    passes = {pass_names(nbytes)[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
+                              "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1) if ms[i] > 0 else None} for i in range(5)}
     return _emit(_line(
         f"Mpts*RKstage/s at {n}^3 MHD (Elsasser), FP32-mixed fused step", value, args, 1e3 * el / args.steps,
         {"workload": f"mhd_{n}^3_rk4_elsasser", "N": n, "nu": nu, "eta": eta, "dt": dt, "cfl": 0.5,
@@ -545,8 +554,8 @@ def bench_rb(args, emit=True):
     value = pts * 4 * args.steps / el / 1e6
     v64 = pts * 4 * args.steps / res["fp64"][0] / 1e6
     roof, canon_stage = roofline(ms, nbytes, N, 8, "rb")
-    passes = {PASS_NAMES[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
-                              "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
+    passes = {pass_names(nbytes)[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
+                              "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1) if ms[i] > 0 else None} for i in range(5)}
     return _emit(_line(
         f"Mpts*RKstage/s at {n}^3 Rayleigh-Benard (SCFT), FP32-mixed fused step", value, args,
         1e3 * el / args.steps,
@@ -735,13 +744,13 @@ def main():
     # roofline of the dominant kernel (the longest pass)
     traffic = load_traffic(f"ns{args.n}_fp32mixed")
     k = int(np.argmax(ms))
-    roof, canon_stage = roofline(ms, nbytes, N, 8, "ns", traffic_of(traffic, PASS_NAMES[k], var32[k]))
+    roof, canon_stage = roofline(ms, nbytes, N, 8, "ns", traffic_of(traffic, pass_names(nbytes)[k], var32[k]))
     roof["traffic_source"] = ("profiles/traffic_ns512_fp32mixed.md (same launch variant)" if roof["traffic"] is not None
                               else "no counter reading for this launch variant")
     stage_bytes = float(sum(nbytes))
     stage_ms = float(sum(ms))
-    passes = {PASS_NAMES[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
-                              "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1)} for i in range(5)}
+    passes = {pass_names(nbytes)[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
+                              "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1) if ms[i] > 0 else None} for i in range(5)}
 
     cpu = None
     if not args.no_cpu:

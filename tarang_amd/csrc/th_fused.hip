@@ -94,6 +94,11 @@ struct th_state {
     hipEvent_t cev = nullptr;    // recorded after the last upload from hcoef
     bool cev_live = false;
     int var[NPASS] = {0, 0, 0, 0, 0};   // launch-shape variant per pass (autotuner)
+    // Chained step (NS, unforced, one box): each stage's x-forward also does the next stage's x-inverse, so a
+    // step runs 4 passes per stage; X1 must then hold the x-inverse of S when a step starts (chain_ok), which a
+    // single x-inverse launch restores after anything else has used the buffers or changed S.
+    bool chain_ok = false;
+    bool graph_chained = false;
     bool forced = false;         // energy / helicity forcing (th_state_set_force)
     ForceBand fb{};
     double* fbox = nullptr;      // projected force P(F) of the stage input, 3 complex FP64 per band-box mode
@@ -199,6 +204,7 @@ extern "C" int th_state_destroy(th_state* s)
 static int state_upload(th_state* s, int box, int comp, const void* src, int src_on_host, hipStream_t st)
 {
     if (!s || !src || comp < 0 || comp >= s->ncomp || box < 0 || box >= s->nbox) { set_error("th_state_upload: bad args"); return E_ARG; }
+    s->chain_ok = false;   // S changes
     th_plan* p = s->plan;
     TH_CHECK_HIP(hipSetDevice(p->ctx->device));
     const void* dsrc = src;
@@ -316,7 +322,7 @@ static int set_coefs(th_state* s, const double* dt, const double* nu, const doub
 // Pass `pass` of RK stage `stage` (1..4) for every box of s, launch-shape variant var.  Stage 1
 // reads the state, later stages the stage field; pass 5 writes the stage field / accumulator
 // (and the state in stage 4).
-template <typename R> static int run_pass(th_state* s, int pass, int var, int stage, hipStream_t st)
+template <typename R> static int run_pass(th_state* s, int pass, int var, int stage, hipStream_t st, bool chain = false)
 {
     th_plan* p = s->plan;
     const FG& g = s->fg;
@@ -349,7 +355,7 @@ template <typename R> static int run_pass(th_state* s, int pass, int var, int st
     default:
         if (ph == TH_RB) return d_sx_fwd_rb<R>(g, X1, S, Vs, acc, twx, p->twd<R>(4 * g.N1), sc, var, st);
         return ph == TH_MHD ? d_xfwd<R, 1>(g, X1, S, Vs, acc, twx, sc, var, st)
-                            : d_xfwd<R, 0>(g, X1, S, Vs, acc, twx, sc, var, st);
+                            : d_xfwd<R, 0>(g, X1, S, Vs, acc, twx, sc, var, st, chain);
     }
 }
 
@@ -459,8 +465,9 @@ __global__ void k_force_post(ForceBand b, FG g, const StageCoef* __restrict__ co
 }
 
 // One RK4 step of every box in s, as 4 stages x 5 pass launches on stream st (forced: plus the
-// two band kernels around each stage).
-template <typename R> static int enqueue_step(th_state* s, hipStream_t st)
+// two band kernels around each stage); chained: 4 passes per stage, the x-inverse done by the previous
+// stage's x-forward (the first stage's by the step before, or by the priming launch in th_step).
+template <typename R> static int enqueue_step(th_state* s, hipStream_t st, bool chained = false)
 {
     const long nb = (long)s->fb.nx * s->fb.ny * s->fb.nz;
     const dim3 fgrid((unsigned)((nb + 255) / 256));
@@ -470,7 +477,10 @@ template <typename R> static int enqueue_step(th_state* s, hipStream_t st)
             hipLaunchKernelGGL((k_force_pre<R>), fgrid, dim3(256), 0, st, s->fb, s->fg, src, s->fbox);
             TH_CHECK_HIP(hipGetLastError());
         }
-        for (int k = 0; k < NPASS; k++) TRY(run_pass<R>(s, k, s->var[k], stage, st));
+        for (int k = 0; k < NPASS; k++) {
+            if (chained && k == PASS_XINV) continue;
+            TRY(run_pass<R>(s, k, s->var[k], stage, st, chained));
+        }
         if (s->forced) {
             hipLaunchKernelGGL((k_force_post<R>), fgrid, dim3(256), 0, st, s->fb, s->fg,
                                (const StageCoef*)(s->dcoef + (size_t)(stage - 1) * s->nbox), (cplx<R>*)s->S,
@@ -484,6 +494,7 @@ template <typename R> static int enqueue_step(th_state* s, hipStream_t st)
 extern "C" int th_state_set_force(th_state* s, const th_force* f)
 {
     if (!s) { set_error("th_state_set_force: null state"); return E_ARG; }
+    s->chain_ok = false;
     TH_CHECK_HIP(hipSetDevice(s->plan->ctx->device));
     // steps already enqueued read the force buffer and the captured graph: let them finish first
     TH_CHECK_HIP(hipStreamSynchronize(s->plan->ctx->stream));
@@ -547,6 +558,13 @@ static int ensure_twiddles(th_plan* p, const FG& g, hipStream_t st)
     return OK;
 }
 
+// whether th_step runs this state chained (one box of NS without forcing, N1 with a chained x-forward)
+static bool step_chained(const th_state* s)
+{
+    return s->physics == TH_NS && !s->forced && s->nbox == 1 &&
+           (s->plan->prec ? d_xfwd_chainable<float>(s->fg) : d_xfwd_chainable<double>(s->fg));
+}
+
 extern "C" int th_step(th_plan* p, th_state* s, double dt, const th_phys* ph, int nsteps)
 {
     if (!p || !s || !ph || s->plan != p || nsteps < 0) { set_error("th_step: bad args"); return E_ARG; }
@@ -557,17 +575,28 @@ extern "C" int th_step(th_plan* p, th_state* s, double dt, const th_phys* ph, in
     double d0, d1;
     diss_of(s, ph, &d0, &d1);
     TRY(set_coefs(s, &dt, &d0, &d1, st));
+    const bool chained = step_chained(s);
+    if (s->gexec && s->graph_chained != chained) {
+        (void)hipGraphExecDestroy(s->gexec);
+        s->gexec = nullptr;
+        if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
+    }
     if (!s->gexec) {
         TH_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int r = p->prec ? enqueue_step<float>(s, st) : enqueue_step<double>(s, st);
+        int r = p->prec ? enqueue_step<float>(s, st, chained) : enqueue_step<double>(s, st, chained);
         hipGraph_t graph = nullptr;
         hipError_t e = hipStreamEndCapture(st, &graph);
         if (r) { if (graph) (void)hipGraphDestroy(graph); return r; }
         if (e != hipSuccess) { set_error("hipStreamEndCapture: %s", hipGetErrorString(e)); return E_HIP; }
         s->graph = graph;
         TH_CHECK_HIP(hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0));
+        s->graph_chained = chained;
     }
+    if (chained && !s->chain_ok && nsteps > 0)   // X1 <- x-inverse of S, the first stage's input
+        TRY(p->prec ? run_pass<float>(s, PASS_XINV, s->var[PASS_XINV], 1, st)
+                    : run_pass<double>(s, PASS_XINV, s->var[PASS_XINV], 1, st));
     for (int i = 0; i < nsteps; i++) TH_CHECK_HIP(hipGraphLaunch(s->gexec, st));
+    if (nsteps > 0) s->chain_ok = chained;
     return OK;
 }
 
@@ -728,6 +757,7 @@ template <typename R> static int flux_mhd_impl(th_state* s, const double* radii,
 
 extern "C" int th_flux_mhd_fused(th_plan* p, th_state* s, const double* radii, int nsph, double* out)
 {
+    if (s) s->chain_ok = false;   // the pipelines use X1..X3
     if (!p || !s || s->plan != p || !radii || nsph < 2 || !out) { set_error("th_flux_mhd_fused: bad args"); return E_ARG; }
     if (s->physics != TH_MHD) { set_error("th_flux_mhd_fused: MHD state required"); return E_UNSUPPORTED; }
     if (radii[nsph] < 1e3) { set_error("th_flux_mhd_fused: radii[nsph] must be the infinite radius"); return E_ARG; }
@@ -737,6 +767,7 @@ extern "C" int th_flux_mhd_fused(th_plan* p, th_state* s, const double* radii, i
 
 extern "C" int th_flux_shell_tr(th_plan* p, th_state* s, const double* radii, int nsh, double* Pi, double* T)
 {
+    if (s) s->chain_ok = false;   // the pipelines use X1..X3
     if (!p || !s || s->plan != p || !radii || nsh < 2) { set_error("th_flux_shell_tr: bad args"); return E_ARG; }
     if (s->physics != TH_NS) { set_error("th_flux_shell_tr: NS state required"); return E_UNSUPPORTED; }
     if (radii[nsh] < 1e3) { set_error("th_flux_shell_tr: radii[nsh] must be the infinite radius"); return E_ARG; }
@@ -746,7 +777,8 @@ extern "C" int th_flux_shell_tr(th_plan* p, th_state* s, const double* radii, in
 
 // ------------------------------------------------------------------------------------------
 // Per-pass timing of one fused RK stage with HIP events on the plan's stream, and each pass's
-// algorithmic HBM bytes (compulsory reads + writes of kept data; padding excluded).
+// algorithmic HBM bytes (compulsory reads + writes of kept data; padding excluded).  A chained state is timed as
+// th_step runs it: the x-inverse once beforehand (not timed: 0 ms, 0 bytes) and the chained x-forward.
 // ------------------------------------------------------------------------------------------
 template <typename R> static int profile_impl(th_state* s, double dt, double nu, double eta, int stage, int nrep,
                                               double* ms, double* bytes)
@@ -762,13 +794,16 @@ template <typename R> static int profile_impl(th_state* s, double dt, double nu,
         if (hipMalloc(&keep, sbytes) != hipSuccess) { set_error("th_profile_passes: out of device memory"); return E_NOMEM; }
         TH_CHECK_HIP(hipMemcpyAsync(keep, s->S, sbytes, hipMemcpyDeviceToDevice, st));
     }
+    const bool chained = step_chained(s);
+    s->chain_ok = false;   // the passes below leave other data in X1 (and stage 4 restores S)
+    if (chained) TRY(run_pass<R>(s, PASS_XINV, s->var[PASS_XINV], stage, st));
     hipEvent_t ev[6];
     for (int i = 0; i < 6; i++) TH_CHECK_HIP(hipEventCreate(&ev[i]));
     double tot[5] = {0, 0, 0, 0, 0};
     for (int rep = 0; rep < nrep; rep++) {
         TH_CHECK_HIP(hipEventRecord(ev[0], st));
         for (int k = 0; k < NPASS; k++) {
-            TRY(run_pass<R>(s, k, s->var[k], stage, st));
+            if (!(chained && k == PASS_XINV)) TRY(run_pass<R>(s, k, s->var[k], stage, st, chained));
             TH_CHECK_HIP(hipEventRecord(ev[k + 1], st));
         }
         TH_CHECK_HIP(hipEventSynchronize(ev[5]));
@@ -791,10 +826,15 @@ template <typename R> static int profile_impl(th_state* s, double dt, double nu,
     int state_rw = (stage == 1 || stage == 4) ? 3 : 4;   // V0 (+acc) read, Vs / acc / S written
     double extra = 0;
     if (s->physics == TH_RB && stage > 1) extra = 2 * cS;  // V1 and T of the stage field (force)
-    const double b[5] = {e * (nst * cS + nin * cA), e * (nin * cA + nin * cB), e * (nin * cB + nout * cB),
-                         e * (nout * cB + nout * cA), e * (nout * cA + state_rw * nst * cS + extra)};
+    double b[5] = {e * (nst * cS + nin * cA), e * (nin * cA + nin * cB), e * (nin * cB + nout * cB),
+                   e * (nout * cB + nout * cA), e * (nout * cA + state_rw * nst * cS + extra)};
+    if (chained) {
+        // no stage-field write (stages 1-3), the next stage's A lines written instead
+        b[PASS_XFWD] = e * (nout * cA + (state_rw - (stage < 4 ? 1 : 0)) * nst * cS + nin * cA);
+        b[PASS_XINV] = 0;
+    }
     for (int k = 0; k < 5; k++) {
-        ms[k] = tot[k] / nrep;
+        ms[k] = (chained && k == PASS_XINV) ? 0.0 : tot[k] / nrep;
         bytes[k] = b[k];
     }
     return OK;
@@ -826,14 +866,18 @@ template <typename R> static int tune_impl(th_state* s, int nrep, hipStream_t st
     TH_CHECK_HIP(hipEventCreate(&e0));
     TH_CHECK_HIP(hipEventCreate(&e1));
     int r = OK;
+    // a chained state's x-forward is timed as th_step runs it (with the next x-inverse folded in; it writes its
+    // A lines over D, which the trials after it read as they would D: the work is the same)
+    const bool chained = step_chained(s);
     for (int k = 0; k < NPASS && !r; k++) r = run_pass<R>(s, k, s->var[k], 2, st);   // live inputs for every pass
     for (int k = 0; k < NPASS && !r; k++) {
         int best = s->var[k];
         double tbest = 1e300;
+        const bool ch = chained && k == PASS_XFWD;
         for (int v = 0; v < pass_nvar(k) && !r; v++) {
-            if ((r = run_pass<R>(s, k, v, 2, st))) break;
+            if ((r = run_pass<R>(s, k, v, 2, st, ch))) break;
             if (hipEventRecord(e0, st) != hipSuccess) { r = E_HIP; break; }
-            for (int i = 0; i < nrep && !r; i++) r = run_pass<R>(s, k, v, 2, st);
+            for (int i = 0; i < nrep && !r; i++) r = run_pass<R>(s, k, v, 2, st, ch);
             if (r || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { r = r ? r : E_HIP; break; }
             float t = 0;
             (void)hipEventElapsedTime(&t, e0, e1);
@@ -850,6 +894,7 @@ template <typename R> static int tune_impl(th_state* s, int nrep, hipStream_t st
 
 static int state_tune(th_state* s, const double* dt, const double* nu, const double* eta, int nrep, hipStream_t st)
 {
+    s->chain_ok = false;   // the trials run every pass on the state's buffers
     th_plan* p = s->plan;
     TRY(ensure_twiddles(p, s->fg, st));
     TRY(set_coefs(s, dt, nu, eta, st));

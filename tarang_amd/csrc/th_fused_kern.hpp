@@ -435,26 +435,34 @@ __device__ __forceinline__ Dc dadd(const Dc& a, const Dc& b) { return dmk(a.x + 
 // One RK4 stage update of a field value V0 (state), stage field Vs and accumulator tot
 // (single_time_step.cc:54-86 with a,b,c of time_advance.cc:120-153).
 // s0 and a0 are the values of S and acc at this point (a0 unused in stage 1), loaded by the caller.
-template <typename R>
-__device__ __forceinline__ void rk_store(int stage, double dt, double eh, double e1, const Dc& rhs, const cplx<R>& s0,
-                                         const cplx<R>& a0, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc)
+// Returns the stage's output, the field the next stage transforms: the stage field (stages 1-3) or the new
+// state (stage 4), as stored.  WVS = false leaves the stage field unwritten (its only reader, the next x-inverse,
+// then runs on the returned value in registers).
+template <typename R, bool WVS = true>
+__device__ __forceinline__ cplx<R> rk_store(int stage, double dt, double eh, double e1, const Dc& rhs, const cplx<R>& s0,
+                                            const cplx<R>& a0, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc)
 {
     const Dc V0 = dl(s0);
+    cplx<R> out;
     if (stage == 1) {
         Dc v = dadd(V0, dsc(rhs, 0.5 * dt));
-        *Vs = ds<R>(dsc(v, eh));
+        out = ds<R>(dsc(v, eh));
         *acc = ds<R>(dsc(dsc(rhs, e1), dt / 6));
     } else if (stage == 2) {
-        *Vs = ds<R>(dadd(dsc(V0, eh), dsc(rhs, 0.5 * dt)));
+        out = ds<R>(dadd(dsc(V0, eh), dsc(rhs, 0.5 * dt)));
         *acc = ds<R>(dadd(dl(a0), dsc(dsc(rhs, eh), dt / 3)));
     } else if (stage == 3) {
         Dc v = dadd(dsc(V0, eh), dsc(rhs, 1.0 * dt));
-        *Vs = ds<R>(dsc(v, eh));
+        out = ds<R>(dsc(v, eh));
         *acc = ds<R>(dadd(dl(a0), dsc(dsc(rhs, eh), dt / 3)));
     } else {
         Dc a = dadd(dl(a0), dsc(rhs, dt / 6));
-        *S = ds<R>(dadd(dsc(V0, e1), a));
+        out = ds<R>(dadd(dsc(V0, e1), a));
+        *S = out;
+        return out;
     }
+    if constexpr (WVS) *Vs = out;
+    return out;
 }
 template <typename R>
 __device__ __forceinline__ void rk_update(int stage, double dt, double eh, double e1, const Dc& rhs, cplx<R>* S,
@@ -473,8 +481,12 @@ template <int L, int P, int E, int RL> constexpr int xnext(int i)
     return -1;
 }
 
-template <typename R, int L, int P, int T, int MODE>
-__global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 0 && sizeof(R) == 4 && L % 3 != 0) ? 4 : 1))) void k_fx_fwd(FG g, const cplx<R>* __restrict__ D, cplx<R>* S, cplx<R>* Vs,
+// CHAIN (NS): the epilogue also performs the next stage's x-inverse: the stage output (stage field, or the new
+// state in stage 4) stays in the accumulator registers in place of N and is transformed back along x, and the
+// A lines are written over this tile's columns of D (components 0-2; every product of the tile has been read
+// by then, and no other workgroup reads these columns); the stage field itself is not written.
+template <typename R, int L, int P, int T, int MODE, bool CHAIN = false>
+__global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 0 && sizeof(R) == 4 && L % 3 != 0) ? 4 : 1))) void k_fx_fwd(FG g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs,
                                                   cplx<R>* acc, const cplx<R>* __restrict__ tw,
                                                   const StageCoef* __restrict__ coefs)
 {
@@ -562,7 +574,9 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
             for (int a = 0; a < NA; a++) acc_iK(Nacc[a][i], al[a] * Kx + be[a], v[i]);
         }
     }
-    if (!ok) return;
+    static_assert(!CHAIN || (MODE == 0 && F::R0 == F::RL), "chained x-inverse: NS, first radix = last radix");
+    if constexpr (!CHAIN)
+        if (!ok) return;
     const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
     // Epilogue, software-pipelined over the kept registers: the state and accumulator values of
     // the next register are loaded before this register's results are stored.  Vector-memory
@@ -588,23 +602,35 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
             a0[a] = accr[a * g.cS + idx];
         }
     };
+    // CHAIN: no prefetch (its registers would push the 1024-thread FP32 kernel past 128 VGPRs; the chained
+    // epilogue stores no stage field for a load to wait on in stages 1-3)
     cplx<R> nS[NC], nA[NC];
-    fetch(xnext<L, P, E, F::RL>(-1), nS, nA);
+    if (!CHAIN && ok) fetch(xnext<L, P, E, F::RL>(-1), nS, nA);
     const double kyz2 = dKy * dKy + dKz * dKz;
     const double eyh = exp(sc.fh[0] * kyz2), ey1 = exp(sc.f1[0] * kyz2);
     const double eyhb = (MODE == 1) ? exp(sc.fh[1] * kyz2) : 0.0, ey1b = (MODE == 1) ? exp(sc.f1[1] * kyz2) : 0.0;
 #pragma unroll
     for (int i = 0; i < E; i++) {
-        if (xdrop<L, P, E, F::RL>(i)) continue;
-        cplx<R> s0[NC], a0[NC];
+        if (xdrop<L, P, E, F::RL>(i)) {
+            if constexpr (CHAIN)
 #pragma unroll
-        for (int a = 0; a < NC; a++) {
-            s0[a] = nS[a];
-            a0[a] = nA[a];
+                for (int a = 0; a < 3; a++) Nacc[a][i] = czero<R>();
+            continue;
         }
-        if (xnext<L, P, E, F::RL>(i) >= 0) fetch(xnext<L, P, E, F::RL>(i), nS, nA);
+        cplx<R> s0[NC], a0[NC];
+        if constexpr (CHAIN) {
+            if (ok) fetch(i, s0, a0);
+        } else {
+#pragma unroll
+            for (int a = 0; a < NC; a++) {
+                s0[a] = nS[a];
+                a0[a] = nA[a];
+            }
+            if (ok && xnext<L, P, E, F::RL>(i) >= 0) fetch(xnext<L, P, E, F::RL>(i), nS, nA);
+        }
         const int l = F::out_pos(t, i);
-        if (lkept(g.mx, l)) {
+        cplx<R> keep[3] = {czero<R>(), czero<R>(), czero<R>()};
+        if (ok && lkept(g.mx, l)) {
             const double dKx = kx_four(l, g.N1) * g.f1;
             const double K2 = ksq_f(dKx, dKy, dKz);
             const long idx = row_idx(i);
@@ -634,8 +660,8 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
             const double eh = exl[0][l] * eyh, e1 = exl[1][l] * ey1;
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                rk_store<R>(stage, sc.dt, eh, e1, rhs[a], s0[a], a0[a], S + a * g.cS + idx, Vs + a * g.cS + idx,
-                            acc + a * g.cS + idx);
+                keep[a] = rk_store<R, !CHAIN>(stage, sc.dt, eh, e1, rhs[a], s0[a], a0[a], S + a * g.cS + idx,
+                                              Vs + a * g.cS + idx, acc + a * g.cS + idx);
             if constexpr (MODE == 1) {
                 const double ehb = exl[2][l] * eyhb, e1b = exl[3][l] * ey1b;
 #pragma unroll
@@ -648,7 +674,26 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
                 }
             }
         }
+        if constexpr (CHAIN)
+#pragma unroll
+            for (int a = 0; a < 3; a++) Nacc[a][i] = keep[a];   // rows outside the kept modes: 0
         __builtin_amdgcn_sched_barrier(0);   // one register at a time: bounds the live state values
+    }
+    if constexpr (CHAIN) {
+        // the next stage's x-inverse (k_fx_inv MODE 0): register i holds row F::out_pos(t, i), which is
+        // FI::in_pos(t, i) since the schedule's first and last radix agree
+        using FI = LineFFT<R, L, P, +1>;
+        cplx<R>* Aout = const_cast<cplx<R>*>(D);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {   // unrolled: Nacc[a] must stay a compile-time register
+            FI::template run<T, false, NoPad, true>(Nacc[a], lds + c, t, twl + opaque_zero());
+            if (ok) {
+                cplx<R>* out = Aout + a * g.cA + m;
+#pragma unroll
+                for (int i = 0; i < E; i++) out[(long)FI::out_pos(t, i) * g.sx] = Nacc[a][i];
+            }
+            __builtin_amdgcn_sched_barrier(0);   // one line at a time
+        }
     }
 }
 
@@ -1080,7 +1125,7 @@ template <typename R, int L, int MODE> constexpr int xfwd_T()
     return (t * P > 512) ? (512 / P < 1 ? 1 : 512 / P) : t;
 }
 
-template <typename R, int L, int MODE, int V>
+template <typename R, int L, int MODE, int V, bool CHAIN = false>
 static int launch_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw,
                        const StageCoef* sc, hipStream_t st)
 {
@@ -1088,8 +1133,18 @@ static int launch_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, c
     // variant 1: half-width tiles; variant 2: double-width tiles (full 128-byte rows at 512 lines)
     constexpr int T = (V == 1 && L >= 128 && T0 > 1) ? T0 / 2 : (V == 2) ? 2 * T0 : T0;
     const int ntz = (g.nkz + T - 1) / T;
-    hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D, S, Vs, acc, tw,
-                       sc);
+    if constexpr (CHAIN) {
+        if constexpr (MODE == 0 && LineFFT<R, L, P, -1>::R0 == LineFFT<R, L, P, -1>::RL) {
+            hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE, true>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D,
+                               S, Vs, acc, tw, sc);
+        } else {
+            set_error("chained x-forward: no chained kernel for N1 = %d", L);
+            return E_UNSUPPORTED;
+        }
+    } else {
+        hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D, S, Vs, acc,
+                           tw, sc);
+    }
     TH_CHECK_HIP(hipGetLastError());
     return OK;
 }
@@ -1178,14 +1233,27 @@ static int yfwd_v(const FG& g, const cplx<R>* Cc, cplx<R>* D, int nc, const cplx
 }
 template <typename R, int MODE, int L>
 static int xfwd_v(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw, const StageCoef* sc,
-                  int var, hipStream_t st)
+                  int var, hipStream_t st, bool chain)
 {
+    if (chain) {
+        if constexpr (L >= 128)
+            if (var == 1) return launch_xfwd<R, L, MODE, 1, true>(g, D, S, Vs, acc, tw, sc, st);
+        if constexpr (L >= 128 && sizeof(R) == 4 && MODE == 0 && L % 3 != 0 && 2 * xfwd_T<R, L, MODE>() * (L / xfwd_E(L)) <= 1024)
+            if (var == 2) return launch_xfwd<R, L, MODE, 2, true>(g, D, S, Vs, acc, tw, sc, st);
+        return launch_xfwd<R, L, MODE, 0, true>(g, D, S, Vs, acc, tw, sc, st);
+    }
     if constexpr (L >= 128)
         if (var == 1) return launch_xfwd<R, L, MODE, 1>(g, D, S, Vs, acc, tw, sc, st);
     // 1024-thread workgroups hold a wave to 128 VGPRs: only the FP32 NS kernels fit (124)
     if constexpr (L >= 128 && sizeof(R) == 4 && MODE == 0 && L % 3 != 0 && 2 * xfwd_T<R, L, MODE>() * (L / xfwd_E(L)) <= 1024)
         if (var == 2) return launch_xfwd<R, L, MODE, 2>(g, D, S, Vs, acc, tw, sc, st);
     return launch_xfwd<R, L, MODE, 0>(g, D, S, Vs, acc, tw, sc, st);
+}
+// whether the chained x-forward exists for this line length and variant (NS, first radix = last radix)
+template <typename R, int L, int MODE, int V> constexpr bool xfwd_chainable()
+{
+    constexpr int E = xfwd_E(L), P = L / E;
+    return MODE == 0 && LineFFT<R, L, P, -1>::R0 == LineFFT<R, L, P, -1>::RL;
 }
 
 template <typename R, int MODE>
@@ -1218,10 +1286,18 @@ int d_yfwd(const FG& g, const cplx<R>* Cc, cplx<R>* D, int nc, const cplx<R>* tw
 }
 template <typename R, int MODE>
 int d_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw, const StageCoef* sc,
-           int var, hipStream_t st)
+           int var, hipStream_t st, bool chain)
 {
-#define C_(n) case n: return xfwd_v<R, MODE, n>(g, D, S, Vs, acc, tw, sc, var, st);
+#define C_(n) case n: return xfwd_v<R, MODE, n>(g, D, S, Vs, acc, tw, sc, var, st, chain);
     switch (g.N1) { TH_FLENS(C_) default: set_error("fused: N1 %d unsupported", g.N1); return E_UNSUPPORTED; }
+#undef C_
+}
+template <typename R>
+bool d_xfwd_chainable(const FG& g)
+{
+    // every variant of a line length uses the same points per thread, hence the same radix schedule
+#define C_(n) case n: return xfwd_chainable<R, n, 0, 0>();
+    switch (g.N1) { TH_FLENS(C_) default: return false; }
 #undef C_
 }
 template <typename R>
@@ -1285,9 +1361,10 @@ template <typename R> int d_scatter(const FG& g, const cplx<R>* S, cplx<R>* ref,
                                 const cplx<R>*, const StageCoef*, int, hipStream_t);                                  \
     template int d_yfwd<R>(const FG&, const cplx<R>*, cplx<R>*, int, const cplx<R>*, int, hipStream_t);              \
     template int d_xfwd<R, 0>(const FG&, const cplx<R>*, cplx<R>*, cplx<R>*, cplx<R>*, const cplx<R>*,              \
-                              const StageCoef*, int, hipStream_t);                                                     \
+                              const StageCoef*, int, hipStream_t, bool);                                               \
     template int d_xfwd<R, 1>(const FG&, const cplx<R>*, cplx<R>*, cplx<R>*, cplx<R>*, const cplx<R>*,              \
-                              const StageCoef*, int, hipStream_t);                                                     \
+                              const StageCoef*, int, hipStream_t, bool);                                               \
+    template bool d_xfwd_chainable<R>(const FG&);                                                                      \
     template int d_xfwd_et<R>(const FG&, const cplx<R>*, const cplx<R>*, const cplx<R>*, const double*, int, int,    \
                               double*, int*, hipStream_t);                                                             \
     template int d_gather<R>(const FG&, const cplx<R>*, cplx<R>*, hipStream_t);                                        \
