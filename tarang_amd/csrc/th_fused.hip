@@ -558,10 +558,10 @@ static int ensure_twiddles(th_plan* p, const FG& g, hipStream_t st)
     return OK;
 }
 
-// whether th_step runs this state chained (one box of NS without forcing, N1 with a chained x-forward)
+// whether a step runs this state chained (NS without forcing, N1 with a chained x-forward; any number of boxes)
 static bool step_chained(const th_state* s)
 {
-    return s->physics == TH_NS && !s->forced && s->nbox == 1 &&
+    return s->physics == TH_NS && !s->forced &&
            (s->plan->prec ? d_xfwd_chainable<float>(s->fg) : d_xfwd_chainable<double>(s->fg));
 }
 
@@ -1079,7 +1079,8 @@ extern "C" int th_ensemble_step(th_ensemble* e, const double* dt, const double* 
             th_ensemble::Cls& k = e->cls[c];
             he = hipStreamWaitEvent(k.stream, e->fork, 0);
             if (he != hipSuccess) break;
-            r = e->prec ? enqueue_step<float>(k.st, k.stream) : enqueue_step<double>(k.st, k.stream);
+            const bool ch = step_chained(k.st);
+            r = e->prec ? enqueue_step<float>(k.st, k.stream, ch) : enqueue_step<double>(k.st, k.stream, ch);
             if (r) break;
             he = hipEventRecord(k.done, k.stream);
             if (he == hipSuccess) he = hipStreamWaitEvent(st, k.done, 0);
@@ -1095,7 +1096,15 @@ extern "C" int th_ensemble_step(th_ensemble* e, const double* dt, const double* 
         e->graph = graph;
         TH_CHECK_HIP(hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0));
     }
+    // chained classes need the x-inverse of their state in X1 before the first stage (see th_step)
+    if (nsteps > 0)
+        for (th_ensemble::Cls& k : e->cls)
+            if (step_chained(k.st) && !k.st->chain_ok)
+                TRY(e->prec ? run_pass<float>(k.st, PASS_XINV, k.st->var[PASS_XINV], 1, st)
+                            : run_pass<double>(k.st, PASS_XINV, k.st->var[PASS_XINV], 1, st));
     for (int i = 0; i < nsteps; i++) TH_CHECK_HIP(hipGraphLaunch(e->gexec, st));
+    if (nsteps > 0)
+        for (th_ensemble::Cls& k : e->cls) k.st->chain_ok = step_chained(k.st);
     return OK;
 }
 

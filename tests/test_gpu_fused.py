@@ -258,6 +258,47 @@ def test_fused_rb_scft_fp32_mixed(ctx):
         assert rel(H(out[i]), ref) < 2e-5, i
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_chained_step_after_other_uses_of_the_state(ctx, prec, tol):
+    """The NS step chains each stage's x-inverse into the previous x-forward, so its first stage needs the
+    x-inverse of S in the pipeline buffer.  Every operation that changes S or uses those buffers in between
+    (an upload, a flux evaluation, per-pass timing, tuning, forcing switched on and off) must leave the next
+    step equal to the oracle's."""
+    from tarang_amd import Plan, State
+    N = (32, 32, 32)
+    o = Oracle(N)
+    p = Plan(ctx, N, precision=prec)
+    s = State(p)
+    V = o.ic_random(13)
+    s.upload([p.to_dev(v) for v in V])
+    nu, dt = 0.01, 2e-3
+    radii = np.concatenate([np.arange(8, dtype=float), [10000.0]])
+
+    def step_and_check(what):
+        s.step(dt, nu=nu, steps=2)
+        o.rk4_ns(V, nu, dt, steps=2)
+        out = s.download()
+        for i in range(3):
+            assert rel(H(out[i]), V[i]) < tol, (what, i)
+
+    step_and_check("first")
+    W = o.ic_random(14)                      # a new state uploaded between steps
+    V[:] = [w.copy() for w in W]
+    s.upload([p.to_dev(v) for v in V])
+    step_and_check("upload")
+    s.flux_shell_tr(radii)
+    step_and_check("flux")
+    for stage in (1, 2, 3, 4):
+        s.profile_passes(dt, nu=nu, stage=stage, nrep=1)
+    step_and_check("profile")
+    s.tune(dt, nu=nu, nrep=1)
+    step_and_check("tune")
+    s.set_force(1, 1.5, 2.5, 0.0, 0.0)       # energy supply 0: the forced step is the free step
+    step_and_check("forced, zero supply")
+    s.set_force(0, 0, 0, 0, 0)
+    step_and_check("forcing off")
+
+
 def test_plan_destroyed_before_its_state(ctx):
     """th_plan_destroy on a plan with a live state defers the release to th_state_destroy (the order
     a garbage collector may pick for a reference cycle), and the state keeps working meanwhile."""
