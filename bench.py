@@ -299,11 +299,19 @@ def bench_ensemble(args, emit=True):
         torch.cuda.synchronize()
         res[name] = time.perf_counter() - t0
         log(f"bench: {name} {args.steps} steps {res[name]:.4f} s")
-    # the batched and the sequential runs advanced every box by the same steps from the same IC
-    worst = 0.0
+    # the batched and the sequential runs advanced every box by the same steps from the same IC: bit-identical
+    # where a class runs the same launch variants as its single boxes (the two are tuned separately; other
+    # variants round differently in FP32)
+    worst, same_var_diff = 0.0, 0.0
+    var_cls = {}
     for b in (0, 24, 48, nb - 1):
         a, r = ens.download(b), singles[b].download()
-        worst = max(worst, max(float((x - y).abs().max()) for x, y in zip(a, r)))
+        d = max(float((x - y).abs().max()) for x, y in zip(a, r))
+        worst = max(worst, d)
+        ve, vs = ens.variants(b), singles[b].variants
+        var_cls[shapes[b][0]] = {"ensemble": ve, "sequential": vs}
+        if ve == vs:
+            same_var_diff = max(same_var_diff, d)
     pts = float(sum(N[0] * N[1] * N[2] for N in shapes))
     value = pts * 4 * args.steps / res["ensemble"] / 1e6
     seqv = pts * 4 * args.steps / res["sequential"] / 1e6
@@ -358,7 +366,9 @@ def bench_ensemble(args, emit=True):
                    "ic": "random-phase k^4 exp(-2(k/4)^2), E=0.5, seed 100+b", "precision": "fp32-mixed"},
         "sequential": {"value": round(seqv, 1), "ms_per_step": round(1e3 * res["sequential"] / args.steps, 3),
                        "speedup_ensemble_over_sequential": round(value / seqv, 3),
-                       "max_abs_diff_sampled_boxes": worst},
+                       "max_abs_diff_sampled_boxes": worst,
+                       "max_abs_diff_sampled_boxes_same_variants": same_var_diff,
+                       "launch_variants_by_size": var_cls},
         "roofline": {"bound": "hbm", "kernel": "whole_ensemble_step", "achieved": round(ach, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_stage": stage_bytes,

@@ -349,8 +349,10 @@ int th_ensemble_plan(th_ensemble *ens, int box, th_plan **plan);
 int th_ensemble_upload(th_ensemble *ens, int box, int comp, const void *src, int src_on_host);
 int th_ensemble_download(th_ensemble *ens, int box, int comp, void *dst, int dst_on_host);
 int th_ensemble_step(th_ensemble *ens, const double *dt, const double *nu, int nsteps);
-/* th_tune for every size class of the ensemble (per-box dt and nu as in th_ensemble_step) */
+/* th_tune for every size class of the ensemble (per-box dt and nu as in th_ensemble_step);
+ * th_ensemble_variants reads the launch variants of box `box`'s size class (th_state_variants' order) */
 int th_ensemble_tune(th_ensemble *ens, const double *dt, const double *nu, int nrep);
+int th_ensemble_variants(const th_ensemble *ens, int box, int var[5]);
 
 /* ---- field files (host arrays only; no GPU is touched) ----
  * th_field_write == IncFluid::Output_field output_field.cc:43-58 (CV_output Cvf.cc:715-720,

@@ -144,6 +144,7 @@ def lib():
         "th_state_set_variants": (i, [vp, C.POINTER(C.c_int)]),
         "th_state_variants": (i, [vp, C.POINTER(C.c_int)]),
         "th_ensemble_tune": (i, [vp, dp, dp, i]),
+        "th_ensemble_variants": (i, [vp, i, C.POINTER(C.c_int)]),
         "th_ensemble_create": (i, [vp, C.POINTER(vp), i, C.POINTER(C.c_int), i]),
         "th_ensemble_destroy": (i, [vp]),
         "th_ensemble_plan": (i, [vp, i, C.POINTER(vp)]),
@@ -881,6 +882,12 @@ class Ensemble:
         p = self.plan(0)
         p._pre()
         p._post(lib().th_ensemble_tune(self.h, dtp, nup, nrep), "th_ensemble_tune")
+
+    def variants(self, box):
+        """Launch variants of box `box`'s size class (x-inverse, y-inverse, z, y-forward, x-forward)."""
+        v = (C.c_int * 5)()
+        _check(lib().th_ensemble_variants(self.h, int(box), v), "th_ensemble_variants")
+        return list(v)
 
     def step(self, dt, nu, steps=1, sync=True):
         dt_, dtp = _dbl(dt)

@@ -1108,6 +1108,12 @@ extern "C" int th_ensemble_step(th_ensemble* e, const double* dt, const double* 
     return OK;
 }
 
+extern "C" int th_ensemble_variants(const th_ensemble* e, int box, int* var)
+{
+    if (!e || !var || box < 0 || box >= e->nbox) { set_error("th_ensemble_variants: bad args"); return E_ARG; }
+    return th_state_variants(e->cls[e->cls_of[box]].st, var);
+}
+
 extern "C" int th_ensemble_tune(th_ensemble* e, const double* dt, const double* nu, int nrep)
 {
     if (!e || !dt || !nu || nrep < 1) { set_error("th_ensemble_tune: bad args"); return E_ARG; }

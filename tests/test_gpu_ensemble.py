@@ -148,3 +148,37 @@ def test_ensemble_uneven_64_boxes_fp32(ctx):
         p = ens.plan(b)
         E1 = p.total_energy(ens.download(b))
         assert 0 < E1 < E0[b], (b, E0[b], E1)
+
+
+def test_tuned_ensemble_matches_single_boxes_on_its_variants(ctx):
+    """After th_ensemble_tune, th_ensemble_variants names each size class's launch variants; single-box
+    states set to those variants (th_state_set_variants) reproduce the tuned, chained ensemble bit for
+    bit; a box index outside the ensemble is refused."""
+    from tarang_amd import Ensemble, Plan, State
+    shapes = [(128,) * 3, (64,) * 3, (128,) * 3, (96,) * 3]
+    rng = np.random.default_rng(11)
+    nu = rng.uniform(5e-4, 5e-3, len(shapes))
+    dt = rng.uniform(1e-3, 2e-3, len(shapes))
+    log("tuned ensemble fp32 shapes=" + str(shapes))
+    ens = Ensemble(ctx, shapes, precision="fp32")
+    for b in range(len(shapes)):
+        ens.upload(b, ens.plan(b).ic_random(200 + b))
+    ens.tune(dt, nu, nrep=1)
+    singles = []
+    for b, N in enumerate(shapes):
+        var = ens.variants(b)
+        assert len(var) == 5 and all(v >= 0 for v in var), var
+        assert var == ens.variants([i for i, M in enumerate(shapes) if M == N][0])
+        s = State(Plan(ctx, N, precision="fp32"))
+        s.upload(ens.plan(b).ic_random(200 + b))
+        s.variants = var
+        assert s.variants == var
+        singles.append(s)
+    ens.step(dt, nu, steps=2)
+    for b, s in enumerate(singles):
+        s.step(float(dt[b]), nu=float(nu[b]), steps=2)
+        got, ref = ens.download(b), s.download()
+        for i in range(3):
+            assert np.array_equal(H(got[i]), H(ref[i])), (b, i, ens.variants(b))
+    with pytest.raises(RuntimeError):
+        ens.variants(len(shapes))
