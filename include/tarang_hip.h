@@ -323,7 +323,8 @@ int th_dft16_probe(th_ctx *ctx, int method, int sign, const void *x, void *y, lo
 /* ---- launch-shape autotuner (no counterpart in the reference, whose transforms are planned by
  * FFTW_MEASURE in Init_fftw_plan_FOUR four_tr.cc:44-98) ----
  * th_tune times every launch-shape variant of the five fused passes on the state's own buffers
- * (one RK stage, nrep launches each), keeps the fastest in the state and records it in the
+ * (one RK stage; three interleaved rounds of nrep launches per variant, the best round counting),
+ * keeps the fastest in the state and records it in the
  * context's tune cache (th_ctx_create's tune_cache_path), from which later states of the same
  * workload start.  var (may be NULL) receives the chosen variant per pass: x-inverse, y-inverse,
  * fused z, y-forward, x-forward.  The state's fields are unchanged.
