@@ -94,7 +94,7 @@ struct th_state {
     hipEvent_t cev = nullptr;    // recorded after the last upload from hcoef
     bool cev_live = false;
     int var[NPASS] = {0, 0, 0, 0, 0};   // launch-shape variant per pass (autotuner)
-    // Chained step (NS, unforced, one box): each stage's x-forward also does the next stage's x-inverse, so a
+    // Chained step (NS or MHD, unforced): each stage's x-forward also does the next stage's x-inverse, so a
     // step runs 4 passes per stage; X1 must then hold the x-inverse of S when a step starts (chain_ok), which a
     // single x-inverse launch restores after anything else has used the buffers or changed S.
     bool chain_ok = false;
@@ -354,7 +354,7 @@ template <typename R> static int run_pass(th_state* s, int pass, int var, int st
     case PASS_YFWD: return d_yfwd<R>(g, X3, X1, nout, twy, var, st);
     default:
         if (ph == TH_RB) return d_sx_fwd_rb<R>(g, X1, S, Vs, acc, twx, p->twd<R>(4 * g.N1), sc, var, st);
-        return ph == TH_MHD ? d_xfwd<R, 1>(g, X1, S, Vs, acc, twx, sc, var, st)
+        return ph == TH_MHD ? d_xfwd<R, 1>(g, X1, S, Vs, acc, twx, sc, var, st, chain)
                             : d_xfwd<R, 0>(g, X1, S, Vs, acc, twx, sc, var, st, chain);
     }
 }
@@ -558,10 +558,11 @@ static int ensure_twiddles(th_plan* p, const FG& g, hipStream_t st)
     return OK;
 }
 
-// whether a step runs this state chained (NS without forcing, N1 with a chained x-forward; any number of boxes)
+// whether a step runs this state chained (NS or MHD without forcing, N1 with a chained x-forward; any number of
+// boxes)
 static bool step_chained(const th_state* s)
 {
-    return s->physics == TH_NS && !s->forced &&
+    return (s->physics == TH_NS || s->physics == TH_MHD) && !s->forced &&
            (s->plan->prec ? d_xfwd_chainable<float>(s->fg) : d_xfwd_chainable<double>(s->fg));
 }
 

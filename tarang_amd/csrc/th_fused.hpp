@@ -112,8 +112,9 @@ int d_yfwd(const FG& g, const cplx<R>* Cc, cplx<R>* D, int nc, const cplx<R>* tw
 template <typename R, int MODE>
 int d_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, const cplx<R>* tw, const StageCoef* sc,
            int var, hipStream_t st, bool chain = false);
-// chain: the NS x-forward also transforms its stage output back along x and writes the next stage's A lines over
-// D's components 0-2 (the stage field is not written); d_xfwd_chainable says whether N1 has that kernel
+// chain: the x-forward also transforms its stage output back along x and writes the next stage's A lines over
+// D's first components (NS: U in 0-2; MHD: Z+ in 0-2, Z- in 3-5; the stage field is not written);
+// d_xfwd_chainable says whether N1 has that kernel
 template <typename R>
 bool d_xfwd_chainable(const FG& g);
 template <typename R>

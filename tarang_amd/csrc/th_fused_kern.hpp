@@ -481,10 +481,11 @@ template <int L, int P, int E, int RL> constexpr int xnext(int i)
     return -1;
 }
 
-// CHAIN (NS): the epilogue also performs the next stage's x-inverse: the stage output (stage field, or the new
+// CHAIN: the epilogue also performs the next stage's x-inverse: the stage output (stage field, or the new
 // state in stage 4) stays in the accumulator registers in place of N and is transformed back along x, and the
-// A lines are written over this tile's columns of D (components 0-2; every product of the tile has been read
-// by then, and no other workgroup reads these columns); the stage field itself is not written.
+// A lines are written over this tile's columns of D (every product of the tile has been read by then, and no
+// other workgroup reads these columns); the stage field itself is not written.  NS: components 0-2 = U.
+// MHD: the Elsasser lines of k_fx_inv MODE 1, Z+ = U + B into components 0-2 and Z- = U - B into 3-5.
 template <typename R, int L, int P, int T, int MODE, bool CHAIN = false>
 __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 0 && sizeof(R) == 4 && L % 3 != 0) ? 4 : 1))) void k_fx_fwd(FG g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs,
                                                   cplx<R>* acc, const cplx<R>* __restrict__ tw,
@@ -574,7 +575,7 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
             for (int a = 0; a < NA; a++) acc_iK(Nacc[a][i], al[a] * Kx + be[a], v[i]);
         }
     }
-    static_assert(!CHAIN || (MODE == 0 && F::R0 == F::RL), "chained x-inverse: NS, first radix = last radix");
+    static_assert(!CHAIN || (MODE <= 1 && F::R0 == F::RL), "chained x-inverse: NS or MHD, first radix = last radix");
     if constexpr (!CHAIN)
         if (!ok) return;
     const double dKy = ky_of(ll(g.my, cy), g.N2) * g.f2, dKz = cz * g.f3;
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
         if (xdrop<L, P, E, F::RL>(i)) {
             if constexpr (CHAIN)
 #pragma unroll
-                for (int a = 0; a < 3; a++) Nacc[a][i] = czero<R>();
+                for (int a = 0; a < NA; a++) Nacc[a][i] = czero<R>();
             continue;
         }
         cplx<R> s0[NC], a0[NC];
@@ -630,6 +631,7 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
         }
         const int l = F::out_pos(t, i);
         cplx<R> keep[3] = {czero<R>(), czero<R>(), czero<R>()};
+        cplx<R> keepb[3] = {czero<R>(), czero<R>(), czero<R>()};   // MHD: B's stage output
         if (ok && lkept(g.mx, l)) {
             const double dKx = kx_four(l, g.N1) * g.f1;
             const double K2 = ksq_f(dKx, dKy, dKz);
@@ -669,23 +671,34 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
                     Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
                     Dc nb = dsc(dmk(np.x - nm.x, np.y - nm.y), 0.5 * g.norm);
                     Dc rb = dmk(-nb.x, -nb.y);
-                    rk_store<R>(stage, sc.dt, ehb, e1b, rb, s0[3 + a], a0[3 + a], S + (3 + a) * g.cS + idx,
-                                Vs + (3 + a) * g.cS + idx, acc + (3 + a) * g.cS + idx);
+                    keepb[a] = rk_store<R, !CHAIN>(stage, sc.dt, ehb, e1b, rb, s0[3 + a], a0[3 + a],
+                                                   S + (3 + a) * g.cS + idx, Vs + (3 + a) * g.cS + idx,
+                                                   acc + (3 + a) * g.cS + idx);
                 }
             }
         }
-        if constexpr (CHAIN)
+        if constexpr (CHAIN) {
+            // rows outside the kept modes: 0
+            if constexpr (MODE == 0) {
 #pragma unroll
-            for (int a = 0; a < 3; a++) Nacc[a][i] = keep[a];   // rows outside the kept modes: 0
+                for (int a = 0; a < 3; a++) Nacc[a][i] = keep[a];
+            } else {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    Nacc[a][i] = keep[a] + keepb[a];
+                    Nacc[3 + a][i] = keep[a] - keepb[a];
+                }
+            }
+        }
         __builtin_amdgcn_sched_barrier(0);   // one register at a time: bounds the live state values
     }
     if constexpr (CHAIN) {
-        // the next stage's x-inverse (k_fx_inv MODE 0): register i holds row F::out_pos(t, i), which is
+        // the next stage's x-inverse (k_fx_inv MODE 0 / 1): register i holds row F::out_pos(t, i), which is
         // FI::in_pos(t, i) since the schedule's first and last radix agree
         using FI = LineFFT<R, L, P, +1>;
         cplx<R>* Aout = const_cast<cplx<R>*>(D);
 #pragma unroll
-        for (int a = 0; a < 3; a++) {   // unrolled: Nacc[a] must stay a compile-time register
+        for (int a = 0; a < NA; a++) {   // unrolled: Nacc[a] must stay a compile-time register
             FI::template run<T, false, NoPad, true>(Nacc[a], lds + c, t, twl + opaque_zero());
             if (ok) {
                 cplx<R>* out = Aout + a * g.cA + m;
@@ -1134,7 +1147,7 @@ static int launch_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, c
     constexpr int T = (V == 1 && L >= 128 && T0 > 1) ? T0 / 2 : (V == 2) ? 2 * T0 : T0;
     const int ntz = (g.nkz + T - 1) / T;
     if constexpr (CHAIN) {
-        if constexpr (MODE == 0 && LineFFT<R, L, P, -1>::R0 == LineFFT<R, L, P, -1>::RL) {
+        if constexpr (MODE <= 1 && LineFFT<R, L, P, -1>::R0 == LineFFT<R, L, P, -1>::RL) {
             hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE, true>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D,
                                S, Vs, acc, tw, sc);
         } else {
@@ -1249,11 +1262,11 @@ static int xfwd_v(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R
         if (var == 2) return launch_xfwd<R, L, MODE, 2>(g, D, S, Vs, acc, tw, sc, st);
     return launch_xfwd<R, L, MODE, 0>(g, D, S, Vs, acc, tw, sc, st);
 }
-// whether the chained x-forward exists for this line length and variant (NS, first radix = last radix)
+// whether the chained x-forward exists for this line length and variant (NS or MHD, first radix = last radix)
 template <typename R, int L, int MODE, int V> constexpr bool xfwd_chainable()
 {
     constexpr int E = xfwd_E(L), P = L / E;
-    return MODE == 0 && LineFFT<R, L, P, -1>::R0 == LineFFT<R, L, P, -1>::RL;
+    return MODE <= 1 && LineFFT<R, L, P, -1>::R0 == LineFFT<R, L, P, -1>::RL;
 }
 
 template <typename R, int MODE>
@@ -1295,7 +1308,7 @@ int d_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc,
 template <typename R>
 bool d_xfwd_chainable(const FG& g)
 {
-    // every variant of a line length uses the same points per thread, hence the same radix schedule
+    // every variant of a line length, NS or MHD, uses the same points per thread, hence the same radix schedule
 #define C_(n) case n: return xfwd_chainable<R, n, 0, 0>();
     switch (g.N1) { TH_FLENS(C_) default: return false; }
 #undef C_

@@ -299,6 +299,44 @@ def test_chained_step_after_other_uses_of_the_state(ctx, prec, tol):
     step_and_check("forcing off")
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+@pytest.mark.parametrize("N", [(32, 32, 32), (64, 16, 16)])
+def test_chained_mhd_step_after_other_uses_of_the_state(ctx, prec, tol, N):
+    """The MHD step chains the next stage's Elsasser x-inverse (Z+ = U + B, Z- = U - B) into each x-forward;
+    an upload, the fused MHD fluxes, per-pass timing and tuning between steps must leave the next step equal
+    to the oracle's Time_advance(IncVF&)."""
+    from tarang_amd import Plan, State, abi
+    log(f"chained mhd {prec} N={N}")
+    o = Oracle(N)
+    p = Plan(ctx, N, precision=prec)
+    s = State(p, abi.TH_MHD)
+    U, B = o.ic_random(21), o.ic_random(22)
+    s.upload([p.to_dev(a) for a in U + B])
+    nu, eta, dt = 0.01, 0.02, 2e-3
+    radii = np.concatenate([np.arange(8, dtype=float), [10000.0]])
+
+    def step_and_check(what):
+        s.step(dt, nu=nu, eta=eta, steps=2)
+        o.rk4_mhd(U, B, nu, eta, dt, steps=2)
+        out = s.download()
+        for i, ref in enumerate(U + B):
+            assert rel(H(out[i]), ref) < tol, (what, i)
+
+    step_and_check("first")
+    U2, B2 = o.ic_random(23), o.ic_random(24)
+    U[:] = [u.copy() for u in U2]
+    B[:] = [b.copy() for b in B2]
+    s.upload([p.to_dev(a) for a in U + B])
+    step_and_check("upload")
+    s.flux_mhd(radii)
+    step_and_check("flux")
+    for stage in (1, 2, 3, 4):
+        s.profile_passes(dt, nu=nu, eta=eta, stage=stage, nrep=1)
+    step_and_check("profile")
+    s.tune(dt, nu=nu, eta=eta, nrep=1)
+    step_and_check("tune")
+
+
 def test_plan_destroyed_before_its_state(ctx):
     """th_plan_destroy on a plan with a live state defers the release to th_state_destroy (the order
     a garbage collector may pick for a reference cycle), and the state keeps working meanwhile."""
