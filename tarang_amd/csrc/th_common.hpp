@@ -25,6 +25,13 @@ template <typename R> __device__ __forceinline__ cplx<R> add_iwd(cplx<R> s, cplx
 }
 template <typename R> __host__ __device__ __forceinline__ cplx<R> mul_mi(cplx<R> a) { return mk<R>(a.y, -a.x); }     // -i*a
 template <typename R> __host__ __device__ __forceinline__ cplx<R> czero() { return mk<R>(R(0), R(0)); }
+// base + a 32-bit byte offset: with a wave-uniform base this is the SGPR-base + 32-bit VGPR-offset form of a
+// global access (no 64-bit address per lane).  Offsets within one component of a fused-state array stay below
+// 4 GB for every supported grid (N <= 512 per axis).
+template <typename T> __device__ __forceinline__ T* at_byte(T* base, unsigned off)
+{
+    return (T*)((char*)base + off);
+}
 
 // Wavenumber helpers (trunk/lib/four/four_inline.h:88-138, trunk/lib/scft/scft_inline.h:86)
 __host__ __device__ __forceinline__ int kx_four(int l1, int N1) { return (l1 <= N1 / 2) ? l1 : l1 - N1; }

@@ -509,7 +509,6 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     const int cy = tile / ntz;
     const int cz = (tile % ntz) * T + c;
     const bool ok = cz < g.nkz;
-    const long m = (long)cy * g.KZP + cz;
     const StageCoef sc = coefs[box];
     for (int l = threadIdx.x; l < L; l += T * P) {
         const double kx = kx_four(l, g.N1) * g.f1, kx2 = kx * kx;
@@ -538,6 +537,11 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     // reads column cz = tile * T + c of row in_pos(t, i) = t + in_pos(0, i).
     const long mu = (long)cy * g.KZP + (long)(tile % ntz) * T;
     const int lane_off = c + t * (int)g.sx;
+    const unsigned lane_b = (unsigned)lane_off * (unsigned)sizeof(cplx<R>);
+    // Where T divides the KZP padding (KZP is nkz rounded up to 16 FP32 / 8 FP64 elements) a column past nkz
+    // (!ok) still lies inside its row, so the loads are unconditional: such a column's values only ever reach its
+    // own lines, which are never stored.  Wider tiles (short lines) keep the guard.
+    constexpr bool PADOK = (sizeof(R) == 4 ? 16 : 8) % T == 0;
 #pragma unroll 1
     for (int p = 0; p < NP; p++) {
         cplx<R> v[E];
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
 #pragma unroll
         for (int i = 0; i < E; i++) {
             const cplx<R>* row = src + (long)F::in_pos(0, i) * g.sx;
-            v[i] = ok ? row[lane_off] : czero<R>();
+            v[i] = (PADOK || ok) ? *at_byte(row, lane_b) : czero<R>();
         }
         F::template run<T, false, NoPad, true>(v, lds + c, t, twl + opaque_zero());
         // contributions (a1 <- K_k1) and (a2 <- K_k2) of product p; k = 0 means Kx (per row)
@@ -585,22 +589,25 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     // for the store to complete.
     constexpr int NC = (MODE == 1) ? 6 : 3;
     const int stage = sc.stage;
-    auto row_idx = [&](int i) -> long {
+    auto row_u = [&](int i) -> long {
         // compact row of l = t + l0: the kept lanes of one register all sit on one side of the
-        // dealiased gap (P < L/3), so the row shift is uniform per register
+        // dealiased gap (P < L/3), so the row shift is uniform per register; the lane adds lane_off
         const int l0 = F::out_pos(0, i);
-        return (long)(l0 - (l0 >= g.mx.h ? g.mx.N - g.mx.nk : 0)) * g.sx + mu + lane_off;
+        return (long)(l0 - (l0 >= g.mx.h ? g.mx.N - g.mx.nk : 0)) * g.sx;
     };
     // Unconditional loads (a branch around them would pull the first use, and its wait, up to the
     // load): lanes in the dealiased gap read row 0 of their column, and stage 1, which has no
     // accumulator yet, reads S a second time (a cache hit); neither value is used.
     const cplx<R>* accr = stage > 1 ? acc : S;
+    // byte offset of register i's row for this lane from the tile's column base (S + a cS + mu); gap lanes
+    // (not loaded from, see above) take row 0
+    auto row_b = [&](int i) -> unsigned { return lane_b + (unsigned)(row_u(i) * (long)sizeof(cplx<R>)); };
     auto fetch = [&](int i, cplx<R> (&s0)[NC], cplx<R> (&a0)[NC]) {
-        const long idx = lkept(g.mx, F::out_pos(t, i)) ? row_idx(i) : mu + lane_off;
+        const unsigned b = lkept(g.mx, F::out_pos(t, i)) ? row_b(i) : lane_b;
 #pragma unroll
         for (int a = 0; a < NC; a++) {
-            s0[a] = S[a * g.cS + idx];
-            a0[a] = accr[a * g.cS + idx];
+            s0[a] = *at_byte(S + a * g.cS + mu, b);
+            a0[a] = *at_byte(accr + a * g.cS + mu, b);
         }
     };
     // CHAIN: no prefetch (its registers would push the 1024-thread FP32 kernel past 128 VGPRs; the chained
@@ -635,7 +642,7 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
         if (ok && lkept(g.mx, l)) {
             const double dKx = kx_four(l, g.N1) * g.f1;
             const double K2 = ksq_f(dKx, dKy, dKz);
-            const long idx = row_idx(i);
+            const unsigned bk = row_b(i);
             Dc Nu[3];
             if (MODE == 0) {
 #pragma unroll
@@ -662,8 +669,8 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
             const double eh = exl[0][l] * eyh, e1 = exl[1][l] * ey1;
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                keep[a] = rk_store<R, !CHAIN>(stage, sc.dt, eh, e1, rhs[a], s0[a], a0[a], S + a * g.cS + idx,
-                                              Vs + a * g.cS + idx, acc + a * g.cS + idx);
+                keep[a] = rk_store<R, !CHAIN>(stage, sc.dt, eh, e1, rhs[a], s0[a], a0[a], at_byte(S + a * g.cS + mu, bk),
+                                              at_byte(Vs + a * g.cS + mu, bk), at_byte(acc + a * g.cS + mu, bk));
             if constexpr (MODE == 1) {
                 const double ehb = exl[2][l] * eyhb, e1b = exl[3][l] * ey1b;
 #pragma unroll
@@ -672,8 +679,9 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
                     Dc nb = dsc(dmk(np.x - nm.x, np.y - nm.y), 0.5 * g.norm);
                     Dc rb = dmk(-nb.x, -nb.y);
                     keepb[a] = rk_store<R, !CHAIN>(stage, sc.dt, ehb, e1b, rb, s0[3 + a], a0[3 + a],
-                                                   S + (3 + a) * g.cS + idx, Vs + (3 + a) * g.cS + idx,
-                                                   acc + (3 + a) * g.cS + idx);
+                                                   at_byte(S + (3 + a) * g.cS + mu, bk),
+                                                   at_byte(Vs + (3 + a) * g.cS + mu, bk),
+                                                   at_byte(acc + (3 + a) * g.cS + mu, bk));
                 }
             }
         }
@@ -701,9 +709,10 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
         for (int a = 0; a < NA; a++) {   // unrolled: Nacc[a] must stay a compile-time register
             FI::template run<T, false, NoPad, true>(Nacc[a], lds + c, t, twl + opaque_zero());
             if (ok) {
-                cplx<R>* out = Aout + a * g.cA + m;
+                // row FI::out_pos(t, i) = t + FI::out_pos(0, i) of column mu + c: a uniform row base plus lane_b
+                cplx<R>* out = Aout + a * g.cA + mu;
 #pragma unroll
-                for (int i = 0; i < E; i++) out[(long)FI::out_pos(t, i) * g.sx] = Nacc[a][i];
+                for (int i = 0; i < E; i++) *at_byte(out + (long)FI::out_pos(0, i) * g.sx, lane_b) = Nacc[a][i];
             }
             __builtin_amdgcn_sched_barrier(0);   // one line at a time
         }
