@@ -498,9 +498,10 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     constexpr int NX = (MODE == 1) ? 4 : 2;
     __shared__ cplx<R> lds[2 * L * T];   // two exchange buffers (LineFFT DBUF)
     __shared__ cplx<R> twl[L];
-    // exp(c Kx^2) per row for the integrating factors c = fh[0], f1[0] (MHD: and fh[1], f1[1]):
-    // exp(c K^2) = exp(c Kx^2) exp(c (Ky^2 + Kz^2)), the second factor once per thread
+    // exp(c Kx^2) per row and exp(c (Ky^2 + Kz^2)) per column for the integrating factors c = fh[0], f1[0]
+    // (MHD: and fh[1], f1[1]): exp(c K^2) = exp(c Kx^2) exp(c (Ky^2 + Kz^2))
     __shared__ double exl[NX][L];
+    __shared__ double eyz[NX][T];
     stage_tw<R, L>(twl, tw);
     const int c = threadIdx.x % T, t = threadIdx.x / T;
     const int ntz = (g.nkz + T - 1) / T;
@@ -517,6 +518,15 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
         if constexpr (MODE == 1) {
             exl[2][l] = exp(sc.fh[1] * kx2);
             exl[3][l] = exp(sc.f1[1] * kx2);
+        }
+    }
+    if (t == 0) {   // one thread per column
+        const double ky = ky_of(ll(g.my, cy), g.N2) * g.f2, kz = cz * g.f3, kyz2 = ky * ky + kz * kz;
+        eyz[0][c] = exp(sc.fh[0] * kyz2);
+        eyz[1][c] = exp(sc.f1[0] * kyz2);
+        if constexpr (MODE == 1) {
+            eyz[2][c] = exp(sc.fh[1] * kyz2);
+            eyz[3][c] = exp(sc.f1[1] * kyz2);
         }
     }
     __syncthreads();
@@ -614,9 +624,6 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     // epilogue stores no stage field for a load to wait on in stages 1-3)
     cplx<R> nS[NC], nA[NC];
     if (!CHAIN && ok) fetch(xnext<L, P, E, F::RL>(-1), nS, nA);
-    const double kyz2 = dKy * dKy + dKz * dKz;
-    const double eyh = exp(sc.fh[0] * kyz2), ey1 = exp(sc.f1[0] * kyz2);
-    const double eyhb = (MODE == 1) ? exp(sc.fh[1] * kyz2) : 0.0, ey1b = (MODE == 1) ? exp(sc.f1[1] * kyz2) : 0.0;
 #pragma unroll
     for (int i = 0; i < E; i++) {
         if (xdrop<L, P, E, F::RL>(i)) {
@@ -666,13 +673,13 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
 #pragma unroll
                 for (int a = 0; a < 3; a++) rhs[a] = dmk(-Nu[a].x, -Nu[a].y);
             }
-            const double eh = exl[0][l] * eyh, e1 = exl[1][l] * ey1;
+            const double eh = exl[0][l] * eyz[0][c], e1 = exl[1][l] * eyz[1][c];
 #pragma unroll
             for (int a = 0; a < 3; a++)
                 keep[a] = rk_store<R, !CHAIN>(stage, sc.dt, eh, e1, rhs[a], s0[a], a0[a], at_byte(S + a * g.cS + mu, bk),
                                               at_byte(Vs + a * g.cS + mu, bk), at_byte(acc + a * g.cS + mu, bk));
             if constexpr (MODE == 1) {
-                const double ehb = exl[2][l] * eyhb, e1b = exl[3][l] * ey1b;
+                const double ehb = exl[2][l] * eyz[2][c], e1b = exl[3][l] * eyz[3][c];
 #pragma unroll
                 for (int a = 0; a < 3; a++) {
                     Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
