@@ -32,6 +32,24 @@ template <typename T> __device__ __forceinline__ T* at_byte(T* base, unsigned of
 {
     return (T*)((char*)base + off);
 }
+// Non-temporal store of one complex value: the pipeline buffers are written once and read by the next pass
+// only after every workgroup of this one has finished, and at the grid sizes that matter they exceed the
+// last-level cache, so their lines need not be kept on the way out.
+template <typename R> __device__ __forceinline__ void st_nt(cplx<R>* p, cplx<R> v)
+{
+    typedef R v2 __attribute__((ext_vector_type(2)));
+    v2 x;
+    x.x = v.x;
+    x.y = v.y;
+    __builtin_nontemporal_store(x, (v2*)p);
+}
+// Non-temporal load of one complex value (an input line read once by this pass)
+template <typename R> __device__ __forceinline__ cplx<R> ld_nt(const cplx<R>* p)
+{
+    typedef R v2 __attribute__((ext_vector_type(2)));
+    const v2 x = __builtin_nontemporal_load((const v2*)p);
+    return mk<R>(x.x, x.y);
+}
 
 // Wavenumber helpers (trunk/lib/four/four_inline.h:88-138, trunk/lib/scft/scft_inline.h:86)
 __host__ __device__ __forceinline__ int kx_four(int l1, int N1) { return (l1 <= N1 / 2) ? l1 : l1 - N1; }

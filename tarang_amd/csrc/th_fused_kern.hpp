@@ -77,7 +77,7 @@ __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restric
         if (ok) {
             cplx<R>* out = A + cc * g.cA + m;
 #pragma unroll
-            for (int i = 0; i < F::E; i++) out[(long)F::out_pos(t, i) * g.sx] = v[i];
+            for (int i = 0; i < F::E; i++) st_nt(out + (long)F::out_pos(t, i) * g.sx, v[i]);
         }
 #pragma unroll
         for (int i = 0; i < F::E; i++) v[i] = u[i] - b[i];
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restric
         if (ok) {
             cplx<R>* out = A + (3 + cc) * g.cA + m;
 #pragma unroll
-            for (int i = 0; i < F::E; i++) out[(long)F::out_pos(t, i) * g.sx] = v[i];
+            for (int i = 0; i < F::E; i++) st_nt(out + (long)F::out_pos(t, i) * g.sx, v[i]);
         }
         return;
     }
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(T * P) void k_fx_inv(FG g, const cplx<R>* __restric
     if (ok) {
         cplx<R>* out = A + oc * g.cA + m;
 #pragma unroll
-        for (int i = 0; i < F::E; i++) out[(long)F::out_pos(t, i) * g.sx] = v[i];
+        for (int i = 0; i < F::E; i++) st_nt(out + (long)F::out_pos(t, i) * g.sx, v[i]);
     }
 }
 
@@ -134,13 +134,13 @@ __global__ __launch_bounds__(T * P) void k_fy_inv(FG g, const cplx<R>* __restric
 #pragma unroll
     for (int i = 0; i < F::E; i++) {
         const int l = F::in_pos(t, i);
-        v[i] = (ok && lkept(g.my, l)) ? a[(long)lq(g.my, l) * g.KZP] : czero<R>();
+        v[i] = (ok && lkept(g.my, l)) ? ld_nt(a + (long)lq(g.my, l) * g.KZP) : czero<R>();
     }
     F::template run<T, false, NoPad, false, 0>(v, lds + c, t, twl);   // lds unused before: no first pre-write barrier
     if (ok) {
         cplx<R>* b = B + comp * g.cB + (long)x * g.sB + cz;
 #pragma unroll
-        for (int i = 0; i < F::E; i++) b[(long)F::out_pos(t, i) * g.KZP] = v[i];
+        for (int i = 0; i < F::E; i++) st_nt(b + (long)F::out_pos(t, i) * g.KZP, v[i]);
     }
 }
 
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
 #pragma unroll
         for (int m = 0; m < E; m++) {
             const int k = t + P * m;
-            xa[m] = (!zin(m) && ok && k < NKZ) ? X[k] : czero<R>();
+            xa[m] = (!zin(m) && ok && k < NKZ) ? ld_nt(X + k) : czero<R>();
         }
         wave_sync();
         // every position is written (the zeros too): mixed registers read mirrors up to LZ - NKZ + P
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
 #pragma unroll
             for (int m = 0; m < E; m++) {
                 const int k = t + P * m;
-                if (!zin(m) && ok && k < NKZ) X[k] = o[m];
+                if (!zin(m) && ok && k < NKZ) st_nt(X + k, o[m]);
             }
         } else {
             // FP64: holding every unpacked point would cost 4 E VGPRs
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(TZ * P) void k_fz(FG g, const cplx<R>* __restrict__
                 if (zin(m)) continue;
                 const int k = t + P * m;
                 const cplx<R> o = unpack(m);
-                if (ok && k < NKZ) X[k] = o;
+                if (ok && k < NKZ) st_nt(X + k, o);
             }
         }
     };
@@ -398,14 +398,14 @@ __global__ __launch_bounds__(T * P) void k_fy_fwd(FG g, const cplx<R>* __restric
     const cplx<R>* a = Cin + comp * g.cB + (long)x * g.sB + cz;
     cplx<R> v[F::E];
 #pragma unroll
-    for (int i = 0; i < F::E; i++) v[i] = ok ? a[(long)F::in_pos(t, i) * g.KZP] : czero<R>();
+    for (int i = 0; i < F::E; i++) v[i] = ok ? ld_nt(a + (long)F::in_pos(t, i) * g.KZP) : czero<R>();
     F::template run<T, false, NoPad, false, 0>(v, lds + c, t, twl);   // lds unused before: no first pre-write barrier
     if (ok) {
         cplx<R>* d = D + comp * g.cA + (long)x * g.sx + cz;
 #pragma unroll
         for (int i = 0; i < F::E; i++) {
             const int l = F::out_pos(t, i);
-            if (lkept(g.my, l)) d[(long)lq(g.my, l) * g.KZP] = v[i];
+            if (lkept(g.my, l)) st_nt(d + (long)lq(g.my, l) * g.KZP, v[i]);
         }
     }
 }
@@ -447,14 +447,14 @@ __device__ __forceinline__ cplx<R> rk_store(int stage, double dt, double eh, dou
     if (stage == 1) {
         Dc v = dadd(V0, dsc(rhs, 0.5 * dt));
         out = ds<R>(dsc(v, eh));
-        *acc = ds<R>(dsc(dsc(rhs, e1), dt / 6));
+        st_nt(acc, ds<R>(dsc(dsc(rhs, e1), dt / 6)));
     } else if (stage == 2) {
         out = ds<R>(dadd(dsc(V0, eh), dsc(rhs, 0.5 * dt)));
-        *acc = ds<R>(dadd(dl(a0), dsc(dsc(rhs, eh), dt / 3)));
+        st_nt(acc, ds<R>(dadd(dl(a0), dsc(dsc(rhs, eh), dt / 3))));
     } else if (stage == 3) {
         Dc v = dadd(dsc(V0, eh), dsc(rhs, 1.0 * dt));
         out = ds<R>(dsc(v, eh));
-        *acc = ds<R>(dadd(dl(a0), dsc(dsc(rhs, eh), dt / 3)));
+        st_nt(acc, ds<R>(dadd(dl(a0), dsc(dsc(rhs, eh), dt / 3))));
     } else {
         Dc a = dadd(dl(a0), dsc(rhs, dt / 6));
         out = ds<R>(dadd(dsc(V0, e1), a));
