@@ -780,7 +780,7 @@ __global__ __launch_bounds__(T * P) void k_sx_inv(FG g, const cplx<R>* __restric
         for (int i = 0; i < F::E; i++) {
             const int j = F::out_pos(t, i);
             const R sg = (sn && j >= L / 2) ? R(-1) : R(1);
-            out[(long)j * g.sx] = cscale(v[i], sg);
+            st_nt(out + (long)j * g.sx, cscale(v[i], sg));
         }
     }
 }
