@@ -384,14 +384,58 @@ static void diss_of(const th_state* s, const th_phys* ph, double* d0, double* d1
 //   stage 1: Vs += dt/2 e_h P(F), acc += dt/6 e_1 P(F)      stage 2: Vs += dt/2 P(F), acc += dt/3 e_h P(F)
 //   stage 3: Vs += dt e_h P(F),   acc += dt/3 e_h P(F)      stage 4: S += dt/6 P(F)
 // with e_h = exp(-nu dt K^2 / 2), e_1 = exp(-nu dt K^2) (the integrating factors of rk_store).
+// A chained forced step writes no stage field: in stages 2-4, k_force_recon recovers the stage input's band
+// modes from the A lines (the x-inverse of that input, which the previous stage's x-forward wrote) by a direct
+// DFT along x; the x-forward has transformed the stage output without the force share into the next stage's A
+// lines, so k_force_post records the share it adds to the next stage's input (S in stage 4) per band mode, and
+// k_force_chain_fix adds its x-inverse to the A lines of the band's columns.
 // ------------------------------------------------------------------------------------------
 __host__ __device__ __forceinline__ bool fused_stored(const FG& g, int lx, int ly, int lz)
 {
     return lz < g.nkz && lkept(g.mx, lx) && lkept(g.my, ly);
 }
 
+// Chained forced step, stages 2-4: the stage input at every band mode's reality source (sx, sy, lz),
+// v_c = (1/N1) sum_x A[c][x][sy][lz] e^{-2 pi i sx x / N1}, into vbox (6 doubles per band mode; 0 where the
+// state stores no such mode).  One workgroup of 256 threads per band mode.
 template <typename R>
-__global__ void k_force_pre(ForceBand b, FG g, const cplx<R>* __restrict__ src, double* __restrict__ fbox)
+__global__ __launch_bounds__(256) void k_force_recon(ForceBand b, FG g, const cplx<R>* __restrict__ A,
+                                                     double* __restrict__ vbox)
+{
+    const long t = blockIdx.x;
+    int lx, ly, lz, sx, sy;
+    band_mode(b, t, lx, ly, lz);
+    (void)reality_source(b, lx, ly, lz, sx, sy);
+    __shared__ double red[6][256];
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const bool st = fused_stored(g, sx, sy, lz);
+    if (st) {
+        const cplx<R>* col = A + (long)lq(g.my, sy) * g.KZP + lz;
+        for (int x = threadIdx.x; x < g.N1; x += blockDim.x) {
+            const long m = ((long)sx * x) % g.N1;   // e^{-2 pi i sx x / N1} from the exact phase index
+            double sn, cs;
+            sincospi(2.0 * (double)m / g.N1, &sn, &cs);
+            for (int c = 0; c < 3; c++) {
+                const cplx<R> a = col[c * g.cA + (long)x * g.sx];
+                acc[2 * c] += (double)a.x * cs + (double)a.y * sn;
+                acc[2 * c + 1] += (double)a.y * cs - (double)a.x * sn;
+            }
+        }
+    }
+    for (int j = 0; j < 6; j++) red[j][threadIdx.x] = acc[j];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int j = 0; j < 6; j++) red[j][threadIdx.x] += red[j][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) vbox[6 * t + threadIdx.x] = st ? red[threadIdx.x][0] / g.N1 : 0.0;
+}
+
+// vbox (chained stages 2-4): the stage input's band modes from k_force_recon instead of src
+template <typename R>
+__global__ void k_force_pre(ForceBand b, FG g, const cplx<R>* __restrict__ src, const double* __restrict__ vbox,
+                            double* __restrict__ fbox)
 {
     const long n = (long)b.nx * b.ny * b.nz;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -400,7 +444,9 @@ __global__ void k_force_pre(ForceBand b, FG g, const cplx<R>* __restrict__ src, 
     band_mode(b, t, lx, ly, lz);
     const bool mirror = reality_source(b, lx, ly, lz, sx, sy);
     cplx<R> v[3] = {czero<R>(), czero<R>(), czero<R>()};
-    if (fused_stored(g, sx, sy, lz)) {
+    if (vbox) {
+        for (int c = 0; c < 3; c++) v[c] = mk<R>((R)vbox[6 * t + 2 * c], (R)vbox[6 * t + 2 * c + 1]);
+    } else if (fused_stored(g, sx, sy, lz)) {
         const long i = (long)lq(g.mx, sx) * g.sx + (long)lq(g.my, sy) * g.KZP + lz;
         for (int c = 0; c < 3; c++) v[c] = src[c * g.cS + i];
     }
@@ -424,13 +470,16 @@ __global__ void k_force_pre(ForceBand b, FG g, const cplx<R>* __restrict__ src, 
     }
 }
 
+// chained: no stage field to update (the next stage's A lines get the share through k_force_chain_fix)
 template <typename R>
 __global__ void k_force_post(ForceBand b, FG g, const StageCoef* __restrict__ coef, cplx<R>* S, cplx<R>* Vs,
-                             cplx<R>* acc, const double* __restrict__ fbox)
+                             cplx<R>* acc, double* __restrict__ fbox, int chained)
 {
     const long n = (long)b.nx * b.ny * b.nz;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
+    double* share = fbox + 6 * n + 6 * t;   // what this stage adds to the next stage's input (0: nothing)
+    for (int j = 0; j < 6; j++) share[j] = 0.0;
     int lx, ly, lz;
     band_mode(b, t, lx, ly, lz);
     if (!fused_stored(g, lx, ly, lz)) return;
@@ -453,14 +502,56 @@ __global__ void k_force_post(ForceBand b, FG g, const StageCoef* __restrict__ co
     for (int c = 0; c < 3; c++) {
         const long j = c * g.cS + i;
         if (sc.stage < 4) {
-            const cplx<R> v = Vs[j], a = acc[j];
-            Vs[j] = mk<R>((R)((double)v.x + cv * f[2 * c]), (R)((double)v.y + cv * f[2 * c + 1]));
+            if (!chained) {
+                const cplx<R> v = Vs[j];
+                Vs[j] = mk<R>((R)((double)v.x + cv * f[2 * c]), (R)((double)v.y + cv * f[2 * c + 1]));
+            }
+            const cplx<R> a = acc[j];
             acc[j] = mk<R>((R)((double)a.x + ca * f[2 * c]), (R)((double)a.y + ca * f[2 * c + 1]));
+            share[2 * c] = cv * f[2 * c];
+            share[2 * c + 1] = cv * f[2 * c + 1];
         } else {
             const cplx<R> v = S[j];
             const double c4 = sc.dt / 6;
             S[j] = mk<R>((R)((double)v.x + c4 * f[2 * c]), (R)((double)v.y + c4 * f[2 * c + 1]));
+            share[2 * c] = c4 * f[2 * c];
+            share[2 * c + 1] = c4 * f[2 * c + 1];
         }
+    }
+}
+
+// Chained forced step: A[c][x][column] += sum over the column's band modes lx of share * e^{+2 pi i lx x / N1}
+// (the unnormalised inverse line transform of the shares k_force_post recorded).  One workgroup per band
+// column (ty, tz) and component; columns the state does not store, or without a nonzero share, are left alone.
+template <typename R>
+__global__ void k_force_chain_fix(ForceBand b, FG g, cplx<R>* A, const double* __restrict__ share)
+{
+    const int ty = blockIdx.x / b.nz, tz = blockIdx.x % b.nz, c = blockIdx.y;
+    int lx, ly, lz;
+    band_mode(b, (long)ty * b.nz + tz, lx, ly, lz);
+    if (lz >= g.nkz || !lkept(g.my, ly)) return;
+    bool any = false;
+    for (int tx = 0; tx < b.nx; tx++) {
+        const double* d = share + 6 * (((long)tx * b.ny + ty) * b.nz + tz) + 2 * c;
+        any = any || d[0] != 0.0 || d[1] != 0.0;
+    }
+    if (!any) return;
+    cplx<R>* col = A + c * g.cA + (long)lq(g.my, ly) * g.KZP + lz;
+    for (int x = threadIdx.x; x < g.N1; x += blockDim.x) {
+        double sr = 0.0, si = 0.0;
+        for (int tx = 0; tx < b.nx; tx++) {
+            const long t = ((long)tx * b.ny + ty) * b.nz + tz;
+            const double dr = share[6 * t + 2 * c], di = share[6 * t + 2 * c + 1];
+            if (dr == 0.0 && di == 0.0) continue;
+            band_mode(b, t, lx, ly, lz);
+            const long m = ((long)lx * x) % g.N1;   // e^{+2 pi i lx x / N1} from the exact phase index
+            double sn, cs;
+            sincospi(2.0 * (double)m / g.N1, &sn, &cs);
+            sr += dr * cs - di * sn;
+            si += dr * sn + di * cs;
+        }
+        cplx<R>& a = col[(long)x * g.sx];
+        a = mk<R>((R)((double)a.x + sr), (R)((double)a.y + si));
     }
 }
 
@@ -474,7 +565,14 @@ template <typename R> static int enqueue_step(th_state* s, hipStream_t st, bool 
     for (int stage = 1; stage <= 4; stage++) {
         if (s->forced) {
             const cplx<R>* src = (const cplx<R>*)(stage == 1 ? s->S : s->Vs);
-            hipLaunchKernelGGL((k_force_pre<R>), fgrid, dim3(256), 0, st, s->fb, s->fg, src, s->fbox);
+            const double* vbox = nullptr;
+            if (chained && stage > 1) {   // no stage field: its band modes from the A lines
+                vbox = (const double*)s->fbox + 12 * nb;
+                hipLaunchKernelGGL((k_force_recon<R>), dim3((unsigned)nb), dim3(256), 0, st, s->fb, s->fg,
+                                   (const cplx<R>*)s->X1, (double*)vbox);
+                TH_CHECK_HIP(hipGetLastError());
+            }
+            hipLaunchKernelGGL((k_force_pre<R>), fgrid, dim3(256), 0, st, s->fb, s->fg, src, vbox, s->fbox);
             TH_CHECK_HIP(hipGetLastError());
         }
         for (int k = 0; k < NPASS; k++) {
@@ -484,8 +582,13 @@ template <typename R> static int enqueue_step(th_state* s, hipStream_t st, bool 
         if (s->forced) {
             hipLaunchKernelGGL((k_force_post<R>), fgrid, dim3(256), 0, st, s->fb, s->fg,
                                (const StageCoef*)(s->dcoef + (size_t)(stage - 1) * s->nbox), (cplx<R>*)s->S,
-                               (cplx<R>*)s->Vs, (cplx<R>*)s->acc, (const double*)s->fbox);
+                               (cplx<R>*)s->Vs, (cplx<R>*)s->acc, (double*)s->fbox, chained ? 1 : 0);
             TH_CHECK_HIP(hipGetLastError());
+            if (chained) {
+                hipLaunchKernelGGL((k_force_chain_fix<R>), dim3((unsigned)(s->fb.ny * s->fb.nz), 3), dim3(256), 0, st,
+                                   s->fb, s->fg, (cplx<R>*)s->X1, (const double*)s->fbox + 6 * nb);
+                TH_CHECK_HIP(hipGetLastError());
+            }
         }
     }
     return OK;
@@ -526,7 +629,8 @@ extern "C" int th_state_set_force(th_state* s, const th_force* f)
                 return E_UNSUPPORTED;
             }
         }
-        const size_t need = (size_t)nb * 6;
+        // per band mode: P(F); the stage's share and the reconstructed stage input (chained steps)
+        const size_t need = (size_t)nb * 18;
         if (need > s->fbox_cap) {
             // the old buffer goes away: the state is unforced (and its captured step void) until this succeeds
             s->forced = false;
@@ -558,11 +662,11 @@ static int ensure_twiddles(th_plan* p, const FG& g, hipStream_t st)
     return OK;
 }
 
-// whether a step runs this state chained (NS or MHD without forcing, N1 with a chained x-forward; any number of
+// whether a step runs this state chained (NS, forced or not, or MHD; N1 with a chained x-forward; any number of
 // boxes)
 static bool step_chained(const th_state* s)
 {
-    return (s->physics == TH_NS || s->physics == TH_MHD) && !s->forced &&
+    return (s->physics == TH_NS || s->physics == TH_MHD) &&
            (s->plan->prec ? d_xfwd_chainable<float>(s->fg) : d_xfwd_chainable<double>(s->fg));
 }
 

@@ -156,12 +156,15 @@ def test_force_argument_errors(ctx):
 @pytest.mark.parametrize("N,kf,inner,outer", [((16, 16, 16), (1.0, 1.0, 1.0), 1.5, 3.5),
                                               ((32, 32, 32), (1.0, 1.0, 1.0), 0.5, 2.5),
                                               ((48, 32, 24), (1.0, 1.0, 1.0), 2.0, 4.0),
-                                              ((16, 16, 16), (1.0, 1.0, 1.0), 3.0, 6.5)])
+                                              ((16, 16, 16), (1.0, 1.0, 1.0), 3.0, 6.5),
+                                              ((64, 32, 32), (1.0, 1.0, 1.0), 1.5, 3.5),
+                                              ((64, 16, 32), (1.0, 2.0, 1.0), 0.5, 4.5)])
 @pytest.mark.parametrize("proc,p3,p4", [(1, 0.5, 0.3), (2, 0.6, 0.2)])
 def test_fused_forced_step_matches_oracle(ctx, N, kf, inner, outer, proc, p3, p4):
     """th_state_set_force + th_step: the band kernels around each fused stage give the oracle's forced
-    RK4 step (force of each stage field, projected, into the rhs).  The last band reaches past N/3
-    (Dealias_force)."""
+    RK4 step (force of each stage field, projected, into the rhs).  The fourth band reaches past N/3
+    (Dealias_force); N1 = 48 and 64 step chained (the force's share of each stage output is added to the
+    next stage's x-transformed lines), 16 and 32 unchained."""
     from tarang_amd import Plan, State
     log(f"fused forced N={N} band=({inner},{outer}] proc={proc}")
     o = Oracle(N, kfactor=kf)
@@ -178,13 +181,13 @@ def test_fused_forced_step_matches_oracle(ctx, N, kf, inner, outer, proc, p3, p4
         assert rel(H(out[i]), V[i]) < 1e-12
 
 
-def test_fused_forced_fp32_mixed_and_switch_off(ctx):
+@pytest.mark.parametrize("N", [(32, 32, 32), (64, 32, 32)])
+def test_fused_forced_fp32_mixed_and_switch_off(ctx, N):
     """FP32-mixed fused forced step vs the oracle from the same rounded start (<= 1e-5); with the
     force switched off again the state steps exactly as a never-forced state; energy_supply = 0
-    leaves the step bit-identical to the unforced one."""
+    leaves the step bit-identical to the unforced one (unchained at N1 = 32, chained at 64)."""
     from tarang_amd import Plan, State
-    log("fused forced fp32 + off")
-    N = (32, 32, 32)
+    log(f"fused forced fp32 + off N={N}")
     o = Oracle(N)
     p = Plan(ctx, N, precision="fp32")
     V = o.ic_random(9, E0=0.5, kp=3.0)
