@@ -556,8 +556,9 @@ __global__ void k_force_chain_fix(ForceBand b, FG g, cplx<R>* A, const double* _
 }
 
 // One RK4 step of every box in s, as 4 stages x 5 pass launches on stream st (forced: plus the
-// two band kernels around each stage); chained: 4 passes per stage, the x-inverse done by the previous
-// stage's x-forward (the first stage's by the step before, or by the priming launch in th_step).
+// band kernels around each stage); chained: 4 passes per stage, the x-inverse done by the previous
+// stage's x-forward (the first stage's by the step before, or by the priming launch in th_step), and a forced
+// chained stage adds k_force_recon before and k_force_chain_fix after.
 template <typename R> static int enqueue_step(th_state* s, hipStream_t st, bool chained = false)
 {
     const long nb = (long)s->fb.nx * s->fb.ny * s->fb.nz;
