@@ -2,7 +2,8 @@
 // device-resident states (single boxes and batched boxes of one size), the hipGraph-captured RK4
 // step, per-box stage coefficients, energy transfer from shell givers, the per-pass profiler, the
 // launch-shape autotuner, the uneven-box ensemble and the forced step.  The five passes and their
-// kernels are described in th_fused_kern.hpp.
+// kernels are described in th_fused_kern.hpp; NS and MHD states step chained (four launches per stage: each
+// stage's x-forward also performs the next stage's x-inverse).
 #include <stdio.h>
 #include <array>
 #include <string>

@@ -13,7 +13,9 @@
 //                  exp(-nu K^2 a dt) and the RK4 stage update (single_time_step.cc, Compute_RK4_parts)
 // MHD uses the Elsasser form (compute_nlin_VF_MHD.cc) with 6 inverse and 9 product lines; the energy
 // transfer (ET_compute_nlin.cc) uses helper U and a shell-masked giver G and bins 2 mult Re(N U*)
-// (four_ET.cc) in the pass-5 epilogue instead of writing N.
+// (four_ET.cc) in the pass-5 epilogue instead of writing N.  Chained NS and MHD steps fold pass 1 of the
+// next stage into pass 5 (k_fx_fwd CHAIN).  Each pipeline buffer is written once and read once by the next
+// pass, so outputs go out with non-temporal stores and the y and z passes load non-temporally.
 
 #pragma once
 #include <type_traits>
