@@ -284,7 +284,9 @@ int th_state_upload(th_state *state, int comp, const void *src, int src_on_host)
 int th_state_download(th_state *state, int comp, void *dst, int dst_on_host);
 /* No counterpart in the reference: which = 0 state, 1 RK stage field, 2 RK4 accumulator, each in the
  * reference layout.  After th_step the stage field is the input of the step's last RK stage, the field
- * whose force (Compute_force / Compute_force_RB) the reference's Force arrays hold after Time_advance. */
+ * whose force (Compute_force / Compute_force_RB) the reference's Force arrays hold after Time_advance.
+ * A chained step (NS, MHD) writes no stage field; it keeps the last stage's input only for a forced state,
+ * and which = 1 after a chained step of an unforced state returns E_UNSUPPORTED. */
 int th_state_download_buffer(th_state *state, int which, int comp, void *dst, int dst_on_host);
 int th_step(th_plan *plan, th_state *state, double dt, const th_phys *p, int nsteps);
 /* Forcing of later th_step calls (force_field_proc 1 / 2 recomputed from every stage field, as

@@ -100,6 +100,9 @@ struct th_state {
     // single x-inverse launch restores after anything else has used the buffers or changed S.
     bool chain_ok = false;
     bool graph_chained = false;
+    // the stage-field buffer holds the last step's last-stage input (false after a chained step of an unforced
+    // state, which never writes it)
+    bool stage_kept = true;
     bool forced = false;         // energy / helicity forcing (th_state_set_force)
     ForceBand fb{};
     double* fbox = nullptr;      // projected force P(F) of the stage input, 3 complex FP64 per band-box mode
@@ -259,6 +262,10 @@ static int state_download(th_state* s, int box, int comp, void* dst, int dst_on_
 extern "C" int th_state_download_buffer(th_state* s, int which, int comp, void* dst, int dst_on_host)
 {
     if (!s || which < 0 || which > 2) { set_error("th_state_download_buffer: bad args"); return E_ARG; }
+    if (which == 1 && !s->stage_kept) {
+        set_error("th_state_download_buffer: the chained step of an unforced NS or MHD state keeps no stage field");
+        return E_UNSUPPORTED;
+    }
     void* keep = s->S;
     s->S = which == 0 ? s->S : which == 1 ? s->Vs : s->acc;
     int r = state_download(s, 0, comp, dst, dst_on_host, s->plan->ctx->stream);
@@ -471,7 +478,8 @@ __global__ void k_force_pre(ForceBand b, FG g, const cplx<R>* __restrict__ src, 
     }
 }
 
-// chained: no stage field to update (the next stage's A lines get the share through k_force_chain_fix)
+// chained: no stage field to update (the next stage's A lines get the share through k_force_chain_fix) except
+// the stage-3 output, which the chained x-forward keeps for a forced state
 template <typename R>
 __global__ void k_force_post(ForceBand b, FG g, const StageCoef* __restrict__ coef, cplx<R>* S, cplx<R>* Vs,
                              cplx<R>* acc, double* __restrict__ fbox, int chained)
@@ -503,7 +511,7 @@ __global__ void k_force_post(ForceBand b, FG g, const StageCoef* __restrict__ co
     for (int c = 0; c < 3; c++) {
         const long j = c * g.cS + i;
         if (sc.stage < 4) {
-            if (!chained) {
+            if (!chained || sc.stage == 3) {   // chained: only the kept stage-3 output (FG::keep3)
                 const cplx<R> v = Vs[j];
                 Vs[j] = mk<R>((R)((double)v.x + cv * f[2 * c]), (R)((double)v.y + cv * f[2 * c + 1]));
             }
@@ -636,6 +644,7 @@ extern "C" int th_state_set_force(th_state* s, const th_force* f)
         if (need > s->fbox_cap) {
             // the old buffer goes away: the state is unforced (and its captured step void) until this succeeds
             s->forced = false;
+            s->fg.keep3 = 0;
             if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
             if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
             if (s->fbox) TH_CHECK_HIP(hipFree(s->fbox));
@@ -651,6 +660,7 @@ extern "C" int th_state_set_force(th_state* s, const th_force* f)
         s->fb = fb;
         s->forced = true;
     }
+    s->fg.keep3 = s->forced ? 1 : 0;
     // the captured step changes
     if (s->gexec) { (void)hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
     if (s->graph) { (void)hipGraphDestroy(s->graph); s->graph = nullptr; }
@@ -703,7 +713,10 @@ extern "C" int th_step(th_plan* p, th_state* s, double dt, const th_phys* ph, in
         TRY(p->prec ? run_pass<float>(s, PASS_XINV, s->var[PASS_XINV], 1, st)
                     : run_pass<double>(s, PASS_XINV, s->var[PASS_XINV], 1, st));
     for (int i = 0; i < nsteps; i++) TH_CHECK_HIP(hipGraphLaunch(s->gexec, st));
-    if (nsteps > 0) s->chain_ok = chained;
+    if (nsteps > 0) {
+        s->chain_ok = chained;
+        s->stage_kept = !chained || s->forced;
+    }
     return OK;
 }
 

@@ -63,6 +63,7 @@ struct FG {
     int scft;              // sin/cos along x (SCFT basis), else Fourier
     double f1, f2, f3;
     double norm; // 1 / (2 N1 N2 N3): the line transforms' 1 / (N1 N2 N3) and the z pass's unpacking 1/2 (SCFT: 1/2 more)
+    int keep3 = 0; // chained x-forward of a forced state: write the stage-3 output (the last stage's input field)
 };
 
 struct StageCoef {

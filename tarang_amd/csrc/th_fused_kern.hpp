@@ -440,9 +440,9 @@ __device__ __forceinline__ Dc dadd(const Dc& a, const Dc& b) { return dmk(a.x + 
 // Returns the stage's output, the field the next stage transforms: the stage field (stages 1-3) or the new
 // state (stage 4), as stored.  WVS = false leaves the stage field unwritten (its only reader, the next x-inverse,
 // then runs on the returned value in registers).
-template <typename R, bool WVS = true>
+template <typename R>
 __device__ __forceinline__ cplx<R> rk_store(int stage, double dt, double eh, double e1, const Dc& rhs, const cplx<R>& s0,
-                                            const cplx<R>& a0, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc)
+                                            const cplx<R>& a0, cplx<R>* S, cplx<R>* Vs, cplx<R>* acc, bool wvs = true)
 {
     const Dc V0 = dl(s0);
     cplx<R> out;
@@ -463,7 +463,7 @@ __device__ __forceinline__ cplx<R> rk_store(int stage, double dt, double eh, dou
         *S = out;
         return out;
     }
-    if constexpr (WVS) *Vs = out;
+    if (wvs) *Vs = out;
     return out;
 }
 template <typename R>
@@ -486,7 +486,8 @@ template <int L, int P, int E, int RL> constexpr int xnext(int i)
 // CHAIN: the epilogue also performs the next stage's x-inverse: the stage output (stage field, or the new
 // state in stage 4) stays in the accumulator registers in place of N and is transformed back along x, and the
 // A lines are written over this tile's columns of D (every product of the tile has been read by then, and no
-// other workgroup reads these columns); the stage field itself is not written.  NS: components 0-2 = U.
+// other workgroup reads these columns); the stage field itself is not written, except the stage-3 output (the
+// last stage's input) of a forced state (FG::keep3), from which the host evaluates the outputs' force.  NS: components 0-2 = U.
 // MHD: the Elsasser lines of k_fx_inv MODE 1, Z+ = U + B into components 0-2 and Z- = U - B into 3-5.
 template <typename R, int L, int P, int T, int MODE, bool CHAIN = false>
 __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 0 && sizeof(R) == 4 && L % 3 != 0) ? 4 : 1))) void k_fx_fwd(FG g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs,
@@ -601,6 +602,7 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
     // for the store to complete.
     constexpr int NC = (MODE == 1) ? 6 : 3;
     const int stage = sc.stage;
+    const bool wvs = !CHAIN || (g.keep3 && stage == 3);   // uniform
     auto row_u = [&](int i) -> long {
         // compact row of l = t + l0: the kept lanes of one register all sit on one side of the
         // dealiased gap (P < L/3), so the row shift is uniform per register; the lane adds lane_off
@@ -678,8 +680,8 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
             const double eh = exl[0][l] * eyz[0][c], e1 = exl[1][l] * eyz[1][c];
 #pragma unroll
             for (int a = 0; a < 3; a++)
-                keep[a] = rk_store<R, !CHAIN>(stage, sc.dt, eh, e1, rhs[a], s0[a], a0[a], at_byte(S + a * g.cS + mu, bk),
-                                              at_byte(Vs + a * g.cS + mu, bk), at_byte(acc + a * g.cS + mu, bk));
+                keep[a] = rk_store<R>(stage, sc.dt, eh, e1, rhs[a], s0[a], a0[a], at_byte(S + a * g.cS + mu, bk),
+                                      at_byte(Vs + a * g.cS + mu, bk), at_byte(acc + a * g.cS + mu, bk), wvs);
             if constexpr (MODE == 1) {
                 const double ehb = exl[2][l] * eyz[2][c], e1b = exl[3][l] * eyz[3][c];
 #pragma unroll
@@ -687,10 +689,10 @@ __global__ __launch_bounds__(T * P) __attribute__((amdgpu_waves_per_eu((MODE == 
                     Dc np = dl(Nacc[a][i]), nm = dl(Nacc[3 + a][i]);
                     Dc nb = dsc(dmk(np.x - nm.x, np.y - nm.y), 0.5 * g.norm);
                     Dc rb = dmk(-nb.x, -nb.y);
-                    keepb[a] = rk_store<R, !CHAIN>(stage, sc.dt, ehb, e1b, rb, s0[3 + a], a0[3 + a],
-                                                   at_byte(S + (3 + a) * g.cS + mu, bk),
-                                                   at_byte(Vs + (3 + a) * g.cS + mu, bk),
-                                                   at_byte(acc + (3 + a) * g.cS + mu, bk));
+                    keepb[a] = rk_store<R>(stage, sc.dt, ehb, e1b, rb, s0[3 + a], a0[3 + a],
+                                           at_byte(S + (3 + a) * g.cS + mu, bk),
+                                           at_byte(Vs + (3 + a) * g.cS + mu, bk),
+                                           at_byte(acc + (3 + a) * g.cS + mu, bk), wvs);
                 }
             }
         }

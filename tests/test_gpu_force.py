@@ -218,6 +218,35 @@ def test_fused_forced_fp32_mixed_and_switch_off(ctx, N):
         assert np.array_equal(H(oc[i]), H(ob[i]))
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_chained_forced_state_keeps_last_stage_field(ctx, prec, tol):
+    """A chained forced step keeps the last RK stage's input field (th_state_download_buffer 1: the field the
+    reference's Force arrays come from after Time_advance) equal to the oracle's; after a chained step of an
+    unforced state the buffer is refused rather than stale."""
+    from tarang_amd import Plan, State
+    log(f"chained forced stage field {prec}")
+    N = (64, 32, 32)
+    o = Oracle(N)
+    p = Plan(ctx, N, precision=prec)
+    V = o.ic_random(8, E0=0.5, kp=3.0)
+    d = [p.to_dev(v) for v in V]
+    V = [H(x) for x in d]
+    s = State(p)
+    s.upload(d)
+    s.set_force(1, 1.5, 3.5, 0.5, 0.3)
+    last = o.zeros(3)
+    o.step_ns_forced(V, 0.01, 1e-3, 1, 1.5, 3.5, 0.5, 0.3, steps=2, last_stage=last)
+    s.step(1e-3, nu=0.01, steps=2)
+    got = s.download_buffer(1)
+    for i in range(3):
+        assert rel(H(got[i]), last[i]) < tol, i
+    u = State(p)
+    u.upload(d)
+    u.step(1e-3, nu=0.01, steps=1)
+    with pytest.raises(RuntimeError, match="keeps no stage field"):
+        u.download_buffer(1)
+
+
 def test_fused_force_band_outside_the_stored_modes(ctx):
     """A band whose modes the dealiased state does not store (|kx| = N/3 without Dealias_force) is refused
     by the fused state; the plain stepper runs it."""

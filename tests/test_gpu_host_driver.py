@@ -377,15 +377,16 @@ def test_driver_forced_run_matches_oracle(band):
         assert abs(v - Ek[k]) <= 1e-12 * max(Ek)
 
 
+@pytest.mark.parametrize("N", [32, 64])
 @pytest.mark.parametrize("path", ["fused", "plain"])
-def test_driver_forced_output_columns(tmp_path, path):
+def test_driver_forced_output_columns(tmp_path, path, N):
     """Forced run with out=DIR: the force-spectrum column of spectrum.d (Compute_force_shell_spectrum
     force_spectrum.cc:48-53, 2 sum_c Re V_c.F_c* per shell) and the force-feed column of flux.d
     (Compute_force_feed_shell compute_force_feed.cc:70-76, Shell_mult_all(F, V) over shell_radius).  The
     reference writes them after Time_advance, whose stages recompute the force (time_advance.cc:100-155), so
     F is the force of the last RK stage's field, V the new velocity; the oracle's forced step reports that
-    field."""
-    N, steps, dt, nu = 32, 3, 1e-3, 0.01
+    field.  At N = 64 the fused state steps chained and keeps that field from its stage-3 x-forward."""
+    steps, dt, nu = 3, 1e-3, 0.01
     fp = (1, 1.5, 3.5, 0.3, 0.2)
     (tmp_path / "out").mkdir()
     spec, flux, glob = run_driver(N, steps, dt, nu, "random", "FP64", path, f"out={tmp_path}",
