@@ -36,9 +36,9 @@ def log(msg):
     sys.stdout.flush()
 
 
-@pytest.fixture(scope="module")
-def ref128():
-    N = (128, 128, 128)
+@pytest.fixture(scope="module", params=[(128, 128, 128), (64, 64, 64)])
+def ref_ns(request):
+    N = request.param
     o = Oracle(N)
     V0 = o.ic_random(31)
     V = [a.copy() for a in V0]
@@ -47,9 +47,9 @@ def ref128():
 
 
 @pytest.mark.parametrize("v", [0, 1, 2, 3])
-def test_variants_match_oracle_fp64(ctx, ref128, v):
+def test_variants_match_oracle_fp64(ctx, ref_ns, v):
     from tarang_amd import Plan, State
-    N, V0, V1 = ref128
+    N, V0, V1 = ref_ns
     var = [min(v, n - 1) for n in NVAR]
     log(f"variants {var} fp64 N={N}")
     p = Plan(ctx, N)
@@ -64,9 +64,9 @@ def test_variants_match_oracle_fp64(ctx, ref128, v):
 
 
 @pytest.mark.parametrize("v", [0, 1, 2, 3])
-def test_variants_fp32_mixed(ctx, ref128, v):
+def test_variants_fp32_mixed(ctx, ref_ns, v):
     from tarang_amd import Plan, State
-    N, V0, V1 = ref128
+    N, V0, V1 = ref_ns
     var = [min(v, n - 1) for n in NVAR]
     log(f"variants {var} fp32 N={N}")
     p = Plan(ctx, N, precision="fp32")
@@ -79,9 +79,9 @@ def test_variants_fp32_mixed(ctx, ref128, v):
         assert rel(H(out[i]), V1[i]) < 2e-5, (var, i)
 
 
-def test_tune_keeps_state_and_step_matches_oracle(ctx, ref128):
+def test_tune_keeps_state_and_step_matches_oracle(ctx, ref_ns):
     from tarang_amd import Plan, State
-    N, V0, V1 = ref128
+    N, V0, V1 = ref_ns
     p = Plan(ctx, N)
     s = State(p)
     s.upload([p.to_dev(a) for a in V0])
