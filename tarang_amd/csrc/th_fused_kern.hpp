@@ -1166,6 +1166,11 @@ static int launch_xfwd(const FG& g, const cplx<R>* D, cplx<R>* S, cplx<R>* Vs, c
     // variant 1: half-width tiles; variant 2: double-width tiles (full 128-byte rows at 512 lines)
     constexpr int T = (V == 1 && L >= 128 && T0 > 1) ? T0 / 2 : (V == 2) ? 2 * T0 : T0;
     const int ntz = (g.nkz + T - 1) / T;
+    // the kernel's unconditional product loads (tiles that divide the row padding) need the last tile inside a row
+    if ((sizeof(R) == 4 ? 16 : 8) % T == 0 && ntz * T > g.KZP) {
+        set_error("x-forward: %d columns of %d-column tiles exceed the row padding %d", ntz * T, T, g.KZP);
+        return E_UNSUPPORTED;
+    }
     if constexpr (CHAIN) {
         if constexpr (MODE <= 1 && LineFFT<R, L, P, -1>::R0 == LineFFT<R, L, P, -1>::RL) {
             hipLaunchKernelGGL((k_fx_fwd<R, L, P, T, MODE, true>), dim3(g.my.nk * ntz, g.nbox), dim3(T * P), 0, st, g, D,
