@@ -449,7 +449,11 @@ def bench_mhd(args, emit=True):
     flux_s = time.perf_counter() - t0
     pts = float(n) ** 3
     value = pts * 4 * args.steps / el / 1e6
-    roof, canon_stage = roofline(ms, nbytes, N, 8, "mhd")
+    traffic = load_traffic(f"mhd{n}_fp32mixed")
+    k = int(np.argmax(ms))
+    roof, canon_stage = roofline(ms, nbytes, N, 8, "mhd", traffic_of(traffic, pass_names(nbytes)[k], var[k]))
+    roof["traffic_source"] = (f"profiles/traffic_mhd{n}_fp32mixed.md (same launch variant)" if roof["traffic"] is not None
+                              else "no counter reading for this launch variant")
     passes = {pass_names(nbytes)[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
                               "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1) if ms[i] > 0 else None} for i in range(5)}
     return _emit(_line(
@@ -664,6 +668,15 @@ def main():
         print("bench.py: this build runs one box per GPU on a single MI355X; multi-GPU runs "
               "(--gpus > 1) are not part of it", flush=True)
         sys.exit(2)
+    if args.calib_copy_mib > 0:
+        # a device copy of known size before the workload: counter runs scale their byte counters by it
+        import torch
+        src = torch.ones(args.calib_copy_mib * (1 << 18), dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: calibration copy of {args.calib_copy_mib} MiB done")
+        del src, dst
     if args.workload == "ensemble":
         bench_ensemble(args)
         return
@@ -686,13 +699,6 @@ def main():
 
     N = (args.n, args.n, args.n)
     nu = 1e-3
-    if args.calib_copy_mib > 0:
-        src = torch.ones(args.calib_copy_mib * (1 << 18), dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"bench: calibration copy of {args.calib_copy_mib} MiB done")
-        del src, dst
     ctx = Context(0)
     p64 = Plan(ctx, N, precision="fp64")
     p32 = Plan(ctx, N, precision="fp32")
@@ -831,7 +837,7 @@ def other_workloads(args):
         r = d.get("roofline") or {}
         e = {"metric": d["metric"], "value": d["value"], "unit": d["unit"], "ms_per_step": d["ms_per_step"],
              "workload": d["config"]["workload"],
-             "roofline": {k: r.get(k) for k in ("kernel", "achieved", "frac", "unit")}}
+             "roofline": {k: r.get(k) for k in ("kernel", "achieved", "frac", "unit", "traffic")}}
         for k in ("flux_mhd_fused", "energy_after", "Nu_fp32", "Nu_fp64", "Nu_plain_fp32", "fp64_gpu", "plain_fp32_gpu",
                   "sequential", "launch_variants"):
             if k in d:
