@@ -530,6 +530,7 @@ def bench_rb(args, emit=True):
     plain path (separate transforms and array passes, the reference's operation sequence) is timed beside
     it in FP32-mixed.  Reports Nu after the timed steps from each path.  Algorithmic bytes per stage:
     72 S (S = N1 N2 (N3/2+1) x 8 B)."""
+    import numpy as np
     import torch
     from tarang_amd import Context, Plan, State, abi
     n = args.rb_n
@@ -538,11 +539,14 @@ def bench_rb(args, emit=True):
     Ra, Pr, dt = 1e7, 1.0, 1e-3
     ctx = Context(0)
     res = {}
-    for prec in ("fp32", "fp64"):
+    # --no-fp64: the FP32-mixed fused step alone (counter runs); --variants fixes its launch variants
+    for prec in (("fp32",) if args.no_fp64 else ("fp32", "fp64")):
         p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision=prec)
         s = State(p, abi.TH_RB)
         s.upload(p.empty(3) + [p.ic_random_scalar(4, 1e-3)])
-        var = s.tune(dt, Ra=Ra, Pr=Pr, nrep=3) if not args.no_tune else s.variants
+        if args.variants and prec == "fp32":
+            s.variants = [int(x) for x in args.variants.split(",")]
+        var = s.tune(dt, Ra=Ra, Pr=Pr, nrep=3) if not (args.no_tune or (args.variants and prec == "fp32")) else s.variants
         s.step(dt, Ra=Ra, Pr=Pr, steps=args.warmup)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -553,21 +557,27 @@ def bench_rb(args, emit=True):
         ms, nbytes = pass_times(s, dt, Ra=Ra, Pr=Pr)
         res[prec] = (el, p.nusselt(out[:3], out[3], Ra, Pr), ms, nbytes, var)
         del s, out, p
-    p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision="fp32")
-    f = p.empty(3) + [p.ic_random_scalar(4, 1e-3)]
-    p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.steps)
-    torch.cuda.synchronize()
-    el_plain = time.perf_counter() - t0
-    Nu_plain = p.nusselt(f[:3], f[3], Ra, Pr)
-    del f, p
+    el_plain, Nu_plain = None, None
+    if not args.no_fp64:
+        p = Plan(ctx, N, kfactor=kf, basis="SCFT", precision="fp32")
+        f = p.empty(3) + [p.ic_random_scalar(4, 1e-3)]
+        p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p.step_plain(abi.TH_RB, f, dt, Ra=Ra, Pr=Pr, steps=args.steps)
+        torch.cuda.synchronize()
+        el_plain = time.perf_counter() - t0
+        Nu_plain = p.nusselt(f[:3], f[3], Ra, Pr)
+        del f, p
     pts = float(n) ** 3
     el, Nu, ms, nbytes, var = res["fp32"]
     value = pts * 4 * args.steps / el / 1e6
-    v64 = pts * 4 * args.steps / res["fp64"][0] / 1e6
-    roof, canon_stage = roofline(ms, nbytes, N, 8, "rb")
+    v64 = pts * 4 * args.steps / res["fp64"][0] / 1e6 if "fp64" in res else None
+    traffic = load_traffic(f"rb{n}_fp32mixed")
+    k = int(np.argmax(ms))
+    roof, canon_stage = roofline(ms, nbytes, N, 8, "rb", traffic_of(traffic, pass_names(nbytes)[k], var[k]))
+    roof["traffic_source"] = (f"profiles/traffic_rb{n}_fp32mixed.md (same launch variant)" if roof["traffic"] is not None
+                              else "no counter reading for this launch variant")
     passes = {pass_names(nbytes)[i]: {"ms": round(float(ms[i]), 4), "GB": round(nbytes[i] / 1e9, 4),
                               "GB_s": round(nbytes[i] / (ms[i] * 1e-3) / 1e9, 1) if ms[i] > 0 else None} for i in range(5)}
     return _emit(_line(
@@ -578,10 +588,11 @@ def bench_rb(args, emit=True):
         {"dtype": "fp32-mixed (fp32 storage/FFT; fp64 integrating factor, twiddles, sums)",
          "data": "synthetic (seeded temperature perturbation generated on the GPU)",
          "path": "fused (5 passes per RK stage, hipGraph replay)", "launch_variants": var,
-         "Nu_fp32": Nu, "Nu_fp64": res["fp64"][1], "Nu_plain_fp32": Nu_plain,
-         "fp64_gpu": {"value": round(v64, 1), "ms_per_step": round(1e3 * res["fp64"][0] / args.steps, 3)},
-         "plain_fp32_gpu": {"value": round(pts * 4 * args.steps / el_plain / 1e6, 1),
-                            "ms_per_step": round(1e3 * el_plain / args.steps, 3)},
+         "Nu_fp32": Nu, "Nu_fp64": res["fp64"][1] if "fp64" in res else None, "Nu_plain_fp32": Nu_plain,
+         "fp64_gpu": ({"value": round(v64, 1), "ms_per_step": round(1e3 * res["fp64"][0] / args.steps, 3)}
+                      if "fp64" in res else None),
+         "plain_fp32_gpu": ({"value": round(pts * 4 * args.steps / el_plain / 1e6, 1),
+                             "ms_per_step": round(1e3 * el_plain / args.steps, 3)} if el_plain else None),
          "roofline": roof,
          "stage": {"canonical_GB_per_stage": round(canon_stage / 1e9, 3),
                    "pruned_GB_per_stage": round(float(sum(nbytes)) / 1e9, 3),
@@ -830,6 +841,8 @@ def other_workloads(args):
     torch.cuda.empty_cache()
     ea = copy.copy(args)
     ea.no_cpu = True
+    ea.variants = None       # the NS line's fixed launch variants do not apply to the other workloads
+    ea.no_fp64 = False       # and the RB line keeps its FP64 and plain-path legs
     out = {}
     for name, fn in (("mhd_512", bench_mhd), ("rb_256", bench_rb), ("ensemble_64_boxes", bench_ensemble)):
         log(f"bench: other workload {name}")
